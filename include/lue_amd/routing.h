@@ -1,0 +1,156 @@
+/*
+ * C ABI of the MI355X-native LUE flow-routing path (liblue_amd.so).
+ *
+ * Each entry point replaces one explicit instantiation of LUE's routing operations for
+ * DefaultValuePolicies (ldd uint8, material float | double), which LUE exports from
+ * liblue_framework_routing_operation (reference: source/framework/algorithm/CMakeLists.txt:1980-2025,
+ * source/framework/algorithm/source/routing_operation/ *.cpp.in templates). The C++ front end
+ * (lue_amd/include/lue/framework/...) maps lue::value_policies::* and lue::* calls on
+ * PartitionedArray onto these functions.
+ *
+ * Conventions
+ *   - All raster pointers are DEVICE pointers to whole arrays, row-major, rows x cols cells
+ *     (the partition grid does not change results; partitions are contiguous sub-blocks in LUE).
+ *   - A raster argument may be NULL; the argument is then the Scalar that follows it, broadcast to
+ *     every cell (LUE's Scalar<E> overloads, accumulating_router.hpp:727-786).
+ *   - `stream` is a hipStream_t (NULL: the default stream). The call enqueues its work on `stream`
+ *     and returns when the outputs are complete (it waits for the stream, so the status can report
+ *     invalid networks).
+ *   - No-data: ldd 255 (numeric_limits<uint8>::max()), float NaN. Cells whose inputs are no-data or
+ *     outside the operation's domain produce NaN, as do all cells downstream of them.
+ *   - Return value: LUE_AMD_SUCCESS or an error status; lue_amd_last_error() gives a message.
+ */
+#ifndef LUE_AMD_ROUTING_H
+#define LUE_AMD_ROUTING_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LUE_AMD_SUCCESS 0
+#define LUE_AMD_ERROR_INVALID_ARGUMENT 1
+#define LUE_AMD_ERROR_SHAPE_MISMATCH 2
+#define LUE_AMD_ERROR_OUT_OF_MEMORY 3
+#define LUE_AMD_ERROR_HIP 4
+#define LUE_AMD_ERROR_INVALID_NETWORK 5
+#define LUE_AMD_ERROR_NO_DEVICE 6
+
+/* Message for a status code, and the detail message of the last failing call on this thread. */
+const char* lue_amd_status_string(int status);
+const char* lue_amd_last_error(void);
+
+/* Library version, "major.minor.patch". */
+const char* lue_amd_version(void);
+
+/*
+ * Timing of the routing sweeps with HIP events on the caller's stream (no reference counterpart; it plays
+ * the role of LUE's QA stopwatch instrumentation, source/quality_assurance/cxx). enable != 0 resets and
+ * starts collection on this process; read returns the summed time of the first (all-tile) rounds, of the
+ * later rounds, the number of routing calls, of rounds and of cells.
+ */
+int lue_amd_profile_enable(int enable);
+int lue_amd_profile_read(double* first_round_ms, double* next_rounds_ms, long long* calls, long long* rounds,
+                         long long* cells);
+
+/* Device memory used by the C++ front end's device-resident partitions. */
+int lue_amd_device_malloc(void** ptr, size_t bytes);
+int lue_amd_device_free(void* ptr);
+int lue_amd_copy_to_device(void* dst, const void* src, size_t bytes, void* stream);
+int lue_amd_copy_to_host(void* dst, const void* src, size_t bytes, void* stream);
+int lue_amd_stream_synchronize(void* stream);
+
+/*
+ * Inflow count: number of upstream neighbours per cell, 255 where the ldd is no-data.
+ * Replaces lue::inflow_count<std::uint8_t>(policies, flow_direction)
+ * (ALG/inflow_count.hpp:29-42, ALG/definition/inflow_count.hpp:665-726, instantiated for uint8 at
+ * ALG/definition/inflow_count.hpp:731-735). ALG = source/framework/algorithm/include/lue/framework/algorithm
+ */
+int lue_amd_inflow_count_u8(const uint8_t* ldd, int64_t rows, int64_t cols, uint8_t* inflow_count, void* stream);
+
+/*
+ * accu: outflow = sum of upstream outflow + inflow.
+ * Replaces lue::value_policies::accu(flow_direction, inflow | Scalar inflow)
+ * (ALG/value_policies/accu.hpp:33-55, ALG/accu.hpp:12-28, ALG/definition/accu.hpp:226-264).
+ */
+int lue_amd_accu_f32(const uint8_t* ldd, int64_t rows, int64_t cols, const float* inflow, float inflow_scalar,
+                     float* outflow, void* stream);
+int lue_amd_accu_f64(const uint8_t* ldd, int64_t rows, int64_t cols, const double* inflow,
+                     double inflow_scalar, double* outflow, void* stream);
+
+/*
+ * accu_threshold: material above the threshold flows on, the rest stays (remainder).
+ * Replaces lue::value_policies::accu_threshold(flow_direction, inflow, threshold), all four
+ * raster/Scalar combinations (ALG/value_policies/accu_threshold.hpp:34-87, ALG/accu_threshold.hpp:9-50,
+ * ALG/definition/accu_threshold.hpp:29-395).
+ */
+int lue_amd_accu_threshold_f32(const uint8_t* ldd, int64_t rows, int64_t cols, const float* inflow,
+                               float inflow_scalar, const float* threshold, float threshold_scalar, float* outflow,
+                               float* remainder, void* stream);
+int lue_amd_accu_threshold_f64(const uint8_t* ldd, int64_t rows, int64_t cols, const double* inflow,
+                               double inflow_scalar, const double* threshold, double threshold_scalar,
+                               double* outflow, double* remainder, void* stream);
+
+/*
+ * accu_fraction: a fraction of the material flows on, the rest stays (remainder).
+ * Replaces lue::value_policies::accu_fraction(flow_direction, inflow, fraction), all four
+ * raster/Scalar combinations (ALG/value_policies/accu_fraction.hpp, ALG/accu_fraction.hpp:9-50,
+ * ALG/definition/accu_fraction.hpp:29-387).
+ */
+int lue_amd_accu_fraction_f32(const uint8_t* ldd, int64_t rows, int64_t cols, const float* inflow,
+                              float inflow_scalar, const float* fraction, float fraction_scalar, float* outflow,
+                              float* remainder, void* stream);
+int lue_amd_accu_fraction_f64(const uint8_t* ldd, int64_t rows, int64_t cols, const double* inflow,
+                              double inflow_scalar, const double* fraction, double fraction_scalar,
+                              double* outflow, double* remainder, void* stream);
+
+/*
+ * kinematic_wave: new discharge per cell from the upstream new discharge, the current discharge and
+ * the lateral inflow (non-linear kinematic wave, Newton solve per cell).
+ * Replaces both lue::value_policies::kinematic_wave overloads (ALG/value_policies/kinematic_wave.hpp:43-91,
+ * ALG/kinematic_wave.hpp:9-32, ALG/definition/kinematic_wave.hpp:606-684): current_outflow and inflow
+ * are rasters; alpha, beta, channel_length are rasters or Scalars; time_step_duration is a Scalar.
+ * f64: Newton in FP64 (LUE's algorithm). f32: Newton in FP32 plus one FP64 correction per cell, upstream
+ * sums carried in compensated form.
+ */
+int lue_amd_kinematic_wave_f32(const uint8_t* ldd, int64_t rows, int64_t cols, const float* current_outflow,
+                               const float* inflow, const float* alpha, float alpha_scalar, const float* beta,
+                               float beta_scalar, float time_step_duration, const float* channel_length,
+                               float channel_length_scalar, float* outflow, void* stream);
+int lue_amd_kinematic_wave_f64(const uint8_t* ldd, int64_t rows, int64_t cols, const double* current_outflow,
+                               const double* inflow, const double* alpha, double alpha_scalar, const double* beta,
+                               double beta_scalar, double time_step_duration, const double* channel_length,
+                               double channel_length_scalar, double* outflow, void* stream);
+
+/*
+ * Kinematic-wave Newton solve of single cells, elementwise over n inputs (device pointers).
+ * Replaces lue::detail::iterate_to_new_discharge<T> (ALG/definition/kinematic_wave.hpp:145-254),
+ * including extraction for negative lateral inflow; used by the unit tests of the solve.
+ */
+int lue_amd_iterate_to_new_discharge_f32(const float* upstream_discharge, const float* current_discharge,
+                                         const float* lateral_inflow, const float* alpha, const float* beta,
+                                         const float* time_step_duration, const float* channel_length,
+                                         float* new_discharge, int64_t n, void* stream);
+int lue_amd_iterate_to_new_discharge_f64(const double* upstream_discharge, const double* current_discharge,
+                                         const double* lateral_inflow, const double* alpha, const double* beta,
+                                         const double* time_step_duration, const double* channel_length,
+                                         double* new_discharge, int64_t n, void* stream);
+
+/*
+ * Seeded synthetic inputs of the benchmark workloads (no reference counterpart: LUE's own tests and
+ * benchmarks build inputs in C++). ldd from the DEM e(r,c) = (rows - r) + 50 |sin(pi c / 256)| + 0.9 u,
+ * sqrt(2)-weighted steepest descent, bottom row outlets; uniform fields lo + (hi - lo) u.
+ */
+int lue_amd_synthetic_ldd(uint64_t seed, int64_t rows, int64_t cols, uint8_t* ldd, void* stream);
+int lue_amd_synthetic_uniform_f32(uint64_t seed, uint64_t stream_id, double lo, double hi, int64_t n, float* out,
+                                  void* stream);
+int lue_amd_synthetic_uniform_f64(uint64_t seed, uint64_t stream_id, double lo, double hi, int64_t n, double* out,
+                                  void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LUE_AMD_ROUTING_H */

@@ -1,0 +1,110 @@
+"""Loader of liblue_amd.so, the HIP implementation behind the C ABI in include/lue_amd/routing.h.
+
+The library is built in-tree (lue_amd/build.py). There is no fallback: if the library is missing or a
+call fails, an exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblue_amd.so")
+
+_lib = None
+
+c_p = ctypes.c_void_p
+c_i64 = ctypes.c_int64
+c_u64 = ctypes.c_uint64
+c_f = ctypes.c_float
+c_d = ctypes.c_double
+c_int = ctypes.c_int
+c_size = ctypes.c_size_t
+
+# name -> argument types (restype int unless listed in _RESTYPES)
+SIGNATURES = {
+    "lue_amd_status_string": [c_int],
+    "lue_amd_last_error": [],
+    "lue_amd_version": [],
+    "lue_amd_profile_enable": [c_int],
+    "lue_amd_profile_read": [ctypes.POINTER(c_d), ctypes.POINTER(c_d)] + [ctypes.POINTER(ctypes.c_longlong)] * 3,
+    "lue_amd_device_malloc": [ctypes.POINTER(c_p), c_size],
+    "lue_amd_device_free": [c_p],
+    "lue_amd_copy_to_device": [c_p, c_p, c_size, c_p],
+    "lue_amd_copy_to_host": [c_p, c_p, c_size, c_p],
+    "lue_amd_stream_synchronize": [c_p],
+    "lue_amd_inflow_count_u8": [c_p, c_i64, c_i64, c_p, c_p],
+    "lue_amd_accu_f32": [c_p, c_i64, c_i64, c_p, c_f, c_p, c_p],
+    "lue_amd_accu_f64": [c_p, c_i64, c_i64, c_p, c_d, c_p, c_p],
+    "lue_amd_accu_threshold_f32": [c_p, c_i64, c_i64, c_p, c_f, c_p, c_f, c_p, c_p, c_p],
+    "lue_amd_accu_threshold_f64": [c_p, c_i64, c_i64, c_p, c_d, c_p, c_d, c_p, c_p, c_p],
+    "lue_amd_accu_fraction_f32": [c_p, c_i64, c_i64, c_p, c_f, c_p, c_f, c_p, c_p, c_p],
+    "lue_amd_accu_fraction_f64": [c_p, c_i64, c_i64, c_p, c_d, c_p, c_d, c_p, c_p, c_p],
+    "lue_amd_kinematic_wave_f32": [c_p, c_i64, c_i64, c_p, c_p, c_p, c_f, c_p, c_f, c_f, c_p, c_f, c_p, c_p],
+    "lue_amd_kinematic_wave_f64": [c_p, c_i64, c_i64, c_p, c_p, c_p, c_d, c_p, c_d, c_d, c_p, c_d, c_p, c_p],
+    "lue_amd_iterate_to_new_discharge_f32": [c_p] * 8 + [c_i64, c_p],
+    "lue_amd_iterate_to_new_discharge_f64": [c_p] * 8 + [c_i64, c_p],
+    "lue_amd_synthetic_ldd": [c_u64, c_i64, c_i64, c_p, c_p],
+    "lue_amd_synthetic_uniform_f32": [c_u64, c_u64, c_d, c_d, c_i64, c_p, c_p],
+    "lue_amd_synthetic_uniform_f64": [c_u64, c_u64, c_d, c_d, c_i64, c_p, c_p],
+}
+
+_RESTYPES = {
+    "lue_amd_status_string": ctypes.c_char_p,
+    "lue_amd_last_error": ctypes.c_char_p,
+    "lue_amd_version": ctypes.c_char_p,
+}
+
+
+class RoutingError(RuntimeError):
+    """A C ABI call returned an error status."""
+
+    def __init__(self, status, message):
+        super().__init__(message)
+        self.status = status
+
+
+class InvalidNetworkError(RoutingError):
+    pass
+
+
+def load():
+    """Load the in-tree library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `python -m lue_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, argtypes in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = _RESTYPES.get(name, c_int)
+    _lib = lib
+    return lib
+
+
+def profile_enable(enable=True):
+    check(load().lue_amd_profile_enable(1 if enable else 0))
+
+
+def profile_read():
+    """(first-round ms, later-rounds ms, calls, rounds, cells) summed since profile_enable()."""
+    first, rest = c_d(), c_d()
+    calls, rounds, cells = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+    check(load().lue_amd_profile_read(ctypes.byref(first), ctypes.byref(rest), ctypes.byref(calls),
+                                      ctypes.byref(rounds), ctypes.byref(cells)))
+    return first.value, rest.value, calls.value, rounds.value, cells.value
+
+
+def check(status):
+    if status != 0:
+        lib = load()
+        message = f"{lib.lue_amd_status_string(status).decode()}: {lib.lue_amd_last_error().decode()}"
+        if status == 5:
+            raise InvalidNetworkError(status, message)
+        if status == 2:
+            raise ValueError(message)
+        raise RoutingError(status, message)

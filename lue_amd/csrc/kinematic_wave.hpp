@@ -1,0 +1,206 @@
+// Per-cell kinematic-wave solve, device side.
+//
+// Equation (reference: source/framework/algorithm/include/lue/framework/algorithm/definition/
+// kinematic_wave.hpp:13-142): with c1 = dt / dx,
+//     f(Q) = c1 Q + alpha Q^beta - K = 0,     K = c1 Q_up + alpha Q_cur^beta + dt q,
+//     f'(Q) = c1 + alpha beta Q^(beta - 1),
+// initial guess linearised at (Q_cur + Q_up) / 2 and clamped to numeric_limits<T>::min() (:61-91).
+// LUE solves it with boost::math::tools::newton_raphson_iterate, digits = floor(0.6 digits(T)),
+// at most 10 iterations (:195-213); dry cells (no upstream, current or lateral water) give 0 (:181-250).
+//
+// Two device solvers:
+//   - solve_bracketed<T>: Newton-Raphson inside a shrinking bracket [0, max] with Boost's documented
+//     step rules (an out-of-bracket step halves the distance to the bound, a step that does not
+//     converge becomes a bounded jump, stop when |step| <= |Q| 2^(1-digits)). This is the plain FP64
+//     drop-in, and the first stage of the FP32 path.
+//   - the FP32 path then applies ONE Newton correction in FP64 to the FP32 root, with K formed in FP64
+//     from the compensated upstream sum; this is the only FP64 arithmetic on that path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+
+namespace lue_amd {
+
+    template<typename T>
+    struct NumericLimits;
+
+    template<>
+    struct NumericLimits<float>
+    {
+        __host__ __device__ static constexpr float min() { return FLT_MIN; }
+        __host__ __device__ static constexpr float max() { return FLT_MAX; }
+        static constexpr int digits = 24;
+    };
+
+    template<>
+    struct NumericLimits<double>
+    {
+        __host__ __device__ static constexpr double min() { return DBL_MIN; }
+        __host__ __device__ static constexpr double max() { return DBL_MAX; }
+        static constexpr int digits = 53;
+    };
+
+    __device__ inline float dpow(float x, float y)
+    {
+        return powf(x, y);
+    }
+
+    __device__ inline double dpow(double x, double y)
+    {
+        return pow(x, y);
+    }
+
+    // Coefficients of one cell's equation.
+    template<typename T>
+    struct KinematicWaveCell
+    {
+        T q_up;       // summed new discharge of the upstream cells
+        T q_cur;      // current discharge
+        T q_lat;      // lateral inflow, >= 0 (value policies make negative inflow no-data)
+        T alpha;
+        T beta;
+        T dt;
+        T dx;
+        T alpha_pow;  // alpha * q_cur^beta, evaluated once per cell outside the dependency chain
+    };
+
+    // Solve f(Q) = 0 from the linearised guess. Returns the root; `converged` tells whether the
+    // iteration stopped on the step criterion (true) or ran out of iterations (false).
+    template<typename T>
+    __device__ inline T solve_bracketed(KinematicWaveCell<T> const& cell, bool& converged)
+    {
+        T const c1 = cell.dt / cell.dx;
+        T const known = (c1 * cell.q_up) + cell.alpha_pow + (cell.dt * cell.q_lat);
+        T const alpha_beta = cell.alpha * cell.beta;
+
+        // guess (kinematic_wave.hpp:61-91)
+        T x = NumericLimits<T>::min();
+        if ((cell.q_cur + cell.q_up != T{0}) || cell.beta >= T{1})
+        {
+            T const a_b_pq = alpha_beta * dpow((cell.q_cur + cell.q_up) / T{2}, cell.beta - T{1});
+            x = (c1 * cell.q_up + a_b_pq * cell.q_cur + cell.dt * cell.q_lat) / (c1 + a_b_pq);
+            x = fmax(x, NumericLimits<T>::min());
+        }
+
+        int const digits = static_cast<int>(NumericLimits<T>::digits * 0.6);
+        T const factor = ldexp(T{1}, 1 - digits);
+        T lo = T{0};
+        T hi = NumericLimits<T>::max();
+        T step = NumericLimits<T>::max();
+        T step_prev = step;
+        T step_prev2 = step;
+        int left = 10;
+        converged = false;
+
+        while (true)
+        {
+            step_prev2 = step_prev;
+            step_prev = step;
+            T const xb = dpow(x, cell.beta);
+            T const f0 = (c1 * x) + (cell.alpha * xb) - known;
+            T const f1 = c1 + (alpha_beta * dpow(x, cell.beta - T{1}));
+            --left;
+            if (f0 == T{0})
+            {
+                converged = true;
+                break;
+            }
+            step = f0 / f1;
+            if (fabs(step * 2) > fabs(step_prev2))
+            {
+                T const shift = (step > 0) ? (x - lo) / 2 : (x - hi) / 2;
+                if ((x != 0) && (fabs(shift) > fabs(x)))
+                {
+                    step = copysign(fabs(x) * T(1.1f), step);
+                }
+                else
+                {
+                    step = shift;
+                }
+                step_prev = 3 * step;
+                step_prev2 = 3 * step;
+            }
+            T const previous = x;
+            x -= step;
+            if (x <= lo)
+            {
+                step = T(0.5f) * (previous - lo);
+                x = previous - step;
+                if (x == lo || x == hi)
+                {
+                    break;
+                }
+            }
+            else if (x >= hi)
+            {
+                step = T(0.5f) * (previous - hi);
+                x = previous - step;
+                if (x == lo || x == hi)
+                {
+                    break;
+                }
+            }
+            if (step > 0)
+            {
+                hi = previous;
+            }
+            else
+            {
+                lo = previous;
+            }
+            if (!(fabs(x * factor) < fabs(step)))
+            {
+                converged = true;
+                break;
+            }
+            if (left == 0)
+            {
+                break;
+            }
+        }
+        return x;
+    }
+
+    // Plain FP64 drop-in: iterate_to_new_discharge with lateral inflow >= 0 (kinematic_wave.hpp:145-254).
+    __device__ inline double new_discharge_f64(KinematicWaveCell<double> const& cell)
+    {
+        if (cell.q_up + cell.q_cur > 0 || cell.q_lat > 0)
+        {
+            bool converged;
+            return solve_bracketed<double>(cell, converged);
+        }
+        return 0.0;
+    }
+
+    // FP32 path: FP32 iteration, then one FP64 Newton correction of the FP32 root. `q_up` is the
+    // compensated (FP64) upstream sum, `alpha_pow` = alpha * q_cur^beta evaluated in FP64.
+    __device__ inline double new_discharge_f32_corrected(
+        double q_up, float q_cur, float q_lat, float alpha, float beta, float dt, float dx, double alpha_pow)
+    {
+        if (!(q_up + q_cur > 0 || q_lat > 0))
+        {
+            return 0.0;
+        }
+        KinematicWaveCell<float> cell{static_cast<float>(q_up), q_cur, q_lat, alpha, beta, dt, dx,
+                                      static_cast<float>(alpha_pow)};
+        bool converged;
+        float const x32 = solve_bracketed<float>(cell, converged);
+        if (!converged || !(x32 > 0.0f))
+        {
+            // No root to refine: the FP32 iterate is the result (as in LUE's float instantiation).
+            return static_cast<double>(x32);
+        }
+        double const c1 = static_cast<double>(dt) / static_cast<double>(dx);
+        double const known = (c1 * q_up) + alpha_pow + (static_cast<double>(dt) * static_cast<double>(q_lat));
+        double const x = static_cast<double>(x32);
+        double const xb = pow(x, static_cast<double>(beta));
+        double const f0 = (c1 * x) + (static_cast<double>(alpha) * xb) - known;
+        double const f1 = c1 + static_cast<double>(beta) * (static_cast<double>(alpha) * xb) / x;
+        double const corrected = x - f0 / f1;
+        return corrected > 0.0 ? corrected : x;
+    }
+
+}  // namespace lue_amd

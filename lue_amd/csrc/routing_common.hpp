@@ -1,0 +1,58 @@
+// Shared definitions of the flow-routing kernels: ldd decoding, neighbour tables, tile geometry.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lue_amd {
+
+    // ldd no-data value: numeric_limits<uint8>::max()
+    // (reference: source/framework/algorithm/include/lue/framework/algorithm/flow_direction.hpp:7-35)
+    constexpr std::uint8_t no_data_ldd = 255;
+
+    // Neighbour k of a cell, clockwise from north: N, NE, E, SE, S, SW, W, NW.
+    // idx0 is the row (north = -1), idx1 the column.
+    __host__ __device__ constexpr int neighbour_dr(int k)
+    {
+        return (k == 0 || k == 1 || k == 7) ? -1 : ((k == 3 || k == 4 || k == 5) ? 1 : 0);
+    }
+
+    __host__ __device__ constexpr int neighbour_dc(int k)
+    {
+        return (k == 1 || k == 2 || k == 3) ? 1 : ((k == 5 || k == 6 || k == 7) ? -1 : 0);
+    }
+
+    // ldd keypad code (7 8 9 / 4 5 6 / 1 2 3) -> neighbour index of the downstream cell, or -1 for a
+    // sink (5) and for any code that is not a direction, which LUE also treats as a sink
+    // (detail/flow_direction.hpp:11-75 leaves the offsets at zero for such codes).
+    // Nibble `code` of the table holds the direction, 0xF for none.
+    __host__ __device__ inline int ldd_direction(std::uint32_t code)
+    {
+        constexpr std::uint64_t table = 0xFFFFFF10'72F6345Full;
+        // code: 0  1  2  3  4  5  6  7  8  9
+        // dir : F  5  4  3  6  F  2  7  0  1
+        int const d = code < 10 ? static_cast<int>((table >> (4 * code)) & 0xFu) : 0xF;
+        return d == 0xF ? -1 : d;
+    }
+
+    __host__ __device__ constexpr int opposite(int k)
+    {
+        return (k + 4) & 7;
+    }
+
+    // Status codes of the C ABI (include/lue_amd/routing.h).
+    enum Status : int {
+        status_success = 0,
+        status_invalid_argument = 1,
+        status_shape_mismatch = 2,
+        status_out_of_memory = 3,
+        status_hip_error = 4,
+        status_invalid_network = 5,
+        status_no_device = 6,
+    };
+
+    // Operation kinds of the accumulating router.
+    enum Op : int { op_accu = 0, op_threshold = 1, op_fraction = 2, op_kinematic_wave = 3 };
+
+}  // namespace lue_amd

@@ -1,0 +1,603 @@
+// Tiled topological sweep of the accumulating router on gfx950.
+//
+// What is computed: LUE's accumulating router (reference: source/framework/algorithm/include/lue/
+// framework/algorithm/definition/accumulating_router.hpp:525-1209) with one of four cell functors
+// (accu.hpp, accu_threshold.hpp, accu_fraction.hpp, kinematic_wave.hpp). A cell is solved once all of
+// its upstream neighbours are solved; its value is a function of the sum of their outflows and of its
+// own inputs.
+//
+// How. The raster is cut into TW x TH tiles. One launch ("round") processes a set of tiles; a
+// workgroup owns one tile at a time and
+//   1. stages the tile's ldd with a one-cell halo, derives per cell the mask of upstream neighbours
+//      and stages the inputs of the tile's pending cells in LDS (coalesced loads);
+//   2. counts, per pending cell, the upstream neighbours that are not solved yet (neighbours in other
+//      tiles count as solved when their border stamp is from an earlier round);
+//   3. walks: every lane starts at each of its initially ready cells and continues downstream while
+//      it is the last upstream contributor of the next cell (LDS atomics, workgroup scope). The cell
+//      value gathers the upstream outflows in fixed neighbour order (N, NE, E, SE, S, SW, W, NW), so
+//      results do not depend on which lane arrived last;
+//   4. writes the solved cells' outputs coalesced, a pending bitmap and border stamps, and queues
+//      the neighbouring tiles that received flow from a solved cell for the next round.
+// Rounds repeat until no tile is queued. Tiles only exchange data across launches (kernel
+// boundaries), so no device-scope fence runs inside a kernel.
+//
+// Precision. T = double: plain FP64. T = float: outflows carry a compensation term (out + lo is the
+// FP64 sum rounded to FP32 pairs), sums are formed in FP64, so FP32 rounding does not accumulate along
+// long flow paths. Kinematic wave: FP32 Newton + one FP64 correction (kinematic_wave.hpp).
+#pragma once
+
+#include "kinematic_wave.hpp"
+#include "routing_common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <type_traits>
+
+namespace lue_amd {
+
+    constexpr std::uint32_t stamp_pending = 0xFFFFFFFFu;
+
+    template<typename T>
+    struct SweepArgs
+    {
+        std::uint8_t const* ldd;
+        std::int64_t rows;
+        std::int64_t cols;
+        int tiles_x;
+        int tiles_y;
+
+        // Arguments: raster pointer, or nullptr and the scalar.
+        T const* in1;  // accu*: inflow        kinematic wave: current outflow
+        T in1_s;
+        T const* in2;  // threshold | fraction | kinematic wave: lateral inflow
+        T in2_s;
+        T const* alpha;
+        T alpha_s;
+        T const* beta;
+        T beta_s;
+        T const* dx;
+        T dx_s;
+        T dt;
+
+        T* out;            // outflow
+        T* rem;            // remainder (threshold, fraction)
+        float* out_lo;     // compensation of `out` (float path), written for cells read across rounds
+
+        std::uint32_t* pending_bits;   // per tile: TW*TH/32 words, bit set = cell not solved yet
+        std::uint32_t* border_stamp;   // per tile: perimeter cells, round in which the cell was solved
+        std::uint32_t* tile_flag;      // 3 x tiles: queued for round r (index r % 3)
+        std::uint32_t* worklist;       // 3 x tiles
+        std::uint32_t* work_count;     // 3 counters
+        unsigned long long* counters;  // [0] solved cells, [1] no-data cells
+    };
+
+    template<int TW, int TH>
+    struct Tile
+    {
+        static constexpr int n = TW * TH;
+        static constexpr int hw = TW + 2;
+        static constexpr int hn = (TW + 2) * (TH + 2);
+        static constexpr int perimeter = 2 * TW + 2 * (TH - 2);
+        static constexpr int words = n / 32;
+        static_assert(n % 32 == 0, "tile must hold a multiple of 32 cells");
+        static_assert(TW >= 2 && TH >= 2, "tile too small");
+
+        // Index of a border cell on the tile perimeter, -1 for interior cells.
+        __host__ __device__ static int perimeter_index(int lr, int lc)
+        {
+            if (lr == 0)
+                return lc;
+            if (lr == TH - 1)
+                return TW + lc;
+            if (lc == 0)
+                return 2 * TW + (lr - 1);
+            if (lc == TW - 1)
+                return 2 * TW + (TH - 2) + (lr - 1);
+            return -1;
+        }
+    };
+
+    // Per-cell inputs staged in LDS.
+    template<int OP, typename T>
+    struct CellInput;
+
+    template<typename T>
+    struct CellInput<op_accu, T>
+    {
+        T x;  // external inflow, NaN when no-data or outside the domain
+    };
+
+    template<typename T>
+    struct CellInput<op_threshold, T>
+    {
+        T x;  // NaN when either argument is no-data or outside the domain
+        T p;
+    };
+
+    template<typename T>
+    struct CellInput<op_fraction, T>
+    {
+        T x;
+        T p;
+    };
+
+    template<typename T>
+    struct CellInput<op_kinematic_wave, T>
+    {
+        T q_cur;      // NaN when any argument is no-data or outside the domain
+        T q_lat;
+        T alpha;
+        T beta;
+        T dx;
+        double alpha_pow;  // alpha * q_cur^beta in FP64
+    };
+
+    template<int OP, typename T, int TW, int TH>
+    struct SweepLds
+    {
+        using G = Tile<TW, TH>;
+        static constexpr bool compensated = std::is_same_v<T, float>;
+        static constexpr bool two_outputs = OP == op_threshold || OP == op_fraction;
+
+        CellInput<OP, T> in[G::n];
+        T hi[G::n];
+        float lo[compensated ? G::n : 1];
+        T rem[two_outputs ? G::n : 1];
+        std::uint32_t count[G::n / 4];  // four 8-bit counters per word
+        std::uint8_t ldd[G::hn];
+        std::uint8_t upmask[G::n];
+        std::uint8_t state[G::n];  // see state_* below
+        unsigned int solved;
+        unsigned int no_data;
+    };
+
+    // Cell states inside a round.
+    constexpr std::uint8_t state_absent = 0;   // outside the array or ldd no-data
+    constexpr std::uint8_t state_pending = 1;  // not solved yet
+    constexpr std::uint8_t state_solved = 2;   // solved in an earlier round (value in HBM)
+    constexpr std::uint8_t state_new = 3;      // solved in this round (value in LDS)
+
+    template<typename T>
+    __device__ inline T value_of(T const* raster, T scalar, std::int64_t idx)
+    {
+        return raster ? raster[idx] : scalar;
+    }
+
+    template<typename T>
+    __device__ inline T quiet_nan()
+    {
+        return std::numeric_limits<T>::quiet_NaN();
+    }
+
+    template<int OP, typename T>
+    __device__ inline void stage_input(SweepArgs<T> const& a, std::int64_t g, CellInput<OP, T>& c)
+    {
+        if constexpr (OP == op_accu)
+        {
+            T const x = value_of(a.in1, a.in1_s, g);
+            // value_policies/accu.hpp:9-19: inflow >= 0
+            c.x = (x >= T{0}) ? x : quiet_nan<T>();
+        }
+        else if constexpr (OP == op_threshold || OP == op_fraction)
+        {
+            T const x = value_of(a.in1, a.in1_s, g);
+            T const p = value_of(a.in2, a.in2_s, g);
+            // value_policies/accu_threshold.hpp:9-20, value_policies/accu_fraction.hpp:9-20
+            bool const ok = OP == op_threshold ? (x >= T{0} && p >= T{0})
+                                               : (x >= T{0} && p >= T{0} && p <= T{1});
+            c.x = ok ? x : quiet_nan<T>();
+            c.p = p;
+        }
+        else
+        {
+            T const q_cur = value_of(a.in1, a.in1_s, g);
+            T const q_lat = value_of(a.in2, a.in2_s, g);
+            T const alpha = value_of(a.alpha, a.alpha_s, g);
+            T const beta = value_of(a.beta, a.beta_s, g);
+            T const dx = value_of(a.dx, a.dx_s, g);
+            // value_policies/kinematic_wave.hpp:9-22 plus no-data (NaN) in any argument
+            bool const ok = q_cur >= T{0} && q_lat >= T{0} && dx > T{0} && !isnan(alpha) && !isnan(beta) &&
+                            !isnan(a.dt);
+            c.q_cur = ok ? q_cur : quiet_nan<T>();
+            c.q_lat = q_lat;
+            c.alpha = alpha;
+            c.beta = beta;
+            c.dx = dx;
+            c.alpha_pow = ok ? static_cast<double>(alpha) * pow(static_cast<double>(q_cur), static_cast<double>(beta))
+                             : 0.0;
+        }
+    }
+
+    // Solve one cell given the FP64 sum of its upstream outflows.
+    template<int OP, typename T>
+    __device__ inline void solve_cell(
+        SweepArgs<T> const& a, CellInput<OP, T> const& in, double upstream, bool poisoned, double& out, T& rem)
+    {
+        if constexpr (OP == op_accu)
+        {
+            // accu.hpp:102-123: out = sum(upstream) + external inflow
+            out = (poisoned || isnan(in.x)) ? quiet_nan<double>() : upstream + static_cast<double>(in.x);
+        }
+        else if constexpr (OP == op_threshold)
+        {
+            // accu_threshold.hpp:124-169
+            if (poisoned || isnan(in.x) || isnan(in.p))
+            {
+                out = quiet_nan<double>();
+                rem = quiet_nan<T>();
+            }
+            else
+            {
+                double const total = upstream + static_cast<double>(in.x);
+                double const thr = static_cast<double>(in.p);
+                if (total > thr)
+                {
+                    rem = in.p;
+                    out = total - thr;
+                }
+                else
+                {
+                    rem = static_cast<T>(total);
+                    out = 0.0;
+                }
+            }
+        }
+        else if constexpr (OP == op_fraction)
+        {
+            // accu_fraction.hpp:137-162
+            if (poisoned || isnan(in.x) || isnan(in.p))
+            {
+                out = quiet_nan<double>();
+                rem = quiet_nan<T>();
+            }
+            else
+            {
+                double const total = upstream + static_cast<double>(in.x);
+                double const delta = static_cast<double>(in.p) * total;
+                rem = static_cast<T>(total - delta);
+                out = delta;
+            }
+        }
+        else
+        {
+            // kinematic_wave.hpp:416-456
+            if (poisoned || isnan(in.q_cur))
+            {
+                out = quiet_nan<double>();
+            }
+            else if constexpr (std::is_same_v<T, double>)
+            {
+                KinematicWaveCell<double> cell{upstream, in.q_cur, in.q_lat, in.alpha, in.beta, a.dt, in.dx,
+                                               in.alpha_pow};
+                out = new_discharge_f64(cell);
+            }
+            else
+            {
+                out = new_discharge_f32_corrected(
+                    upstream, in.q_cur, in.q_lat, in.alpha, in.beta, a.dt, in.dx, in.alpha_pow);
+            }
+        }
+    }
+
+    // One round over one tile. All threads of the workgroup call it.
+    template<int OP, typename T, int TW, int TH>
+    __device__ void sweep_tile(SweepArgs<T> const& a, SweepLds<OP, T, TW, TH>& s, int tile, std::uint32_t round)
+    {
+        using G = Tile<TW, TH>;
+        constexpr bool compensated = SweepLds<OP, T, TW, TH>::compensated;
+        constexpr bool two_outputs = SweepLds<OP, T, TW, TH>::two_outputs;
+        constexpr int cells_per_thread = (G::n + 255) / 256;
+        static_assert(cells_per_thread <= 32, "ready mask holds 32 cells per thread");
+
+        int const tid = threadIdx.x;
+        int const nthreads = blockDim.x;
+        int const ty = tile / a.tiles_x;
+        int const tx = tile % a.tiles_x;
+        std::int64_t const r0 = static_cast<std::int64_t>(ty) * TH;
+        std::int64_t const c0 = static_cast<std::int64_t>(tx) * TW;
+        std::uint32_t* const pbits = a.pending_bits + static_cast<std::int64_t>(tile) * G::words;
+        std::uint32_t* const stamps = a.border_stamp + static_cast<std::int64_t>(tile) * G::perimeter;
+
+        if (tid == 0)
+        {
+            s.solved = 0;
+            s.no_data = 0;
+        }
+
+        // 1. ldd with halo; cells outside the array read as no-data
+        for (int i = tid; i < G::hn; i += nthreads)
+        {
+            int const hr = i / G::hw;
+            int const hc = i % G::hw;
+            std::int64_t const r = r0 + hr - 1;
+            std::int64_t const c = c0 + hc - 1;
+            s.ldd[i] = (r >= 0 && r < a.rows && c >= 0 && c < a.cols) ? a.ldd[r * a.cols + c] : no_data_ldd;
+        }
+        __syncthreads();
+
+        // 2. state, upstream mask and staged inputs of pending cells
+        for (int i = tid; i < G::n; i += nthreads)
+        {
+            int const lr = i / TW;
+            int const lc = i % TW;
+            int const h = (lr + 1) * G::hw + (lc + 1);
+            std::uint8_t const code = s.ldd[h];
+            std::uint8_t st;
+            std::uint8_t mask = 0;
+            if (code == no_data_ldd)
+            {
+                st = state_absent;
+                std::int64_t const r = r0 + lr;
+                std::int64_t const c = c0 + lc;
+                if (round == 0 && r < a.rows && c < a.cols)
+                {
+                    // mark_no_data (accumulating_router.hpp:866-872)
+                    std::int64_t const g = r * a.cols + c;
+                    a.out[g] = quiet_nan<T>();
+                    if constexpr (two_outputs)
+                    {
+                        a.rem[g] = quiet_nan<T>();
+                    }
+                    atomicAdd(&s.no_data, 1u);
+                }
+            }
+            else
+            {
+                bool const pending = round == 0 || ((pbits[i >> 5] >> (i & 31)) & 1u);
+                st = pending ? state_pending : state_solved;
+                for (int k = 0; k < 8; ++k)
+                {
+                    std::uint8_t const ucode = s.ldd[h + neighbour_dr(k) * G::hw + neighbour_dc(k)];
+                    if (ucode != no_data_ldd && ldd_direction(ucode) == opposite(k))
+                    {
+                        mask |= static_cast<std::uint8_t>(1u << k);
+                    }
+                }
+                if (pending)
+                {
+                    std::int64_t const g = (r0 + lr) * a.cols + (c0 + lc);
+                    stage_input<OP, T>(a, g, s.in[i]);
+                }
+            }
+            s.state[i] = st;
+            s.upmask[i] = mask;
+        }
+        __syncthreads();
+
+        // 3. counts of unsolved upstream neighbours; initially ready cells
+        std::uint32_t ready = 0;
+        for (int j = 0; j < cells_per_thread; ++j)
+        {
+            int const i = tid + j * nthreads;
+            if (i >= G::n)
+                break;
+            std::uint32_t n = 0;
+            if (s.state[i] == state_pending)
+            {
+                int const lr = i / TW;
+                int const lc = i % TW;
+                std::uint8_t const mask = s.upmask[i];
+                for (int k = 0; k < 8; ++k)
+                {
+                    if (!((mask >> k) & 1u))
+                        continue;
+                    int const ur = lr + neighbour_dr(k);
+                    int const uc = lc + neighbour_dc(k);
+                    if (ur >= 0 && ur < TH && uc >= 0 && uc < TW)
+                    {
+                        n += s.state[ur * TW + uc] == state_pending ? 1u : 0u;
+                    }
+                    else
+                    {
+                        // upstream neighbour in another tile: solved when stamped in an earlier round
+                        if (round == 0)
+                        {
+                            ++n;
+                        }
+                        else
+                        {
+                            int const nty = ty + (ur < 0 ? -1 : (ur >= TH ? 1 : 0));
+                            int const ntx = tx + (uc < 0 ? -1 : (uc >= TW ? 1 : 0));
+                            int const nlr = ur < 0 ? TH - 1 : (ur >= TH ? 0 : ur);
+                            int const nlc = uc < 0 ? TW - 1 : (uc >= TW ? 0 : uc);
+                            int const ntile = nty * a.tiles_x + ntx;
+                            std::uint32_t const st =
+                                a.border_stamp[static_cast<std::int64_t>(ntile) * G::perimeter +
+                                               G::perimeter_index(nlr, nlc)];
+                            n += st < round ? 0u : 1u;
+                        }
+                    }
+                }
+                if (n == 0)
+                    ready |= 1u << j;
+            }
+            // 8-bit counter, written as a byte of the packed word
+            reinterpret_cast<std::uint8_t*>(s.count)[i] = static_cast<std::uint8_t>(n);
+        }
+        __syncthreads();
+
+        // 4. walks
+        for (int j = 0; j < cells_per_thread; ++j)
+        {
+            if (!((ready >> j) & 1u))
+                continue;
+            int i = tid + j * nthreads;
+            while (true)
+            {
+                int const lr = i / TW;
+                int const lc = i % TW;
+                std::uint8_t const mask = s.upmask[i];
+                double upstream = 0.0;
+                bool poisoned = false;
+                for (int k = 0; k < 8; ++k)
+                {
+                    if (!((mask >> k) & 1u))
+                        continue;
+                    int const ur = lr + neighbour_dr(k);
+                    int const uc = lc + neighbour_dc(k);
+                    double v;
+                    if (ur >= 0 && ur < TH && uc >= 0 && uc < TW && s.state[ur * TW + uc] == state_new)
+                    {
+                        int const u = ur * TW + uc;
+                        v = static_cast<double>(s.hi[u]);
+                        if constexpr (compensated)
+                            v += static_cast<double>(s.lo[u]);
+                    }
+                    else
+                    {
+                        // solved in an earlier round, value in HBM
+                        std::int64_t const ug = (r0 + ur) * a.cols + (c0 + uc);
+                        v = static_cast<double>(a.out[ug]);
+                        if constexpr (compensated)
+                            v += static_cast<double>(a.out_lo[ug]);
+                    }
+                    if (isnan(v))
+                        poisoned = true;
+                    else
+                        upstream += v;
+                }
+                double out;
+                T rem{};
+                solve_cell<OP, T>(a, s.in[i], upstream, poisoned, out, rem);
+                T const hi = static_cast<T>(out);
+                s.hi[i] = hi;
+                if constexpr (compensated)
+                    s.lo[i] = isnan(out) ? 0.0f : static_cast<float>(out - static_cast<double>(hi));
+                if constexpr (two_outputs)
+                    s.rem[i] = rem;
+
+                int const dir = ldd_direction(s.ldd[(lr + 1) * G::hw + (lc + 1)]);
+                int next = -1;
+                if (dir >= 0)
+                {
+                    int const dr = lr + neighbour_dr(dir);
+                    int const dc = lc + neighbour_dc(dir);
+                    if (dr >= 0 && dr < TH && dc >= 0 && dc < TW &&
+                        s.ldd[(dr + 1) * G::hw + (dc + 1)] != no_data_ldd)
+                    {
+                        next = dr * TW + dc;
+                    }
+                }
+                // publish: the state write is ordered before the counter update (release)
+                __hip_atomic_store(&s.state[i], state_new, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (next < 0)
+                    break;
+                std::uint32_t const shift = 8u * static_cast<std::uint32_t>(next & 3);
+                std::uint32_t const old = __hip_atomic_fetch_sub(
+                    &s.count[next >> 2], 1u << shift, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (((old >> shift) & 0xFFu) != 1u)
+                    break;  // confluence: another stream still has to arrive
+                i = next;
+            }
+        }
+        __syncthreads();
+
+        // 5. write back: outputs of cells solved in this round (coalesced), compensation terms that later
+        // rounds read, pending bitmap, border stamps, and queue the tiles that receive flow
+        std::uint32_t const next_slot = (round + 1) % 3;
+        unsigned int solved = 0;
+        for (int i = tid; i < G::n; i += nthreads)
+        {
+            std::uint8_t const st = s.state[i];
+            int const lr = i / TW;
+            int const lc = i % TW;
+            std::int64_t const r = r0 + lr;
+            std::int64_t const c = c0 + lc;
+            int const p = G::perimeter_index(lr, lc);
+            if (st == state_new)
+            {
+                ++solved;
+                std::int64_t const g = r * a.cols + c;
+                a.out[g] = s.hi[i];
+                if constexpr (two_outputs)
+                    a.rem[g] = s.rem[i];
+                int const dir = ldd_direction(s.ldd[(lr + 1) * G::hw + (lc + 1)]);
+                if (dir >= 0)
+                {
+                    int const dr = lr + neighbour_dr(dir);
+                    int const dc = lc + neighbour_dc(dir);
+                    bool const inside = dr >= 0 && dr < TH && dc >= 0 && dc < TW;
+                    bool const target_ok = s.ldd[(dr + 1) * G::hw + (dc + 1)] != no_data_ldd;
+                    if (target_ok && (!inside || s.state[dr * TW + dc] == state_pending))
+                    {
+                        // read by a later round
+                        if constexpr (compensated)
+                            a.out_lo[g] = s.lo[i];
+                        if (!inside)
+                        {
+                            int const nty = ty + (dr < 0 ? -1 : (dr >= TH ? 1 : 0));
+                            int const ntx = tx + (dc < 0 ? -1 : (dc >= TW ? 1 : 0));
+                            int const ntile = nty * a.tiles_x + ntx;
+                            std::uint32_t* flag = a.tile_flag + static_cast<std::int64_t>(next_slot) *
+                                                                     a.tiles_x * a.tiles_y + ntile;
+                            if (atomicExch(flag, 1u) == 0u)
+                            {
+                                std::uint32_t const slot = atomicAdd(&a.work_count[next_slot], 1u);
+                                a.worklist[static_cast<std::int64_t>(next_slot) * a.tiles_x * a.tiles_y + slot] =
+                                    static_cast<std::uint32_t>(ntile);
+                            }
+                        }
+                    }
+                }
+                if (p >= 0)
+                    stamps[p] = round;
+            }
+            else if (round == 0 && p >= 0)
+            {
+                stamps[p] = st == state_pending ? stamp_pending : 0u;
+            }
+        }
+        // pending bitmap: one word per 32 cells
+        for (int w = tid; w < G::words; w += nthreads)
+        {
+            std::uint32_t bits = 0;
+            for (int b = 0; b < 32; ++b)
+            {
+                bits |= (s.state[w * 32 + b] == state_pending ? 1u : 0u) << b;
+            }
+            pbits[w] = bits;
+        }
+        atomicAdd(&s.solved, solved);
+        __syncthreads();
+        if (tid == 0)
+        {
+            atomicAdd(&a.counters[0], static_cast<unsigned long long>(s.solved));
+            if (round == 0)
+                atomicAdd(&a.counters[1], static_cast<unsigned long long>(s.no_data));
+        }
+        __syncthreads();
+    }
+
+    // Round 0: every tile. Grid = number of tiles.
+    template<int OP, typename T, int TW, int TH>
+    __global__ void __launch_bounds__(256) sweep_first_round(SweepArgs<T> a)
+    {
+        __shared__ SweepLds<OP, T, TW, TH> s;
+        sweep_tile<OP, T, TW, TH>(a, s, blockIdx.x, 0u);
+    }
+
+    // Round r >= 1: the tiles queued by round r - 1. Workgroup b takes items b, b + grid, ...
+    template<int OP, typename T, int TW, int TH>
+    __global__ void __launch_bounds__(256) sweep_next_round(SweepArgs<T> a, std::uint32_t round)
+    {
+        __shared__ SweepLds<OP, T, TW, TH> s;
+        std::uint32_t const slot = round % 3;
+        std::int64_t const ntiles = static_cast<std::int64_t>(a.tiles_x) * a.tiles_y;
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+        {
+            // the slot that round + 1 fills was last read by round - 1
+            a.work_count[(round + 2) % 3] = 0u;
+        }
+        std::uint32_t const count = a.work_count[slot];
+        for (std::uint32_t item = blockIdx.x; item < count; item += gridDim.x)
+        {
+            int const tile = static_cast<int>(a.worklist[slot * ntiles + item]);
+            if (threadIdx.x == 0)
+                a.tile_flag[slot * ntiles + tile] = 0u;
+            sweep_tile<OP, T, TW, TH>(a, s, tile, round);
+        }
+    }
+
+}  // namespace lue_amd

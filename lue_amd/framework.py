@@ -1,0 +1,215 @@
+"""Python front end of the routing path, mirroring lue.framework's routing operations.
+
+Reference bindings: source/framework/python/source/algorithm/routing_operation/{accu,accu_threshold,
+accu_fraction,kinematic_wave,inflow_count}.cpp, which call lue::value_policies::* on PartitionedArray.
+
+PartitionedArray here is device resident: one contiguous row-major raster in HBM (a torch tensor used as
+device memory) plus LUE's partition shape, which is kept as metadata (results do not depend on it). A
+Scalar argument is a Python number or numpy scalar (LUE's Scalar<E> and plain-value overloads).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+
+FLOAT_TYPES = {torch.float32: "f32", torch.float64: "f64"}
+
+
+def _default_device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("lue_amd needs a GPU: no HIP device is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+class PartitionedArray:
+    """Device-resident 2-D array with LUE's partitioning metadata.
+
+    Mirrors lue::PartitionedArray<E, 2> (reference source/framework/partitioned_array/include/lue/framework/
+    partitioned_array_decl.hpp:23-98): shape, partition shape, shape in partitions (the last partition
+    along each dimension absorbs the remainder, domain_decomposition.hpp:91-183).
+    """
+
+    def __init__(self, data: torch.Tensor, partition_shape=None):
+        if data.dim() != 2:
+            raise ValueError("a PartitionedArray is 2-D")
+        if data.device.type != "cuda":
+            raise ValueError("a PartitionedArray lives in device memory")
+        self.data = data.contiguous()
+        shape = tuple(self.data.shape)
+        if partition_shape is None:
+            partition_shape = shape
+        partition_shape = tuple(int(v) for v in partition_shape)
+        if any(p <= 0 or p > s for p, s in zip(partition_shape, shape)):
+            raise ValueError(f"Partition shape {partition_shape} must fit in array shape {shape}")
+        self.partition_shape = partition_shape
+
+    @property
+    def shape(self):
+        return tuple(self.data.shape)
+
+    @property
+    def dtype(self):
+        return self.data.dtype
+
+    @property
+    def shape_in_partitions(self):
+        return tuple(s // p for s, p in zip(self.shape, self.partition_shape))
+
+    @property
+    def nr_partitions(self):
+        a, b = self.shape_in_partitions
+        return a * b
+
+    def numpy(self):
+        return self.data.cpu().numpy()
+
+    @classmethod
+    def from_numpy(cls, array, partition_shape=None, device=None):
+        device = device or _default_device()
+        return cls(torch.from_numpy(np.ascontiguousarray(array)).to(device), partition_shape)
+
+
+def create_partitioned_array(array_shape, partition_shape, dtype, fill_value, device=None):
+    """lue.framework.create_array / lue::create_partitioned_array(shape, partition_shape, fill)."""
+    device = device or _default_device()
+    data = torch.full(tuple(array_shape), fill_value, dtype=dtype, device=device)
+    return PartitionedArray(data, partition_shape)
+
+
+def _stream():
+    return ctypes_stream(torch.cuda.current_stream())
+
+
+def ctypes_stream(stream):
+    return stream.cuda_stream
+
+
+def _ptr(t: torch.Tensor):
+    return t.data_ptr()
+
+
+def _verify_compatible(flow_direction, *arrays):
+    """detail::verify_compatible (ALG/detail/verify_compatible.hpp:22-41): same shape, same partitioning."""
+    for a in arrays:
+        if isinstance(a, PartitionedArray):
+            if a.shape != flow_direction.shape:
+                raise ValueError(f"Shapes differ: {a.shape} != {flow_direction.shape}")
+            if a.shape_in_partitions != flow_direction.shape_in_partitions:
+                raise ValueError(f"Shapes in partitions differ: {a.shape_in_partitions} != "
+                                 f"{flow_direction.shape_in_partitions}")
+            if a.data.device != flow_direction.data.device:
+                raise ValueError("arrays live on different devices")
+
+
+def _element_type(*args):
+    types = {a.dtype for a in args if isinstance(a, PartitionedArray)}
+    if len(types) > 1:
+        raise TypeError(f"material arguments must share one element type, got {types}")
+    dtype = types.pop() if types else torch.float32
+    if dtype not in FLOAT_TYPES:
+        raise TypeError(f"material element type must be float32 or float64, not {dtype}")
+    return dtype
+
+
+def _arg(value, dtype):
+    """(pointer or None, scalar) for a raster or Scalar argument."""
+    if isinstance(value, PartitionedArray):
+        if value.dtype != dtype:
+            raise TypeError(f"expected {dtype}, got {value.dtype}")
+        return _ptr(value.data), 0.0
+    return None, float(value)
+
+
+def _check_ldd(flow_direction):
+    if not isinstance(flow_direction, PartitionedArray) or flow_direction.dtype != torch.uint8:
+        raise TypeError("flow_direction must be a PartitionedArray of uint8")
+
+
+def _output(flow_direction, dtype):
+    return PartitionedArray(torch.empty(flow_direction.shape, dtype=dtype, device=flow_direction.data.device),
+                            flow_direction.partition_shape)
+
+
+def inflow_count(flow_direction):
+    """lue::inflow_count<uint8>(flow_direction) (ALG/inflow_count.hpp:29-42)."""
+    _check_ldd(flow_direction)
+    lib = _lib.load()
+    out = _output(flow_direction, torch.uint8)
+    rows, cols = flow_direction.shape
+    _lib.check(lib.lue_amd_inflow_count_u8(_ptr(flow_direction.data), rows, cols, _ptr(out.data), _stream()))
+    return out
+
+
+def accu(flow_direction, inflow):
+    """lue.framework.accu(flow_direction, inflow) -> outflow (value_policies/accu.hpp:33-55)."""
+    _check_ldd(flow_direction)
+    _verify_compatible(flow_direction, inflow)
+    dtype = _element_type(inflow)
+    lib = _lib.load()
+    out = _output(flow_direction, dtype)
+    p, s = _arg(inflow, dtype)
+    rows, cols = flow_direction.shape
+    fn = getattr(lib, f"lue_amd_accu_{FLOAT_TYPES[dtype]}")
+    _lib.check(fn(_ptr(flow_direction.data), rows, cols, p, s, _ptr(out.data), _stream()))
+    return out
+
+
+def _split(name, flow_direction, inflow, param):
+    _check_ldd(flow_direction)
+    _verify_compatible(flow_direction, inflow, param)
+    dtype = _element_type(inflow, param)
+    lib = _lib.load()
+    out = _output(flow_direction, dtype)
+    rem = _output(flow_direction, dtype)
+    p1, s1 = _arg(inflow, dtype)
+    p2, s2 = _arg(param, dtype)
+    rows, cols = flow_direction.shape
+    fn = getattr(lib, f"lue_amd_{name}_{FLOAT_TYPES[dtype]}")
+    _lib.check(fn(_ptr(flow_direction.data), rows, cols, p1, s1, p2, s2, _ptr(out.data), _ptr(rem.data),
+                  _stream()))
+    return out, rem
+
+
+def accu_threshold(flow_direction, inflow, threshold):
+    """lue.framework.accu_threshold -> (outflow, remainder) (value_policies/accu_threshold.hpp:34-87)."""
+    return _split("accu_threshold", flow_direction, inflow, threshold)
+
+
+def accu_fraction(flow_direction, inflow, fraction):
+    """lue.framework.accu_fraction -> (outflow, remainder) (value_policies/accu_fraction.hpp)."""
+    return _split("accu_fraction", flow_direction, inflow, fraction)
+
+
+def kinematic_wave(flow_direction, current_outflow, inflow, alpha, beta, time_step_duration, channel_length):
+    """lue.framework.kinematic_wave -> new outflow (value_policies/kinematic_wave.hpp:43-91)."""
+    _check_ldd(flow_direction)
+    if not isinstance(current_outflow, PartitionedArray) or not isinstance(inflow, PartitionedArray):
+        raise TypeError("current_outflow and inflow must be PartitionedArrays")
+    _verify_compatible(flow_direction, current_outflow, inflow, alpha, beta, channel_length)
+    dtype = _element_type(current_outflow, inflow, alpha, beta, channel_length)
+    if isinstance(time_step_duration, PartitionedArray):
+        raise TypeError("time_step_duration is a Scalar")
+    lib = _lib.load()
+    out = _output(flow_direction, dtype)
+    pa, sa = _arg(alpha, dtype)
+    pb, sb = _arg(beta, dtype)
+    pl, sl = _arg(channel_length, dtype)
+    rows, cols = flow_direction.shape
+    fn = getattr(lib, f"lue_amd_kinematic_wave_{FLOAT_TYPES[dtype]}")
+    _lib.check(fn(_ptr(flow_direction.data), rows, cols, _ptr(current_outflow.data), _ptr(inflow.data), pa, sa,
+                  pb, sb, float(time_step_duration), pl, sl, _ptr(out.data), _stream()))
+    return out
+
+
+def iterate_to_new_discharge(upstream, current, lateral, alpha, beta, dt, dx):
+    """Elementwise lue::detail::iterate_to_new_discharge over device tensors of one float type."""
+    tensors = [upstream, current, lateral, alpha, beta, dt, dx]
+    dtype = upstream.dtype
+    tensors = [t.to(dtype).contiguous() for t in tensors]
+    lib = _lib.load()
+    out = torch.empty_like(tensors[0])
+    fn = getattr(lib, f"lue_amd_iterate_to_new_discharge_{FLOAT_TYPES[dtype]}")
+    _lib.check(fn(*[_ptr(t) for t in tensors], _ptr(out), tensors[0].numel(), _stream()))
+    return out

@@ -1,0 +1,167 @@
+"""ctypes front end of the oracle (CPU restatement of LUE's flow-routing path, lue_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by bench.py's CPU-baseline
+leg, as the checker. The product path (package lue_amd) never imports it.
+
+Every function takes whole rasters (row-major numpy arrays) plus LUE's partition shape, which fixes the
+partition-phase order of the restated router (domain_decomposition.hpp:91-183 semantics: the last
+partition along each dimension absorbs the remainder).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblue_oracle.so")
+_lib = None
+
+ERRORS = {0: "success", 1: "invalid argument", 5: "invalid flow network (cycle)"}
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def build() -> str:
+    """Compile the oracle with make (g++ -O3)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "liblue_oracle.so"], check=True)
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    p = ctypes.c_void_p
+    i64 = ctypes.c_int64
+    ci = ctypes.c_int
+    for suffix, ct in (("f32", ctypes.c_float), ("f64", ctypes.c_double)):
+        getattr(lib, f"oracle_accu_{suffix}").argtypes = [p, i64, i64, i64, i64, p, ct, p, ci]
+        for op in ("threshold", "fraction"):
+            getattr(lib, f"oracle_accu_{op}_{suffix}").argtypes = [p, i64, i64, i64, i64, p, ct, p, ct, p, p, ci]
+        getattr(lib, f"oracle_kinematic_wave_{suffix}").argtypes = [
+            p, i64, i64, i64, i64, p, p, p, ct, p, ct, ct, p, ct, p, ci]
+        f = getattr(lib, f"oracle_iterate_to_new_discharge_{suffix}")
+        f.argtypes = [ct] * 7
+        f.restype = ct
+    lib.oracle_inflow_count.argtypes = [p, i64, i64, i64, i64, p, ci]
+    _lib = lib
+    return lib
+
+
+def default_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _dtype_suffix(dtype) -> tuple[str, type]:
+    dtype = np.dtype(dtype)
+    if dtype == np.float32:
+        return "f32", ctypes.c_float
+    if dtype == np.float64:
+        return "f64", ctypes.c_double
+    raise TypeError(f"unsupported element type {dtype}")
+
+
+def _raster(a, shape, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    if a.shape != tuple(shape):
+        raise ValueError(f"Shapes differ: {a.shape} != {tuple(shape)}")
+    return a
+
+
+def _arg(value, shape, dtype):
+    """Raster (numpy array) or Scalar (python number) argument -> (pointer, scalar, keepalive)."""
+    if np.isscalar(value) or (isinstance(value, np.ndarray) and value.ndim == 0):
+        return None, float(value), None
+    a = _raster(value, shape, dtype)
+    return a.ctypes.data, 0.0, a
+
+
+def _check(rc):
+    if rc != 0:
+        raise OracleError(ERRORS.get(rc, f"error {rc}"))
+
+
+def _prep(ldd, partition_shape):
+    ldd = np.ascontiguousarray(ldd, dtype=np.uint8)
+    if ldd.ndim != 2:
+        raise ValueError("ldd must be 2-D")
+    rows, cols = ldd.shape
+    prows, pcols = partition_shape if partition_shape is not None else (rows, cols)
+    return ldd, rows, cols, prows, pcols
+
+
+def inflow_count(ldd, partition_shape=None, nthreads=None):
+    lib = _load()
+    ldd, rows, cols, prows, pcols = _prep(ldd, partition_shape)
+    out = np.empty((rows, cols), dtype=np.uint8)
+    _check(lib.oracle_inflow_count(ldd.ctypes.data, rows, cols, prows, pcols, out.ctypes.data,
+                                   nthreads or default_threads()))
+    return out
+
+
+def accu(ldd, inflow, partition_shape=None, dtype=np.float32, nthreads=None):
+    lib = _load()
+    ldd, rows, cols, prows, pcols = _prep(ldd, partition_shape)
+    sfx, _ = _dtype_suffix(dtype)
+    pin, sin, keep = _arg(inflow, (rows, cols), dtype)
+    out = np.empty((rows, cols), dtype=dtype)
+    _check(getattr(lib, f"oracle_accu_{sfx}")(ldd.ctypes.data, rows, cols, prows, pcols, pin, sin,
+                                              out.ctypes.data, nthreads or default_threads()))
+    del keep
+    return out
+
+
+def _accu2(op, ldd, inflow, second, partition_shape, dtype, nthreads):
+    lib = _load()
+    ldd, rows, cols, prows, pcols = _prep(ldd, partition_shape)
+    sfx, _ = _dtype_suffix(dtype)
+    pin, sin, k1 = _arg(inflow, (rows, cols), dtype)
+    p2, s2, k2 = _arg(second, (rows, cols), dtype)
+    out = np.empty((rows, cols), dtype=dtype)
+    rem = np.empty((rows, cols), dtype=dtype)
+    _check(getattr(lib, f"oracle_accu_{op}_{sfx}")(ldd.ctypes.data, rows, cols, prows, pcols, pin, sin, p2,
+                                                   s2, out.ctypes.data, rem.ctypes.data,
+                                                   nthreads or default_threads()))
+    del k1, k2
+    return out, rem
+
+
+def accu_threshold(ldd, inflow, threshold, partition_shape=None, dtype=np.float32, nthreads=None):
+    return _accu2("threshold", ldd, inflow, threshold, partition_shape, dtype, nthreads)
+
+
+def accu_fraction(ldd, inflow, fraction, partition_shape=None, dtype=np.float32, nthreads=None):
+    return _accu2("fraction", ldd, inflow, fraction, partition_shape, dtype, nthreads)
+
+
+def kinematic_wave(ldd, current_outflow, inflow, alpha, beta, time_step_duration, channel_length,
+                   partition_shape=None, dtype=np.float64, nthreads=None):
+    lib = _load()
+    ldd, rows, cols, prows, pcols = _prep(ldd, partition_shape)
+    sfx, _ = _dtype_suffix(dtype)
+    shape = (rows, cols)
+    qc = _raster(current_outflow, shape, dtype)
+    ql = _raster(inflow, shape, dtype)
+    pa, sa, ka = _arg(alpha, shape, dtype)
+    pb, sb, kb = _arg(beta, shape, dtype)
+    pl, sl, kl = _arg(channel_length, shape, dtype)
+    out = np.empty(shape, dtype=dtype)
+    _check(getattr(lib, f"oracle_kinematic_wave_{sfx}")(
+        ldd.ctypes.data, rows, cols, prows, pcols, qc.ctypes.data, ql.ctypes.data, pa, sa, pb, sb,
+        float(time_step_duration), pl, sl, out.ctypes.data, nthreads or default_threads()))
+    del ka, kb, kl
+    return out
+
+
+def iterate_to_new_discharge(upstream, current, lateral, alpha, beta, dt, dx, dtype=np.float64):
+    lib = _load()
+    sfx, _ = _dtype_suffix(dtype)
+    return getattr(lib, f"oracle_iterate_to_new_discharge_{sfx}")(upstream, current, lateral, alpha, beta, dt, dx)

@@ -1,0 +1,184 @@
+"""Independent brute-force models of the routing operations, for tiny rasters.
+
+These do not restate LUE's router: they use other algorithms (walk from every cell to its outlet;
+Kahn's algorithm over the cell graph; closed forms) so that a plausible mistake in the oracle or in the
+GPU path (a dropped term, a wrong neighbour offset, a transposed index, a wrong split) shows up as a
+difference. Pure Python loops: only for small inputs.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+OFFSETS = {7: (-1, -1), 8: (-1, 0), 9: (-1, 1), 4: (0, -1), 6: (0, 1), 1: (1, -1), 2: (1, 0), 3: (1, 1)}
+ND = 255
+
+
+def downstream(ldd, r, c):
+    """Downstream cell of (r, c) or None for sinks, flow off the array and flow into no-data."""
+    code = int(ldd[r, c])
+    if code not in OFFSETS:
+        return None
+    dr, dc = OFFSETS[code]
+    r2, c2 = r + dr, c + dc
+    rows, cols = ldd.shape
+    if not (0 <= r2 < rows and 0 <= c2 < cols) or ldd[r2, c2] == ND:
+        return None
+    return r2, c2
+
+
+def random_ldd(rng, rows, cols, nodata_fraction=0.0, pit_fraction=0.0, flat_fraction=0.0):
+    """Acyclic ldd: steepest descent on a random surface; optional no-data, pits and random codes on
+    'flat' cells that still point downhill (a cell only points to a strictly lower neighbour)."""
+    z = rng.random((rows, cols))
+    ldd = np.full((rows, cols), 5, dtype=np.uint8)
+    code_of = {v: k for k, v in OFFSETS.items()}
+    for r in range(rows):
+        for c in range(cols):
+            best, best_drop = None, 0.0
+            lower = []
+            for (dr, dc) in code_of:
+                r2, c2 = r + dr, c + dc
+                if 0 <= r2 < rows and 0 <= c2 < cols:
+                    drop = (z[r, c] - z[r2, c2]) / (math.sqrt(2.0) if dr and dc else 1.0)
+                    if drop > 0:
+                        lower.append((dr, dc))
+                    if drop > best_drop:
+                        best, best_drop = (dr, dc), drop
+            if best is None:
+                # no lower neighbour: a pit, or flow off the array edge when on the border
+                if r == 0:
+                    ldd[r, c] = 8
+                elif r == rows - 1:
+                    ldd[r, c] = 2
+                elif c == 0:
+                    ldd[r, c] = 4
+                elif c == cols - 1:
+                    ldd[r, c] = 6
+                else:
+                    ldd[r, c] = 5
+                continue
+            if lower and rng.random() < flat_fraction:
+                best = lower[rng.integers(len(lower))]
+            ldd[r, c] = code_of[best]
+            if rng.random() < pit_fraction:
+                ldd[r, c] = 5
+    if nodata_fraction > 0:
+        ldd[rng.random((rows, cols)) < nodata_fraction] = ND
+    return ldd
+
+
+def inflow_count(ldd):
+    rows, cols = ldd.shape
+    out = np.zeros((rows, cols), dtype=np.int64)
+    for r in range(rows):
+        for c in range(cols):
+            if ldd[r, c] == ND:
+                continue
+            d = downstream(ldd, r, c)
+            if d is not None:
+                out[d] += 1
+    out[ldd == ND] = 255
+    return out.astype(np.uint8)
+
+
+def accu_by_walking(ldd, material):
+    """Every cell adds its material to each cell on its path to the outlet (O(N * L))."""
+    rows, cols = ldd.shape
+    out = np.zeros((rows, cols), dtype=np.float64)
+    nan_mark = np.zeros((rows, cols), dtype=bool)
+    for r in range(rows):
+        for c in range(cols):
+            if ldd[r, c] == ND:
+                continue
+            m = float(material[r, c]) if np.ndim(material) else float(material)
+            bad = math.isnan(m) or m < 0
+            cell = (r, c)
+            while cell is not None:
+                if bad:
+                    nan_mark[cell] = True
+                else:
+                    out[cell] += m
+                cell = downstream(ldd, *cell)
+    out[nan_mark] = np.nan
+    out[ldd == ND] = np.nan
+    return out
+
+
+def topological_order(ldd):
+    """Kahn's algorithm over cells (not LUE's walk order)."""
+    rows, cols = ldd.shape
+    indeg = np.zeros((rows, cols), dtype=np.int64)
+    for r in range(rows):
+        for c in range(cols):
+            if ldd[r, c] != ND:
+                d = downstream(ldd, r, c)
+                if d is not None:
+                    indeg[d] += 1
+    ready = [(r, c) for r in range(rows) for c in range(cols) if ldd[r, c] != ND and indeg[r, c] == 0]
+    order = []
+    while ready:
+        cell = ready.pop()
+        order.append(cell)
+        d = downstream(ldd, *cell)
+        if d is not None:
+            indeg[d] -= 1
+            if indeg[d] == 0:
+                ready.append(d)
+    return order
+
+
+def _value(x, r, c):
+    return float(x[r, c]) if np.ndim(x) else float(x)
+
+
+def route_split(ldd, material, param, kind):
+    """accu_threshold / accu_fraction over a topological order; returns (outflow, remainder)."""
+    rows, cols = ldd.shape
+    inflow = np.zeros((rows, cols))
+    out = np.full((rows, cols), np.nan)
+    rem = np.full((rows, cols), np.nan)
+    poisoned = np.zeros((rows, cols), dtype=bool)
+    for (r, c) in topological_order(ldd):
+        m = _value(material, r, c)
+        p = _value(param, r, c)
+        valid = not (math.isnan(m) or math.isnan(p) or m < 0 or p < 0 or (kind == "fraction" and p > 1))
+        if poisoned[r, c] or not valid:
+            poisoned[r, c] = True
+        else:
+            total = inflow[r, c] + m
+            if kind == "threshold":
+                o, rm = (total - p, p) if total > p else (0.0, total)
+            else:
+                o, rm = p * total, total - p * total
+            out[r, c], rem[r, c] = o, rm
+        d = downstream(ldd, r, c)
+        if d is not None:
+            if poisoned[r, c]:
+                poisoned[d] = True
+            else:
+                inflow[d] += out[r, c]
+    return out, rem
+
+
+def kinematic_wave_linear(ldd, q_cur, q_lat, alpha, dt, dx):
+    """kinematic_wave with beta = 1: the equation (dt/dx) Q + alpha Q = (dt/dx) Q_up + alpha Q_cur + dt q
+    is linear, so Q = K / (dt/dx + alpha). Topological order, no Newton iteration."""
+    rows, cols = ldd.shape
+    up = np.zeros((rows, cols))
+    out = np.full((rows, cols), np.nan)
+    for (r, c) in topological_order(ldd):
+        a = _value(alpha, r, c)
+        k = (dt / dx) * up[r, c] + a * float(q_cur[r, c]) + dt * max(float(q_lat[r, c]), 0.0)
+        out[r, c] = k / (dt / dx + a) if (up[r, c] + q_cur[r, c] > 0 or q_lat[r, c] > 0) else 0.0
+        d = downstream(ldd, r, c)
+        if d is not None:
+            up[d] += out[r, c]
+    return out
+
+
+def kinematic_residual(q_new, q_up, q_cur, q_lat, alpha, beta, dt, dx):
+    """f(Q) of the kinematic-wave equation, relative to its known terms."""
+    known = (dt / dx) * q_up + alpha * q_cur ** beta + dt * q_lat
+    return ((dt / dx) * q_new + alpha * q_new ** beta - known) / np.maximum(known, 1e-300)

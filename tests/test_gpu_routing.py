@@ -1,0 +1,252 @@
+"""GPU parity of the HIP routing path (through the C ABI) against LUE's golden vectors and the oracle."""
+import numpy as np
+import pytest
+
+import reference_models as rm
+from conftest import assert_close_boost, assert_equal_nan, golden_arg, load_golden, raster
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def fw():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lue_amd import framework
+
+    return framework
+
+
+def pa(fw, array, partition_shape=None):
+    return fw.PartitionedArray.from_numpy(array, partition_shape)
+
+
+def arg(fw, value, partition_shape):
+    return pa(fw, value, partition_shape) if isinstance(value, np.ndarray) else value
+
+
+# ---------------------------------------------------------------------------------------------------
+# LUE's own test vectors
+
+@pytest.mark.parametrize("case", load_golden("inflow_count"), ids=lambda c: c["name"])
+def test_inflow_count_golden(fw, case):
+    ldd = pa(fw, raster(case["ldd"], np.uint8), case["partition_shape"])
+    got = fw.inflow_count(ldd).numpy()
+    assert_equal_nan(got, raster(case["expected"]["count"]["raster"], np.uint8))
+
+
+ROUTING_CASES = [c for op in ("accu", "accu_threshold", "accu_fraction", "kinematic_wave")
+                 for c in load_golden(op) if c["kind"] != "iterate_to_new_discharge"]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("case", ROUTING_CASES, ids=lambda c: f"{c['kind']}-{c['name']}")
+def test_routing_golden(fw, case, dtype):
+    ps = case["partition_shape"]
+    ldd = pa(fw, raster(case["ldd"], np.uint8), ps)
+    args = {k: arg(fw, golden_arg(v, dtype), ps) for k, v in case["args"].items()}
+    kind = case["kind"]
+    if kind == "accu":
+        got = {"outflow": fw.accu(ldd, args["inflow"])}
+    elif kind == "accu_threshold":
+        out, rem = fw.accu_threshold(ldd, args["inflow"], args["threshold"])
+        got = {"outflow": out, "remainder": rem}
+    elif kind == "accu_fraction":
+        out, rem = fw.accu_fraction(ldd, args["inflow"], args["fraction"])
+        got = {"outflow": out, "remainder": rem}
+    else:
+        shape = ldd.shape
+        for key in ("current_outflow", "inflow"):
+            if not isinstance(args[key], fw.PartitionedArray):
+                args[key] = pa(fw, np.full(shape, args[key], dtype), ps)
+        got = {"outflow": fw.kinematic_wave(ldd, args["current_outflow"], args["inflow"], args["alpha"],
+                                            args["beta"], args["time_step_duration"], args["channel_length"])}
+    got = {k: v.numpy() for k, v in got.items()}
+    if case.get("property") == "all_positive":
+        assert (got["outflow"] > 0).all()
+    for key, exp in case["expected"].items():
+        want = raster(exp["raster"], dtype)
+        if exp["compare"] == "equal":
+            assert_equal_nan(got[key], want)
+        else:
+            assert_close_boost(got[key], want, 1e-6)
+
+
+NEWTON_CASES = [c for c in load_golden("kinematic_wave") if c["kind"] == "iterate_to_new_discharge"]
+
+
+@pytest.mark.parametrize("case", NEWTON_CASES, ids=lambda c: c["name"])
+def test_iterate_to_new_discharge_golden(fw, case):
+    dtype = torch.float64 if case["element"] == "double" else torch.float32
+    args = np.array([c["args"] for c in case["calls"]], dtype=np.float64)
+    cols = [torch.tensor(args[:, k], dtype=dtype, device="cuda") for k in range(7)]
+    got = fw.iterate_to_new_discharge(*cols).cpu().numpy()
+    want = np.array([c["want"] for c in case["calls"]])
+    assert_close_boost(got, want, 1e-6)
+
+
+# ---------------------------------------------------------------------------------------------------
+# Random networks against the oracle (and brute force), several partition shapes
+
+SHAPES = [((9, 9), (3, 3)), ((37, 53), (10, 20)), ((130, 70), (64, 64)), ((200, 300), (50, 60))]
+
+
+def rel_err(got, want):
+    got = np.asarray(got, np.float64)
+    want = np.asarray(want, np.float64)
+    both_nan = np.isnan(got) & np.isnan(want)
+    assert (np.isnan(got) == np.isnan(want)).all(), "no-data cells differ"
+    err = np.abs(got - want) / np.maximum(np.abs(want), 1e-30)
+    return np.where(both_nan, 0.0, err)
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("shape,pshape", SHAPES)
+def test_inflow_count_random(fw, oracle_mod, seed, shape, pshape):
+    rng = np.random.default_rng(seed)
+    ldd = rm.random_ldd(rng, *shape, nodata_fraction=0.05, pit_fraction=0.02, flat_fraction=0.3)
+    got = fw.inflow_count(pa(fw, ldd, pshape)).numpy()
+    assert_equal_nan(got, oracle_mod.inflow_count(ldd, pshape))
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("shape,pshape", SHAPES)
+def test_accu_random(fw, oracle_mod, seed, shape, pshape):
+    rng = np.random.default_rng(10 + seed)
+    ldd = rm.random_ldd(rng, *shape, nodata_fraction=0.03, pit_fraction=0.02, flat_fraction=0.3)
+    material = rng.random(shape)
+    material[rng.random(shape) < 0.01] = np.nan
+    material[rng.random(shape) < 0.01] = -1.0
+    want = oracle_mod.accu(ldd, material, pshape, np.float64)
+    got64 = fw.accu(pa(fw, ldd, pshape), pa(fw, material, pshape)).numpy()
+    assert rel_err(got64, want).max() <= 1e-12
+    got32 = fw.accu(pa(fw, ldd, pshape), pa(fw, material.astype(np.float32), pshape)).numpy()
+    want32in = oracle_mod.accu(ldd, material.astype(np.float32).astype(np.float64), pshape, np.float64)
+    assert rel_err(got32, want32in).max() <= 1e-6
+    # upstream cell count, exact
+    ones = fw.accu(pa(fw, ldd, pshape), 1.0).numpy()
+    assert_equal_nan(ones, rm.accu_by_walking(ldd, 1.0).astype(np.float32))
+
+
+@pytest.mark.parametrize("kind", ["threshold", "fraction"])
+@pytest.mark.parametrize("seed", range(2))
+@pytest.mark.parametrize("shape,pshape", SHAPES)
+def test_split_random(fw, oracle_mod, kind, seed, shape, pshape):
+    rng = np.random.default_rng(20 + seed)
+    ldd = rm.random_ldd(rng, *shape, nodata_fraction=0.03, flat_fraction=0.3)
+    material = rng.random(shape)
+    param = rng.random(shape) * (0.5 if kind == "threshold" else 1.0)
+    param[rng.random(shape) < 0.01] = np.nan
+    fn = fw.accu_threshold if kind == "threshold" else fw.accu_fraction
+    ofn = oracle_mod.accu_threshold if kind == "threshold" else oracle_mod.accu_fraction
+    want_out, want_rem = ofn(ldd, material, param, pshape, np.float64)
+    out, rem = fn(pa(fw, ldd, pshape), pa(fw, material, pshape), pa(fw, param, pshape))
+    # out = total - threshold cancels: compare against the magnitude of the total material
+    scale = np.nan_to_num(np.abs(want_out) + np.abs(want_rem)) + 1e-30
+    assert (np.isnan(out.numpy()) == np.isnan(want_out)).all()
+    assert np.nanmax(np.abs(out.numpy() - want_out) / scale) <= 1e-12
+    assert np.nanmax(np.abs(rem.numpy() - want_rem) / scale) <= 1e-12
+
+
+def kinwave_inputs(rng, shape, dtype):
+    q_cur = (rng.random(shape) * 5).astype(dtype)
+    q_lat = (rng.random(shape) * 1e-4).astype(dtype)
+    alpha = (1.5 + rng.random(shape)).astype(dtype)
+    return q_cur, q_lat, alpha
+
+
+@pytest.mark.parametrize("seed", range(2))
+@pytest.mark.parametrize("shape,pshape", SHAPES)
+def test_kinematic_wave_random(fw, oracle_mod, seed, shape, pshape):
+    rng = np.random.default_rng(30 + seed)
+    ldd = rm.random_ldd(rng, *shape, nodata_fraction=0.02, flat_fraction=0.3)
+    q_cur, q_lat, alpha = kinwave_inputs(rng, shape, np.float64)
+    want = oracle_mod.kinematic_wave(ldd, q_cur, q_lat, alpha, 0.6, 3600.0, 100.0, pshape, np.float64)
+    got64 = fw.kinematic_wave(pa(fw, ldd, pshape), pa(fw, q_cur, pshape), pa(fw, q_lat, pshape),
+                              pa(fw, alpha, pshape), 0.6, 3600.0, 100.0).numpy()
+    assert rel_err(got64, want).max() <= 1e-12
+    # FP32 inputs: compare with the FP64 oracle on the same (FP32-representable) inputs
+    f32 = [a.astype(np.float32) for a in (q_cur, q_lat, alpha)]
+    want32in = oracle_mod.kinematic_wave(ldd, *[a.astype(np.float64) for a in f32], 0.6, 3600.0, 100.0, pshape,
+                                         np.float64)
+    got32 = fw.kinematic_wave(pa(fw, ldd, pshape), *[pa(fw, a, pshape) for a in f32], 0.6, 3600.0,
+                              100.0).numpy()
+    assert rel_err(got32, want32in).max() <= 1e-6
+
+
+def test_kinematic_wave_linear_closed_form(fw):
+    rng = np.random.default_rng(5)
+    ldd = rm.random_ldd(rng, 40, 50, flat_fraction=0.3)
+    q_cur, q_lat, alpha = kinwave_inputs(rng, ldd.shape, np.float64)
+    want = rm.kinematic_wave_linear(ldd, q_cur, q_lat, alpha, 3600.0, 100.0)
+    got = fw.kinematic_wave(pa(fw, ldd), pa(fw, q_cur), pa(fw, q_lat), pa(fw, alpha), 1.0, 3600.0, 100.0)
+    np.testing.assert_allclose(got.numpy(), want, rtol=1e-12)
+
+
+def test_cycle_is_reported(fw):
+    from lue_amd import _lib
+
+    ldd = np.full((70, 70), 2, dtype=np.uint8)
+    ldd[10, 10], ldd[10, 11], ldd[11, 11], ldd[11, 10] = 6, 2, 4, 8  # 2x2 loop
+    with pytest.raises(_lib.InvalidNetworkError):
+        fw.accu(pa(fw, ldd), 1.0)
+
+
+def test_shape_mismatch_raises(fw):
+    ldd = pa(fw, np.full((8, 8), 2, dtype=np.uint8), (4, 4))
+    with pytest.raises(ValueError, match="Shapes differ"):
+        fw.accu(ldd, pa(fw, np.ones((8, 9), np.float32), (4, 4)))
+
+
+# ---------------------------------------------------------------------------------------------------
+# Workload-shaped inputs (synthetic DEM) and the device generators
+
+def test_device_generators_match_host():
+    from lue_amd import synthetic
+
+    ldd = synthetic.ldd_device(20261203, 300, 700).cpu().numpy()
+    assert np.array_equal(ldd, synthetic.ldd_host(20261203, 300, 700))
+    u = synthetic.uniform_device(3, 1, 0.0, 5.0, (300, 700)).cpu().numpy()
+    assert np.array_equal(u, synthetic.uniform_host(3, 1, 0.0, 5.0, (300, 700), np.float32))
+    u64 = synthetic.uniform_device(3, 2, 1.5, 2.5, (30, 70), torch.float64).cpu().numpy()
+    assert np.array_equal(u64, synthetic.uniform_host(3, 2, 1.5, 2.5, (30, 70), np.float64))
+
+
+def test_config0_accu_1000(fw, oracle_mod):
+    """Plain accu on one 1000 x 1000 partition of the synthetic DEM, material f32 U[0, 1)."""
+    from lue_amd import synthetic
+
+    ldd = synthetic.ldd_host(20261203, 1000, 1000)
+    material = synthetic.uniform_host(20261203, synthetic.STREAM_MATERIAL, 0.0, 1.0, (1000, 1000))
+    got = fw.accu(pa(fw, ldd), pa(fw, material)).numpy()
+    want64 = oracle_mod.accu(ldd, material.astype(np.float64), None, np.float64)
+    want32 = oracle_mod.accu(ldd, material, None, np.float32)  # LUE's own float arithmetic
+    assert rel_err(got, want64).max() <= 1e-6
+    assert rel_err(want32, want64).max() > rel_err(got, want64).max()  # compensation pays off
+    ones = fw.accu(pa(fw, ldd), 1.0).numpy()
+    assert ones[-1].sum() == 1000 * 1000  # every cell drains to the bottom row
+
+
+def test_config1_kinematic_wave_2048(fw, oracle_mod):
+    """kinematic_wave on the synthetic DEM (2048^2, 1024^2 partitions): FP32 path vs FP64 oracle."""
+    from lue_amd import synthetic
+
+    n = 2048
+    seed = 20261204
+    ldd = synthetic.ldd_host(seed, n, n)
+    q_cur = synthetic.uniform_host(seed, synthetic.STREAM_CURRENT, 0.0, 5.0, (n, n), np.float64)
+    q_lat = synthetic.uniform_host(seed, synthetic.STREAM_LATERAL, 0.0, 1e-4, (n, n), np.float64)
+    alpha = synthetic.uniform_host(seed, synthetic.STREAM_ALPHA, 1.5, 2.5, (n, n), np.float64)
+    ps = (1024, 1024)
+    want = oracle_mod.kinematic_wave(ldd, q_cur, q_lat, alpha, 0.6, 3600.0, 100.0, ps, np.float64)
+    got64 = fw.kinematic_wave(pa(fw, ldd, ps), pa(fw, q_cur, ps), pa(fw, q_lat, ps), pa(fw, alpha, ps), 0.6,
+                              3600.0, 100.0).numpy()
+    assert rel_err(got64, want).max() <= 1e-12
+    f32 = [a.astype(np.float32) for a in (q_cur, q_lat, alpha)]
+    want32in = oracle_mod.kinematic_wave(ldd, *[a.astype(np.float64) for a in f32], 0.6, 3600.0, 100.0, ps,
+                                         np.float64)
+    got32 = fw.kinematic_wave(pa(fw, ldd, ps), *[pa(fw, a, ps) for a in f32], 0.6, 3600.0, 100.0).numpy()
+    err = rel_err(got32, want32in)
+    assert err.max() <= 1e-5, err.max()

@@ -1,0 +1,191 @@
+"""Pin the oracle: LUE's own test vectors, independent brute-force models and closed forms (CPU only)."""
+import math
+
+import numpy as np
+import pytest
+
+import reference_models as rm
+from conftest import assert_close_boost, assert_equal_nan, golden_arg, load_golden, raster
+
+
+# ---------------------------------------------------------------------------------------------------
+# LUE's golden vectors (tests/golden/*.json, extracted from source/framework/algorithm/test/*.cpp)
+
+@pytest.mark.parametrize("case", load_golden("inflow_count"), ids=lambda c: c["name"])
+def test_inflow_count_golden(oracle_mod, case):
+    ldd = raster(case["ldd"], np.uint8)
+    got = oracle_mod.inflow_count(ldd, case["partition_shape"])
+    assert_equal_nan(got, raster(case["expected"]["count"]["raster"], np.uint8))
+
+
+def _run_golden(oracle_mod, case, dtype):
+    ldd = raster(case["ldd"], np.uint8)
+    args = {k: golden_arg(v, dtype) for k, v in case["args"].items()}
+    ps = case["partition_shape"]
+    kind = case["kind"]
+    if kind == "accu":
+        return {"outflow": oracle_mod.accu(ldd, args["inflow"], ps, dtype)}
+    if kind in ("accu_threshold", "accu_fraction"):
+        fn = getattr(oracle_mod, kind)
+        out, rem = fn(ldd, args["inflow"], args["threshold" if kind == "accu_threshold" else "fraction"], ps, dtype)
+        return {"outflow": out, "remainder": rem}
+    if kind == "kinematic_wave":
+        shape = ldd.shape
+        qc = args["current_outflow"]
+        ql = args["inflow"]
+        qc = qc if isinstance(qc, np.ndarray) else np.full(shape, qc, dtype)
+        ql = ql if isinstance(ql, np.ndarray) else np.full(shape, ql, dtype)
+        return {"outflow": oracle_mod.kinematic_wave(ldd, qc, ql, args["alpha"], args["beta"],
+                                                     args["time_step_duration"], args["channel_length"], ps,
+                                                     dtype)}
+    raise AssertionError(kind)
+
+
+ROUTING_CASES = [c for op in ("accu", "accu_threshold", "accu_fraction", "kinematic_wave")
+                 for c in load_golden(op) if c["kind"] != "iterate_to_new_discharge"]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("case", ROUTING_CASES, ids=lambda c: f"{c['kind']}-{c['name']}")
+def test_routing_golden(oracle_mod, case, dtype):
+    got = _run_golden(oracle_mod, case, dtype)
+    if case.get("property") == "all_positive":
+        assert (got["outflow"] > 0).all()
+    for key, exp in case["expected"].items():
+        want = raster(exp["raster"], dtype)
+        if exp["compare"] == "equal":
+            assert_equal_nan(got[key], want)
+        else:
+            assert_close_boost(got[key], want, 1e-6)
+
+
+NEWTON_CASES = [c for c in load_golden("kinematic_wave") if c["kind"] == "iterate_to_new_discharge"]
+
+
+@pytest.mark.parametrize("case", NEWTON_CASES, ids=lambda c: c["name"])
+def test_iterate_to_new_discharge_golden(oracle_mod, case):
+    dtype = np.float64 if case["element"] == "double" else np.float32
+    for call in case["calls"]:
+        got = oracle_mod.iterate_to_new_discharge(*call["args"], dtype=dtype)
+        assert_close_boost(np.array([got]), np.array([call["want"]]), 1e-6)
+
+
+# ---------------------------------------------------------------------------------------------------
+# Independent models on random networks (brute force; different algorithms than LUE's router)
+
+SHAPES = [((7, 9), (7, 9)), ((12, 17), (5, 4)), ((16, 16), (4, 8)), ((13, 11), (3, 3))]
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("shape,pshape", SHAPES)
+def test_inflow_count_brute_force(oracle_mod, seed, shape, pshape):
+    rng = np.random.default_rng(seed)
+    ldd = rm.random_ldd(rng, *shape, nodata_fraction=0.1, pit_fraction=0.05, flat_fraction=0.3)
+    assert_equal_nan(oracle_mod.inflow_count(ldd, pshape), rm.inflow_count(ldd))
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("shape,pshape", SHAPES)
+def test_accu_brute_force(oracle_mod, seed, shape, pshape):
+    rng = np.random.default_rng(100 + seed)
+    ldd = rm.random_ldd(rng, *shape, nodata_fraction=0.05, pit_fraction=0.05, flat_fraction=0.3)
+    # small integers: every order of summation is exact, so the comparison is exact
+    material = rng.integers(0, 5, size=shape).astype(np.float64)
+    material[rng.random(shape) < 0.03] = np.nan
+    material[rng.random(shape) < 0.03] = -1.0  # outside the domain -> no-data downstream
+    want = rm.accu_by_walking(ldd, material)
+    for dtype in (np.float32, np.float64):
+        assert_equal_nan(oracle_mod.accu(ldd, material.astype(dtype), pshape, dtype), want.astype(dtype))
+    # accu(ldd, 1) is the upstream cell count, including the cell itself
+    ones = rm.accu_by_walking(ldd, 1.0)
+    assert_equal_nan(oracle_mod.accu(ldd, 1.0, pshape, np.float64), ones)
+
+
+@pytest.mark.parametrize("kind", ["threshold", "fraction"])
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("shape,pshape", SHAPES)
+def test_split_brute_force(oracle_mod, kind, seed, shape, pshape):
+    rng = np.random.default_rng(200 + seed)
+    ldd = rm.random_ldd(rng, *shape, nodata_fraction=0.05, flat_fraction=0.3)
+    material = rng.random(shape)
+    param = rng.random(shape) * (0.5 if kind == "threshold" else 1.0)
+    param[rng.random(shape) < 0.03] = np.nan
+    want_out, want_rem = rm.route_split(ldd, material, param, kind)
+    fn = oracle_mod.accu_threshold if kind == "threshold" else oracle_mod.accu_fraction
+    out, rem = fn(ldd, material, param, pshape, np.float64)
+    np.testing.assert_allclose(out, want_out, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(rem, want_rem, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("shape,pshape", SHAPES)
+def test_kinematic_wave_linear_closed_form(oracle_mod, seed, shape, pshape):
+    """beta = 1 makes the kinematic-wave equation linear; the Newton solve must hit the closed form."""
+    rng = np.random.default_rng(300 + seed)
+    ldd = rm.random_ldd(rng, *shape, flat_fraction=0.3)
+    q_cur = rng.random(shape) * 5
+    q_cur[rng.random(shape) < 0.1] = 0.0
+    q_lat = rng.random(shape) * 1e-3
+    alpha = 1.5 + rng.random(shape)
+    want = rm.kinematic_wave_linear(ldd, q_cur, q_lat, alpha, 3600.0, 100.0)
+    got = oracle_mod.kinematic_wave(ldd, q_cur, q_lat, alpha, 1.0, 3600.0, 100.0, pshape, np.float64)
+    np.testing.assert_allclose(got, want, rtol=1e-12)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-9), (np.float32, 2e-4)])
+def test_newton_residual(oracle_mod, dtype, tol):
+    """Converged roots satisfy f(Q) = 0 (kinematic_wave.hpp:101-116) to the Newton stop criterion."""
+    rng = np.random.default_rng(7)
+    for _ in range(300):
+        q_up, q_cur = rng.random(2) * 10
+        q_lat = rng.random() * 1e-3
+        alpha = 0.5 + 5.5 * rng.random()
+        beta = 0.5 + 1.5 * rng.random()
+        dt, dx = 1 + 99 * rng.random(), 1 + 99 * rng.random()
+        q = oracle_mod.iterate_to_new_discharge(q_up, q_cur, q_lat, alpha, beta, dt, dx, dtype=dtype)
+        res = rm.kinematic_residual(q, q_up, q_cur, q_lat, alpha, beta, dt, dx)
+        assert abs(res) < tol, (q_up, q_cur, q_lat, alpha, beta, dt, dx, q, res)
+
+
+def test_newton_random_input_nonnegative(oracle_mod):
+    """ALGT/kinematic_wave_test.cpp:316-373: random inputs, extraction included, never negative."""
+    rng = np.random.default_rng(11)
+    for dtype in (np.float32, np.float64):
+        for _ in range(2000):
+            q_up, q_cur = rng.random(2) * 1000
+            alpha, beta = 0.5 + 5.5 * rng.random(), 0.5 + 1.5 * rng.random()
+            dt, dx = 1 + 99 * rng.random(), 1 + 99 * rng.random()
+            q_lat = (rng.random() * 2000 - 1000) / dx
+            q = oracle_mod.iterate_to_new_discharge(q_up, q_cur, q_lat, alpha, beta, dt, dx, dtype=dtype)
+            assert q >= 0 and math.isfinite(q)
+
+
+def test_partitioning_does_not_change_integer_results(oracle_mod):
+    rng = np.random.default_rng(5)
+    ldd = rm.random_ldd(rng, 40, 37, nodata_fraction=0.02, flat_fraction=0.4)
+    ref = oracle_mod.accu(ldd, 1.0, (40, 37), np.float64)
+    for ps in [(1, 1), (3, 5), (8, 8), (40, 1), (13, 37)]:
+        assert_equal_nan(oracle_mod.accu(ldd, 1.0, ps, np.float64), ref)
+
+
+def test_cycle_is_reported(oracle_mod):
+    ldd = np.array([[6, 2], [8, 4]], dtype=np.uint8)  # 2x2 loop
+    with pytest.raises(oracle_mod.OracleError):
+        oracle_mod.accu(ldd, 1.0, (2, 2), np.float64)
+
+
+def test_planted_mistake_is_caught(oracle_mod):
+    """A wrong neighbour offset in a model of the router shows up against the walking model."""
+    rng = np.random.default_rng(9)
+    ldd = rm.random_ldd(rng, 10, 10, flat_fraction=0.3)
+    want = rm.accu_by_walking(ldd, 1.0)
+    saved = dict(rm.OFFSETS)
+    try:
+        rm.OFFSETS[3] = (1, 0)  # SE decoded as S
+        wrong = rm.accu_by_walking(ldd, 1.0)
+    finally:
+        rm.OFFSETS.clear()
+        rm.OFFSETS.update(saved)
+    if (ldd == 3).any():
+        assert not np.array_equal(np.nan_to_num(wrong), np.nan_to_num(want))
+    assert_equal_nan(oracle_mod.accu(ldd, 1.0, (4, 4), np.float64), want)
