@@ -203,7 +203,7 @@ def test_shape_mismatch_raises(fw):
 # ---------------------------------------------------------------------------------------------------
 # Workload-shaped inputs (synthetic DEM) and the device generators
 
-def test_device_generators_match_host():
+def test_device_generators_match_host(fw):
     from lue_amd import synthetic
 
     ldd = synthetic.ldd_device(20261203, 300, 700).cpu().numpy()
