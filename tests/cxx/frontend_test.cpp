@@ -1,0 +1,181 @@
+// Golden cases of LUE's routing tests, run through the C++ front end (lue::value_policies::* on
+// lue::PartitionedArray) and the C ABI. The cases (golden_cases.inc) are generated by tests/test_cxx_frontend.py
+// from tests/golden/*.json. Prints one line per case and exits non-zero if any case fails.
+#include "lue/framework/routing.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <limits>
+#include <string>
+#include <vector>
+
+namespace {
+
+    using Shape = std::array<lue::Count, 2>;
+
+    struct Arg
+    {
+        bool raster;
+        double scalar;
+        std::vector<double> values;
+    };
+
+    struct Case
+    {
+        char const* name;
+        char const* kind;
+        Shape shape;
+        Shape partition_shape;
+        std::vector<double> ldd;
+        std::vector<Arg> args;
+        std::vector<std::vector<double>> expected;  // outflow [, remainder] | count
+        bool close;                                  // Boost tolerance 1e-6 instead of exact
+        bool all_positive;
+    };
+
+    double const nan = std::numeric_limits<double>::quiet_NaN();
+
+#include "golden_cases.inc"
+
+    template<typename E>
+    auto make_raster(Case const& c, std::vector<double> const& values) -> lue::PartitionedArray<E, 2>
+    {
+        lue::PartitionedArray<E, 2> a{c.shape, c.partition_shape};
+        std::vector<E> host(values.begin(), values.end());
+        a.from_host(host);
+        return a;
+    }
+
+    bool equal(double got, double want, bool close)
+    {
+        if (std::isnan(got) || std::isnan(want))
+            return std::isnan(got) && std::isnan(want);
+        if (!close)
+            return got == want;
+        if (got == 0 || want == 0)
+            return std::fabs(got) < 1e-6 && std::fabs(want) < 1e-6;
+        double const d = std::fabs(got - want);
+        return d / std::fabs(got) <= 1e-6 && d / std::fabs(want) <= 1e-6;
+    }
+
+    template<typename E>
+    bool compare(Case const& c, lue::PartitionedArray<E, 2> const& got, std::vector<double> const& want)
+    {
+        auto const host = got.to_host();
+        for (std::size_t i = 0; i < host.size(); ++i)
+        {
+            if (!equal(static_cast<double>(host[i]), want[i], c.close))
+            {
+                std::printf("  %s: cell %zu got %.9g want %.9g\n", c.name, i, static_cast<double>(host[i]), want[i]);
+                return false;
+            }
+        }
+        return true;
+    }
+
+    template<typename E>
+    bool run(Case const& c)
+    {
+        using namespace lue;
+        auto const ldd = make_raster<std::uint8_t>(c, c.ldd);
+        std::string const kind = c.kind;
+        if (kind == "inflow_count")
+        {
+            auto const count = inflow_count<std::uint8_t>(ldd);
+            return compare(c, count, c.expected[0]);
+        }
+        auto raster = [&](std::size_t k) { return make_raster<E>(c, c.args[k].values); };
+        auto scalar = [&](std::size_t k) { return Scalar<E>{static_cast<E>(c.args[k].scalar)}; };
+        if (kind == "accu")
+        {
+            auto out = c.args[0].raster ? value_policies::accu(ldd, raster(0)) : value_policies::accu(ldd, scalar(0));
+            return compare(c, out, c.expected[0]);
+        }
+        if (kind == "accu_threshold" || kind == "accu_fraction")
+        {
+            bool const thr = kind == "accu_threshold";
+            auto call = [&](auto const& a, auto const& b)
+            { return thr ? value_policies::accu_threshold(ldd, a, b) : value_policies::accu_fraction(ldd, a, b); };
+            std::tuple<PartitionedArray<E, 2>, PartitionedArray<E, 2>> result =
+                c.args[0].raster ? (c.args[1].raster ? call(raster(0), raster(1)) : call(raster(0), scalar(1)))
+                                 : (c.args[1].raster ? call(scalar(0), raster(1)) : call(scalar(0), scalar(1)));
+            bool ok = true;
+            for (std::size_t k = 0; k < c.expected.size(); ++k)
+                ok = compare(c, k == 0 ? std::get<0>(result) : std::get<1>(result), c.expected[k]) && ok;
+            return ok;
+        }
+        if (kind == "kinematic_wave")
+        {
+            auto full = [&](std::size_t k)
+            {
+                return c.args[k].raster ? raster(k)
+                                        : create_partitioned_array<E>(c.shape, c.partition_shape,
+                                                                      static_cast<E>(c.args[k].scalar));
+            };
+            auto const current = full(0);
+            auto const inflow = full(1);
+            bool const rasters = c.args[2].raster || c.args[3].raster || c.args[5].raster;
+            auto out = rasters ? value_policies::kinematic_wave(ldd, current, inflow, full(2), full(3),
+                                                                Scalar<E>{static_cast<E>(c.args[4].scalar)}, full(5))
+                               : value_policies::kinematic_wave(ldd, current, inflow, scalar(2), scalar(3),
+                                                                scalar(4), scalar(5));
+            if (c.all_positive)
+            {
+                for (E v : out.to_host())
+                    if (!(v > 0))
+                        return false;
+            }
+            return c.expected.empty() || compare(c, out, c.expected[0]);
+        }
+        std::printf("  unknown kind %s\n", c.kind);
+        return false;
+    }
+
+    bool shape_mismatch_throws()
+    {
+        auto const ldd = lue::create_partitioned_array<std::uint8_t>({8, 8}, {4, 4}, 2);
+        auto const inflow = lue::create_partitioned_array<float>({8, 9}, {4, 4}, 1.0f);
+        try
+        {
+            auto out = lue::value_policies::accu(ldd, inflow);
+        }
+        catch (std::runtime_error const& e)
+        {
+            return std::string(e.what()).rfind("Shapes differ", 0) == 0;
+        }
+        return false;
+    }
+
+}  // namespace
+
+int main()
+{
+    int failed = 0;
+    int passed = 0;
+    for (Case const& c : cases())
+    {
+        for (int precision = 0; precision < 2; ++precision)
+        {
+            if (std::string(c.kind) == "inflow_count" && precision == 1)
+                continue;
+            bool ok;
+            try
+            {
+                ok = precision == 0 ? run<float>(c) : run<double>(c);
+            }
+            catch (std::exception const& e)
+            {
+                std::printf("  %s: exception %s\n", c.name, e.what());
+                ok = false;
+            }
+            std::printf("%s %s %s %s\n", ok ? "PASS" : "FAIL", c.kind, c.name, precision == 0 ? "float" : "double");
+            (ok ? passed : failed) += 1;
+        }
+    }
+    bool const ok = shape_mismatch_throws();
+    std::printf("%s shape_mismatch_throws\n", ok ? "PASS" : "FAIL");
+    failed += ok ? 0 : 1;
+    std::printf("%d passed, %d failed\n", passed + (ok ? 1 : 0), failed);
+    return failed == 0 ? 0 : 1;
+}
