@@ -48,9 +48,10 @@ namespace lue_amd {
         return powf(x, y);
     }
 
+    // x > 0 here: exp(y log x) avoids pow's special-case handling
     __device__ inline double dpow(double x, double y)
     {
-        return pow(x, y);
+        return exp(y * log(x));
     }
 
     // Coefficients of one cell's equation.
@@ -63,8 +64,8 @@ namespace lue_amd {
         T alpha;
         T beta;
         T dt;
-        T dx;
-        T alpha_pow;  // alpha * q_cur^beta, evaluated once per cell outside the dependency chain
+        T c1;         // dt / dx
+        T b;          // alpha q_cur^beta + dt q_lat, evaluated once per cell outside the dependency chain
     };
 
     // Solve f(Q) = 0 from the linearised guess. Returns the root; `converged` tells whether the
@@ -72,8 +73,8 @@ namespace lue_amd {
     template<typename T>
     __device__ inline T solve_bracketed(KinematicWaveCell<T> const& cell, bool& converged)
     {
-        T const c1 = cell.dt / cell.dx;
-        T const known = (c1 * cell.q_up) + cell.alpha_pow + (cell.dt * cell.q_lat);
+        T const c1 = cell.c1;
+        T const known = (c1 * cell.q_up) + cell.b;
         T const alpha_beta = cell.alpha * cell.beta;
 
         // guess (kinematic_wave.hpp:61-91)
@@ -101,7 +102,7 @@ namespace lue_amd {
             step_prev = step;
             T const xb = dpow(x, cell.beta);
             T const f0 = (c1 * x) + (cell.alpha * xb) - known;
-            T const f1 = c1 + (alpha_beta * dpow(x, cell.beta - T{1}));
+            T const f1 = c1 + (alpha_beta * xb / x);
             --left;
             if (f0 == T{0})
             {
@@ -175,32 +176,90 @@ namespace lue_amd {
         return 0.0;
     }
 
-    // FP32 path: FP32 iteration, then one FP64 Newton correction of the FP32 root. `q_up` is the
-    // compensated (FP64) upstream sum, `alpha_pow` = alpha * q_cur^beta evaluated in FP64.
-    __device__ inline double new_discharge_f32_corrected(
-        double q_up, float q_cur, float q_lat, float alpha, float beta, float dt, float dx, double alpha_pow)
+    // Hardware approximations (about 1 ulp): v_log_f32, v_exp_f32, v_rcp_f32.
+    __device__ inline float fast_log2(float x)
+    {
+        return __builtin_amdgcn_logf(x);
+    }
+
+    __device__ inline float fast_exp2(float x)
+    {
+        return __builtin_amdgcn_exp2f(x);
+    }
+
+    __device__ inline float fast_rcp(float x)
+    {
+        return __builtin_amdgcn_rcpf(x);
+    }
+
+    // FP32 Newton from LUE's linearised guess, with the hardware transcendentals; stops on LUE's FP32
+    // criterion |step| <= |Q| 2^-13 (digits = floor(0.6 * 24) = 14). Needs q_up + q_cur > 0.
+    __device__ inline float solve_fast_f32(
+        float c1, float alpha, float beta, float dt, float q_up, float q_cur, float q_lat, float known, bool& converged)
+    {
+        float const ab = alpha * beta;
+        float const a_b_pq = ab * fast_exp2((beta - 1.0f) * fast_log2(0.5f * (q_cur + q_up)));
+        float x = (c1 * q_up + a_b_pq * q_cur + dt * q_lat) * fast_rcp(c1 + a_b_pq);
+        x = fmaxf(x, FLT_MIN);
+        converged = false;
+        for (int iteration = 0; iteration < 10; ++iteration)
+        {
+            float const xb = fast_exp2(beta * fast_log2(x));
+            float const f = fmaf(c1, x, fmaf(alpha, xb, -known));
+            float const fp = fmaf(ab * xb, fast_rcp(x), c1);
+            float const step = f * fast_rcp(fp);
+            float xn = x - step;
+            xn = xn > 0.0f ? xn : 0.5f * x;
+            bool const done = fabsf(step) <= fabsf(xn) * 0x1p-13f;
+            x = xn;
+            if (done)
+            {
+                converged = true;
+                break;
+            }
+        }
+        return x;
+    }
+
+    // FP32 path of one cell. q_up is the compensated upstream sum (FP64), b = alpha q_cur^beta + dt q_lat
+    // and c1 = dt / dx are evaluated in FP64 outside the dependency chain. The FP32 root gets one FP64
+    // Newton correction, the only FP64 arithmetic on the chain.
+    // Dry cells that receive water with nothing upstream (q_up + q_cur = 0, beta < 1) start from
+    // numeric_limits::min() in LUE, where the FP32 and FP64 iterations behave differently; those take the
+    // FP64 iteration so that results follow LUE's FP64 arithmetic.
+    __device__ inline double new_discharge_f32_path(
+        double q_up, float q_cur, float q_lat, float alpha, float beta, float dt, double b, double c1)
     {
         if (!(q_up + q_cur > 0 || q_lat > 0))
         {
             return 0.0;
         }
-        KinematicWaveCell<float> cell{static_cast<float>(q_up), q_cur, q_lat, alpha, beta, dt, dx,
-                                      static_cast<float>(alpha_pow)};
-        bool converged;
-        float const x32 = solve_bracketed<float>(cell, converged);
-        if (!converged || !(x32 > 0.0f))
+        if (q_up + q_cur == 0.0 && beta < 1.0f)
         {
-            // No root to refine: the FP32 iterate is the result (as in LUE's float instantiation).
-            return static_cast<double>(x32);
+            KinematicWaveCell<double> cell{q_up, q_cur, q_lat, alpha, beta, dt, c1, b};
+            bool converged;
+            return solve_bracketed<double>(cell, converged);
         }
-        double const c1 = static_cast<double>(dt) / static_cast<double>(dx);
-        double const known = (c1 * q_up) + alpha_pow + (static_cast<double>(dt) * static_cast<double>(q_lat));
+        double const known = c1 * q_up + b;
+        bool converged;
+        float const x32 = solve_fast_f32(static_cast<float>(c1), alpha, beta, dt, static_cast<float>(q_up), q_cur,
+                                         q_lat, static_cast<float>(known), converged);
         double const x = static_cast<double>(x32);
-        double const xb = pow(x, static_cast<double>(beta));
-        double const f0 = (c1 * x) + (static_cast<double>(alpha) * xb) - known;
-        double const f1 = c1 + static_cast<double>(beta) * (static_cast<double>(alpha) * xb) / x;
+        double const xb = dpow(x, static_cast<double>(beta));
+        double const ax = static_cast<double>(alpha) * xb;
+        double const f0 = (c1 * x) + ax - known;
+        double const f1 = c1 + static_cast<double>(beta) * ax / x;
         double const corrected = x - f0 / f1;
         return corrected > 0.0 ? corrected : x;
+    }
+
+    // Elementwise FP32 solve with the corrected root, used by iterate_to_new_discharge_f32.
+    __device__ inline double new_discharge_f32_corrected(
+        double q_up, float q_cur, float q_lat, float alpha, float beta, float dt, float dx, double alpha_pow)
+    {
+        double const c1 = static_cast<double>(dt) / static_cast<double>(dx);
+        double const b = alpha_pow + static_cast<double>(dt) * static_cast<double>(q_lat);
+        return new_discharge_f32_path(q_up, q_cur, q_lat, alpha, beta, dt, b, c1);
     }
 
 }  // namespace lue_amd

@@ -126,12 +126,12 @@ namespace lue_amd {
     template<typename T>
     struct CellInput<op_kinematic_wave, T>
     {
-        T q_cur;      // NaN when any argument is no-data or outside the domain
+        T q_cur;   // NaN when any argument is no-data or outside the domain
         T q_lat;
         T alpha;
         T beta;
-        T dx;
-        double alpha_pow;  // alpha * q_cur^beta in FP64
+        double b;   // alpha q_cur^beta + dt q_lat, FP64
+        double c1;  // dt / dx, FP64
     };
 
     template<int OP, typename T, int TW, int TH>
@@ -204,9 +204,11 @@ namespace lue_amd {
             c.q_lat = q_lat;
             c.alpha = alpha;
             c.beta = beta;
-            c.dx = dx;
-            c.alpha_pow = ok ? static_cast<double>(alpha) * pow(static_cast<double>(q_cur), static_cast<double>(beta))
-                             : 0.0;
+            double const dt = static_cast<double>(a.dt);
+            c.c1 = dt / static_cast<double>(dx);
+            c.b = ok ? static_cast<double>(alpha) * pow(static_cast<double>(q_cur), static_cast<double>(beta)) +
+                           dt * static_cast<double>(q_lat)
+                     : 0.0;
         }
     }
 
@@ -269,14 +271,12 @@ namespace lue_amd {
             }
             else if constexpr (std::is_same_v<T, double>)
             {
-                KinematicWaveCell<double> cell{upstream, in.q_cur, in.q_lat, in.alpha, in.beta, a.dt, in.dx,
-                                               in.alpha_pow};
+                KinematicWaveCell<double> cell{upstream, in.q_cur, in.q_lat, in.alpha, in.beta, a.dt, in.c1, in.b};
                 out = new_discharge_f64(cell);
             }
             else
             {
-                out = new_discharge_f32_corrected(
-                    upstream, in.q_cur, in.q_lat, in.alpha, in.beta, a.dt, in.dx, in.alpha_pow);
+                out = new_discharge_f32_path(upstream, in.q_cur, in.q_lat, in.alpha, in.beta, a.dt, in.b, in.c1);
             }
         }
     }
