@@ -55,6 +55,14 @@ int lue_amd_profile_enable(int enable);
 int lue_amd_profile_read(double* first_round_ms, double* next_rounds_ms, long long* calls, long long* rounds,
                          long long* cells);
 
+/*
+ * Tuning of the sweep (no reference counterpart): after at least min_rounds tile rounds, once at most
+ * max(tiles * tile_fraction, 4) tiles are queued (or after max_rounds rounds), the remaining cells are
+ * solved by chain streaming. min_rounds < 0: tile rounds only. Defaults: 4, 24, 1/64. Results do not
+ * depend on the policy beyond floating point summation order.
+ */
+int lue_amd_set_tail_policy(int min_rounds, int max_rounds, double tile_fraction);
+
 /* Device memory used by the C++ front end's device-resident partitions. */
 int lue_amd_device_malloc(void** ptr, size_t bytes);
 int lue_amd_device_free(void* ptr);

@@ -26,6 +26,7 @@ SIGNATURES = {
     "lue_amd_status_string": [c_int],
     "lue_amd_last_error": [],
     "lue_amd_version": [],
+    "lue_amd_set_tail_policy": [c_int, c_int, c_d],
     "lue_amd_profile_enable": [c_int],
     "lue_amd_profile_read": [ctypes.POINTER(c_d), ctypes.POINTER(c_d)] + [ctypes.POINTER(ctypes.c_longlong)] * 3,
     "lue_amd_device_malloc": [ctypes.POINTER(c_p), c_size],
@@ -84,6 +85,11 @@ def load():
         fn.restype = _RESTYPES.get(name, c_int)
     _lib = lib
     return lib
+
+
+def set_tail_policy(min_rounds=4, max_rounds=24, tile_fraction=1.0 / 64.0):
+    """min_rounds < 0: tile rounds only (no chain-streaming tail)."""
+    check(load().lue_amd_set_tail_policy(min_rounds, max_rounds, tile_fraction))
 
 
 def profile_enable(enable=True):

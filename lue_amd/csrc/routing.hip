@@ -7,8 +7,10 @@
 #include "routing_common.hpp"
 #include "sweep.hpp"
 #include "synthetic.hpp"
+#include "tail.hpp"
 
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -33,6 +35,17 @@ namespace lue_amd {
             long long cells{0};
         };
         Profile profile;
+
+        // When the round loop hands over to the chain-streaming tail: after at least `min_rounds`
+        // rounds, once at most max(tiles * tile_fraction, 4) tiles are queued, and always after
+        // `max_rounds` rounds. min_rounds < 0 disables the tail (rounds only).
+        struct TailPolicy
+        {
+            int min_rounds{4};
+            int max_rounds{24};
+            double tile_fraction{1.0 / 64.0};
+        };
+        TailPolicy tail_policy;
 
         int fail(int status, std::string message)
         {
@@ -99,6 +112,162 @@ namespace lue_amd {
             return e;
         }
 
+
+        template<typename P>
+        struct DeviceArray
+        {
+            P* ptr{};
+            hipStream_t stream{};
+            ~DeviceArray()
+            {
+                if (ptr)
+                    (void)hipFreeAsync(ptr, stream);
+            }
+        };
+
+        template<typename P>
+        hipError_t alloc_array(DeviceArray<P>& a, std::size_t count, hipStream_t stream)
+        {
+            a.stream = stream;
+            return alloc(a.ptr, count, stream);
+        }
+
+        template<typename In, typename Out>
+        hipError_t exclusive_scan(In const* in, Out* out, std::size_t n, hipStream_t stream)
+        {
+            std::size_t bytes = 0;
+            hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, stream);
+            if (e != hipSuccess)
+                return e;
+            void* temp = nullptr;
+            e = hipMallocAsync(&temp, std::max<std::size_t>(bytes, 1), stream);
+            if (e != hipSuccess)
+                return e;
+            e = hipcub::DeviceScan::ExclusiveSum(temp, bytes, in, out, n, stream);
+            hipError_t const f = hipFreeAsync(temp, stream);
+            return e != hipSuccess ? e : f;
+        }
+
+        inline unsigned blocks_for(std::uint64_t n, unsigned threads = 256)
+        {
+            return static_cast<unsigned>(std::max<std::uint64_t>((n + threads - 1) / threads, 1));
+        }
+
+        // Solve all pending cells by chain streaming (tail.hpp). Adds the number of solved cells to *solved.
+        template<int OP, typename T, int TW, int TH>
+        int run_tail(SweepArgs<T> const& a, hipStream_t stream, unsigned long long* solved_total)
+        {
+            std::int64_t const ntiles = static_cast<std::int64_t>(a.tiles_x) * a.tiles_y;
+            std::uint64_t const ncells = static_cast<std::uint64_t>(a.rows * a.cols);
+            if (ncells >= (1ull << 32))
+                return fail(status_invalid_argument, "chain tail needs fewer than 2^32 cells");
+
+            DeviceArray<std::uint32_t> counts, offsets;
+            LUE_AMD_HIP(alloc_array(counts, ntiles + 1, stream));
+            LUE_AMD_HIP(alloc_array(offsets, ntiles + 1, stream));
+            LUE_AMD_HIP(hipMemsetAsync(counts.ptr + ntiles, 0, sizeof(std::uint32_t), stream));
+            hipLaunchKernelGGL((tail_count_kernel<TW, TH>), dim3(blocks_for(ntiles)), dim3(256), 0, stream,
+                               a.pending_bits, ntiles, counts.ptr);
+            LUE_AMD_HIP(hipGetLastError());
+            LUE_AMD_HIP(exclusive_scan(counts.ptr, offsets.ptr, static_cast<std::size_t>(ntiles + 1), stream));
+            std::uint32_t n = 0;
+            LUE_AMD_HIP(hipMemcpyAsync(&n, offsets.ptr + ntiles, sizeof(n), hipMemcpyDeviceToHost, stream));
+            LUE_AMD_HIP(hipStreamSynchronize(stream));
+            if (n == 0)
+                return status_success;
+
+            DeviceArray<std::uint32_t> cells, index_of, is_last, chain_of_last;
+            DeviceArray<std::int32_t> next, upstream_count, succ[2], rank[2], last[2];
+            LUE_AMD_HIP(alloc_array(cells, n, stream));
+            LUE_AMD_HIP(alloc_array(index_of, ncells, stream));
+            LUE_AMD_HIP(alloc_array(next, n, stream));
+            LUE_AMD_HIP(alloc_array(upstream_count, n, stream));
+            for (int b = 0; b < 2; ++b)
+            {
+                LUE_AMD_HIP(alloc_array(succ[b], n, stream));
+                LUE_AMD_HIP(alloc_array(rank[b], n, stream));
+                LUE_AMD_HIP(alloc_array(last[b], n, stream));
+            }
+            LUE_AMD_HIP(alloc_array(is_last, n + 1, stream));
+            LUE_AMD_HIP(alloc_array(chain_of_last, n + 1, stream));
+            LUE_AMD_HIP(hipMemsetAsync(upstream_count.ptr, 0, n * sizeof(std::int32_t), stream));
+            LUE_AMD_HIP(hipMemsetAsync(is_last.ptr + n, 0, sizeof(std::uint32_t), stream));
+
+            hipLaunchKernelGGL((tail_compact_kernel<TW, TH>), dim3(static_cast<unsigned>(ntiles)), dim3(64), 0, stream,
+                               a.pending_bits, a.tiles_x, a.cols, offsets.ptr, cells.ptr, index_of.ptr);
+            hipLaunchKernelGGL((tail_links_kernel<TW, TH>), dim3(blocks_for(n)), dim3(256), 0, stream, a.ldd, a.rows,
+                               a.cols, a.tiles_x, a.pending_bits, cells.ptr, index_of.ptr, n, next.ptr,
+                               upstream_count.ptr);
+            hipLaunchKernelGGL(tail_rank_init_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, next.ptr,
+                               upstream_count.ptr, n, succ[0].ptr, rank[0].ptr, last[0].ptr);
+            int cur = 0;
+            for (std::uint32_t reach = 1; reach < n; reach <<= 1)
+            {
+                hipLaunchKernelGGL(tail_rank_step_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, succ[cur].ptr,
+                                   rank[cur].ptr, last[cur].ptr, n, succ[1 - cur].ptr, rank[1 - cur].ptr,
+                                   last[1 - cur].ptr);
+                cur = 1 - cur;
+            }
+            hipLaunchKernelGGL(tail_flag_last_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, next.ptr,
+                               upstream_count.ptr, n, is_last.ptr);
+            LUE_AMD_HIP(hipGetLastError());
+            LUE_AMD_HIP(exclusive_scan(is_last.ptr, chain_of_last.ptr, static_cast<std::size_t>(n + 1), stream));
+            std::uint32_t nchains = 0;
+            LUE_AMD_HIP(hipMemcpyAsync(&nchains, chain_of_last.ptr + n, sizeof(nchains), hipMemcpyDeviceToHost, stream));
+            LUE_AMD_HIP(hipStreamSynchronize(stream));
+            if (nchains == 0)
+                return fail(status_invalid_network, "invalid flow direction network: pending cells form cycles");
+
+            DeviceArray<std::uint32_t> length, base, cursor_and_count, chain_cells;
+            DeviceArray<std::int32_t> first, arrivals, downstream, ready;
+            DeviceArray<unsigned long long> solved;
+            LUE_AMD_HIP(alloc_array(length, nchains + 1, stream));
+            LUE_AMD_HIP(alloc_array(base, nchains + 1, stream));
+            LUE_AMD_HIP(alloc_array(first, nchains, stream));
+            LUE_AMD_HIP(alloc_array(arrivals, nchains, stream));
+            LUE_AMD_HIP(alloc_array(downstream, nchains, stream));
+            LUE_AMD_HIP(alloc_array(ready, nchains, stream));
+            LUE_AMD_HIP(alloc_array(cursor_and_count, 2, stream));
+            LUE_AMD_HIP(alloc_array(chain_cells, n, stream));
+            LUE_AMD_HIP(alloc_array(solved, 1, stream));
+            // a chain whose first cell never appears (cells on a cycle) keeps length 0 and is never ready
+            LUE_AMD_HIP(hipMemsetAsync(length.ptr, 0, (nchains + 1) * sizeof(std::uint32_t), stream));
+            LUE_AMD_HIP(hipMemsetAsync(arrivals.ptr, 0xFF, nchains * sizeof(std::int32_t), stream));
+            LUE_AMD_HIP(hipMemsetAsync(cursor_and_count.ptr, 0, 2 * sizeof(std::uint32_t), stream));
+            LUE_AMD_HIP(hipMemsetAsync(solved.ptr, 0, sizeof(unsigned long long), stream));
+
+            hipLaunchKernelGGL(tail_chain_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, next.ptr,
+                               upstream_count.ptr, rank[cur].ptr, last[cur].ptr, chain_of_last.ptr, n, length.ptr,
+                               first.ptr, arrivals.ptr);
+            LUE_AMD_HIP(exclusive_scan(length.ptr, base.ptr, static_cast<std::size_t>(nchains + 1), stream));
+            hipLaunchKernelGGL(tail_downstream_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, next.ptr,
+                               last[cur].ptr, chain_of_last.ptr, is_last.ptr, n, downstream.ptr);
+            hipLaunchKernelGGL(tail_layout_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, cells.ptr, rank[cur].ptr,
+                               last[cur].ptr, is_last.ptr, chain_of_last.ptr, base.ptr, length.ptr, n,
+                               chain_cells.ptr);
+            hipLaunchKernelGGL(tail_ready_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, arrivals.ptr,
+                               nchains, ready.ptr, cursor_and_count.ptr + 1);
+            LUE_AMD_HIP(hipGetLastError());
+            std::uint32_t nready = 0;
+            LUE_AMD_HIP(hipMemcpyAsync(&nready, cursor_and_count.ptr + 1, sizeof(nready), hipMemcpyDeviceToHost, stream));
+            LUE_AMD_HIP(hipStreamSynchronize(stream));
+
+            TailChains t{chain_cells.ptr, base.ptr,  length.ptr,           downstream.ptr, arrivals.ptr,
+                         ready.ptr,       nready,    cursor_and_count.ptr, solved.ptr};
+            // one wavefront per chain; 4 wavefronts per workgroup, enough workgroups to fill the device
+            unsigned const grid = static_cast<unsigned>(std::min<std::uint64_t>((nready + 3) / 4, 2048));
+            if (nready > 0)
+            {
+                hipLaunchKernelGGL((tail_execute_kernel<OP, T>), dim3(std::max(grid, 1u)), dim3(256), 0, stream, a, t);
+                LUE_AMD_HIP(hipGetLastError());
+            }
+            unsigned long long tail_solved = 0;
+            LUE_AMD_HIP(hipMemcpyAsync(&tail_solved, solved.ptr, sizeof(tail_solved), hipMemcpyDeviceToHost, stream));
+            LUE_AMD_HIP(hipStreamSynchronize(stream));
+            *solved_total += tail_solved;
+            return status_success;
+        }
+
         // Run the tiled sweep to completion: round 0 over all tiles, then queued tiles until none is left.
         template<int OP, typename T, int TW, int TH>
         int run_sweep(SweepArgs<T> a, hipStream_t stream)
@@ -149,10 +318,14 @@ namespace lue_amd {
 
             unsigned const grid = static_cast<unsigned>(std::min<std::int64_t>(ntiles, 2048));
             std::uint32_t round = 1;
-            std::uint32_t batch = 2;
             std::uint32_t pending_tiles = 0;
+            bool use_tail = false;
+            TailPolicy const policy = tail_policy;
+            std::int64_t const tail_tiles =
+                std::max<std::int64_t>(static_cast<std::int64_t>(ntiles * policy.tile_fraction), 4);
             while (true)
             {
+                std::uint32_t const batch = policy.min_rounds >= 0 ? 4u : std::min<std::uint32_t>(round, 64u);
                 for (std::uint32_t k = 0; k < batch; ++k, ++round)
                 {
                     hipLaunchKernelGGL((sweep_next_round<OP, T, TW, TH>), dim3(grid), dim3(256), 0, stream, a, round);
@@ -163,9 +336,20 @@ namespace lue_amd {
                 LUE_AMD_HIP(hipStreamSynchronize(stream));
                 if (pending_tiles == 0)
                     break;
+                if (policy.min_rounds >= 0 && static_cast<int>(round) > policy.min_rounds &&
+                    (pending_tiles <= tail_tiles || static_cast<int>(round) > policy.max_rounds))
+                {
+                    use_tail = true;
+                    break;
+                }
                 if (round > 0xFFFF0000u)
                     return fail(status_invalid_network, "routing did not converge");
-                batch = std::min<std::uint32_t>(batch * 2, 64);
+            }
+            unsigned long long tail_solved = 0;
+            if (use_tail)
+            {
+                if (int const st = run_tail<OP, T, TW, TH>(a, stream, &tail_solved))
+                    return st;
             }
             if (profile.enabled)
                 LUE_AMD_HIP(hipEventRecord(ev[2], stream));
@@ -185,11 +369,11 @@ namespace lue_amd {
                 for (auto& e : ev)
                     (void)hipEventDestroy(e);
             }
-            if (counters[0] + counters[1] != static_cast<unsigned long long>(a.rows * a.cols))
+            if (counters[0] + counters[1] + tail_solved != static_cast<unsigned long long>(a.rows * a.cols))
             {
                 return fail(status_invalid_network,
                             "invalid flow direction network: " +
-                                std::to_string(a.rows * a.cols - counters[0] - counters[1]) +
+                                std::to_string(a.rows * a.cols - counters[0] - counters[1] - tail_solved) +
                                 " cells lie on or downstream of a cycle");
             }
             return status_success;
@@ -366,6 +550,14 @@ const char* lue_amd_last_error(void)
 const char* lue_amd_version(void)
 {
     return "0.1.0";
+}
+
+int lue_amd_set_tail_policy(int min_rounds, int max_rounds, double tile_fraction)
+{
+    if (max_rounds < 0 || !(tile_fraction >= 0.0) || tile_fraction > 1.0)
+        return fail(status_invalid_argument, "bad tail policy");
+    tail_policy = TailPolicy{min_rounds, max_rounds, tile_fraction};
+    return status_success;
 }
 
 int lue_amd_profile_enable(int enable)

@@ -1,0 +1,490 @@
+// Chain-streaming tail of the routing sweep.
+//
+// After the first rounds of the tiled sweep (sweep.hpp) almost every cell is solved; what is left are
+// the long flow paths that cross many tiles (main streams). Solving them one tile per round would cost
+// one launch per tile crossing. Instead the pending cells are cut into chains and streamed:
+//   1. compact: list the pending cells (pending bitmaps of the tiles), map cell -> list index;
+//   2. links: per pending cell, its downstream pending cell and the number of pending upstream cells;
+//      a chain continues from a cell into its downstream cell when that cell has exactly one pending
+//      upstream cell, and starts at every other pending cell (no pending upstream: ready; two or more:
+//      a confluence of chains);
+//   3. list ranking by pointer jumping gives every cell its distance to the end of its chain and the
+//      chain's last cell, from which chains are laid out contiguously in flow order;
+//   4. execute: one wavefront per chain. The wavefront loads 64 consecutive chain cells at a time
+//      (cell index, the outflow of the upstream neighbours that are not on the chain, the inputs),
+//      then runs the cell function serially along the block (values broadcast with readlane).
+//      When a chain ends at a confluence, the wavefront decrements the confluence's arrival counter
+//      (agent-scope acquire/release, once per chain); the last arriving wavefront continues with
+//      the chain that starts there. No wavefront ever waits, so the order of work cannot deadlock.
+// Kinematic wave, FP32 path: the serial part is the FP32 Newton solve only. After each block the 64
+// lanes compute, in parallel, the FP64 residual correction of their cell and its sensitivity to the
+// upstream discharge; the corrections are propagated along the block (affine recurrence, first order
+// in the FP32 error) and the corrected discharge becomes the start of the next block.
+#pragma once
+
+#include "kinematic_wave.hpp"
+#include "routing_common.hpp"
+#include "sweep.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lue_amd {
+
+    struct TailChains
+    {
+        std::uint32_t const* cells;     // chain cells in flow order, chains back to back
+        std::uint32_t const* base;      // per chain: offset into cells
+        std::uint32_t const* length;    // per chain
+        std::int32_t const* downstream; // per chain: chain starting at the confluence below the last cell, or -1
+        std::int32_t* arrivals;         // per chain: chains still to arrive at its first cell
+        std::int32_t const* ready;      // chains without pending upstream
+        std::uint32_t nready;
+        std::uint32_t* cursor;          // next index into `ready`
+        unsigned long long* solved;     // cells solved by the tail
+    };
+
+    template<int TW, int TH>
+    __device__ inline bool cell_pending(std::uint32_t const* pending_bits, int tiles_x, std::int64_t cols, std::int64_t r,
+                                        std::int64_t c)
+    {
+        using G = Tile<TW, TH>;
+        std::int64_t const tile = (r / TH) * tiles_x + (c / TW);
+        int const i = static_cast<int>((r % TH) * TW + (c % TW));
+        return (pending_bits[tile * G::words + (i >> 5)] >> (i & 31)) & 1u;
+    }
+
+    // Pending cells per tile.
+    template<int TW, int TH>
+    __global__ void tail_count_kernel(std::uint32_t const* pending_bits, std::int64_t ntiles, std::uint32_t* counts)
+    {
+        using G = Tile<TW, TH>;
+        std::int64_t const tile = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+        if (tile >= ntiles)
+            return;
+        std::uint32_t n = 0;
+        for (int w = 0; w < G::words; ++w)
+            n += __popc(pending_bits[tile * G::words + w]);
+        counts[tile] = n;
+    }
+
+    // List the pending cells of each tile at its offset; record cell -> list index.
+    template<int TW, int TH>
+    __global__ void tail_compact_kernel(std::uint32_t const* pending_bits, int tiles_x, std::int64_t cols,
+                                        std::uint32_t const* offsets, std::uint32_t* cells, std::uint32_t* index_of)
+    {
+        using G = Tile<TW, TH>;
+        int const tile = blockIdx.x;
+        int const ty = tile / tiles_x;
+        int const tx = tile % tiles_x;
+        std::uint32_t const* bits = pending_bits + static_cast<std::int64_t>(tile) * G::words;
+        for (int w = threadIdx.x; w < G::words; w += blockDim.x)
+        {
+            std::uint32_t word = bits[w];
+            if (!word)
+                continue;
+            std::uint32_t pos = offsets[tile];
+            for (int v = 0; v < w; ++v)
+                pos += __popc(bits[v]);
+            while (word)
+            {
+                int const b = __ffs(word) - 1;
+                word &= word - 1;
+                int const i = w * 32 + b;
+                std::int64_t const r = static_cast<std::int64_t>(ty) * TH + i / TW;
+                std::int64_t const c = static_cast<std::int64_t>(tx) * TW + i % TW;
+                std::uint32_t const cell = static_cast<std::uint32_t>(r * cols + c);
+                cells[pos] = cell;
+                index_of[cell] = pos;
+                ++pos;
+            }
+        }
+    }
+
+    // Downstream pending cell of every pending cell, and pending upstream counts.
+    template<int TW, int TH>
+    __global__ void tail_links_kernel(std::uint8_t const* ldd, std::int64_t rows, std::int64_t cols, int tiles_x,
+                                      std::uint32_t const* pending_bits, std::uint32_t const* cells,
+                                      std::uint32_t const* index_of, std::uint32_t n, std::int32_t* next,
+                                      std::int32_t* upstream_count)
+    {
+        std::uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+        if (p >= n)
+            return;
+        std::uint32_t const cell = cells[p];
+        std::int64_t const r = cell / cols;
+        std::int64_t const c = cell % cols;
+        int const dir = ldd_direction(ldd[cell]);
+        std::int32_t nx = -1;
+        if (dir >= 0)
+        {
+            std::int64_t const r2 = r + neighbour_dr(dir);
+            std::int64_t const c2 = c + neighbour_dc(dir);
+            if (r2 >= 0 && r2 < rows && c2 >= 0 && c2 < cols && ldd[r2 * cols + c2] != no_data_ldd &&
+                cell_pending<TW, TH>(pending_bits, tiles_x, cols, r2, c2))
+            {
+                nx = static_cast<std::int32_t>(index_of[r2 * cols + c2]);
+                atomicAdd(&upstream_count[nx], 1);
+            }
+        }
+        next[p] = nx;
+    }
+
+    // Chain successor (link), and the initial state of list ranking.
+    __global__ void tail_rank_init_kernel(std::int32_t const* next, std::int32_t const* upstream_count, std::uint32_t n,
+                                          std::int32_t* succ, std::int32_t* rank, std::int32_t* last)
+    {
+        std::uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+        if (p >= n)
+            return;
+        std::int32_t const nx = next[p];
+        bool const linked = nx >= 0 && upstream_count[nx] == 1;
+        succ[p] = linked ? nx : -1;
+        rank[p] = linked ? 1 : 0;
+        last[p] = linked ? nx : static_cast<std::int32_t>(p);
+    }
+
+    // One pointer-jumping step (double buffered).
+    __global__ void tail_rank_step_kernel(std::int32_t const* succ, std::int32_t const* rank, std::int32_t const* last,
+                                          std::uint32_t n, std::int32_t* succ2, std::int32_t* rank2,
+                                          std::int32_t* last2)
+    {
+        std::uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+        if (p >= n)
+            return;
+        std::int32_t const s = succ[p];
+        if (s >= 0)
+        {
+            succ2[p] = succ[s];
+            rank2[p] = rank[p] + rank[s];
+            last2[p] = last[s];
+        }
+        else
+        {
+            succ2[p] = -1;
+            rank2[p] = rank[p];
+            last2[p] = last[p];
+        }
+    }
+
+    // Chain ids: chains are numbered by their last cell; is_last[p] = 1 for last cells.
+    __global__ void tail_flag_last_kernel(std::int32_t const* next, std::int32_t const* upstream_count, std::uint32_t n,
+                                          std::uint32_t* is_last)
+    {
+        std::uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+        if (p >= n)
+            return;
+        std::int32_t const nx = next[p];
+        is_last[p] = (nx >= 0 && upstream_count[nx] == 1) ? 0u : 1u;
+    }
+
+    // Per chain (numbered by the exclusive scan of is_last): length, first cell, downstream link.
+    __global__ void tail_chain_kernel(std::int32_t const* next, std::int32_t const* upstream_count,
+                                      std::int32_t const* rank, std::int32_t const* last, std::uint32_t const* chain_of_last,
+                                      std::uint32_t n, std::uint32_t* length, std::int32_t* first,
+                                      std::int32_t* arrivals)
+    {
+        std::uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+        if (p >= n)
+            return;
+        std::int32_t const u = upstream_count[p];
+        if (u != 1)
+        {
+            // p starts a chain; its end is a chain end (a cycle without entry has neither)
+            std::uint32_t const chain = chain_of_last[last[p]];
+            length[chain] = static_cast<std::uint32_t>(rank[p]) + 1;
+            first[chain] = static_cast<std::int32_t>(p);
+            arrivals[chain] = u;
+        }
+    }
+
+    __global__ void tail_downstream_kernel(std::int32_t const* next, std::int32_t const* last,
+                                           std::uint32_t const* chain_of_last, std::uint32_t const* is_last,
+                                           std::uint32_t n, std::int32_t* downstream)
+    {
+        std::uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+        if (p >= n || !is_last[p])
+            return;
+        std::uint32_t const chain = chain_of_last[p];
+        std::int32_t const nx = next[p];
+        // the chain below starts at nx (a confluence); it is numbered by its own last cell
+        downstream[chain] = nx >= 0 ? static_cast<std::int32_t>(chain_of_last[last[nx]]) : -1;
+    }
+
+    // Lay the cells out per chain in flow order: position = length - 1 - rank.
+    // Cells on a cycle without entry have no chain end; they are skipped here and never solved (the
+    // caller reports an invalid network).
+    __global__ void tail_layout_kernel(std::uint32_t const* cells, std::int32_t const* rank, std::int32_t const* last,
+                                       std::uint32_t const* is_last, std::uint32_t const* chain_of_last,
+                                       std::uint32_t const* base, std::uint32_t const* length, std::uint32_t n,
+                                       std::uint32_t* chain_cells)
+    {
+        std::uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+        if (p >= n)
+            return;
+        std::int32_t const end = last[p];
+        if (end < 0 || static_cast<std::uint32_t>(end) >= n || !is_last[end])
+            return;
+        std::uint32_t const chain = chain_of_last[end];
+        std::uint32_t const len = length[chain];
+        std::int32_t const r = rank[p];
+        if (len == 0 || r < 0 || static_cast<std::uint32_t>(r) >= len)
+            return;
+        std::uint32_t const pos = base[chain] + len - 1 - static_cast<std::uint32_t>(r);
+        if (pos < n)
+            chain_cells[pos] = cells[p];
+    }
+
+    __global__ void tail_ready_kernel(std::int32_t const* arrivals, std::uint32_t nchains, std::int32_t* ready,
+                                      std::uint32_t* nready)
+    {
+        std::uint32_t const chain = blockIdx.x * blockDim.x + threadIdx.x;
+        if (chain >= nchains || arrivals[chain] != 0)
+            return;
+        ready[atomicAdd(nready, 1u)] = static_cast<std::int32_t>(chain);
+    }
+
+    // ---------------------------------------------------------------------------------------------------
+    // Execution
+
+    __device__ inline double read_lane(double v, int lane)
+    {
+        std::uint64_t const bits = __double_as_longlong(v);
+        std::uint32_t const lo = __builtin_amdgcn_readlane(static_cast<std::uint32_t>(bits), lane);
+        std::uint32_t const hi = __builtin_amdgcn_readlane(static_cast<std::uint32_t>(bits >> 32), lane);
+        return __longlong_as_double((static_cast<std::uint64_t>(hi) << 32) | lo);
+    }
+
+    __device__ inline float read_lane(float v, int lane)
+    {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+    }
+
+    __device__ inline int read_lane(int v, int lane)
+    {
+        return __builtin_amdgcn_readlane(v, lane);
+    }
+
+    // Sum (FP64, fixed neighbour order) of the outflow of the upstream neighbours of `cell`, except `skip`.
+    template<typename T>
+    __device__ inline double side_inflow(SweepArgs<T> const& a, std::uint32_t cell, std::int64_t skip, bool& poisoned)
+    {
+        std::int64_t const r = cell / a.cols;
+        std::int64_t const c = cell % a.cols;
+        double sum = 0.0;
+        for (int k = 0; k < 8; ++k)
+        {
+            std::int64_t const r2 = r + neighbour_dr(k);
+            std::int64_t const c2 = c + neighbour_dc(k);
+            if (r2 < 0 || r2 >= a.rows || c2 < 0 || c2 >= a.cols)
+                continue;
+            std::int64_t const u = r2 * a.cols + c2;
+            std::uint8_t const code = a.ldd[u];
+            if (code == no_data_ldd || ldd_direction(code) != opposite(k) || u == skip)
+                continue;
+            double v = static_cast<double>(a.out[u]);
+            if constexpr (std::is_same_v<T, float>)
+                v += static_cast<double>(a.out_lo[u]);
+            if (isnan(v))
+                poisoned = true;
+            else
+                sum += v;
+        }
+        return sum;
+    }
+
+    // One wavefront per chain; 64-cell blocks.
+    template<int OP, typename T>
+    __global__ void __launch_bounds__(256) tail_execute_kernel(SweepArgs<T> a, TailChains t)
+    {
+        constexpr bool compensated = std::is_same_v<T, float>;
+        constexpr bool two_outputs = OP == op_threshold || OP == op_fraction;
+        int const lane = threadIdx.x & 63;
+        int chain = -1;
+        unsigned long long solved = 0;
+        while (true)
+        {
+            if (chain < 0)
+            {
+                int next = -1;
+                if (lane == 0)
+                {
+                    std::uint32_t const slot = atomicAdd(t.cursor, 1u);
+                    next = slot < t.nready ? t.ready[slot] : -1;
+                }
+                chain = read_lane(next, 0);
+                if (chain < 0)
+                    break;
+            }
+            std::uint32_t const base = t.base[chain];
+            std::uint32_t const length = t.length[chain];
+            double carry = 0.0;        // outflow of the previous chain cell
+            double carry_delta = 0.0;  // kinematic wave FP32: correction of `carry` (first order)
+            for (std::uint32_t block = 0; block < length; block += 64)
+            {
+                std::uint32_t const k = block + lane;
+                bool const valid = k < length;
+                std::uint32_t const cell = valid ? t.cells[base + k] : 0u;
+                std::int64_t const prev = (valid && k > 0) ? static_cast<std::int64_t>(t.cells[base + k - 1]) : -1;
+                bool poisoned = false;
+                double const side = valid ? side_inflow<T>(a, cell, prev, poisoned) : 0.0;
+                CellInput<OP, T> in{};
+                if (valid)
+                    stage_input<OP, T>(a, cell, in);
+                int const nb = static_cast<int>(length - block < 64 ? length - block : 64);
+
+                double value = 0.0;  // this lane's cell outflow (FP64)
+                T rem{};
+                if constexpr (OP == op_kinematic_wave && compensated)
+                {
+                    // serial FP32 solve along the block; FP64 correction afterwards, in parallel
+                    double q_up_mine = 0.0;
+                    float q32_mine = 0.0f;
+                    for (int j = 0; j < nb; ++j)
+                    {
+                        double const s = read_lane(side, j);
+                        int const bad = read_lane(static_cast<int>(poisoned), j);
+                        float const q_cur = read_lane(in.q_cur, j);
+                        double const q_up = carry + s;
+                        double q = carry;  // NaN stays NaN
+                        if (bad || isnan(q_up) || isnan(q_cur))
+                        {
+                            q = __longlong_as_double(0x7ff8000000000000ll);
+                        }
+                        else
+                        {
+                            float const q_lat = read_lane(in.q_lat, j);
+                            float const alpha = read_lane(in.alpha, j);
+                            float const beta = read_lane(in.beta, j);
+                            double const b = read_lane(in.b, j);
+                            double const c1 = read_lane(in.c1, j);
+                            if (!(q_up + q_cur > 0 || q_lat > 0))
+                            {
+                                q = 0.0;
+                            }
+                            else if (q_up + q_cur == 0.0 && beta < 1.0f)
+                            {
+                                q = new_discharge_f32_path(q_up, q_cur, q_lat, alpha, beta, a.dt, b, c1);
+                            }
+                            else
+                            {
+                                bool converged;
+                                float const known = static_cast<float>(c1 * q_up + b);
+                                q = static_cast<double>(solve_fast_f32(static_cast<float>(c1), alpha, beta, a.dt,
+                                                                      static_cast<float>(q_up), q_cur, q_lat, known,
+                                                                      converged));
+                            }
+                        }
+                        if (lane == j)
+                        {
+                            q_up_mine = q_up;
+                            q32_mine = static_cast<float>(q);
+                        }
+                        carry = q;
+                    }
+                    // parallel FP64 correction: eps = G(q_up) - q, s = dG/dq_up (per lane)
+                    double eps = 0.0;
+                    double sens = 0.0;
+                    double const q = static_cast<double>(q32_mine);
+                    if (valid && q > 0.0 && !isnan(q))
+                    {
+                        double const xb = dpow(q, static_cast<double>(in.beta));
+                        double const ax = static_cast<double>(in.alpha) * xb;
+                        double const f0 = in.c1 * q + ax - (in.c1 * q_up_mine + in.b);
+                        double const f1 = in.c1 + static_cast<double>(in.beta) * ax / q;
+                        eps = -f0 / f1;
+                        sens = in.c1 / f1;
+                    }
+                    // affine scan along the block: delta_k = eps_k + sens_k * delta_{k-1}
+                    double mul = sens;
+                    double add = eps;
+                    for (int offset = 1; offset < 64; offset <<= 1)
+                    {
+                        double const m_prev = __shfl_up(mul, offset, 64);
+                        double const a_prev = __shfl_up(add, offset, 64);
+                        if (lane >= offset)
+                        {
+                            add = add + mul * a_prev;
+                            mul = mul * m_prev;
+                        }
+                    }
+                    double const delta = add + mul * carry_delta;
+                    value = isnan(q) ? q : (valid ? (q + delta > 0.0 ? q + delta : q) : 0.0);
+                    // the corrected last value starts the next block
+                    carry = read_lane(value, nb - 1);
+                    carry_delta = 0.0;
+                }
+                else
+                {
+                    for (int j = 0; j < nb; ++j)
+                    {
+                        double const s = read_lane(side, j);
+                        int const bad = read_lane(static_cast<int>(poisoned), j);
+                        double v;
+                        T r{};
+                        if constexpr (OP == op_accu)
+                        {
+                            T const x = read_lane(in.x, j);
+                            CellInput<OP, T> cj{x};
+                            solve_cell<OP, T>(a, cj, carry + s, bad != 0 || isnan(carry), v, r);
+                        }
+                        else if constexpr (OP == op_threshold || OP == op_fraction)
+                        {
+                            CellInput<OP, T> cj{read_lane(in.x, j), read_lane(in.p, j)};
+                            solve_cell<OP, T>(a, cj, carry + s, bad != 0 || isnan(carry), v, r);
+                        }
+                        else
+                        {
+                            CellInput<OP, T> cj{read_lane(in.q_cur, j), read_lane(in.q_lat, j), read_lane(in.alpha, j),
+                                                read_lane(in.beta, j), read_lane(in.b, j), read_lane(in.c1, j)};
+                            solve_cell<OP, T>(a, cj, carry + s, bad != 0 || isnan(carry), v, r);
+                        }
+                        if (lane == j)
+                        {
+                            value = v;
+                            rem = r;
+                        }
+                        carry = v;
+                    }
+                }
+                if (valid)
+                {
+                    T const hi = static_cast<T>(value);
+                    a.out[cell] = hi;
+                    if constexpr (compensated)
+                        a.out_lo[cell] = isnan(value) ? 0.0f : static_cast<float>(value - static_cast<double>(hi));
+                    if constexpr (two_outputs)
+                        a.rem[cell] = rem;
+                }
+                solved += valid ? 1 : 0;
+            }
+            // end of the chain: arrive at the confluence below; the last arrival continues there.
+            // The release covers the stores of all lanes of the wavefront.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            int next = -1;
+            if (lane == 0)
+            {
+                int const down = t.downstream[chain];
+                if (down >= 0)
+                {
+                    int const before = __hip_atomic_fetch_add(&t.arrivals[down], -1, __ATOMIC_ACQ_REL,
+                                                              __HIP_MEMORY_SCOPE_AGENT);
+                    next = before == 1 ? down : -1;
+                }
+            }
+            chain = read_lane(next, 0);
+            if (chain >= 0)
+            {
+                // acquire for every lane of the wavefront, not only the one that did the atomic
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+        }
+        // count solved cells (lanes hold their own counts)
+        for (int offset = 32; offset > 0; offset >>= 1)
+            solved += __shfl_down(solved, offset, 64);
+        if (lane == 0)
+            atomicAdd(t.solved, solved);
+    }
+
+}  // namespace lue_amd

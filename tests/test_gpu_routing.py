@@ -250,3 +250,80 @@ def test_config1_kinematic_wave_2048(fw, oracle_mod):
     got32 = fw.kinematic_wave(pa(fw, ldd, ps), *[pa(fw, a, ps) for a in f32], 0.6, 3600.0, 100.0).numpy()
     err = rel_err(got32, want32in)
     assert err.max() <= 1e-5, err.max()
+
+
+# ---------------------------------------------------------------------------------------------------
+# Tile rounds only vs chain-streaming tail: same results
+
+POLICIES = {"rounds_only": (-1, 0, 0.0), "default": (4, 24, 1.0 / 64.0), "tail_early": (1, 1, 1.0)}
+
+
+@pytest.fixture()
+def policy_lib(fw):
+    from lue_amd import _lib
+
+    yield _lib
+    _lib.set_tail_policy(*POLICIES["default"])
+
+
+@pytest.mark.parametrize("policy", list(POLICIES))
+def test_tail_policy_accu(fw, oracle_mod, policy_lib, policy):
+    from lue_amd import synthetic
+
+    policy_lib.set_tail_policy(*POLICIES[policy])
+    n0, n1 = 700, 900
+    ldd = synthetic.ldd_host(99, n0, n1)
+    material = synthetic.uniform_host(99, synthetic.STREAM_MATERIAL, 0.0, 1.0, (n0, n1))
+    ones = fw.accu(pa(fw, ldd), 1.0).numpy()
+    assert_equal_nan(ones, oracle_mod.accu(ldd, 1.0, None, np.float32))
+    got = fw.accu(pa(fw, ldd), pa(fw, material)).numpy()
+    want = oracle_mod.accu(ldd, material.astype(np.float64), None, np.float64)
+    assert rel_err(got, want).max() <= 1e-6
+    got64 = fw.accu(pa(fw, ldd), pa(fw, material.astype(np.float64))).numpy()
+    assert rel_err(got64, want).max() <= 1e-12
+
+
+@pytest.mark.parametrize("policy", list(POLICIES))
+def test_tail_policy_split_and_kinematic_wave(fw, oracle_mod, policy_lib, policy):
+    from lue_amd import synthetic
+
+    policy_lib.set_tail_policy(*POLICIES[policy])
+    n0, n1 = 600, 500
+    seed = 7
+    ldd = synthetic.ldd_host(seed, n0, n1)
+    shape = (n0, n1)
+    material = synthetic.uniform_host(seed, synthetic.STREAM_MATERIAL, 0.0, 1.0, shape, np.float64)
+    thr = synthetic.uniform_host(seed, synthetic.STREAM_THRESHOLD, 0.0, 0.5, shape, np.float64)
+    out, rem = fw.accu_threshold(pa(fw, ldd), pa(fw, material), pa(fw, thr))
+    w_out, w_rem = oracle_mod.accu_threshold(ldd, material, thr, None, np.float64)
+    scale = np.abs(w_out) + np.abs(w_rem) + 1e-30
+    assert np.max(np.abs(out.numpy() - w_out) / scale) <= 1e-12
+    assert np.max(np.abs(rem.numpy() - w_rem) / scale) <= 1e-12
+    q_cur = synthetic.uniform_host(seed, synthetic.STREAM_CURRENT, 0.0, 5.0, shape, np.float32)
+    q_lat = synthetic.uniform_host(seed, synthetic.STREAM_LATERAL, 0.0, 1e-4, shape, np.float32)
+    alpha = synthetic.uniform_host(seed, synthetic.STREAM_ALPHA, 1.5, 2.5, shape, np.float32)
+    want = oracle_mod.kinematic_wave(ldd, *[x.astype(np.float64) for x in (q_cur, q_lat, alpha)], 0.6, 3600.0,
+                                     100.0, None, np.float64)
+    got32 = fw.kinematic_wave(pa(fw, ldd), pa(fw, q_cur), pa(fw, q_lat), pa(fw, alpha), 0.6, 3600.0, 100.0).numpy()
+    assert rel_err(got32, want).max() <= 1e-6
+    got64 = fw.kinematic_wave(pa(fw, ldd), *[pa(fw, x.astype(np.float64)) for x in (q_cur, q_lat, alpha)], 0.6,
+                              3600.0, 100.0).numpy()
+    assert rel_err(got64, want).max() <= 1e-12
+
+
+@pytest.mark.parametrize("policy", list(POLICIES))
+def test_cycles_are_reported_with_every_policy(fw, policy_lib, policy):
+    from lue_amd import _lib
+
+    policy_lib.set_tail_policy(*POLICIES[policy])
+    ldd = np.full((150, 140), 2, dtype=np.uint8)
+    ldd[0, 0], ldd[0, 1], ldd[1, 1], ldd[1, 0] = 6, 2, 4, 8      # cycle without entry
+    ldd[80, 70], ldd[80, 71], ldd[81, 71], ldd[81, 70] = 6, 2, 4, 8  # cycle fed from upstream
+    with pytest.raises(_lib.InvalidNetworkError):
+        fw.accu(pa(fw, ldd), 1.0)
+    ldd[0, 0], ldd[0, 1], ldd[1, 1], ldd[1, 0] = 2, 2, 2, 2
+    with pytest.raises(_lib.InvalidNetworkError):
+        fw.accu(pa(fw, ldd), 1.0)
+    ldd[80, 70], ldd[80, 71], ldd[81, 71], ldd[81, 70] = 2, 2, 2, 2
+    ones = fw.accu(pa(fw, ldd), 1.0).numpy()
+    assert ones[-1].sum() == ldd.size
