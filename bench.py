@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--fp64-steps", type=int, default=2)
     p.add_argument("--cpu-sample", type=int, default=4096, help="side of the CPU-baseline sample (0: skip)")
     p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--tail-after", type=int, default=8, help="tile rounds before the chain tail (-1: none)")
     return p.parse_args()
 
 
@@ -85,44 +86,82 @@ def main():
     f64 = {k: fw.PartitionedArray(v.data.double(), ps) for k, v in f32.items()}
     torch.cuda.synchronize()
 
-    def step(inp):
-        acc = fw.accu(ldd, inp["material"])
-        q = fw.kinematic_wave(ldd, inp["q_cur"], inp["q_lat"], inp["alpha"], 0.6, 3600.0, 100.0)
+    # networks: the sweep schedule of the ldd, recorded once (tile shape per operation)
+    log("[bench] recording flow networks")
+    t0 = time.perf_counter()
+    net_accu = fw.FlowNetwork(ldd, tile_shape=(64, 64), tail_after=args.tail_after)
+    net_kw = fw.FlowNetwork(ldd, tile_shape=(32, 32), tail_after=args.tail_after)
+    torch.cuda.synchronize()
+    t_network = time.perf_counter() - t0
+    s_accu = torch.cuda.Stream()
+    s_kw = torch.cuda.Stream()
+
+    def step(inp, out=None):
+        # accu and kinematic_wave are independent: they run concurrently on two streams
+        with torch.cuda.stream(s_accu):
+            acc = net_accu.accu(inp["material"], out=None if out is None else out[0])
+        with torch.cuda.stream(s_kw):
+            q = net_kw.kinematic_wave(inp["q_cur"], inp["q_lat"], inp["alpha"], 0.6, 3600.0, 100.0,
+                                      out=None if out is None else out[1])
         return acc, q
 
-    # FP32 path ---------------------------------------------------------------------------------
-    log("[bench] fp32 warmup")
-    for _ in range(args.warmup):
-        out32 = step(f32)
-    torch.cuda.synchronize()
-    log(f"[bench] fp32 timed {args.steps} steps")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out32 = step(f32)
-    torch.cuda.synchronize()
-    t32 = (time.perf_counter() - t0) / args.steps
+    def sync():
+        s_accu.synchronize()
+        s_kw.synchronize()
 
-    # per-kernel timing of each operation with HIP events on the launch stream
+    def timed(inp, steps, warmup):
+        out = step(inp)
+        for _ in range(warmup):
+            step(inp, out)
+        sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(inp, out)
+        sync()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps, out
+
+    # FP32 path -----------------------------------------------------------------------------------
+    log(f"[bench] fp32 timed {args.steps} steps")
+    t32, out32 = timed(f32, args.steps, args.warmup)
+
+    # per-kernel timing of each operation with HIP events on its launch stream
     kernel_ms = {}
-    for name, fn in (("accu_f32", lambda: fw.accu(ldd, f32["material"])),
-                     ("kinematic_wave_f32", lambda: fw.kinematic_wave(ldd, f32["q_cur"], f32["q_lat"],
-                                                                        f32["alpha"], 0.6, 3600.0, 100.0))):
+    for name, fn in (("accu_f32", lambda: net_accu.accu(f32["material"], out=out32[0])),
+                     ("kinematic_wave_f32", lambda: net_kw.kinematic_wave(f32["q_cur"], f32["q_lat"], f32["alpha"],
+                                                                          0.6, 3600.0, 100.0, out=out32[1]))):
         _lib.profile_enable(True)
         for _ in range(args.steps):
             fn()
+        torch.cuda.synchronize()
         first, rest, calls, rounds, _ = _lib.profile_read()
         _lib.profile_enable(False)
-        kernel_ms[name] = dict(first_round_ms=first / calls, later_rounds_ms=rest / calls, rounds=rounds / calls)
+        kernel_ms[name] = dict(first_round_ms=first / calls, later_rounds_and_tail_ms=rest / calls,
+                               total_ms=(first + rest) / calls, rounds=rounds / calls)
 
-    # FP64 drop-in --------------------------------------------------------------------------------
-    log("[bench] fp64 steps")
-    out64 = step(f64)
+    # drop-in per-call path (records its schedule in every call)
+    log("[bench] per-call fp32 step")
+    fw.accu(ldd, f32["material"])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.fp64_steps):
-        out64 = step(f64)
+        fw.accu(ldd, f32["material"])
+        fw.kinematic_wave(ldd, f32["q_cur"], f32["q_lat"], f32["alpha"], 0.6, 3600.0, 100.0)
     torch.cuda.synchronize()
-    t64 = (time.perf_counter() - t0) / args.fp64_steps
+    t32_call = (time.perf_counter() - t0) / args.fp64_steps
+
+    # FP64 drop-in --------------------------------------------------------------------------------
+    log("[bench] fp64 steps")
+    t64, out64 = timed(f64, args.fp64_steps, 0)
+    t0 = time.perf_counter()
+    for _ in range(args.fp64_steps):
+        fw.accu(ldd, f64["material"])
+        fw.kinematic_wave(ldd, f64["q_cur"], f64["q_lat"], f64["alpha"], 0.6, 3600.0, 100.0)
+    torch.cuda.synchronize()
+    t64_call = (time.perf_counter() - t0) / args.fp64_steps
+    info_kw = net_kw.info()
+    info_accu = net_accu.info()
 
     # accuracy of the FP32 path against the FP64 drop-in on the same inputs
     def rel(a, b):
@@ -189,7 +228,11 @@ def main():
                      "kernel": f"sweep_first_round ({dominant})"},
         "cpu_baseline": cpu,
         "fp64_dropin": {"value": round(value64, 1), "unit": "Mcells/s", "ms_per_step": round(t64 * 1e3, 3),
-                        "fp32_speedup": round(value / value64, 3)},
+                        "fp32_speedup": round(value / value64, 3),
+                        "per_call_ms_per_step": round(t64_call * 1e3, 3)},
+        "fp32_per_call": {"value": round(cells / t32_call / 1e6, 1), "ms_per_step": round(t32_call * 1e3, 3),
+                          "note": "drop-in entry points, schedule recorded inside every call"},
+        "network": {"record_ms": round(t_network * 1e3, 1), "kinematic_wave": info_kw, "accu": info_accu},
         "accuracy_vs_fp64": {"accu_max_rel": acc_err[0], "accu_mean_rel": acc_err[1],
                              "kinematic_wave_max_rel": kw_err[0], "kinematic_wave_mean_rel": kw_err[1]},
         "kernels": kernel_ms,

@@ -56,12 +56,11 @@ int lue_amd_profile_read(double* first_round_ms, double* next_rounds_ms, long lo
                          long long* cells);
 
 /*
- * Tuning of the sweep (no reference counterpart): after at least min_rounds tile rounds, once at most
- * max(tiles * tile_fraction, 4) tiles are queued (or after max_rounds rounds), the remaining cells are
- * solved by chain streaming. min_rounds < 0: tile rounds only. Defaults: 4, 24, 1/64. Results do not
- * depend on the policy beyond floating point summation order.
+ * Tuning of the per-call entry points (no reference counterpart): number of tile rounds before the
+ * remaining cells are solved by chain streaming; -1: tile rounds only. Default 8. Results do not depend
+ * on it beyond floating point summation order.
  */
-int lue_amd_set_tail_policy(int min_rounds, int max_rounds, double tile_fraction);
+int lue_amd_set_sweep_options(int tail_after);
 
 /* Device memory used by the C++ front end's device-resident partitions. */
 int lue_amd_device_malloc(void** ptr, size_t bytes);
@@ -156,6 +155,55 @@ int lue_amd_synthetic_uniform_f32(uint64_t seed, uint64_t stream_id, double lo, 
                                   void* stream);
 int lue_amd_synthetic_uniform_f64(uint64_t seed, uint64_t stream_id, double lo, double hi, int64_t n, double* out,
                                   void* stream);
+
+/*
+ * Flow networks (no reference counterpart; LUE recomputes the inflow counts of an ldd in every call).
+ * A network is a batch of n catchments (independent ldd rasters, device pointers, kept borrowed: they
+ * must outlive the network and must not change) together with the schedule of the routing sweep, which
+ * depends on the ldd only: the tiles of each round and the chains of the tail. Operations on a network
+ * run that schedule as a fixed sequence of launches on `stream`, for all catchments at once, and return
+ * without waiting for the results (synchronise the stream before reading them).
+ * tile_shape: 0 = 32x32, 1 = 64x32, 2 = 64x64 cells per tile (operations that do not fit the shape in
+ * LDS return LUE_AMD_ERROR_INVALID_ARGUMENT). tail_after: tile rounds before the chain-streaming tail,
+ * -1 for tile rounds only. Creating a network validates it (cycles: LUE_AMD_ERROR_INVALID_NETWORK).
+ * The arguments of the operations are host arrays with one device pointer per catchment; an argument
+ * array may be NULL where the operation accepts a Scalar, which then applies to every catchment.
+ */
+typedef struct lue_amd_network_s* lue_amd_network;
+
+int lue_amd_network_create(int n, const uint8_t* const* ldd, const int64_t* rows, const int64_t* cols, int tile_shape,
+                           int tail_after, void* stream, lue_amd_network* network);
+int lue_amd_network_destroy(lue_amd_network network);
+/* info[0..9]: catchments, cells, tiles, recorded rounds, tail cells, chains, ready chains, no-data cells,
+ * tile shape, tail_after */
+int lue_amd_network_info(lue_amd_network network, int64_t* info, int n);
+
+int lue_amd_network_accu_f32(lue_amd_network network, const float* const* inflow, float inflow_scalar,
+                             float* const* outflow, void* stream);
+int lue_amd_network_accu_f64(lue_amd_network network, const double* const* inflow, double inflow_scalar,
+                             double* const* outflow, void* stream);
+int lue_amd_network_accu_threshold_f32(lue_amd_network network, const float* const* inflow, float inflow_scalar,
+                                       const float* const* threshold, float threshold_scalar, float* const* outflow,
+                                       float* const* remainder, void* stream);
+int lue_amd_network_accu_threshold_f64(lue_amd_network network, const double* const* inflow, double inflow_scalar,
+                                       const double* const* threshold, double threshold_scalar,
+                                       double* const* outflow, double* const* remainder, void* stream);
+int lue_amd_network_accu_fraction_f32(lue_amd_network network, const float* const* inflow, float inflow_scalar,
+                                      const float* const* fraction, float fraction_scalar, float* const* outflow,
+                                      float* const* remainder, void* stream);
+int lue_amd_network_accu_fraction_f64(lue_amd_network network, const double* const* inflow, double inflow_scalar,
+                                      const double* const* fraction, double fraction_scalar, double* const* outflow,
+                                      double* const* remainder, void* stream);
+int lue_amd_network_kinematic_wave_f32(lue_amd_network network, const float* const* current_outflow,
+                                       const float* const* inflow, const float* const* alpha, float alpha_scalar,
+                                       const float* const* beta, float beta_scalar, float time_step_duration,
+                                       const float* const* channel_length, float channel_length_scalar,
+                                       float* const* outflow, void* stream);
+int lue_amd_network_kinematic_wave_f64(lue_amd_network network, const double* const* current_outflow,
+                                       const double* const* inflow, const double* const* alpha, double alpha_scalar,
+                                       const double* const* beta, double beta_scalar, double time_step_duration,
+                                       const double* const* channel_length, double channel_length_scalar,
+                                       double* const* outflow, void* stream);
 
 #ifdef __cplusplus
 }

@@ -26,7 +26,18 @@ SIGNATURES = {
     "lue_amd_status_string": [c_int],
     "lue_amd_last_error": [],
     "lue_amd_version": [],
-    "lue_amd_set_tail_policy": [c_int, c_int, c_d],
+    "lue_amd_set_sweep_options": [c_int],
+    "lue_amd_network_create": [c_int, c_p, c_p, c_p, c_int, c_int, c_p, ctypes.POINTER(c_p)],
+    "lue_amd_network_destroy": [c_p],
+    "lue_amd_network_info": [c_p, c_p, c_int],
+    "lue_amd_network_accu_f32": [c_p, c_p, c_f, c_p, c_p],
+    "lue_amd_network_accu_f64": [c_p, c_p, c_d, c_p, c_p],
+    "lue_amd_network_accu_threshold_f32": [c_p, c_p, c_f, c_p, c_f, c_p, c_p, c_p],
+    "lue_amd_network_accu_threshold_f64": [c_p, c_p, c_d, c_p, c_d, c_p, c_p, c_p],
+    "lue_amd_network_accu_fraction_f32": [c_p, c_p, c_f, c_p, c_f, c_p, c_p, c_p],
+    "lue_amd_network_accu_fraction_f64": [c_p, c_p, c_d, c_p, c_d, c_p, c_p, c_p],
+    "lue_amd_network_kinematic_wave_f32": [c_p, c_p, c_p, c_p, c_f, c_p, c_f, c_f, c_p, c_f, c_p, c_p],
+    "lue_amd_network_kinematic_wave_f64": [c_p, c_p, c_p, c_p, c_d, c_p, c_d, c_d, c_p, c_d, c_p, c_p],
     "lue_amd_profile_enable": [c_int],
     "lue_amd_profile_read": [ctypes.POINTER(c_d), ctypes.POINTER(c_d)] + [ctypes.POINTER(ctypes.c_longlong)] * 3,
     "lue_amd_device_malloc": [ctypes.POINTER(c_p), c_size],
@@ -87,9 +98,9 @@ def load():
     return lib
 
 
-def set_tail_policy(min_rounds=4, max_rounds=24, tile_fraction=1.0 / 64.0):
-    """min_rounds < 0: tile rounds only (no chain-streaming tail)."""
-    check(load().lue_amd_set_tail_policy(min_rounds, max_rounds, tile_fraction))
+def set_sweep_options(tail_after=8):
+    """Tile rounds before the chain-streaming tail in per-call operations; -1: tile rounds only."""
+    check(load().lue_amd_set_sweep_options(tail_after))
 
 
 def profile_enable(enable=True):

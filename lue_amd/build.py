@@ -18,7 +18,7 @@ OUTPUT = os.path.join(HERE, "liblue_amd.so")
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
-    "-std=c++17",
+    "-std=c++20",
     "-fPIC",
     "-shared",
     # packed-FP32 instructions stay off (they are generated from plain float code otherwise)

@@ -7,10 +7,10 @@
 #include "routing_common.hpp"
 #include "sweep.hpp"
 #include "synthetic.hpp"
+#include "network.hpp"
 #include "tail.hpp"
 
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -20,52 +20,117 @@
 #include <vector>
 
 namespace lue_amd {
+
+    std::string& last_error_message()
+    {
+        thread_local std::string message;
+        return message;
+    }
+
+    Profile& profile()
+    {
+        static Profile p;
+        return p;
+    }
+
     namespace {
 
-        thread_local std::string last_error;
+        // Rounds of tiles before the chain-streaming tail takes over, for per-call entry points
+        // (lue_amd_set_sweep_options); -1: tile rounds only.
+        int default_tail_after = 8;
 
-        // Optional per-call timing of the sweep launches with HIP events on the caller's stream.
-        struct Profile
+        template<int OP, typename T, int TW, int TH>
+        constexpr bool fits()
         {
-            bool enabled{false};
-            double first_round_ms{0.0};
-            double next_rounds_ms{0.0};
-            long long calls{0};
-            long long rounds{0};
-            long long cells{0};
-        };
-        Profile profile;
-
-        // When the round loop hands over to the chain-streaming tail: after at least `min_rounds`
-        // rounds, once at most max(tiles * tile_fraction, 4) tiles are queued, and always after
-        // `max_rounds` rounds. min_rounds < 0 disables the tail (rounds only).
-        struct TailPolicy
-        {
-            int min_rounds{4};
-            int max_rounds{24};
-            double tile_fraction{1.0 / 64.0};
-        };
-        TailPolicy tail_policy;
-
-        int fail(int status, std::string message)
-        {
-            last_error = std::move(message);
-            return status;
+            return sizeof(SweepLds<OP, T, TW, TH>) <= 160 * 1024;
         }
 
-        int hip_fail(hipError_t e, char const* what)
+        // Call f.template operator()<TW, TH>() for a tile shape the operation fits in.
+        template<int OP, typename T, typename F>
+        int with_tile_shape(int shape, F&& f)
         {
-            int const status = e == hipErrorOutOfMemory ? status_out_of_memory : status_hip_error;
-            return fail(status, std::string(what) + ": " + hipGetErrorString(e));
+            switch (shape)
+            {
+                case tile_32x32:
+                    if constexpr (fits<OP, T, 32, 32>())
+                        return f.template operator()<32, 32>();
+                    break;
+                case tile_64x32:
+                    if constexpr (fits<OP, T, 64, 32>())
+                        return f.template operator()<64, 32>();
+                    break;
+                case tile_64x64:
+                    if constexpr (fits<OP, T, 64, 64>())
+                        return f.template operator()<64, 64>();
+                    break;
+                default:
+                    break;
+            }
+            return fail(status_invalid_argument, "tile shape does not fit this operation in LDS");
         }
 
-#define LUE_AMD_HIP(call)                                                                                       \
-    do                                                                                                          \
-    {                                                                                                           \
-        hipError_t const e_ = (call);                                                                           \
-        if (e_ != hipSuccess)                                                                                   \
-            return hip_fail(e_, #call);                                                                         \
-    } while (0)
+        // Tile shape of per-call entry points: the largest one whose LDS footprint leaves two workgroups
+        // per CU.
+        template<int OP, typename T>
+        constexpr int default_tile_shape()
+        {
+            if constexpr (sizeof(SweepLds<OP, T, 64, 64>) <= 80 * 1024)
+                return tile_64x64;
+            else if constexpr (sizeof(SweepLds<OP, T, 64, 32>) <= 80 * 1024)
+                return tile_64x32;
+            else
+                return tile_32x32;
+        }
+
+        // Scalars of an operation (rasters come per catchment in CatchmentIO).
+        template<typename T>
+        struct Scalars
+        {
+            T in1_s{};
+            T in2_s{};
+            T alpha_s{};
+            T beta_s{};
+            T dx_s{};
+            T dt{};
+        };
+
+        template<typename T>
+        SweepArgs<T> args_from(Scalars<T> const& sc)
+        {
+            SweepArgs<T> a{};
+            a.in1_s = sc.in1_s;
+            a.in2_s = sc.in2_s;
+            a.alpha_s = sc.alpha_s;
+            a.beta_s = sc.beta_s;
+            a.dx_s = sc.dx_s;
+            a.dt = sc.dt;
+            return a;
+        }
+
+        // One operation on one raster: record into a temporary network.
+        template<int OP, typename T>
+        int route_once(std::uint8_t const* ldd, std::int64_t rows, std::int64_t cols, Scalars<T> const& sc,
+                       CatchmentIO<T> const& io, hipStream_t stream)
+        {
+            Network net;
+            if (int const st = network_init(net, 1, &ldd, &rows, &cols, default_tile_shape<OP, T>(),
+                                            default_tail_after, stream))
+                return st;
+            std::vector<CatchmentIO<T>> table{io};
+            return with_tile_shape<OP, T>(net.tile_shape, [&]<int TW, int TH>()
+                                          { return record_run<OP, T, TW, TH>(net, args_from(sc), table, stream); });
+        }
+
+        // One operation on a recorded network.
+        template<int OP, typename T>
+        int route_network(Network const& net, Scalars<T> const& sc, std::vector<CatchmentIO<T>> const& io,
+                          hipStream_t stream)
+        {
+            if (io.size() != net.cats_host.size())
+                return fail(status_invalid_argument, "one input/output set per catchment");
+            return with_tile_shape<OP, T>(net.tile_shape, [&]<int TW, int TH>()
+                                          { return replay_run<OP, T, TW, TH>(net, args_from(sc), io, stream); });
+        }
 
         int check_shape(void const* ldd, std::int64_t rows, std::int64_t cols)
         {
@@ -73,320 +138,7 @@ namespace lue_amd {
                 return fail(status_invalid_argument, "ldd must not be null");
             if (rows <= 0 || cols <= 0)
                 return fail(status_invalid_argument, "array shape must not be empty");
-            if (rows > (1ll << 31) || cols > (1ll << 31))
-                return fail(status_invalid_argument, "array extent too large");
             return status_success;
-        }
-
-        // Scratch of one sweep, allocated stream-ordered.
-        struct SweepScratch
-        {
-            std::uint32_t* pending_bits{};
-            std::uint32_t* border_stamp{};
-            std::uint32_t* tile_flag{};
-            std::uint32_t* worklist{};
-            std::uint32_t* work_count{};
-            unsigned long long* counters{};
-            float* out_lo{};
-            hipStream_t stream{};
-
-            ~SweepScratch()
-            {
-                for (void* p : {static_cast<void*>(pending_bits), static_cast<void*>(border_stamp),
-                                static_cast<void*>(tile_flag), static_cast<void*>(worklist),
-                                static_cast<void*>(work_count), static_cast<void*>(counters),
-                                static_cast<void*>(out_lo)})
-                {
-                    if (p)
-                        (void)hipFreeAsync(p, stream);
-                }
-            }
-        };
-
-        template<typename P>
-        hipError_t alloc(P*& p, std::size_t count, hipStream_t stream)
-        {
-            void* raw = nullptr;
-            hipError_t const e = hipMallocAsync(&raw, std::max<std::size_t>(count, 1) * sizeof(P), stream);
-            p = static_cast<P*>(raw);
-            return e;
-        }
-
-
-        template<typename P>
-        struct DeviceArray
-        {
-            P* ptr{};
-            hipStream_t stream{};
-            ~DeviceArray()
-            {
-                if (ptr)
-                    (void)hipFreeAsync(ptr, stream);
-            }
-        };
-
-        template<typename P>
-        hipError_t alloc_array(DeviceArray<P>& a, std::size_t count, hipStream_t stream)
-        {
-            a.stream = stream;
-            return alloc(a.ptr, count, stream);
-        }
-
-        template<typename In, typename Out>
-        hipError_t exclusive_scan(In const* in, Out* out, std::size_t n, hipStream_t stream)
-        {
-            std::size_t bytes = 0;
-            hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, stream);
-            if (e != hipSuccess)
-                return e;
-            void* temp = nullptr;
-            e = hipMallocAsync(&temp, std::max<std::size_t>(bytes, 1), stream);
-            if (e != hipSuccess)
-                return e;
-            e = hipcub::DeviceScan::ExclusiveSum(temp, bytes, in, out, n, stream);
-            hipError_t const f = hipFreeAsync(temp, stream);
-            return e != hipSuccess ? e : f;
-        }
-
-        inline unsigned blocks_for(std::uint64_t n, unsigned threads = 256)
-        {
-            return static_cast<unsigned>(std::max<std::uint64_t>((n + threads - 1) / threads, 1));
-        }
-
-        // Solve all pending cells by chain streaming (tail.hpp). Adds the number of solved cells to *solved.
-        template<int OP, typename T, int TW, int TH>
-        int run_tail(SweepArgs<T> const& a, hipStream_t stream, unsigned long long* solved_total)
-        {
-            std::int64_t const ntiles = static_cast<std::int64_t>(a.tiles_x) * a.tiles_y;
-            std::uint64_t const ncells = static_cast<std::uint64_t>(a.rows * a.cols);
-            if (ncells >= (1ull << 32))
-                return fail(status_invalid_argument, "chain tail needs fewer than 2^32 cells");
-
-            DeviceArray<std::uint32_t> counts, offsets;
-            LUE_AMD_HIP(alloc_array(counts, ntiles + 1, stream));
-            LUE_AMD_HIP(alloc_array(offsets, ntiles + 1, stream));
-            LUE_AMD_HIP(hipMemsetAsync(counts.ptr + ntiles, 0, sizeof(std::uint32_t), stream));
-            hipLaunchKernelGGL((tail_count_kernel<TW, TH>), dim3(blocks_for(ntiles)), dim3(256), 0, stream,
-                               a.pending_bits, ntiles, counts.ptr);
-            LUE_AMD_HIP(hipGetLastError());
-            LUE_AMD_HIP(exclusive_scan(counts.ptr, offsets.ptr, static_cast<std::size_t>(ntiles + 1), stream));
-            std::uint32_t n = 0;
-            LUE_AMD_HIP(hipMemcpyAsync(&n, offsets.ptr + ntiles, sizeof(n), hipMemcpyDeviceToHost, stream));
-            LUE_AMD_HIP(hipStreamSynchronize(stream));
-            if (n == 0)
-                return status_success;
-
-            DeviceArray<std::uint32_t> cells, index_of, is_last, chain_of_last;
-            DeviceArray<std::int32_t> next, upstream_count, succ[2], rank[2], last[2];
-            LUE_AMD_HIP(alloc_array(cells, n, stream));
-            LUE_AMD_HIP(alloc_array(index_of, ncells, stream));
-            LUE_AMD_HIP(alloc_array(next, n, stream));
-            LUE_AMD_HIP(alloc_array(upstream_count, n, stream));
-            for (int b = 0; b < 2; ++b)
-            {
-                LUE_AMD_HIP(alloc_array(succ[b], n, stream));
-                LUE_AMD_HIP(alloc_array(rank[b], n, stream));
-                LUE_AMD_HIP(alloc_array(last[b], n, stream));
-            }
-            LUE_AMD_HIP(alloc_array(is_last, n + 1, stream));
-            LUE_AMD_HIP(alloc_array(chain_of_last, n + 1, stream));
-            LUE_AMD_HIP(hipMemsetAsync(upstream_count.ptr, 0, n * sizeof(std::int32_t), stream));
-            LUE_AMD_HIP(hipMemsetAsync(is_last.ptr + n, 0, sizeof(std::uint32_t), stream));
-
-            hipLaunchKernelGGL((tail_compact_kernel<TW, TH>), dim3(static_cast<unsigned>(ntiles)), dim3(64), 0, stream,
-                               a.pending_bits, a.tiles_x, a.cols, offsets.ptr, cells.ptr, index_of.ptr);
-            hipLaunchKernelGGL((tail_links_kernel<TW, TH>), dim3(blocks_for(n)), dim3(256), 0, stream, a.ldd, a.rows,
-                               a.cols, a.tiles_x, a.pending_bits, cells.ptr, index_of.ptr, n, next.ptr,
-                               upstream_count.ptr);
-            hipLaunchKernelGGL(tail_rank_init_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, next.ptr,
-                               upstream_count.ptr, n, succ[0].ptr, rank[0].ptr, last[0].ptr);
-            int cur = 0;
-            for (std::uint32_t reach = 1; reach < n; reach <<= 1)
-            {
-                hipLaunchKernelGGL(tail_rank_step_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, succ[cur].ptr,
-                                   rank[cur].ptr, last[cur].ptr, n, succ[1 - cur].ptr, rank[1 - cur].ptr,
-                                   last[1 - cur].ptr);
-                cur = 1 - cur;
-            }
-            hipLaunchKernelGGL(tail_flag_last_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, next.ptr,
-                               upstream_count.ptr, n, is_last.ptr);
-            LUE_AMD_HIP(hipGetLastError());
-            LUE_AMD_HIP(exclusive_scan(is_last.ptr, chain_of_last.ptr, static_cast<std::size_t>(n + 1), stream));
-            std::uint32_t nchains = 0;
-            LUE_AMD_HIP(hipMemcpyAsync(&nchains, chain_of_last.ptr + n, sizeof(nchains), hipMemcpyDeviceToHost, stream));
-            LUE_AMD_HIP(hipStreamSynchronize(stream));
-            if (nchains == 0)
-                return fail(status_invalid_network, "invalid flow direction network: pending cells form cycles");
-
-            DeviceArray<std::uint32_t> length, base, cursor_and_count, chain_cells;
-            DeviceArray<std::int32_t> first, arrivals, downstream, ready;
-            DeviceArray<unsigned long long> solved;
-            LUE_AMD_HIP(alloc_array(length, nchains + 1, stream));
-            LUE_AMD_HIP(alloc_array(base, nchains + 1, stream));
-            LUE_AMD_HIP(alloc_array(first, nchains, stream));
-            LUE_AMD_HIP(alloc_array(arrivals, nchains, stream));
-            LUE_AMD_HIP(alloc_array(downstream, nchains, stream));
-            LUE_AMD_HIP(alloc_array(ready, nchains, stream));
-            LUE_AMD_HIP(alloc_array(cursor_and_count, 2, stream));
-            LUE_AMD_HIP(alloc_array(chain_cells, n, stream));
-            LUE_AMD_HIP(alloc_array(solved, 1, stream));
-            // a chain whose first cell never appears (cells on a cycle) keeps length 0 and is never ready
-            LUE_AMD_HIP(hipMemsetAsync(length.ptr, 0, (nchains + 1) * sizeof(std::uint32_t), stream));
-            LUE_AMD_HIP(hipMemsetAsync(arrivals.ptr, 0xFF, nchains * sizeof(std::int32_t), stream));
-            LUE_AMD_HIP(hipMemsetAsync(cursor_and_count.ptr, 0, 2 * sizeof(std::uint32_t), stream));
-            LUE_AMD_HIP(hipMemsetAsync(solved.ptr, 0, sizeof(unsigned long long), stream));
-
-            hipLaunchKernelGGL(tail_chain_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, next.ptr,
-                               upstream_count.ptr, rank[cur].ptr, last[cur].ptr, chain_of_last.ptr, n, length.ptr,
-                               first.ptr, arrivals.ptr);
-            LUE_AMD_HIP(exclusive_scan(length.ptr, base.ptr, static_cast<std::size_t>(nchains + 1), stream));
-            hipLaunchKernelGGL(tail_downstream_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, next.ptr,
-                               last[cur].ptr, chain_of_last.ptr, is_last.ptr, n, downstream.ptr);
-            hipLaunchKernelGGL(tail_layout_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, cells.ptr, rank[cur].ptr,
-                               last[cur].ptr, is_last.ptr, chain_of_last.ptr, base.ptr, length.ptr, n,
-                               chain_cells.ptr);
-            hipLaunchKernelGGL(tail_ready_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, arrivals.ptr,
-                               nchains, ready.ptr, cursor_and_count.ptr + 1);
-            LUE_AMD_HIP(hipGetLastError());
-            std::uint32_t nready = 0;
-            LUE_AMD_HIP(hipMemcpyAsync(&nready, cursor_and_count.ptr + 1, sizeof(nready), hipMemcpyDeviceToHost, stream));
-            LUE_AMD_HIP(hipStreamSynchronize(stream));
-
-            TailChains t{chain_cells.ptr, base.ptr,  length.ptr,           downstream.ptr, arrivals.ptr,
-                         ready.ptr,       nready,    cursor_and_count.ptr, solved.ptr};
-            // one wavefront per chain; 4 wavefronts per workgroup, enough workgroups to fill the device
-            unsigned const grid = static_cast<unsigned>(std::min<std::uint64_t>((nready + 3) / 4, 2048));
-            if (nready > 0)
-            {
-                hipLaunchKernelGGL((tail_execute_kernel<OP, T>), dim3(std::max(grid, 1u)), dim3(256), 0, stream, a, t);
-                LUE_AMD_HIP(hipGetLastError());
-            }
-            unsigned long long tail_solved = 0;
-            LUE_AMD_HIP(hipMemcpyAsync(&tail_solved, solved.ptr, sizeof(tail_solved), hipMemcpyDeviceToHost, stream));
-            LUE_AMD_HIP(hipStreamSynchronize(stream));
-            *solved_total += tail_solved;
-            return status_success;
-        }
-
-        // Run the tiled sweep to completion: round 0 over all tiles, then queued tiles until none is left.
-        template<int OP, typename T, int TW, int TH>
-        int run_sweep(SweepArgs<T> a, hipStream_t stream)
-        {
-            using G = Tile<TW, TH>;
-            a.tiles_x = static_cast<int>((a.cols + TW - 1) / TW);
-            a.tiles_y = static_cast<int>((a.rows + TH - 1) / TH);
-            std::int64_t const ntiles = static_cast<std::int64_t>(a.tiles_x) * a.tiles_y;
-            if (ntiles > (1ll << 31) - 1)
-                return fail(status_invalid_argument, "too many tiles");
-
-            SweepScratch s;
-            s.stream = stream;
-            LUE_AMD_HIP(alloc(s.pending_bits, ntiles * G::words, stream));
-            LUE_AMD_HIP(alloc(s.border_stamp, ntiles * G::perimeter, stream));
-            LUE_AMD_HIP(alloc(s.tile_flag, 3 * ntiles, stream));
-            LUE_AMD_HIP(alloc(s.worklist, 3 * ntiles, stream));
-            LUE_AMD_HIP(alloc(s.work_count, 4, stream));
-            LUE_AMD_HIP(alloc(s.counters, 2, stream));
-            if constexpr (std::is_same_v<T, float>)
-            {
-                LUE_AMD_HIP(alloc(s.out_lo, static_cast<std::size_t>(a.rows * a.cols), stream));
-            }
-            LUE_AMD_HIP(hipMemsetAsync(s.tile_flag, 0, 3 * ntiles * sizeof(std::uint32_t), stream));
-            LUE_AMD_HIP(hipMemsetAsync(s.work_count, 0, 4 * sizeof(std::uint32_t), stream));
-            LUE_AMD_HIP(hipMemsetAsync(s.counters, 0, 2 * sizeof(unsigned long long), stream));
-
-            a.pending_bits = s.pending_bits;
-            a.border_stamp = s.border_stamp;
-            a.tile_flag = s.tile_flag;
-            a.worklist = s.worklist;
-            a.work_count = s.work_count;
-            a.counters = s.counters;
-            a.out_lo = s.out_lo;
-
-            hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-            if (profile.enabled)
-            {
-                for (auto& e : ev)
-                    LUE_AMD_HIP(hipEventCreate(&e));
-                LUE_AMD_HIP(hipEventRecord(ev[0], stream));
-            }
-            hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH>), dim3(static_cast<unsigned>(ntiles)), dim3(256), 0,
-                               stream, a);
-            LUE_AMD_HIP(hipGetLastError());
-            if (profile.enabled)
-                LUE_AMD_HIP(hipEventRecord(ev[1], stream));
-
-            unsigned const grid = static_cast<unsigned>(std::min<std::int64_t>(ntiles, 2048));
-            std::uint32_t round = 1;
-            std::uint32_t pending_tiles = 0;
-            bool use_tail = false;
-            TailPolicy const policy = tail_policy;
-            std::int64_t const tail_tiles =
-                std::max<std::int64_t>(static_cast<std::int64_t>(ntiles * policy.tile_fraction), 4);
-            while (true)
-            {
-                std::uint32_t const batch = policy.min_rounds >= 0 ? 4u : std::min<std::uint32_t>(round, 64u);
-                for (std::uint32_t k = 0; k < batch; ++k, ++round)
-                {
-                    hipLaunchKernelGGL((sweep_next_round<OP, T, TW, TH>), dim3(grid), dim3(256), 0, stream, a, round);
-                }
-                LUE_AMD_HIP(hipGetLastError());
-                LUE_AMD_HIP(hipMemcpyAsync(&pending_tiles, s.work_count + (round % 3), sizeof(std::uint32_t),
-                                           hipMemcpyDeviceToHost, stream));
-                LUE_AMD_HIP(hipStreamSynchronize(stream));
-                if (pending_tiles == 0)
-                    break;
-                if (policy.min_rounds >= 0 && static_cast<int>(round) > policy.min_rounds &&
-                    (pending_tiles <= tail_tiles || static_cast<int>(round) > policy.max_rounds))
-                {
-                    use_tail = true;
-                    break;
-                }
-                if (round > 0xFFFF0000u)
-                    return fail(status_invalid_network, "routing did not converge");
-            }
-            unsigned long long tail_solved = 0;
-            if (use_tail)
-            {
-                if (int const st = run_tail<OP, T, TW, TH>(a, stream, &tail_solved))
-                    return st;
-            }
-            if (profile.enabled)
-                LUE_AMD_HIP(hipEventRecord(ev[2], stream));
-            unsigned long long counters[2] = {0, 0};
-            LUE_AMD_HIP(hipMemcpyAsync(counters, s.counters, sizeof(counters), hipMemcpyDeviceToHost, stream));
-            LUE_AMD_HIP(hipStreamSynchronize(stream));
-            if (profile.enabled)
-            {
-                float first = 0.0f, rest = 0.0f;
-                LUE_AMD_HIP(hipEventElapsedTime(&first, ev[0], ev[1]));
-                LUE_AMD_HIP(hipEventElapsedTime(&rest, ev[1], ev[2]));
-                profile.first_round_ms += first;
-                profile.next_rounds_ms += rest;
-                profile.calls += 1;
-                profile.rounds += round;
-                profile.cells += a.rows * a.cols;
-                for (auto& e : ev)
-                    (void)hipEventDestroy(e);
-            }
-            if (counters[0] + counters[1] + tail_solved != static_cast<unsigned long long>(a.rows * a.cols))
-            {
-                return fail(status_invalid_network,
-                            "invalid flow direction network: " +
-                                std::to_string(a.rows * a.cols - counters[0] - counters[1] - tail_solved) +
-                                " cells lie on or downstream of a cycle");
-            }
-            return status_success;
-        }
-
-        template<typename T>
-        SweepArgs<T> base_args(std::uint8_t const* ldd, std::int64_t rows, std::int64_t cols)
-        {
-            SweepArgs<T> a{};
-            a.ldd = ldd;
-            a.rows = rows;
-            a.cols = cols;
-            return a;
         }
 
         template<typename T>
@@ -397,14 +149,10 @@ namespace lue_amd {
                 return st;
             if (!outflow)
                 return fail(status_invalid_argument, "outflow must not be null");
-            auto a = base_args<T>(ldd, rows, cols);
-            a.in1 = inflow;
-            a.in1_s = inflow_s;
-            a.out = outflow;
-            if constexpr (std::is_same_v<T, float>)
-                return run_sweep<op_accu, T, 64, 64>(a, static_cast<hipStream_t>(stream));
-            else
-                return run_sweep<op_accu, T, 64, 32>(a, static_cast<hipStream_t>(stream));
+            Scalars<T> sc{};
+            sc.in1_s = inflow_s;
+            CatchmentIO<T> io{inflow, nullptr, nullptr, nullptr, nullptr, outflow, nullptr};
+            return route_once<op_accu, T>(ldd, rows, cols, sc, io, static_cast<hipStream_t>(stream));
         }
 
         template<int OP, typename T>
@@ -415,17 +163,11 @@ namespace lue_amd {
                 return st;
             if (!outflow || !remainder)
                 return fail(status_invalid_argument, "outflow and remainder must not be null");
-            auto a = base_args<T>(ldd, rows, cols);
-            a.in1 = inflow;
-            a.in1_s = inflow_s;
-            a.in2 = param;
-            a.in2_s = param_s;
-            a.out = outflow;
-            a.rem = remainder;
-            if constexpr (std::is_same_v<T, float>)
-                return run_sweep<OP, T, 64, 32>(a, static_cast<hipStream_t>(stream));
-            else
-                return run_sweep<OP, T, 32, 32>(a, static_cast<hipStream_t>(stream));
+            Scalars<T> sc{};
+            sc.in1_s = inflow_s;
+            sc.in2_s = param_s;
+            CatchmentIO<T> io{inflow, param, nullptr, nullptr, nullptr, outflow, remainder};
+            return route_once<OP, T>(ldd, rows, cols, sc, io, static_cast<hipStream_t>(stream));
         }
 
         template<typename T>
@@ -437,21 +179,29 @@ namespace lue_amd {
                 return st;
             if (!outflow || !current || !inflow)
                 return fail(status_invalid_argument, "current_outflow, inflow and outflow must not be null");
-            auto a = base_args<T>(ldd, rows, cols);
-            a.in1 = current;
-            a.in2 = inflow;
-            a.alpha = alpha;
-            a.alpha_s = alpha_s;
-            a.beta = beta;
-            a.beta_s = beta_s;
-            a.dx = dx;
-            a.dx_s = dx_s;
-            a.dt = dt;
-            a.out = outflow;
-            return run_sweep<op_kinematic_wave, T, 32, 32>(a, static_cast<hipStream_t>(stream));
+            Scalars<T> sc{};
+            sc.alpha_s = alpha_s;
+            sc.beta_s = beta_s;
+            sc.dx_s = dx_s;
+            sc.dt = dt;
+            CatchmentIO<T> io{current, inflow, alpha, beta, dx, outflow, nullptr};
+            return route_once<op_kinematic_wave, T>(ldd, rows, cols, sc, io, static_cast<hipStream_t>(stream));
         }
 
-        // iterate_to_new_discharge, elementwise (kinematic_wave.hpp:145-254, extraction included)
+        // Per-catchment pointer tables of the network entry points (host arrays of device pointers; a null
+        // table means the scalar).
+        template<typename T>
+        T const* entry(T const* const* table, int c)
+        {
+            return table ? table[c] : nullptr;
+        }
+
+        template<typename T>
+        T* entry(T* const* table, int c)
+        {
+            return table ? table[c] : nullptr;
+        }
+
         template<typename T>
         __global__ void iterate_kernel(T const* up, T const* cur, T const* lat, T const* alpha, T const* beta,
                                        T const* dt, T const* dx, T* out, std::int64_t n)
@@ -544,7 +294,7 @@ const char* lue_amd_status_string(int status)
 
 const char* lue_amd_last_error(void)
 {
-    return last_error.c_str();
+    return last_error_message().c_str();
 }
 
 const char* lue_amd_version(void)
@@ -552,18 +302,18 @@ const char* lue_amd_version(void)
     return "0.1.0";
 }
 
-int lue_amd_set_tail_policy(int min_rounds, int max_rounds, double tile_fraction)
+int lue_amd_set_sweep_options(int tail_after)
 {
-    if (max_rounds < 0 || !(tile_fraction >= 0.0) || tile_fraction > 1.0)
-        return fail(status_invalid_argument, "bad tail policy");
-    tail_policy = TailPolicy{min_rounds, max_rounds, tile_fraction};
+    if (tail_after < -1)
+        return fail(status_invalid_argument, "tail_after must be -1 (tile rounds only) or >= 0");
+    default_tail_after = tail_after;
     return status_success;
 }
 
 int lue_amd_profile_enable(int enable)
 {
-    profile = Profile{};
-    profile.enabled = enable != 0;
+    profile() = Profile{};
+    profile().enabled = enable != 0;
     return status_success;
 }
 
@@ -572,11 +322,11 @@ int lue_amd_profile_read(double* first_round_ms, double* next_rounds_ms, long lo
 {
     if (!first_round_ms || !next_rounds_ms || !calls || !rounds || !cells)
         return fail(status_invalid_argument, "null pointer");
-    *first_round_ms = profile.first_round_ms;
-    *next_rounds_ms = profile.next_rounds_ms;
-    *calls = profile.calls;
-    *rounds = profile.rounds;
-    *cells = profile.cells;
+    *first_round_ms = profile().first_round_ms;
+    *next_rounds_ms = profile().next_rounds_ms;
+    *calls = profile().calls;
+    *rounds = profile().rounds;
+    *cells = profile().cells;
     return status_success;
 }
 
@@ -743,5 +493,158 @@ int lue_amd_synthetic_uniform_f64(uint64_t seed, uint64_t stream_id, double lo, 
 {
     return uniform_impl<double>(seed, stream_id, lo, hi, n, out, stream);
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------------
+// Flow networks: a batch of catchments with the recorded sweep schedule
+
+struct lue_amd_network_s
+{
+    lue_amd::Network net;
+};
+
+extern "C" {
+
+int lue_amd_network_create(int n, const uint8_t* const* ldd, const int64_t* rows, const int64_t* cols, int tile_shape,
+                           int tail_after, void* stream, lue_amd_network* network)
+{
+    if (!network)
+        return fail(status_invalid_argument, "network must not be null");
+    *network = nullptr;
+    if (tail_after < -1)
+        return fail(status_invalid_argument, "tail_after must be -1 (tile rounds only) or >= 0");
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    auto handle = std::make_unique<lue_amd_network_s>();
+    Network& net = handle->net;
+    if (int const st = network_init(net, n, ldd, reinterpret_cast<std::int64_t const*>(rows),
+                                    reinterpret_cast<std::int64_t const*>(cols), tile_shape, tail_after, s))
+        return st;
+    // record the schedule with accu(ldd, 1) into scratch (every operation fits every tile shape of accu f32)
+    DeviceArray<float> scratch;
+    LUE_AMD_HIP(scratch.allocate(static_cast<std::size_t>(net.ncells), s));
+    std::vector<CatchmentIO<float>> io;
+    for (auto const& cat : net.cats_host)
+        io.push_back(CatchmentIO<float>{nullptr, nullptr, nullptr, nullptr, nullptr, scratch.ptr + cat.cell_base, nullptr});
+    Scalars<float> sc{};
+    sc.in1_s = 1.0f;
+    int const st = with_tile_shape<op_accu, float>(net.tile_shape, [&]<int TW, int TH>()
+                                                   { return record_run<op_accu, float, TW, TH>(net, args_from(sc), io, s); });
+    if (st)
+        return st;
+    LUE_AMD_HIP(hipStreamSynchronize(s));
+    *network = handle.release();
+    return status_success;
+}
+
+int lue_amd_network_destroy(lue_amd_network network)
+{
+    if (network)
+    {
+        int device = 0;
+        (void)hipGetDevice(&device);
+        (void)hipDeviceSynchronize();
+        delete network;
+    }
+    return status_success;
+}
+
+int lue_amd_network_info(lue_amd_network network, int64_t* info, int n)
+{
+    if (!network || !info || n < 0)
+        return fail(status_invalid_argument, "null argument");
+    Network const& net = network->net;
+    int64_t const values[] = {static_cast<int64_t>(net.cats_host.size()),
+                              net.ncells,
+                              net.ntiles,
+                              net.rounds(),
+                              net.tail_cells,
+                              net.nchains,
+                              net.nready,
+                              net.nodata_cells,
+                              net.tile_shape,
+                              net.tail_after};
+    for (int i = 0; i < n && i < static_cast<int>(sizeof(values) / sizeof(values[0])); ++i)
+        info[i] = values[i];
+    return status_success;
+}
+
+#define LUE_AMD_NETWORK_OPS(T, SUFFIX)                                                                          \
+    int lue_amd_network_accu_##SUFFIX(lue_amd_network network, const T* const* inflow, T inflow_scalar,         \
+                                      T* const* outflow, void* stream)                                          \
+    {                                                                                                           \
+        if (!network || !outflow)                                                                               \
+            return fail(status_invalid_argument, "null argument");                                              \
+        Network const& net = network->net;                                                                      \
+        Scalars<T> sc{};                                                                                        \
+        sc.in1_s = inflow_scalar;                                                                               \
+        std::vector<CatchmentIO<T>> io;                                                                         \
+        for (int c = 0; c < static_cast<int>(net.cats_host.size()); ++c)                                        \
+        {                                                                                                       \
+            if (!outflow[c])                                                                                    \
+                return fail(status_invalid_argument, "outflow must not be null");                              \
+            io.push_back(CatchmentIO<T>{entry(inflow, c), nullptr, nullptr, nullptr, nullptr, outflow[c], nullptr}); \
+        }                                                                                                       \
+        return route_network<op_accu, T>(net, sc, io, static_cast<hipStream_t>(stream));                        \
+    }                                                                                                           \
+    int lue_amd_network_accu_threshold_##SUFFIX(lue_amd_network network, const T* const* inflow, T inflow_scalar, \
+                                                const T* const* threshold, T threshold_scalar, T* const* outflow, \
+                                                T* const* remainder, void* stream)                              \
+    {                                                                                                           \
+        if (!network || !outflow || !remainder)                                                                 \
+            return fail(status_invalid_argument, "null argument");                                              \
+        Network const& net = network->net;                                                                      \
+        Scalars<T> sc{};                                                                                        \
+        sc.in1_s = inflow_scalar;                                                                               \
+        sc.in2_s = threshold_scalar;                                                                            \
+        std::vector<CatchmentIO<T>> io;                                                                         \
+        for (int c = 0; c < static_cast<int>(net.cats_host.size()); ++c)                                        \
+            io.push_back(CatchmentIO<T>{entry(inflow, c), entry(threshold, c), nullptr, nullptr, nullptr, outflow[c], \
+                                        remainder[c]});                                                         \
+        return route_network<op_threshold, T>(net, sc, io, static_cast<hipStream_t>(stream));                   \
+    }                                                                                                           \
+    int lue_amd_network_accu_fraction_##SUFFIX(lue_amd_network network, const T* const* inflow, T inflow_scalar, \
+                                               const T* const* fraction, T fraction_scalar, T* const* outflow,  \
+                                               T* const* remainder, void* stream)                               \
+    {                                                                                                           \
+        if (!network || !outflow || !remainder)                                                                 \
+            return fail(status_invalid_argument, "null argument");                                              \
+        Network const& net = network->net;                                                                      \
+        Scalars<T> sc{};                                                                                        \
+        sc.in1_s = inflow_scalar;                                                                               \
+        sc.in2_s = fraction_scalar;                                                                             \
+        std::vector<CatchmentIO<T>> io;                                                                         \
+        for (int c = 0; c < static_cast<int>(net.cats_host.size()); ++c)                                        \
+            io.push_back(CatchmentIO<T>{entry(inflow, c), entry(fraction, c), nullptr, nullptr, nullptr, outflow[c], \
+                                        remainder[c]});                                                         \
+        return route_network<op_fraction, T>(net, sc, io, static_cast<hipStream_t>(stream));                    \
+    }                                                                                                           \
+    int lue_amd_network_kinematic_wave_##SUFFIX(lue_amd_network network, const T* const* current_outflow,       \
+                                                const T* const* inflow, const T* const* alpha, T alpha_scalar,  \
+                                                const T* const* beta, T beta_scalar, T time_step_duration,      \
+                                                const T* const* channel_length, T channel_length_scalar,        \
+                                                T* const* outflow, void* stream)                                \
+    {                                                                                                           \
+        if (!network || !outflow || !current_outflow || !inflow)                                               \
+            return fail(status_invalid_argument, "null argument");                                              \
+        Network const& net = network->net;                                                                      \
+        Scalars<T> sc{};                                                                                        \
+        sc.alpha_s = alpha_scalar;                                                                              \
+        sc.beta_s = beta_scalar;                                                                                \
+        sc.dx_s = channel_length_scalar;                                                                        \
+        sc.dt = time_step_duration;                                                                             \
+        std::vector<CatchmentIO<T>> io;                                                                         \
+        for (int c = 0; c < static_cast<int>(net.cats_host.size()); ++c)                                        \
+        {                                                                                                       \
+            if (!current_outflow[c] || !inflow[c] || !outflow[c])                                               \
+                return fail(status_invalid_argument, "null raster");                                            \
+            io.push_back(CatchmentIO<T>{current_outflow[c], inflow[c], entry(alpha, c), entry(beta, c),         \
+                                        entry(channel_length, c), outflow[c], nullptr});                        \
+        }                                                                                                       \
+        return route_network<op_kinematic_wave, T>(net, sc, io, static_cast<hipStream_t>(stream));              \
+    }
+
+LUE_AMD_NETWORK_OPS(float, f32)
+LUE_AMD_NETWORK_OPS(double, f64)
 
 }  // extern "C"

@@ -1,4 +1,4 @@
-// Tiled topological sweep of the accumulating router on gfx950.
+// Tiled topological sweep of the accumulating router on gfx950, over a batch of catchments.
 //
 // What is computed: LUE's accumulating router (reference: source/framework/algorithm/include/lue/
 // framework/algorithm/definition/accumulating_router.hpp:525-1209) with one of four cell functors
@@ -6,8 +6,9 @@
 // its upstream neighbours are solved; its value is a function of the sum of their outflows and of its
 // own inputs.
 //
-// How. The raster is cut into TW x TH tiles. One launch ("round") processes a set of tiles; a
-// workgroup owns one tile at a time and
+// How. Every catchment (an independent raster) is cut into TW x TH tiles; the tiles of all catchments
+// of a batch are numbered consecutively. One launch ("round") processes a set of tiles; a workgroup
+// owns one tile at a time and
 //   1. stages the tile's ldd with a one-cell halo, derives per cell the mask of upstream neighbours
 //      and stages the inputs of the tile's pending cells in LDS (coalesced loads);
 //   2. counts, per pending cell, the upstream neighbours that are not solved yet (neighbours in other
@@ -16,10 +17,11 @@
 //      it is the last upstream contributor of the next cell (LDS atomics, workgroup scope). The cell
 //      value gathers the upstream outflows in fixed neighbour order (N, NE, E, SE, S, SW, W, NW), so
 //      results do not depend on which lane arrived last;
-//   4. writes the solved cells' outputs coalesced, a pending bitmap and border stamps, and queues
-//      the neighbouring tiles that received flow from a solved cell for the next round.
-// Rounds repeat until no tile is queued. Tiles only exchange data across launches (kernel
-// boundaries), so no device-scope fence runs inside a kernel.
+//   4. writes the solved cells' outputs coalesced, a pending bitmap and border stamps, and, when the
+//      schedule is being recorded, queues the neighbouring tiles that received flow for the next round.
+// Which tiles take part in which round depends on the ldd only, so a recorded schedule (per-round tile
+// lists) is replayed by later operations on the same network without queueing. Tiles only exchange
+// data across launches (kernel boundaries), so no device-scope fence runs inside a kernel.
 //
 // Precision. T = double: plain FP64. T = float: outflows carry a compensation term (out + lo is the
 // FP64 sum rounded to FP32 pairs), sums are formed in FP64, so FP32 rounding does not accumulate along
@@ -33,44 +35,63 @@
 
 #include <cmath>
 #include <cstdint>
+#include <limits>
 #include <type_traits>
 
 namespace lue_amd {
 
     constexpr std::uint32_t stamp_pending = 0xFFFFFFFFu;
 
-    template<typename T>
-    struct SweepArgs
+    // One raster of a batch.
+    struct Catchment
     {
         std::uint8_t const* ldd;
         std::int64_t rows;
         std::int64_t cols;
         int tiles_x;
         int tiles_y;
+        std::int64_t tile_base;  // global id of the first tile
+        std::int64_t cell_base;  // offset of the catchment's cells in per-cell scratch of the batch
+    };
 
-        // Arguments: raster pointer, or nullptr and the scalar.
+    // Inputs and outputs of one catchment (raster pointers; nullptr: the scalar of SweepArgs).
+    template<typename T>
+    struct CatchmentIO
+    {
         T const* in1;  // accu*: inflow        kinematic wave: current outflow
-        T in1_s;
         T const* in2;  // threshold | fraction | kinematic wave: lateral inflow
-        T in2_s;
         T const* alpha;
-        T alpha_s;
         T const* beta;
-        T beta_s;
         T const* dx;
+        T* out;        // outflow
+        T* rem;        // remainder (threshold, fraction)
+    };
+
+    template<typename T>
+    struct SweepArgs
+    {
+        Catchment const* cats;
+        CatchmentIO<T> const* io;
+        std::uint32_t const* tile_cat;  // catchment of every global tile
+        int ncats;
+        std::int64_t ntiles;
+
+        T in1_s;
+        T in2_s;
+        T alpha_s;
+        T beta_s;
         T dx_s;
         T dt;
 
-        T* out;            // outflow
-        T* rem;            // remainder (threshold, fraction)
-        float* out_lo;     // compensation of `out` (float path), written for cells read across rounds
+        float* out_lo;  // compensation of `out` (float path) by global cell, for cells read across rounds
 
         std::uint32_t* pending_bits;   // per tile: TW*TH/32 words, bit set = cell not solved yet
         std::uint32_t* border_stamp;   // per tile: perimeter cells, round in which the cell was solved
-        std::uint32_t* tile_flag;      // 3 x tiles: queued for round r (index r % 3)
-        std::uint32_t* worklist;       // 3 x tiles
-        std::uint32_t* work_count;     // 3 counters
-        unsigned long long* counters;  // [0] solved cells, [1] no-data cells
+        std::uint32_t* tile_flag;      // 3 x tiles: queued for round r (index r % 3); recording only
+        std::uint32_t* worklist;       // 3 x tiles; recording only
+        std::uint32_t* work_count;     // 3 counters; recording only
+        unsigned long long* counters;  // [0] solved cells, [1] no-data cells; recording only
+        int record;                    // queue tiles and count cells (schedule recording)
     };
 
     template<int TW, int TH>
@@ -126,7 +147,7 @@ namespace lue_amd {
     template<typename T>
     struct CellInput<op_kinematic_wave, T>
     {
-        T q_cur;   // NaN when any argument is no-data or outside the domain
+        T q_cur;    // NaN when any argument is no-data or outside the domain
         T q_lat;
         T alpha;
         T beta;
@@ -172,18 +193,19 @@ namespace lue_amd {
     }
 
     template<int OP, typename T>
-    __device__ inline void stage_input(SweepArgs<T> const& a, std::int64_t g, CellInput<OP, T>& c)
+    __device__ inline void stage_input(SweepArgs<T> const& a, CatchmentIO<T> const& io, std::int64_t g,
+                                       CellInput<OP, T>& c)
     {
         if constexpr (OP == op_accu)
         {
-            T const x = value_of(a.in1, a.in1_s, g);
+            T const x = value_of(io.in1, a.in1_s, g);
             // value_policies/accu.hpp:9-19: inflow >= 0
             c.x = (x >= T{0}) ? x : quiet_nan<T>();
         }
         else if constexpr (OP == op_threshold || OP == op_fraction)
         {
-            T const x = value_of(a.in1, a.in1_s, g);
-            T const p = value_of(a.in2, a.in2_s, g);
+            T const x = value_of(io.in1, a.in1_s, g);
+            T const p = value_of(io.in2, a.in2_s, g);
             // value_policies/accu_threshold.hpp:9-20, value_policies/accu_fraction.hpp:9-20
             bool const ok = OP == op_threshold ? (x >= T{0} && p >= T{0})
                                                : (x >= T{0} && p >= T{0} && p <= T{1});
@@ -192,11 +214,11 @@ namespace lue_amd {
         }
         else
         {
-            T const q_cur = value_of(a.in1, a.in1_s, g);
-            T const q_lat = value_of(a.in2, a.in2_s, g);
-            T const alpha = value_of(a.alpha, a.alpha_s, g);
-            T const beta = value_of(a.beta, a.beta_s, g);
-            T const dx = value_of(a.dx, a.dx_s, g);
+            T const q_cur = value_of(io.in1, a.in1_s, g);
+            T const q_lat = value_of(io.in2, a.in2_s, g);
+            T const alpha = value_of(io.alpha, a.alpha_s, g);
+            T const beta = value_of(io.beta, a.beta_s, g);
+            T const dx = value_of(io.dx, a.dx_s, g);
             // value_policies/kinematic_wave.hpp:9-22 plus no-data (NaN) in any argument
             bool const ok = q_cur >= T{0} && q_lat >= T{0} && dx > T{0} && !isnan(alpha) && !isnan(beta) &&
                             !isnan(a.dt);
@@ -283,7 +305,8 @@ namespace lue_amd {
 
     // One round over one tile. All threads of the workgroup call it.
     template<int OP, typename T, int TW, int TH>
-    __device__ void sweep_tile(SweepArgs<T> const& a, SweepLds<OP, T, TW, TH>& s, int tile, std::uint32_t round)
+    __device__ void sweep_tile(SweepArgs<T> const& a, SweepLds<OP, T, TW, TH>& s, std::int64_t tile,
+                               std::uint32_t round)
     {
         using G = Tile<TW, TH>;
         constexpr bool compensated = SweepLds<OP, T, TW, TH>::compensated;
@@ -293,12 +316,18 @@ namespace lue_amd {
 
         int const tid = threadIdx.x;
         int const nthreads = blockDim.x;
-        int const ty = tile / a.tiles_x;
-        int const tx = tile % a.tiles_x;
+        Catchment const cat = a.cats[a.tile_cat[tile]];
+        CatchmentIO<T> const io = a.io[a.tile_cat[tile]];
+        std::int64_t const local = tile - cat.tile_base;
+        int const ty = static_cast<int>(local / cat.tiles_x);
+        int const tx = static_cast<int>(local % cat.tiles_x);
+        std::int64_t const rows = cat.rows;
+        std::int64_t const cols = cat.cols;
         std::int64_t const r0 = static_cast<std::int64_t>(ty) * TH;
         std::int64_t const c0 = static_cast<std::int64_t>(tx) * TW;
-        std::uint32_t* const pbits = a.pending_bits + static_cast<std::int64_t>(tile) * G::words;
-        std::uint32_t* const stamps = a.border_stamp + static_cast<std::int64_t>(tile) * G::perimeter;
+        std::uint32_t* const pbits = a.pending_bits + tile * G::words;
+        std::uint32_t* const stamps = a.border_stamp + tile * G::perimeter;
+        float* const out_lo = compensated ? a.out_lo + cat.cell_base : nullptr;
 
         if (tid == 0)
         {
@@ -313,7 +342,7 @@ namespace lue_amd {
             int const hc = i % G::hw;
             std::int64_t const r = r0 + hr - 1;
             std::int64_t const c = c0 + hc - 1;
-            s.ldd[i] = (r >= 0 && r < a.rows && c >= 0 && c < a.cols) ? a.ldd[r * a.cols + c] : no_data_ldd;
+            s.ldd[i] = (r >= 0 && r < rows && c >= 0 && c < cols) ? cat.ldd[r * cols + c] : no_data_ldd;
         }
         __syncthreads();
 
@@ -331,14 +360,14 @@ namespace lue_amd {
                 st = state_absent;
                 std::int64_t const r = r0 + lr;
                 std::int64_t const c = c0 + lc;
-                if (round == 0 && r < a.rows && c < a.cols)
+                if (round == 0 && r < rows && c < cols)
                 {
                     // mark_no_data (accumulating_router.hpp:866-872)
-                    std::int64_t const g = r * a.cols + c;
-                    a.out[g] = quiet_nan<T>();
+                    std::int64_t const g = r * cols + c;
+                    io.out[g] = quiet_nan<T>();
                     if constexpr (two_outputs)
                     {
-                        a.rem[g] = quiet_nan<T>();
+                        io.rem[g] = quiet_nan<T>();
                     }
                     atomicAdd(&s.no_data, 1u);
                 }
@@ -357,8 +386,8 @@ namespace lue_amd {
                 }
                 if (pending)
                 {
-                    std::int64_t const g = (r0 + lr) * a.cols + (c0 + lc);
-                    stage_input<OP, T>(a, g, s.in[i]);
+                    std::int64_t const g = (r0 + lr) * cols + (c0 + lc);
+                    stage_input<OP, T>(a, io, g, s.in[i]);
                 }
             }
             s.state[i] = st;
@@ -389,25 +418,20 @@ namespace lue_amd {
                     {
                         n += s.state[ur * TW + uc] == state_pending ? 1u : 0u;
                     }
+                    else if (round == 0)
+                    {
+                        ++n;
+                    }
                     else
                     {
                         // upstream neighbour in another tile: solved when stamped in an earlier round
-                        if (round == 0)
-                        {
-                            ++n;
-                        }
-                        else
-                        {
-                            int const nty = ty + (ur < 0 ? -1 : (ur >= TH ? 1 : 0));
-                            int const ntx = tx + (uc < 0 ? -1 : (uc >= TW ? 1 : 0));
-                            int const nlr = ur < 0 ? TH - 1 : (ur >= TH ? 0 : ur);
-                            int const nlc = uc < 0 ? TW - 1 : (uc >= TW ? 0 : uc);
-                            int const ntile = nty * a.tiles_x + ntx;
-                            std::uint32_t const st =
-                                a.border_stamp[static_cast<std::int64_t>(ntile) * G::perimeter +
-                                               G::perimeter_index(nlr, nlc)];
-                            n += st < round ? 0u : 1u;
-                        }
+                        int const nty = ty + (ur < 0 ? -1 : (ur >= TH ? 1 : 0));
+                        int const ntx = tx + (uc < 0 ? -1 : (uc >= TW ? 1 : 0));
+                        int const nlr = ur < 0 ? TH - 1 : (ur >= TH ? 0 : ur);
+                        int const nlc = uc < 0 ? TW - 1 : (uc >= TW ? 0 : uc);
+                        std::int64_t const ntile = cat.tile_base + static_cast<std::int64_t>(nty) * cat.tiles_x + ntx;
+                        std::uint32_t const st = a.border_stamp[ntile * G::perimeter + G::perimeter_index(nlr, nlc)];
+                        n += st < round ? 0u : 1u;
                     }
                 }
                 if (n == 0)
@@ -448,10 +472,10 @@ namespace lue_amd {
                     else
                     {
                         // solved in an earlier round, value in HBM
-                        std::int64_t const ug = (r0 + ur) * a.cols + (c0 + uc);
-                        v = static_cast<double>(a.out[ug]);
+                        std::int64_t const ug = (r0 + ur) * cols + (c0 + uc);
+                        v = static_cast<double>(io.out[ug]);
                         if constexpr (compensated)
-                            v += static_cast<double>(a.out_lo[ug]);
+                            v += static_cast<double>(out_lo[ug]);
                     }
                     if (isnan(v))
                         poisoned = true;
@@ -509,10 +533,10 @@ namespace lue_amd {
             if (st == state_new)
             {
                 ++solved;
-                std::int64_t const g = r * a.cols + c;
-                a.out[g] = s.hi[i];
+                std::int64_t const g = r * cols + c;
+                io.out[g] = s.hi[i];
                 if constexpr (two_outputs)
-                    a.rem[g] = s.rem[i];
+                    io.rem[g] = s.rem[i];
                 int const dir = ldd_direction(s.ldd[(lr + 1) * G::hw + (lc + 1)]);
                 if (dir >= 0)
                 {
@@ -524,19 +548,17 @@ namespace lue_amd {
                     {
                         // read by a later round
                         if constexpr (compensated)
-                            a.out_lo[g] = s.lo[i];
-                        if (!inside)
+                            out_lo[g] = s.lo[i];
+                        if (!inside && a.record)
                         {
                             int const nty = ty + (dr < 0 ? -1 : (dr >= TH ? 1 : 0));
                             int const ntx = tx + (dc < 0 ? -1 : (dc >= TW ? 1 : 0));
-                            int const ntile = nty * a.tiles_x + ntx;
-                            std::uint32_t* flag = a.tile_flag + static_cast<std::int64_t>(next_slot) *
-                                                                     a.tiles_x * a.tiles_y + ntile;
+                            std::int64_t const ntile = cat.tile_base + static_cast<std::int64_t>(nty) * cat.tiles_x + ntx;
+                            std::uint32_t* flag = a.tile_flag + next_slot * a.ntiles + ntile;
                             if (atomicExch(flag, 1u) == 0u)
                             {
                                 std::uint32_t const slot = atomicAdd(&a.work_count[next_slot], 1u);
-                                a.worklist[static_cast<std::int64_t>(next_slot) * a.tiles_x * a.tiles_y + slot] =
-                                    static_cast<std::uint32_t>(ntile);
+                                a.worklist[next_slot * a.ntiles + slot] = static_cast<std::uint32_t>(ntile);
                             }
                         }
                     }
@@ -559,18 +581,21 @@ namespace lue_amd {
             }
             pbits[w] = bits;
         }
-        atomicAdd(&s.solved, solved);
-        __syncthreads();
-        if (tid == 0)
+        if (a.record)
         {
-            atomicAdd(&a.counters[0], static_cast<unsigned long long>(s.solved));
-            if (round == 0)
-                atomicAdd(&a.counters[1], static_cast<unsigned long long>(s.no_data));
+            atomicAdd(&s.solved, solved);
+            __syncthreads();
+            if (tid == 0)
+            {
+                atomicAdd(&a.counters[0], static_cast<unsigned long long>(s.solved));
+                if (round == 0)
+                    atomicAdd(&a.counters[1], static_cast<unsigned long long>(s.no_data));
+            }
         }
         __syncthreads();
     }
 
-    // Round 0: every tile. Grid = number of tiles.
+    // Round 0: every tile of the batch. Grid = number of tiles.
     template<int OP, typename T, int TW, int TH>
     __global__ void __launch_bounds__(256) sweep_first_round(SweepArgs<T> a)
     {
@@ -578,13 +603,12 @@ namespace lue_amd {
         sweep_tile<OP, T, TW, TH>(a, s, blockIdx.x, 0u);
     }
 
-    // Round r >= 1: the tiles queued by round r - 1. Workgroup b takes items b, b + grid, ...
+    // Round r >= 1 while recording: the tiles queued by round r - 1. Workgroup b takes items b, b + grid, ...
     template<int OP, typename T, int TW, int TH>
     __global__ void __launch_bounds__(256) sweep_next_round(SweepArgs<T> a, std::uint32_t round)
     {
         __shared__ SweepLds<OP, T, TW, TH> s;
         std::uint32_t const slot = round % 3;
-        std::int64_t const ntiles = static_cast<std::int64_t>(a.tiles_x) * a.tiles_y;
         if (blockIdx.x == 0 && threadIdx.x == 0)
         {
             // the slot that round + 1 fills was last read by round - 1
@@ -593,10 +617,22 @@ namespace lue_amd {
         std::uint32_t const count = a.work_count[slot];
         for (std::uint32_t item = blockIdx.x; item < count; item += gridDim.x)
         {
-            int const tile = static_cast<int>(a.worklist[slot * ntiles + item]);
+            std::int64_t const tile = a.worklist[slot * a.ntiles + item];
             if (threadIdx.x == 0)
-                a.tile_flag[slot * ntiles + tile] = 0u;
+                a.tile_flag[slot * a.ntiles + tile] = 0u;
             sweep_tile<OP, T, TW, TH>(a, s, tile, round);
+        }
+    }
+
+    // Round r >= 1 replayed from a recorded tile list.
+    template<int OP, typename T, int TW, int TH>
+    __global__ void __launch_bounds__(256) sweep_list_round(SweepArgs<T> a, std::uint32_t const* tiles,
+                                                            std::uint32_t count, std::uint32_t round)
+    {
+        __shared__ SweepLds<OP, T, TW, TH> s;
+        for (std::uint32_t item = blockIdx.x; item < count; item += gridDim.x)
+        {
+            sweep_tile<OP, T, TW, TH>(a, s, tiles[item], round);
         }
     }
 

@@ -34,7 +34,8 @@ namespace lue_amd {
 
     struct TailChains
     {
-        std::uint32_t const* cells;     // chain cells in flow order, chains back to back
+        std::uint32_t const* cells;     // chain cells (global cell ids) in flow order, chains back to back
+        std::uint32_t const* chain_cat; // per chain: catchment
         std::uint32_t const* base;      // per chain: offset into cells
         std::uint32_t const* length;    // per chain
         std::int32_t const* downstream; // per chain: chain starting at the confluence below the last cell, or -1
@@ -46,13 +47,29 @@ namespace lue_amd {
     };
 
     template<int TW, int TH>
-    __device__ inline bool cell_pending(std::uint32_t const* pending_bits, int tiles_x, std::int64_t cols, std::int64_t r,
+    __device__ inline bool cell_pending(std::uint32_t const* pending_bits, Catchment const& cat, std::int64_t r,
                                         std::int64_t c)
     {
         using G = Tile<TW, TH>;
-        std::int64_t const tile = (r / TH) * tiles_x + (c / TW);
+        std::int64_t const tile = cat.tile_base + (r / TH) * cat.tiles_x + (c / TW);
         int const i = static_cast<int>((r % TH) * TW + (c % TW));
         return (pending_bits[tile * G::words + (i >> 5)] >> (i & 31)) & 1u;
+    }
+
+    // Catchment of a global cell id (cell_base is increasing).
+    __device__ inline int find_catchment(Catchment const* cats, int ncats, std::uint64_t cell)
+    {
+        int lo = 0;
+        int hi = ncats - 1;
+        while (lo < hi)
+        {
+            int const mid = (lo + hi + 1) / 2;
+            if (static_cast<std::uint64_t>(cats[mid].cell_base) <= cell)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        return lo;
     }
 
     // Pending cells per tile.
@@ -69,16 +86,19 @@ namespace lue_amd {
         counts[tile] = n;
     }
 
-    // List the pending cells of each tile at its offset; record cell -> list index.
+    // List the pending cells of each tile at its offset; record global cell id -> list index.
     template<int TW, int TH>
-    __global__ void tail_compact_kernel(std::uint32_t const* pending_bits, int tiles_x, std::int64_t cols,
-                                        std::uint32_t const* offsets, std::uint32_t* cells, std::uint32_t* index_of)
+    __global__ void tail_compact_kernel(std::uint32_t const* pending_bits, Catchment const* cats,
+                                        std::uint32_t const* tile_cat, std::uint32_t const* offsets,
+                                        std::uint32_t* cells, std::uint32_t* index_of)
     {
         using G = Tile<TW, TH>;
-        int const tile = blockIdx.x;
-        int const ty = tile / tiles_x;
-        int const tx = tile % tiles_x;
-        std::uint32_t const* bits = pending_bits + static_cast<std::int64_t>(tile) * G::words;
+        std::int64_t const tile = blockIdx.x;
+        Catchment const cat = cats[tile_cat[tile]];
+        std::int64_t const local = tile - cat.tile_base;
+        int const ty = static_cast<int>(local / cat.tiles_x);
+        int const tx = static_cast<int>(local % cat.tiles_x);
+        std::uint32_t const* bits = pending_bits + tile * G::words;
         for (int w = threadIdx.x; w < G::words; w += blockDim.x)
         {
             std::uint32_t word = bits[w];
@@ -94,7 +114,7 @@ namespace lue_amd {
                 int const i = w * 32 + b;
                 std::int64_t const r = static_cast<std::int64_t>(ty) * TH + i / TW;
                 std::int64_t const c = static_cast<std::int64_t>(tx) * TW + i % TW;
-                std::uint32_t const cell = static_cast<std::uint32_t>(r * cols + c);
+                std::uint32_t const cell = static_cast<std::uint32_t>(cat.cell_base + r * cat.cols + c);
                 cells[pos] = cell;
                 index_of[cell] = pos;
                 ++pos;
@@ -104,27 +124,28 @@ namespace lue_amd {
 
     // Downstream pending cell of every pending cell, and pending upstream counts.
     template<int TW, int TH>
-    __global__ void tail_links_kernel(std::uint8_t const* ldd, std::int64_t rows, std::int64_t cols, int tiles_x,
-                                      std::uint32_t const* pending_bits, std::uint32_t const* cells,
-                                      std::uint32_t const* index_of, std::uint32_t n, std::int32_t* next,
-                                      std::int32_t* upstream_count)
+    __global__ void tail_links_kernel(Catchment const* cats, int ncats, std::uint32_t const* pending_bits,
+                                      std::uint32_t const* cells, std::uint32_t const* index_of, std::uint32_t n,
+                                      std::int32_t* next, std::int32_t* upstream_count)
     {
         std::uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
         if (p >= n)
             return;
         std::uint32_t const cell = cells[p];
-        std::int64_t const r = cell / cols;
-        std::int64_t const c = cell % cols;
-        int const dir = ldd_direction(ldd[cell]);
+        Catchment const cat = cats[find_catchment(cats, ncats, cell)];
+        std::int64_t const local = cell - cat.cell_base;
+        std::int64_t const r = local / cat.cols;
+        std::int64_t const c = local % cat.cols;
+        int const dir = ldd_direction(cat.ldd[local]);
         std::int32_t nx = -1;
         if (dir >= 0)
         {
             std::int64_t const r2 = r + neighbour_dr(dir);
             std::int64_t const c2 = c + neighbour_dc(dir);
-            if (r2 >= 0 && r2 < rows && c2 >= 0 && c2 < cols && ldd[r2 * cols + c2] != no_data_ldd &&
-                cell_pending<TW, TH>(pending_bits, tiles_x, cols, r2, c2))
+            if (r2 >= 0 && r2 < cat.rows && c2 >= 0 && c2 < cat.cols && cat.ldd[r2 * cat.cols + c2] != no_data_ldd &&
+                cell_pending<TW, TH>(pending_bits, cat, r2, c2))
             {
-                nx = static_cast<std::int32_t>(index_of[r2 * cols + c2]);
+                nx = static_cast<std::int32_t>(index_of[cat.cell_base + r2 * cat.cols + c2]);
                 atomicAdd(&upstream_count[nx], 1);
             }
         }
@@ -182,8 +203,9 @@ namespace lue_amd {
     // Per chain (numbered by the exclusive scan of is_last): length, first cell, downstream link.
     __global__ void tail_chain_kernel(std::int32_t const* next, std::int32_t const* upstream_count,
                                       std::int32_t const* rank, std::int32_t const* last, std::uint32_t const* chain_of_last,
-                                      std::uint32_t n, std::uint32_t* length, std::int32_t* first,
-                                      std::int32_t* arrivals)
+                                      std::uint32_t const* cells, Catchment const* cats, int ncats, std::uint32_t n,
+                                      std::uint32_t* length, std::int32_t* first, std::int32_t* arrivals,
+                                      std::uint32_t* chain_cat)
     {
         std::uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
         if (p >= n)
@@ -196,6 +218,7 @@ namespace lue_amd {
             length[chain] = static_cast<std::uint32_t>(rank[p]) + 1;
             first[chain] = static_cast<std::int32_t>(p);
             arrivals[chain] = u;
+            chain_cat[chain] = static_cast<std::uint32_t>(find_catchment(cats, ncats, cells[p]));
         }
     }
 
@@ -266,26 +289,28 @@ namespace lue_amd {
         return __builtin_amdgcn_readlane(v, lane);
     }
 
-    // Sum (FP64, fixed neighbour order) of the outflow of the upstream neighbours of `cell`, except `skip`.
+    // Sum (FP64, fixed neighbour order) of the outflow of the upstream neighbours of local cell `cell`,
+    // except local cell `skip`.
     template<typename T>
-    __device__ inline double side_inflow(SweepArgs<T> const& a, std::uint32_t cell, std::int64_t skip, bool& poisoned)
+    __device__ inline double side_inflow(Catchment const& cat, CatchmentIO<T> const& io, float const* out_lo,
+                                         std::int64_t cell, std::int64_t skip, bool& poisoned)
     {
-        std::int64_t const r = cell / a.cols;
-        std::int64_t const c = cell % a.cols;
+        std::int64_t const r = cell / cat.cols;
+        std::int64_t const c = cell % cat.cols;
         double sum = 0.0;
         for (int k = 0; k < 8; ++k)
         {
             std::int64_t const r2 = r + neighbour_dr(k);
             std::int64_t const c2 = c + neighbour_dc(k);
-            if (r2 < 0 || r2 >= a.rows || c2 < 0 || c2 >= a.cols)
+            if (r2 < 0 || r2 >= cat.rows || c2 < 0 || c2 >= cat.cols)
                 continue;
-            std::int64_t const u = r2 * a.cols + c2;
-            std::uint8_t const code = a.ldd[u];
+            std::int64_t const u = r2 * cat.cols + c2;
+            std::uint8_t const code = cat.ldd[u];
             if (code == no_data_ldd || ldd_direction(code) != opposite(k) || u == skip)
                 continue;
-            double v = static_cast<double>(a.out[u]);
+            double v = static_cast<double>(io.out[u]);
             if constexpr (std::is_same_v<T, float>)
-                v += static_cast<double>(a.out_lo[u]);
+                v += static_cast<double>(out_lo[u]);
             if (isnan(v))
                 poisoned = true;
             else
@@ -319,19 +344,24 @@ namespace lue_amd {
             }
             std::uint32_t const base = t.base[chain];
             std::uint32_t const length = t.length[chain];
+            std::uint32_t const ci = t.chain_cat[chain];
+            Catchment const cat = a.cats[ci];
+            CatchmentIO<T> const io = a.io[ci];
+            float* const out_lo = compensated ? a.out_lo + cat.cell_base : nullptr;
             double carry = 0.0;        // outflow of the previous chain cell
             double carry_delta = 0.0;  // kinematic wave FP32: correction of `carry` (first order)
             for (std::uint32_t block = 0; block < length; block += 64)
             {
                 std::uint32_t const k = block + lane;
                 bool const valid = k < length;
-                std::uint32_t const cell = valid ? t.cells[base + k] : 0u;
-                std::int64_t const prev = (valid && k > 0) ? static_cast<std::int64_t>(t.cells[base + k - 1]) : -1;
+                std::int64_t const cell = valid ? static_cast<std::int64_t>(t.cells[base + k]) - cat.cell_base : 0;
+                std::int64_t const prev =
+                    (valid && k > 0) ? static_cast<std::int64_t>(t.cells[base + k - 1]) - cat.cell_base : -1;
                 bool poisoned = false;
-                double const side = valid ? side_inflow<T>(a, cell, prev, poisoned) : 0.0;
+                double const side = valid ? side_inflow<T>(cat, io, out_lo, cell, prev, poisoned) : 0.0;
                 CellInput<OP, T> in{};
                 if (valid)
-                    stage_input<OP, T>(a, cell, in);
+                    stage_input<OP, T>(a, io, cell, in);
                 int const nb = static_cast<int>(length - block < 64 ? length - block : 64);
 
                 double value = 0.0;  // this lane's cell outflow (FP64)
@@ -451,11 +481,11 @@ namespace lue_amd {
                 if (valid)
                 {
                     T const hi = static_cast<T>(value);
-                    a.out[cell] = hi;
+                    io.out[cell] = hi;
                     if constexpr (compensated)
-                        a.out_lo[cell] = isnan(value) ? 0.0f : static_cast<float>(value - static_cast<double>(hi));
+                        out_lo[cell] = isnan(value) ? 0.0f : static_cast<float>(value - static_cast<double>(hi));
                     if constexpr (two_outputs)
-                        a.rem[cell] = rem;
+                        io.rem[cell] = rem;
                 }
                 solved += valid ? 1 : 0;
             }

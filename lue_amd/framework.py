@@ -213,3 +213,146 @@ def iterate_to_new_discharge(upstream, current, lateral, alpha, beta, dt, dx):
     fn = getattr(lib, f"lue_amd_iterate_to_new_discharge_{FLOAT_TYPES[dtype]}")
     _lib.check(fn(*[_ptr(t) for t in tensors], _ptr(out), tensors[0].numel(), _stream()))
     return out
+
+
+# ---------------------------------------------------------------------------------------------------
+# Flow networks: resident ldd rasters with their recorded sweep schedule (C ABI lue_amd_network_*)
+
+TILE_SHAPES = {(32, 32): 0, (64, 32): 1, (64, 64): 2}
+
+
+def _pointer_array(values):
+    import ctypes
+
+    arr = (ctypes.c_void_p * len(values))(*values)
+    return arr
+
+
+class FlowNetwork:
+    """One or more catchments (uint8 ldd PartitionedArrays) and the schedule of their routing sweep.
+
+    The schedule depends on the ldd only, so it is recorded once; every operation then runs as a fixed
+    launch sequence over all catchments at once. The ldd arrays must not change while the network lives.
+    tile_shape: (width, height) of the sweep tiles, one of TILE_SHAPES; tail_after: tile rounds before the
+    chain-streaming tail (-1: tile rounds only).
+    """
+
+    def __init__(self, flow_directions, tile_shape=(32, 32), tail_after=8):
+        import ctypes
+
+        if isinstance(flow_directions, PartitionedArray):
+            flow_directions = [flow_directions]
+        self.flow_directions = list(flow_directions)
+        for ldd in self.flow_directions:
+            _check_ldd(ldd)
+        lib = _lib.load()
+        n = len(self.flow_directions)
+        ptrs = _pointer_array([_ptr(a.data) for a in self.flow_directions])
+        rows = (ctypes.c_int64 * n)(*[a.shape[0] for a in self.flow_directions])
+        cols = (ctypes.c_int64 * n)(*[a.shape[1] for a in self.flow_directions])
+        handle = ctypes.c_void_p()
+        self._handle = None
+        _lib.check(lib.lue_amd_network_create(n, ptrs, rows, cols, TILE_SHAPES[tuple(tile_shape)], tail_after,
+                                              _stream(), ctypes.byref(handle)))
+        self._handle = handle
+
+    def close(self):
+        if getattr(self, "_handle", None):
+            _lib.load().lue_amd_network_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.flow_directions)
+
+    def info(self):
+        import ctypes
+
+        values = (ctypes.c_int64 * 10)()
+        _lib.check(_lib.load().lue_amd_network_info(self._handle, values, 10))
+        keys = ["catchments", "cells", "tiles", "rounds", "tail_cells", "chains", "ready_chains", "no_data_cells",
+                "tile_shape", "tail_after"]
+        return dict(zip(keys, list(values)))
+
+    # -- argument helpers -------------------------------------------------------------------------
+    def _rasters(self, value, dtype, name):
+        """Per-catchment pointer array (or None and a scalar)."""
+        if isinstance(value, (list, tuple)):
+            if len(value) != len(self):
+                raise ValueError(f"{name}: one raster per catchment")
+            for v, ldd in zip(value, self.flow_directions):
+                if not isinstance(v, PartitionedArray) or v.dtype != dtype:
+                    raise TypeError(f"{name}: PartitionedArrays of {dtype} expected")
+                _verify_compatible(ldd, v)
+            return _pointer_array([_ptr(v.data) for v in value]), 0.0, value
+        if isinstance(value, PartitionedArray):
+            if len(self) != 1:
+                raise ValueError(f"{name}: one raster per catchment")
+            return self._rasters([value], dtype, name)
+        return None, float(value), None
+
+    def _outputs(self, dtype, out=None):
+        if out is None:
+            out = [_output(ldd, dtype) for ldd in self.flow_directions]
+        elif isinstance(out, PartitionedArray):
+            out = [out]
+        return _pointer_array([_ptr(o.data) for o in out]), out
+
+    @staticmethod
+    def _dtype(*values):
+        for v in values:
+            if isinstance(v, (list, tuple)) and v:
+                return v[0].dtype
+            if isinstance(v, PartitionedArray):
+                return v.dtype
+        return torch.float32
+
+    def _unwrap(self, outs):
+        return outs[0] if len(self) == 1 else outs
+
+    # -- operations (asynchronous on the current stream) --------------------------------------------
+    def accu(self, inflow, out=None):
+        dtype = self._dtype(inflow)
+        p, s, keep = self._rasters(inflow, dtype, "inflow")
+        po, outs = self._outputs(dtype, out)
+        fn = getattr(_lib.load(), f"lue_amd_network_accu_{FLOAT_TYPES[dtype]}")
+        _lib.check(fn(self._handle, p, s, po, _stream()))
+        del keep
+        return self._unwrap(outs)
+
+    def _split(self, name, inflow, param, out=None, remainder=None):
+        dtype = self._dtype(inflow, param)
+        p1, s1, k1 = self._rasters(inflow, dtype, "inflow")
+        p2, s2, k2 = self._rasters(param, dtype, name)
+        po, outs = self._outputs(dtype, out)
+        pr, rems = self._outputs(dtype, remainder)
+        fn = getattr(_lib.load(), f"lue_amd_network_accu_{name}_{FLOAT_TYPES[dtype]}")
+        _lib.check(fn(self._handle, p1, s1, p2, s2, po, pr, _stream()))
+        del k1, k2
+        return self._unwrap(outs), self._unwrap(rems)
+
+    def accu_threshold(self, inflow, threshold, out=None, remainder=None):
+        return self._split("threshold", inflow, threshold, out, remainder)
+
+    def accu_fraction(self, inflow, fraction, out=None, remainder=None):
+        return self._split("fraction", inflow, fraction, out, remainder)
+
+    def kinematic_wave(self, current_outflow, inflow, alpha, beta, time_step_duration, channel_length, out=None):
+        dtype = self._dtype(current_outflow, inflow)
+        pc, _, kc = self._rasters(current_outflow, dtype, "current_outflow")
+        pq, _, kq = self._rasters(inflow, dtype, "inflow")
+        if pc is None or pq is None:
+            raise TypeError("current_outflow and inflow are rasters")
+        pa, sa, ka = self._rasters(alpha, dtype, "alpha")
+        pb, sb, kb = self._rasters(beta, dtype, "beta")
+        pl, sl, kl = self._rasters(channel_length, dtype, "channel_length")
+        po, outs = self._outputs(dtype, out)
+        fn = getattr(_lib.load(), f"lue_amd_network_kinematic_wave_{FLOAT_TYPES[dtype]}")
+        _lib.check(fn(self._handle, pc, pq, pa, sa, pb, sb, float(time_step_duration), pl, sl, po, _stream()))
+        del kc, kq, ka, kb, kl
+        return self._unwrap(outs)

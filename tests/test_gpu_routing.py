@@ -255,7 +255,7 @@ def test_config1_kinematic_wave_2048(fw, oracle_mod):
 # ---------------------------------------------------------------------------------------------------
 # Tile rounds only vs chain-streaming tail: same results
 
-POLICIES = {"rounds_only": (-1, 0, 0.0), "default": (4, 24, 1.0 / 64.0), "tail_early": (1, 1, 1.0)}
+POLICIES = {"rounds_only": -1, "default": 8, "tail_early": 1, "tail_at_once": 0}
 
 
 @pytest.fixture()
@@ -263,14 +263,14 @@ def policy_lib(fw):
     from lue_amd import _lib
 
     yield _lib
-    _lib.set_tail_policy(*POLICIES["default"])
+    _lib.set_sweep_options(POLICIES["default"])
 
 
 @pytest.mark.parametrize("policy", list(POLICIES))
 def test_tail_policy_accu(fw, oracle_mod, policy_lib, policy):
     from lue_amd import synthetic
 
-    policy_lib.set_tail_policy(*POLICIES[policy])
+    policy_lib.set_sweep_options(POLICIES[policy])
     n0, n1 = 700, 900
     ldd = synthetic.ldd_host(99, n0, n1)
     material = synthetic.uniform_host(99, synthetic.STREAM_MATERIAL, 0.0, 1.0, (n0, n1))
@@ -287,7 +287,7 @@ def test_tail_policy_accu(fw, oracle_mod, policy_lib, policy):
 def test_tail_policy_split_and_kinematic_wave(fw, oracle_mod, policy_lib, policy):
     from lue_amd import synthetic
 
-    policy_lib.set_tail_policy(*POLICIES[policy])
+    policy_lib.set_sweep_options(POLICIES[policy])
     n0, n1 = 600, 500
     seed = 7
     ldd = synthetic.ldd_host(seed, n0, n1)
@@ -315,7 +315,7 @@ def test_tail_policy_split_and_kinematic_wave(fw, oracle_mod, policy_lib, policy
 def test_cycles_are_reported_with_every_policy(fw, policy_lib, policy):
     from lue_amd import _lib
 
-    policy_lib.set_tail_policy(*POLICIES[policy])
+    policy_lib.set_sweep_options(POLICIES[policy])
     ldd = np.full((150, 140), 2, dtype=np.uint8)
     ldd[0, 0], ldd[0, 1], ldd[1, 1], ldd[1, 0] = 6, 2, 4, 8      # cycle without entry
     ldd[80, 70], ldd[80, 71], ldd[81, 71], ldd[81, 70] = 6, 2, 4, 8  # cycle fed from upstream
