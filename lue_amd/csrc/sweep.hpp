@@ -167,11 +167,15 @@ namespace lue_amd {
         float lo[compensated ? G::n : 1];
         T rem[two_outputs ? G::n : 1];
         std::uint32_t count[G::n / 4];  // four 8-bit counters per word
+        std::int16_t next[G::n];        // downstream cell inside the tile, -1 if none
+        std::uint16_t queue[G::n];      // initially ready cells
         std::uint8_t ldd[G::hn];
         std::uint8_t upmask[G::n];
         std::uint8_t state[G::n];  // see state_* below
         unsigned int solved;
         unsigned int no_data;
+        unsigned int queued;
+        unsigned int taken;
     };
 
     // Cell states inside a round.
@@ -312,7 +316,6 @@ namespace lue_amd {
         constexpr bool compensated = SweepLds<OP, T, TW, TH>::compensated;
         constexpr bool two_outputs = SweepLds<OP, T, TW, TH>::two_outputs;
         constexpr int cells_per_thread = (G::n + 255) / 256;
-        static_assert(cells_per_thread <= 32, "ready mask holds 32 cells per thread");
 
         int const tid = threadIdx.x;
         int const nthreads = blockDim.x;
@@ -333,6 +336,8 @@ namespace lue_amd {
         {
             s.solved = 0;
             s.no_data = 0;
+            s.queued = 0;
+            s.taken = 0;
         }
 
         // 1. ldd with halo; cells outside the array read as no-data
@@ -392,67 +397,99 @@ namespace lue_amd {
             }
             s.state[i] = st;
             s.upmask[i] = mask;
+            int next = -1;
+            if (st != state_absent)
+            {
+                int const dir = ldd_direction(code);
+                if (dir >= 0)
+                {
+                    int const dr = lr + neighbour_dr(dir);
+                    int const dc = lc + neighbour_dc(dir);
+                    if (dr >= 0 && dr < TH && dc >= 0 && dc < TW && s.ldd[(dr + 1) * G::hw + (dc + 1)] != no_data_ldd)
+                        next = dr * TW + dc;
+                }
+            }
+            s.next[i] = static_cast<std::int16_t>(next);
         }
         __syncthreads();
 
-        // 3. counts of unsolved upstream neighbours; initially ready cells
-        std::uint32_t ready = 0;
+        // 3. counts of unsolved upstream neighbours; the initially ready cells are compacted into an LDS
+        // queue (wavefront ballot + prefix count, one LDS atomic per wavefront)
+        int const lane = tid & 63;
+        std::uint64_t const lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
         for (int j = 0; j < cells_per_thread; ++j)
         {
             int const i = tid + j * nthreads;
-            if (i >= G::n)
-                break;
-            std::uint32_t n = 0;
-            if (s.state[i] == state_pending)
+            bool is_ready = false;
+            if (i < G::n)
             {
-                int const lr = i / TW;
-                int const lc = i % TW;
-                std::uint8_t const mask = s.upmask[i];
-                for (int k = 0; k < 8; ++k)
+                std::uint32_t n = 0;
+                if (s.state[i] == state_pending)
                 {
-                    if (!((mask >> k) & 1u))
-                        continue;
-                    int const ur = lr + neighbour_dr(k);
-                    int const uc = lc + neighbour_dc(k);
-                    if (ur >= 0 && ur < TH && uc >= 0 && uc < TW)
+                    int const lr = i / TW;
+                    int const lc = i % TW;
+                    std::uint8_t const mask = s.upmask[i];
+                    for (int k = 0; k < 8; ++k)
                     {
-                        n += s.state[ur * TW + uc] == state_pending ? 1u : 0u;
+                        if (!((mask >> k) & 1u))
+                            continue;
+                        int const ur = lr + neighbour_dr(k);
+                        int const uc = lc + neighbour_dc(k);
+                        if (ur >= 0 && ur < TH && uc >= 0 && uc < TW)
+                        {
+                            n += s.state[ur * TW + uc] == state_pending ? 1u : 0u;
+                        }
+                        else if (round == 0)
+                        {
+                            ++n;
+                        }
+                        else
+                        {
+                            // upstream neighbour in another tile: solved when stamped in an earlier round
+                            int const nty = ty + (ur < 0 ? -1 : (ur >= TH ? 1 : 0));
+                            int const ntx = tx + (uc < 0 ? -1 : (uc >= TW ? 1 : 0));
+                            int const nlr = ur < 0 ? TH - 1 : (ur >= TH ? 0 : ur);
+                            int const nlc = uc < 0 ? TW - 1 : (uc >= TW ? 0 : uc);
+                            std::int64_t const ntile =
+                                cat.tile_base + static_cast<std::int64_t>(nty) * cat.tiles_x + ntx;
+                            std::uint32_t const st =
+                                a.border_stamp[ntile * G::perimeter + G::perimeter_index(nlr, nlc)];
+                            n += st < round ? 0u : 1u;
+                        }
                     }
-                    else if (round == 0)
-                    {
-                        ++n;
-                    }
-                    else
-                    {
-                        // upstream neighbour in another tile: solved when stamped in an earlier round
-                        int const nty = ty + (ur < 0 ? -1 : (ur >= TH ? 1 : 0));
-                        int const ntx = tx + (uc < 0 ? -1 : (uc >= TW ? 1 : 0));
-                        int const nlr = ur < 0 ? TH - 1 : (ur >= TH ? 0 : ur);
-                        int const nlc = uc < 0 ? TW - 1 : (uc >= TW ? 0 : uc);
-                        std::int64_t const ntile = cat.tile_base + static_cast<std::int64_t>(nty) * cat.tiles_x + ntx;
-                        std::uint32_t const st = a.border_stamp[ntile * G::perimeter + G::perimeter_index(nlr, nlc)];
-                        n += st < round ? 0u : 1u;
-                    }
+                    is_ready = n == 0;
                 }
-                if (n == 0)
-                    ready |= 1u << j;
+                // 8-bit counter, written as a byte of the packed word
+                reinterpret_cast<std::uint8_t*>(s.count)[i] = static_cast<std::uint8_t>(n);
             }
-            // 8-bit counter, written as a byte of the packed word
-            reinterpret_cast<std::uint8_t*>(s.count)[i] = static_cast<std::uint8_t>(n);
+            std::uint64_t const ballot = __ballot(is_ready);
+            if (ballot)
+            {
+                unsigned int base = 0;
+                if (lane == 0)
+                    base = atomicAdd(&s.queued, static_cast<unsigned int>(__popcll(ballot)));
+                base = __shfl(base, 0, 64);
+                if (is_ready)
+                    s.queue[base + __popcll(ballot & lanes_below)] = static_cast<std::uint16_t>(i);
+            }
         }
         __syncthreads();
 
-        // 4. walks
-        for (int j = 0; j < cells_per_thread; ++j)
+        // 4. walks: lanes take initially ready cells from the queue and walk downstream while they are the
+        // last upstream contributor of the next cell
+        unsigned int const queued = s.queued;
+        while (true)
         {
-            if (!((ready >> j) & 1u))
-                continue;
-            int i = tid + j * nthreads;
+            unsigned int const item = atomicAdd(&s.taken, 1u);
+            if (item >= queued)
+                break;
+            int i = s.queue[item];
             while (true)
             {
                 int const lr = i / TW;
                 int const lc = i % TW;
                 std::uint8_t const mask = s.upmask[i];
+                int const next = s.next[i];
                 double upstream = 0.0;
                 bool poisoned = false;
                 for (int k = 0; k < 8; ++k)
@@ -461,8 +498,10 @@ namespace lue_amd {
                         continue;
                     int const ur = lr + neighbour_dr(k);
                     int const uc = lc + neighbour_dc(k);
+                    bool const inside = ur >= 0 && ur < TH && uc >= 0 && uc < TW;
                     double v;
-                    if (ur >= 0 && ur < TH && uc >= 0 && uc < TW && s.state[ur * TW + uc] == state_new)
+                    // in round 0 every upstream cell of a ready cell lies in the tile and was solved now
+                    if (inside && (round == 0 || s.state[ur * TW + uc] == state_new))
                     {
                         int const u = ur * TW + uc;
                         v = static_cast<double>(s.hi[u]);
@@ -491,19 +530,6 @@ namespace lue_amd {
                     s.lo[i] = isnan(out) ? 0.0f : static_cast<float>(out - static_cast<double>(hi));
                 if constexpr (two_outputs)
                     s.rem[i] = rem;
-
-                int const dir = ldd_direction(s.ldd[(lr + 1) * G::hw + (lc + 1)]);
-                int next = -1;
-                if (dir >= 0)
-                {
-                    int const dr = lr + neighbour_dr(dir);
-                    int const dc = lc + neighbour_dc(dir);
-                    if (dr >= 0 && dr < TH && dc >= 0 && dc < TW &&
-                        s.ldd[(dr + 1) * G::hw + (dc + 1)] != no_data_ldd)
-                    {
-                        next = dr * TW + dc;
-                    }
-                }
                 // publish: the state write is ordered before the counter update (release)
                 __hip_atomic_store(&s.state[i], state_new, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (next < 0)

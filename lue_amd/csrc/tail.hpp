@@ -28,6 +28,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cstdint>
 
 namespace lue_amd {
@@ -319,7 +320,36 @@ namespace lue_amd {
         return sum;
     }
 
-    // One wavefront per chain; 64-cell blocks.
+    // Inputs of one lane's cell in a 64-cell block of a chain.
+    template<int OP, typename T>
+    struct BlockLane
+    {
+        std::int64_t cell;  // local cell index in its catchment
+        bool valid;
+        bool bad;           // an upstream neighbour off the chain is no-data
+        double side;        // FP64 sum of the outflow of the upstream neighbours off the chain
+        CellInput<OP, T> in;
+    };
+
+    template<int OP, typename T>
+    __device__ inline BlockLane<OP, T> load_block_lane(SweepArgs<T> const& a, TailChains const& t, Catchment const& cat,
+                                                       CatchmentIO<T> const& io, float const* out_lo,
+                                                       std::uint32_t base, std::uint32_t length, std::uint32_t k)
+    {
+        BlockLane<OP, T> d{};
+        d.valid = k < length;
+        if (d.valid)
+        {
+            d.cell = static_cast<std::int64_t>(t.cells[base + k]) - cat.cell_base;
+            std::int64_t const prev = k > 0 ? static_cast<std::int64_t>(t.cells[base + k - 1]) - cat.cell_base : -1;
+            d.side = side_inflow<T>(cat, io, out_lo, d.cell, prev, d.bad);
+            stage_input<OP, T>(a, io, d.cell, d.in);
+        }
+        return d;
+    }
+
+    // One wavefront per chain, 64-cell blocks. The data of the next block is loaded while the current
+    // block runs its serial part.
     template<int OP, typename T>
     __global__ void __launch_bounds__(256) tail_execute_kernel(SweepArgs<T> a, TailChains t)
     {
@@ -328,6 +358,7 @@ namespace lue_amd {
         int const lane = threadIdx.x & 63;
         int chain = -1;
         unsigned long long solved = 0;
+        double const nan64 = __longlong_as_double(0x7ff8000000000000ll);
         while (true)
         {
             if (chain < 0)
@@ -348,83 +379,170 @@ namespace lue_amd {
             Catchment const cat = a.cats[ci];
             CatchmentIO<T> const io = a.io[ci];
             float* const out_lo = compensated ? a.out_lo + cat.cell_base : nullptr;
-            double carry = 0.0;        // outflow of the previous chain cell
-            double carry_delta = 0.0;  // kinematic wave FP32: correction of `carry` (first order)
+            double carry = 0.0;  // outflow of the previous chain cell (FP64)
+            float warm = 0.0f;   // kinematic wave FP32: an iterate of the previous cell, to start Newton from
+            float warm_pow = 0.0f;
+            bool have_warm = false;
+            BlockLane<OP, T> next_block = load_block_lane<OP, T>(a, t, cat, io, out_lo, base, length, lane);
             for (std::uint32_t block = 0; block < length; block += 64)
             {
-                std::uint32_t const k = block + lane;
-                bool const valid = k < length;
-                std::int64_t const cell = valid ? static_cast<std::int64_t>(t.cells[base + k]) - cat.cell_base : 0;
-                std::int64_t const prev =
-                    (valid && k > 0) ? static_cast<std::int64_t>(t.cells[base + k - 1]) - cat.cell_base : -1;
-                bool poisoned = false;
-                double const side = valid ? side_inflow<T>(cat, io, out_lo, cell, prev, poisoned) : 0.0;
-                CellInput<OP, T> in{};
-                if (valid)
-                    stage_input<OP, T>(a, io, cell, in);
+                BlockLane<OP, T> const d = next_block;
+                if (block + 64 < length)
+                    next_block = load_block_lane<OP, T>(a, t, cat, io, out_lo, base, length, block + 64 + lane);
                 int const nb = static_cast<int>(length - block < 64 ? length - block : 64);
-
                 double value = 0.0;  // this lane's cell outflow (FP64)
                 T rem{};
-                if constexpr (OP == op_kinematic_wave && compensated)
+
+                if constexpr (OP == op_accu)
                 {
-                    // serial FP32 solve along the block; FP64 correction afterwards, in parallel
-                    double q_up_mine = 0.0;
-                    float q32_mine = 0.0f;
+                    // linear: out_k = out_{k-1} + side_k + x_k, an inclusive scan along the block
+                    double term = (!d.valid) ? 0.0 : ((d.bad || isnan(d.in.x)) ? nan64 : d.side + static_cast<double>(d.in.x));
+                    for (int offset = 1; offset < 64; offset <<= 1)
+                    {
+                        double const prev = __shfl_up(term, offset, 64);
+                        if (lane >= offset)
+                            term += prev;
+                    }
+                    value = carry + term;
+                    carry = read_lane(value, nb - 1);
+                }
+                else if constexpr (OP == op_threshold || OP == op_fraction)
+                {
+                    double const term = (d.bad || isnan(d.in.x) || isnan(d.in.p)) ? nan64 : d.side + static_cast<double>(d.in.x);
                     for (int j = 0; j < nb; ++j)
                     {
-                        double const s = read_lane(side, j);
-                        int const bad = read_lane(static_cast<int>(poisoned), j);
-                        float const q_cur = read_lane(in.q_cur, j);
-                        double const q_up = carry + s;
-                        double q = carry;  // NaN stays NaN
-                        if (bad || isnan(q_up) || isnan(q_cur))
+                        double const total = carry + read_lane(term, j);
+                        double const p = static_cast<double>(read_lane(d.in.p, j));
+                        double out;
+                        double r;
+                        if constexpr (OP == op_threshold)
                         {
-                            q = __longlong_as_double(0x7ff8000000000000ll);
+                            bool const over = total > p;
+                            out = isnan(total) ? total : (over ? total - p : 0.0);
+                            r = over ? p : total;
                         }
                         else
                         {
-                            float const q_lat = read_lane(in.q_lat, j);
-                            float const alpha = read_lane(in.alpha, j);
-                            float const beta = read_lane(in.beta, j);
-                            double const b = read_lane(in.b, j);
-                            double const c1 = read_lane(in.c1, j);
+                            out = p * total;
+                            r = total - out;
+                        }
+                        if (lane == j)
+                        {
+                            value = out;
+                            rem = isnan(total) ? quiet_nan<T>() : static_cast<T>(r);
+                        }
+                        carry = out;
+                    }
+                }
+                else if constexpr (std::is_same_v<T, double>)
+                {
+                    // plain FP64 kinematic wave: LUE's Newton per cell along the block
+                    for (int j = 0; j < nb; ++j)
+                    {
+                        double const s = read_lane(d.side, j);
+                        int const bad = read_lane(static_cast<int>(d.bad), j);
+                        CellInput<OP, T> cj{read_lane(d.in.q_cur, j), read_lane(d.in.q_lat, j), read_lane(d.in.alpha, j),
+                                            read_lane(d.in.beta, j), read_lane(d.in.b, j), read_lane(d.in.c1, j)};
+                        double v;
+                        T r{};
+                        solve_cell<OP, T>(a, cj, carry + s, bad != 0 || isnan(carry), v, r);
+                        if (lane == j)
+                            value = v;
+                        carry = v;
+                    }
+                }
+                else
+                {
+                    // FP32 kinematic wave: serial FP32 Newton (warm start from the previous cell), then the
+                    // FP64 correction of every cell in parallel, propagated along the block to first order
+                    double q_up_mine = 0.0;
+                    float q32_mine = 0.0f;
+                    bool const uniform_beta = io.beta == nullptr;
+                    for (int j = 0; j < nb; ++j)
+                    {
+                        double const q_up = carry + read_lane(d.side, j);
+                        float const q_cur = read_lane(d.in.q_cur, j);
+                        int const bad = read_lane(static_cast<int>(d.bad), j);
+                        float q;
+                        if (bad || isnan(q_up) || isnan(q_cur))
+                        {
+                            q = __int_as_float(0x7fc00000);
+                            have_warm = false;
+                        }
+                        else
+                        {
+                            float const q_lat = read_lane(d.in.q_lat, j);
+                            float const alpha = read_lane(d.in.alpha, j);
+                            float const beta = read_lane(d.in.beta, j);
+                            double const b = read_lane(d.in.b, j);
+                            double const c1 = read_lane(d.in.c1, j);
                             if (!(q_up + q_cur > 0 || q_lat > 0))
                             {
-                                q = 0.0;
+                                q = 0.0f;
+                                have_warm = false;
                             }
                             else if (q_up + q_cur == 0.0 && beta < 1.0f)
                             {
-                                q = new_discharge_f32_path(q_up, q_cur, q_lat, alpha, beta, a.dt, b, c1);
+                                q = static_cast<float>(new_discharge_f32_path(q_up, q_cur, q_lat, alpha, beta, a.dt, b, c1));
+                                have_warm = false;
                             }
                             else
                             {
-                                bool converged;
+                                float const c1f = static_cast<float>(c1);
                                 float const known = static_cast<float>(c1 * q_up + b);
-                                q = static_cast<double>(solve_fast_f32(static_cast<float>(c1), alpha, beta, a.dt,
-                                                                      static_cast<float>(q_up), q_cur, q_lat, known,
-                                                                      converged));
+                                float x;
+                                float xb;
+                                if (have_warm && uniform_beta)
+                                {
+                                    x = warm;
+                                    xb = warm_pow;
+                                }
+                                else
+                                {
+                                    float const ab = alpha * beta;
+                                    float const a_b_pq = ab * fast_exp2((beta - 1.0f) * fast_log2(0.5f * (q_cur + static_cast<float>(q_up))));
+                                    x = (c1f * static_cast<float>(q_up) + a_b_pq * q_cur + a.dt * q_lat) * fast_rcp(c1f + a_b_pq);
+                                    x = fmaxf(x, FLT_MIN);
+                                    xb = fast_exp2(beta * fast_log2(x));
+                                }
+                                for (int iteration = 0; iteration < 10; ++iteration)
+                                {
+                                    float const f = fmaf(c1f, x, fmaf(alpha, xb, -known));
+                                    float const fp = fmaf(alpha * beta * xb, fast_rcp(x), c1f);
+                                    float const step = f * fast_rcp(fp);
+                                    float xn = x - step;
+                                    xn = xn > 0.0f ? xn : 0.5f * x;
+                                    bool const done = fabsf(step) <= fabsf(xn) * 0x1p-13f;
+                                    warm = x;  // an iterate whose power is known: the next cell's start
+                                    warm_pow = xb;
+                                    x = xn;
+                                    if (done)
+                                        break;
+                                    xb = fast_exp2(beta * fast_log2(x));
+                                }
+                                q = x;
+                                have_warm = true;
                             }
                         }
                         if (lane == j)
                         {
                             q_up_mine = q_up;
-                            q32_mine = static_cast<float>(q);
+                            q32_mine = q;
                         }
-                        carry = q;
+                        carry = static_cast<double>(q);
                     }
-                    // parallel FP64 correction: eps = G(q_up) - q, s = dG/dq_up (per lane)
+                    // FP64 correction in parallel: eps = G(q_up) - q, sens = dG/dq_up (per lane)
                     double eps = 0.0;
                     double sens = 0.0;
                     double const q = static_cast<double>(q32_mine);
-                    if (valid && q > 0.0 && !isnan(q))
+                    if (d.valid && q > 0.0 && !isnan(q))
                     {
-                        double const xb = dpow(q, static_cast<double>(in.beta));
-                        double const ax = static_cast<double>(in.alpha) * xb;
-                        double const f0 = in.c1 * q + ax - (in.c1 * q_up_mine + in.b);
-                        double const f1 = in.c1 + static_cast<double>(in.beta) * ax / q;
+                        double const xb = dpow(q, static_cast<double>(d.in.beta));
+                        double const ax = static_cast<double>(d.in.alpha) * xb;
+                        double const f0 = d.in.c1 * q + ax - (d.in.c1 * q_up_mine + d.in.b);
+                        double const f1 = d.in.c1 + static_cast<double>(d.in.beta) * ax / q;
                         eps = -f0 / f1;
-                        sens = in.c1 / f1;
+                        sens = d.in.c1 / f1;
                     }
                     // affine scan along the block: delta_k = eps_k + sens_k * delta_{k-1}
                     double mul = sens;
@@ -439,55 +557,21 @@ namespace lue_amd {
                             mul = mul * m_prev;
                         }
                     }
-                    double const delta = add + mul * carry_delta;
-                    value = isnan(q) ? q : (valid ? (q + delta > 0.0 ? q + delta : q) : 0.0);
+                    value = isnan(q) ? q : (d.valid ? (q + add > 0.0 ? q + add : q) : 0.0);
                     // the corrected last value starts the next block
                     carry = read_lane(value, nb - 1);
-                    carry_delta = 0.0;
                 }
-                else
-                {
-                    for (int j = 0; j < nb; ++j)
-                    {
-                        double const s = read_lane(side, j);
-                        int const bad = read_lane(static_cast<int>(poisoned), j);
-                        double v;
-                        T r{};
-                        if constexpr (OP == op_accu)
-                        {
-                            T const x = read_lane(in.x, j);
-                            CellInput<OP, T> cj{x};
-                            solve_cell<OP, T>(a, cj, carry + s, bad != 0 || isnan(carry), v, r);
-                        }
-                        else if constexpr (OP == op_threshold || OP == op_fraction)
-                        {
-                            CellInput<OP, T> cj{read_lane(in.x, j), read_lane(in.p, j)};
-                            solve_cell<OP, T>(a, cj, carry + s, bad != 0 || isnan(carry), v, r);
-                        }
-                        else
-                        {
-                            CellInput<OP, T> cj{read_lane(in.q_cur, j), read_lane(in.q_lat, j), read_lane(in.alpha, j),
-                                                read_lane(in.beta, j), read_lane(in.b, j), read_lane(in.c1, j)};
-                            solve_cell<OP, T>(a, cj, carry + s, bad != 0 || isnan(carry), v, r);
-                        }
-                        if (lane == j)
-                        {
-                            value = v;
-                            rem = r;
-                        }
-                        carry = v;
-                    }
-                }
-                if (valid)
+
+                if (d.valid)
                 {
                     T const hi = static_cast<T>(value);
-                    io.out[cell] = hi;
+                    io.out[d.cell] = hi;
                     if constexpr (compensated)
-                        out_lo[cell] = isnan(value) ? 0.0f : static_cast<float>(value - static_cast<double>(hi));
+                        out_lo[d.cell] = isnan(value) ? 0.0f : static_cast<float>(value - static_cast<double>(hi));
                     if constexpr (two_outputs)
-                        io.rem[cell] = rem;
+                        io.rem[d.cell] = rem;
                 }
-                solved += valid ? 1 : 0;
+                solved += d.valid ? 1 : 0;
             }
             // end of the chain: arrive at the confluence below; the last arrival continues there.
             // The release covers the stores of all lanes of the wavefront.
