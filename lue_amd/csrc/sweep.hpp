@@ -475,6 +475,49 @@ namespace lue_amd {
         }
         __syncthreads();
 
+        // Sum (FP64, neighbour order) of the outflow of the upstream neighbours of tile cell i: solved in
+        // this round (LDS) or earlier (HBM).
+        auto gather = [&](int i, double& upstream, bool& poisoned)
+        {
+            int const lr = i / TW;
+            int const lc = i % TW;
+            std::uint8_t const mask = s.upmask[i];
+            upstream = 0.0;
+            poisoned = false;
+            for (int k = 0; k < 8; ++k)
+            {
+                if (!((mask >> k) & 1u))
+                    continue;
+                int const ur = lr + neighbour_dr(k);
+                int const uc = lc + neighbour_dc(k);
+                bool const inside = ur >= 0 && ur < TH && uc >= 0 && uc < TW;
+                double v;
+                // in round 0 every upstream cell of a ready cell lies in the tile and was solved now
+                if (inside && (round == 0 || s.state[ur * TW + uc] == state_new))
+                {
+                    int const u = ur * TW + uc;
+                    v = static_cast<double>(s.hi[u]);
+                    if constexpr (compensated)
+                        v += static_cast<double>(s.lo[u]);
+                }
+                else
+                {
+                    std::int64_t const ug = (r0 + ur) * cols + (c0 + uc);
+                    v = static_cast<double>(io.out[ug]);
+                    if constexpr (compensated)
+                        v += static_cast<double>(out_lo[ug]);
+                }
+                if (isnan(v))
+                    poisoned = true;
+                else
+                    upstream += v;
+            }
+        };
+
+        // Kinematic wave FP32: the walks solve the FP32 roots only; the FP64 correction of every cell is
+        // computed afterwards in parallel and propagated along the flow paths of the tile (first order).
+        constexpr bool deferred = OP == op_kinematic_wave && compensated;
+
         // 4. walks: lanes take initially ready cells from the queue and walk downstream while they are the
         // last upstream contributor of the next cell
         unsigned int const queued = s.queued;
@@ -484,52 +527,45 @@ namespace lue_amd {
             if (item >= queued)
                 break;
             int i = s.queue[item];
+            bool have_warm = false;
+            float warm_x = 0.0f;
+            float warm_pow = 0.0f;
             while (true)
             {
-                int const lr = i / TW;
-                int const lc = i % TW;
-                std::uint8_t const mask = s.upmask[i];
                 int const next = s.next[i];
-                double upstream = 0.0;
-                bool poisoned = false;
-                for (int k = 0; k < 8; ++k)
+                double upstream;
+                bool poisoned;
+                gather(i, upstream, poisoned);
+                if constexpr (deferred)
                 {
-                    if (!((mask >> k) & 1u))
-                        continue;
-                    int const ur = lr + neighbour_dr(k);
-                    int const uc = lc + neighbour_dc(k);
-                    bool const inside = ur >= 0 && ur < TH && uc >= 0 && uc < TW;
-                    double v;
-                    // in round 0 every upstream cell of a ready cell lies in the tile and was solved now
-                    if (inside && (round == 0 || s.state[ur * TW + uc] == state_new))
+                    CellInput<OP, T> const& in = s.in[i];
+                    float q;
+                    if (poisoned || isnan(in.q_cur))
                     {
-                        int const u = ur * TW + uc;
-                        v = static_cast<double>(s.hi[u]);
-                        if constexpr (compensated)
-                            v += static_cast<double>(s.lo[u]);
+                        q = __int_as_float(0x7fc00000);
+                        have_warm = false;
                     }
                     else
                     {
-                        // solved in an earlier round, value in HBM
-                        std::int64_t const ug = (r0 + ur) * cols + (c0 + uc);
-                        v = static_cast<double>(io.out[ug]);
-                        if constexpr (compensated)
-                            v += static_cast<double>(out_lo[ug]);
+                        have_warm = have_warm && io.beta == nullptr;
+                        q = kinematic_wave_root_f32(static_cast<float>(in.c1), in.alpha, in.beta, a.dt, upstream,
+                                                    in.q_cur, in.q_lat, in.b, in.c1, have_warm, warm_x, warm_pow);
                     }
-                    if (isnan(v))
-                        poisoned = true;
-                    else
-                        upstream += v;
+                    s.hi[i] = q;
+                    s.lo[i] = 0.0f;
                 }
-                double out;
-                T rem{};
-                solve_cell<OP, T>(a, s.in[i], upstream, poisoned, out, rem);
-                T const hi = static_cast<T>(out);
-                s.hi[i] = hi;
-                if constexpr (compensated)
-                    s.lo[i] = isnan(out) ? 0.0f : static_cast<float>(out - static_cast<double>(hi));
-                if constexpr (two_outputs)
-                    s.rem[i] = rem;
+                else
+                {
+                    double out;
+                    T rem{};
+                    solve_cell<OP, T>(a, s.in[i], upstream, poisoned, out, rem);
+                    T const hi = static_cast<T>(out);
+                    s.hi[i] = hi;
+                    if constexpr (compensated)
+                        s.lo[i] = isnan(out) ? 0.0f : static_cast<float>(out - static_cast<double>(hi));
+                    if constexpr (two_outputs)
+                        s.rem[i] = rem;
+                }
                 // publish: the state write is ordered before the counter update (release)
                 __hip_atomic_store(&s.state[i], state_new, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (next < 0)
@@ -543,6 +579,121 @@ namespace lue_amd {
             }
         }
         __syncthreads();
+
+        if constexpr (deferred)
+        {
+            // 4b. per cell solved in this round: eps = one FP64 Newton step at the FP32 root, sens = d root /
+            // d upstream; stored in place of the no longer needed inputs (b <- eps, alpha <- sens). Count the
+            // upstream cells solved in this round for the propagation walk.
+            if (tid == 0)
+            {
+                s.queued = 0;
+                s.taken = 0;
+            }
+            __syncthreads();
+            for (int j = 0; j < cells_per_thread; ++j)
+            {
+                int const i = tid + j * nthreads;
+                bool is_ready = false;
+                if (i < G::n && s.state[i] == state_new)
+                {
+                    CellInput<OP, T>& in = s.in[i];
+                    double const q = static_cast<double>(s.hi[i]);
+                    double eps = 0.0;
+                    double sens = 0.0;
+                    if (q > 0.0 && !isnan(q))
+                    {
+                        double upstream;
+                        bool poisoned;
+                        gather(i, upstream, poisoned);
+                        kinematic_wave_correction(q, upstream, static_cast<double>(in.alpha), static_cast<double>(in.beta),
+                                                  in.b, in.c1, eps, sens);
+                    }
+                    in.b = eps;
+                    in.alpha = static_cast<float>(sens);
+                    int const lr = i / TW;
+                    int const lc = i % TW;
+                    std::uint8_t const mask = s.upmask[i];
+                    std::uint32_t n = 0;
+                    for (int k = 0; k < 8; ++k)
+                    {
+                        int const ur = lr + neighbour_dr(k);
+                        int const uc = lc + neighbour_dc(k);
+                        if (((mask >> k) & 1u) && ur >= 0 && ur < TH && uc >= 0 && uc < TW &&
+                            s.state[ur * TW + uc] == state_new)
+                            ++n;
+                    }
+                    reinterpret_cast<std::uint8_t*>(s.count)[i] = static_cast<std::uint8_t>(n);
+                    is_ready = n == 0;
+                }
+                std::uint64_t const ballot = __ballot(is_ready);
+                if (ballot)
+                {
+                    unsigned int base = 0;
+                    if (lane == 0)
+                        base = atomicAdd(&s.queued, static_cast<unsigned int>(__popcll(ballot)));
+                    base = __shfl(base, 0, 64);
+                    if (is_ready)
+                        s.queue[base + __popcll(ballot & lanes_below)] = static_cast<std::uint16_t>(i);
+                }
+            }
+            __syncthreads();
+            // 4c. propagate: delta = eps + sens * sum(delta of upstream cells solved in this round); delta is
+            // kept in place of c1
+            unsigned int const queued2 = s.queued;
+            while (true)
+            {
+                unsigned int const item = atomicAdd(&s.taken, 1u);
+                if (item >= queued2)
+                    break;
+                int i = s.queue[item];
+                while (true)
+                {
+                    int const lr = i / TW;
+                    int const lc = i % TW;
+                    std::uint8_t const mask = s.upmask[i];
+                    double sum = 0.0;
+                    for (int k = 0; k < 8; ++k)
+                    {
+                        int const ur = lr + neighbour_dr(k);
+                        int const uc = lc + neighbour_dc(k);
+                        if (((mask >> k) & 1u) && ur >= 0 && ur < TH && uc >= 0 && uc < TW &&
+                            s.state[ur * TW + uc] == state_new)
+                            sum += s.in[ur * TW + uc].c1;
+                    }
+                    CellInput<OP, T>& in = s.in[i];
+                    in.c1 = in.b + static_cast<double>(in.alpha) * sum;
+                    int next = s.next[i];
+                    if (next >= 0 && s.state[next] != state_new)
+                        next = -1;
+                    if (next < 0)
+                        break;
+                    std::uint32_t const shift = 8u * static_cast<std::uint32_t>(next & 3);
+                    std::uint32_t const old = __hip_atomic_fetch_sub(
+                        &s.count[next >> 2], 1u << shift, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (((old >> shift) & 0xFFu) != 1u)
+                        break;
+                    i = next;
+                }
+            }
+            __syncthreads();
+            // 4d. corrected values
+            for (int i = tid; i < G::n; i += nthreads)
+            {
+                if (s.state[i] != state_new)
+                    continue;
+                float const q = s.hi[i];
+                if (isnan(q))
+                    continue;
+                double v = static_cast<double>(q) + s.in[i].c1;
+                if (!(v > 0.0))
+                    v = static_cast<double>(q);
+                float const hi = static_cast<float>(v);
+                s.hi[i] = hi;
+                s.lo[i] = static_cast<float>(v - static_cast<double>(hi));
+            }
+            __syncthreads();
+        }
 
         // 5. write back: outputs of cells solved in this round (coalesced), compensation terms that later
         // rounds read, pending bitmap, border stamps, and queue the tiles that receive flow

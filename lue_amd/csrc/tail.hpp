@@ -476,53 +476,9 @@ namespace lue_amd {
                             float const beta = read_lane(d.in.beta, j);
                             double const b = read_lane(d.in.b, j);
                             double const c1 = read_lane(d.in.c1, j);
-                            if (!(q_up + q_cur > 0 || q_lat > 0))
-                            {
-                                q = 0.0f;
-                                have_warm = false;
-                            }
-                            else if (q_up + q_cur == 0.0 && beta < 1.0f)
-                            {
-                                q = static_cast<float>(new_discharge_f32_path(q_up, q_cur, q_lat, alpha, beta, a.dt, b, c1));
-                                have_warm = false;
-                            }
-                            else
-                            {
-                                float const c1f = static_cast<float>(c1);
-                                float const known = static_cast<float>(c1 * q_up + b);
-                                float x;
-                                float xb;
-                                if (have_warm && uniform_beta)
-                                {
-                                    x = warm;
-                                    xb = warm_pow;
-                                }
-                                else
-                                {
-                                    float const ab = alpha * beta;
-                                    float const a_b_pq = ab * fast_exp2((beta - 1.0f) * fast_log2(0.5f * (q_cur + static_cast<float>(q_up))));
-                                    x = (c1f * static_cast<float>(q_up) + a_b_pq * q_cur + a.dt * q_lat) * fast_rcp(c1f + a_b_pq);
-                                    x = fmaxf(x, FLT_MIN);
-                                    xb = fast_exp2(beta * fast_log2(x));
-                                }
-                                for (int iteration = 0; iteration < 10; ++iteration)
-                                {
-                                    float const f = fmaf(c1f, x, fmaf(alpha, xb, -known));
-                                    float const fp = fmaf(alpha * beta * xb, fast_rcp(x), c1f);
-                                    float const step = f * fast_rcp(fp);
-                                    float xn = x - step;
-                                    xn = xn > 0.0f ? xn : 0.5f * x;
-                                    bool const done = fabsf(step) <= fabsf(xn) * 0x1p-13f;
-                                    warm = x;  // an iterate whose power is known: the next cell's start
-                                    warm_pow = xb;
-                                    x = xn;
-                                    if (done)
-                                        break;
-                                    xb = fast_exp2(beta * fast_log2(x));
-                                }
-                                q = x;
-                                have_warm = true;
-                            }
+                            have_warm = have_warm && uniform_beta;
+                            q = kinematic_wave_root_f32(static_cast<float>(c1), alpha, beta, a.dt, q_up, q_cur, q_lat, b,
+                                                        c1, have_warm, warm, warm_pow);
                         }
                         if (lane == j)
                         {
@@ -537,12 +493,8 @@ namespace lue_amd {
                     double const q = static_cast<double>(q32_mine);
                     if (d.valid && q > 0.0 && !isnan(q))
                     {
-                        double const xb = dpow(q, static_cast<double>(d.in.beta));
-                        double const ax = static_cast<double>(d.in.alpha) * xb;
-                        double const f0 = d.in.c1 * q + ax - (d.in.c1 * q_up_mine + d.in.b);
-                        double const f1 = d.in.c1 + static_cast<double>(d.in.beta) * ax / q;
-                        eps = -f0 / f1;
-                        sens = d.in.c1 / f1;
+                        kinematic_wave_correction(q, q_up_mine, static_cast<double>(d.in.alpha),
+                                                  static_cast<double>(d.in.beta), d.in.b, d.in.c1, eps, sens);
                     }
                     // affine scan along the block: delta_k = eps_k + sens_k * delta_{k-1}
                     double mul = sens;
