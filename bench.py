@@ -42,9 +42,13 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--size", type=int, default=16384)
     p.add_argument("--fp64-steps", type=int, default=2)
-    p.add_argument("--cpu-sample", type=int, default=4096, help="side of the CPU-baseline sample (0: skip)")
+    p.add_argument("--cpu-seconds", type=float, default=10.0,
+                   help="minimum CPU-baseline time: the full workload is repeated until it is reached (0: skip)")
     p.add_argument("--cpu-threads", type=int, default=0)
-    p.add_argument("--tail-after", type=int, default=8, help="tile rounds before the chain tail (-1: none)")
+    p.add_argument("--accu-tile", default="64x32", help="sweep tile of the accu network (WxH)")
+    p.add_argument("--kw-tile", default="64x32", help="sweep tile of the kinematic-wave network (WxH)")
+    p.add_argument("--accu-tail-after", type=int, default=8, help="tile rounds before the chain tail (-1: none)")
+    p.add_argument("--kw-tail-after", type=int, default=2)
     return p.parse_args()
 
 
@@ -89,8 +93,10 @@ def main():
     # networks: the sweep schedule of the ldd, recorded once (tile shape per operation)
     log("[bench] recording flow networks")
     t0 = time.perf_counter()
-    net_accu = fw.FlowNetwork(ldd, tile_shape=(64, 64), tail_after=args.tail_after)
-    net_kw = fw.FlowNetwork(ldd, tile_shape=(32, 32), tail_after=args.tail_after)
+    accu_tile = tuple(int(v) for v in args.accu_tile.split("x"))
+    kw_tile = tuple(int(v) for v in args.kw_tile.split("x"))
+    net_accu = fw.FlowNetwork(ldd, tile_shape=accu_tile, tail_after=args.accu_tail_after)
+    net_kw = fw.FlowNetwork(ldd, tile_shape=kw_tile, tail_after=args.kw_tail_after)
     torch.cuda.synchronize()
     t_network = time.perf_counter() - t0
     s_accu = torch.cuda.Stream()
@@ -185,27 +191,30 @@ def main():
     dom_ms = kernel_ms[dominant]["first_round_ms"]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
 
-    # CPU baseline: the oracle on a bounded sample ------------------------------------------------------
+    # CPU baseline: the oracle on the same inputs, copied to the host ---------------------------------
     cpu = None
-    if args.cpu_sample > 0:
+    if args.cpu_seconds > 0:
         from oracle import oracle
 
-        m = args.cpu_sample
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"[bench] cpu baseline: oracle on {m}x{m}, {threads} threads")
-        ldd_h = syn.ldd_host(SEED, m, m)
-        mat = syn.uniform_host(SEED, syn.STREAM_MATERIAL, 0.0, 1.0, (m, m), np.float32)
-        qc = syn.uniform_host(SEED, syn.STREAM_CURRENT, 0.0, 5.0, (m, m), np.float32)
-        ql = syn.uniform_host(SEED, syn.STREAM_LATERAL, 0.0, 1e-4, (m, m), np.float32)
-        al = syn.uniform_host(SEED, syn.STREAM_ALPHA, 1.5, 2.5, (m, m), np.float32)
-        cps = (min(1024, m), min(1024, m))
+        log(f"[bench] cpu baseline: oracle on {n}x{n}, {threads} threads, >= {args.cpu_seconds} s")
+        host = {k: v.data.cpu().numpy() for k, v in f32.items()}
+        ldd_h = ldd.data.cpu().numpy()
+        reps = 0
         t0 = time.perf_counter()
-        oracle.accu(ldd_h, mat, cps, np.float32, threads)
-        oracle.kinematic_wave(ldd_h, qc, ql, al, 0.6, 3600.0, 100.0, cps, np.float32, threads)
-        tc = time.perf_counter() - t0
-        cpu = {"value": m * m / tc / 1e6, "unit": "Mcells/s", "cores": threads, "kind": "port",
-               "sample": f"accu + kinematic_wave, float, {m}x{m} synthetic DEM, {cps[0]}x{cps[1]} partitions, "
-                         f"LUE-algorithm CPU restatement (oracle/), {tc:.1f} s"}
+        while True:
+            oracle.accu(ldd_h, host["material"], ps, np.float32, threads)
+            oracle.kinematic_wave(ldd_h, host["q_cur"], host["q_lat"], host["alpha"], 0.6, 3600.0, 100.0, ps,
+                                  np.float32, threads)
+            reps += 1
+            tc = time.perf_counter() - t0
+            if tc >= args.cpu_seconds:
+                break
+        cpu = {"value": round(reps * cells / tc / 1e6, 2), "unit": "Mcells/s", "cores": threads, "kind": "port",
+               "sample": f"accu + kinematic_wave (float), the same {n}x{n} inputs and {ps[0]}x{ps[1]} partitions, "
+                         f"{reps} repetition(s) in {tc:.1f} s; LUE-algorithm CPU restatement (oracle/), "
+                         f"partition-parallel std::thread pool, host has {os.cpu_count()} logical CPUs"}
+        del host, ldd_h
 
     result = {
         "metric": "Mcells/s accu+kinwave @16384^2 (FP32 path, <=1e-5 rel vs FP64)",

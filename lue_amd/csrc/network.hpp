@@ -17,7 +17,10 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -115,6 +118,66 @@ namespace lue_amd {
         hipError_t const f = hipFreeAsync(temp, stream);
         return e != hipSuccess ? e : f;
     }
+
+    // Pinned host staging for small per-call tables (the per-catchment input/output pointers): a ring of
+    // slots per device. A slot is reused once the copy that read it has completed (an event recorded
+    // after the copy), so a call returns without waiting for the stream.
+    class HostStaging
+    {
+    public:
+        static constexpr int slot_count = 16;
+
+        // Copy `bytes` from `src` to the device buffer `dst` in stream order.
+        hipError_t copy_to_device(void* dst, void const* src, std::size_t bytes, hipStream_t stream)
+        {
+            int device = 0;
+            hipError_t e = hipGetDevice(&device);
+            if (e != hipSuccess)
+                return e;
+            std::lock_guard<std::mutex> lock(mutex_);
+            auto& ring = rings_[device];
+            Slot& s = ring.slots[ring.next];
+            ring.next = (ring.next + 1) % slot_count;
+            if (s.done)
+            {
+                if ((e = hipEventSynchronize(s.done)) != hipSuccess)
+                    return e;
+            }
+            else if ((e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming)) != hipSuccess)
+                return e;
+            if (s.capacity < bytes)
+            {
+                if (s.host)
+                    (void)hipHostFree(s.host);
+                s.host = nullptr;
+                s.capacity = 0;
+                std::size_t const cap = std::max<std::size_t>(bytes, 4096);
+                if ((e = hipHostMalloc(&s.host, cap, hipHostMallocDefault)) != hipSuccess)
+                    return e;
+                s.capacity = cap;
+            }
+            std::memcpy(s.host, src, bytes);
+            if ((e = hipMemcpyAsync(dst, s.host, bytes, hipMemcpyHostToDevice, stream)) != hipSuccess)
+                return e;
+            return hipEventRecord(s.done, stream);
+        }
+
+    private:
+        struct Slot
+        {
+            void* host{};
+            std::size_t capacity{};
+            hipEvent_t done{};
+        };
+        struct Ring
+        {
+            Slot slots[slot_count];
+            int next{0};
+        };
+        std::mutex mutex_;
+        std::map<int, Ring> rings_;
+    };
+    HostStaging& host_staging();
 
     inline unsigned blocks_for(std::uint64_t n, unsigned threads = 256)
     {
@@ -277,9 +340,7 @@ namespace lue_amd {
     int upload_io(Workspace& w, std::vector<CatchmentIO<T>> const& io, SweepArgs<T>& a, hipStream_t stream)
     {
         LUE_AMD_HIP(w.io.allocate(sizeof(CatchmentIO<T>) * io.size(), stream));
-        LUE_AMD_HIP(hipMemcpyAsync(w.io.ptr, io.data(), sizeof(CatchmentIO<T>) * io.size(), hipMemcpyHostToDevice,
-                                   stream));
-        LUE_AMD_HIP(hipStreamSynchronize(stream));
+        LUE_AMD_HIP(host_staging().copy_to_device(w.io.ptr, io.data(), sizeof(CatchmentIO<T>) * io.size(), stream));
         a.io = reinterpret_cast<CatchmentIO<T> const*>(w.io.ptr);
         return status_success;
     }

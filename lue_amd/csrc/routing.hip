@@ -33,6 +33,13 @@ namespace lue_amd {
         return p;
     }
 
+    HostStaging& host_staging()
+    {
+        // never destroyed: pinned slots and events stay valid until the process ends
+        static HostStaging* s = new HostStaging;
+        return *s;
+    }
+
     namespace {
 
         // Rounds of tiles before the chain-streaming tail takes over, for per-call entry points

@@ -52,6 +52,18 @@ namespace lue_amd {
         status_no_device = 6,
     };
 
+    // Pointer into device memory (address space 1). Accesses through it compile to global_* instructions.
+    // Through generic pointers the compiler may merge an LDS access and an HBM access of two branches
+    // into one flat access, which then waits on both the LDS and the vector-memory counters.
+    template<typename P>
+    using global_ptr = P __attribute__((address_space(1)))*;
+
+    template<typename P>
+    __device__ inline global_ptr<P> as_global(P* p)
+    {
+        return (global_ptr<P>)p;
+    }
+
     // Operation kinds of the accumulating router.
     enum Op : int { op_accu = 0, op_threshold = 1, op_fraction = 2, op_kinematic_wave = 3 };
 

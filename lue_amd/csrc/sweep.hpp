@@ -187,7 +187,7 @@ namespace lue_amd {
     template<typename T>
     __device__ inline T value_of(T const* raster, T scalar, std::int64_t idx)
     {
-        return raster ? raster[idx] : scalar;
+        return raster ? as_global(raster)[idx] : scalar;
     }
 
     template<typename T>
@@ -319,8 +319,12 @@ namespace lue_amd {
 
         int const tid = threadIdx.x;
         int const nthreads = blockDim.x;
-        Catchment const cat = a.cats[a.tile_cat[tile]];
-        CatchmentIO<T> const io = a.io[a.tile_cat[tile]];
+        std::uint32_t const cat_index = as_global(a.tile_cat)[tile];
+        Catchment const cat = as_global(a.cats)[cat_index];
+        CatchmentIO<T> const io = as_global(a.io)[cat_index];
+        auto const g_ldd = as_global(cat.ldd);
+        auto const g_out = as_global(io.out);
+        auto const g_rem = as_global(io.rem);
         std::int64_t const local = tile - cat.tile_base;
         int const ty = static_cast<int>(local / cat.tiles_x);
         int const tx = static_cast<int>(local % cat.tiles_x);
@@ -328,9 +332,9 @@ namespace lue_amd {
         std::int64_t const cols = cat.cols;
         std::int64_t const r0 = static_cast<std::int64_t>(ty) * TH;
         std::int64_t const c0 = static_cast<std::int64_t>(tx) * TW;
-        std::uint32_t* const pbits = a.pending_bits + tile * G::words;
-        std::uint32_t* const stamps = a.border_stamp + tile * G::perimeter;
-        float* const out_lo = compensated ? a.out_lo + cat.cell_base : nullptr;
+        auto const pbits = as_global(a.pending_bits + tile * G::words);
+        auto const stamps = as_global(a.border_stamp + tile * G::perimeter);
+        auto const out_lo = as_global(compensated ? a.out_lo + cat.cell_base : nullptr);
 
         if (tid == 0)
         {
@@ -347,7 +351,7 @@ namespace lue_amd {
             int const hc = i % G::hw;
             std::int64_t const r = r0 + hr - 1;
             std::int64_t const c = c0 + hc - 1;
-            s.ldd[i] = (r >= 0 && r < rows && c >= 0 && c < cols) ? cat.ldd[r * cols + c] : no_data_ldd;
+            s.ldd[i] = (r >= 0 && r < rows && c >= 0 && c < cols) ? g_ldd[r * cols + c] : no_data_ldd;
         }
         __syncthreads();
 
@@ -369,10 +373,10 @@ namespace lue_amd {
                 {
                     // mark_no_data (accumulating_router.hpp:866-872)
                     std::int64_t const g = r * cols + c;
-                    io.out[g] = quiet_nan<T>();
+                    g_out[g] = quiet_nan<T>();
                     if constexpr (two_outputs)
                     {
-                        io.rem[g] = quiet_nan<T>();
+                        g_rem[g] = quiet_nan<T>();
                     }
                     atomicAdd(&s.no_data, 1u);
                 }
@@ -453,7 +457,7 @@ namespace lue_amd {
                             std::int64_t const ntile =
                                 cat.tile_base + static_cast<std::int64_t>(nty) * cat.tiles_x + ntx;
                             std::uint32_t const st =
-                                a.border_stamp[ntile * G::perimeter + G::perimeter_index(nlr, nlc)];
+                                as_global(a.border_stamp)[ntile * G::perimeter + G::perimeter_index(nlr, nlc)];
                             n += st < round ? 0u : 1u;
                         }
                     }
@@ -503,7 +507,7 @@ namespace lue_amd {
                 else
                 {
                     std::int64_t const ug = (r0 + ur) * cols + (c0 + uc);
-                    v = static_cast<double>(io.out[ug]);
+                    v = static_cast<double>(g_out[ug]);
                     if constexpr (compensated)
                         v += static_cast<double>(out_lo[ug]);
                 }
@@ -711,9 +715,9 @@ namespace lue_amd {
             {
                 ++solved;
                 std::int64_t const g = r * cols + c;
-                io.out[g] = s.hi[i];
+                g_out[g] = s.hi[i];
                 if constexpr (two_outputs)
-                    io.rem[g] = s.rem[i];
+                    g_rem[g] = s.rem[i];
                 int const dir = ldd_direction(s.ldd[(lr + 1) * G::hw + (lc + 1)]);
                 if (dir >= 0)
                 {
