@@ -320,8 +320,8 @@ namespace lue_amd {
         int const tid = threadIdx.x;
         int const nthreads = blockDim.x;
         std::uint32_t const cat_index = as_global(a.tile_cat)[tile];
-        Catchment const cat = as_global(a.cats)[cat_index];
-        CatchmentIO<T> const io = as_global(a.io)[cat_index];
+        Catchment const cat = a.cats[cat_index];
+        CatchmentIO<T> const io = a.io[cat_index];
         auto const g_ldd = as_global(cat.ldd);
         auto const g_out = as_global(io.out);
         auto const g_rem = as_global(io.rem);
