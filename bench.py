@@ -45,10 +45,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="minimum CPU-baseline time: the full workload is repeated until it is reached (0: skip)")
     p.add_argument("--cpu-threads", type=int, default=0)
-    p.add_argument("--accu-tile", default="64x32", help="sweep tile of the accu network (WxH)")
-    p.add_argument("--kw-tile", default="64x32", help="sweep tile of the kinematic-wave network (WxH)")
-    p.add_argument("--accu-tail-after", type=int, default=8, help="tile rounds before the chain tail (-1: none)")
-    p.add_argument("--kw-tail-after", type=int, default=2)
+    p.add_argument("--accu-tile", default="32x32", help="sweep tile of the accu network (WxH)")
+    p.add_argument("--kw-tile", default="32x32", help="sweep tile of the kinematic-wave network (WxH)")
+    p.add_argument("--schedule", default="chains", choices=["chains", "rounds"])
     return p.parse_args()
 
 
@@ -95,8 +94,8 @@ def main():
     t0 = time.perf_counter()
     accu_tile = tuple(int(v) for v in args.accu_tile.split("x"))
     kw_tile = tuple(int(v) for v in args.kw_tile.split("x"))
-    net_accu = fw.FlowNetwork(ldd, tile_shape=accu_tile, tail_after=args.accu_tail_after)
-    net_kw = fw.FlowNetwork(ldd, tile_shape=kw_tile, tail_after=args.kw_tail_after)
+    net_accu = fw.FlowNetwork(ldd, tile_shape=accu_tile, schedule=args.schedule)
+    net_kw = fw.FlowNetwork(ldd, tile_shape=kw_tile, schedule=args.schedule)
     torch.cuda.synchronize()
     t_network = time.perf_counter() - t0
     s_accu = torch.cuda.Stream()
@@ -143,7 +142,7 @@ def main():
         torch.cuda.synchronize()
         first, rest, calls, rounds, _ = _lib.profile_read()
         _lib.profile_enable(False)
-        kernel_ms[name] = dict(first_round_ms=first / calls, later_rounds_and_tail_ms=rest / calls,
+        kernel_ms[name] = dict(first_round_ms=first / calls, after_first_round_ms=rest / calls,
                                total_ms=(first + rest) / calls, rounds=rounds / calls)
 
     # drop-in per-call path (records its schedule in every call)

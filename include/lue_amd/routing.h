@@ -56,11 +56,16 @@ int lue_amd_profile_read(double* first_round_ms, double* next_rounds_ms, long lo
                          long long* cells);
 
 /*
- * Tuning of the per-call entry points (no reference counterpart): number of tile rounds before the
- * remaining cells are solved by chain streaming; -1: tile rounds only. Default 8. Results do not depend
- * on it beyond floating point summation order.
+ * Schedules of the routing sweep after its first round over all tiles (no reference counterpart):
+ * LUE_AMD_SCHEDULE_CHAINS (default): the cells the first round leaves pending are solved along heavy-path
+ * chains, one launch per chain level; LUE_AMD_SCHEDULE_ROUNDS: further rounds over the tiles that received
+ * flow until none is left. Results do not depend on the schedule beyond floating point summation order.
  */
-int lue_amd_set_sweep_options(int tail_after);
+#define LUE_AMD_SCHEDULE_CHAINS 0
+#define LUE_AMD_SCHEDULE_ROUNDS 1
+
+/* Schedule of the per-call entry points (default LUE_AMD_SCHEDULE_CHAINS). */
+int lue_amd_set_sweep_options(int schedule);
 
 /* Device memory used by the C++ front end's device-resident partitions. */
 int lue_amd_device_malloc(void** ptr, size_t bytes);
@@ -160,22 +165,22 @@ int lue_amd_synthetic_uniform_f64(uint64_t seed, uint64_t stream_id, double lo, 
  * Flow networks (no reference counterpart; LUE recomputes the inflow counts of an ldd in every call).
  * A network is a batch of n catchments (independent ldd rasters, device pointers, kept borrowed: they
  * must outlive the network and must not change) together with the schedule of the routing sweep, which
- * depends on the ldd only: the tiles of each round and the chains of the tail. Operations on a network
+ * depends on the ldd only (the chain layout, or the tiles of each round). Operations on a network
  * run that schedule as a fixed sequence of launches on `stream`, for all catchments at once, and return
  * without waiting for the results (synchronise the stream before reading them).
  * tile_shape: 0 = 32x32, 1 = 64x32, 2 = 64x64 cells per tile (operations that do not fit the shape in
- * LDS return LUE_AMD_ERROR_INVALID_ARGUMENT). tail_after: tile rounds before the chain-streaming tail,
- * -1 for tile rounds only. Creating a network validates it (cycles: LUE_AMD_ERROR_INVALID_NETWORK).
+ * LDS return LUE_AMD_ERROR_INVALID_ARGUMENT). schedule: LUE_AMD_SCHEDULE_*. Creating a network validates
+ * it (cycles: LUE_AMD_ERROR_INVALID_NETWORK).
  * The arguments of the operations are host arrays with one device pointer per catchment; an argument
  * array may be NULL where the operation accepts a Scalar, which then applies to every catchment.
  */
 typedef struct lue_amd_network_s* lue_amd_network;
 
 int lue_amd_network_create(int n, const uint8_t* const* ldd, const int64_t* rows, const int64_t* cols, int tile_shape,
-                           int tail_after, void* stream, lue_amd_network* network);
+                           int schedule, void* stream, lue_amd_network* network);
 int lue_amd_network_destroy(lue_amd_network network);
-/* info[0..9]: catchments, cells, tiles, recorded rounds, tail cells, chains, ready chains, no-data cells,
- * tile shape, tail_after */
+/* info[0..11]: catchments, cells, tiles, recorded rounds, cells left pending by the first round, chains,
+ * chain levels, no-data cells, tile shape, schedule, longest chain, side references */
 int lue_amd_network_info(lue_amd_network network, int64_t* info, int n);
 
 int lue_amd_network_accu_f32(lue_amd_network network, const float* const* inflow, float inflow_scalar,

@@ -98,9 +98,12 @@ def load():
     return lib
 
 
-def set_sweep_options(tail_after=8):
-    """Tile rounds before the chain-streaming tail in per-call operations; -1: tile rounds only."""
-    check(load().lue_amd_set_sweep_options(tail_after))
+SCHEDULES = {"chains": 0, "rounds": 1}
+
+
+def set_sweep_options(schedule="chains"):
+    """Schedule of the per-call operations after the first round of tiles: "chains" or "rounds"."""
+    check(load().lue_amd_set_sweep_options(SCHEDULES[schedule]))
 
 
 def profile_enable(enable=True):

@@ -1,0 +1,783 @@
+// Chain phase of the routing sweep: the cells that round 0 of the tiled sweep leaves pending.
+//
+// Round 0 (sweep.hpp) solves every cell whose upstream area lies inside its own tile. What is left,
+// the pending cells, are the cells downstream of a tile border crossing: a forest (each pending cell
+// has at most one pending downstream cell) whose long paths are the main streams. They are solved by
+// a heavy-path decomposition of that forest, recorded once per flow network:
+//   1. compact the pending cells (tile after tile), link each to its pending downstream cell;
+//   2. Euler tour of the forest (children in neighbour order) and list ranking by pointer jumping give
+//      every pending cell the number of pending cells upstream of it (its subtree);
+//   3. the heavy child of a cell is its pending upstream neighbour with the largest subtree; a second
+//      Euler tour with heavy children first numbers the cells in preorder, in which every heavy path
+//      (a chain) is one contiguous run;
+//   4. the level of a chain is one more than the highest level of the chains that join it from the
+//      side (light children); 0 when none do. Any path crosses at most log2(n) light edges, so there
+//      are few levels. Chains are laid out level after level, each in flow order (upstream first);
+//   5. per chain cell, the references to the values it receives from the side: light children (their
+//      chain positions) and upstream cells in other tiles that round 0 solved (raster cells).
+// Running an operation: round 0 solves the non-pending cells and stages the inputs of the pending ones
+// at their chain positions, together with the FP64 sum of their upstream cells in the same tile that
+// round 0 solved. Then one launch per level: a wavefront takes a work item (one long chain, or several
+// short consecutive chains of at most 64 cells), 64 positions at a time, and runs the cell function
+// along the chains (scans where the function allows it, serially with lane broadcasts where it does
+// not). A final pass writes the chain results to the output rasters. Levels only exchange data across
+// launches, so no kernel synchronises workgroups.
+#pragma once
+
+#include "kinematic_wave.hpp"
+#include "routing_common.hpp"
+#include "sweep.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lue_amd {
+
+    // A side reference: bit 63 set -> chain position in the low 32 bits; else catchment in bits 32-62 and
+    // the local cell index in the low 32 bits.
+    constexpr std::uint64_t ref_position_tag = 1ull << 63;
+
+    template<int TW, int TH>
+    __device__ inline bool cell_pending(std::uint32_t const* pending_bits, Catchment const& cat, std::int64_t r,
+                                        std::int64_t c)
+    {
+        using G = Tile<TW, TH>;
+        std::int64_t const tile = cat.tile_base + (r / TH) * cat.tiles_x + (c / TW);
+        int const i = static_cast<int>((r % TH) * TW + (c % TW));
+        return (as_global(pending_bits)[tile * G::words + (i >> 5)] >> (i & 31)) & 1u;
+    }
+
+    // Catchment of a global cell id (cell_base is increasing).
+    __device__ inline int find_catchment(Catchment const* cats, int ncats, std::uint64_t cell)
+    {
+        int lo = 0;
+        int hi = ncats - 1;
+        while (lo < hi)
+        {
+            int const mid = (lo + hi + 1) / 2;
+            if (static_cast<std::uint64_t>(cats[mid].cell_base) <= cell)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        return lo;
+    }
+
+    // Pending upstream neighbours (local cell indices) of local cell `local`, in neighbour order.
+    template<int TW, int TH>
+    __device__ inline int pending_upstream(Catchment const& cat, std::uint32_t const* pending_bits,
+                                           std::int64_t local, std::int64_t (&up)[8], int (&dir)[8])
+    {
+        std::int64_t const r = local / cat.cols;
+        std::int64_t const c = local % cat.cols;
+        auto const ldd = as_global(cat.ldd);
+        int n = 0;
+        for (int k = 0; k < 8; ++k)
+        {
+            std::int64_t const r2 = r + neighbour_dr(k);
+            std::int64_t const c2 = c + neighbour_dc(k);
+            if (r2 < 0 || r2 >= cat.rows || c2 < 0 || c2 >= cat.cols)
+                continue;
+            std::uint8_t const code = ldd[r2 * cat.cols + c2];
+            if (code == no_data_ldd || ldd_direction(code) != opposite(k))
+                continue;
+            if (!cell_pending<TW, TH>(pending_bits, cat, r2, c2))
+                continue;
+            up[n] = r2 * cat.cols + c2;
+            dir[n] = k;
+            ++n;
+        }
+        return n;
+    }
+
+    // ---------------------------------------------------------------------------------------------------
+    // Recording
+
+    // Pending cells per tile.
+    template<int TW, int TH>
+    __global__ void chains_count_kernel(std::uint32_t const* pending_bits, std::int64_t ntiles, std::uint32_t* counts)
+    {
+        using G = Tile<TW, TH>;
+        std::int64_t const tile = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+        if (tile >= ntiles)
+            return;
+        std::uint32_t n = 0;
+        for (int w = 0; w < G::words; ++w)
+            n += __popc(pending_bits[tile * G::words + w]);
+        counts[tile] = n;
+    }
+
+    // List the pending cells of each tile at its offset (cell order inside the tile); record global cell
+    // id -> list index.
+    template<int TW, int TH>
+    __global__ void chains_compact_kernel(std::uint32_t const* pending_bits, Catchment const* cats,
+                                          std::uint32_t const* tile_cat, std::uint32_t const* offsets,
+                                          std::uint32_t* cells, std::uint32_t* index_of)
+    {
+        using G = Tile<TW, TH>;
+        std::int64_t const tile = blockIdx.x;
+        Catchment const cat = cats[tile_cat[tile]];
+        std::int64_t const local = tile - cat.tile_base;
+        int const ty = static_cast<int>(local / cat.tiles_x);
+        int const tx = static_cast<int>(local % cat.tiles_x);
+        std::uint32_t const* bits = pending_bits + tile * G::words;
+        for (int w = threadIdx.x; w < G::words; w += blockDim.x)
+        {
+            std::uint32_t word = bits[w];
+            if (!word)
+                continue;
+            std::uint32_t pos = offsets[tile];
+            for (int v = 0; v < w; ++v)
+                pos += __popc(bits[v]);
+            while (word)
+            {
+                int const b = __ffs(word) - 1;
+                word &= word - 1;
+                int const i = w * 32 + b;
+                std::int64_t const r = static_cast<std::int64_t>(ty) * TH + i / TW;
+                std::int64_t const c = static_cast<std::int64_t>(tx) * TW + i % TW;
+                std::uint32_t const cell = static_cast<std::uint32_t>(cat.cell_base + r * cat.cols + c);
+                cells[pos] = cell;
+                index_of[cell] = pos;
+                ++pos;
+            }
+        }
+    }
+
+    // Parent (pending downstream cell) of every pending cell, -1 at the end of a pending path.
+    template<int TW, int TH>
+    __global__ void chains_parent_kernel(Catchment const* cats, int ncats, std::uint32_t const* pending_bits,
+                                         std::uint32_t const* cells, std::uint32_t const* index_of, std::uint32_t n,
+                                         std::int32_t* parent)
+    {
+        std::uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
+        if (q >= n)
+            return;
+        std::uint32_t const cell = cells[q];
+        Catchment const cat = cats[find_catchment(cats, ncats, cell)];
+        std::int64_t const local = cell - cat.cell_base;
+        std::int64_t const r = local / cat.cols;
+        std::int64_t const c = local % cat.cols;
+        int const dir = ldd_direction(cat.ldd[local]);
+        std::int32_t p = -1;
+        if (dir >= 0)
+        {
+            std::int64_t const r2 = r + neighbour_dr(dir);
+            std::int64_t const c2 = c + neighbour_dc(dir);
+            if (r2 >= 0 && r2 < cat.rows && c2 >= 0 && c2 < cat.cols && cat.ldd[r2 * cat.cols + c2] != no_data_ldd &&
+                cell_pending<TW, TH>(pending_bits, cat, r2, c2))
+            {
+                p = static_cast<std::int32_t>(index_of[cat.cell_base + r2 * cat.cols + c2]);
+            }
+        }
+        parent[q] = p;
+    }
+
+    // Children of pending cell q in tour order: the heavy child first (when given), then the others in
+    // neighbour order. Returns the count.
+    template<int TW, int TH>
+    __device__ inline int tour_children(Catchment const& cat, std::uint32_t const* pending_bits,
+                                        std::uint32_t const* index_of, std::int64_t local, std::int32_t heavy,
+                                        std::int32_t (&child)[8])
+    {
+        std::int64_t up[8];
+        int dir[8];
+        int const m = pending_upstream<TW, TH>(cat, pending_bits, local, up, dir);
+        int n = 0;
+        if (heavy >= 0)
+            child[n++] = heavy;
+        for (int j = 0; j < m; ++j)
+        {
+            std::int32_t const q = static_cast<std::int32_t>(as_global(index_of)[cat.cell_base + up[j]]);
+            if (q != heavy)
+                child[n++] = q;
+        }
+        return n;
+    }
+
+    // Euler tour successor of every tour element: enter(q) = 2q, exit(q) = 2q + 1.
+    // enter(q) -> enter(first child) or exit(q); exit(q) -> enter(next sibling) or exit(parent) or end (-1).
+    template<int TW, int TH>
+    __global__ void chains_tour_kernel(Catchment const* cats, int ncats, std::uint32_t const* pending_bits,
+                                       std::uint32_t const* cells, std::uint32_t const* index_of,
+                                       std::int32_t const* parent, std::int32_t const* heavy, std::uint32_t n,
+                                       std::int32_t* succ)
+    {
+        std::uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
+        if (q >= n)
+            return;
+        std::uint32_t const cell = cells[q];
+        Catchment const cat = cats[find_catchment(cats, ncats, cell)];
+        std::int32_t child[8];
+        int const nc = tour_children<TW, TH>(cat, pending_bits, index_of, cell - cat.cell_base,
+                                             heavy ? heavy[q] : -1, child);
+        succ[2 * q] = nc > 0 ? 2 * child[0] : static_cast<std::int32_t>(2 * q + 1);
+        std::int32_t const d = parent[q];
+        std::int32_t s = -1;
+        if (d >= 0)
+        {
+            std::int32_t sib[8];
+            std::uint32_t const dcell = cells[d];
+            int const ns = tour_children<TW, TH>(cat, pending_bits, index_of, dcell - cat.cell_base,
+                                                 heavy ? heavy[d] : -1, sib);
+            s = 2 * d + 1;
+            for (int j = 0; j + 1 < ns; ++j)
+            {
+                if (sib[j] == static_cast<std::int32_t>(q))
+                {
+                    s = 2 * sib[j + 1];
+                    break;
+                }
+            }
+        }
+        succ[2 * q + 1] = s;
+    }
+
+    // List ranking: value = number of enter elements from the element to the end of its list (inclusive),
+    // last = the list's final element.
+    __global__ inline void chains_rank_init_kernel(std::int32_t const* succ, std::uint32_t n2, std::int32_t* next,
+                                                   std::int32_t* value, std::int32_t* last)
+    {
+        std::uint32_t const e = blockIdx.x * blockDim.x + threadIdx.x;
+        if (e >= n2)
+            return;
+        std::int32_t const s = succ[e];
+        next[e] = s;
+        value[e] = (e & 1u) == 0 ? 1 : 0;
+        last[e] = s < 0 ? static_cast<std::int32_t>(e) : s;
+    }
+
+    // One pointer-jumping step (double buffered); counts the elements still linked.
+    __global__ inline void chains_rank_step_kernel(std::int32_t const* next, std::int32_t const* value,
+                                                   std::int32_t const* last, std::uint32_t n2, std::int32_t* next2,
+                                                   std::int32_t* value2, std::int32_t* last2, std::uint32_t* linked)
+    {
+        std::uint32_t const e = blockIdx.x * blockDim.x + threadIdx.x;
+        bool still = false;
+        if (e < n2)
+        {
+            std::int32_t const s = next[e];
+            if (s >= 0)
+            {
+                std::int32_t const s2 = next[s];
+                next2[e] = s2;
+                value2[e] = value[e] + value[s];
+                last2[e] = last[s];
+                still = s2 >= 0;
+            }
+            else
+            {
+                next2[e] = -1;
+                value2[e] = value[e];
+                last2[e] = last[e];
+            }
+        }
+        std::uint64_t const ballot = __ballot(still);
+        if ((threadIdx.x & 63) == 0 && ballot)
+            atomicAdd(linked, static_cast<std::uint32_t>(__popcll(ballot)));
+    }
+
+    // Counts entries of `perm` that are out of range or repeated (perm must be a permutation of [0, n)).
+    __global__ inline void chains_check_permutation_kernel(std::uint32_t const* perm, std::uint32_t n,
+                                                           std::uint32_t* seen, std::uint32_t* bad)
+    {
+        std::uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
+        if (q >= n)
+            return;
+        std::uint32_t const p = perm[q];
+        if (p >= n || atomicAdd(&seen[p], 1u) != 0u)
+            atomicAdd(bad, 1u);
+    }
+
+    // Heavy child: the child with the largest subtree (subtree = value(enter) - value(exit)); ties go to
+    // the first in neighbour order.
+    template<int TW, int TH>
+    __global__ void chains_heavy_kernel(Catchment const* cats, int ncats, std::uint32_t const* pending_bits,
+                                        std::uint32_t const* cells, std::uint32_t const* index_of,
+                                        std::int32_t const* value, std::uint32_t n, std::int32_t* heavy)
+    {
+        std::uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
+        if (q >= n)
+            return;
+        std::uint32_t const cell = cells[q];
+        Catchment const cat = cats[find_catchment(cats, ncats, cell)];
+        std::int32_t child[8];
+        int const nc = tour_children<TW, TH>(cat, pending_bits, index_of, cell - cat.cell_base, -1, child);
+        std::int32_t best = -1;
+        std::int32_t best_size = -1;
+        for (int j = 0; j < nc; ++j)
+        {
+            std::int32_t const c = child[j];
+            std::int32_t const size = value[2 * c] - value[2 * c + 1];
+            if (size > best_size)
+            {
+                best = c;
+                best_size = size;
+            }
+        }
+        heavy[q] = best;
+    }
+
+    __global__ inline void chains_root_flag_kernel(std::int32_t const* parent, std::uint32_t n, std::uint32_t* flag)
+    {
+        std::uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
+        if (q < n)
+            flag[q] = parent[q] < 0 ? 1u : 0u;
+    }
+
+    __global__ inline void chains_tree_size_kernel(std::int32_t const* parent, std::int32_t const* value,
+                                                   std::uint32_t const* root_index, std::uint32_t n,
+                                                   std::uint32_t* tree_size)
+    {
+        std::uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
+        if (q < n && parent[q] < 0)
+            tree_size[root_index[q]] = static_cast<std::uint32_t>(value[2 * q]);
+    }
+
+    // Preorder number (heavy children first) of every pending cell, and head flags by preorder.
+    __global__ inline void chains_preorder_kernel(std::int32_t const* parent, std::int32_t const* heavy,
+                                                  std::int32_t const* value, std::int32_t const* last,
+                                                  std::uint32_t const* root_index, std::uint32_t const* tree_base,
+                                                  std::uint32_t n, std::uint32_t* pre, std::uint32_t* head_flag)
+    {
+        std::uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
+        if (q >= n)
+            return;
+        std::int32_t const end = last[2 * q];  // exit(root)
+        std::uint32_t const root = static_cast<std::uint32_t>(end) >> 1;
+        std::uint32_t const size = static_cast<std::uint32_t>(value[2 * root]);
+        std::uint32_t const p = tree_base[root_index[root]] + size - static_cast<std::uint32_t>(value[2 * q]);
+        pre[q] = p;
+        std::int32_t const d = parent[q];
+        if (p < n)
+            head_flag[p] = (d < 0 || heavy[d] != static_cast<std::int32_t>(q)) ? 1u : 0u;
+    }
+
+    // Chains are numbered by preorder of their heads: chain_pre[c] = preorder of head of chain c.
+    __global__ inline void chains_head_kernel(std::uint32_t const* head_flag, std::uint32_t const* chain_incl,
+                                              std::uint32_t n, std::uint32_t* chain_pre)
+    {
+        std::uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+        if (p < n && head_flag[p])
+            chain_pre[chain_incl[p] - 1] = p;
+    }
+
+    __global__ inline void chains_length_kernel(std::uint32_t const* chain_pre, std::uint32_t nchains, std::uint32_t n,
+                                                std::uint32_t* length)
+    {
+        std::uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
+        if (c < nchains)
+            length[c] = (c + 1 < nchains ? chain_pre[c + 1] : n) - chain_pre[c];
+    }
+
+    // One relaxation of the levels: level(chain of parent(h)) >= level(chain of h) + 1 for every head h
+    // with a parent.
+    __global__ inline void chains_level_kernel(std::int32_t const* parent, std::uint32_t const* pre,
+                                               std::uint32_t const* head_flag, std::uint32_t const* chain_incl,
+                                               std::uint32_t n, std::uint32_t* level, std::uint32_t* changed)
+    {
+        std::uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
+        if (q >= n)
+            return;
+        std::uint32_t const p = pre[q];
+        std::int32_t const d = parent[q];
+        if (!head_flag[p] || d < 0)
+            return;
+        std::uint32_t const c = chain_incl[p] - 1;
+        std::uint32_t const cd = chain_incl[pre[d]] - 1;
+        std::uint32_t const want = level[c] + 1;
+        if (level[cd] < want)
+        {
+            if (atomicMax(&level[cd], want) < want)
+                atomicAdd(changed, 1u);
+        }
+    }
+
+    // Lengths of the chains of one level (others 0), for the per-level exclusive scan.
+    __global__ inline void chains_level_mask_kernel(std::uint32_t const* level, std::uint32_t const* length,
+                                                    std::uint32_t nchains, std::uint32_t which, std::uint32_t* out)
+    {
+        std::uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
+        if (c < nchains)
+            out[c] = level[c] == which ? length[c] : 0u;
+    }
+
+    __global__ inline void chains_level_base_kernel(std::uint32_t const* level, std::uint32_t const* scan,
+                                                    std::uint32_t nchains, std::uint32_t which, std::uint32_t offset,
+                                                    std::uint32_t* base)
+    {
+        std::uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
+        if (c < nchains && level[c] == which)
+            base[c] = offset + scan[c];
+    }
+
+    // Position of every pending cell: chains in flow order (the most upstream cell first).
+    __global__ inline void chains_position_kernel(std::uint32_t const* pre, std::uint32_t const* chain_incl,
+                                                  std::uint32_t const* chain_pre, std::uint32_t const* length,
+                                                  std::uint32_t const* base, std::uint32_t n, std::uint32_t* pos_of,
+                                                  std::uint8_t* seg_start)
+    {
+        std::uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
+        if (q >= n)
+            return;
+        std::uint32_t const p = pre[q];
+        std::uint32_t const c = chain_incl[p] - 1;
+        std::uint32_t const offset = p - chain_pre[c];
+        std::uint32_t const at = base[c] + length[c] - 1 - offset;
+        pos_of[q] = at;
+        if (at < n)
+            seg_start[at] = offset + 1 == length[c] ? 1 : 0;
+    }
+
+    // Side references of every pending cell, in neighbour order: pending upstream cells other than the
+    // heavy child (chain positions) and upstream cells in other tiles that round 0 solves (raster cells).
+    // Pass 1 counts (fill == false), pass 2 writes.
+    template<int TW, int TH>
+    __global__ void chains_refs_kernel(Catchment const* cats, int ncats, std::uint32_t const* pending_bits,
+                                       std::uint32_t const* cells, std::uint32_t const* index_of,
+                                       std::int32_t const* heavy, std::uint32_t const* pos_of, std::uint32_t n,
+                                       bool fill, std::uint32_t* count, std::uint32_t const* offset,
+                                       std::uint64_t* refs)
+    {
+        std::uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
+        if (q >= n)
+            return;
+        std::uint32_t const cell = cells[q];
+        int const ci = find_catchment(cats, ncats, cell);
+        Catchment const cat = cats[ci];
+        std::int64_t const local = cell - cat.cell_base;
+        std::int64_t const r = local / cat.cols;
+        std::int64_t const c = local % cat.cols;
+        std::int32_t const h = heavy[q];
+        std::uint32_t const p = pos_of[q];
+        std::uint32_t k_out = fill ? offset[p] : 0u;
+        std::uint32_t m = 0;
+        for (int k = 0; k < 8; ++k)
+        {
+            std::int64_t const r2 = r + neighbour_dr(k);
+            std::int64_t const c2 = c + neighbour_dc(k);
+            if (r2 < 0 || r2 >= cat.rows || c2 < 0 || c2 >= cat.cols)
+                continue;
+            std::int64_t const u = r2 * cat.cols + c2;
+            std::uint8_t const code = cat.ldd[u];
+            if (code == no_data_ldd || ldd_direction(code) != opposite(k))
+                continue;
+            std::uint64_t ref;
+            if (cell_pending<TW, TH>(pending_bits, cat, r2, c2))
+            {
+                std::uint32_t const uq = index_of[cat.cell_base + u];
+                if (static_cast<std::int32_t>(uq) == h)
+                    continue;  // the chain predecessor
+                ref = ref_position_tag | pos_of[uq];
+            }
+            else if (r2 / TH != r / TH || c2 / TW != c / TW)
+            {
+                ref = (static_cast<std::uint64_t>(ci) << 32) | static_cast<std::uint64_t>(u);
+            }
+            else
+            {
+                continue;  // same tile, solved by round 0: in the staged side sum
+            }
+            if (fill)
+                refs[k_out++] = ref;
+            ++m;
+        }
+        if (!fill)
+            count[p] = m;
+    }
+
+    // ---------------------------------------------------------------------------------------------------
+    // Execution
+
+    __device__ inline double read_lane(double v, int lane)
+    {
+        std::uint64_t const bits = __double_as_longlong(v);
+        std::uint32_t const lo = __builtin_amdgcn_readlane(static_cast<std::uint32_t>(bits), lane);
+        std::uint32_t const hi = __builtin_amdgcn_readlane(static_cast<std::uint32_t>(bits >> 32), lane);
+        return __longlong_as_double((static_cast<std::uint64_t>(hi) << 32) | lo);
+    }
+
+    __device__ inline float read_lane(float v, int lane)
+    {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+    }
+
+    __device__ inline int read_lane(int v, int lane)
+    {
+        return __builtin_amdgcn_readlane(v, lane);
+    }
+
+    // Chain layout and per-call buffers of one operation.
+    template<int OP, typename T>
+    struct ChainRun
+    {
+        std::uint32_t const* item_start;  // per work item: first position
+        std::uint32_t const* item_count;  // per work item: positions
+        std::uint8_t const* seg_start;    // per position: 1 at the first (most upstream) cell of a chain
+        std::uint32_t const* ref_off;     // per position (+1): side references
+        std::uint64_t const* refs;
+        CellInput<OP, T> const* staged;   // per position: inputs (staged by round 0)
+        double const* side;               // per position: FP64 sum of the same-tile upstream cells round 0 solved
+        double* res;                      // per position: outflow (FP64)
+        T* res_rem;                       // per position: remainder (threshold, fraction)
+        int uniform_beta;                 // kinematic wave: beta is a scalar (Newton may start from the previous cell)
+    };
+
+    template<int OP, typename T>
+    __device__ inline double side_total(SweepArgs<T> const& a, ChainRun<OP, T> const& c, std::uint32_t p)
+    {
+        double sum = as_global(c.side)[p];
+        std::uint32_t const begin = as_global(c.ref_off)[p];
+        std::uint32_t const end = as_global(c.ref_off)[p + 1];
+        for (std::uint32_t j = begin; j < end; ++j)
+        {
+            std::uint64_t const ref = as_global(c.refs)[j];
+            double v;
+            if (ref & ref_position_tag)
+            {
+                v = as_global(c.res)[static_cast<std::uint32_t>(ref)];
+            }
+            else
+            {
+                std::uint32_t const ci = static_cast<std::uint32_t>(ref >> 32);
+                std::uint32_t const u = static_cast<std::uint32_t>(ref);
+                v = static_cast<double>(as_global(a.io[ci].out)[u]);
+                if constexpr (std::is_same_v<T, float>)
+                    v += static_cast<double>(as_global(a.out_lo)[a.cats[ci].cell_base + u]);
+            }
+            sum += v;
+        }
+        return sum;
+    }
+
+    template<int OP, typename T>
+    struct ChainLane
+    {
+        bool valid;
+        bool start;    // first cell of a chain
+        double extra;  // side inflow (FP64)
+        CellInput<OP, T> in;
+    };
+
+    template<int OP, typename T>
+    __device__ inline ChainLane<OP, T> load_chain_lane(SweepArgs<T> const& a, ChainRun<OP, T> const& c,
+                                                       std::uint32_t start, std::uint32_t count, std::uint32_t k)
+    {
+        ChainLane<OP, T> d{};
+        d.valid = k < count;
+        if (d.valid)
+        {
+            std::uint32_t const p = start + k;
+            d.start = as_global(c.seg_start)[p] != 0;
+            d.extra = side_total<OP, T>(a, c, p);
+            d.in = c.staged[p];
+        }
+        return d;
+    }
+
+    // One level of chains. A wavefront per work item (4 per workgroup).
+    template<int OP, typename T>
+    __global__ void __launch_bounds__(256)
+        chains_run_kernel(SweepArgs<T> a, ChainRun<OP, T> c, std::uint32_t item_begin, std::uint32_t item_end)
+    {
+        constexpr bool two_outputs = OP == op_threshold || OP == op_fraction;
+        int const lane = threadIdx.x & 63;
+        std::uint32_t const item = item_begin + blockIdx.x * 4 + (threadIdx.x >> 6);
+        if (item >= item_end)
+            return;
+        std::uint32_t const start = as_global(c.item_start)[item];
+        std::uint32_t const count = as_global(c.item_count)[item];
+        double const nan64 = __longlong_as_double(0x7ff8000000000000ll);
+        double carry = 0.0;  // outflow of the previous chain cell (FP64)
+        float warm = 0.0f;   // kinematic wave FP32: an iterate of the previous cell, to start Newton from
+        float warm_pow = 0.0f;
+        bool have_warm = false;
+        ChainLane<OP, T> next = load_chain_lane<OP, T>(a, c, start, count, lane);
+        for (std::uint32_t block = 0; block < count; block += 64)
+        {
+            ChainLane<OP, T> const d = next;
+            if (block + 64 < count)
+                next = load_chain_lane<OP, T>(a, c, start, count, block + 64 + lane);
+            int const nb = static_cast<int>(count - block < 64 ? count - block : 64);
+            double value = 0.0;
+            T rem{};
+
+            if constexpr (OP == op_accu)
+            {
+                // linear: out_k = out_{k-1} + side_k + x_k, a segmented inclusive scan along the block
+                double term = !d.valid ? 0.0 : (isnan(d.in.x) ? nan64 : d.extra + static_cast<double>(d.in.x));
+                int flag = d.valid && d.start ? 1 : 0;
+                for (int offset = 1; offset < 64; offset <<= 1)
+                {
+                    double const t_prev = __shfl_up(term, offset, 64);
+                    int const f_prev = __shfl_up(flag, offset, 64);
+                    if (lane >= offset)
+                    {
+                        if (!flag)
+                            term += t_prev;
+                        flag |= f_prev;
+                    }
+                }
+                value = flag ? term : carry + term;
+                carry = read_lane(value, nb - 1);
+            }
+            else if constexpr (OP == op_threshold || OP == op_fraction)
+            {
+                for (int j = 0; j < nb; ++j)
+                {
+                    int const first = read_lane(static_cast<int>(d.start), j);
+                    double const up = (first ? 0.0 : carry) + read_lane(d.extra, j);
+                    CellInput<OP, T> cj{read_lane(d.in.x, j), read_lane(d.in.p, j)};
+                    double out;
+                    T r{};
+                    solve_cell<OP, T>(a, cj, up, isnan(up), out, r);
+                    if (lane == j)
+                    {
+                        value = out;
+                        rem = r;
+                    }
+                    carry = out;
+                }
+            }
+            else if constexpr (std::is_same_v<T, double>)
+            {
+                // plain FP64 kinematic wave: LUE's Newton per cell along the block
+                for (int j = 0; j < nb; ++j)
+                {
+                    int const first = read_lane(static_cast<int>(d.start), j);
+                    double const up = (first ? 0.0 : carry) + read_lane(d.extra, j);
+                    CellInput<OP, T> cj{read_lane(d.in.q_cur, j), read_lane(d.in.q_lat, j), read_lane(d.in.alpha, j),
+                                        read_lane(d.in.beta, j), read_lane(d.in.b, j), read_lane(d.in.c1, j)};
+                    double v;
+                    T r{};
+                    solve_cell<OP, T>(a, cj, up, isnan(up), v, r);
+                    if (lane == j)
+                        value = v;
+                    carry = v;
+                }
+            }
+            else
+            {
+                // FP32 kinematic wave: serial FP32 Newton (warm start from the previous cell), then the FP64
+                // correction of every cell in parallel, propagated along the chains to first order
+                double q_up_mine = 0.0;
+                float q32_mine = 0.0f;
+                for (int j = 0; j < nb; ++j)
+                {
+                    int const first = read_lane(static_cast<int>(d.start), j);
+                    if (first)
+                        have_warm = false;
+                    double const q_up = (first ? 0.0 : carry) + read_lane(d.extra, j);
+                    float const q_cur = read_lane(d.in.q_cur, j);
+                    float q;
+                    if (isnan(q_up) || isnan(q_cur))
+                    {
+                        q = __int_as_float(0x7fc00000);
+                        have_warm = false;
+                    }
+                    else
+                    {
+                        float const q_lat = read_lane(d.in.q_lat, j);
+                        float const alpha = read_lane(d.in.alpha, j);
+                        float const beta = read_lane(d.in.beta, j);
+                        double const b = read_lane(d.in.b, j);
+                        double const c1 = read_lane(d.in.c1, j);
+                        have_warm = have_warm && c.uniform_beta;
+                        q = kinematic_wave_root_f32(static_cast<float>(c1), alpha, beta, a.dt, q_up, q_cur, q_lat, b, c1,
+                                                    have_warm, warm, warm_pow);
+                    }
+                    if (lane == j)
+                    {
+                        q_up_mine = q_up;
+                        q32_mine = q;
+                    }
+                    carry = static_cast<double>(q);
+                }
+                double eps = 0.0;
+                double sens = 0.0;
+                double const q = static_cast<double>(q32_mine);
+                if (d.valid && q > 0.0 && !isnan(q))
+                {
+                    kinematic_wave_correction(q, q_up_mine, static_cast<double>(d.in.alpha),
+                                              static_cast<double>(d.in.beta), d.in.b, d.in.c1, eps, sens);
+                }
+                // affine scan: delta_k = eps_k + sens_k * delta_{k-1}; a chain start does not depend on
+                // the cell before it, and the first cell of the block starts from a corrected value
+                double mul = (d.valid && d.start) ? 0.0 : sens;
+                double add = eps;
+                for (int offset = 1; offset < 64; offset <<= 1)
+                {
+                    double const m_prev = __shfl_up(mul, offset, 64);
+                    double const a_prev = __shfl_up(add, offset, 64);
+                    if (lane >= offset)
+                    {
+                        add = add + mul * a_prev;
+                        mul = mul * m_prev;
+                    }
+                }
+                value = isnan(q) ? q : (d.valid ? (q + add > 0.0 ? q + add : q) : 0.0);
+                carry = read_lane(value, nb - 1);
+            }
+
+            if (d.valid)
+            {
+                std::uint32_t const p = start + block + lane;
+                as_global(c.res)[p] = value;
+                if constexpr (two_outputs)
+                    as_global(c.res_rem)[p] = rem;
+            }
+        }
+    }
+
+    // Write the chain results of the pending cells of every tile to the output rasters.
+    template<int OP, typename T, int TW, int TH>
+    __global__ void __launch_bounds__(256)
+        chains_scatter_kernel(SweepArgs<T> a, std::uint32_t const* tile_off, std::uint32_t const* pos_of,
+                              double const* res, T const* res_rem)
+    {
+        using G = Tile<TW, TH>;
+        constexpr bool two_outputs = OP == op_threshold || OP == op_fraction;
+        __shared__ std::uint32_t prefix[G::words + 1];
+        __shared__ std::uint32_t words[G::words];
+        std::int64_t const tile = blockIdx.x;
+        std::uint32_t const begin = as_global(tile_off)[tile];
+        if (as_global(tile_off)[tile + 1] == begin)
+            return;
+        auto const bits = as_global(a.pending_bits + tile * G::words);
+        for (int w = threadIdx.x; w < G::words; w += blockDim.x)
+            words[w] = bits[w];
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            std::uint32_t sum = 0;
+            for (int w = 0; w < G::words; ++w)
+            {
+                prefix[w] = sum;
+                sum += __popc(words[w]);
+            }
+            prefix[G::words] = sum;
+        }
+        __syncthreads();
+        std::uint32_t const cat_index = as_global(a.tile_cat)[tile];
+        Catchment const cat = a.cats[cat_index];
+        CatchmentIO<T> const io = a.io[cat_index];
+        std::int64_t const local = tile - cat.tile_base;
+        std::int64_t const r0 = (local / cat.tiles_x) * TH;
+        std::int64_t const c0 = (local % cat.tiles_x) * TW;
+        for (int i = threadIdx.x; i < G::n; i += blockDim.x)
+        {
+            std::uint32_t const word = words[i >> 5];
+            std::uint32_t const b = static_cast<std::uint32_t>(i & 31);
+            if (!((word >> b) & 1u))
+                continue;
+            std::uint32_t const rank = prefix[i >> 5] + __popc(word & ((1u << b) - 1u));
+            std::uint32_t const p = as_global(pos_of)[begin + rank];
+            std::int64_t const g = (r0 + i / TW) * cat.cols + (c0 + i % TW);
+            as_global(io.out)[g] = static_cast<T>(as_global(res)[p]);
+            if constexpr (two_outputs)
+                as_global(io.rem)[g] = as_global(res_rem)[p];
+        }
+    }
+
+}  // namespace lue_amd

@@ -1,16 +1,17 @@
 // Host side of the routing sweep: flow networks (a batch of catchments with its recorded schedule) and
 // the two ways to run an operation on one:
-//   - record: round 0, then rounds over the tiles queued by the previous round, then after a fixed number
-//     of rounds the chain-streaming tail. The rounds' tile lists and the tail's chains depend on the ldd
-//     only; they are kept in the network. Host synchronisation after each round (queue length).
-//   - replay: the same launches from the recorded tile lists and chains, without queueing and without
-//     any host synchronisation (the whole operation is a fixed sequence of kernels on one stream).
+//   - record: round 0 of the tiled sweep, then, depending on the schedule,
+//       chains (default): the heavy-path chain layout of the cells round 0 leaves pending (chains.hpp);
+//       rounds: rounds over the tiles queued by the previous round until no tile is left;
+//     both depend on the ldd only and are kept in the network. Host synchronisation while recording.
+//   - replay: the same launches from the recorded schedule, without any host synchronisation (the whole
+//     operation is a fixed sequence of kernels on one stream).
 // Per-call entry points record into a temporary network; network entry points replay.
 #pragma once
 
 #include "routing_common.hpp"
 #include "sweep.hpp"
-#include "tail.hpp"
+#include "chains.hpp"
 
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -179,6 +180,22 @@ namespace lue_amd {
     };
     HostStaging& host_staging();
 
+    template<typename In, typename Out>
+    hipError_t inclusive_scan(In const* in, Out* out, std::size_t n, hipStream_t stream)
+    {
+        std::size_t bytes = 0;
+        hipError_t e = hipcub::DeviceScan::InclusiveSum(nullptr, bytes, in, out, n, stream);
+        if (e != hipSuccess)
+            return e;
+        void* temp = nullptr;
+        e = hipMallocAsync(&temp, std::max<std::size_t>(bytes, 1), stream);
+        if (e != hipSuccess)
+            return e;
+        e = hipcub::DeviceScan::InclusiveSum(temp, bytes, in, out, n, stream);
+        hipError_t const f = hipFreeAsync(temp, stream);
+        return e != hipSuccess ? e : f;
+    }
+
     inline unsigned blocks_for(std::uint64_t n, unsigned threads = 256)
     {
         return static_cast<unsigned>(std::max<std::uint64_t>((n + threads - 1) / threads, 1));
@@ -198,12 +215,19 @@ namespace lue_amd {
             dst[i] = src[i];
     }
 
+    // Schedules of the sweep after round 0.
+    enum Schedule : int {
+        schedule_chains = 0,  // round 0, then the pending cells as heavy-path chains, level after level
+        schedule_rounds = 1,  // rounds over the tiles that received flow, until none is left
+        schedule_count = 2
+    };
+
     // A batch of catchments with its recorded schedule.
     struct Network
     {
         int device{0};
         int tile_shape{tile_32x32};
-        int tail_after{8};  // rounds before the tail takes over (< 0: never)
+        int schedule{schedule_chains};
         std::vector<Catchment> cats_host;
         std::int64_t ntiles{0};
         std::int64_t ncells{0};
@@ -212,38 +236,44 @@ namespace lue_amd {
         DeviceArray<std::uint32_t> tile_cat;
 
         bool recorded{false};
-        std::vector<std::uint64_t> round_offset;  // tiles of round r (r >= 1): [round_offset[r-1], round_offset[r])
+
+        // rounds schedule: tiles of round r (r >= 1) are [round_offset[r-1], round_offset[r])
+        std::vector<std::uint64_t> round_offset;
         DeviceArray<std::uint32_t> round_tiles;
 
-        bool has_tail{false};
-        std::uint32_t tail_cells{0};
+        // chains schedule
+        std::uint32_t pending{0};         // cells left pending by round 0
         std::uint32_t nchains{0};
-        std::uint32_t nready{0};
         std::uint32_t longest_chain{0};
-        DeviceArray<std::uint32_t> chain_cells;
-        DeviceArray<std::uint32_t> chain_base;
-        DeviceArray<std::uint32_t> chain_length;
-        DeviceArray<std::uint32_t> chain_cat;
-        DeviceArray<std::int32_t> chain_downstream;
-        DeviceArray<std::int32_t> chain_arrivals;
-        DeviceArray<std::int32_t> chain_ready;
+        std::uint64_t nrefs{0};
+        std::vector<std::uint32_t> level_items;  // work items of level l: [level_items[l], level_items[l+1])
+        DeviceArray<std::uint32_t> tile_off;     // per tile (+1): first compact index of its pending cells
+        DeviceArray<std::uint32_t> pos_of;       // per compact index: chain position
+        DeviceArray<std::uint8_t> seg_start;     // per position
+        DeviceArray<std::uint32_t> ref_off;      // per position (+1)
+        DeviceArray<std::uint64_t> refs;
+        DeviceArray<std::uint32_t> item_start;
+        DeviceArray<std::uint32_t> item_count;
 
         auto rounds() const -> int { return static_cast<int>(round_offset.empty() ? 0 : round_offset.size() - 1); }
+        auto levels() const -> int { return static_cast<int>(level_items.empty() ? 0 : level_items.size() - 1); }
     };
 
     // Describe a batch: tiles are numbered catchment after catchment.
     inline int network_init(Network& net, int n, std::uint8_t const* const* ldd, std::int64_t const* rows,
-                            std::int64_t const* cols, int tile_shape, int tail_after, hipStream_t stream)
+                            std::int64_t const* cols, int tile_shape, int schedule, hipStream_t stream)
     {
         if (n <= 0 || !ldd || !rows || !cols)
             return fail(status_invalid_argument, "a network needs at least one catchment");
         if (tile_shape < 0 || tile_shape >= tile_shape_count)
             return fail(status_invalid_argument, "unknown tile shape");
+        if (schedule < 0 || schedule >= schedule_count)
+            return fail(status_invalid_argument, "unknown schedule");
         int tw, th;
         tile_extent(tile_shape, tw, th);
         LUE_AMD_HIP(hipGetDevice(&net.device));
         net.tile_shape = tile_shape;
-        net.tail_after = tail_after;
+        net.schedule = schedule;
         net.cats_host.clear();
         std::int64_t tiles = 0;
         std::int64_t cells = 0;
@@ -254,7 +284,7 @@ namespace lue_amd {
                 return fail(status_invalid_argument, "ldd must not be null");
             if (rows[c] <= 0 || cols[c] <= 0)
                 return fail(status_invalid_argument, "array shape must not be empty");
-            if (rows[c] > (1ll << 31) || cols[c] > (1ll << 31))
+            if (rows[c] > (1ll << 31) || cols[c] > (1ll << 31) || rows[c] * cols[c] > (1ll << 32))
                 return fail(status_invalid_argument, "array extent too large");
             Catchment cat{};
             cat.ldd = ldd[c];
@@ -294,10 +324,11 @@ namespace lue_amd {
         DeviceArray<std::uint32_t> worklist;
         DeviceArray<std::uint32_t> work_count;
         DeviceArray<unsigned long long> counters;
-        DeviceArray<std::int32_t> arrivals;
-        DeviceArray<std::uint32_t> cursor;
-        DeviceArray<unsigned long long> tail_solved;
         DeviceArray<unsigned char> io;
+        DeviceArray<unsigned char> staged;
+        DeviceArray<double> side;
+        DeviceArray<double> res;
+        DeviceArray<unsigned char> res_rem;
     };
 
     template<typename T, int TW, int TH>
@@ -332,6 +363,10 @@ namespace lue_amd {
         a.work_count = w.work_count.ptr;
         a.counters = w.counters.ptr;
         a.record = recording ? 1 : 0;
+        a.stage_off = nullptr;
+        a.stage_pos = nullptr;
+        a.staged = nullptr;
+        a.stage_side = nullptr;
         return status_success;
     }
 
@@ -345,224 +380,462 @@ namespace lue_amd {
         return status_success;
     }
 
-    // Build the tail's chains from the pending bitmaps and keep them in the network.
-    template<int TW, int TH>
-    int build_tail(Network& net, std::uint32_t const* pending_bits, hipStream_t stream)
+    template<typename P>
+    hipError_t copy_to_host(P* dst, P const* src, std::size_t n, hipStream_t stream)
     {
-        std::int64_t const ntiles = net.ntiles;
-        DeviceArray<std::uint32_t> counts, offsets;
-        LUE_AMD_HIP(counts.allocate(static_cast<std::size_t>(ntiles + 1), stream));
-        LUE_AMD_HIP(offsets.allocate(static_cast<std::size_t>(ntiles + 1), stream));
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, counts.ptr + ntiles, 0u, 1ull);
-        hipLaunchKernelGGL((tail_count_kernel<TW, TH>), dim3(blocks_for(ntiles)), dim3(256), 0, stream, pending_bits,
-                           ntiles, counts.ptr);
-        LUE_AMD_HIP(hipGetLastError());
-        LUE_AMD_HIP(exclusive_scan(counts.ptr, offsets.ptr, static_cast<std::size_t>(ntiles + 1), stream));
-        std::uint32_t n = 0;
-        LUE_AMD_HIP(hipMemcpyAsync(&n, offsets.ptr + ntiles, sizeof(n), hipMemcpyDeviceToHost, stream));
-        LUE_AMD_HIP(hipStreamSynchronize(stream));
-        net.tail_cells = n;
-        net.has_tail = n > 0;
-        if (n == 0)
-            return status_success;
+        hipError_t e = hipMemcpyAsync(dst, src, sizeof(P) * n, hipMemcpyDeviceToHost, stream);
+        return e != hipSuccess ? e : hipStreamSynchronize(stream);
+    }
 
-        DeviceArray<std::uint32_t> cells, index_of, is_last, chain_of_last;
-        DeviceArray<std::int32_t> next, upstream_count, succ[2], rank[2], last[2], first;
-        LUE_AMD_HIP(cells.allocate(n, stream));
-        LUE_AMD_HIP(index_of.allocate(static_cast<std::size_t>(net.ncells), stream));
-        LUE_AMD_HIP(next.allocate(n, stream));
-        LUE_AMD_HIP(upstream_count.allocate(n, stream));
+    // List ranking of the tour elements (pointer jumping, double buffered). `cur` names the buffers that
+    // hold the result. Lists that do not end (cycles in the flow network) leave `cyclic` set.
+    inline int rank_lists(std::int32_t const* succ, std::uint32_t n2, DeviceArray<std::int32_t> (&next)[2],
+                          DeviceArray<std::int32_t> (&value)[2], DeviceArray<std::int32_t> (&last)[2], int& cur,
+                          bool& cyclic, hipStream_t stream)
+    {
         for (int b = 0; b < 2; ++b)
         {
-            LUE_AMD_HIP(succ[b].allocate(n, stream));
-            LUE_AMD_HIP(rank[b].allocate(n, stream));
-            LUE_AMD_HIP(last[b].allocate(n, stream));
+            LUE_AMD_HIP(next[b].allocate(n2, stream));
+            LUE_AMD_HIP(value[b].allocate(n2, stream));
+            LUE_AMD_HIP(last[b].allocate(n2, stream));
         }
-        LUE_AMD_HIP(is_last.allocate(n + 1, stream));
-        LUE_AMD_HIP(chain_of_last.allocate(n + 1, stream));
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(n)), dim3(256), 0, stream,
-                           reinterpret_cast<std::uint32_t*>(upstream_count.ptr), 0u, static_cast<std::uint64_t>(n));
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, is_last.ptr + n, 0u, 1ull);
-
-        int const ncats = static_cast<int>(net.cats_host.size());
-        hipLaunchKernelGGL((tail_compact_kernel<TW, TH>), dim3(static_cast<unsigned>(ntiles)), dim3(64), 0, stream,
-                           pending_bits, net.cats.ptr, net.tile_cat.ptr, offsets.ptr, cells.ptr, index_of.ptr);
-        hipLaunchKernelGGL((tail_links_kernel<TW, TH>), dim3(blocks_for(n)), dim3(256), 0, stream, net.cats.ptr, ncats,
-                           pending_bits, cells.ptr, index_of.ptr, n, next.ptr, upstream_count.ptr);
-        hipLaunchKernelGGL(tail_rank_init_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, next.ptr,
-                           upstream_count.ptr, n, succ[0].ptr, rank[0].ptr, last[0].ptr);
-        int cur = 0;
-        for (std::uint32_t reach = 1; reach < n; reach <<= 1)
+        DeviceArray<std::uint32_t> linked;
+        LUE_AMD_HIP(linked.allocate(1, stream));
+        hipLaunchKernelGGL(chains_rank_init_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, succ, n2, next[0].ptr,
+                           value[0].ptr, last[0].ptr);
+        cur = 0;
+        int steps = 2;
+        for (std::uint64_t span = 1; span < n2; span <<= 1)
+            ++steps;
+        std::uint32_t still = 1;
+        for (int step = 0; step < steps && still > 0; ++step)
         {
-            hipLaunchKernelGGL(tail_rank_step_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, succ[cur].ptr,
-                               rank[cur].ptr, last[cur].ptr, n, succ[1 - cur].ptr, rank[1 - cur].ptr, last[1 - cur].ptr);
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, linked.ptr, 0u, 1ull);
+            hipLaunchKernelGGL(chains_rank_step_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, next[cur].ptr,
+                               value[cur].ptr, last[cur].ptr, n2, next[1 - cur].ptr, value[1 - cur].ptr,
+                               last[1 - cur].ptr, linked.ptr);
+            LUE_AMD_HIP(hipGetLastError());
             cur = 1 - cur;
+            LUE_AMD_HIP(copy_to_host(&still, linked.ptr, 1, stream));
         }
-        hipLaunchKernelGGL(tail_flag_last_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, next.ptr,
-                           upstream_count.ptr, n, is_last.ptr);
+        cyclic = still > 0;
+        return status_success;
+    }
+
+    // Whether perm[0, n) is a permutation of [0, n); recording stops with an error instead of indexing out
+    // of bounds if it is not.
+    inline int check_permutation(std::uint32_t const* perm, std::uint32_t n, char const* what, hipStream_t stream)
+    {
+        DeviceArray<std::uint32_t> seen, bad;
+        LUE_AMD_HIP(seen.allocate(n, stream));
+        LUE_AMD_HIP(bad.allocate(1, stream));
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, seen.ptr, 0u,
+                           static_cast<std::uint64_t>(n));
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, bad.ptr, 0u, 1ull);
+        hipLaunchKernelGGL(chains_check_permutation_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, perm, n,
+                           seen.ptr, bad.ptr);
         LUE_AMD_HIP(hipGetLastError());
-        LUE_AMD_HIP(exclusive_scan(is_last.ptr, chain_of_last.ptr, static_cast<std::size_t>(n + 1), stream));
+        std::uint32_t nbad = 0;
+        LUE_AMD_HIP(copy_to_host(&nbad, bad.ptr, 1, stream));
+        if (nbad != 0)
+            return fail(status_hip_error, std::string("chain layout: ") + what + " is not a permutation");
+        return status_success;
+    }
+
+    // Record the chain schedule from the pending bitmaps of round 0.
+    template<int TW, int TH>
+    int build_chains(Network& net, std::uint32_t const* pending_bits, hipStream_t stream)
+    {
+        std::int64_t const ntiles = net.ntiles;
+        int const ncats = static_cast<int>(net.cats_host.size());
+        {
+            DeviceArray<std::uint32_t> counts;
+            LUE_AMD_HIP(counts.allocate(static_cast<std::size_t>(ntiles + 1), stream));
+            LUE_AMD_HIP(net.tile_off.allocate(static_cast<std::size_t>(ntiles + 1), stream));
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, counts.ptr + ntiles, 0u, 1ull);
+            hipLaunchKernelGGL((chains_count_kernel<TW, TH>), dim3(blocks_for(ntiles)), dim3(256), 0, stream,
+                               pending_bits, ntiles, counts.ptr);
+            LUE_AMD_HIP(hipGetLastError());
+            LUE_AMD_HIP(exclusive_scan(counts.ptr, net.tile_off.ptr, static_cast<std::size_t>(ntiles + 1), stream));
+        }
+        std::uint32_t n = 0;
+        LUE_AMD_HIP(copy_to_host(&n, net.tile_off.ptr + ntiles, 1, stream));
+        net.pending = n;
+        net.level_items.assign(1, 0);
+        if (n == 0)
+            return status_success;
+        if (n >= (1u << 30))
+            return fail(status_invalid_argument, "too many cells left for the chain schedule (2^30)");
+
+        DeviceArray<std::uint32_t> cells, index_of;
+        DeviceArray<std::int32_t> parent, heavy, succ;
+        LUE_AMD_HIP(cells.allocate(n, stream));
+        LUE_AMD_HIP(index_of.allocate(static_cast<std::size_t>(net.ncells), stream));
+        LUE_AMD_HIP(parent.allocate(n, stream));
+        LUE_AMD_HIP(heavy.allocate(n, stream));
+        LUE_AMD_HIP(succ.allocate(2 * static_cast<std::size_t>(n), stream));
+        hipLaunchKernelGGL((chains_compact_kernel<TW, TH>), dim3(static_cast<unsigned>(ntiles)), dim3(64), 0, stream,
+                           pending_bits, net.cats.ptr, net.tile_cat.ptr, net.tile_off.ptr, cells.ptr, index_of.ptr);
+        hipLaunchKernelGGL((chains_parent_kernel<TW, TH>), dim3(blocks_for(n)), dim3(256), 0, stream, net.cats.ptr,
+                           ncats, pending_bits, cells.ptr, index_of.ptr, n, parent.ptr);
+        LUE_AMD_HIP(hipGetLastError());
+
+        // subtree sizes (Euler tour, children in neighbour order), heavy children
+        DeviceArray<std::int32_t> next[2], value[2], last[2];
+        int cur = 0;
+        bool cyclic = false;
+        hipLaunchKernelGGL((chains_tour_kernel<TW, TH>), dim3(blocks_for(n)), dim3(256), 0, stream, net.cats.ptr, ncats,
+                           pending_bits, cells.ptr, index_of.ptr, parent.ptr, nullptr, n, succ.ptr);
+        LUE_AMD_HIP(hipGetLastError());
+        if (int const st = rank_lists(succ.ptr, 2 * n, next, value, last, cur, cyclic, stream))
+            return st;
+        if (cyclic)
+            return fail(status_invalid_network, "invalid flow direction network: cells on or downstream of a cycle");
+        hipLaunchKernelGGL((chains_heavy_kernel<TW, TH>), dim3(blocks_for(n)), dim3(256), 0, stream, net.cats.ptr,
+                           ncats, pending_bits, cells.ptr, index_of.ptr, value[cur].ptr, n, heavy.ptr);
+        LUE_AMD_HIP(hipGetLastError());
+
+        // preorder with heavy children first: every chain is a contiguous run
+        hipLaunchKernelGGL((chains_tour_kernel<TW, TH>), dim3(blocks_for(n)), dim3(256), 0, stream, net.cats.ptr, ncats,
+                           pending_bits, cells.ptr, index_of.ptr, parent.ptr, heavy.ptr, n, succ.ptr);
+        LUE_AMD_HIP(hipGetLastError());
+        if (int const st = rank_lists(succ.ptr, 2 * n, next, value, last, cur, cyclic, stream))
+            return st;
+        if (cyclic)
+            return fail(status_invalid_network, "invalid flow direction network: cells on or downstream of a cycle");
+        DeviceArray<std::uint32_t> root_flag, root_index, tree_size, tree_base, pre, head_flag, chain_incl;
+        LUE_AMD_HIP(root_flag.allocate(n + 1, stream));
+        LUE_AMD_HIP(root_index.allocate(n + 1, stream));
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, root_flag.ptr + n, 0u, 1ull);
+        hipLaunchKernelGGL(chains_root_flag_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, parent.ptr, n,
+                           root_flag.ptr);
+        LUE_AMD_HIP(exclusive_scan(root_flag.ptr, root_index.ptr, n + 1, stream));
+        std::uint32_t nroots = 0;
+        LUE_AMD_HIP(copy_to_host(&nroots, root_index.ptr + n, 1, stream));
+        LUE_AMD_HIP(tree_size.allocate(nroots + 1, stream));
+        LUE_AMD_HIP(tree_base.allocate(nroots + 1, stream));
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, tree_size.ptr + nroots, 0u, 1ull);
+        hipLaunchKernelGGL(chains_tree_size_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, parent.ptr,
+                           value[cur].ptr, root_index.ptr, n, tree_size.ptr);
+        LUE_AMD_HIP(exclusive_scan(tree_size.ptr, tree_base.ptr, nroots + 1, stream));
+        LUE_AMD_HIP(pre.allocate(n, stream));
+        LUE_AMD_HIP(head_flag.allocate(n, stream));
+        LUE_AMD_HIP(chain_incl.allocate(n, stream));
+        hipLaunchKernelGGL(chains_preorder_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, parent.ptr, heavy.ptr,
+                           value[cur].ptr, last[cur].ptr, root_index.ptr, tree_base.ptr, n, pre.ptr, head_flag.ptr);
+        LUE_AMD_HIP(hipGetLastError());
+        if (int const st = check_permutation(pre.ptr, n, "preorder", stream))
+            return st;
+        LUE_AMD_HIP(inclusive_scan(head_flag.ptr, chain_incl.ptr, n, stream));
         std::uint32_t nchains = 0;
-        LUE_AMD_HIP(hipMemcpyAsync(&nchains, chain_of_last.ptr + n, sizeof(nchains), hipMemcpyDeviceToHost, stream));
-        LUE_AMD_HIP(hipStreamSynchronize(stream));
-        if (nchains == 0)
-            return fail(status_invalid_network, "invalid flow direction network: cells on a cycle");
+        LUE_AMD_HIP(copy_to_host(&nchains, chain_incl.ptr + n - 1, 1, stream));
         net.nchains = nchains;
 
-        DeviceArray<std::uint32_t> ready_count;
-        LUE_AMD_HIP(net.chain_length.allocate(nchains + 1, stream));
-        LUE_AMD_HIP(net.chain_base.allocate(nchains + 1, stream));
-        LUE_AMD_HIP(net.chain_cat.allocate(nchains, stream));
-        LUE_AMD_HIP(net.chain_downstream.allocate(nchains, stream));
-        LUE_AMD_HIP(net.chain_arrivals.allocate(nchains, stream));
-        LUE_AMD_HIP(net.chain_ready.allocate(nchains, stream));
-        LUE_AMD_HIP(net.chain_cells.allocate(n, stream));
-        LUE_AMD_HIP(first.allocate(nchains, stream));
-        LUE_AMD_HIP(ready_count.allocate(1, stream));
-        // a chain whose first cell is never found (a cycle without entry) keeps length 0 and never gets ready
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(nchains + 1)), dim3(256), 0, stream,
-                           net.chain_length.ptr, 0u, static_cast<std::uint64_t>(nchains + 1));
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream,
-                           reinterpret_cast<std::uint32_t*>(net.chain_arrivals.ptr), 0xFFFFFFFFu,
+        // chains: preorder of their heads, lengths, levels
+        DeviceArray<std::uint32_t> chain_pre, length, level, base, changed;
+        LUE_AMD_HIP(chain_pre.allocate(nchains, stream));
+        LUE_AMD_HIP(length.allocate(nchains, stream));
+        LUE_AMD_HIP(level.allocate(nchains, stream));
+        LUE_AMD_HIP(base.allocate(nchains, stream));
+        LUE_AMD_HIP(changed.allocate(1, stream));
+        hipLaunchKernelGGL(chains_head_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, head_flag.ptr, chain_incl.ptr,
+                           n, chain_pre.ptr);
+        hipLaunchKernelGGL(chains_length_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, chain_pre.ptr, nchains,
+                           n, length.ptr);
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, level.ptr, 0u,
                            static_cast<std::uint64_t>(nchains));
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, net.chain_cat.ptr, 0u,
-                           static_cast<std::uint64_t>(nchains));
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, ready_count.ptr, 0u, 1ull);
-        hipLaunchKernelGGL(tail_chain_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, next.ptr, upstream_count.ptr,
-                           rank[cur].ptr, last[cur].ptr, chain_of_last.ptr, cells.ptr, net.cats.ptr, ncats, n,
-                           net.chain_length.ptr, first.ptr, net.chain_arrivals.ptr, net.chain_cat.ptr);
-        LUE_AMD_HIP(exclusive_scan(net.chain_length.ptr, net.chain_base.ptr, static_cast<std::size_t>(nchains + 1),
-                                   stream));
-        hipLaunchKernelGGL(tail_downstream_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, next.ptr, last[cur].ptr,
-                           chain_of_last.ptr, is_last.ptr, n, net.chain_downstream.ptr);
-        hipLaunchKernelGGL(tail_layout_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, cells.ptr, rank[cur].ptr,
-                           last[cur].ptr, is_last.ptr, chain_of_last.ptr, net.chain_base.ptr, net.chain_length.ptr, n,
-                           net.chain_cells.ptr);
-        hipLaunchKernelGGL(tail_ready_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, net.chain_arrivals.ptr,
-                           nchains, net.chain_ready.ptr, ready_count.ptr);
-        LUE_AMD_HIP(hipGetLastError());
-        LUE_AMD_HIP(hipMemcpyAsync(&net.nready, ready_count.ptr, sizeof(std::uint32_t), hipMemcpyDeviceToHost, stream));
-        LUE_AMD_HIP(hipStreamSynchronize(stream));
-        return status_success;
-    }
-
-    // Run the tail of one operation from the network's chains.
-    template<int OP, typename T>
-    int launch_tail(Network const& net, Workspace& w, SweepArgs<T> const& a, hipStream_t stream, bool count_solved)
-    {
-        if (!net.has_tail || net.nready == 0)
-            return status_success;
-        LUE_AMD_HIP(w.arrivals.allocate(net.nchains, stream));
-        LUE_AMD_HIP(w.cursor.allocate(1, stream));
-        LUE_AMD_HIP(w.tail_solved.allocate(1, stream));
-        hipLaunchKernelGGL(copy_i32_kernel, dim3(blocks_for(net.nchains)), dim3(256), 0, stream, w.arrivals.ptr,
-                           net.chain_arrivals.ptr, static_cast<std::uint64_t>(net.nchains));
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, w.cursor.ptr, 0u, 1ull);
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream,
-                           reinterpret_cast<std::uint32_t*>(w.tail_solved.ptr), 0u, 2ull);
-        TailChains t{net.chain_cells.ptr, net.chain_cat.ptr, net.chain_base.ptr, net.chain_length.ptr,
-                     net.chain_downstream.ptr, w.arrivals.ptr, net.chain_ready.ptr, net.nready, w.cursor.ptr,
-                     w.tail_solved.ptr};
-        (void)count_solved;
-        // one wavefront per chain, 4 per workgroup
-        unsigned const grid = static_cast<unsigned>(std::min<std::uint64_t>((net.nready + 3) / 4, 2048));
-        hipLaunchKernelGGL((tail_execute_kernel<OP, T>), dim3(std::max(grid, 1u)), dim3(256), 0, stream, a, t);
-        LUE_AMD_HIP(hipGetLastError());
-        return status_success;
-    }
-
-    // Record: run the operation while recording the schedule into `net`.
-    template<int OP, typename T, int TW, int TH>
-    int record_run(Network& net, SweepArgs<T> a, std::vector<CatchmentIO<T>> const& io, hipStream_t stream)
-    {
-        Workspace w;
-        if (int const st = workspace_init<T, TW, TH>(net, w, a, true, stream))
-            return st;
-        if (int const st = upload_io<T>(w, io, a, stream))
-            return st;
-        hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-        if (profile().enabled)
+        for (int iteration = 0; iteration < 64; ++iteration)
         {
+            std::uint32_t count = 0;
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, changed.ptr, 0u, 1ull);
+            hipLaunchKernelGGL(chains_level_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, parent.ptr, pre.ptr,
+                               head_flag.ptr, chain_incl.ptr, n, level.ptr, changed.ptr);
+            LUE_AMD_HIP(hipGetLastError());
+            LUE_AMD_HIP(copy_to_host(&count, changed.ptr, 1, stream));
+            if (count == 0)
+                break;
+            if (iteration == 63)
+                return fail(status_invalid_network, "chain levels did not converge");
+        }
+
+        // layout (host): chains level after level, in preorder of their heads inside a level; work items are
+        // one long chain, or consecutive short chains of at most 64 cells together
+        std::vector<std::uint32_t> h_level(nchains), h_length(nchains), h_base(nchains);
+        LUE_AMD_HIP(copy_to_host(h_level.data(), level.ptr, nchains, stream));
+        LUE_AMD_HIP(copy_to_host(h_length.data(), length.ptr, nchains, stream));
+        std::uint32_t nlevels = 0;
+        for (std::uint32_t c = 0; c < nchains; ++c)
+            nlevels = std::max(nlevels, h_level[c] + 1);
+        std::vector<std::uint64_t> level_cells(nlevels + 1, 0);
+        for (std::uint32_t c = 0; c < nchains; ++c)
+            level_cells[h_level[c] + 1] += h_length[c];
+        for (std::uint32_t l = 0; l < nlevels; ++l)
+            level_cells[l + 1] += level_cells[l];
+        std::vector<std::uint64_t> at(level_cells.begin(), level_cells.end() - 1);
+        std::vector<std::vector<std::pair<std::uint32_t, std::uint32_t>>> items(nlevels);
+        std::vector<std::pair<std::uint32_t, std::uint32_t>> open(nlevels, {0u, 0u});
+        std::uint32_t longest = 0;
+        for (std::uint32_t c = 0; c < nchains; ++c)
+        {
+            std::uint32_t const l = h_level[c];
+            std::uint32_t const len = h_length[c];
+            std::uint32_t const b = static_cast<std::uint32_t>(at[l]);
+            h_base[c] = b;
+            at[l] += len;
+            longest = std::max(longest, len);
+            auto& cur_item = open[l];
+            if (len > 64)
+            {
+                items[l].emplace_back(b, len);
+                continue;
+            }
+            if (cur_item.second > 0 && (cur_item.first + cur_item.second != b || cur_item.second + len > 64))
+            {
+                items[l].push_back(cur_item);
+                cur_item = {0u, 0u};
+            }
+            if (cur_item.second == 0)
+                cur_item.first = b;
+            cur_item.second += len;
+        }
+        std::vector<std::uint32_t> h_item_start, h_item_count;
+        net.level_items.assign(1, 0);
+        for (std::uint32_t l = 0; l < nlevels; ++l)
+        {
+            if (open[l].second > 0)
+                items[l].push_back(open[l]);
+            // long chains first
+            std::stable_sort(items[l].begin(), items[l].end(),
+                             [](auto const& x, auto const& y) { return x.second > y.second; });
+            for (auto const& it : items[l])
+            {
+                h_item_start.push_back(it.first);
+                h_item_count.push_back(it.second);
+            }
+            net.level_items.push_back(static_cast<std::uint32_t>(h_item_start.size()));
+        }
+        net.longest_chain = longest;
+        LUE_AMD_HIP(hipMemcpyAsync(base.ptr, h_base.data(), sizeof(std::uint32_t) * nchains, hipMemcpyHostToDevice,
+                                   stream));
+        LUE_AMD_HIP(net.item_start.allocate(h_item_start.size(), stream));
+        LUE_AMD_HIP(net.item_count.allocate(h_item_count.size(), stream));
+        LUE_AMD_HIP(hipMemcpyAsync(net.item_start.ptr, h_item_start.data(), sizeof(std::uint32_t) * h_item_start.size(),
+                                   hipMemcpyHostToDevice, stream));
+        LUE_AMD_HIP(hipMemcpyAsync(net.item_count.ptr, h_item_count.data(), sizeof(std::uint32_t) * h_item_count.size(),
+                                   hipMemcpyHostToDevice, stream));
+
+        // positions, chain starts, side references
+        LUE_AMD_HIP(net.pos_of.allocate(n, stream));
+        LUE_AMD_HIP(net.seg_start.allocate(n, stream));
+        hipLaunchKernelGGL(chains_position_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, pre.ptr, chain_incl.ptr,
+                           chain_pre.ptr, length.ptr, base.ptr, n, net.pos_of.ptr, net.seg_start.ptr);
+        LUE_AMD_HIP(hipGetLastError());
+        if (int const st = check_permutation(net.pos_of.ptr, n, "chain positions", stream))
+            return st;
+        DeviceArray<std::uint32_t> ref_count;
+        LUE_AMD_HIP(ref_count.allocate(n + 1, stream));
+        LUE_AMD_HIP(net.ref_off.allocate(n + 1, stream));
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, ref_count.ptr + n, 0u, 1ull);
+        hipLaunchKernelGGL((chains_refs_kernel<TW, TH>), dim3(blocks_for(n)), dim3(256), 0, stream, net.cats.ptr, ncats,
+                           pending_bits, cells.ptr, index_of.ptr, heavy.ptr, net.pos_of.ptr, n, false, ref_count.ptr,
+                           nullptr, nullptr);
+        LUE_AMD_HIP(hipGetLastError());
+        LUE_AMD_HIP(exclusive_scan(ref_count.ptr, net.ref_off.ptr, n + 1, stream));
+        std::uint32_t nrefs = 0;
+        LUE_AMD_HIP(copy_to_host(&nrefs, net.ref_off.ptr + n, 1, stream));
+        net.nrefs = nrefs;
+        LUE_AMD_HIP(net.refs.allocate(std::max<std::uint32_t>(nrefs, 1), stream));
+        hipLaunchKernelGGL((chains_refs_kernel<TW, TH>), dim3(blocks_for(n)), dim3(256), 0, stream, net.cats.ptr, ncats,
+                           pending_bits, cells.ptr, index_of.ptr, heavy.ptr, net.pos_of.ptr, n, true, nullptr,
+                           net.ref_off.ptr, net.refs.ptr);
+        LUE_AMD_HIP(hipGetLastError());
+        LUE_AMD_HIP(hipStreamSynchronize(stream));  // host vectors go out of scope
+        return status_success;
+    }
+
+    // Per-call buffers of the chain schedule and the staging arguments of round 0.
+    template<int OP, typename T>
+    int chains_workspace(Network const& net, Workspace& w, SweepArgs<T>& a, hipStream_t stream)
+    {
+        std::size_t const n = std::max<std::uint32_t>(net.pending, 1);
+        LUE_AMD_HIP(w.staged.allocate(n * sizeof(CellInput<OP, T>), stream));
+        LUE_AMD_HIP(w.side.allocate(n, stream));
+        LUE_AMD_HIP(w.res.allocate(n, stream));
+        if constexpr (OP == op_threshold || OP == op_fraction)
+        {
+            LUE_AMD_HIP(w.res_rem.allocate(n * sizeof(T), stream));
+        }
+        a.stage_off = net.tile_off.ptr;
+        a.stage_pos = net.pos_of.ptr;
+        a.staged = w.staged.ptr;
+        a.stage_side = w.side.ptr;
+        return status_success;
+    }
+
+    // Levels and the final scatter of the chain schedule.
+    template<int OP, typename T, int TW, int TH>
+    int launch_chains(Network const& net, Workspace& w, SweepArgs<T> const& a, std::vector<CatchmentIO<T>> const& io,
+                      hipStream_t stream)
+    {
+        if (net.pending == 0)
+            return status_success;
+        bool uniform_beta = true;
+        for (auto const& x : io)
+            uniform_beta = uniform_beta && x.beta == nullptr;
+        ChainRun<OP, T> c{net.item_start.ptr,
+                          net.item_count.ptr,
+                          net.seg_start.ptr,
+                          net.ref_off.ptr,
+                          net.refs.ptr,
+                          reinterpret_cast<CellInput<OP, T> const*>(w.staged.ptr),
+                          w.side.ptr,
+                          w.res.ptr,
+                          reinterpret_cast<T*>(w.res_rem.ptr),
+                          uniform_beta ? 1 : 0};
+        for (int l = 0; l < net.levels(); ++l)
+        {
+            std::uint32_t const begin = net.level_items[l];
+            std::uint32_t const end = net.level_items[l + 1];
+            if (end == begin)
+                continue;
+            hipLaunchKernelGGL((chains_run_kernel<OP, T>), dim3((end - begin + 3) / 4), dim3(256), 0, stream, a, c, begin,
+                               end);
+        }
+        hipLaunchKernelGGL((chains_scatter_kernel<OP, T, TW, TH>), dim3(static_cast<unsigned>(net.ntiles)), dim3(256), 0,
+                           stream, a, net.tile_off.ptr, net.pos_of.ptr, w.res.ptr,
+                           reinterpret_cast<T const*>(w.res_rem.ptr));
+        LUE_AMD_HIP(hipGetLastError());
+        return status_success;
+    }
+
+    // Timing of one operation (lue_amd_profile_*): events around round 0 and the rest.
+    struct OpTimer
+    {
+        hipEvent_t ev[3]{nullptr, nullptr, nullptr};
+        bool on{false};
+
+        int start(hipStream_t stream)
+        {
+            on = profile().enabled;
+            if (!on)
+                return status_success;
             for (auto& e : ev)
                 LUE_AMD_HIP(hipEventCreate(&e));
             LUE_AMD_HIP(hipEventRecord(ev[0], stream));
+            return status_success;
         }
-        hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH>), dim3(static_cast<unsigned>(net.ntiles)), dim3(256), 0,
-                           stream, a);
-        LUE_AMD_HIP(hipGetLastError());
-        if (profile().enabled)
-            LUE_AMD_HIP(hipEventRecord(ev[1], stream));
 
-        net.round_offset.assign(1, 0);
-        std::vector<std::uint32_t> tiles_of_rounds;
-        unsigned const grid = static_cast<unsigned>(std::min<std::int64_t>(net.ntiles, 2048));
-        std::uint32_t round = 1;
-        bool tail = false;
-        while (true)
+        int mark(hipStream_t stream)
         {
-            // the tiles queued for this round
-            std::uint32_t const slot = round % 3;
-            std::uint32_t count = 0;
-            LUE_AMD_HIP(hipMemcpyAsync(&count, w.work_count.ptr + slot, sizeof(count), hipMemcpyDeviceToHost, stream));
-            LUE_AMD_HIP(hipStreamSynchronize(stream));
-            if (count == 0)
-                break;
-            if (net.tail_after >= 0 && static_cast<int>(round) > net.tail_after)
-            {
-                tail = true;
-                break;
-            }
-            std::size_t const at = tiles_of_rounds.size();
-            tiles_of_rounds.resize(at + count);
-            LUE_AMD_HIP(hipMemcpyAsync(tiles_of_rounds.data() + at, w.worklist.ptr + static_cast<std::int64_t>(slot) * net.ntiles,
-                                       sizeof(std::uint32_t) * count, hipMemcpyDeviceToHost, stream));
-            hipLaunchKernelGGL((sweep_next_round<OP, T, TW, TH>), dim3(std::min<unsigned>(grid, count)), dim3(256), 0,
-                               stream, a, round);
-            LUE_AMD_HIP(hipGetLastError());
-            net.round_offset.push_back(tiles_of_rounds.size());
-            ++round;
-            if (round > 0xFFFF0000u)
-                return fail(status_invalid_network, "routing did not converge");
+            if (on)
+                LUE_AMD_HIP(hipEventRecord(ev[1], stream));
+            return status_success;
         }
-        LUE_AMD_HIP(hipStreamSynchronize(stream));
-        LUE_AMD_HIP(net.round_tiles.allocate(tiles_of_rounds.size(), stream));
-        if (!tiles_of_rounds.empty())
+
+        int stop(hipStream_t stream, long long rounds, std::int64_t cells)
         {
-            LUE_AMD_HIP(hipMemcpyAsync(net.round_tiles.ptr, tiles_of_rounds.data(),
-                                       sizeof(std::uint32_t) * tiles_of_rounds.size(), hipMemcpyHostToDevice, stream));
-        }
-        net.has_tail = false;
-        net.nready = 0;
-        if (tail)
-        {
-            if (int const st = build_tail<TW, TH>(net, w.pending_bits.ptr, stream))
-                return st;
-            if (int const st = launch_tail<OP, T>(net, w, a, stream, true))
-                return st;
-        }
-        if (profile().enabled)
+            if (!on)
+                return status_success;
             LUE_AMD_HIP(hipEventRecord(ev[2], stream));
-        unsigned long long counters[2] = {0, 0};
-        unsigned long long tail_solved = 0;
-        LUE_AMD_HIP(hipMemcpyAsync(counters, w.counters.ptr, sizeof(counters), hipMemcpyDeviceToHost, stream));
-        if (net.has_tail && net.nready > 0)
-            LUE_AMD_HIP(hipMemcpyAsync(&tail_solved, w.tail_solved.ptr, sizeof(tail_solved), hipMemcpyDeviceToHost, stream));
-        LUE_AMD_HIP(hipStreamSynchronize(stream));
-        if (profile().enabled)
-        {
+            LUE_AMD_HIP(hipEventSynchronize(ev[2]));
             float first = 0.0f, rest = 0.0f;
             LUE_AMD_HIP(hipEventElapsedTime(&first, ev[0], ev[1]));
             LUE_AMD_HIP(hipEventElapsedTime(&rest, ev[1], ev[2]));
             profile().first_round_ms += first;
             profile().next_rounds_ms += rest;
             profile().calls += 1;
-            profile().rounds += round;
-            profile().cells += net.ncells;
-            for (auto& e : ev)
-                (void)hipEventDestroy(e);
+            profile().rounds += rounds;
+            profile().cells += cells;
+            return status_success;
         }
+
+        ~OpTimer()
+        {
+            for (auto& e : ev)
+                if (e)
+                    (void)hipEventDestroy(e);
+        }
+    };
+
+    template<int OP, typename T, int TW, int TH>
+    int replay_run(Network const& net, SweepArgs<T> a, std::vector<CatchmentIO<T>> const& io, hipStream_t stream);
+
+    // Record: run round 0 (and, for the rounds schedule, the later rounds) while recording the schedule into
+    // `net`; the chain schedule is then built and the operation replayed.
+    template<int OP, typename T, int TW, int TH>
+    int record_run(Network& net, SweepArgs<T> a, std::vector<CatchmentIO<T>> const& io, hipStream_t stream,
+                   bool produce_output = true)
+    {
+        Workspace w;
+        if (int const st = workspace_init<T, TW, TH>(net, w, a, true, stream))
+            return st;
+        if (int const st = upload_io<T>(w, io, a, stream))
+            return st;
+        OpTimer timer;
+        bool const chains = net.schedule == schedule_chains;
+        if (!chains)
+        {
+            if (int const st = timer.start(stream))
+                return st;
+        }
+        hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH>), dim3(static_cast<unsigned>(net.ntiles)), dim3(256), 0,
+                           stream, a);
+        LUE_AMD_HIP(hipGetLastError());
+        if (int const st = timer.mark(stream))
+            return st;
+
+        net.round_offset.assign(1, 0);
+        std::uint32_t round = 1;
+        if (!chains)
+        {
+            std::vector<std::uint32_t> tiles_of_rounds;
+            unsigned const grid = static_cast<unsigned>(std::min<std::int64_t>(net.ntiles, 2048));
+            while (true)
+            {
+                // the tiles queued for this round
+                std::uint32_t const slot = round % 3;
+                std::uint32_t count = 0;
+                LUE_AMD_HIP(copy_to_host(&count, w.work_count.ptr + slot, 1, stream));
+                if (count == 0)
+                    break;
+                std::size_t const at = tiles_of_rounds.size();
+                tiles_of_rounds.resize(at + count);
+                LUE_AMD_HIP(hipMemcpyAsync(tiles_of_rounds.data() + at,
+                                           w.worklist.ptr + static_cast<std::int64_t>(slot) * net.ntiles,
+                                           sizeof(std::uint32_t) * count, hipMemcpyDeviceToHost, stream));
+                hipLaunchKernelGGL((sweep_next_round<OP, T, TW, TH>), dim3(std::min<unsigned>(grid, count)), dim3(256),
+                                   0, stream, a, round);
+                LUE_AMD_HIP(hipGetLastError());
+                net.round_offset.push_back(tiles_of_rounds.size());
+                ++round;
+                if (round > 0xFFFF0000u)
+                    return fail(status_invalid_network, "routing did not converge");
+            }
+            LUE_AMD_HIP(hipStreamSynchronize(stream));
+            LUE_AMD_HIP(net.round_tiles.allocate(tiles_of_rounds.size(), stream));
+            if (!tiles_of_rounds.empty())
+            {
+                LUE_AMD_HIP(hipMemcpyAsync(net.round_tiles.ptr, tiles_of_rounds.data(),
+                                           sizeof(std::uint32_t) * tiles_of_rounds.size(), hipMemcpyHostToDevice,
+                                           stream));
+            }
+            if (int const st = timer.stop(stream, round, net.ncells))
+                return st;
+        }
+        unsigned long long counters[2] = {0, 0};
+        LUE_AMD_HIP(copy_to_host(counters, w.counters.ptr, 2, stream));
         net.nodata_cells = static_cast<std::int64_t>(counters[1]);
-        std::uint64_t const total = counters[0] + counters[1] + tail_solved;
+        if (chains)
+        {
+            if (int const st = build_chains<TW, TH>(net, w.pending_bits.ptr, stream))
+                return st;
+            if (counters[0] + counters[1] + net.pending != static_cast<std::uint64_t>(net.ncells))
+                return fail(status_invalid_network, "invalid flow direction network: cells not accounted for");
+            net.recorded = true;
+            // round 0 of the recording did not stage the pending cells: run the operation on the schedule
+            return produce_output ? replay_run<OP, T, TW, TH>(net, a, io, stream) : status_success;
+        }
+        std::uint64_t const total = counters[0] + counters[1];
         if (total != static_cast<std::uint64_t>(net.ncells))
         {
             return fail(status_invalid_network,
@@ -584,44 +857,38 @@ namespace lue_amd {
             return st;
         if (int const st = upload_io<T>(w, io, a, stream))
             return st;
-        hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-        if (profile().enabled)
+        bool const chains = net.schedule == schedule_chains;
+        if (chains && net.pending > 0)
         {
-            for (auto& e : ev)
-                LUE_AMD_HIP(hipEventCreate(&e));
-            LUE_AMD_HIP(hipEventRecord(ev[0], stream));
+            if (int const st = chains_workspace<OP, T>(net, w, a, stream))
+                return st;
         }
+        OpTimer timer;
+        if (int const st = timer.start(stream))
+            return st;
         hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH>), dim3(static_cast<unsigned>(net.ntiles)), dim3(256), 0,
                            stream, a);
-        if (profile().enabled)
-            LUE_AMD_HIP(hipEventRecord(ev[1], stream));
-        for (int r = 1; r <= net.rounds(); ++r)
-        {
-            std::uint64_t const begin = net.round_offset[r - 1];
-            std::uint32_t const count = static_cast<std::uint32_t>(net.round_offset[r] - begin);
-            unsigned const grid = std::min<unsigned>(count, 2048);
-            hipLaunchKernelGGL((sweep_list_round<OP, T, TW, TH>), dim3(grid), dim3(256), 0, stream, a,
-                               net.round_tiles.ptr + begin, count, static_cast<std::uint32_t>(r));
-        }
         LUE_AMD_HIP(hipGetLastError());
-        if (int const st = launch_tail<OP, T>(net, w, a, stream, false))
+        if (int const st = timer.mark(stream))
             return st;
-        if (profile().enabled)
+        if (chains)
         {
-            LUE_AMD_HIP(hipEventRecord(ev[2], stream));
-            LUE_AMD_HIP(hipEventSynchronize(ev[2]));
-            float first = 0.0f, rest = 0.0f;
-            LUE_AMD_HIP(hipEventElapsedTime(&first, ev[0], ev[1]));
-            LUE_AMD_HIP(hipEventElapsedTime(&rest, ev[1], ev[2]));
-            profile().first_round_ms += first;
-            profile().next_rounds_ms += rest;
-            profile().calls += 1;
-            profile().rounds += net.rounds() + 1;
-            profile().cells += net.ncells;
-            for (auto& e : ev)
-                (void)hipEventDestroy(e);
+            if (int const st = launch_chains<OP, T, TW, TH>(net, w, a, io, stream))
+                return st;
         }
-        return status_success;
+        else
+        {
+            for (int r = 1; r <= net.rounds(); ++r)
+            {
+                std::uint64_t const begin = net.round_offset[r - 1];
+                std::uint32_t const count = static_cast<std::uint32_t>(net.round_offset[r] - begin);
+                unsigned const grid = std::min<unsigned>(count, 2048);
+                hipLaunchKernelGGL((sweep_list_round<OP, T, TW, TH>), dim3(grid), dim3(256), 0, stream, a,
+                                   net.round_tiles.ptr + begin, count, static_cast<std::uint32_t>(r));
+            }
+            LUE_AMD_HIP(hipGetLastError());
+        }
+        return timer.stop(stream, chains ? net.levels() + 1 : net.rounds() + 1, net.ncells);
     }
 
 }  // namespace lue_amd

@@ -8,7 +8,6 @@
 #include "sweep.hpp"
 #include "synthetic.hpp"
 #include "network.hpp"
-#include "tail.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -42,9 +41,8 @@ namespace lue_amd {
 
     namespace {
 
-        // Rounds of tiles before the chain-streaming tail takes over, for per-call entry points
-        // (lue_amd_set_sweep_options); -1: tile rounds only.
-        int default_tail_after = 8;
+        // Schedule of per-call entry points (lue_amd_set_sweep_options).
+        int default_schedule = schedule_chains;
 
         template<int OP, typename T, int TW, int TH>
         constexpr bool fits()
@@ -121,7 +119,7 @@ namespace lue_amd {
         {
             Network net;
             if (int const st = network_init(net, 1, &ldd, &rows, &cols, default_tile_shape<OP, T>(),
-                                            default_tail_after, stream))
+                                            default_schedule, stream))
                 return st;
             std::vector<CatchmentIO<T>> table{io};
             return with_tile_shape<OP, T>(net.tile_shape, [&]<int TW, int TH>()
@@ -309,11 +307,11 @@ const char* lue_amd_version(void)
     return "0.1.0";
 }
 
-int lue_amd_set_sweep_options(int tail_after)
+int lue_amd_set_sweep_options(int schedule)
 {
-    if (tail_after < -1)
-        return fail(status_invalid_argument, "tail_after must be -1 (tile rounds only) or >= 0");
-    default_tail_after = tail_after;
+    if (schedule < 0 || schedule >= schedule_count)
+        return fail(status_invalid_argument, "unknown schedule");
+    default_schedule = schedule;
     return status_success;
 }
 
@@ -514,18 +512,16 @@ struct lue_amd_network_s
 extern "C" {
 
 int lue_amd_network_create(int n, const uint8_t* const* ldd, const int64_t* rows, const int64_t* cols, int tile_shape,
-                           int tail_after, void* stream, lue_amd_network* network)
+                           int schedule, void* stream, lue_amd_network* network)
 {
     if (!network)
         return fail(status_invalid_argument, "network must not be null");
     *network = nullptr;
-    if (tail_after < -1)
-        return fail(status_invalid_argument, "tail_after must be -1 (tile rounds only) or >= 0");
     hipStream_t const s = static_cast<hipStream_t>(stream);
     auto handle = std::make_unique<lue_amd_network_s>();
     Network& net = handle->net;
     if (int const st = network_init(net, n, ldd, reinterpret_cast<std::int64_t const*>(rows),
-                                    reinterpret_cast<std::int64_t const*>(cols), tile_shape, tail_after, s))
+                                    reinterpret_cast<std::int64_t const*>(cols), tile_shape, schedule, s))
         return st;
     // record the schedule with accu(ldd, 1) into scratch (every operation fits every tile shape of accu f32)
     DeviceArray<float> scratch;
@@ -536,7 +532,8 @@ int lue_amd_network_create(int n, const uint8_t* const* ldd, const int64_t* rows
     Scalars<float> sc{};
     sc.in1_s = 1.0f;
     int const st = with_tile_shape<op_accu, float>(net.tile_shape, [&]<int TW, int TH>()
-                                                   { return record_run<op_accu, float, TW, TH>(net, args_from(sc), io, s); });
+                                                   { return record_run<op_accu, float, TW, TH>(net, args_from(sc), io, s,
+                                                                                               false); });
     if (st)
         return st;
     LUE_AMD_HIP(hipStreamSynchronize(s));
@@ -565,12 +562,14 @@ int lue_amd_network_info(lue_amd_network network, int64_t* info, int n)
                               net.ncells,
                               net.ntiles,
                               net.rounds(),
-                              net.tail_cells,
+                              net.pending,
                               net.nchains,
-                              net.nready,
+                              net.levels(),
                               net.nodata_cells,
                               net.tile_shape,
-                              net.tail_after};
+                              net.schedule,
+                              net.longest_chain,
+                              static_cast<int64_t>(net.nrefs)};
     for (int i = 0; i < n && i < static_cast<int>(sizeof(values) / sizeof(values[0])); ++i)
         info[i] = values[i];
     return status_success;

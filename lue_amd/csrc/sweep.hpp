@@ -92,6 +92,13 @@ namespace lue_amd {
         std::uint32_t* work_count;     // 3 counters; recording only
         unsigned long long* counters;  // [0] solved cells, [1] no-data cells; recording only
         int record;                    // queue tiles and count cells (schedule recording)
+
+        // Chain schedule, round 0 of a run: the inputs of the cells left pending are staged at their
+        // chain positions (null stage_off: no staging).
+        std::uint32_t const* stage_off;  // per tile (+1): first compact index of its pending cells
+        std::uint32_t const* stage_pos;  // per compact index (tile after tile, cell order): chain position
+        void* staged;                    // per chain position: CellInput<OP, T>
+        double* stage_side;              // per chain position: FP64 sum of the same-tile upstream cells solved
     };
 
     template<int TW, int TH>
@@ -172,6 +179,8 @@ namespace lue_amd {
         std::uint8_t ldd[G::hn];
         std::uint8_t upmask[G::n];
         std::uint8_t state[G::n];  // see state_* below
+        std::uint32_t pword[G::words];   // pending bitmap of the tile
+        std::uint32_t pprefix[G::words]; // pending cells before each bitmap word
         unsigned int solved;
         unsigned int no_data;
         unsigned int queued;
@@ -761,6 +770,7 @@ namespace lue_amd {
                 bits |= (s.state[w * 32 + b] == state_pending ? 1u : 0u) << b;
             }
             pbits[w] = bits;
+            s.pword[w] = bits;
         }
         if (a.record)
         {
@@ -774,6 +784,54 @@ namespace lue_amd {
             }
         }
         __syncthreads();
+
+        // 6. chain schedule: stage the inputs of the pending cells at their chain positions, with the FP64
+        // sum (neighbour order) of their upstream cells in this tile solved in this round
+        if (round == 0 && a.stage_off != nullptr)
+        {
+            if (tid == 0)
+            {
+                std::uint32_t sum = 0;
+                for (int w = 0; w < G::words; ++w)
+                {
+                    s.pprefix[w] = sum;
+                    sum += __popc(s.pword[w]);
+                }
+            }
+            __syncthreads();
+            std::uint32_t const begin = as_global(a.stage_off)[tile];
+            CellInput<OP, T>* const staged = static_cast<CellInput<OP, T>*>(a.staged);
+            for (int i = tid; i < G::n; i += nthreads)
+            {
+                if (s.state[i] != state_pending)
+                    continue;
+                std::uint32_t const word = s.pword[i >> 5];
+                std::uint32_t const b = static_cast<std::uint32_t>(i & 31);
+                std::uint32_t const rank = s.pprefix[i >> 5] + __popc(word & ((1u << b) - 1u));
+                std::uint32_t const pos = as_global(a.stage_pos)[begin + rank];
+                int const lr = i / TW;
+                int const lc = i % TW;
+                std::uint8_t const mask = s.upmask[i];
+                double side = 0.0;
+                for (int k = 0; k < 8; ++k)
+                {
+                    int const ur = lr + neighbour_dr(k);
+                    int const uc = lc + neighbour_dc(k);
+                    if (((mask >> k) & 1u) && ur >= 0 && ur < TH && uc >= 0 && uc < TW &&
+                        s.state[ur * TW + uc] == state_new)
+                    {
+                        int const u = ur * TW + uc;
+                        double v = static_cast<double>(s.hi[u]);
+                        if constexpr (compensated)
+                            v += static_cast<double>(s.lo[u]);
+                        side += v;
+                    }
+                }
+                staged[pos] = s.in[i];
+                as_global(a.stage_side)[pos] = side;
+            }
+            __syncthreads();
+        }
     }
 
     // Round 0: every tile of the batch. Grid = number of tiles.

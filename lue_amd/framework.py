@@ -233,11 +233,11 @@ class FlowNetwork:
 
     The schedule depends on the ldd only, so it is recorded once; every operation then runs as a fixed
     launch sequence over all catchments at once. The ldd arrays must not change while the network lives.
-    tile_shape: (width, height) of the sweep tiles, one of TILE_SHAPES; tail_after: tile rounds before the
-    chain-streaming tail (-1: tile rounds only).
+    tile_shape: (width, height) of the sweep tiles, one of TILE_SHAPES; schedule: "chains" (the cells the first
+    round of tiles leaves pending are solved along heavy-path chains) or "rounds" (further rounds of tiles).
     """
 
-    def __init__(self, flow_directions, tile_shape=(32, 32), tail_after=8):
+    def __init__(self, flow_directions, tile_shape=(32, 32), schedule="chains"):
         import ctypes
 
         if isinstance(flow_directions, PartitionedArray):
@@ -252,7 +252,8 @@ class FlowNetwork:
         cols = (ctypes.c_int64 * n)(*[a.shape[1] for a in self.flow_directions])
         handle = ctypes.c_void_p()
         self._handle = None
-        _lib.check(lib.lue_amd_network_create(n, ptrs, rows, cols, TILE_SHAPES[tuple(tile_shape)], tail_after,
+        _lib.check(lib.lue_amd_network_create(n, ptrs, rows, cols, TILE_SHAPES[tuple(tile_shape)],
+                                              _lib.SCHEDULES[schedule],
                                               _stream(), ctypes.byref(handle)))
         self._handle = handle
 
@@ -273,10 +274,10 @@ class FlowNetwork:
     def info(self):
         import ctypes
 
-        values = (ctypes.c_int64 * 10)()
-        _lib.check(_lib.load().lue_amd_network_info(self._handle, values, 10))
-        keys = ["catchments", "cells", "tiles", "rounds", "tail_cells", "chains", "ready_chains", "no_data_cells",
-                "tile_shape", "tail_after"]
+        keys = ["catchments", "cells", "tiles", "rounds", "pending_cells", "chains", "levels", "no_data_cells",
+                "tile_shape", "schedule", "longest_chain", "side_refs"]
+        values = (ctypes.c_int64 * len(keys))()
+        _lib.check(_lib.load().lue_amd_network_info(self._handle, values, len(keys)))
         return dict(zip(keys, list(values)))
 
     # -- argument helpers -------------------------------------------------------------------------

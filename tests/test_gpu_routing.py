@@ -253,9 +253,9 @@ def test_config1_kinematic_wave_2048(fw, oracle_mod):
 
 
 # ---------------------------------------------------------------------------------------------------
-# Tile rounds only vs chain-streaming tail: same results
+# Schedules after the first round of tiles (heavy-path chains, tile rounds): same results
 
-POLICIES = {"rounds_only": -1, "default": 8, "tail_early": 1, "tail_at_once": 0}
+POLICIES = {"chains": "chains", "rounds": "rounds"}
 
 
 @pytest.fixture()
@@ -263,7 +263,7 @@ def policy_lib(fw):
     from lue_amd import _lib
 
     yield _lib
-    _lib.set_sweep_options(POLICIES["default"])
+    _lib.set_sweep_options("chains")
 
 
 @pytest.mark.parametrize("policy", list(POLICIES))
