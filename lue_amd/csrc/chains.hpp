@@ -576,6 +576,134 @@ namespace lue_amd {
         return d;
     }
 
+    // Inclusive scan of affine maps along the lanes of a wavefront. Lane k holds x_k = add + mul x_{k-1};
+    // `flag` marks lanes that do not depend on the lanes before them (chain starts). On return
+    // x_k = add + mul x_in where x_in is the value before the first lane, or x_k = add where flag is set.
+    __device__ inline void affine_scan(double& add, double& mul, int& flag)
+    {
+        int const lane = threadIdx.x & 63;
+        for (int offset = 1; offset < 64; offset <<= 1)
+        {
+            double const a_prev = __shfl_up(add, offset, 64);
+            double const m_prev = __shfl_up(mul, offset, 64);
+            int const f_prev = __shfl_up(flag, offset, 64);
+            if (lane >= offset && !flag)
+            {
+                add = fma(mul, a_prev, add);
+                mul = mul * m_prev;
+                flag = f_prev;
+            }
+        }
+    }
+
+    // Kinematic wave on one block of chain positions, all lanes at once. Newton's method on the coupled
+    // system of the block
+    //     F_k(Q) = c1_k (Q_k - Q_{k-1} - extra_k) + alpha_k Q_k^beta_k - b_k = 0,
+    // Q_{k-1} being the previous position on the chain (`carry` before the first lane, nothing at a chain
+    // start); per cell this is LUE's equation with Q_up = Q_{k-1} + extra_k. The Jacobian is lower
+    // bidiagonal (d_k = c1_k + alpha_k beta_k Q_k^(beta_k - 1) on the diagonal, -c1_k below it), so a
+    // Newton step dQ_k = (-F_k + c1_k dQ_{k-1}) / d_k is one affine scan. The start is the system linearised
+    // at Q_cur, an affine recurrence as well. F is concave in every Q_k and the Jacobian inverse-positive,
+    // so the iterates approach the root from below after the first step. The iteration stops when every
+    // |dQ_k| <= |Q_k| 2^(1 - digits), LUE's criterion (digits 31 for double, 14 for float).
+    // T = float: the iterations use the hardware FP32 power (log2, exp2) and approximate reciprocals; one
+    // FP64 Newton step follows (the FP64 residual correction).
+    // Returns false, with q untouched, when a lane needs LUE's per-cell iteration: no-data, a zero current
+    // discharge (dry cells and cells whose iteration starts from numeric_limits::min()), a non-positive
+    // iterate, or no convergence within 16 steps.
+    template<typename T>
+    __device__ inline bool kinematic_wave_block(ChainLane<op_kinematic_wave, T> const& d, double carry, double& q)
+    {
+        int const lane = threadIdx.x & 63;
+        bool const valid = d.valid;
+        bool const start = valid && d.start;
+        double const q_cur = static_cast<double>(d.in.q_cur);
+        if (__ballot(valid && !(q_cur > 0.0 && !isnan(d.extra))))
+            return false;
+        double const c1 = d.in.c1;
+        double const alpha = static_cast<double>(d.in.alpha);
+        double const beta = static_cast<double>(d.in.beta);
+        double const b = d.in.b;
+        float const beta32 = static_cast<float>(d.in.beta);
+
+        // start: F linearised at Q_cur gives Q_k = w_k (Q_{k-1} + extra_k) + g_k
+        double add = 0.0;
+        double mul = 0.0;
+        int flag = (start || !valid) ? 1 : 0;
+        if (valid)
+        {
+            double const pc = static_cast<double>(fast_exp2(beta32 * fast_log2(static_cast<float>(q_cur))));
+            double const inv = 1.0 / (c1 + alpha * beta * pc / q_cur);
+            mul = c1 * inv;
+            add = mul * d.extra + (b - (1.0 - beta) * alpha * pc) * inv;
+        }
+        affine_scan(add, mul, flag);
+        double x = valid ? (flag ? add : fma(mul, carry, add)) : 1.0;
+        if (__ballot(valid && !(x > 0.0)))
+            return false;
+
+        constexpr double tol = std::is_same_v<T, double> ? 0x1p-30 : 0x1p-13;
+        bool converged = false;
+        for (int iteration = 0; iteration < 16 && !converged; ++iteration)
+        {
+            double prev = __shfl_up(x, 1, 64);
+            if (lane == 0)
+                prev = carry;
+            if (start)
+                prev = 0.0;
+            double xb;
+            double inv;
+            if constexpr (std::is_same_v<T, double>)
+            {
+                xb = dpow(x, beta);
+            }
+            else
+            {
+                xb = static_cast<double>(fast_exp2(beta32 * fast_log2(static_cast<float>(x))));
+            }
+            double const ax = alpha * xb;
+            double const f = c1 * (x - prev - d.extra) + ax - b;
+            double const fp = c1 + beta * ax / x;
+            if constexpr (std::is_same_v<T, double>)
+                inv = 1.0 / fp;
+            else
+                inv = static_cast<double>(fast_rcp(static_cast<float>(fp)));
+            add = valid ? -f * inv : 0.0;
+            mul = valid ? c1 * inv : 0.0;
+            flag = (start || !valid) ? 1 : 0;
+            affine_scan(add, mul, flag);
+            double const xn = x + add;  // the carry is exact: its step is 0
+            if (__ballot(valid && !(xn > 0.0)))
+                return false;
+            converged = __ballot(valid && !(fabs(add) <= fabs(xn) * tol)) == 0;
+            x = xn;
+        }
+        if (!converged)
+            return false;
+        if constexpr (std::is_same_v<T, float>)
+        {
+            // FP64 residual correction
+            double prev = __shfl_up(x, 1, 64);
+            if (lane == 0)
+                prev = carry;
+            if (start)
+                prev = 0.0;
+            double const ax = alpha * dpow(x, beta);
+            double const f = c1 * (x - prev - d.extra) + ax - b;
+            double const inv = 1.0 / (c1 + beta * ax / x);
+            add = valid ? -f * inv : 0.0;
+            mul = valid ? c1 * inv : 0.0;
+            flag = (start || !valid) ? 1 : 0;
+            affine_scan(add, mul, flag);
+            double const xn = x + add;
+            if (__ballot(valid && !(xn > 0.0)))
+                return false;
+            x = xn;
+        }
+        q = valid ? x : 0.0;
+        return true;
+    }
+
     // One level of chains. A wavefront per work item (4 per workgroup).
     template<int OP, typename T>
     __global__ void __launch_bounds__(256)
@@ -639,6 +767,11 @@ namespace lue_amd {
                     }
                     carry = out;
                 }
+            }
+            else if (kinematic_wave_block<T>(d, carry, value))
+            {
+                carry = read_lane(value, nb - 1);
+                have_warm = false;
             }
             else if constexpr (std::is_same_v<T, double>)
             {
