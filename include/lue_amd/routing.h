@@ -67,6 +67,14 @@ int lue_amd_profile_read(double* first_round_ms, double* next_rounds_ms, long lo
 /* Schedule of the per-call entry points (default LUE_AMD_SCHEDULE_CHAINS). */
 int lue_amd_set_sweep_options(int schedule);
 
+/*
+ * Per-call entry points keep the schedules they record (no reference counterpart; LUE recomputes the
+ * inflow counts in every call): up to `entries` of them (default 4, 0 disables), found again by device,
+ * ldd pointer, shape, tile shape, schedule and a 64-bit fingerprint of the ldd bytes computed in every
+ * call, so an ldd changed in place is recorded anew. The ldd must be 4-byte aligned while caching.
+ */
+int lue_amd_set_network_cache(int entries);
+
 /* Device memory used by the C++ front end's device-resident partitions. */
 int lue_amd_device_malloc(void** ptr, size_t bytes);
 int lue_amd_device_free(void* ptr);

@@ -27,6 +27,7 @@ SIGNATURES = {
     "lue_amd_last_error": [],
     "lue_amd_version": [],
     "lue_amd_set_sweep_options": [c_int],
+    "lue_amd_set_network_cache": [c_int],
     "lue_amd_network_create": [c_int, c_p, c_p, c_p, c_int, c_int, c_p, ctypes.POINTER(c_p)],
     "lue_amd_network_destroy": [c_p],
     "lue_amd_network_info": [c_p, c_p, c_int],
@@ -104,6 +105,11 @@ SCHEDULES = {"chains": 0, "rounds": 1}
 def set_sweep_options(schedule="chains"):
     """Schedule of the per-call operations after the first round of tiles: "chains" or "rounds"."""
     check(load().lue_amd_set_sweep_options(SCHEDULES[schedule]))
+
+
+def set_network_cache(entries=4):
+    """Schedules the per-call operations keep (0: record in every call)."""
+    check(load().lue_amd_set_network_cache(entries))
 
 
 def profile_enable(enable=True):

@@ -518,18 +518,27 @@ namespace lue_amd {
         std::uint32_t const* ref_off;     // per position (+1): side references
         std::uint64_t const* refs;
         CellInput<OP, T> const* staged;   // per position: inputs (staged by round 0)
-        double const* side;               // per position: FP64 sum of the same-tile upstream cells round 0 solved
+        double* side;                     // per position: FP64 sum of the upstream cells off the chain
         double* res;                      // per position: outflow (FP64)
         T* res_rem;                       // per position: remainder (threshold, fraction)
         int uniform_beta;                 // kinematic wave: beta is a scalar (Newton may start from the previous cell)
     };
 
+    // Add the side references of positions [pos_begin, pos_end) (one chain level) to their staged side sums, in
+    // reference order (neighbour order). The referenced values are final: raster cells solved by round 0,
+    // and chain cells of lower levels.
     template<int OP, typename T>
-    __device__ inline double side_total(SweepArgs<T> const& a, ChainRun<OP, T> const& c, std::uint32_t p)
+    __global__ void __launch_bounds__(256)
+        chains_gather_kernel(SweepArgs<T> a, ChainRun<OP, T> c, std::uint32_t pos_begin, std::uint32_t pos_end)
     {
-        double sum = as_global(c.side)[p];
+        std::uint32_t const p = pos_begin + blockIdx.x * blockDim.x + threadIdx.x;
+        if (p >= pos_end)
+            return;
         std::uint32_t const begin = as_global(c.ref_off)[p];
         std::uint32_t const end = as_global(c.ref_off)[p + 1];
+        if (begin == end)
+            return;
+        double sum = as_global(c.side)[p];
         for (std::uint32_t j = begin; j < end; ++j)
         {
             std::uint64_t const ref = as_global(c.refs)[j];
@@ -548,7 +557,7 @@ namespace lue_amd {
             }
             sum += v;
         }
-        return sum;
+        as_global(c.side)[p] = sum;
     }
 
     template<int OP, typename T>
@@ -570,7 +579,7 @@ namespace lue_amd {
         {
             std::uint32_t const p = start + k;
             d.start = as_global(c.seg_start)[p] != 0;
-            d.extra = side_total<OP, T>(a, c, p);
+            d.extra = as_global(c.side)[p];
             d.in = c.staged[p];
         }
         return d;
@@ -721,12 +730,15 @@ namespace lue_amd {
         float warm = 0.0f;   // kinematic wave FP32: an iterate of the previous cell, to start Newton from
         float warm_pow = 0.0f;
         bool have_warm = false;
+        // data of the next two blocks is in flight while a block runs
         ChainLane<OP, T> next = load_chain_lane<OP, T>(a, c, start, count, lane);
+        ChainLane<OP, T> next2 = load_chain_lane<OP, T>(a, c, start, count, 64 + lane);
         for (std::uint32_t block = 0; block < count; block += 64)
         {
             ChainLane<OP, T> const d = next;
-            if (block + 64 < count)
-                next = load_chain_lane<OP, T>(a, c, start, count, block + 64 + lane);
+            next = next2;
+            if (block + 128 < count)
+                next2 = load_chain_lane<OP, T>(a, c, start, count, block + 128 + lane);
             int const nb = static_cast<int>(count - block < 64 ? count - block : 64);
             double value = 0.0;
             T rem{};

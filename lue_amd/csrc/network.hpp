@@ -247,6 +247,7 @@ namespace lue_amd {
         std::uint32_t longest_chain{0};
         std::uint64_t nrefs{0};
         std::vector<std::uint32_t> level_items;  // work items of level l: [level_items[l], level_items[l+1])
+        std::vector<std::uint32_t> level_pos;    // positions of level l: [level_pos[l], level_pos[l+1])
         DeviceArray<std::uint32_t> tile_off;     // per tile (+1): first compact index of its pending cells
         DeviceArray<std::uint32_t> pos_of;       // per compact index: chain position
         DeviceArray<std::uint8_t> seg_start;     // per position
@@ -257,7 +258,54 @@ namespace lue_amd {
 
         auto rounds() const -> int { return static_cast<int>(round_offset.empty() ? 0 : round_offset.size() - 1); }
         auto levels() const -> int { return static_cast<int>(level_items.empty() ? 0 : level_items.size() - 1); }
+
+        // Free the buffers on the null stream from now on (the recording stream may not outlive the network;
+        // the owner synchronises the device before destroying it).
+        void detach_stream()
+        {
+            cats.stream = nullptr;
+            tile_cat.stream = nullptr;
+            round_tiles.stream = nullptr;
+            tile_off.stream = nullptr;
+            pos_of.stream = nullptr;
+            seg_start.stream = nullptr;
+            ref_off.stream = nullptr;
+            refs.stream = nullptr;
+            item_start.stream = nullptr;
+            item_count.stream = nullptr;
+        }
     };
+
+    // 64-bit fingerprint of n bytes: sum over 32-bit words of splitmix64(word + index * golden ratio)
+    // (order-sensitive, reduction-friendly). `bytes` must be 4-byte aligned; the tail is zero-padded.
+    __global__ inline void fingerprint_kernel(std::uint8_t const* bytes, std::uint64_t n, unsigned long long* out)
+    {
+        std::uint64_t const words = (n + 3) / 4;
+        std::uint64_t sum = 0;
+        for (std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < words;
+             i += static_cast<std::uint64_t>(gridDim.x) * blockDim.x)
+        {
+            std::uint32_t w;
+            if (4 * i + 4 <= n)
+            {
+                w = as_global(reinterpret_cast<std::uint32_t const*>(bytes))[i];
+            }
+            else
+            {
+                w = 0;
+                for (std::uint64_t j = 4 * i; j < n; ++j)
+                    w |= static_cast<std::uint32_t>(bytes[j]) << (8 * (j - 4 * i));
+            }
+            std::uint64_t z = static_cast<std::uint64_t>(w) + (i + 1) * 0x9E3779B97F4A7C15ull;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            sum += z ^ (z >> 31);
+        }
+        for (int offset = 32; offset > 0; offset >>= 1)
+            sum += __shfl_down(sum, offset, 64);
+        if ((threadIdx.x & 63) == 0)
+            atomicAdd(out, static_cast<unsigned long long>(sum));
+    }
 
     // Describe a batch: tiles are numbered catchment after catchment.
     inline int network_init(Network& net, int n, std::uint8_t const* const* ldd, std::int64_t const* rows,
@@ -462,6 +510,7 @@ namespace lue_amd {
         LUE_AMD_HIP(copy_to_host(&n, net.tile_off.ptr + ntiles, 1, stream));
         net.pending = n;
         net.level_items.assign(1, 0);
+        net.level_pos.assign(1, 0);
         if (n == 0)
             return status_success;
         if (n >= (1u << 30))
@@ -600,6 +649,7 @@ namespace lue_amd {
         }
         std::vector<std::uint32_t> h_item_start, h_item_count;
         net.level_items.assign(1, 0);
+        net.level_pos.assign(level_cells.begin(), level_cells.end());
         for (std::uint32_t l = 0; l < nlevels; ++l)
         {
             if (open[l].second > 0)
@@ -694,6 +744,13 @@ namespace lue_amd {
                           uniform_beta ? 1 : 0};
         for (int l = 0; l < net.levels(); ++l)
         {
+            std::uint32_t const pos_begin = net.level_pos[l];
+            std::uint32_t const pos_end = net.level_pos[l + 1];
+            if (pos_end > pos_begin)
+            {
+                hipLaunchKernelGGL((chains_gather_kernel<OP, T>), dim3(blocks_for(pos_end - pos_begin)), dim3(256), 0,
+                                   stream, a, c, pos_begin, pos_end);
+            }
             std::uint32_t const begin = net.level_items[l];
             std::uint32_t const end = net.level_items[l + 1];
             if (end == begin)

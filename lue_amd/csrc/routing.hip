@@ -13,6 +13,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <list>
+#include <memory>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -112,18 +115,118 @@ namespace lue_amd {
             return a;
         }
 
-        // One operation on one raster: record into a temporary network.
+        // Networks of the per-call entry points, most recently used first. A network is found again by
+        // device, ldd pointer, shape, tile shape, schedule and a fingerprint of the ldd bytes (computed in
+        // every call), so a changed ldd is recorded anew.
+        struct CacheKey
+        {
+            int device;
+            void const* ldd;
+            std::int64_t rows;
+            std::int64_t cols;
+            int tile_shape;
+            int schedule;
+            std::uint64_t fingerprint;
+
+            bool operator==(CacheKey const&) const = default;
+        };
+
+        struct NetworkCache
+        {
+            std::mutex mutex;
+            std::list<std::pair<CacheKey, std::shared_ptr<Network>>> entries;
+            int capacity = 4;
+
+            std::shared_ptr<Network> find(CacheKey const& key)
+            {
+                std::lock_guard<std::mutex> lock(mutex);
+                for (auto it = entries.begin(); it != entries.end(); ++it)
+                {
+                    if (it->first == key)
+                    {
+                        entries.splice(entries.begin(), entries, it);
+                        return it->second;
+                    }
+                }
+                return nullptr;
+            }
+
+            void insert(CacheKey const& key, std::shared_ptr<Network> net)
+            {
+                std::lock_guard<std::mutex> lock(mutex);
+                entries.emplace_front(key, std::move(net));
+                trim();
+            }
+
+            // Drop entries beyond the capacity. Their buffers are freed on the null stream after the
+            // device has finished all work, since a replay may still use them.
+            void trim()
+            {
+                if (static_cast<int>(entries.size()) <= capacity)
+                    return;
+                (void)hipDeviceSynchronize();
+                while (static_cast<int>(entries.size()) > capacity)
+                    entries.pop_back();
+            }
+        };
+
+        NetworkCache& network_cache()
+        {
+            // never destroyed: freeing device memory at process exit is not needed
+            static NetworkCache* cache = new NetworkCache;
+            return *cache;
+        }
+
+        int ldd_fingerprint(std::uint8_t const* ldd, std::int64_t n, hipStream_t stream, std::uint64_t& fingerprint)
+        {
+            if (reinterpret_cast<std::uintptr_t>(ldd) % 4 != 0)
+                return fail(status_invalid_argument, "ldd must be 4-byte aligned");
+            DeviceArray<unsigned long long> sum;
+            LUE_AMD_HIP(sum.allocate(1, stream));
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<std::uint32_t*>(sum.ptr),
+                               0u, 2ull);
+            unsigned const grid = static_cast<unsigned>(std::min<std::int64_t>((n / 4 + 255) / 256 + 1, 4096));
+            hipLaunchKernelGGL(fingerprint_kernel, dim3(grid), dim3(256), 0, stream, ldd, static_cast<std::uint64_t>(n),
+                               sum.ptr);
+            LUE_AMD_HIP(hipGetLastError());
+            unsigned long long value = 0;
+            LUE_AMD_HIP(copy_to_host(&value, sum.ptr, 1, stream));
+            fingerprint = value;
+            return status_success;
+        }
+
+        // One operation on one raster: replay a cached network of the ldd, or record one.
         template<int OP, typename T>
         int route_once(std::uint8_t const* ldd, std::int64_t rows, std::int64_t cols, Scalars<T> const& sc,
                        CatchmentIO<T> const& io, hipStream_t stream)
         {
-            Network net;
-            if (int const st = network_init(net, 1, &ldd, &rows, &cols, default_tile_shape<OP, T>(),
-                                            default_schedule, stream))
-                return st;
             std::vector<CatchmentIO<T>> table{io};
-            return with_tile_shape<OP, T>(net.tile_shape, [&]<int TW, int TH>()
-                                          { return record_run<OP, T, TW, TH>(net, args_from(sc), table, stream); });
+            int const shape = default_tile_shape<OP, T>();
+            NetworkCache& cache = network_cache();
+            CacheKey key{0, ldd, rows, cols, shape, default_schedule, 0};
+            bool const caching = cache.capacity > 0;
+            if (caching)
+            {
+                LUE_AMD_HIP(hipGetDevice(&key.device));
+                if (int const st = ldd_fingerprint(ldd, rows * cols, stream, key.fingerprint))
+                    return st;
+                if (std::shared_ptr<Network> net = cache.find(key))
+                {
+                    return with_tile_shape<OP, T>(net->tile_shape, [&]<int TW, int TH>()
+                                                  { return replay_run<OP, T, TW, TH>(*net, args_from(sc), table, stream); });
+                }
+            }
+            auto net = std::make_shared<Network>();
+            if (int const st = network_init(*net, 1, &ldd, &rows, &cols, shape, default_schedule, stream))
+                return st;
+            int const st = with_tile_shape<OP, T>(net->tile_shape, [&]<int TW, int TH>()
+                                                  { return record_run<OP, T, TW, TH>(*net, args_from(sc), table, stream); });
+            if (st == status_success && caching)
+            {
+                net->detach_stream();
+                cache.insert(key, std::move(net));
+            }
+            return st;
         }
 
         // One operation on a recorded network.
@@ -312,6 +415,16 @@ int lue_amd_set_sweep_options(int schedule)
     if (schedule < 0 || schedule >= schedule_count)
         return fail(status_invalid_argument, "unknown schedule");
     default_schedule = schedule;
+    return status_success;
+}
+
+int lue_amd_set_network_cache(int entries)
+{
+    if (entries < 0)
+        return fail(status_invalid_argument, "entries must be >= 0");
+    NetworkCache& cache = network_cache();
+    cache.capacity = entries;
+    cache.trim();
     return status_success;
 }
 
@@ -548,6 +661,7 @@ int lue_amd_network_destroy(lue_amd_network network)
         int device = 0;
         (void)hipGetDevice(&device);
         (void)hipDeviceSynchronize();
+        network->net.detach_stream();
         delete network;
     }
     return status_success;

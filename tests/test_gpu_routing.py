@@ -327,3 +327,26 @@ def test_cycles_are_reported_with_every_policy(fw, policy_lib, policy):
     ldd[80, 70], ldd[80, 71], ldd[81, 71], ldd[81, 70] = 2, 2, 2, 2
     ones = fw.accu(pa(fw, ldd), 1.0).numpy()
     assert ones[-1].sum() == ldd.size
+
+
+def test_per_call_network_cache_follows_ldd_changes(fw, oracle_mod):
+    """Per-call operations keep their recorded schedule, keyed by a fingerprint of the ldd bytes."""
+    from lue_amd import _lib, synthetic
+
+    ldd = synthetic.ldd_host(21, 300, 310)
+    material = synthetic.uniform_host(21, synthetic.STREAM_MATERIAL, 0.0, 1.0, (300, 310))
+    arr = pa(fw, ldd)
+    mat = pa(fw, material)
+    first = fw.accu(arr, mat).numpy()   # records
+    second = fw.accu(arr, mat).numpy()  # replays the kept schedule
+    assert np.array_equal(first, second)
+    # change the ldd in place: the bottom half drains to pits
+    arr.data[150:, :] = 5
+    changed = fw.accu(arr, mat).numpy()
+    want = oracle_mod.accu(arr.numpy(), material.astype(np.float64), None, np.float64)
+    assert rel_err(changed, want).max() <= 1e-6
+    try:
+        _lib.set_network_cache(0)
+        assert np.array_equal(fw.accu(arr, mat).numpy(), changed)
+    finally:
+        _lib.set_network_cache(4)
