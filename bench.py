@@ -1,23 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark of the routing hot path: accu + kinematic_wave on a 16384 x 16384 synthetic catchment.
 
-One step = accu(ldd, material) followed by one kinematic_wave(ldd, Q_cur, q, alpha, beta, dt, dx)
-timestep over the whole raster, on one GPU, inputs resident in HBM:
+One step = accu(ldd, material) and one kinematic_wave(ldd, Q_cur, q, alpha, beta, dt, dx) timestep over the
+whole raster, on one GPU, inputs resident in HBM:
   - ldd from the seeded DEM e(r,c) = (R - r) + 50 |sin(pi c / 256)| + 0.9 u (sqrt(2)-weighted D8,
     bottom row outlets), material U[0,1), Q_cur U[0,5), q U[0,1e-4), alpha 1.5 + U[0,1), beta 0.6,
     dt 3600 s, dx 100 m (the kinematic-wave configuration scaled to 16384^2);
   - value = raster cells per second through both operations, FP32 path (compensated sums, FP32 Newton
-    plus one FP64 correction); the plain FP64 drop-in is timed beside it.
+    plus one FP64 correction) on flow networks recorded once; the plain FP64 path and the per-call
+    entry points are timed beside it.
 
-Prints one JSON line. The CPU baseline is the oracle (C++ restatement of LUE's algorithm), timed on a
-bounded sample on the host cores.
+Prints one JSON line. The roofline is that of the dominant kernel (the first round of the tiled sweep of
+the kinematic wave): algorithmic bytes per launch / its HIP-event time, and its HBM traffic per launch
+from two rocprofv3 --pmc runs of this script (FETCH_SIZE, WRITE_SIZE), calibrated with a copy kernel of
+known size. The CPU baseline is the oracle (C++ restatement of LUE's algorithm) on the same inputs on the
+host cores.
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -34,9 +43,12 @@ BYTES_KW_F32 = 1 + 4 + 4 + 4 + 4
 BYTES_ACCU_F64 = 1 + 8 + 8
 BYTES_KW_F64 = 1 + 8 + 8 + 8 + 8
 
+KERNELS = {"kinematic_wave_f32": "sweep_first_round<3, float", "accu_f32": "sweep_first_round<0, float"}
+COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read, 1 GiB written
+
 
 def parse():
-    p = argparse.ArgumentParser(description=__doc__)
+    p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
@@ -48,6 +60,8 @@ def parse():
     p.add_argument("--accu-tile", default="32x32", help="sweep tile of the accu network (WxH)")
     p.add_argument("--kw-tile", default="32x32", help="sweep tile of the kinematic-wave network (WxH)")
     p.add_argument("--schedule", default="chains", choices=["chains", "rounds"])
+    p.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic runs")
+    p.add_argument("--probe", action="store_true", help=argparse.SUPPRESS)  # child run under rocprofv3 --pmc
     return p.parse_args()
 
 
@@ -55,47 +69,137 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def tile(text):
+    return tuple(int(v) for v in text.split("x"))
+
+
+def make_inputs(n, dev, torch):
+    from lue_amd import framework as fw
+    from lue_amd import synthetic as syn
+
+    ps = (1024, 1024) if n >= 1024 else (n, n)
+    ldd = fw.PartitionedArray(syn.ldd_device(SEED, n, n, dev), ps)
+
+    def field(stream, lo, hi):
+        return fw.PartitionedArray(syn.uniform_device(SEED, stream, lo, hi, (n, n), torch.float32, dev), ps)
+
+    f32 = dict(material=field(syn.STREAM_MATERIAL, 0.0, 1.0), q_cur=field(syn.STREAM_CURRENT, 0.0, 5.0),
+               q_lat=field(syn.STREAM_LATERAL, 0.0, 1e-4), alpha=field(syn.STREAM_ALPHA, 1.5, 2.5))
+    return ps, ldd, f32
+
+
+def probe(args):
+    """Child run under rocprofv3 --pmc: each operation twice on recorded networks, then the copy kernel."""
+    import torch
+
+    from lue_amd import framework as fw
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    _, ldd, f32 = make_inputs(args.size, dev, torch)
+    net_accu = fw.FlowNetwork(ldd, tile_shape=tile(args.accu_tile), schedule=args.schedule)
+    net_kw = fw.FlowNetwork(ldd, tile_shape=tile(args.kw_tile), schedule=args.schedule)
+    for _ in range(2):
+        net_accu.accu(f32["material"])
+        net_kw.kinematic_wave(f32["q_cur"], f32["q_lat"], f32["alpha"], 0.6, 3600.0, 100.0)
+    torch.cuda.synchronize()
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print("[probe] done", flush=True)
+
+
+def read_counters(directory, counter):
+    """(kernel name, value) per dispatch, in dispatch order, from rocprofv3's counter CSV."""
+    files = glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True)
+    rows = []
+    for name in files:
+        with open(name, newline="") as f:
+            for r in csv.DictReader(f):
+                if r.get("Counter_Name") == counter:
+                    rows.append((int(r.get("Dispatch_Id", 0)), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return [(k, v) for _, k, v in rows]
+
+
+def measure_traffic(args):
+    """HBM bytes per dispatch of the first-round kernels from two rocprofv3 --pmc runs of --probe."""
+    rocprof = shutil.which("rocprofv3")
+    if rocprof is None or args.no_traffic:
+        return None
+    values = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out = tempfile.mkdtemp(prefix="lue_amd_pmc_", dir="/tmp")
+        cmd = [rocprof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pmc", "--", sys.executable,
+               os.path.abspath(__file__), "--probe", "--size", str(args.size), "--accu-tile", args.accu_tile,
+               "--kw-tile", args.kw_tile, "--schedule", args.schedule]
+        log(f"[bench] traffic: rocprofv3 --pmc {counter}")
+        env = dict(os.environ, TMPDIR="/tmp")
+        try:
+            run = subprocess.run(cmd, cwd="/tmp", env=env, timeout=300, capture_output=True, text=True)
+        except subprocess.TimeoutExpired:
+            log(f"[bench] traffic: {counter} run timed out")
+            return None
+        rows = read_counters(out, counter)
+        shutil.rmtree(out, ignore_errors=True)
+        if run.returncode != 0 or not rows:
+            log(f"[bench] traffic: {counter} run failed (rc {run.returncode}): {run.stderr[-400:]}")
+            return None
+        copies = [v for k, v in rows if "elementwise" in k]
+        if not copies:
+            return None
+        values[counter] = dict(copy_kib=copies[-1])
+        for key, pattern in KERNELS.items():
+            found = [v for k, v in rows if pattern in k]
+            if found:
+                values[counter][key] = found[-1]
+    result = {}
+    for key in KERNELS:
+        if key not in values["FETCH_SIZE"] or key not in values["WRITE_SIZE"]:
+            continue
+        f_factor = COPY_BYTES / (values["FETCH_SIZE"]["copy_kib"] * 1024.0)
+        w_factor = COPY_BYTES / (values["WRITE_SIZE"]["copy_kib"] * 1024.0)
+        fetch = values["FETCH_SIZE"][key] * 1024.0 * f_factor
+        write = values["WRITE_SIZE"][key] * 1024.0 * w_factor
+        result[key] = dict(bytes=fetch + write, fetch_bytes=fetch, write_bytes=write, fetch_factor=f_factor,
+                           write_factor=w_factor)
+    return result
+
+
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
-                                   "(see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.probe:
+        probe(args)
+        return
+
+    # traffic runs first, as child processes, before this process touches the GPU
+    traffic = measure_traffic(args)
 
     import torch
 
     from lue_amd import _lib
     from lue_amd import framework as fw
-    from lue_amd import synthetic as syn
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     n = args.size
-    ps = (1024, 1024) if n >= 1024 else (n, n)
-    shape = (n, n)
     cells = n * n
-
     log(f"[bench] generating {n}x{n} inputs on the device")
-    ldd = fw.PartitionedArray(syn.ldd_device(SEED, n, n, dev), ps)
-
-    def field(stream, lo, hi, dtype):
-        return fw.PartitionedArray(syn.uniform_device(SEED, stream, lo, hi, shape, dtype, dev), ps)
-
-    f32 = dict(material=field(syn.STREAM_MATERIAL, 0.0, 1.0, torch.float32),
-               q_cur=field(syn.STREAM_CURRENT, 0.0, 5.0, torch.float32),
-               q_lat=field(syn.STREAM_LATERAL, 0.0, 1e-4, torch.float32),
-               alpha=field(syn.STREAM_ALPHA, 1.5, 2.5, torch.float32))
+    ps, ldd, f32 = make_inputs(n, dev, torch)
     f64 = {k: fw.PartitionedArray(v.data.double(), ps) for k, v in f32.items()}
     torch.cuda.synchronize()
 
-    # networks: the sweep schedule of the ldd, recorded once (tile shape per operation)
+    # networks: the sweep schedule of the ldd, recorded once
     log("[bench] recording flow networks")
     t0 = time.perf_counter()
-    accu_tile = tuple(int(v) for v in args.accu_tile.split("x"))
-    kw_tile = tuple(int(v) for v in args.kw_tile.split("x"))
-    net_accu = fw.FlowNetwork(ldd, tile_shape=accu_tile, schedule=args.schedule)
-    net_kw = fw.FlowNetwork(ldd, tile_shape=kw_tile, schedule=args.schedule)
+    net_accu = fw.FlowNetwork(ldd, tile_shape=tile(args.accu_tile), schedule=args.schedule)
+    net_kw = fw.FlowNetwork(ldd, tile_shape=tile(args.kw_tile), schedule=args.schedule)
     torch.cuda.synchronize()
     t_network = time.perf_counter() - t0
     s_accu = torch.cuda.Stream()
@@ -110,28 +214,22 @@ def main():
                                       out=None if out is None else out[1])
         return acc, q
 
-    def sync():
-        s_accu.synchronize()
-        s_kw.synchronize()
-
     def timed(inp, steps, warmup):
         out = step(inp)
         for _ in range(warmup):
             step(inp, out)
-        sync()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
             step(inp, out)
-        sync()
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps, out
 
-    # FP32 path -----------------------------------------------------------------------------------
+    # FP32 path ------------------------------------------------------------------------------------
     log(f"[bench] fp32 timed {args.steps} steps")
     t32, out32 = timed(f32, args.steps, args.warmup)
 
-    # per-kernel timing of each operation with HIP events on its launch stream
+    # per-operation kernel time with HIP events on the launch stream: the first round, and the rest
     kernel_ms = {}
     for name, fn in (("accu_f32", lambda: net_accu.accu(f32["material"], out=out32[0])),
                      ("kinematic_wave_f32", lambda: net_kw.kinematic_wave(f32["q_cur"], f32["q_lat"], f32["alpha"],
@@ -140,35 +238,38 @@ def main():
         for _ in range(args.steps):
             fn()
         torch.cuda.synchronize()
-        first, rest, calls, rounds, _ = _lib.profile_read()
+        first, rest, calls, _, _ = _lib.profile_read()
         _lib.profile_enable(False)
         kernel_ms[name] = dict(first_round_ms=first / calls, after_first_round_ms=rest / calls,
-                               total_ms=(first + rest) / calls, rounds=rounds / calls)
+                               total_ms=(first + rest) / calls)
 
-    # drop-in per-call path (records its schedule in every call)
-    log("[bench] per-call fp32 step")
-    fw.accu(ldd, f32["material"])
+    # drop-in per-call path (keeps the schedules it records, keyed by an ldd fingerprint) --------------
+    log("[bench] per-call steps")
+
+    def per_call(inp):
+        fw.accu(ldd, inp["material"])
+        fw.kinematic_wave(ldd, inp["q_cur"], inp["q_lat"], inp["alpha"], 0.6, 3600.0, 100.0)
+
+    per_call(f32)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.fp64_steps):
-        fw.accu(ldd, f32["material"])
-        fw.kinematic_wave(ldd, f32["q_cur"], f32["q_lat"], f32["alpha"], 0.6, 3600.0, 100.0)
+        per_call(f32)
     torch.cuda.synchronize()
     t32_call = (time.perf_counter() - t0) / args.fp64_steps
 
-    # FP64 drop-in --------------------------------------------------------------------------------
+    # FP64 path ------------------------------------------------------------------------------------
     log("[bench] fp64 steps")
-    t64, out64 = timed(f64, args.fp64_steps, 0)
+    t64, out64 = timed(f64, args.fp64_steps, 1)
+    per_call(f64)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.fp64_steps):
-        fw.accu(ldd, f64["material"])
-        fw.kinematic_wave(ldd, f64["q_cur"], f64["q_lat"], f64["alpha"], 0.6, 3600.0, 100.0)
+        per_call(f64)
     torch.cuda.synchronize()
     t64_call = (time.perf_counter() - t0) / args.fp64_steps
-    info_kw = net_kw.info()
-    info_accu = net_accu.info()
 
-    # accuracy of the FP32 path against the FP64 drop-in on the same inputs
+    # accuracy of the FP32 path against the FP64 path on the same inputs
     def rel(a, b):
         a = a.data.double()
         b = b.data
@@ -178,19 +279,21 @@ def main():
 
     acc_err = rel(out32[0], out64[0])
     kw_err = rel(out32[1], out64[1])
+    info_kw = net_kw.info()
+    info_accu = net_accu.info()
 
     value = cells / t32 / 1e6
     value64 = cells / t64 / 1e6
 
-    # roofline of the dominant kernel: the all-tile round of the kinematic-wave sweep
-    kw = kernel_ms["kinematic_wave_f32"]
-    ac = kernel_ms["accu_f32"]
-    dominant = "kinematic_wave_f32" if kw["first_round_ms"] >= ac["first_round_ms"] else "accu_f32"
-    dom_bytes = cells * (BYTES_KW_F32 if dominant == "kinematic_wave_f32" else BYTES_ACCU_F32)
+    # roofline of the dominant kernel: the first round of the sweep of one operation
+    dominant = max(kernel_ms, key=lambda k: kernel_ms[k]["first_round_ms"])
+    per_cell = BYTES_KW_F32 if dominant == "kinematic_wave_f32" else BYTES_ACCU_F32
     dom_ms = kernel_ms[dominant]["first_round_ms"]
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    achieved = cells * per_cell / (dom_ms * 1e-3) / 1e9
+    dom_traffic = traffic.get(dominant) if traffic else None
+    dom_tile = args.kw_tile if dominant == "kinematic_wave_f32" else args.accu_tile
 
-    # CPU baseline: the oracle on the same inputs, copied to the host ---------------------------------
+    # CPU baseline: the oracle on the same inputs, copied to the host --------------------------------
     cpu = None
     if args.cpu_seconds > 0:
         from oracle import oracle
@@ -232,18 +335,21 @@ def main():
                                f"{ps[0]}x{ps[1]} partitions, beta 0.6, dt 3600 s, dx 100 m",
                    "cells": cells, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"sweep_first_round ({dominant})"},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(dom_traffic["bytes"]) if dom_traffic else None,
+                     "kernel": f"{KERNELS[dominant]}, {dom_tile.replace('x', ', ')}>",
+                     "algorithmic_bytes": cells * per_cell, "kernel_ms": round(dom_ms, 3)},
         "cpu_baseline": cpu,
         "fp64_dropin": {"value": round(value64, 1), "unit": "Mcells/s", "ms_per_step": round(t64 * 1e3, 3),
                         "fp32_speedup": round(value / value64, 3),
                         "per_call_ms_per_step": round(t64_call * 1e3, 3)},
         "fp32_per_call": {"value": round(cells / t32_call / 1e6, 1), "ms_per_step": round(t32_call * 1e3, 3),
-                          "note": "drop-in entry points, schedule recorded inside every call"},
+                          "note": "drop-in entry points (schedule kept per ldd fingerprint)"},
         "network": {"record_ms": round(t_network * 1e3, 1), "kinematic_wave": info_kw, "accu": info_accu},
         "accuracy_vs_fp64": {"accu_max_rel": acc_err[0], "accu_mean_rel": acc_err[1],
                              "kinematic_wave_max_rel": kw_err[0], "kinematic_wave_mean_rel": kw_err[1]},
         "kernels": kernel_ms,
+        "traffic": traffic,
         "step_hbm_gbs": round(cells * (BYTES_ACCU_F32 + BYTES_KW_F32) / t32 / 1e9, 1),
     }
     print(json.dumps(result), flush=True)

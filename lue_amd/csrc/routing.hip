@@ -77,17 +77,13 @@ namespace lue_amd {
             return fail(status_invalid_argument, "tile shape does not fit this operation in LDS");
         }
 
-        // Tile shape of per-call entry points: the largest one whose LDS footprint leaves two workgroups
-        // per CU.
+        // Tile shape of per-call entry points. With the chain schedule the first round's time is set by the
+        // latency of a tile's walks and by how many tiles fit a CU at once, which favours the smallest tiles
+        // (32x32 measured fastest for every operation at 16384^2 cells).
         template<int OP, typename T>
         constexpr int default_tile_shape()
         {
-            if constexpr (sizeof(SweepLds<OP, T, 64, 64>) <= 80 * 1024)
-                return tile_64x64;
-            else if constexpr (sizeof(SweepLds<OP, T, 64, 32>) <= 80 * 1024)
-                return tile_64x32;
-            else
-                return tile_32x32;
+            return tile_32x32;
         }
 
         // Scalars of an operation (rasters come per catchment in CatchmentIO).
