@@ -1,0 +1,272 @@
+// Per-cell definitions shared by the tiled sweep (sweep.hpp) and the chain phase (chains.hpp): the
+// catchments of a batch, the arguments of one operation, the per-cell inputs and the cell functions of
+// LUE's accumulating router (reference: source/framework/algorithm/include/lue/framework/algorithm/
+// definition/{accu,accu_threshold,accu_fraction,kinematic_wave}.hpp).
+#pragma once
+
+#include "kinematic_wave.hpp"
+#include "routing_common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <limits>
+#include <type_traits>
+
+namespace lue_amd {
+
+    constexpr std::uint32_t stamp_pending = 0xFFFFFFFFu;
+
+    // One raster of a batch.
+    struct Catchment
+    {
+        std::uint8_t const* ldd;
+        std::int64_t rows;
+        std::int64_t cols;
+        int tiles_x;
+        int tiles_y;
+        std::int64_t tile_base;  // global id of the first tile
+        std::int64_t cell_base;  // offset of the catchment's cells in per-cell scratch of the batch
+    };
+
+    // Inputs and outputs of one catchment (raster pointers; nullptr: the scalar of SweepArgs).
+    template<typename T>
+    struct CatchmentIO
+    {
+        T const* in1;  // accu*: inflow        kinematic wave: current outflow
+        T const* in2;  // threshold | fraction | kinematic wave: lateral inflow
+        T const* alpha;
+        T const* beta;
+        T const* dx;
+        T* out;        // outflow
+        T* rem;        // remainder (threshold, fraction)
+    };
+
+    template<typename T>
+    struct SweepArgs
+    {
+        Catchment const* cats;
+        CatchmentIO<T> const* io;
+        std::uint32_t const* tile_cat;  // catchment of every global tile
+        int ncats;
+        std::int64_t ntiles;
+
+        T in1_s;
+        T in2_s;
+        T alpha_s;
+        T beta_s;
+        T dx_s;
+        T dt;
+
+        float* out_lo;  // compensation of `out` (float path) by global cell, for cells read across rounds
+
+        std::uint32_t* pending_bits;   // per tile: TW*TH/32 words, bit set = cell not solved yet
+        std::uint32_t* border_stamp;   // per tile: perimeter cells, round in which the cell was solved
+        std::uint32_t* tile_flag;      // 3 x tiles: queued for round r (index r % 3); recording only
+        std::uint32_t* worklist;       // 3 x tiles; recording only
+        std::uint32_t* work_count;     // 3 counters; recording only
+        unsigned long long* counters;  // [0] solved cells, [1] no-data cells; recording only
+        int record;                    // queue tiles and count cells (schedule recording)
+
+        // Chain schedule, round 0 of a run: the inputs of the cells left pending are staged at their
+        // chain positions (null stage_off: no staging).
+        std::uint32_t const* stage_off;  // per tile (+1): first compact index of its pending cells
+        std::uint32_t const* stage_pos;  // per compact index (tile after tile, cell order): chain position
+        void* staged;                    // per chain position: CellInput<OP, T>
+        double* stage_side;              // per chain position: FP64 sum of the same-tile upstream cells solved
+
+        // Chain schedule, round 0 of a run: in-tile chain layout (tile_layout.hpp; null tile_order: walks)
+        std::uint16_t const* tile_order;
+        std::uint8_t const* tile_flags;
+        std::uint16_t const* tile_split;
+        std::uint8_t const* tile_nlevels;
+    };
+
+    template<int TW, int TH>
+    struct Tile
+    {
+        static constexpr int n = TW * TH;
+        static constexpr int hw = TW + 2;
+        static constexpr int hn = (TW + 2) * (TH + 2);
+        static constexpr int perimeter = 2 * TW + 2 * (TH - 2);
+        static constexpr int words = n / 32;
+        static_assert(n % 32 == 0, "tile must hold a multiple of 32 cells");
+        static_assert(TW >= 2 && TH >= 2, "tile too small");
+
+        // Index of a border cell on the tile perimeter, -1 for interior cells.
+        __host__ __device__ static int perimeter_index(int lr, int lc)
+        {
+            if (lr == 0)
+                return lc;
+            if (lr == TH - 1)
+                return TW + lc;
+            if (lc == 0)
+                return 2 * TW + (lr - 1);
+            if (lc == TW - 1)
+                return 2 * TW + (TH - 2) + (lr - 1);
+            return -1;
+        }
+    };
+
+    // Per-cell inputs staged in LDS.
+    template<int OP, typename T>
+    struct CellInput;
+
+    template<typename T>
+    struct CellInput<op_accu, T>
+    {
+        T x;  // external inflow, NaN when no-data or outside the domain
+    };
+
+    template<typename T>
+    struct CellInput<op_threshold, T>
+    {
+        T x;  // NaN when either argument is no-data or outside the domain
+        T p;
+    };
+
+    template<typename T>
+    struct CellInput<op_fraction, T>
+    {
+        T x;
+        T p;
+    };
+
+    template<typename T>
+    struct CellInput<op_kinematic_wave, T>
+    {
+        T q_cur;    // NaN when any argument is no-data or outside the domain
+        T q_lat;
+        T alpha;
+        T beta;
+        double b;   // alpha q_cur^beta + dt q_lat, FP64
+        double c1;  // dt / dx, FP64
+    };
+
+
+    template<typename T>
+    __device__ inline T value_of(T const* raster, T scalar, std::int64_t idx)
+    {
+        return raster ? as_global(raster)[idx] : scalar;
+    }
+
+    template<typename T>
+    __device__ inline T quiet_nan()
+    {
+        return std::numeric_limits<T>::quiet_NaN();
+    }
+
+    template<int OP, typename T>
+    __device__ inline void stage_input(SweepArgs<T> const& a, CatchmentIO<T> const& io, std::int64_t g,
+                                       CellInput<OP, T>& c)
+    {
+        if constexpr (OP == op_accu)
+        {
+            T const x = value_of(io.in1, a.in1_s, g);
+            // value_policies/accu.hpp:9-19: inflow >= 0
+            c.x = (x >= T{0}) ? x : quiet_nan<T>();
+        }
+        else if constexpr (OP == op_threshold || OP == op_fraction)
+        {
+            T const x = value_of(io.in1, a.in1_s, g);
+            T const p = value_of(io.in2, a.in2_s, g);
+            // value_policies/accu_threshold.hpp:9-20, value_policies/accu_fraction.hpp:9-20
+            bool const ok = OP == op_threshold ? (x >= T{0} && p >= T{0})
+                                               : (x >= T{0} && p >= T{0} && p <= T{1});
+            c.x = ok ? x : quiet_nan<T>();
+            c.p = p;
+        }
+        else
+        {
+            T const q_cur = value_of(io.in1, a.in1_s, g);
+            T const q_lat = value_of(io.in2, a.in2_s, g);
+            T const alpha = value_of(io.alpha, a.alpha_s, g);
+            T const beta = value_of(io.beta, a.beta_s, g);
+            T const dx = value_of(io.dx, a.dx_s, g);
+            // value_policies/kinematic_wave.hpp:9-22 plus no-data (NaN) in any argument
+            bool const ok = q_cur >= T{0} && q_lat >= T{0} && dx > T{0} && !isnan(alpha) && !isnan(beta) &&
+                            !isnan(a.dt);
+            c.q_cur = ok ? q_cur : quiet_nan<T>();
+            c.q_lat = q_lat;
+            c.alpha = alpha;
+            c.beta = beta;
+            double const dt = static_cast<double>(a.dt);
+            c.c1 = dt / static_cast<double>(dx);
+            c.b = ok ? static_cast<double>(alpha) * pow(static_cast<double>(q_cur), static_cast<double>(beta)) +
+                           dt * static_cast<double>(q_lat)
+                     : 0.0;
+        }
+    }
+
+    // Solve one cell given the FP64 sum of its upstream outflows.
+    template<int OP, typename T>
+    __device__ inline void solve_cell(
+        SweepArgs<T> const& a, CellInput<OP, T> const& in, double upstream, bool poisoned, double& out, T& rem)
+    {
+        if constexpr (OP == op_accu)
+        {
+            // accu.hpp:102-123: out = sum(upstream) + external inflow
+            out = (poisoned || isnan(in.x)) ? quiet_nan<double>() : upstream + static_cast<double>(in.x);
+        }
+        else if constexpr (OP == op_threshold)
+        {
+            // accu_threshold.hpp:124-169
+            if (poisoned || isnan(in.x) || isnan(in.p))
+            {
+                out = quiet_nan<double>();
+                rem = quiet_nan<T>();
+            }
+            else
+            {
+                double const total = upstream + static_cast<double>(in.x);
+                double const thr = static_cast<double>(in.p);
+                if (total > thr)
+                {
+                    rem = in.p;
+                    out = total - thr;
+                }
+                else
+                {
+                    rem = static_cast<T>(total);
+                    out = 0.0;
+                }
+            }
+        }
+        else if constexpr (OP == op_fraction)
+        {
+            // accu_fraction.hpp:137-162
+            if (poisoned || isnan(in.x) || isnan(in.p))
+            {
+                out = quiet_nan<double>();
+                rem = quiet_nan<T>();
+            }
+            else
+            {
+                double const total = upstream + static_cast<double>(in.x);
+                double const delta = static_cast<double>(in.p) * total;
+                rem = static_cast<T>(total - delta);
+                out = delta;
+            }
+        }
+        else
+        {
+            // kinematic_wave.hpp:416-456
+            if (poisoned || isnan(in.q_cur))
+            {
+                out = quiet_nan<double>();
+            }
+            else if constexpr (std::is_same_v<T, double>)
+            {
+                KinematicWaveCell<double> cell{upstream, in.q_cur, in.q_lat, in.alpha, in.beta, a.dt, in.c1, in.b};
+                out = new_discharge_f64(cell);
+            }
+            else
+            {
+                out = new_discharge_f32_path(upstream, in.q_cur, in.q_lat, in.alpha, in.beta, a.dt, in.b, in.c1);
+            }
+        }
+    }
+
+
+}  // namespace lue_amd

@@ -255,6 +255,11 @@ namespace lue_amd {
         DeviceArray<std::uint64_t> refs;
         DeviceArray<std::uint32_t> item_start;
         DeviceArray<std::uint32_t> item_count;
+        // in-tile chain layout of round 0 (tile_layout.hpp)
+        DeviceArray<std::uint16_t> tile_order;
+        DeviceArray<std::uint8_t> tile_flags;
+        DeviceArray<std::uint16_t> tile_split;
+        DeviceArray<std::uint8_t> tile_nlevels;
 
         auto rounds() const -> int { return static_cast<int>(round_offset.empty() ? 0 : round_offset.size() - 1); }
         auto levels() const -> int { return static_cast<int>(level_items.empty() ? 0 : level_items.size() - 1); }
@@ -273,6 +278,10 @@ namespace lue_amd {
             refs.stream = nullptr;
             item_start.stream = nullptr;
             item_count.stream = nullptr;
+            tile_order.stream = nullptr;
+            tile_flags.stream = nullptr;
+            tile_split.stream = nullptr;
+            tile_nlevels.stream = nullptr;
         }
     };
 
@@ -415,6 +424,10 @@ namespace lue_amd {
         a.stage_pos = nullptr;
         a.staged = nullptr;
         a.stage_side = nullptr;
+        a.tile_order = nullptr;
+        a.tile_flags = nullptr;
+        a.tile_split = nullptr;
+        a.tile_nlevels = nullptr;
         return status_success;
     }
 
@@ -703,6 +716,30 @@ namespace lue_amd {
         return status_success;
     }
 
+    // Record the in-tile chain layout of round 0 from the pending bitmaps.
+    template<int TW, int TH>
+    int build_tile_layout(Network& net, std::uint32_t const* pending_bits, hipStream_t stream)
+    {
+        using G = Tile<TW, TH>;
+        std::size_t const ntiles = static_cast<std::size_t>(net.ntiles);
+        LUE_AMD_HIP(net.tile_order.allocate(ntiles * G::n, stream));
+        LUE_AMD_HIP(net.tile_flags.allocate(ntiles * G::n, stream));
+        LUE_AMD_HIP(net.tile_split.allocate(ntiles * tile_split_entries, stream));
+        LUE_AMD_HIP(net.tile_nlevels.allocate(ntiles, stream));
+        DeviceArray<std::uint32_t> error;
+        LUE_AMD_HIP(error.allocate(1, stream));
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, error.ptr, 0u, 1ull);
+        hipLaunchKernelGGL((tile_layout_kernel<TW, TH>), dim3(static_cast<unsigned>(ntiles)), dim3(256), 0, stream,
+                           net.cats.ptr, net.tile_cat.ptr, pending_bits, net.tile_order.ptr, net.tile_flags.ptr,
+                           net.tile_split.ptr, net.tile_nlevels.ptr, error.ptr);
+        LUE_AMD_HIP(hipGetLastError());
+        std::uint32_t errors = 0;
+        LUE_AMD_HIP(copy_to_host(&errors, error.ptr, 1, stream));
+        if (errors != 0)
+            return fail(status_hip_error, "in-tile chain layout: too many levels");
+        return status_success;
+    }
+
     // Per-call buffers of the chain schedule and the staging arguments of round 0.
     template<int OP, typename T>
     int chains_workspace(Network const& net, Workspace& w, SweepArgs<T>& a, hipStream_t stream)
@@ -886,6 +923,8 @@ namespace lue_amd {
         {
             if (int const st = build_chains<TW, TH>(net, w.pending_bits.ptr, stream))
                 return st;
+            if (int const st = build_tile_layout<TW, TH>(net, w.pending_bits.ptr, stream))
+                return st;
             if (counters[0] + counters[1] + net.pending != static_cast<std::uint64_t>(net.ncells))
                 return fail(status_invalid_network, "invalid flow direction network: cells not accounted for");
             net.recorded = true;
@@ -919,6 +958,13 @@ namespace lue_amd {
         {
             if (int const st = chains_workspace<OP, T>(net, w, a, stream))
                 return st;
+        }
+        if (chains)
+        {
+            a.tile_order = net.tile_order.ptr;
+            a.tile_flags = net.tile_flags.ptr;
+            a.tile_split = net.tile_split.ptr;
+            a.tile_nlevels = net.tile_nlevels.ptr;
         }
         OpTimer timer;
         if (int const st = timer.start(stream))

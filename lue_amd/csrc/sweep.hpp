@@ -28,6 +28,9 @@
 // long flow paths. Kinematic wave: FP32 Newton + one FP64 correction (kinematic_wave.hpp).
 #pragma once
 
+#include "block_solve.hpp"
+#include "cells.hpp"
+#include "tile_layout.hpp"
 #include "kinematic_wave.hpp"
 #include "routing_common.hpp"
 
@@ -39,128 +42,6 @@
 #include <type_traits>
 
 namespace lue_amd {
-
-    constexpr std::uint32_t stamp_pending = 0xFFFFFFFFu;
-
-    // One raster of a batch.
-    struct Catchment
-    {
-        std::uint8_t const* ldd;
-        std::int64_t rows;
-        std::int64_t cols;
-        int tiles_x;
-        int tiles_y;
-        std::int64_t tile_base;  // global id of the first tile
-        std::int64_t cell_base;  // offset of the catchment's cells in per-cell scratch of the batch
-    };
-
-    // Inputs and outputs of one catchment (raster pointers; nullptr: the scalar of SweepArgs).
-    template<typename T>
-    struct CatchmentIO
-    {
-        T const* in1;  // accu*: inflow        kinematic wave: current outflow
-        T const* in2;  // threshold | fraction | kinematic wave: lateral inflow
-        T const* alpha;
-        T const* beta;
-        T const* dx;
-        T* out;        // outflow
-        T* rem;        // remainder (threshold, fraction)
-    };
-
-    template<typename T>
-    struct SweepArgs
-    {
-        Catchment const* cats;
-        CatchmentIO<T> const* io;
-        std::uint32_t const* tile_cat;  // catchment of every global tile
-        int ncats;
-        std::int64_t ntiles;
-
-        T in1_s;
-        T in2_s;
-        T alpha_s;
-        T beta_s;
-        T dx_s;
-        T dt;
-
-        float* out_lo;  // compensation of `out` (float path) by global cell, for cells read across rounds
-
-        std::uint32_t* pending_bits;   // per tile: TW*TH/32 words, bit set = cell not solved yet
-        std::uint32_t* border_stamp;   // per tile: perimeter cells, round in which the cell was solved
-        std::uint32_t* tile_flag;      // 3 x tiles: queued for round r (index r % 3); recording only
-        std::uint32_t* worklist;       // 3 x tiles; recording only
-        std::uint32_t* work_count;     // 3 counters; recording only
-        unsigned long long* counters;  // [0] solved cells, [1] no-data cells; recording only
-        int record;                    // queue tiles and count cells (schedule recording)
-
-        // Chain schedule, round 0 of a run: the inputs of the cells left pending are staged at their
-        // chain positions (null stage_off: no staging).
-        std::uint32_t const* stage_off;  // per tile (+1): first compact index of its pending cells
-        std::uint32_t const* stage_pos;  // per compact index (tile after tile, cell order): chain position
-        void* staged;                    // per chain position: CellInput<OP, T>
-        double* stage_side;              // per chain position: FP64 sum of the same-tile upstream cells solved
-    };
-
-    template<int TW, int TH>
-    struct Tile
-    {
-        static constexpr int n = TW * TH;
-        static constexpr int hw = TW + 2;
-        static constexpr int hn = (TW + 2) * (TH + 2);
-        static constexpr int perimeter = 2 * TW + 2 * (TH - 2);
-        static constexpr int words = n / 32;
-        static_assert(n % 32 == 0, "tile must hold a multiple of 32 cells");
-        static_assert(TW >= 2 && TH >= 2, "tile too small");
-
-        // Index of a border cell on the tile perimeter, -1 for interior cells.
-        __host__ __device__ static int perimeter_index(int lr, int lc)
-        {
-            if (lr == 0)
-                return lc;
-            if (lr == TH - 1)
-                return TW + lc;
-            if (lc == 0)
-                return 2 * TW + (lr - 1);
-            if (lc == TW - 1)
-                return 2 * TW + (TH - 2) + (lr - 1);
-            return -1;
-        }
-    };
-
-    // Per-cell inputs staged in LDS.
-    template<int OP, typename T>
-    struct CellInput;
-
-    template<typename T>
-    struct CellInput<op_accu, T>
-    {
-        T x;  // external inflow, NaN when no-data or outside the domain
-    };
-
-    template<typename T>
-    struct CellInput<op_threshold, T>
-    {
-        T x;  // NaN when either argument is no-data or outside the domain
-        T p;
-    };
-
-    template<typename T>
-    struct CellInput<op_fraction, T>
-    {
-        T x;
-        T p;
-    };
-
-    template<typename T>
-    struct CellInput<op_kinematic_wave, T>
-    {
-        T q_cur;    // NaN when any argument is no-data or outside the domain
-        T q_lat;
-        T alpha;
-        T beta;
-        double b;   // alpha q_cur^beta + dt q_lat, FP64
-        double c1;  // dt / dx, FP64
-    };
 
     template<int OP, typename T, int TW, int TH>
     struct SweepLds
@@ -187,134 +68,12 @@ namespace lue_amd {
         unsigned int taken;
     };
 
+
     // Cell states inside a round.
     constexpr std::uint8_t state_absent = 0;   // outside the array or ldd no-data
     constexpr std::uint8_t state_pending = 1;  // not solved yet
     constexpr std::uint8_t state_solved = 2;   // solved in an earlier round (value in HBM)
     constexpr std::uint8_t state_new = 3;      // solved in this round (value in LDS)
-
-    template<typename T>
-    __device__ inline T value_of(T const* raster, T scalar, std::int64_t idx)
-    {
-        return raster ? as_global(raster)[idx] : scalar;
-    }
-
-    template<typename T>
-    __device__ inline T quiet_nan()
-    {
-        return std::numeric_limits<T>::quiet_NaN();
-    }
-
-    template<int OP, typename T>
-    __device__ inline void stage_input(SweepArgs<T> const& a, CatchmentIO<T> const& io, std::int64_t g,
-                                       CellInput<OP, T>& c)
-    {
-        if constexpr (OP == op_accu)
-        {
-            T const x = value_of(io.in1, a.in1_s, g);
-            // value_policies/accu.hpp:9-19: inflow >= 0
-            c.x = (x >= T{0}) ? x : quiet_nan<T>();
-        }
-        else if constexpr (OP == op_threshold || OP == op_fraction)
-        {
-            T const x = value_of(io.in1, a.in1_s, g);
-            T const p = value_of(io.in2, a.in2_s, g);
-            // value_policies/accu_threshold.hpp:9-20, value_policies/accu_fraction.hpp:9-20
-            bool const ok = OP == op_threshold ? (x >= T{0} && p >= T{0})
-                                               : (x >= T{0} && p >= T{0} && p <= T{1});
-            c.x = ok ? x : quiet_nan<T>();
-            c.p = p;
-        }
-        else
-        {
-            T const q_cur = value_of(io.in1, a.in1_s, g);
-            T const q_lat = value_of(io.in2, a.in2_s, g);
-            T const alpha = value_of(io.alpha, a.alpha_s, g);
-            T const beta = value_of(io.beta, a.beta_s, g);
-            T const dx = value_of(io.dx, a.dx_s, g);
-            // value_policies/kinematic_wave.hpp:9-22 plus no-data (NaN) in any argument
-            bool const ok = q_cur >= T{0} && q_lat >= T{0} && dx > T{0} && !isnan(alpha) && !isnan(beta) &&
-                            !isnan(a.dt);
-            c.q_cur = ok ? q_cur : quiet_nan<T>();
-            c.q_lat = q_lat;
-            c.alpha = alpha;
-            c.beta = beta;
-            double const dt = static_cast<double>(a.dt);
-            c.c1 = dt / static_cast<double>(dx);
-            c.b = ok ? static_cast<double>(alpha) * pow(static_cast<double>(q_cur), static_cast<double>(beta)) +
-                           dt * static_cast<double>(q_lat)
-                     : 0.0;
-        }
-    }
-
-    // Solve one cell given the FP64 sum of its upstream outflows.
-    template<int OP, typename T>
-    __device__ inline void solve_cell(
-        SweepArgs<T> const& a, CellInput<OP, T> const& in, double upstream, bool poisoned, double& out, T& rem)
-    {
-        if constexpr (OP == op_accu)
-        {
-            // accu.hpp:102-123: out = sum(upstream) + external inflow
-            out = (poisoned || isnan(in.x)) ? quiet_nan<double>() : upstream + static_cast<double>(in.x);
-        }
-        else if constexpr (OP == op_threshold)
-        {
-            // accu_threshold.hpp:124-169
-            if (poisoned || isnan(in.x) || isnan(in.p))
-            {
-                out = quiet_nan<double>();
-                rem = quiet_nan<T>();
-            }
-            else
-            {
-                double const total = upstream + static_cast<double>(in.x);
-                double const thr = static_cast<double>(in.p);
-                if (total > thr)
-                {
-                    rem = in.p;
-                    out = total - thr;
-                }
-                else
-                {
-                    rem = static_cast<T>(total);
-                    out = 0.0;
-                }
-            }
-        }
-        else if constexpr (OP == op_fraction)
-        {
-            // accu_fraction.hpp:137-162
-            if (poisoned || isnan(in.x) || isnan(in.p))
-            {
-                out = quiet_nan<double>();
-                rem = quiet_nan<T>();
-            }
-            else
-            {
-                double const total = upstream + static_cast<double>(in.x);
-                double const delta = static_cast<double>(in.p) * total;
-                rem = static_cast<T>(total - delta);
-                out = delta;
-            }
-        }
-        else
-        {
-            // kinematic_wave.hpp:416-456
-            if (poisoned || isnan(in.q_cur))
-            {
-                out = quiet_nan<double>();
-            }
-            else if constexpr (std::is_same_v<T, double>)
-            {
-                KinematicWaveCell<double> cell{upstream, in.q_cur, in.q_lat, in.alpha, in.beta, a.dt, in.c1, in.b};
-                out = new_discharge_f64(cell);
-            }
-            else
-            {
-                out = new_discharge_f32_path(upstream, in.q_cur, in.q_lat, in.alpha, in.beta, a.dt, in.b, in.c1);
-            }
-        }
-    }
 
     // One round over one tile. All threads of the workgroup call it.
     template<int OP, typename T, int TW, int TH>
@@ -426,218 +185,125 @@ namespace lue_amd {
         }
         __syncthreads();
 
-        // 3. counts of unsolved upstream neighbours; the initially ready cells are compacted into an LDS
-        // queue (wavefront ballot + prefix count, one LDS atomic per wavefront)
-        int const lane = tid & 63;
-        std::uint64_t const lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-        for (int j = 0; j < cells_per_thread; ++j)
+        bool const layout = round == 0 && a.tile_order != nullptr;
+        if (layout)
         {
-            int const i = tid + j * nthreads;
-            bool is_ready = false;
-            if (i < G::n)
+            // 3-4 (recorded in-tile chains, tile_layout.hpp): level after level; wavefront w solves the
+            // positions [split[l][w], split[l][w + 1]) in blocks of 64. The light upstream cells of a chain
+            // lie in lower levels, the heavy one is the previous position of the chain.
+            auto const order = as_global(a.tile_order + tile * G::n);
+            auto const flags = as_global(a.tile_flags + tile * G::n);
+            auto const split = as_global(a.tile_split + tile * tile_split_entries);
+            int const levels = min(static_cast<int>(as_global(a.tile_nlevels)[tile]), tile_max_levels);
+            static_assert((G::n & (G::n - 1)) == 0, "tile cell count must be a power of two");
+            int const wave = tid >> 6;
+            int const lane_id = tid & 63;
+            bool const uniform_beta = io.beta == nullptr;
+            for (int l = 0; l < levels; ++l)
             {
-                std::uint32_t n = 0;
-                if (s.state[i] == state_pending)
+                int const lo = split[l * 5 + wave];
+                int const hi = split[l * 5 + wave + 1];
+                BlockState st;
+                for (int blk = lo; blk < hi; blk += 64)
                 {
-                    int const lr = i / TW;
-                    int const lc = i % TW;
-                    std::uint8_t const mask = s.upmask[i];
-                    for (int k = 0; k < 8; ++k)
+                    int const p = blk + lane_id;
+                    int const nb = hi - blk < 64 ? hi - blk : 64;
+                    ChainLane<OP, T> d{};
+                    int i = 0;
+                    d.valid = p < hi;
+                    if (d.valid)
                     {
-                        if (!((mask >> k) & 1u))
-                            continue;
-                        int const ur = lr + neighbour_dr(k);
-                        int const uc = lc + neighbour_dc(k);
-                        if (ur >= 0 && ur < TH && uc >= 0 && uc < TW)
+                        i = order[p] & (G::n - 1);  // in bounds whatever the recorded data
+                        std::uint8_t const f = flags[p];
+                        d.start = (f & 1u) != 0;
+                        int const heavy = (f & 16u) ? ((f >> 1) & 7) : -1;
+                        int const lr = i / TW;
+                        int const lc = i % TW;
+                        std::uint8_t const mask = s.upmask[i];
+                        double extra = 0.0;
+                        for (int k = 0; k < 8; ++k)
                         {
-                            n += s.state[ur * TW + uc] == state_pending ? 1u : 0u;
+                            int const ur = lr + neighbour_dr(k);
+                            int const uc = lc + neighbour_dc(k);
+                            if (((mask >> k) & 1u) && k != heavy && ur >= 0 && ur < TH && uc >= 0 && uc < TW)
+                            {
+                                int const u = ur * TW + uc;
+                                double v = static_cast<double>(s.hi[u]);
+                                if constexpr (compensated)
+                                    v += static_cast<double>(s.lo[u]);
+                                extra += v;
+                            }
                         }
-                        else if (round == 0)
-                        {
-                            ++n;
-                        }
-                        else
-                        {
-                            // upstream neighbour in another tile: solved when stamped in an earlier round
-                            int const nty = ty + (ur < 0 ? -1 : (ur >= TH ? 1 : 0));
-                            int const ntx = tx + (uc < 0 ? -1 : (uc >= TW ? 1 : 0));
-                            int const nlr = ur < 0 ? TH - 1 : (ur >= TH ? 0 : ur);
-                            int const nlc = uc < 0 ? TW - 1 : (uc >= TW ? 0 : uc);
-                            std::int64_t const ntile =
-                                cat.tile_base + static_cast<std::int64_t>(nty) * cat.tiles_x + ntx;
-                            std::uint32_t const st =
-                                as_global(a.border_stamp)[ntile * G::perimeter + G::perimeter_index(nlr, nlc)];
-                            n += st < round ? 0u : 1u;
-                        }
+                        d.extra = extra;
+                        d.in = s.in[i];
                     }
-                    is_ready = n == 0;
-                }
-                // 8-bit counter, written as a byte of the packed word
-                reinterpret_cast<std::uint8_t*>(s.count)[i] = static_cast<std::uint8_t>(n);
-            }
-            std::uint64_t const ballot = __ballot(is_ready);
-            if (ballot)
-            {
-                unsigned int base = 0;
-                if (lane == 0)
-                    base = atomicAdd(&s.queued, static_cast<unsigned int>(__popcll(ballot)));
-                base = __shfl(base, 0, 64);
-                if (is_ready)
-                    s.queue[base + __popcll(ballot & lanes_below)] = static_cast<std::uint16_t>(i);
-            }
-        }
-        __syncthreads();
-
-        // Sum (FP64, neighbour order) of the outflow of the upstream neighbours of tile cell i: solved in
-        // this round (LDS) or earlier (HBM).
-        auto gather = [&](int i, double& upstream, bool& poisoned)
-        {
-            int const lr = i / TW;
-            int const lc = i % TW;
-            std::uint8_t const mask = s.upmask[i];
-            upstream = 0.0;
-            poisoned = false;
-            for (int k = 0; k < 8; ++k)
-            {
-                if (!((mask >> k) & 1u))
-                    continue;
-                int const ur = lr + neighbour_dr(k);
-                int const uc = lc + neighbour_dc(k);
-                bool const inside = ur >= 0 && ur < TH && uc >= 0 && uc < TW;
-                double v;
-                // in round 0 every upstream cell of a ready cell lies in the tile and was solved now
-                if (inside && (round == 0 || s.state[ur * TW + uc] == state_new))
-                {
-                    int const u = ur * TW + uc;
-                    v = static_cast<double>(s.hi[u]);
-                    if constexpr (compensated)
-                        v += static_cast<double>(s.lo[u]);
-                }
-                else
-                {
-                    std::int64_t const ug = (r0 + ur) * cols + (c0 + uc);
-                    v = static_cast<double>(g_out[ug]);
-                    if constexpr (compensated)
-                        v += static_cast<double>(out_lo[ug]);
-                }
-                if (isnan(v))
-                    poisoned = true;
-                else
-                    upstream += v;
-            }
-        };
-
-        // Kinematic wave FP32: the walks solve the FP32 roots only; the FP64 correction of every cell is
-        // computed afterwards in parallel and propagated along the flow paths of the tile (first order).
-        constexpr bool deferred = OP == op_kinematic_wave && compensated;
-
-        // 4. walks: lanes take initially ready cells from the queue and walk downstream while they are the
-        // last upstream contributor of the next cell
-        unsigned int const queued = s.queued;
-        while (true)
-        {
-            unsigned int const item = atomicAdd(&s.taken, 1u);
-            if (item >= queued)
-                break;
-            int i = s.queue[item];
-            bool have_warm = false;
-            float warm_x = 0.0f;
-            float warm_pow = 0.0f;
-            while (true)
-            {
-                int const next = s.next[i];
-                double upstream;
-                bool poisoned;
-                gather(i, upstream, poisoned);
-                if constexpr (deferred)
-                {
-                    CellInput<OP, T> const& in = s.in[i];
-                    float q;
-                    if (poisoned || isnan(in.q_cur))
-                    {
-                        q = __int_as_float(0x7fc00000);
-                        have_warm = false;
-                    }
-                    else
-                    {
-                        have_warm = have_warm && io.beta == nullptr;
-                        q = kinematic_wave_root_f32(static_cast<float>(in.c1), in.alpha, in.beta, a.dt, upstream,
-                                                    in.q_cur, in.q_lat, in.b, in.c1, have_warm, warm_x, warm_pow);
-                    }
-                    s.hi[i] = q;
-                    s.lo[i] = 0.0f;
-                }
-                else
-                {
-                    double out;
+                    double value;
                     T rem{};
-                    solve_cell<OP, T>(a, s.in[i], upstream, poisoned, out, rem);
-                    T const hi = static_cast<T>(out);
-                    s.hi[i] = hi;
-                    if constexpr (compensated)
-                        s.lo[i] = isnan(out) ? 0.0f : static_cast<float>(out - static_cast<double>(hi));
-                    if constexpr (two_outputs)
-                        s.rem[i] = rem;
+                    solve_block<OP, T>(a, d, nb, uniform_beta, st, value, rem);
+                    if (d.valid)
+                    {
+                        T const h = static_cast<T>(value);
+                        s.hi[i] = h;
+                        if constexpr (compensated)
+                            s.lo[i] = isnan(value) ? 0.0f : static_cast<float>(value - static_cast<double>(h));
+                        if constexpr (two_outputs)
+                            s.rem[i] = rem;
+                        s.state[i] = state_new;
+                    }
                 }
-                // publish: the state write is ordered before the counter update (release)
-                __hip_atomic_store(&s.state[i], state_new, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (next < 0)
-                    break;
-                std::uint32_t const shift = 8u * static_cast<std::uint32_t>(next & 3);
-                std::uint32_t const old = __hip_atomic_fetch_sub(
-                    &s.count[next >> 2], 1u << shift, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (((old >> shift) & 0xFFu) != 1u)
-                    break;  // confluence: another stream still has to arrive
-                i = next;
+                __syncthreads();
             }
         }
-        __syncthreads();
-
-        if constexpr (deferred)
+        else
         {
-            // 4b. per cell solved in this round: eps = one FP64 Newton step at the FP32 root, sens = d root /
-            // d upstream; stored in place of the no longer needed inputs (b <- eps, alpha <- sens). Count the
-            // upstream cells solved in this round for the propagation walk.
-            if (tid == 0)
-            {
-                s.queued = 0;
-                s.taken = 0;
-            }
-            __syncthreads();
+            // 3. counts of unsolved upstream neighbours; the initially ready cells are compacted into an LDS
+            // queue (wavefront ballot + prefix count, one LDS atomic per wavefront)
+            int const lane = tid & 63;
+            std::uint64_t const lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
             for (int j = 0; j < cells_per_thread; ++j)
             {
                 int const i = tid + j * nthreads;
                 bool is_ready = false;
-                if (i < G::n && s.state[i] == state_new)
+                if (i < G::n)
                 {
-                    CellInput<OP, T>& in = s.in[i];
-                    double const q = static_cast<double>(s.hi[i]);
-                    double eps = 0.0;
-                    double sens = 0.0;
-                    if (q > 0.0 && !isnan(q))
-                    {
-                        double upstream;
-                        bool poisoned;
-                        gather(i, upstream, poisoned);
-                        kinematic_wave_correction(q, upstream, static_cast<double>(in.alpha), static_cast<double>(in.beta),
-                                                  in.b, in.c1, eps, sens);
-                    }
-                    in.b = eps;
-                    in.alpha = static_cast<float>(sens);
-                    int const lr = i / TW;
-                    int const lc = i % TW;
-                    std::uint8_t const mask = s.upmask[i];
                     std::uint32_t n = 0;
-                    for (int k = 0; k < 8; ++k)
+                    if (s.state[i] == state_pending)
                     {
-                        int const ur = lr + neighbour_dr(k);
-                        int const uc = lc + neighbour_dc(k);
-                        if (((mask >> k) & 1u) && ur >= 0 && ur < TH && uc >= 0 && uc < TW &&
-                            s.state[ur * TW + uc] == state_new)
-                            ++n;
+                        int const lr = i / TW;
+                        int const lc = i % TW;
+                        std::uint8_t const mask = s.upmask[i];
+                        for (int k = 0; k < 8; ++k)
+                        {
+                            if (!((mask >> k) & 1u))
+                                continue;
+                            int const ur = lr + neighbour_dr(k);
+                            int const uc = lc + neighbour_dc(k);
+                            if (ur >= 0 && ur < TH && uc >= 0 && uc < TW)
+                            {
+                                n += s.state[ur * TW + uc] == state_pending ? 1u : 0u;
+                            }
+                            else if (round == 0)
+                            {
+                                ++n;
+                            }
+                            else
+                            {
+                                // upstream neighbour in another tile: solved when stamped in an earlier round
+                                int const nty = ty + (ur < 0 ? -1 : (ur >= TH ? 1 : 0));
+                                int const ntx = tx + (uc < 0 ? -1 : (uc >= TW ? 1 : 0));
+                                int const nlr = ur < 0 ? TH - 1 : (ur >= TH ? 0 : ur);
+                                int const nlc = uc < 0 ? TW - 1 : (uc >= TW ? 0 : uc);
+                                std::int64_t const ntile =
+                                    cat.tile_base + static_cast<std::int64_t>(nty) * cat.tiles_x + ntx;
+                                std::uint32_t const st =
+                                    as_global(a.border_stamp)[ntile * G::perimeter + G::perimeter_index(nlr, nlc)];
+                                n += st < round ? 0u : 1u;
+                            }
+                        }
+                        is_ready = n == 0;
                     }
+                    // 8-bit counter, written as a byte of the packed word
                     reinterpret_cast<std::uint8_t*>(s.count)[i] = static_cast<std::uint8_t>(n);
-                    is_ready = n == 0;
                 }
                 std::uint64_t const ballot = __ballot(is_ready);
                 if (ballot)
@@ -651,61 +317,226 @@ namespace lue_amd {
                 }
             }
             __syncthreads();
-            // 4c. propagate: delta = eps + sens * sum(delta of upstream cells solved in this round); delta is
-            // kept in place of c1
-            unsigned int const queued2 = s.queued;
+
+            // Sum (FP64, neighbour order) of the outflow of the upstream neighbours of tile cell i: solved in
+            // this round (LDS) or earlier (HBM).
+            auto gather = [&](int i, double& upstream, bool& poisoned)
+            {
+                int const lr = i / TW;
+                int const lc = i % TW;
+                std::uint8_t const mask = s.upmask[i];
+                upstream = 0.0;
+                poisoned = false;
+                for (int k = 0; k < 8; ++k)
+                {
+                    if (!((mask >> k) & 1u))
+                        continue;
+                    int const ur = lr + neighbour_dr(k);
+                    int const uc = lc + neighbour_dc(k);
+                    bool const inside = ur >= 0 && ur < TH && uc >= 0 && uc < TW;
+                    double v;
+                    // in round 0 every upstream cell of a ready cell lies in the tile and was solved now
+                    if (inside && (round == 0 || s.state[ur * TW + uc] == state_new))
+                    {
+                        int const u = ur * TW + uc;
+                        v = static_cast<double>(s.hi[u]);
+                        if constexpr (compensated)
+                            v += static_cast<double>(s.lo[u]);
+                    }
+                    else
+                    {
+                        std::int64_t const ug = (r0 + ur) * cols + (c0 + uc);
+                        v = static_cast<double>(g_out[ug]);
+                        if constexpr (compensated)
+                            v += static_cast<double>(out_lo[ug]);
+                    }
+                    if (isnan(v))
+                        poisoned = true;
+                    else
+                        upstream += v;
+                }
+            };
+
+            // Kinematic wave FP32: the walks solve the FP32 roots only; the FP64 correction of every cell is
+            // computed afterwards in parallel and propagated along the flow paths of the tile (first order).
+            constexpr bool deferred = OP == op_kinematic_wave && compensated;
+
+            // 4. walks: lanes take initially ready cells from the queue and walk downstream while they are the
+            // last upstream contributor of the next cell
+            unsigned int const queued = s.queued;
             while (true)
             {
                 unsigned int const item = atomicAdd(&s.taken, 1u);
-                if (item >= queued2)
+                if (item >= queued)
                     break;
                 int i = s.queue[item];
+                bool have_warm = false;
+                float warm_x = 0.0f;
+                float warm_pow = 0.0f;
                 while (true)
                 {
-                    int const lr = i / TW;
-                    int const lc = i % TW;
-                    std::uint8_t const mask = s.upmask[i];
-                    double sum = 0.0;
-                    for (int k = 0; k < 8; ++k)
+                    int const next = s.next[i];
+                    double upstream;
+                    bool poisoned;
+                    gather(i, upstream, poisoned);
+                    if constexpr (deferred)
                     {
-                        int const ur = lr + neighbour_dr(k);
-                        int const uc = lc + neighbour_dc(k);
-                        if (((mask >> k) & 1u) && ur >= 0 && ur < TH && uc >= 0 && uc < TW &&
-                            s.state[ur * TW + uc] == state_new)
-                            sum += s.in[ur * TW + uc].c1;
+                        CellInput<OP, T> const& in = s.in[i];
+                        float q;
+                        if (poisoned || isnan(in.q_cur))
+                        {
+                            q = __int_as_float(0x7fc00000);
+                            have_warm = false;
+                        }
+                        else
+                        {
+                            have_warm = have_warm && io.beta == nullptr;
+                            q = kinematic_wave_root_f32(static_cast<float>(in.c1), in.alpha, in.beta, a.dt, upstream,
+                                                        in.q_cur, in.q_lat, in.b, in.c1, have_warm, warm_x, warm_pow);
+                        }
+                        s.hi[i] = q;
+                        s.lo[i] = 0.0f;
                     }
-                    CellInput<OP, T>& in = s.in[i];
-                    in.c1 = in.b + static_cast<double>(in.alpha) * sum;
-                    int next = s.next[i];
-                    if (next >= 0 && s.state[next] != state_new)
-                        next = -1;
+                    else
+                    {
+                        double out;
+                        T rem{};
+                        solve_cell<OP, T>(a, s.in[i], upstream, poisoned, out, rem);
+                        T const hi = static_cast<T>(out);
+                        s.hi[i] = hi;
+                        if constexpr (compensated)
+                            s.lo[i] = isnan(out) ? 0.0f : static_cast<float>(out - static_cast<double>(hi));
+                        if constexpr (two_outputs)
+                            s.rem[i] = rem;
+                    }
+                    // publish: the state write is ordered before the counter update (release)
+                    __hip_atomic_store(&s.state[i], state_new, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     if (next < 0)
                         break;
                     std::uint32_t const shift = 8u * static_cast<std::uint32_t>(next & 3);
                     std::uint32_t const old = __hip_atomic_fetch_sub(
                         &s.count[next >> 2], 1u << shift, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
                     if (((old >> shift) & 0xFFu) != 1u)
-                        break;
+                        break;  // confluence: another stream still has to arrive
                     i = next;
                 }
             }
             __syncthreads();
-            // 4d. corrected values
-            for (int i = tid; i < G::n; i += nthreads)
+
+            if constexpr (deferred)
             {
-                if (s.state[i] != state_new)
-                    continue;
-                float const q = s.hi[i];
-                if (isnan(q))
-                    continue;
-                double v = static_cast<double>(q) + s.in[i].c1;
-                if (!(v > 0.0))
-                    v = static_cast<double>(q);
-                float const hi = static_cast<float>(v);
-                s.hi[i] = hi;
-                s.lo[i] = static_cast<float>(v - static_cast<double>(hi));
+                // 4b. per cell solved in this round: eps = one FP64 Newton step at the FP32 root, sens = d root /
+                // d upstream; stored in place of the no longer needed inputs (b <- eps, alpha <- sens). Count the
+                // upstream cells solved in this round for the propagation walk.
+                if (tid == 0)
+                {
+                    s.queued = 0;
+                    s.taken = 0;
+                }
+                __syncthreads();
+                for (int j = 0; j < cells_per_thread; ++j)
+                {
+                    int const i = tid + j * nthreads;
+                    bool is_ready = false;
+                    if (i < G::n && s.state[i] == state_new)
+                    {
+                        CellInput<OP, T>& in = s.in[i];
+                        double const q = static_cast<double>(s.hi[i]);
+                        double eps = 0.0;
+                        double sens = 0.0;
+                        if (q > 0.0 && !isnan(q))
+                        {
+                            double upstream;
+                            bool poisoned;
+                            gather(i, upstream, poisoned);
+                            kinematic_wave_correction(q, upstream, static_cast<double>(in.alpha), static_cast<double>(in.beta),
+                                                      in.b, in.c1, eps, sens);
+                        }
+                        in.b = eps;
+                        in.alpha = static_cast<float>(sens);
+                        int const lr = i / TW;
+                        int const lc = i % TW;
+                        std::uint8_t const mask = s.upmask[i];
+                        std::uint32_t n = 0;
+                        for (int k = 0; k < 8; ++k)
+                        {
+                            int const ur = lr + neighbour_dr(k);
+                            int const uc = lc + neighbour_dc(k);
+                            if (((mask >> k) & 1u) && ur >= 0 && ur < TH && uc >= 0 && uc < TW &&
+                                s.state[ur * TW + uc] == state_new)
+                                ++n;
+                        }
+                        reinterpret_cast<std::uint8_t*>(s.count)[i] = static_cast<std::uint8_t>(n);
+                        is_ready = n == 0;
+                    }
+                    std::uint64_t const ballot = __ballot(is_ready);
+                    if (ballot)
+                    {
+                        unsigned int base = 0;
+                        if (lane == 0)
+                            base = atomicAdd(&s.queued, static_cast<unsigned int>(__popcll(ballot)));
+                        base = __shfl(base, 0, 64);
+                        if (is_ready)
+                            s.queue[base + __popcll(ballot & lanes_below)] = static_cast<std::uint16_t>(i);
+                    }
+                }
+                __syncthreads();
+                // 4c. propagate: delta = eps + sens * sum(delta of upstream cells solved in this round); delta is
+                // kept in place of c1
+                unsigned int const queued2 = s.queued;
+                while (true)
+                {
+                    unsigned int const item = atomicAdd(&s.taken, 1u);
+                    if (item >= queued2)
+                        break;
+                    int i = s.queue[item];
+                    while (true)
+                    {
+                        int const lr = i / TW;
+                        int const lc = i % TW;
+                        std::uint8_t const mask = s.upmask[i];
+                        double sum = 0.0;
+                        for (int k = 0; k < 8; ++k)
+                        {
+                            int const ur = lr + neighbour_dr(k);
+                            int const uc = lc + neighbour_dc(k);
+                            if (((mask >> k) & 1u) && ur >= 0 && ur < TH && uc >= 0 && uc < TW &&
+                                s.state[ur * TW + uc] == state_new)
+                                sum += s.in[ur * TW + uc].c1;
+                        }
+                        CellInput<OP, T>& in = s.in[i];
+                        in.c1 = in.b + static_cast<double>(in.alpha) * sum;
+                        int next = s.next[i];
+                        if (next >= 0 && s.state[next] != state_new)
+                            next = -1;
+                        if (next < 0)
+                            break;
+                        std::uint32_t const shift = 8u * static_cast<std::uint32_t>(next & 3);
+                        std::uint32_t const old = __hip_atomic_fetch_sub(
+                            &s.count[next >> 2], 1u << shift, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (((old >> shift) & 0xFFu) != 1u)
+                            break;
+                        i = next;
+                    }
+                }
+                __syncthreads();
+                // 4d. corrected values
+                for (int i = tid; i < G::n; i += nthreads)
+                {
+                    if (s.state[i] != state_new)
+                        continue;
+                    float const q = s.hi[i];
+                    if (isnan(q))
+                        continue;
+                    double v = static_cast<double>(q) + s.in[i].c1;
+                    if (!(v > 0.0))
+                        v = static_cast<double>(q);
+                    float const hi = static_cast<float>(v);
+                    s.hi[i] = hi;
+                    s.lo[i] = static_cast<float>(v - static_cast<double>(hi));
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
 
         // 5. write back: outputs of cells solved in this round (coalesced), compensation terms that later

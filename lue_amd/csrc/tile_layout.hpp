@@ -1,0 +1,312 @@
+// In-tile chain layout of the first round of the tiled sweep (chain schedule).
+//
+// Round 0 solves, in every tile, the cells whose whole upstream area lies in the tile ("local" cells;
+// the others are left to the chain phase, chains.hpp). Walking them cell by cell costs one dependent
+// LDS round trip chain per step. Instead the local cells of a tile are laid out once per flow network as
+// heavy-path chains (the same decomposition as the chain phase, restricted to the tile): the heavy child
+// of a cell is its local upstream neighbour with the most local cells upstream, a chain follows heavy
+// children, and the level of a chain is one more than the highest level of the chains that join it from
+// the side (at most log2(TW * TH) + 1 levels). A run of round 0 then solves level after level, each
+// wavefront taking a contiguous part of the level in blocks of 64 positions (solve_block, block_solve.hpp).
+//
+// Recorded per tile: order[p] = local cell index at position p (chains level after level, each in flow
+// order), flags[p] = chain start (bit 0) | heavy child direction (bits 1-3) | has heavy child (bit 4),
+// and per level the positions of the 4 wavefronts: split[l][w], w = 0..4.
+#pragma once
+
+#include "cells.hpp"
+#include "routing_common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lue_amd {
+
+    constexpr int tile_max_levels = 16;
+    constexpr int tile_split_entries = tile_max_levels * 5;
+
+    template<int TW, int TH>
+    struct TileLayoutLds
+    {
+        using G = Tile<TW, TH>;
+        std::uint8_t ldd[G::hn];
+        std::uint8_t local[G::n];        // 1: solved by round 0
+        std::int16_t parent[G::n];       // local downstream cell in the tile, -1 if none
+        std::int16_t heavy[G::n];        // heavy child, -1 if none
+        std::uint8_t heavy_dir[G::n];
+        std::uint32_t count[G::n];       // children still to be sized
+        std::uint32_t size[G::n];        // local cells upstream, including the cell
+        std::uint16_t queue[G::n];
+        std::int16_t head_of[G::n];
+        std::uint16_t depth[G::n];       // distance from the chain head (its most downstream cell)
+        std::uint16_t length[G::n];      // per head: cells in the chain
+        std::uint32_t level[G::n];       // per head: chain level
+        std::uint16_t base[G::n];        // per head: first position of the chain
+        std::uint32_t level_cells[tile_max_levels];
+        std::uint16_t split[tile_split_entries];
+        unsigned int queued;
+        unsigned int taken;
+        unsigned int changed;
+        unsigned int levels;
+    };
+
+    // One workgroup per tile. `pending_bits`: the cells round 0 leaves pending. On a tile whose levels
+    // exceed tile_max_levels (impossible for heavy paths of at most 64 x 64 cells) `error` is set.
+    template<int TW, int TH>
+    __global__ void __launch_bounds__(256)
+        tile_layout_kernel(Catchment const* cats, std::uint32_t const* tile_cat, std::uint32_t const* pending_bits,
+                           std::uint16_t* order, std::uint8_t* flags, std::uint16_t* split, std::uint8_t* nlevels,
+                           std::uint32_t* error)
+    {
+        using G = Tile<TW, TH>;
+        __shared__ TileLayoutLds<TW, TH> s;
+        std::int64_t const tile = blockIdx.x;
+        int const tid = threadIdx.x;
+        int const nthreads = blockDim.x;
+        Catchment const cat = cats[tile_cat[tile]];
+        std::int64_t const local_tile = tile - cat.tile_base;
+        std::int64_t const r0 = (local_tile / cat.tiles_x) * TH;
+        std::int64_t const c0 = (local_tile % cat.tiles_x) * TW;
+        auto const ldd = as_global(cat.ldd);
+        auto const bits = as_global(pending_bits + tile * G::words);
+
+        if (tid == 0)
+        {
+            s.queued = 0;
+            s.taken = 0;
+            s.levels = 0;
+        }
+        for (int i = tid; i < G::hn; i += nthreads)
+        {
+            std::int64_t const r = r0 + i / G::hw - 1;
+            std::int64_t const c = c0 + i % G::hw - 1;
+            s.ldd[i] = (r >= 0 && r < cat.rows && c >= 0 && c < cat.cols) ? ldd[r * cat.cols + c] : no_data_ldd;
+        }
+        for (int l = tid; l < tile_max_levels; l += nthreads)
+            s.level_cells[l] = 0;
+        __syncthreads();
+
+        // local cells, their local parent, child counts
+        for (int i = tid; i < G::n; i += nthreads)
+        {
+            int const lr = i / TW;
+            int const lc = i % TW;
+            std::uint8_t const code = s.ldd[(lr + 1) * G::hw + (lc + 1)];
+            bool const pending = (bits[i >> 5] >> (i & 31)) & 1u;
+            s.local[i] = (code != no_data_ldd && !pending) ? 1 : 0;
+            s.count[i] = 0;
+            s.heavy[i] = -1;
+            s.head_of[i] = -1;
+        }
+        __syncthreads();
+        for (int i = tid; i < G::n; i += nthreads)
+        {
+            std::int16_t p = -1;
+            if (s.local[i])
+            {
+                int const lr = i / TW;
+                int const lc = i % TW;
+                int const dir = ldd_direction(s.ldd[(lr + 1) * G::hw + (lc + 1)]);
+                if (dir >= 0)
+                {
+                    int const dr = lr + neighbour_dr(dir);
+                    int const dc = lc + neighbour_dc(dir);
+                    if (dr >= 0 && dr < TH && dc >= 0 && dc < TW && s.local[dr * TW + dc])
+                    {
+                        p = static_cast<std::int16_t>(dr * TW + dc);
+                        atomicAdd(&s.count[p], 1u);
+                    }
+                }
+            }
+            s.parent[i] = p;
+        }
+        __syncthreads();
+
+        // subtree sizes: walks from the leaves; the last child to arrive continues downstream
+        for (int i = tid; i < G::n; i += nthreads)
+        {
+            if (s.local[i] && s.count[i] == 0)
+                s.queue[atomicAdd(&s.queued, 1u)] = static_cast<std::uint16_t>(i);
+        }
+        __syncthreads();
+        unsigned int const queued = s.queued;
+        while (true)
+        {
+            unsigned int const item = atomicAdd(&s.taken, 1u);
+            if (item >= queued)
+                break;
+            int i = s.queue[item];
+            while (true)
+            {
+                int const lr = i / TW;
+                int const lc = i % TW;
+                std::uint32_t size = 1;
+                std::uint32_t best_size = 0;
+                int best = -1;
+                int best_dir = 0;
+                for (int k = 0; k < 8; ++k)
+                {
+                    int const ur = lr + neighbour_dr(k);
+                    int const uc = lc + neighbour_dc(k);
+                    if (ur < 0 || ur >= TH || uc < 0 || uc >= TW)
+                        continue;
+                    int const u = ur * TW + uc;
+                    if (!s.local[u] || s.parent[u] != i)
+                        continue;
+                    std::uint32_t const su = __hip_atomic_load(&s.size[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    size += su;
+                    if (su > best_size)
+                    {
+                        best_size = su;
+                        best = u;
+                        best_dir = k;
+                    }
+                }
+                s.heavy[i] = static_cast<std::int16_t>(best);
+                s.heavy_dir[i] = static_cast<std::uint8_t>(best_dir);
+                __hip_atomic_store(&s.size[i], size, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                int const p = s.parent[i];
+                if (p < 0)
+                    break;
+                std::uint32_t const old =
+                    __hip_atomic_fetch_sub(&s.count[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (old != 1u)
+                    break;
+                i = p;
+            }
+        }
+        __syncthreads();
+
+        // chains: a head is a local cell that is not the heavy child of its parent; its chain follows heavy
+        // children upstream
+        for (int i = tid; i < G::n; i += nthreads)
+        {
+            if (!s.local[i])
+                continue;
+            int const p = s.parent[i];
+            if (p >= 0 && s.heavy[p] == i)
+                continue;
+            int j = i;
+            std::uint16_t d = 0;
+            while (j >= 0)
+            {
+                s.head_of[j] = static_cast<std::int16_t>(i);
+                s.depth[j] = d++;
+                j = s.heavy[j];
+            }
+            s.length[i] = d;
+            s.level[i] = 0;
+        }
+        __syncthreads();
+
+        // levels: relax level(chain of parent) >= level(chain) + 1 for every head with a parent
+        for (int iteration = 0; iteration <= tile_max_levels; ++iteration)
+        {
+            if (tid == 0)
+                s.changed = 0;
+            __syncthreads();
+            for (int i = tid; i < G::n; i += nthreads)
+            {
+                if (!s.local[i] || s.head_of[i] != i)
+                    continue;
+                int const p = s.parent[i];
+                if (p < 0)
+                    continue;
+                int const hp = s.head_of[p];
+                std::uint32_t const want = s.level[i] + 1;
+                if (s.level[hp] < want && atomicMax(&s.level[hp], want) < want)
+                    s.changed = 1;
+            }
+            __syncthreads();
+            if (s.changed == 0)
+                break;
+            __syncthreads();
+        }
+        for (int i = tid; i < G::n; i += nthreads)
+        {
+            if (!s.local[i] || s.head_of[i] != i)
+                continue;
+            std::uint32_t const l = s.level[i];
+            if (l >= static_cast<std::uint32_t>(tile_max_levels))
+                atomicAdd(error, 1u);
+            else
+            {
+                atomicAdd(&s.level_cells[l], static_cast<std::uint32_t>(s.length[i]));
+                atomicMax(&s.levels, l + 1);
+            }
+        }
+        __syncthreads();
+
+        // positions: levels one after the other, chains in cell order inside a level; each level is cut at
+        // chain boundaries into 4 parts of about equal size, one per wavefront
+        if (tid == 0)
+        {
+            unsigned int const levels = s.levels;
+            std::uint32_t level_begin[tile_max_levels];
+            std::uint32_t cursor[tile_max_levels];
+            int wave[tile_max_levels];
+            std::uint32_t at = 0;
+            for (unsigned int l = 0; l < levels; ++l)
+            {
+                level_begin[l] = at;
+                cursor[l] = at;
+                wave[l] = 0;
+                s.split[l * 5] = static_cast<std::uint16_t>(at);
+                at += s.level_cells[l];
+            }
+            for (int i = 0; i < G::n; ++i)
+            {
+                if (!s.local[i] || s.head_of[i] != i)
+                    continue;
+                std::uint32_t const l = s.level[i];
+                std::uint32_t const b = cursor[l];
+                std::uint32_t const target = (s.level_cells[l] + 3) / 4;
+                while (wave[l] < 3 && b - level_begin[l] >= static_cast<std::uint32_t>(wave[l] + 1) * target)
+                {
+                    ++wave[l];
+                    s.split[l * 5 + wave[l]] = static_cast<std::uint16_t>(b);
+                }
+                s.base[i] = static_cast<std::uint16_t>(b);
+                cursor[l] = b + s.length[i];
+            }
+            for (unsigned int l = 0; l < levels; ++l)
+            {
+                std::uint32_t const end = level_begin[l] + s.level_cells[l];
+                while (wave[l] < 4)
+                {
+                    ++wave[l];
+                    s.split[l * 5 + wave[l]] = static_cast<std::uint16_t>(end);
+                }
+            }
+            for (unsigned int l = levels; l < static_cast<unsigned int>(tile_max_levels); ++l)
+                for (int w = 0; w < 5; ++w)
+                    s.split[l * 5 + w] = static_cast<std::uint16_t>(at);
+        }
+        __syncthreads();
+
+        std::uint16_t* const tile_order = order + tile * G::n;
+        std::uint8_t* const tile_flags = flags + tile * G::n;
+        for (int i = tid; i < G::n; i += nthreads)
+        {
+            if (!s.local[i])
+                continue;
+            int const h = s.head_of[i];
+            int const len = s.length[h];
+            int const d = s.depth[i];
+            int const pos = s.base[h] + len - 1 - d;
+            if (pos >= G::n)
+                continue;
+            tile_order[pos] = static_cast<std::uint16_t>(i);
+            std::uint8_t f = (d == len - 1) ? 1 : 0;
+            if (s.heavy[i] >= 0)
+                f |= static_cast<std::uint8_t>(16 | (s.heavy_dir[i] << 1));
+            tile_flags[pos] = f;
+        }
+        for (int e = tid; e < tile_split_entries; e += nthreads)
+            split[tile * tile_split_entries + e] = s.split[e];
+        if (tid == 0)
+            nlevels[tile] = static_cast<std::uint8_t>(s.levels);
+    }
+
+}  // namespace lue_amd
