@@ -43,23 +43,180 @@ namespace lue_amd {
         CellInput<OP, T> in;
     };
 
-    // Inclusive scan of affine maps along the lanes of a wavefront. Lane k holds x_k = add + mul x_{k-1};
-    // `flag` marks lanes that do not depend on the lanes before them (chain starts). On return
-    // x_k = add + mul x_in where x_in is the value before the first lane, or x_k = add where flag is set.
+    // Lane shift inside rows of 16 lanes (DPP row_shr:N): lane j gets the value of lane j - N of its row;
+    // the first N lanes of a row get `identity`.
+    template<int N>
+    __device__ inline double row_shr(double v, double identity)
+    {
+        int const lo = __builtin_amdgcn_update_dpp(__double2loint(identity), __double2loint(v), 0x110 + N, 0xF, 0xF, false);
+        int const hi = __builtin_amdgcn_update_dpp(__double2hiint(identity), __double2hiint(v), 0x110 + N, 0xF, 0xF, false);
+        return __hiloint2double(hi, lo);
+    }
+
+    template<int N>
+    __device__ inline float row_shr(float v, float identity)
+    {
+        return __int_as_float(
+            __builtin_amdgcn_update_dpp(__float_as_int(identity), __float_as_int(v), 0x110 + N, 0xF, 0xF, false));
+    }
+
+    template<int N>
+    __device__ inline int row_shr(int v, int identity)
+    {
+        return __builtin_amdgcn_update_dpp(identity, v, 0x110 + N, 0xF, 0xF, false);
+    }
+
+    // Lane shift by one across the wavefront (DPP wave_shr:1): lane j gets the value of lane j - 1.
+    __device__ inline double wave_shr1(double v)
+    {
+        int const lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xF, 0xF, false);
+        int const hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xF, 0xF, false);
+        return __hiloint2double(hi, lo);
+    }
+
+    // Segmented inclusive scans along the 64 lanes of a wavefront: Hillis-Steele inside rows of 16 lanes
+    // (DPP shifts by 1, 2, 4, 8), then the totals of the rows before (lanes 15, 31, 47, read into scalars)
+    // are applied to rows 1-3. `flag` marks lanes whose element does not combine with the lanes before
+    // them (chain starts); on return it is set where the result does not depend on the value before lane 0.
+
+    // Affine maps: lane k holds x_k = add + mul x_{k-1}. On return x_k = add + mul x_in, x_in being the
+    // value before lane 0 (or x_k = add where flag is set).
+    __device__ inline void affine_combine(double a_prev, double m_prev, int f_prev, double& add, double& mul, int& flag)
+    {
+        if (!flag)
+        {
+            add = fma(mul, a_prev, add);
+            mul = mul * m_prev;
+            flag = f_prev;
+        }
+    }
+
+    template<int N>
+    __device__ inline void affine_step(double& add, double& mul, int& flag)
+    {
+        double const a_prev = row_shr<N>(add, 0.0);
+        double const m_prev = row_shr<N>(mul, 1.0);
+        int const f_prev = row_shr<N>(flag, 0);
+        affine_combine(a_prev, m_prev, f_prev, add, mul, flag);
+    }
+
     __device__ inline void affine_scan(double& add, double& mul, int& flag)
     {
-        int const lane = threadIdx.x & 63;
-        for (int offset = 1; offset < 64; offset <<= 1)
+        affine_step<1>(add, mul, flag);
+        affine_step<2>(add, mul, flag);
+        affine_step<4>(add, mul, flag);
+        affine_step<8>(add, mul, flag);
+        // prefixes of rows 1, 2, 3: E1 = R0, E2 = R1 o E1, E3 = R2 o E2
+        double e_a = read_lane(add, 15);
+        double e_m = read_lane(mul, 15);
+        int e_f = read_lane(flag, 15);
+        double r_a = read_lane(add, 31);
+        double r_m = read_lane(mul, 31);
+        int r_f = read_lane(flag, 31);
+        double const e1_a = e_a, e1_m = e_m;
+        int const e1_f = e_f;
+        affine_combine(e1_a, e1_m, e1_f, r_a, r_m, r_f);
+        double const e2_a = r_a, e2_m = r_m;
+        int const e2_f = r_f;
+        double e3_a = read_lane(add, 47);
+        double e3_m = read_lane(mul, 47);
+        int e3_f = read_lane(flag, 47);
+        affine_combine(e2_a, e2_m, e2_f, e3_a, e3_m, e3_f);
+        int const row = (threadIdx.x & 63) >> 4;
+        if (row > 0)
         {
-            double const a_prev = __shfl_up(add, offset, 64);
-            double const m_prev = __shfl_up(mul, offset, 64);
-            int const f_prev = __shfl_up(flag, offset, 64);
-            if (lane >= offset && !flag)
-            {
-                add = fma(mul, a_prev, add);
-                mul = mul * m_prev;
-                flag = f_prev;
-            }
+            double const p_a = row == 1 ? e1_a : (row == 2 ? e2_a : e3_a);
+            double const p_m = row == 1 ? e1_m : (row == 2 ? e2_m : e3_m);
+            int const p_f = row == 1 ? e1_f : (row == 2 ? e2_f : e3_f);
+            affine_combine(p_a, p_m, p_f, add, mul, flag);
+        }
+    }
+
+    // The same in FP32.
+    __device__ inline void affine_combine_f32(float a_prev, float m_prev, int f_prev, float& add, float& mul, int& flag)
+    {
+        if (!flag)
+        {
+            add = fmaf(mul, a_prev, add);
+            mul = mul * m_prev;
+            flag = f_prev;
+        }
+    }
+
+    template<int N>
+    __device__ inline void affine_step_f32(float& add, float& mul, int& flag)
+    {
+        float const a_prev = row_shr<N>(add, 0.0f);
+        float const m_prev = row_shr<N>(mul, 1.0f);
+        int const f_prev = row_shr<N>(flag, 0);
+        affine_combine_f32(a_prev, m_prev, f_prev, add, mul, flag);
+    }
+
+    __device__ inline void affine_scan_f32(float& add, float& mul, int& flag)
+    {
+        affine_step_f32<1>(add, mul, flag);
+        affine_step_f32<2>(add, mul, flag);
+        affine_step_f32<4>(add, mul, flag);
+        affine_step_f32<8>(add, mul, flag);
+        float const e1_a = read_lane(add, 15);
+        float const e1_m = read_lane(mul, 15);
+        int const e1_f = read_lane(flag, 15);
+        float e2_a = read_lane(add, 31);
+        float e2_m = read_lane(mul, 31);
+        int e2_f = read_lane(flag, 31);
+        affine_combine_f32(e1_a, e1_m, e1_f, e2_a, e2_m, e2_f);
+        float e3_a = read_lane(add, 47);
+        float e3_m = read_lane(mul, 47);
+        int e3_f = read_lane(flag, 47);
+        affine_combine_f32(e2_a, e2_m, e2_f, e3_a, e3_m, e3_f);
+        int const row = (threadIdx.x & 63) >> 4;
+        if (row > 0)
+        {
+            float const p_a = row == 1 ? e1_a : (row == 2 ? e2_a : e3_a);
+            float const p_m = row == 1 ? e1_m : (row == 2 ? e2_m : e3_m);
+            int const p_f = row == 1 ? e1_f : (row == 2 ? e2_f : e3_f);
+            affine_combine_f32(p_a, p_m, p_f, add, mul, flag);
+        }
+    }
+
+    // Sums: lane k holds x_k = term + x_{k-1}. On return x_k = term + x_in (or term where flag is set).
+    __device__ inline void sum_combine(double t_prev, int f_prev, double& term, int& flag)
+    {
+        if (!flag)
+        {
+            term += t_prev;
+            flag = f_prev;
+        }
+    }
+
+    template<int N>
+    __device__ inline void sum_step(double& term, int& flag)
+    {
+        double const t_prev = row_shr<N>(term, 0.0);
+        int const f_prev = row_shr<N>(flag, 0);
+        sum_combine(t_prev, f_prev, term, flag);
+    }
+
+    __device__ inline void sum_scan(double& term, int& flag)
+    {
+        sum_step<1>(term, flag);
+        sum_step<2>(term, flag);
+        sum_step<4>(term, flag);
+        sum_step<8>(term, flag);
+        double const e1_t = read_lane(term, 15);
+        int const e1_f = read_lane(flag, 15);
+        double e2_t = read_lane(term, 31);
+        int e2_f = read_lane(flag, 31);
+        sum_combine(e1_t, e1_f, e2_t, e2_f);
+        double e3_t = read_lane(term, 47);
+        int e3_f = read_lane(flag, 47);
+        sum_combine(e2_t, e2_f, e3_t, e3_f);
+        int const row = (threadIdx.x & 63) >> 4;
+        if (row > 0)
+        {
+            double const p_t = row == 1 ? e1_t : (row == 2 ? e2_t : e3_t);
+            int const p_f = row == 1 ? e1_f : (row == 2 ? e2_f : e3_f);
+            sum_combine(p_t, p_f, term, flag);
         }
     }
 
@@ -91,76 +248,95 @@ namespace lue_amd {
         double const alpha = static_cast<double>(d.in.alpha);
         double const beta = static_cast<double>(d.in.beta);
         double const b = d.in.b;
+        float const c1f = static_cast<float>(c1);
         float const beta32 = static_cast<float>(d.in.beta);
+        float const ab32 = static_cast<float>(d.in.alpha) * beta32;
 
         // start: F linearised at Q_cur gives Q_k = w_k (Q_{k-1} + extra_k) + g_k
-        double add = 0.0;
-        double mul = 0.0;
-        int flag = (start || !valid) ? 1 : 0;
-        if (valid)
+        double x;
         {
-            double const pc = static_cast<double>(fast_exp2(beta32 * fast_log2(static_cast<float>(q_cur))));
-            double const inv = 1.0 / (c1 + alpha * beta * pc / q_cur);
-            mul = c1 * inv;
-            add = mul * d.extra + (b - (1.0 - beta) * alpha * pc) * inv;
+            float add = 0.0f;
+            float mul = 0.0f;
+            int flag = (start || !valid) ? 1 : 0;
+            double g = 0.0;
+            if (valid)
+            {
+                float const pc1 = fast_exp2((beta32 - 1.0f) * fast_log2(static_cast<float>(q_cur)));  // Q_cur^(beta-1)
+                float const inv = fast_rcp(c1f + ab32 * pc1);
+                mul = c1f * inv;
+                g = static_cast<double>(mul) * d.extra +
+                    (b - (1.0 - beta) * alpha * q_cur * static_cast<double>(pc1)) * static_cast<double>(inv);
+                add = static_cast<float>(g);
+            }
+            affine_scan_f32(add, mul, flag);
+            x = valid ? (flag ? static_cast<double>(add) : fma(static_cast<double>(mul), carry, static_cast<double>(add)))
+                      : 1.0;
         }
-        affine_scan(add, mul, flag);
-        double x = valid ? (flag ? add : fma(mul, carry, add)) : 1.0;
         if (__ballot(valid && !(x > 0.0)))
             return false;
 
+        // Newton steps: F in FP64; the power (T = float: FP32 hardware log2/exp2), the Jacobian and the
+        // step scan in FP32 for T = float, FP64 for T = double
         constexpr double tol = std::is_same_v<T, double> ? 0x1p-30 : 0x1p-13;
         bool converged = false;
         for (int iteration = 0; iteration < 16 && !converged; ++iteration)
         {
-            double prev = __shfl_up(x, 1, 64);
+            double prev = wave_shr1(x);
             if (lane == 0)
                 prev = carry;
             if (start)
                 prev = 0.0;
-            double xb;
-            double inv;
+            double step;
             if constexpr (std::is_same_v<T, double>)
             {
-                xb = dpow(x, beta);
+                double const pw1 = dpow(x, beta - 1.0);  // x^(beta-1)
+                double const f = c1 * (x - prev - d.extra) + alpha * x * pw1 - b;
+                double const fp = c1 + alpha * beta * pw1;
+                double inv = static_cast<double>(fast_rcp(static_cast<float>(fp)));
+                inv = inv * fma(-fp, inv, 2.0);
+                inv = inv * fma(-fp, inv, 2.0);
+                double add = valid ? -f * inv : 0.0;
+                double mul = valid ? c1 * inv : 0.0;
+                int flag = (start || !valid) ? 1 : 0;
+                affine_scan(add, mul, flag);
+                step = add;  // the carry is exact: its step is 0
             }
             else
             {
-                xb = static_cast<double>(fast_exp2(beta32 * fast_log2(static_cast<float>(x))));
+                float const pw1 = fast_exp2((beta32 - 1.0f) * fast_log2(static_cast<float>(x)));
+                double const f = c1 * (x - prev - d.extra) + alpha * x * static_cast<double>(pw1) - b;
+                float const inv = fast_rcp(c1f + ab32 * pw1);
+                float add = valid ? static_cast<float>(-f) * inv : 0.0f;
+                float mul = valid ? c1f * inv : 0.0f;
+                int flag = (start || !valid) ? 1 : 0;
+                affine_scan_f32(add, mul, flag);
+                step = static_cast<double>(add);
             }
-            double const ax = alpha * xb;
-            double const f = c1 * (x - prev - d.extra) + ax - b;
-            double const fp = c1 + beta * ax / x;
-            if constexpr (std::is_same_v<T, double>)
-                inv = 1.0 / fp;
-            else
-                inv = static_cast<double>(fast_rcp(static_cast<float>(fp)));
-            add = valid ? -f * inv : 0.0;
-            mul = valid ? c1 * inv : 0.0;
-            flag = (start || !valid) ? 1 : 0;
-            affine_scan(add, mul, flag);
-            double const xn = x + add;  // the carry is exact: its step is 0
+            double const xn = x + step;
             if (__ballot(valid && !(xn > 0.0)))
                 return false;
-            converged = __ballot(valid && !(fabs(add) <= fabs(xn) * tol)) == 0;
+            converged = __ballot(valid && !(fabs(step) <= fabs(xn) * tol)) == 0;
             x = xn;
         }
         if (!converged)
             return false;
         if constexpr (std::is_same_v<T, float>)
         {
-            // FP64 residual correction
-            double prev = __shfl_up(x, 1, 64);
+            // FP64 residual correction: one Newton step with the FP64 power and Jacobian
+            double prev = wave_shr1(x);
             if (lane == 0)
                 prev = carry;
             if (start)
                 prev = 0.0;
-            double const ax = alpha * dpow(x, beta);
-            double const f = c1 * (x - prev - d.extra) + ax - b;
-            double const inv = 1.0 / (c1 + beta * ax / x);
-            add = valid ? -f * inv : 0.0;
-            mul = valid ? c1 * inv : 0.0;
-            flag = (start || !valid) ? 1 : 0;
+            double const pw1 = dpow(x, beta - 1.0);
+            double const f = c1 * (x - prev - d.extra) + alpha * x * pw1 - b;
+            double const fp = c1 + alpha * beta * pw1;
+            double inv = static_cast<double>(fast_rcp(static_cast<float>(fp)));
+            inv = inv * fma(-fp, inv, 2.0);
+            inv = inv * fma(-fp, inv, 2.0);
+            double add = valid ? -f * inv : 0.0;
+            double mul = valid ? c1 * inv : 0.0;
+            int flag = (start || !valid) ? 1 : 0;
             affine_scan(add, mul, flag);
             double const xn = x + add;
             if (__ballot(valid && !(xn > 0.0)))
@@ -198,17 +374,7 @@ namespace lue_amd {
             // linear: out_k = out_{k-1} + side_k + x_k, a segmented inclusive scan along the block
             double term = !d.valid ? 0.0 : (isnan(d.in.x) ? nan64 : d.extra + static_cast<double>(d.in.x));
             int flag = d.valid && d.start ? 1 : 0;
-            for (int offset = 1; offset < 64; offset <<= 1)
-            {
-                double const t_prev = __shfl_up(term, offset, 64);
-                int const f_prev = __shfl_up(flag, offset, 64);
-                if (lane >= offset)
-                {
-                    if (!flag)
-                        term += t_prev;
-                    flag |= f_prev;
-                }
-            }
+            sum_scan(term, flag);
             value = flag ? term : st.carry + term;
             st.carry = read_lane(value, nb - 1);
         }
@@ -299,18 +465,10 @@ namespace lue_amd {
             }
             // affine scan: delta_k = eps_k + sens_k * delta_{k-1}; a chain start does not depend on
             // the cell before it, and the first cell of the block starts from a corrected value
-            double mul = (d.valid && d.start) ? 0.0 : sens;
+            double mul = sens;
             double add = eps;
-            for (int offset = 1; offset < 64; offset <<= 1)
-            {
-                double const m_prev = __shfl_up(mul, offset, 64);
-                double const a_prev = __shfl_up(add, offset, 64);
-                if (lane >= offset)
-                {
-                    add = add + mul * a_prev;
-                    mul = mul * m_prev;
-                }
-            }
+            int flag = (!d.valid || d.start) ? 1 : 0;
+            affine_scan(add, mul, flag);
             value = isnan(q) ? q : (d.valid ? (q + add > 0.0 ? q + add : q) : 0.0);
             st.carry = read_lane(value, nb - 1);
         }

@@ -157,20 +157,42 @@ namespace lue_amd {
         return std::numeric_limits<T>::quiet_NaN();
     }
 
+    // Raw input values of one cell (before the domain checks), loaded apart from their use so that the
+    // loads of several cells are in flight together.
     template<int OP, typename T>
-    __device__ inline void stage_input(SweepArgs<T> const& a, CatchmentIO<T> const& io, std::int64_t g,
-                                       CellInput<OP, T>& c)
+    struct RawInput
+    {
+        T v[OP == op_accu ? 1 : (OP == op_kinematic_wave ? 5 : 2)];
+    };
+
+    template<int OP, typename T>
+    __device__ inline void load_raw(SweepArgs<T> const& a, CatchmentIO<T> const& io, std::int64_t g, RawInput<OP, T>& r)
+    {
+        r.v[0] = value_of(io.in1, a.in1_s, g);
+        if constexpr (OP != op_accu)
+            r.v[1] = value_of(io.in2, a.in2_s, g);
+        if constexpr (OP == op_kinematic_wave)
+        {
+            r.v[2] = value_of(io.alpha, a.alpha_s, g);
+            r.v[3] = value_of(io.beta, a.beta_s, g);
+            r.v[4] = value_of(io.dx, a.dx_s, g);
+        }
+    }
+
+    // Domain checks and derived coefficients of one cell from its raw inputs.
+    template<int OP, typename T>
+    __device__ inline void stage_raw(SweepArgs<T> const& a, RawInput<OP, T> const& r, CellInput<OP, T>& c)
     {
         if constexpr (OP == op_accu)
         {
-            T const x = value_of(io.in1, a.in1_s, g);
+            T const x = r.v[0];
             // value_policies/accu.hpp:9-19: inflow >= 0
             c.x = (x >= T{0}) ? x : quiet_nan<T>();
         }
         else if constexpr (OP == op_threshold || OP == op_fraction)
         {
-            T const x = value_of(io.in1, a.in1_s, g);
-            T const p = value_of(io.in2, a.in2_s, g);
+            T const x = r.v[0];
+            T const p = r.v[1];
             // value_policies/accu_threshold.hpp:9-20, value_policies/accu_fraction.hpp:9-20
             bool const ok = OP == op_threshold ? (x >= T{0} && p >= T{0})
                                                : (x >= T{0} && p >= T{0} && p <= T{1});
@@ -179,11 +201,11 @@ namespace lue_amd {
         }
         else
         {
-            T const q_cur = value_of(io.in1, a.in1_s, g);
-            T const q_lat = value_of(io.in2, a.in2_s, g);
-            T const alpha = value_of(io.alpha, a.alpha_s, g);
-            T const beta = value_of(io.beta, a.beta_s, g);
-            T const dx = value_of(io.dx, a.dx_s, g);
+            T const q_cur = r.v[0];
+            T const q_lat = r.v[1];
+            T const alpha = r.v[2];
+            T const beta = r.v[3];
+            T const dx = r.v[4];
             // value_policies/kinematic_wave.hpp:9-22 plus no-data (NaN) in any argument
             bool const ok = q_cur >= T{0} && q_lat >= T{0} && dx > T{0} && !isnan(alpha) && !isnan(beta) &&
                             !isnan(a.dt);
@@ -193,10 +215,19 @@ namespace lue_amd {
             c.beta = beta;
             double const dt = static_cast<double>(a.dt);
             c.c1 = dt / static_cast<double>(dx);
-            c.b = ok ? static_cast<double>(alpha) * pow(static_cast<double>(q_cur), static_cast<double>(beta)) +
-                           dt * static_cast<double>(q_lat)
-                     : 0.0;
+            double const qb = q_cur > T{0} ? dpow(static_cast<double>(q_cur), static_cast<double>(beta))
+                                           : pow(static_cast<double>(q_cur), static_cast<double>(beta));
+            c.b = ok ? static_cast<double>(alpha) * qb + dt * static_cast<double>(q_lat) : 0.0;
         }
+    }
+
+    template<int OP, typename T>
+    __device__ inline void stage_input(SweepArgs<T> const& a, CatchmentIO<T> const& io, std::int64_t g,
+                                       CellInput<OP, T>& c)
+    {
+        RawInput<OP, T> r;
+        load_raw<OP, T>(a, io, g, r);
+        stage_raw<OP, T>(a, r, c);
     }
 
     // Solve one cell given the FP64 sum of its upstream outflows.

@@ -111,21 +111,83 @@ namespace lue_amd {
             s.queued = 0;
             s.taken = 0;
         }
+        // first chain staging index of the tile, loaded early (used at the end)
+        std::uint32_t const stage_begin = (round == 0 && a.stage_off != nullptr) ? as_global(a.stage_off)[tile] : 0u;
 
-        // 1. ldd with halo; cells outside the array read as no-data
-        for (int i = tid; i < G::hn; i += nthreads)
+        // 1. ldd with halo (cells outside the array read as no-data); in round 0 also the raw inputs of every
+        // cell, and with a recorded in-tile layout its order and flags. All loads of a thread are issued
+        // before the first one is used.
+        constexpr int halo_per_thread = (G::hn + 255) / 256;
+        bool const layout = round == 0 && a.tile_order != nullptr;
+        RawInput<OP, T> raw[cells_per_thread];
+        if (round == 0)
         {
+#pragma unroll
+            for (int j = 0; j < cells_per_thread; ++j)
+            {
+                int const i = tid + j * nthreads;
+                std::int64_t const r = r0 + i / TW;
+                std::int64_t const c = c0 + i % TW;
+                if (i < G::n && r < rows && c < cols)
+                    load_raw<OP, T>(a, io, r * cols + c, raw[j]);
+            }
+        }
+        std::uint8_t halo[halo_per_thread];
+#pragma unroll
+        for (int j = 0; j < halo_per_thread; ++j)
+        {
+            int const i = tid + j * nthreads;
             int const hr = i / G::hw;
             int const hc = i % G::hw;
             std::int64_t const r = r0 + hr - 1;
             std::int64_t const c = c0 + hc - 1;
-            s.ldd[i] = (r >= 0 && r < rows && c >= 0 && c < cols) ? g_ldd[r * cols + c] : no_data_ldd;
+            halo[j] = (i < G::hn && r >= 0 && r < rows && c >= 0 && c < cols) ? g_ldd[r * cols + c] : no_data_ldd;
+        }
+        // the walk-only arrays queue and count hold the layout's order and flags
+        std::uint8_t* const layout_flags = reinterpret_cast<std::uint8_t*>(s.count);
+        if (layout)
+        {
+            auto const order = as_global(a.tile_order + tile * G::n);
+            auto const flags = as_global(a.tile_flags + tile * G::n);
+            std::uint16_t o[cells_per_thread];
+            std::uint8_t f[cells_per_thread];
+#pragma unroll
+            for (int j = 0; j < cells_per_thread; ++j)
+            {
+                int const p = tid + j * nthreads;
+                if (p < G::n)
+                {
+                    o[j] = order[p];
+                    f[j] = flags[p];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < cells_per_thread; ++j)
+            {
+                int const p = tid + j * nthreads;
+                if (p < G::n)
+                {
+                    s.queue[p] = o[j];
+                    layout_flags[p] = f[j];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < halo_per_thread; ++j)
+        {
+            int const i = tid + j * nthreads;
+            if (i < G::hn)
+                s.ldd[i] = halo[j];
         }
         __syncthreads();
 
         // 2. state, upstream mask and staged inputs of pending cells
-        for (int i = tid; i < G::n; i += nthreads)
+#pragma unroll
+        for (int j = 0; j < cells_per_thread; ++j)
         {
+            int const i = tid + j * nthreads;
+            if (i >= G::n)
+                continue;
             int const lr = i / TW;
             int const lc = i % TW;
             int const h = (lr + 1) * G::hw + (lc + 1);
@@ -161,7 +223,11 @@ namespace lue_amd {
                         mask |= static_cast<std::uint8_t>(1u << k);
                     }
                 }
-                if (pending)
+                if (round == 0)
+                {
+                    stage_raw<OP, T>(a, raw[j], s.in[i]);
+                }
+                else if (pending)
                 {
                     std::int64_t const g = (r0 + lr) * cols + (c0 + lc);
                     stage_input<OP, T>(a, io, g, s.in[i]);
@@ -169,30 +235,31 @@ namespace lue_amd {
             }
             s.state[i] = st;
             s.upmask[i] = mask;
-            int next = -1;
-            if (st != state_absent)
+            if (!layout)
             {
-                int const dir = ldd_direction(code);
-                if (dir >= 0)
+                int next = -1;
+                if (st != state_absent)
                 {
-                    int const dr = lr + neighbour_dr(dir);
-                    int const dc = lc + neighbour_dc(dir);
-                    if (dr >= 0 && dr < TH && dc >= 0 && dc < TW && s.ldd[(dr + 1) * G::hw + (dc + 1)] != no_data_ldd)
-                        next = dr * TW + dc;
+                    int const dir = ldd_direction(code);
+                    if (dir >= 0)
+                    {
+                        int const dr = lr + neighbour_dr(dir);
+                        int const dc = lc + neighbour_dc(dir);
+                        if (dr >= 0 && dr < TH && dc >= 0 && dc < TW &&
+                            s.ldd[(dr + 1) * G::hw + (dc + 1)] != no_data_ldd)
+                            next = dr * TW + dc;
+                    }
                 }
+                s.next[i] = static_cast<std::int16_t>(next);
             }
-            s.next[i] = static_cast<std::int16_t>(next);
         }
         __syncthreads();
 
-        bool const layout = round == 0 && a.tile_order != nullptr;
         if (layout)
         {
             // 3-4 (recorded in-tile chains, tile_layout.hpp): level after level; wavefront w solves the
             // positions [split[l][w], split[l][w + 1]) in blocks of 64. The light upstream cells of a chain
             // lie in lower levels, the heavy one is the previous position of the chain.
-            auto const order = as_global(a.tile_order + tile * G::n);
-            auto const flags = as_global(a.tile_flags + tile * G::n);
             auto const split = as_global(a.tile_split + tile * tile_split_entries);
             int const levels = min(static_cast<int>(as_global(a.tile_nlevels)[tile]), tile_max_levels);
             static_assert((G::n & (G::n - 1)) == 0, "tile cell count must be a power of two");
@@ -213,8 +280,8 @@ namespace lue_amd {
                     d.valid = p < hi;
                     if (d.valid)
                     {
-                        i = order[p] & (G::n - 1);  // in bounds whatever the recorded data
-                        std::uint8_t const f = flags[p];
+                        i = s.queue[p] & (G::n - 1);  // in bounds whatever the recorded data
+                        std::uint8_t const f = layout_flags[p];
                         d.start = (f & 1u) != 0;
                         int const heavy = (f & 16u) ? ((f >> 1) & 7) : -1;
                         int const lr = i / TW;
@@ -592,16 +659,21 @@ namespace lue_amd {
                 stamps[p] = st == state_pending ? stamp_pending : 0u;
             }
         }
-        // pending bitmap: one word per 32 cells
-        for (int w = tid; w < G::words; w += nthreads)
+        // pending bitmap: one word per 32 cells; a wavefront's 64 consecutive cells give two words (ballot)
+        static_assert(G::n % 256 == 0, "tiles hold a multiple of 256 cells");
+#pragma unroll
+        for (int j = 0; j < cells_per_thread; ++j)
         {
-            std::uint32_t bits = 0;
-            for (int b = 0; b < 32; ++b)
+            int const i = tid + j * nthreads;
+            std::uint64_t const bits = __ballot(s.state[i] == state_pending);
+            if ((tid & 63) == 0)
             {
-                bits |= (s.state[w * 32 + b] == state_pending ? 1u : 0u) << b;
+                int const w = i >> 5;
+                pbits[w] = static_cast<std::uint32_t>(bits);
+                pbits[w + 1] = static_cast<std::uint32_t>(bits >> 32);
+                s.pword[w] = static_cast<std::uint32_t>(bits);
+                s.pword[w + 1] = static_cast<std::uint32_t>(bits >> 32);
             }
-            pbits[w] = bits;
-            s.pword[w] = bits;
         }
         if (a.record)
         {
@@ -630,16 +702,29 @@ namespace lue_amd {
                 }
             }
             __syncthreads();
-            std::uint32_t const begin = as_global(a.stage_off)[tile];
             CellInput<OP, T>* const staged = static_cast<CellInput<OP, T>*>(a.staged);
-            for (int i = tid; i < G::n; i += nthreads)
+            // chain positions of this thread's pending cells, all loads in flight together
+            std::uint32_t pos_of[cells_per_thread];
+#pragma unroll
+            for (int j = 0; j < cells_per_thread; ++j)
             {
-                if (s.state[i] != state_pending)
+                int const i = tid + j * nthreads;
+                pos_of[j] = 0;
+                if (i < G::n && s.state[i] == state_pending)
+                {
+                    std::uint32_t const word = s.pword[i >> 5];
+                    std::uint32_t const b = static_cast<std::uint32_t>(i & 31);
+                    std::uint32_t const rank = s.pprefix[i >> 5] + __popc(word & ((1u << b) - 1u));
+                    pos_of[j] = as_global(a.stage_pos)[stage_begin + rank];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < cells_per_thread; ++j)
+            {
+                int const i = tid + j * nthreads;
+                if (i >= G::n || s.state[i] != state_pending)
                     continue;
-                std::uint32_t const word = s.pword[i >> 5];
-                std::uint32_t const b = static_cast<std::uint32_t>(i & 31);
-                std::uint32_t const rank = s.pprefix[i >> 5] + __popc(word & ((1u << b) - 1u));
-                std::uint32_t const pos = as_global(a.stage_pos)[begin + rank];
+                std::uint32_t const pos = pos_of[j];
                 int const lr = i / TW;
                 int const lc = i % TW;
                 std::uint8_t const mask = s.upmask[i];
