@@ -18,6 +18,11 @@ namespace lue_amd {
 
     constexpr std::uint32_t stamp_pending = 0xFFFFFFFFu;
 
+    // Workgroup of the tile kernels: 4 wavefronts per tile (measured at 16384^2: faster than 8, whose
+    // shorter tile latency is lost to fewer tiles resident per CU).
+    constexpr int sweep_threads = 256;
+    constexpr int sweep_waves = sweep_threads / 64;
+
     // One raster of a batch.
     struct Catchment
     {

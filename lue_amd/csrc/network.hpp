@@ -872,7 +872,8 @@ namespace lue_amd {
             if (int const st = timer.start(stream))
                 return st;
         }
-        hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH>), dim3(static_cast<unsigned>(net.ntiles)), dim3(256), 0,
+        hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH>),
+                           dim3(static_cast<unsigned>(net.ntiles)), dim3(sweep_threads), 0,
                            stream, a);
         LUE_AMD_HIP(hipGetLastError());
         if (int const st = timer.mark(stream))
@@ -897,7 +898,7 @@ namespace lue_amd {
                 LUE_AMD_HIP(hipMemcpyAsync(tiles_of_rounds.data() + at,
                                            w.worklist.ptr + static_cast<std::int64_t>(slot) * net.ntiles,
                                            sizeof(std::uint32_t) * count, hipMemcpyDeviceToHost, stream));
-                hipLaunchKernelGGL((sweep_next_round<OP, T, TW, TH>), dim3(std::min<unsigned>(grid, count)), dim3(256),
+                hipLaunchKernelGGL((sweep_next_round<OP, T, TW, TH>), dim3(std::min<unsigned>(grid, count)), dim3(sweep_threads),
                                    0, stream, a, round);
                 LUE_AMD_HIP(hipGetLastError());
                 net.round_offset.push_back(tiles_of_rounds.size());
@@ -969,7 +970,8 @@ namespace lue_amd {
         OpTimer timer;
         if (int const st = timer.start(stream))
             return st;
-        hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH>), dim3(static_cast<unsigned>(net.ntiles)), dim3(256), 0,
+        hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH>),
+                           dim3(static_cast<unsigned>(net.ntiles)), dim3(sweep_threads), 0,
                            stream, a);
         LUE_AMD_HIP(hipGetLastError());
         if (int const st = timer.mark(stream))
@@ -986,7 +988,7 @@ namespace lue_amd {
                 std::uint64_t const begin = net.round_offset[r - 1];
                 std::uint32_t const count = static_cast<std::uint32_t>(net.round_offset[r] - begin);
                 unsigned const grid = std::min<unsigned>(count, 2048);
-                hipLaunchKernelGGL((sweep_list_round<OP, T, TW, TH>), dim3(grid), dim3(256), 0, stream, a,
+                hipLaunchKernelGGL((sweep_list_round<OP, T, TW, TH>), dim3(grid), dim3(sweep_threads), 0, stream, a,
                                    net.round_tiles.ptr + begin, count, static_cast<std::uint32_t>(r));
             }
             LUE_AMD_HIP(hipGetLastError());

@@ -83,7 +83,7 @@ namespace lue_amd {
         using G = Tile<TW, TH>;
         constexpr bool compensated = SweepLds<OP, T, TW, TH>::compensated;
         constexpr bool two_outputs = SweepLds<OP, T, TW, TH>::two_outputs;
-        constexpr int cells_per_thread = (G::n + 255) / 256;
+        constexpr int cells_per_thread = (G::n + sweep_threads - 1) / sweep_threads;
 
         int const tid = threadIdx.x;
         int const nthreads = blockDim.x;
@@ -117,7 +117,7 @@ namespace lue_amd {
         // 1. ldd with halo (cells outside the array read as no-data); in round 0 also the raw inputs of every
         // cell, and with a recorded in-tile layout its order and flags. All loads of a thread are issued
         // before the first one is used.
-        constexpr int halo_per_thread = (G::hn + 255) / 256;
+        constexpr int halo_per_thread = (G::hn + sweep_threads - 1) / sweep_threads;
         bool const layout = round == 0 && a.tile_order != nullptr;
         RawInput<OP, T> raw[cells_per_thread];
         if (round == 0)
@@ -268,8 +268,8 @@ namespace lue_amd {
             bool const uniform_beta = io.beta == nullptr;
             for (int l = 0; l < levels; ++l)
             {
-                int const lo = split[l * 5 + wave];
-                int const hi = split[l * 5 + wave + 1];
+                int const lo = split[l * (sweep_waves + 1) + wave];
+                int const hi = split[l * (sweep_waves + 1) + wave + 1];
                 BlockState st;
                 for (int blk = lo; blk < hi; blk += 64)
                 {
@@ -660,7 +660,7 @@ namespace lue_amd {
             }
         }
         // pending bitmap: one word per 32 cells; a wavefront's 64 consecutive cells give two words (ballot)
-        static_assert(G::n % 256 == 0, "tiles hold a multiple of 256 cells");
+        static_assert(G::n % sweep_threads == 0, "tiles hold a multiple of sweep_threads cells");
 #pragma unroll
         for (int j = 0; j < cells_per_thread; ++j)
         {
@@ -750,9 +750,26 @@ namespace lue_amd {
         }
     }
 
-    // Round 0: every tile of the batch. Grid = number of tiles.
+    // Workgroups of a tile kernel that fit one CU (160 KB of LDS, 32 wavefront slots), and the wavefronts per
+    // SIMD that follow: the register budget the kernels are compiled for.
     template<int OP, typename T, int TW, int TH>
-    __global__ void __launch_bounds__(256) sweep_first_round(SweepArgs<T> a)
+    constexpr int sweep_workgroups_per_cu()
+    {
+        int const by_lds = static_cast<int>((160 * 1024) / sizeof(SweepLds<OP, T, TW, TH>));
+        int const by_waves = 32 / sweep_waves;
+        int const n = by_lds < by_waves ? by_lds : by_waves;
+        return n < 1 ? 1 : n;
+    }
+
+    template<int OP, typename T, int TW, int TH>
+    constexpr int sweep_waves_per_simd()
+    {
+        return sweep_workgroups_per_cu<OP, T, TW, TH>() * sweep_waves / 4;
+    }
+
+    // Round 0: every tile of the batch, one workgroup per tile (grid = number of tiles).
+    template<int OP, typename T, int TW, int TH>
+    __global__ void __launch_bounds__(sweep_threads, (sweep_waves_per_simd<OP, T, TW, TH>())) sweep_first_round(SweepArgs<T> a)
     {
         __shared__ SweepLds<OP, T, TW, TH> s;
         sweep_tile<OP, T, TW, TH>(a, s, blockIdx.x, 0u);
@@ -760,7 +777,7 @@ namespace lue_amd {
 
     // Round r >= 1 while recording: the tiles queued by round r - 1. Workgroup b takes items b, b + grid, ...
     template<int OP, typename T, int TW, int TH>
-    __global__ void __launch_bounds__(256) sweep_next_round(SweepArgs<T> a, std::uint32_t round)
+    __global__ void __launch_bounds__(sweep_threads, (sweep_waves_per_simd<OP, T, TW, TH>())) sweep_next_round(SweepArgs<T> a, std::uint32_t round)
     {
         __shared__ SweepLds<OP, T, TW, TH> s;
         std::uint32_t const slot = round % 3;
@@ -781,7 +798,7 @@ namespace lue_amd {
 
     // Round r >= 1 replayed from a recorded tile list.
     template<int OP, typename T, int TW, int TH>
-    __global__ void __launch_bounds__(256) sweep_list_round(SweepArgs<T> a, std::uint32_t const* tiles,
+    __global__ void __launch_bounds__(sweep_threads, (sweep_waves_per_simd<OP, T, TW, TH>())) sweep_list_round(SweepArgs<T> a, std::uint32_t const* tiles,
                                                             std::uint32_t count, std::uint32_t round)
     {
         __shared__ SweepLds<OP, T, TW, TH> s;

@@ -11,7 +11,7 @@
 //
 // Recorded per tile: order[p] = local cell index at position p (chains level after level, each in flow
 // order), flags[p] = chain start (bit 0) | heavy child direction (bits 1-3) | has heavy child (bit 4),
-// and per level the positions of the 4 wavefronts: split[l][w], w = 0..4.
+// and per level the positions of the sweep_waves wavefronts: split[l][w], w = 0..sweep_waves.
 #pragma once
 
 #include "cells.hpp"
@@ -24,7 +24,7 @@
 namespace lue_amd {
 
     constexpr int tile_max_levels = 16;
-    constexpr int tile_split_entries = tile_max_levels * 5;
+    constexpr int tile_split_entries = tile_max_levels * (sweep_waves + 1);
 
     template<int TW, int TH>
     struct TileLayoutLds
@@ -239,7 +239,7 @@ namespace lue_amd {
         __syncthreads();
 
         // positions: levels one after the other, chains in cell order inside a level; each level is cut at
-        // chain boundaries into 4 parts of about equal size, one per wavefront
+        // chain boundaries into sweep_waves parts of about equal size, one per wavefront of the tile kernel
         if (tid == 0)
         {
             unsigned int const levels = s.levels;
@@ -252,7 +252,7 @@ namespace lue_amd {
                 level_begin[l] = at;
                 cursor[l] = at;
                 wave[l] = 0;
-                s.split[l * 5] = static_cast<std::uint16_t>(at);
+                s.split[l * (sweep_waves + 1)] = static_cast<std::uint16_t>(at);
                 at += s.level_cells[l];
             }
             for (int i = 0; i < G::n; ++i)
@@ -261,11 +261,11 @@ namespace lue_amd {
                     continue;
                 std::uint32_t const l = s.level[i];
                 std::uint32_t const b = cursor[l];
-                std::uint32_t const target = (s.level_cells[l] + 3) / 4;
-                while (wave[l] < 3 && b - level_begin[l] >= static_cast<std::uint32_t>(wave[l] + 1) * target)
+                std::uint32_t const target = (s.level_cells[l] + sweep_waves - 1) / sweep_waves;
+                while (wave[l] < sweep_waves - 1 && b - level_begin[l] >= static_cast<std::uint32_t>(wave[l] + 1) * target)
                 {
                     ++wave[l];
-                    s.split[l * 5 + wave[l]] = static_cast<std::uint16_t>(b);
+                    s.split[l * (sweep_waves + 1) + wave[l]] = static_cast<std::uint16_t>(b);
                 }
                 s.base[i] = static_cast<std::uint16_t>(b);
                 cursor[l] = b + s.length[i];
@@ -273,15 +273,15 @@ namespace lue_amd {
             for (unsigned int l = 0; l < levels; ++l)
             {
                 std::uint32_t const end = level_begin[l] + s.level_cells[l];
-                while (wave[l] < 4)
+                while (wave[l] < sweep_waves)
                 {
                     ++wave[l];
-                    s.split[l * 5 + wave[l]] = static_cast<std::uint16_t>(end);
+                    s.split[l * (sweep_waves + 1) + wave[l]] = static_cast<std::uint16_t>(end);
                 }
             }
             for (unsigned int l = levels; l < static_cast<unsigned int>(tile_max_levels); ++l)
-                for (int w = 0; w < 5; ++w)
-                    s.split[l * 5 + w] = static_cast<std::uint16_t>(at);
+                for (int w = 0; w <= sweep_waves; ++w)
+                    s.split[l * (sweep_waves + 1) + w] = static_cast<std::uint16_t>(at);
         }
         __syncthreads();
 
