@@ -61,6 +61,9 @@ def parse():
     p.add_argument("--kw-tile", default="32x32", help="sweep tile of the kinematic-wave network (WxH)")
     p.add_argument("--schedule", default="chains", choices=["chains", "rounds"])
     p.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic runs")
+    p.add_argument("--extras", action="store_true",
+                   help="also run the batch (512 catchments, accu_threshold, autotuned), large-domain (32768^2, "
+                        "24 resident timesteps) and serving (200 catchments, Poisson 1000/s) workloads")
     p.add_argument("--probe", action="store_true", help=argparse.SUPPRESS)  # child run under rocprofv3 --pmc
     return p.parse_args()
 
@@ -318,6 +321,26 @@ def main():
                          f"partition-parallel std::thread pool, host has {os.cpu_count()} logical CPUs"}
         del host, ldd_h
 
+    extras = None
+    if args.extras:
+        from lue_amd import workloads
+
+        del f64, out64, out32
+        net_accu.close()
+        net_kw.close()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        extras = {}
+        log("[bench] extras: batch of 512 catchments")
+        extras["batch_accu_threshold"] = workloads.batch_threshold()
+        torch.cuda.empty_cache()
+        log("[bench] extras: 32768^2, 24 timesteps")
+        extras["large_domain"] = workloads.large_domain()
+        torch.cuda.empty_cache()
+        log("[bench] extras: serving")
+        extras["serving"] = workloads.serving_run()
+        torch.cuda.empty_cache()
+
     result = {
         "metric": "Mcells/s accu+kinwave @16384^2 (FP32 path, <=1e-5 rel vs FP64)",
         "value": round(value, 1),
@@ -352,6 +375,8 @@ def main():
         "traffic": traffic,
         "step_hbm_gbs": round(cells * (BYTES_ACCU_F32 + BYTES_KW_F32) / t32 / 1e9, 1),
     }
+    if extras is not None:
+        result["extras"] = extras
     print(json.dumps(result), flush=True)
 
 

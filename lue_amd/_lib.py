@@ -69,6 +69,9 @@ _RESTYPES = {
 }
 
 
+STATUS_INVALID_ARGUMENT = 1
+
+
 class RoutingError(RuntimeError):
     """A C ABI call returned an error status."""
 

@@ -112,3 +112,18 @@ def uniform_device(seed, stream, lo, hi, shape, dtype=torch.float32, device=None
     fn = lib.lue_amd_synthetic_uniform_f32 if dtype == torch.float32 else lib.lue_amd_synthetic_uniform_f64
     _lib.check(fn(seed, stream, lo, hi, out.numel(), out.data_ptr(), _stream()))
     return out
+
+
+# ---- batches of catchments --------------------------------------------------------------------------
+
+def batch_shapes(seed, count, side_lo, side_hi, aspect_lo=0.5, aspect_hi=2.0):
+    """Shapes (rows, cols) of `count` catchments: side log-uniform on [side_lo, side_hi] (the geometric mean
+    of rows and cols), aspect ratio cols / rows uniform on [aspect_lo, aspect_hi]; each side at least 2.
+    Catchment i uses seed + 1 + i for its own fields."""
+    u = unit_f64(hash3(seed, 100, np.arange(count, dtype=np.uint64)))
+    v = unit_f64(hash3(seed, 101, np.arange(count, dtype=np.uint64)))
+    side = np.exp(math.log(side_lo) + (math.log(side_hi) - math.log(side_lo)) * u)
+    aspect = aspect_lo + (aspect_hi - aspect_lo) * v
+    rows = np.maximum(2, np.rint(side / np.sqrt(aspect))).astype(np.int64)
+    cols = np.maximum(2, np.rint(side * np.sqrt(aspect))).astype(np.int64)
+    return [(int(r), int(c)) for r, c in zip(rows, cols)]

@@ -1,0 +1,92 @@
+"""Serving kinematic-wave timesteps: continuous batching over stream slots, HBM-budget admission."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def fw():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lue_amd import framework
+
+    return framework
+
+
+def test_poisson_requests_rate_and_order():
+    from lue_amd import serving
+
+    reqs = serving.poisson_requests(3, 500.0, 4.0, ["a", "b", "c"])
+    times = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(times) >= 0) and times[-1] <= 4.0
+    assert abs(len(reqs) - 2000) < 200  # Poisson: sd about 45
+    assert {r.catchment for r in reqs} == {"a", "b", "c"}
+    assert len({r.seed for r in reqs}) == len(reqs)
+
+
+def _catchments(fw, seed, shapes):
+    from lue_amd import synthetic as syn
+
+    out = []
+    for i, (r, c) in enumerate(shapes):
+        s = seed + 1 + i
+        ldd = fw.PartitionedArray(syn.ldd_device(s, r, c))
+        alpha = fw.PartitionedArray(syn.uniform_device(s, syn.STREAM_ALPHA, 1.5, 2.5, (r, c)))
+        q0 = fw.PartitionedArray(syn.uniform_device(s, syn.STREAM_CURRENT, 0.0, 5.0, (r, c)))
+        out.append((f"c{i}", ldd, alpha, q0))
+    return out
+
+
+@pytest.mark.gpu
+def test_results_are_the_same_alone_and_in_the_mix(fw):
+    """Requests for several catchments overlap on 4 slots; every timestep equals the same sequence of
+    timesteps run alone, one after the other, on the default stream."""
+    from lue_amd import serving
+
+    shapes = [(300, 260), (128, 512), (700, 650), (64, 90), (400, 400), (257, 129)]
+    cats = _catchments(fw, 77, shapes)
+    server = serving.Server(64e9, slots=4, keep_results=True)
+    initial = {}
+    for name, ldd, alpha, q0 in cats:
+        server.add_catchment(name, ldd, alpha, q0)
+        initial[name] = q0.data.clone()
+    # arrivals far faster than the GPU: many requests in flight together, several per catchment
+    requests = serving.poisson_requests(5, 20000.0, 0.01, [c[0] for c in cats])
+    assert len(requests) > 50
+    done = server.run(requests)
+    torch.cuda.synchronize()
+    assert len(done) == len(requests)
+    assert all(r.latency is not None and r.latency >= 0 for r in done)
+    by_catchment = {}
+    for r in sorted(done, key=lambda r: r.seq):
+        by_catchment.setdefault(r.catchment, []).append(r)
+    for name, ldd, alpha, _ in cats:
+        alone = fw.FlowNetwork(ldd)
+        q = fw.PartitionedArray(initial[name].clone())
+        for r in by_catchment.get(name, []):
+            lateral = serving.Server.lateral_inflow(r.seed, ldd.shape, ldd.data.device)
+            q = alone.kinematic_wave(q, lateral, alpha, 0.6, 3600.0, 100.0)
+            torch.cuda.synchronize()
+            assert torch.equal(q.data, r.result.data), name
+    stats = server.statistics(1.0)
+    assert stats["requests"] == len(requests) and stats["p99_ms"] >= stats["p50_ms"]
+
+
+@pytest.mark.gpu
+def test_admission_by_hbm_budget(fw):
+    from lue_amd import serving
+
+    cats = _catchments(fw, 91, [(256, 256), (512, 512), (256, 300)])
+    one = 256 * 256 * (1 + 4 + 4 + 24)
+    server = serving.Server(one * 3, slots=2)
+    server.add_catchment(*cats[0])
+    with pytest.raises(serving.AdmissionError):
+        server.add_catchment(*cats[1])  # 4x larger: beyond the budget
+    server.add_catchment(*cats[2])
+    # transient bytes of a request (32 B/cell) do not fit next to both residents: requests are rejected
+    with pytest.raises(serving.AdmissionError):
+        server.run(serving.poisson_requests(1, 1000.0, 0.005, ["c0"]))
+    server.remove_catchment("c2")
+    done = server.run(serving.poisson_requests(2, 2000.0, 0.01, ["c0"]))
+    assert len(done) > 0 and server.transient_bytes == 0
