@@ -161,7 +161,8 @@ int lue_amd_iterate_to_new_discharge_f64(const double* upstream_discharge, const
 /*
  * Seeded synthetic inputs of the benchmark workloads (no reference counterpart: LUE's own tests and
  * benchmarks build inputs in C++). ldd from the DEM e(r,c) = (rows - r) + 50 |sin(pi c / 256)| + 0.9 u,
- * sqrt(2)-weighted steepest descent, bottom row outlets; uniform fields lo + (hi - lo) u.
+ * sqrt(2)-weighted steepest descent, bottom row outlets; uniform fields lo + (hi - lo) u. The ldd
+ * generator returns when its output is written; the uniform generators return at once (stream order).
  */
 int lue_amd_synthetic_ldd(uint64_t seed, int64_t rows, int64_t cols, uint8_t* ldd, void* stream);
 int lue_amd_synthetic_uniform_f32(uint64_t seed, uint64_t stream_id, double lo, double hi, int64_t n, float* out,

@@ -316,6 +316,26 @@ namespace lue_amd {
             atomicAdd(out, static_cast<unsigned long long>(sum));
     }
 
+    // Keep memory freed by the operations' stream-ordered buffers in the device's pool (instead of returning
+    // it to the system at every synchronisation), so repeated operations do not map their buffers anew.
+    inline hipError_t keep_pool_memory(int device)
+    {
+        static std::mutex mutex;
+        static std::vector<int> done;
+        std::lock_guard<std::mutex> lock(mutex);
+        if (std::find(done.begin(), done.end(), device) != done.end())
+            return hipSuccess;
+        hipMemPool_t pool = nullptr;
+        hipError_t e = hipDeviceGetDefaultMemPool(&pool, device);
+        if (e != hipSuccess)
+            return e;
+        std::uint64_t threshold = ~std::uint64_t{0};
+        e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+        if (e == hipSuccess)
+            done.push_back(device);
+        return e;
+    }
+
     // Describe a batch: tiles are numbered catchment after catchment.
     inline int network_init(Network& net, int n, std::uint8_t const* const* ldd, std::int64_t const* rows,
                             std::int64_t const* cols, int tile_shape, int schedule, hipStream_t stream)
@@ -329,6 +349,7 @@ namespace lue_amd {
         int tw, th;
         tile_extent(tile_shape, tw, th);
         LUE_AMD_HIP(hipGetDevice(&net.device));
+        LUE_AMD_HIP(keep_pool_memory(net.device));
         net.tile_shape = tile_shape;
         net.schedule = schedule;
         net.cats_host.clear();

@@ -370,7 +370,6 @@ namespace lue_amd {
             unsigned const grid = static_cast<unsigned>(std::min<std::int64_t>((n + 255) / 256, 65536));
             hipLaunchKernelGGL((uniform_kernel<T>), dim3(grid), dim3(256), 0, s, seed, stream_id, lo, hi, n, out);
             LUE_AMD_HIP(hipGetLastError());
-            LUE_AMD_HIP(hipStreamSynchronize(s));
             return status_success;
         }
 
