@@ -220,8 +220,15 @@ namespace lue_amd {
             c.beta = beta;
             double const dt = static_cast<double>(a.dt);
             c.c1 = dt / static_cast<double>(dx);
-            double const qb = q_cur > T{0} ? dpow(static_cast<double>(q_cur), static_cast<double>(beta))
-                                           : pow(static_cast<double>(q_cur), static_cast<double>(beta));
+            // b = alpha Q_cur^beta + dt q. FP64 path: FP64 power. FP32 path: the hardware FP32 power (relative
+            // error about 1e-7, far inside the path's tolerance), summed in FP64; the FP64 arithmetic of that
+            // path is the residual correction of the root.
+            double qb;
+            if constexpr (std::is_same_v<T, double>)
+                qb = q_cur > T{0} ? dpow(q_cur, beta) : pow(q_cur, beta);
+            else
+                qb = q_cur > T{0} ? static_cast<double>(__builtin_amdgcn_exp2f(beta * __builtin_amdgcn_logf(q_cur)))
+                                  : static_cast<double>(powf(q_cur, beta));
             c.b = ok ? static_cast<double>(alpha) * qb + dt * static_cast<double>(q_lat) : 0.0;
         }
     }
