@@ -244,10 +244,10 @@ namespace lue_amd {
         double const q_cur = static_cast<double>(d.in.q_cur);
         if (__ballot(valid && !(q_cur > 0.0 && !isnan(d.extra))))
             return false;
-        double const c1 = d.in.c1;
+        double const c1 = kw_c1(d.in);
         double const alpha = static_cast<double>(d.in.alpha);
         double const beta = static_cast<double>(d.in.beta);
-        double const b = d.in.b;
+        double const b = kw_b(d.in);
         float const c1f = static_cast<float>(c1);
         float const beta32 = static_cast<float>(d.in.beta);
         float const ab32 = static_cast<float>(d.in.alpha) * beta32;
@@ -424,6 +424,8 @@ namespace lue_amd {
             // correction of every cell in parallel, propagated along the chains to first order
             double q_up_mine = 0.0;
             float q32_mine = 0.0f;
+            float const q_lat_mine = kw_q_lat(d.in, a.dt);
+            double const b_mine = kw_b(d.in);
             for (int j = 0; j < nb; ++j)
             {
                 int const first = read_lane(static_cast<int>(d.start), j);
@@ -439,11 +441,11 @@ namespace lue_amd {
                 }
                 else
                 {
-                    float const q_lat = read_lane(d.in.q_lat, j);
+                    float const q_lat = read_lane(q_lat_mine, j);
                     float const alpha = read_lane(d.in.alpha, j);
                     float const beta = read_lane(d.in.beta, j);
-                    double const b = read_lane(d.in.b, j);
-                    double const c1 = read_lane(d.in.c1, j);
+                    double const b = read_lane(b_mine, j);
+                    double const c1 = read_lane(kw_c1(d.in), j);
                     st.have_warm = st.have_warm && uniform_beta;
                     q = kinematic_wave_root_f32(static_cast<float>(c1), alpha, beta, a.dt, q_up, q_cur, q_lat, b, c1,
                                                 st.have_warm, st.warm, st.warm_pow);
@@ -461,7 +463,7 @@ namespace lue_amd {
             if (d.valid && q > 0.0 && !isnan(q))
             {
                 kinematic_wave_correction(q, q_up_mine, static_cast<double>(d.in.alpha),
-                                          static_cast<double>(d.in.beta), d.in.b, d.in.c1, eps, sens);
+                                          static_cast<double>(d.in.beta), b_mine, kw_c1(d.in), eps, sens);
             }
             // affine scan: delta_k = eps_k + sens_k * delta_{k-1}; a chain start does not depend on
             // the cell before it, and the first cell of the block starts from a corrected value

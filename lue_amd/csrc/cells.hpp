@@ -138,6 +138,7 @@ namespace lue_amd {
         T p;
     };
 
+    // Kinematic wave, plain FP64 path.
     template<typename T>
     struct CellInput<op_kinematic_wave, T>
     {
@@ -145,9 +146,60 @@ namespace lue_amd {
         T q_lat;
         T alpha;
         T beta;
-        double b;   // alpha q_cur^beta + dt q_lat, FP64
-        double c1;  // dt / dx, FP64
+        double b;   // alpha q_cur^beta + dt q_lat
+        double c1;  // dt / dx
     };
+
+    // Kinematic wave, FP32 path (24 bytes, so that a 32x32 tile's layout kernel fits four workgroups per
+    // CU): b as an unevaluated pair of floats (about 48 significant bits); the lateral inflow itself is
+    // recovered from b where LUE's iteration needs it (kw_q_lat).
+    template<>
+    struct CellInput<op_kinematic_wave, float>
+    {
+        float q_cur;  // NaN when any argument is no-data or outside the domain
+        float alpha;
+        float beta;
+        float c1;     // dt / dx
+        float b_hi;   // b = alpha q_cur^beta + dt q_lat = b_hi + b_lo
+        float b_lo;
+    };
+
+    template<typename T>
+    __device__ inline double kw_b(CellInput<op_kinematic_wave, T> const& c)
+    {
+        if constexpr (std::is_same_v<T, float>)
+            return static_cast<double>(c.b_hi) + static_cast<double>(c.b_lo);
+        else
+            return c.b;
+    }
+
+    template<typename T>
+    __device__ inline double kw_c1(CellInput<op_kinematic_wave, T> const& c)
+    {
+        return static_cast<double>(c.c1);
+    }
+
+    // Q_cur^beta of the FP32 path: the hardware FP32 power (pow for Q_cur = 0).
+    __device__ inline float kw_pow32(float q_cur, float beta)
+    {
+        return q_cur > 0.0f ? __builtin_amdgcn_exp2f(beta * __builtin_amdgcn_logf(q_cur)) : powf(q_cur, beta);
+    }
+
+    // Lateral inflow of a cell: stored (FP64 path) or recovered from b = alpha Q_cur^beta + dt q (FP32 path,
+    // where b was formed with the same power, so the difference is dt q up to FP64 rounding; exact for
+    // Q_cur = 0, the one case where LUE's dryness test reads q). The iteration uses q only as dt q, so for
+    // dt = 0 it is taken as 0.
+    template<typename T>
+    __device__ inline T kw_q_lat(CellInput<op_kinematic_wave, T> const& c, T dt)
+    {
+        if constexpr (std::is_same_v<T, float>)
+            return dt != 0.0f ? static_cast<float>((kw_b(c) - static_cast<double>(c.alpha) *
+                                                                   static_cast<double>(kw_pow32(c.q_cur, c.beta))) /
+                                                   static_cast<double>(dt))
+                              : 0.0f;
+        else
+            return c.q_lat;
+    }
 
 
     template<typename T>
@@ -214,22 +266,31 @@ namespace lue_amd {
             // value_policies/kinematic_wave.hpp:9-22 plus no-data (NaN) in any argument
             bool const ok = q_cur >= T{0} && q_lat >= T{0} && dx > T{0} && !isnan(alpha) && !isnan(beta) &&
                             !isnan(a.dt);
-            c.q_cur = ok ? q_cur : quiet_nan<T>();
-            c.q_lat = q_lat;
-            c.alpha = alpha;
-            c.beta = beta;
             double const dt = static_cast<double>(a.dt);
-            c.c1 = dt / static_cast<double>(dx);
-            // b = alpha Q_cur^beta + dt q. FP64 path: FP64 power. FP32 path: the hardware FP32 power (relative
-            // error about 1e-7, far inside the path's tolerance), summed in FP64; the FP64 arithmetic of that
-            // path is the residual correction of the root.
-            double qb;
-            if constexpr (std::is_same_v<T, double>)
-                qb = q_cur > T{0} ? dpow(q_cur, beta) : pow(q_cur, beta);
+            if constexpr (std::is_same_v<T, float>)
+            {
+                // b with the hardware FP32 power (relative error about 1e-7, far inside the path's tolerance),
+                // summed in FP64 and kept as an FP32 pair
+                double const b = ok ? static_cast<double>(alpha) * static_cast<double>(kw_pow32(q_cur, beta)) +
+                                          dt * static_cast<double>(q_lat)
+                                    : 0.0;
+                c.q_cur = ok ? q_cur : quiet_nan<T>();
+                c.alpha = alpha;
+                c.beta = beta;
+                c.c1 = static_cast<float>(dt / static_cast<double>(dx));
+                c.b_hi = static_cast<float>(b);
+                c.b_lo = static_cast<float>(b - static_cast<double>(c.b_hi));
+            }
             else
-                qb = q_cur > T{0} ? static_cast<double>(__builtin_amdgcn_exp2f(beta * __builtin_amdgcn_logf(q_cur)))
-                                  : static_cast<double>(powf(q_cur, beta));
-            c.b = ok ? static_cast<double>(alpha) * qb + dt * static_cast<double>(q_lat) : 0.0;
+            {
+                c.q_cur = ok ? q_cur : quiet_nan<T>();
+                c.q_lat = q_lat;
+                c.alpha = alpha;
+                c.beta = beta;
+                c.c1 = dt / dx;
+                double const qb = q_cur > T{0} ? dpow(q_cur, beta) : pow(q_cur, beta);
+                c.b = ok ? alpha * qb + dt * q_lat : 0.0;
+            }
         }
     }
 
@@ -306,7 +367,8 @@ namespace lue_amd {
             }
             else
             {
-                out = new_discharge_f32_path(upstream, in.q_cur, in.q_lat, in.alpha, in.beta, a.dt, in.b, in.c1);
+                out = new_discharge_f32_path(upstream, in.q_cur, kw_q_lat(in, a.dt), in.alpha, in.beta, a.dt, kw_b(in),
+                                             kw_c1(in));
             }
         }
     }

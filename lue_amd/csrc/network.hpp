@@ -893,7 +893,7 @@ namespace lue_amd {
             if (int const st = timer.start(stream))
                 return st;
         }
-        hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH>),
+        hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH, false>),
                            dim3(static_cast<unsigned>(net.ntiles)), dim3(sweep_threads), 0,
                            stream, a);
         LUE_AMD_HIP(hipGetLastError());
@@ -991,7 +991,11 @@ namespace lue_amd {
         OpTimer timer;
         if (int const st = timer.start(stream))
             return st;
-        hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH>),
+        if (chains)
+            hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH, true>), dim3(static_cast<unsigned>(net.ntiles)),
+                               dim3(sweep_threads), 0, stream, a);
+        else
+            hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH, false>),
                            dim3(static_cast<unsigned>(net.ntiles)), dim3(sweep_threads), 0,
                            stream, a);
         LUE_AMD_HIP(hipGetLastError());

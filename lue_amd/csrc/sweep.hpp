@@ -43,7 +43,27 @@
 
 namespace lue_amd {
 
-    template<int OP, typename T, int TW, int TH>
+    // LDS of the schedule inside a tile: the walks' counters, links and ready queue, or the recorded in-tile
+    // chain layout (positions and flags).
+    template<bool LAYOUT, int N>
+    struct TileScheduleLds;
+
+    template<int N>
+    struct TileScheduleLds<false, N>
+    {
+        std::uint32_t count[N / 4];  // four 8-bit counters per word
+        std::int16_t next[N];        // downstream cell inside the tile, -1 if none
+        std::uint16_t queue[N];      // initially ready cells
+    };
+
+    template<int N>
+    struct TileScheduleLds<true, N>
+    {
+        std::uint16_t order[N];  // local cell at each position
+        std::uint8_t flags[N];   // chain start, heavy child direction
+    };
+
+    template<int OP, typename T, int TW, int TH, bool LAYOUT = false>
     struct SweepLds
     {
         using G = Tile<TW, TH>;
@@ -54,9 +74,7 @@ namespace lue_amd {
         T hi[G::n];
         float lo[compensated ? G::n : 1];
         T rem[two_outputs ? G::n : 1];
-        std::uint32_t count[G::n / 4];  // four 8-bit counters per word
-        std::int16_t next[G::n];        // downstream cell inside the tile, -1 if none
-        std::uint16_t queue[G::n];      // initially ready cells
+        TileScheduleLds<LAYOUT, G::n> sched;
         std::uint8_t ldd[G::hn];
         std::uint8_t upmask[G::n];
         std::uint8_t state[G::n];  // see state_* below
@@ -76,13 +94,13 @@ namespace lue_amd {
     constexpr std::uint8_t state_new = 3;      // solved in this round (value in LDS)
 
     // One round over one tile. All threads of the workgroup call it.
-    template<int OP, typename T, int TW, int TH>
-    __device__ void sweep_tile(SweepArgs<T> const& a, SweepLds<OP, T, TW, TH>& s, std::int64_t tile,
+    template<int OP, typename T, int TW, int TH, bool LAYOUT>
+    __device__ void sweep_tile(SweepArgs<T> const& a, SweepLds<OP, T, TW, TH, LAYOUT>& s, std::int64_t tile,
                                std::uint32_t round)
     {
         using G = Tile<TW, TH>;
-        constexpr bool compensated = SweepLds<OP, T, TW, TH>::compensated;
-        constexpr bool two_outputs = SweepLds<OP, T, TW, TH>::two_outputs;
+        constexpr bool compensated = SweepLds<OP, T, TW, TH, LAYOUT>::compensated;
+        constexpr bool two_outputs = SweepLds<OP, T, TW, TH, LAYOUT>::two_outputs;
         constexpr int cells_per_thread = (G::n + sweep_threads - 1) / sweep_threads;
 
         int const tid = threadIdx.x;
@@ -118,7 +136,8 @@ namespace lue_amd {
         // cell, and with a recorded in-tile layout its order and flags. All loads of a thread are issued
         // before the first one is used.
         constexpr int halo_per_thread = (G::hn + sweep_threads - 1) / sweep_threads;
-        bool const layout = round == 0 && a.tile_order != nullptr;
+        // the recorded in-tile layout replaces the walks (round 0 of the chain schedule)
+        constexpr bool layout = LAYOUT;
         RawInput<OP, T> raw[cells_per_thread];
         if (round == 0)
         {
@@ -143,9 +162,7 @@ namespace lue_amd {
             std::int64_t const c = c0 + hc - 1;
             halo[j] = (i < G::hn && r >= 0 && r < rows && c >= 0 && c < cols) ? g_ldd[r * cols + c] : no_data_ldd;
         }
-        // the walk-only arrays queue and count hold the layout's order and flags
-        std::uint8_t* const layout_flags = reinterpret_cast<std::uint8_t*>(s.count);
-        if (layout)
+        if constexpr (layout)
         {
             auto const order = as_global(a.tile_order + tile * G::n);
             auto const flags = as_global(a.tile_flags + tile * G::n);
@@ -167,8 +184,8 @@ namespace lue_amd {
                 int const p = tid + j * nthreads;
                 if (p < G::n)
                 {
-                    s.queue[p] = o[j];
-                    layout_flags[p] = f[j];
+                    s.sched.order[p] = o[j];
+                    s.sched.flags[p] = f[j];
                 }
             }
         }
@@ -235,7 +252,7 @@ namespace lue_amd {
             }
             s.state[i] = st;
             s.upmask[i] = mask;
-            if (!layout)
+            if constexpr (!layout)
             {
                 int next = -1;
                 if (st != state_absent)
@@ -250,12 +267,12 @@ namespace lue_amd {
                             next = dr * TW + dc;
                     }
                 }
-                s.next[i] = static_cast<std::int16_t>(next);
+                s.sched.next[i] = static_cast<std::int16_t>(next);
             }
         }
         __syncthreads();
 
-        if (layout)
+        if constexpr (layout)
         {
             // 3-4 (recorded in-tile chains, tile_layout.hpp): level after level; wavefront w solves the
             // positions [split[l][w], split[l][w + 1]) in blocks of 64. The light upstream cells of a chain
@@ -280,8 +297,8 @@ namespace lue_amd {
                     d.valid = p < hi;
                     if (d.valid)
                     {
-                        i = s.queue[p] & (G::n - 1);  // in bounds whatever the recorded data
-                        std::uint8_t const f = layout_flags[p];
+                        i = s.sched.order[p] & (G::n - 1);  // in bounds whatever the recorded data
+                        std::uint8_t const f = s.sched.flags[p];
                         d.start = (f & 1u) != 0;
                         int const heavy = (f & 16u) ? ((f >> 1) & 7) : -1;
                         int const lr = i / TW;
@@ -370,7 +387,7 @@ namespace lue_amd {
                         is_ready = n == 0;
                     }
                     // 8-bit counter, written as a byte of the packed word
-                    reinterpret_cast<std::uint8_t*>(s.count)[i] = static_cast<std::uint8_t>(n);
+                    reinterpret_cast<std::uint8_t*>(s.sched.count)[i] = static_cast<std::uint8_t>(n);
                 }
                 std::uint64_t const ballot = __ballot(is_ready);
                 if (ballot)
@@ -380,7 +397,7 @@ namespace lue_amd {
                         base = atomicAdd(&s.queued, static_cast<unsigned int>(__popcll(ballot)));
                     base = __shfl(base, 0, 64);
                     if (is_ready)
-                        s.queue[base + __popcll(ballot & lanes_below)] = static_cast<std::uint16_t>(i);
+                        s.sched.queue[base + __popcll(ballot & lanes_below)] = static_cast<std::uint16_t>(i);
                 }
             }
             __syncthreads();
@@ -424,10 +441,6 @@ namespace lue_amd {
                 }
             };
 
-            // Kinematic wave FP32: the walks solve the FP32 roots only; the FP64 correction of every cell is
-            // computed afterwards in parallel and propagated along the flow paths of the tile (first order).
-            constexpr bool deferred = OP == op_kinematic_wave && compensated;
-
             // 4. walks: lanes take initially ready cells from the queue and walk downstream while they are the
             // last upstream contributor of the next cell
             unsigned int const queued = s.queued;
@@ -436,53 +449,30 @@ namespace lue_amd {
                 unsigned int const item = atomicAdd(&s.taken, 1u);
                 if (item >= queued)
                     break;
-                int i = s.queue[item];
-                bool have_warm = false;
-                float warm_x = 0.0f;
-                float warm_pow = 0.0f;
+                int i = s.sched.queue[item];
                 while (true)
                 {
-                    int const next = s.next[i];
+                    int const next = s.sched.next[i];
                     double upstream;
                     bool poisoned;
                     gather(i, upstream, poisoned);
-                    if constexpr (deferred)
-                    {
-                        CellInput<OP, T> const& in = s.in[i];
-                        float q;
-                        if (poisoned || isnan(in.q_cur))
-                        {
-                            q = __int_as_float(0x7fc00000);
-                            have_warm = false;
-                        }
-                        else
-                        {
-                            have_warm = have_warm && io.beta == nullptr;
-                            q = kinematic_wave_root_f32(static_cast<float>(in.c1), in.alpha, in.beta, a.dt, upstream,
-                                                        in.q_cur, in.q_lat, in.b, in.c1, have_warm, warm_x, warm_pow);
-                        }
-                        s.hi[i] = q;
-                        s.lo[i] = 0.0f;
-                    }
-                    else
-                    {
-                        double out;
-                        T rem{};
-                        solve_cell<OP, T>(a, s.in[i], upstream, poisoned, out, rem);
-                        T const hi = static_cast<T>(out);
-                        s.hi[i] = hi;
-                        if constexpr (compensated)
-                            s.lo[i] = isnan(out) ? 0.0f : static_cast<float>(out - static_cast<double>(hi));
-                        if constexpr (two_outputs)
-                            s.rem[i] = rem;
-                    }
+                    double out;
+                    T rem{};
+                    solve_cell<OP, T>(a, s.in[i], upstream, poisoned, out, rem);
+                    T const hi = static_cast<T>(out);
+                    s.hi[i] = hi;
+                    if constexpr (compensated)
+                        s.lo[i] = isnan(out) ? 0.0f : static_cast<float>(out - static_cast<double>(hi));
+                    if constexpr (two_outputs)
+                        s.rem[i] = rem;
+
                     // publish: the state write is ordered before the counter update (release)
                     __hip_atomic_store(&s.state[i], state_new, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     if (next < 0)
                         break;
                     std::uint32_t const shift = 8u * static_cast<std::uint32_t>(next & 3);
                     std::uint32_t const old = __hip_atomic_fetch_sub(
-                        &s.count[next >> 2], 1u << shift, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        &s.sched.count[next >> 2], 1u << shift, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
                     if (((old >> shift) & 0xFFu) != 1u)
                         break;  // confluence: another stream still has to arrive
                     i = next;
@@ -490,120 +480,6 @@ namespace lue_amd {
             }
             __syncthreads();
 
-            if constexpr (deferred)
-            {
-                // 4b. per cell solved in this round: eps = one FP64 Newton step at the FP32 root, sens = d root /
-                // d upstream; stored in place of the no longer needed inputs (b <- eps, alpha <- sens). Count the
-                // upstream cells solved in this round for the propagation walk.
-                if (tid == 0)
-                {
-                    s.queued = 0;
-                    s.taken = 0;
-                }
-                __syncthreads();
-                for (int j = 0; j < cells_per_thread; ++j)
-                {
-                    int const i = tid + j * nthreads;
-                    bool is_ready = false;
-                    if (i < G::n && s.state[i] == state_new)
-                    {
-                        CellInput<OP, T>& in = s.in[i];
-                        double const q = static_cast<double>(s.hi[i]);
-                        double eps = 0.0;
-                        double sens = 0.0;
-                        if (q > 0.0 && !isnan(q))
-                        {
-                            double upstream;
-                            bool poisoned;
-                            gather(i, upstream, poisoned);
-                            kinematic_wave_correction(q, upstream, static_cast<double>(in.alpha), static_cast<double>(in.beta),
-                                                      in.b, in.c1, eps, sens);
-                        }
-                        in.b = eps;
-                        in.alpha = static_cast<float>(sens);
-                        int const lr = i / TW;
-                        int const lc = i % TW;
-                        std::uint8_t const mask = s.upmask[i];
-                        std::uint32_t n = 0;
-                        for (int k = 0; k < 8; ++k)
-                        {
-                            int const ur = lr + neighbour_dr(k);
-                            int const uc = lc + neighbour_dc(k);
-                            if (((mask >> k) & 1u) && ur >= 0 && ur < TH && uc >= 0 && uc < TW &&
-                                s.state[ur * TW + uc] == state_new)
-                                ++n;
-                        }
-                        reinterpret_cast<std::uint8_t*>(s.count)[i] = static_cast<std::uint8_t>(n);
-                        is_ready = n == 0;
-                    }
-                    std::uint64_t const ballot = __ballot(is_ready);
-                    if (ballot)
-                    {
-                        unsigned int base = 0;
-                        if (lane == 0)
-                            base = atomicAdd(&s.queued, static_cast<unsigned int>(__popcll(ballot)));
-                        base = __shfl(base, 0, 64);
-                        if (is_ready)
-                            s.queue[base + __popcll(ballot & lanes_below)] = static_cast<std::uint16_t>(i);
-                    }
-                }
-                __syncthreads();
-                // 4c. propagate: delta = eps + sens * sum(delta of upstream cells solved in this round); delta is
-                // kept in place of c1
-                unsigned int const queued2 = s.queued;
-                while (true)
-                {
-                    unsigned int const item = atomicAdd(&s.taken, 1u);
-                    if (item >= queued2)
-                        break;
-                    int i = s.queue[item];
-                    while (true)
-                    {
-                        int const lr = i / TW;
-                        int const lc = i % TW;
-                        std::uint8_t const mask = s.upmask[i];
-                        double sum = 0.0;
-                        for (int k = 0; k < 8; ++k)
-                        {
-                            int const ur = lr + neighbour_dr(k);
-                            int const uc = lc + neighbour_dc(k);
-                            if (((mask >> k) & 1u) && ur >= 0 && ur < TH && uc >= 0 && uc < TW &&
-                                s.state[ur * TW + uc] == state_new)
-                                sum += s.in[ur * TW + uc].c1;
-                        }
-                        CellInput<OP, T>& in = s.in[i];
-                        in.c1 = in.b + static_cast<double>(in.alpha) * sum;
-                        int next = s.next[i];
-                        if (next >= 0 && s.state[next] != state_new)
-                            next = -1;
-                        if (next < 0)
-                            break;
-                        std::uint32_t const shift = 8u * static_cast<std::uint32_t>(next & 3);
-                        std::uint32_t const old = __hip_atomic_fetch_sub(
-                            &s.count[next >> 2], 1u << shift, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        if (((old >> shift) & 0xFFu) != 1u)
-                            break;
-                        i = next;
-                    }
-                }
-                __syncthreads();
-                // 4d. corrected values
-                for (int i = tid; i < G::n; i += nthreads)
-                {
-                    if (s.state[i] != state_new)
-                        continue;
-                    float const q = s.hi[i];
-                    if (isnan(q))
-                        continue;
-                    double v = static_cast<double>(q) + s.in[i].c1;
-                    if (!(v > 0.0))
-                        v = static_cast<double>(q);
-                    float const hi = static_cast<float>(v);
-                    s.hi[i] = hi;
-                    s.lo[i] = static_cast<float>(v - static_cast<double>(hi));
-                }
-                __syncthreads();
-            }
         }
 
         // 5. write back: outputs of cells solved in this round (coalesced), compensation terms that later
@@ -752,34 +628,36 @@ namespace lue_amd {
 
     // Workgroups of a tile kernel that fit one CU (160 KB of LDS, 32 wavefront slots), and the wavefronts per
     // SIMD that follow: the register budget the kernels are compiled for.
-    template<int OP, typename T, int TW, int TH>
+    template<int OP, typename T, int TW, int TH, bool LAYOUT = false>
     constexpr int sweep_workgroups_per_cu()
     {
-        int const by_lds = static_cast<int>((160 * 1024) / sizeof(SweepLds<OP, T, TW, TH>));
+        int const by_lds = static_cast<int>((160 * 1024) / sizeof(SweepLds<OP, T, TW, TH, LAYOUT>));
         int const by_waves = 32 / sweep_waves;
         int const n = by_lds < by_waves ? by_lds : by_waves;
         return n < 1 ? 1 : n;
     }
 
-    template<int OP, typename T, int TW, int TH>
+    template<int OP, typename T, int TW, int TH, bool LAYOUT = false>
     constexpr int sweep_waves_per_simd()
     {
-        return sweep_workgroups_per_cu<OP, T, TW, TH>() * sweep_waves / 4;
+        return sweep_workgroups_per_cu<OP, T, TW, TH, LAYOUT>() * sweep_waves / 4;
     }
 
-    // Round 0: every tile of the batch, one workgroup per tile (grid = number of tiles).
-    template<int OP, typename T, int TW, int TH>
-    __global__ void __launch_bounds__(sweep_threads, (sweep_waves_per_simd<OP, T, TW, TH>())) sweep_first_round(SweepArgs<T> a)
+    // Round 0: every tile of the batch, one workgroup per tile (grid = number of tiles). LAYOUT: with the
+    // recorded in-tile chain layout (chain schedule replays), else with walks (recording, rounds schedule).
+    template<int OP, typename T, int TW, int TH, bool LAYOUT>
+    __global__ void __launch_bounds__(sweep_threads, (sweep_waves_per_simd<OP, T, TW, TH, LAYOUT>()))
+        sweep_first_round(SweepArgs<T> a)
     {
-        __shared__ SweepLds<OP, T, TW, TH> s;
-        sweep_tile<OP, T, TW, TH>(a, s, blockIdx.x, 0u);
+        __shared__ SweepLds<OP, T, TW, TH, LAYOUT> s;
+        sweep_tile<OP, T, TW, TH, LAYOUT>(a, s, blockIdx.x, 0u);
     }
 
     // Round r >= 1 while recording: the tiles queued by round r - 1. Workgroup b takes items b, b + grid, ...
     template<int OP, typename T, int TW, int TH>
     __global__ void __launch_bounds__(sweep_threads, (sweep_waves_per_simd<OP, T, TW, TH>())) sweep_next_round(SweepArgs<T> a, std::uint32_t round)
     {
-        __shared__ SweepLds<OP, T, TW, TH> s;
+        __shared__ SweepLds<OP, T, TW, TH, false> s;
         std::uint32_t const slot = round % 3;
         if (blockIdx.x == 0 && threadIdx.x == 0)
         {
@@ -792,7 +670,7 @@ namespace lue_amd {
             std::int64_t const tile = a.worklist[slot * a.ntiles + item];
             if (threadIdx.x == 0)
                 a.tile_flag[slot * a.ntiles + tile] = 0u;
-            sweep_tile<OP, T, TW, TH>(a, s, tile, round);
+            sweep_tile<OP, T, TW, TH, false>(a, s, tile, round);
         }
     }
 
@@ -801,10 +679,10 @@ namespace lue_amd {
     __global__ void __launch_bounds__(sweep_threads, (sweep_waves_per_simd<OP, T, TW, TH>())) sweep_list_round(SweepArgs<T> a, std::uint32_t const* tiles,
                                                             std::uint32_t count, std::uint32_t round)
     {
-        __shared__ SweepLds<OP, T, TW, TH> s;
+        __shared__ SweepLds<OP, T, TW, TH, false> s;
         for (std::uint32_t item = blockIdx.x; item < count; item += gridDim.x)
         {
-            sweep_tile<OP, T, TW, TH>(a, s, tiles[item], round);
+            sweep_tile<OP, T, TW, TH, false>(a, s, tiles[item], round);
         }
     }
 
