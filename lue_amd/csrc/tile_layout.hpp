@@ -11,7 +11,8 @@
 //
 // Recorded per tile: order[p] = local cell index at position p (chains level after level, each in flow
 // order), flags[p] = chain start (bit 0) | heavy child direction (bits 1-3) | has heavy child (bit 4),
-// and per level the positions of the sweep_waves wavefronts: split[l][w], w = 0..sweep_waves.
+// and per level the positions of the sweep_waves wavefronts: split[l][w], w = 0..sweep_waves (parts made of
+// whole chains, packed so that the level needs few 64-position blocks).
 #pragma once
 
 #include "cells.hpp"
@@ -238,46 +239,75 @@ namespace lue_amd {
         }
         __syncthreads();
 
-        // positions: levels one after the other, chains in cell order inside a level; each level is cut at
-        // chain boundaries into sweep_waves parts of about equal size, one per wavefront of the tile kernel
+        // positions: levels one after the other. Inside a level, each of the sweep_waves wavefronts of the tile
+        // kernel takes a contiguous part made of whole chains and solves it in blocks of 64 positions, so a
+        // level costs the sum over the parts of ceil(part / 64) blocks. The level's ceil(cells / 64) blocks
+        // are dealt out to the wavefronts as capacities (multiples of 64) and the chains, in cell order, go
+        // first-fit into the parts (a chain that fits nowhere goes to the part with the most room left).
         if (tid == 0)
         {
             unsigned int const levels = s.levels;
-            std::uint32_t level_begin[tile_max_levels];
-            std::uint32_t cursor[tile_max_levels];
-            int wave[tile_max_levels];
+            std::uint32_t cap[tile_max_levels][sweep_waves];
+            std::uint32_t part[tile_max_levels][sweep_waves];
+            for (unsigned int l = 0; l < levels; ++l)
+            {
+                std::uint32_t const blocks = (s.level_cells[l] + 63) / 64;
+                for (int w = 0; w < sweep_waves; ++w)
+                {
+                    cap[l][w] = 64 * ((blocks + sweep_waves - 1 - w) / sweep_waves);
+                    part[l][w] = 0;
+                }
+            }
+            // pass 1: the part of every chain (kept in base[] until pass 2)
+            for (int i = 0; i < G::n; ++i)
+            {
+                if (!s.local[i] || s.head_of[i] != i)
+                    continue;
+                std::uint32_t const l = s.level[i];
+                if (l >= levels)
+                {
+                    s.base[i] = 0xffff;  // too many levels (error set above): no position
+                    continue;
+                }
+                std::uint32_t const len = s.length[i];
+                int chosen = -1;
+                int roomiest = 0;
+                for (int w = 0; w < sweep_waves; ++w)
+                {
+                    std::int64_t const room = static_cast<std::int64_t>(cap[l][w]) - part[l][w];
+                    if (chosen < 0 && room >= static_cast<std::int64_t>(len))
+                        chosen = w;
+                    if (room > static_cast<std::int64_t>(cap[l][roomiest]) - part[l][roomiest])
+                        roomiest = w;
+                }
+                if (chosen < 0)
+                    chosen = roomiest;
+                part[l][chosen] += len;
+                s.base[i] = static_cast<std::uint16_t>(chosen);
+            }
+            // part boundaries, then pass 2: positions
             std::uint32_t at = 0;
             for (unsigned int l = 0; l < levels; ++l)
             {
-                level_begin[l] = at;
-                cursor[l] = at;
-                wave[l] = 0;
-                s.split[l * (sweep_waves + 1)] = static_cast<std::uint16_t>(at);
-                at += s.level_cells[l];
+                for (int w = 0; w < sweep_waves; ++w)
+                {
+                    s.split[l * (sweep_waves + 1) + w] = static_cast<std::uint16_t>(at);
+                    std::uint32_t const size = part[l][w];
+                    part[l][w] = at;  // cursor from here on
+                    at += size;
+                }
+                s.split[l * (sweep_waves + 1) + sweep_waves] = static_cast<std::uint16_t>(at);
             }
             for (int i = 0; i < G::n; ++i)
             {
                 if (!s.local[i] || s.head_of[i] != i)
                     continue;
                 std::uint32_t const l = s.level[i];
-                std::uint32_t const b = cursor[l];
-                std::uint32_t const target = (s.level_cells[l] + sweep_waves - 1) / sweep_waves;
-                while (wave[l] < sweep_waves - 1 && b - level_begin[l] >= static_cast<std::uint32_t>(wave[l] + 1) * target)
-                {
-                    ++wave[l];
-                    s.split[l * (sweep_waves + 1) + wave[l]] = static_cast<std::uint16_t>(b);
-                }
-                s.base[i] = static_cast<std::uint16_t>(b);
-                cursor[l] = b + s.length[i];
-            }
-            for (unsigned int l = 0; l < levels; ++l)
-            {
-                std::uint32_t const end = level_begin[l] + s.level_cells[l];
-                while (wave[l] < sweep_waves)
-                {
-                    ++wave[l];
-                    s.split[l * (sweep_waves + 1) + wave[l]] = static_cast<std::uint16_t>(end);
-                }
+                if (l >= levels)
+                    continue;
+                int const w = s.base[i];
+                s.base[i] = static_cast<std::uint16_t>(part[l][w]);
+                part[l][w] += s.length[i];
             }
             for (unsigned int l = levels; l < static_cast<unsigned int>(tile_max_levels); ++l)
                 for (int w = 0; w <= sweep_waves; ++w)
