@@ -54,9 +54,9 @@ namespace lue_amd {
         return exp(y * log(x));
     }
 
-    // x^y for finite x > 0 and |y ln x| < 700, relative error below 2^-53 |y ln x| + 1e-15 (5e-14 at worst;
-    // about 40 FP64 operations, a third of exp(y log x) with the library's full-precision exp and log).
-    // Used by the block Newton iterations, whose roots it moves by about its own error.
+    // x^y for finite x > 0 and |y ln x| < 700, relative error below 2^-52 |y ln x| + 1e-15 (the roundings of
+    // ln x and y ln x; about 40 FP64 operations, a third of exp(y log x) with the library's full-precision
+    // exp and log). Used by the block Newton iterations, whose roots it moves by about its own error.
     //   ln x = e ln 2 + 2 atanh(s), s = (m - 1) / (m + 1), m in [sqrt(1/2), sqrt(2)), |s| < 0.1716: the
     //   atanh series to s^15 leaves 2 |s|^17 / 17 < 1.1e-14;
     //   e^z = 2^k e^r, k = rint(z / ln 2), |r| <= ln 2 / 2 (ln 2 in two parts): Taylor to r^12 leaves
