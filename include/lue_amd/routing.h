@@ -75,6 +75,14 @@ int lue_amd_set_sweep_options(int schedule);
  */
 int lue_amd_set_network_cache(int entries);
 
+/*
+ * Per-call device workspaces come from a cache of device blocks that later calls reuse (a block released
+ * on a stream is reused on that stream at once, on another once its last use has completed). This frees
+ * the idle blocks of the current device, waiting for their last use. No counterpart in the reference
+ * (LUE allocates per partition task).
+ */
+int lue_amd_trim_scratch(void);
+
 /* Device memory used by the C++ front end's device-resident partitions. */
 int lue_amd_device_malloc(void** ptr, size_t bytes);
 int lue_amd_device_free(void* ptr);

@@ -28,6 +28,7 @@ SIGNATURES = {
     "lue_amd_version": [],
     "lue_amd_set_sweep_options": [c_int],
     "lue_amd_set_network_cache": [c_int],
+    "lue_amd_trim_scratch": [],
     "lue_amd_network_create": [c_int, c_p, c_p, c_p, c_int, c_int, c_p, ctypes.POINTER(c_p)],
     "lue_amd_network_destroy": [c_p],
     "lue_amd_network_info": [c_p, c_p, c_int],
@@ -113,6 +114,11 @@ def set_sweep_options(schedule="chains"):
 def set_network_cache(entries=4):
     """Schedules the per-call operations keep (0: record in every call)."""
     check(load().lue_amd_set_network_cache(entries))
+
+
+def trim_scratch():
+    """Free the cached per-call device workspaces of the current device that are idle."""
+    check(load().lue_amd_trim_scratch())
 
 
 def profile_enable(enable=True):

@@ -104,6 +104,167 @@ namespace lue_amd {
         }
     };
 
+    // Device blocks for the per-call workspaces, kept for reuse by later calls instead of a stream-ordered
+    // malloc and free per call (hipFreeAsync has taken tens of milliseconds at times, stalling a caller that
+    // serves many small calls). A released block records an event on the releasing stream; it is handed
+    // out again on that stream at once (stream order), on another stream once the event has completed.
+    // Best fit among the free blocks of at most twice (plus 4 MiB) the request; new blocks come from
+    // hipMalloc, rounded up to 256 KiB. Blocks stay allocated until trim() (an allocation that fails frees the
+    // idle blocks and tries again).
+    class ScratchCache
+    {
+    public:
+        struct Block
+        {
+            void* ptr{};
+            std::size_t bytes{};
+            int device{};
+            hipStream_t stream{};
+            hipEvent_t done{};
+            bool busy{};
+        };
+
+        hipError_t acquire(std::size_t bytes, hipStream_t stream, Block*& out)
+        {
+            out = nullptr;
+            int device = 0;
+            hipError_t e = hipGetDevice(&device);
+            if (e != hipSuccess)
+                return e;
+            bytes = std::max<std::size_t>(bytes, 1);
+            std::size_t const limit = 2 * bytes + (std::size_t{4} << 20);
+            std::lock_guard<std::mutex> lock(mutex_);
+            Block* best = nullptr;
+            for (auto& b : blocks_)
+            {
+                if (b->busy || b->device != device || b->bytes < bytes || b->bytes > limit)
+                    continue;
+                if (best && b->bytes >= best->bytes)
+                    continue;
+                if (b->stream != stream && hipEventQuery(b->done) != hipSuccess)
+                    continue;
+                best = b.get();
+            }
+            if (!best)
+            {
+                auto b = std::make_unique<Block>();
+                b->bytes = (bytes + block_granule - 1) / block_granule * block_granule;
+                b->device = device;
+                e = hipMalloc(&b->ptr, b->bytes);
+                if (e == hipErrorOutOfMemory)
+                {
+                    (void)hipGetLastError();
+                    if ((e = trim_locked(device)) != hipSuccess)
+                        return e;
+                    e = hipMalloc(&b->ptr, b->bytes);
+                }
+                if (e != hipSuccess)
+                    return e;
+                if ((e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming)) != hipSuccess)
+                {
+                    (void)hipFree(b->ptr);
+                    return e;
+                }
+                best = b.get();
+                blocks_.push_back(std::move(b));
+            }
+            best->busy = true;
+            best->stream = stream;
+            out = best;
+            return hipSuccess;
+        }
+
+        void release(Block* b, hipStream_t stream)
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            int current = 0;
+            bool const switched = hipGetDevice(&current) == hipSuccess && current != b->device &&
+                                  hipSetDevice(b->device) == hipSuccess;
+            (void)hipEventRecord(b->done, stream);
+            if (switched)
+                (void)hipSetDevice(current);
+            b->stream = stream;
+            b->busy = false;
+        }
+
+        // Free the idle blocks of the current device (waits for their last use).
+        hipError_t trim()
+        {
+            int device = 0;
+            hipError_t const e = hipGetDevice(&device);
+            if (e != hipSuccess)
+                return e;
+            std::lock_guard<std::mutex> lock(mutex_);
+            return trim_locked(device);
+        }
+
+    private:
+        static constexpr std::size_t block_granule = std::size_t{256} << 10;
+
+        hipError_t trim_locked(int device)
+        {
+            hipError_t result = hipSuccess;
+            std::vector<std::unique_ptr<Block>> kept;
+            for (auto& b : blocks_)
+            {
+                if (b->busy || b->device != device)
+                {
+                    kept.push_back(std::move(b));
+                    continue;
+                }
+                hipError_t e = hipEventSynchronize(b->done);
+                if (e == hipSuccess)
+                    e = hipEventDestroy(b->done);
+                if (e == hipSuccess)
+                    e = hipFree(b->ptr);
+                if (e != hipSuccess && result == hipSuccess)
+                    result = e;
+            }
+            blocks_ = std::move(kept);
+            return result;
+        }
+
+        std::mutex mutex_;
+        std::vector<std::unique_ptr<Block>> blocks_;
+    };
+    ScratchCache& scratch_cache();
+
+    // Per-call device buffer from the scratch cache (same interface as DeviceArray).
+    template<typename P>
+    struct ScratchArray
+    {
+        P* ptr{};
+        hipStream_t stream{};
+        std::size_t count{};
+        ScratchCache::Block* block{};
+
+        ScratchArray() = default;
+        ScratchArray(ScratchArray const&) = delete;
+        auto operator=(ScratchArray const&) -> ScratchArray& = delete;
+        ~ScratchArray() { reset(); }
+
+        void reset()
+        {
+            if (block)
+                scratch_cache().release(block, stream);
+            block = nullptr;
+            ptr = nullptr;
+            count = 0;
+        }
+
+        hipError_t allocate(std::size_t n, hipStream_t s)
+        {
+            reset();
+            stream = s;
+            hipError_t const e = scratch_cache().acquire(std::max<std::size_t>(n, 1) * sizeof(P), s, block);
+            if (e != hipSuccess)
+                return e;
+            count = n;
+            ptr = static_cast<P*>(block->ptr);
+            return hipSuccess;
+        }
+    };
+
     template<typename In, typename Out>
     hipError_t exclusive_scan(In const* in, Out* out, std::size_t n, hipStream_t stream)
     {
@@ -127,6 +288,7 @@ namespace lue_amd {
     {
     public:
         static constexpr int slot_count = 16;
+        static constexpr std::size_t initial_capacity = 4096;
 
         // Copy `bytes` from `src` to the device buffer `dst` in stream order.
         hipError_t copy_to_device(void* dst, void const* src, std::size_t bytes, hipStream_t stream)
@@ -137,22 +299,38 @@ namespace lue_amd {
                 return e;
             std::lock_guard<std::mutex> lock(mutex_);
             auto& ring = rings_[device];
+            if (!ring.ready)
+            {
+                // every slot at once on first use: pinned allocations and event creation stay out of the
+                // calls that follow (a pinned allocation can take milliseconds)
+                for (Slot& x : ring.slots)
+                {
+                    if (!x.done && (e = hipEventCreateWithFlags(&x.done, hipEventDisableTiming)) != hipSuccess)
+                        return e;
+                    if (!x.host)
+                    {
+                        if ((e = hipHostMalloc(&x.host, initial_capacity, hipHostMallocDefault)) != hipSuccess)
+                            return e;
+                        x.capacity = initial_capacity;
+                    }
+                }
+                ring.ready = true;
+            }
             Slot& s = ring.slots[ring.next];
             ring.next = (ring.next + 1) % slot_count;
-            if (s.done)
+            if (s.used)
             {
                 if ((e = hipEventSynchronize(s.done)) != hipSuccess)
                     return e;
             }
-            else if ((e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming)) != hipSuccess)
-                return e;
+            s.used = true;
             if (s.capacity < bytes)
             {
                 if (s.host)
                     (void)hipHostFree(s.host);
                 s.host = nullptr;
                 s.capacity = 0;
-                std::size_t const cap = std::max<std::size_t>(bytes, 4096);
+                std::size_t const cap = std::max<std::size_t>(bytes, initial_capacity);
                 if ((e = hipHostMalloc(&s.host, cap, hipHostMallocDefault)) != hipSuccess)
                     return e;
                 s.capacity = cap;
@@ -169,11 +347,13 @@ namespace lue_amd {
             void* host{};
             std::size_t capacity{};
             hipEvent_t done{};
+            bool used{false};  // `done` marks a copy that read this slot
         };
         struct Ring
         {
             Slot slots[slot_count];
             int next{0};
+            bool ready{false};
         };
         std::mutex mutex_;
         std::map<int, Ring> rings_;
@@ -395,18 +575,18 @@ namespace lue_amd {
     // Scratch of one operation on a network.
     struct Workspace
     {
-        DeviceArray<std::uint32_t> pending_bits;
-        DeviceArray<std::uint32_t> border_stamp;
-        DeviceArray<float> out_lo;
-        DeviceArray<std::uint32_t> tile_flag;
-        DeviceArray<std::uint32_t> worklist;
-        DeviceArray<std::uint32_t> work_count;
-        DeviceArray<unsigned long long> counters;
-        DeviceArray<unsigned char> io;
-        DeviceArray<unsigned char> staged;
-        DeviceArray<double> side;
-        DeviceArray<double> res;
-        DeviceArray<unsigned char> res_rem;
+        ScratchArray<std::uint32_t> pending_bits;
+        ScratchArray<std::uint32_t> border_stamp;
+        ScratchArray<float> out_lo;
+        ScratchArray<std::uint32_t> tile_flag;
+        ScratchArray<std::uint32_t> worklist;
+        ScratchArray<std::uint32_t> work_count;
+        ScratchArray<unsigned long long> counters;
+        ScratchArray<unsigned char> io;
+        ScratchArray<unsigned char> staged;
+        ScratchArray<double> side;
+        ScratchArray<double> res;
+        ScratchArray<unsigned char> res_rem;
     };
 
     template<typename T, int TW, int TH>

@@ -35,6 +35,13 @@ namespace lue_amd {
         return p;
     }
 
+    ScratchCache& scratch_cache()
+    {
+        // never destroyed: freeing device memory at process exit is not needed
+        static ScratchCache* cache = new ScratchCache;
+        return *cache;
+    }
+
     HostStaging& host_staging()
     {
         // never destroyed: pinned slots and events stay valid until the process ends
@@ -177,7 +184,7 @@ namespace lue_amd {
         {
             if (reinterpret_cast<std::uintptr_t>(ldd) % 4 != 0)
                 return fail(status_invalid_argument, "ldd must be 4-byte aligned");
-            DeviceArray<unsigned long long> sum;
+            ScratchArray<unsigned long long> sum;
             LUE_AMD_HIP(sum.allocate(1, stream));
             hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<std::uint32_t*>(sum.ptr),
                                0u, 2ull);
@@ -420,6 +427,12 @@ int lue_amd_set_network_cache(int entries)
     NetworkCache& cache = network_cache();
     cache.capacity = entries;
     cache.trim();
+    return status_success;
+}
+
+int lue_amd_trim_scratch(void)
+{
+    LUE_AMD_HIP(scratch_cache().trim());
     return status_success;
 }
 

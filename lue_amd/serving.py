@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import collections
 import dataclasses
+import gc
 import heapq
 import time
 
@@ -60,7 +61,7 @@ class AdmissionError(RuntimeError):
 class Server:
     """Resident catchments and the request loop. beta, dt and dx are the scalars of every timestep."""
 
-    def __init__(self, hbm_budget_bytes, slots=8, beta=0.6, dt=3600.0, dx=100.0, tile_shape=(32, 32),
+    def __init__(self, hbm_budget_bytes, slots=16, beta=0.6, dt=3600.0, dx=100.0, tile_shape=(32, 32),
                  device=None, keep_results=False):
         self.device = torch.device(device or "cuda")
         self.keep_results = keep_results
@@ -74,6 +75,7 @@ class Server:
         self.transient_bytes = 0
         self.completed: list[Request] = []
         self.cells_done = 0
+        self._warm = False
 
     # -- resident catchments --------------------------------------------------------------------------
     def add_catchment(self, name, ldd, alpha, initial_state):
@@ -85,6 +87,7 @@ class Server:
         network = fw.FlowNetwork(ldd, tile_shape=self.tile_shape)
         self.catchments[name] = Catchment(name, ldd, alpha, initial_state, network, estimate)
         self.resident_bytes += estimate
+        self._warm = False
         return self.catchments[name]
 
     def remove_catchment(self, name):
@@ -93,6 +96,29 @@ class Server:
             raise RuntimeError(f"catchment {name} has a request in flight")
         c.network.close()
         self.resident_bytes -= c.resident_bytes
+
+    def warm_up(self):
+        """Run one discarded timestep of the largest resident catchment on every slot at once (as many slots
+        as the HBM budget admits), so that the memory pools behind the slots' streams hold a request's
+        transient buffers before the first request (growing a pool inside the request loop stalls every slot
+        for milliseconds). States are unchanged."""
+        if not self.catchments:
+            return
+        c = max(self.catchments.values(), key=lambda x: x.ldd.shape[0] * x.ldd.shape[1])
+        need = c.ldd.shape[0] * c.ldd.shape[1] * (4 + 4 + 24)  # as a request: lateral inflow, output, scratch
+        slots = min(len(self.streams), max(0, self.budget - self.resident_bytes) // need)
+        if slots == 0:
+            return
+        torch.cuda.synchronize(self.device)
+        outs = []
+        for stream in self.streams[:slots]:
+            with torch.cuda.stream(stream):
+                lateral = self.lateral_inflow(0, c.ldd.shape, self.device)
+                outs.append((lateral, c.network.kinematic_wave(c.state, lateral, c.alpha, self.beta, self.dt,
+                                                               self.dx)))
+        torch.cuda.synchronize(self.device)
+        del outs
+        self._warm = True
 
     # -- one timestep ------------------------------------------------------------------------------------
     @staticmethod
@@ -134,6 +160,22 @@ class Server:
 
     def run(self, requests, clock=time.perf_counter, poll_sleep=50e-6):
         """Serve `requests` (arrival times relative to the start). Returns the completed requests."""
+        if not self._warm:
+            self.warm_up()
+        # the request loop allocates many short-lived objects; a full collection of the cyclic garbage
+        # collector over the process's long-lived objects stalls it for tens of milliseconds, so those are
+        # frozen and collection is off until the loop ends (request objects are freed by reference counting)
+        gc_was_enabled = gc.isenabled()
+        gc.freeze()
+        gc.disable()
+        try:
+            return self._loop(requests, clock, poll_sleep)
+        finally:
+            gc.unfreeze()
+            if gc_was_enabled:
+                gc.enable()
+
+    def _loop(self, requests, clock, poll_sleep):
         pending = list(requests)
         heapq.heapify(pending)
         waiting = collections.deque()  # arrived, not admitted yet (FIFO)

@@ -97,7 +97,7 @@ def large_domain(seed=20261206, side=32768, steps=24, partition=1024, dtype=torc
 
 
 def serving_run(seed=20261207, count=200, side_lo=256, side_hi=4096, rate=1000.0, duration=10.0,
-                budget_gb=200.0, slots=8, device=None):
+                budget_gb=200.0, slots=16, device=None):
     dev = torch.device(device or "cuda")
     shapes = syn.batch_shapes(seed, count, side_lo, side_hi)
     server = serving.Server(budget_gb * 1e9, slots=slots, device=dev)
@@ -113,6 +113,7 @@ def serving_run(seed=20261207, count=200, side_lo=256, side_hi=4096, rate=1000.0
             names.append(f"c{i}")
         except serving.AdmissionError:
             break
+    server.warm_up()
     _sync()
     setup_s = time.perf_counter() - t0
     requests = serving.poisson_requests(seed, rate, duration, names)

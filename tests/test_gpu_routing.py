@@ -350,3 +350,31 @@ def test_per_call_network_cache_follows_ldd_changes(fw, oracle_mod):
         assert np.array_equal(fw.accu(arr, mat).numpy(), changed)
     finally:
         _lib.set_network_cache(4)
+
+
+@pytest.mark.gpu
+def test_scratch_blocks_reused_across_streams_and_trimmed(fw):
+    """Per-call workspaces come from cached device blocks: runs on two streams that take turns on the same
+    blocks, and runs after the idle blocks are freed, give the same bits."""
+    import torch
+
+    from lue_amd import _lib, synthetic
+
+    ldd = synthetic.ldd_host(33, 520, 480)
+    q_cur = synthetic.uniform_host(33, synthetic.STREAM_CURRENT, 0.0, 5.0, (520, 480))
+    q_lat = synthetic.uniform_host(33, synthetic.STREAM_LATERAL, 0.0, 1e-4, (520, 480))
+    alpha = synthetic.uniform_host(33, synthetic.STREAM_ALPHA, 1.5, 2.5, (520, 480))
+    net = fw.FlowNetwork(pa(fw, ldd))
+    args = [pa(fw, a) for a in (q_cur, q_lat, alpha)]
+    want = net.kinematic_wave(*args, 0.6, 3600.0, 100.0).numpy()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = []
+    torch.cuda.synchronize()
+    for i in range(6):
+        with torch.cuda.stream(streams[i % 2]):
+            outs.append(net.kinematic_wave(*args, 0.6, 3600.0, 100.0))
+    torch.cuda.synchronize()
+    for o in outs:
+        assert np.array_equal(o.numpy(), want)
+    _lib.trim_scratch()
+    assert np.array_equal(net.kinematic_wave(*args, 0.6, 3600.0, 100.0).numpy(), want)
