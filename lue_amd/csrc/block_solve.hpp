@@ -220,6 +220,14 @@ namespace lue_amd {
         }
     }
 
+    // Outcomes of kinematic_wave_block.
+    enum KwBlockResult : int
+    {
+        kw_block_done = 0,    // q holds the block's outflows
+        kw_block_serial = 1,  // a lane needs LUE's per-cell iteration or the block Newton failed: per-cell path
+        kw_block_lue_far = 2  // converged, but LUE's own iteration would not end at the root in some lane
+    };
+
     // Kinematic wave on one block of chain positions, all lanes at once. Newton's method on the coupled
     // system of the block
     //     F_k(Q) = c1_k (Q_k - Q_{k-1} - extra_k) + alpha_k Q_k^beta_k - b_k = 0,
@@ -232,18 +240,20 @@ namespace lue_amd {
     // |dQ_k| <= |Q_k| 2^(1 - digits), LUE's criterion (digits 31 for double, 14 for float).
     // T = float: the iterations use the hardware FP32 power (log2, exp2) and approximate reciprocals; one
     // FP64 Newton step follows (the FP64 residual correction).
-    // Returns false, with q untouched, when a lane needs LUE's per-cell iteration: no-data, a zero current
-    // discharge (dry cells and cells whose iteration starts from numeric_limits::min()), a non-positive
-    // iterate, or no convergence within 16 steps.
+    // Returns kw_block_serial, with q untouched, when a lane needs LUE's per-cell iteration: no-data, a zero
+    // current discharge (dry cells and cells whose iteration starts from numeric_limits::min()), a non-positive
+    // iterate, or no convergence within 16 steps; kw_block_lue_far when LUE's own iteration would stop short
+    // of the root in some lane (lue_guess_converges).
     template<typename T>
-    __device__ inline bool kinematic_wave_block(ChainLane<op_kinematic_wave, T> const& d, double carry, double& q)
+    __device__ inline int kinematic_wave_block(ChainLane<op_kinematic_wave, T> const& d, double carry, double dt,
+                                                double& q)
     {
         int const lane = threadIdx.x & 63;
         bool const valid = d.valid;
         bool const start = valid && d.start;
         double const q_cur = static_cast<double>(d.in.q_cur);
-        if (__ballot(valid && !(q_cur > 0.0 && !isnan(d.extra))))
-            return false;
+        if (__builtin_expect(__ballot(valid && !(q_cur > 0.0 && !isnan(d.extra))) != 0, 0))
+            return kw_block_serial;
         double const c1 = kw_c1(d.in);
         double const alpha = static_cast<double>(d.in.alpha);
         double const beta = static_cast<double>(d.in.beta);
@@ -272,8 +282,8 @@ namespace lue_amd {
             x = valid ? (flag ? static_cast<double>(add) : fma(static_cast<double>(mul), carry, static_cast<double>(add)))
                       : 1.0;
         }
-        if (__ballot(valid && !(x > 0.0)))
-            return false;
+        if (__builtin_expect(__ballot(valid && !(x > 0.0)) != 0, 0))
+            return kw_block_serial;
 
         // Newton steps: F in FP64; the power (T = float: FP32 hardware log2/exp2), the Jacobian and the
         // step scan in FP32 for T = float, FP64 for T = double
@@ -313,13 +323,13 @@ namespace lue_amd {
                 step = static_cast<double>(add);
             }
             double const xn = x + step;
-            if (__ballot(valid && !(xn > 0.0)))
-                return false;
+            if (__builtin_expect(__ballot(valid && !(xn > 0.0)) != 0, 0))
+                return kw_block_serial;
             converged = __ballot(valid && !(fabs(step) <= fabs(xn) * tol)) == 0;
             x = xn;
         }
         if (!converged)
-            return false;
+            return kw_block_serial;
         if constexpr (std::is_same_v<T, float>)
         {
             // FP64 residual correction: one Newton step with the FP64 power and Jacobian
@@ -339,12 +349,31 @@ namespace lue_amd {
             int flag = (start || !valid) ? 1 : 0;
             affine_scan(add, mul, flag);
             double const xn = x + add;
-            if (__ballot(valid && !(xn > 0.0)))
-                return false;
+            if (__builtin_expect(__ballot(valid && !(xn > 0.0)) != 0, 0))
+                return kw_block_serial;
             x = xn;
         }
+        // LUE's own iteration ends at this root only from a guess near it (lue_guess_converges); otherwise
+        // the block takes the per-cell path, which follows LUE's iteration there
+        {
+            double prev = wave_shr1(x);
+            if (lane == 0)
+                prev = carry;
+            if (start)
+                prev = 0.0;
+            float dtq;  // dt q_lat; for T = float recovered from b with the hardware power (enough for a check)
+            if constexpr (std::is_same_v<T, float>)
+                dtq = static_cast<float>(b - alpha * static_cast<double>(fast_exp2(beta32 * fast_log2(d.in.q_cur))));
+            else
+                dtq = static_cast<float>(dt * d.in.q_lat);
+            bool const far = valid && !lue_guess_converges(static_cast<float>(prev + d.extra), static_cast<float>(q_cur),
+                                                           dtq, static_cast<float>(alpha), beta32, c1f,
+                                                           static_cast<float>(x));
+            if (__builtin_expect(__ballot(far) != 0, 0))
+                return kw_block_lue_far;
+        }
         q = valid ? x : 0.0;
-        return true;
+        return kw_block_done;
     }
 
     // State carried from one block of chain positions to the next by the wavefront that runs them.
@@ -355,6 +384,146 @@ namespace lue_amd {
         float warm_pow{0.0f};
         bool have_warm{false};
     };
+
+    // FP32 kinematic wave of one block, cell after cell, each with the per-cell FP32 path
+    // (new_discharge_f32_path, which follows LUE's FP64 iteration where that stops short of the root).
+    // Returns this lane's outflow.
+    __device__ __attribute__((always_inline)) inline double kinematic_wave_cells_f32(bool start, double extra,
+                                                      CellInput<op_kinematic_wave, float> const& in, int nb, float dt,
+                                                      double carry)
+    {
+        int const lane = threadIdx.x & 63;
+        float const q_lat_mine = kw_q_lat(in, dt);
+        double const b_mine = kw_b(in);
+        double value = 0.0;
+        for (int j = 0; j < nb; ++j)
+        {
+            int const first = read_lane(static_cast<int>(start), j);
+            double const up = (first ? 0.0 : carry) + read_lane(extra, j);
+            float const q_cur = read_lane(in.q_cur, j);
+            double v;
+            if (isnan(up) || isnan(q_cur))
+                v = __longlong_as_double(0x7ff8000000000000ll);
+            else
+                v = new_discharge_f32_path_call(up, q_cur, read_lane(q_lat_mine, j), read_lane(in.alpha, j),
+                                                read_lane(in.beta, j), dt, read_lane(b_mine, j),
+                                                read_lane(kw_c1(in), j));
+            if (lane == j)
+                value = v;
+            carry = v;
+        }
+        return value;
+    }
+
+    struct KwSerialOut
+    {
+        double value;  // this lane's outflow
+        double carry;  // outflow of the block's last position
+        float warm;
+        float warm_pow;
+        bool have_warm;
+    };
+
+    // FP32 kinematic wave of one block, cell after cell, for the blocks the block Newton does not take (no-data
+    // or dry cells, non-convergence, cells where LUE's own iteration stops short of the root): serial FP32
+    // Newton (warm start from the previous cell), then the FP64 correction of every cell in parallel,
+    // propagated along the chains to first order.
+    __device__ __attribute__((always_inline)) inline KwSerialOut kinematic_wave_serial_f32(
+        bool valid, bool start, double extra, float in_q_cur, float in_alpha, float in_beta, float in_c1,
+        float in_b_hi, float in_b_lo, int nb, bool uniform_beta, float dt, double carry_in, float warm_in,
+        float warm_pow_in, bool have_warm_in)
+    {
+        int const lane = threadIdx.x & 63;
+        double const nan64 = __longlong_as_double(0x7ff8000000000000ll);
+        CellInput<op_kinematic_wave, float> const in{in_q_cur, in_alpha, in_beta, in_c1, in_b_hi, in_b_lo};
+        BlockState st{carry_in, warm_in, warm_pow_in, have_warm_in};
+        double value = 0.0;
+        double q_up_mine = 0.0;
+        float q32_mine = 0.0f;
+        float const q_lat_mine = kw_q_lat(in, dt);
+        double const b_mine = kw_b(in);
+        double const c1_mine = kw_c1(in);
+        bool exact_mine = false;  // this lane's value is LUE's own FP64 iteration (no correction)
+        double exact_value = 0.0;
+        for (int j = 0; j < nb; ++j)
+        {
+            int const first = read_lane(static_cast<int>(start), j);
+            if (first)
+                st.have_warm = false;
+            double const q_up = (first ? 0.0 : st.carry) + read_lane(extra, j);
+            float const q_cur = read_lane(in.q_cur, j);
+            float q;
+            if (isnan(q_up) || isnan(q_cur))
+            {
+                q = __int_as_float(0x7fc00000);
+                st.have_warm = false;
+            }
+            else
+            {
+                float const q_lat = read_lane(q_lat_mine, j);
+                float const alpha = read_lane(in.alpha, j);
+                float const beta = read_lane(in.beta, j);
+                double const b = read_lane(b_mine, j);
+                double const c1 = read_lane(c1_mine, j);
+                if (q_up + q_cur == 0.0 && beta < 1.0f && q_lat > 0.0f)
+                {
+                    // LUE starts from numeric_limits::min() here: its own FP64 iteration, kept in FP64
+                    double const v = lue_iteration_f64(q_up, q_cur, q_lat, alpha, beta, dt, c1, b);
+                    st.have_warm = false;
+                    if (lane == j)
+                    {
+                        exact_mine = true;
+                        exact_value = v;
+                        q_up_mine = q_up;
+                    }
+                    st.carry = v;
+                    continue;
+                }
+                st.have_warm = st.have_warm && uniform_beta;
+                q = kinematic_wave_root_f32(static_cast<float>(c1), alpha, beta, dt, q_up, q_cur, q_lat, b, c1,
+                                            st.have_warm, st.warm, st.warm_pow);
+            }
+            if (lane == j)
+            {
+                q_up_mine = q_up;
+                q32_mine = q;
+            }
+            st.carry = static_cast<double>(q);
+        }
+        double eps = 0.0;
+        double sens = 0.0;
+        double const q = exact_mine ? exact_value : static_cast<double>(q32_mine);
+        if (valid && q > 0.0 && !isnan(q) && !exact_mine)
+        {
+            kinematic_wave_correction(q, q_up_mine, static_cast<double>(in.alpha), static_cast<double>(in.beta),
+                                      b_mine, c1_mine, eps, sens);
+        }
+        // affine scan: delta_k = eps_k + sens_k * delta_{k-1}; a chain start does not depend on the cell
+        // before it, and the first cell of the block starts from a corrected value
+        double mul = sens;
+        double add = eps;
+        int flag = (!valid || start) ? 1 : 0;
+        affine_scan(add, mul, flag);
+        value = isnan(q) ? q : (valid ? (q + add > 0.0 ? q + add : q) : 0.0);
+        // cells whose LUE iteration would not end at this root (lue_guess_converges): the block again, cell
+        // after cell, each with the per-cell FP32 path, which follows LUE's FP64 iteration there
+        double prev = wave_shr1(value);
+        if (lane == 0)
+            prev = carry_in;
+        if (start)
+            prev = 0.0;
+        float const dtq = static_cast<float>(b_mine - static_cast<double>(in.alpha) *
+                                                         static_cast<double>(fast_exp2(in.beta * fast_log2(in.q_cur))));
+        bool const suspect = valid && !exact_mine && value > 0.0 && !isnan(value) &&
+                             !lue_guess_converges(static_cast<float>(prev + extra), in.q_cur, dtq, in.alpha, in.beta,
+                                                  static_cast<float>(c1_mine), static_cast<float>(value));
+        if (__builtin_expect(__ballot(suspect) != 0, 0))
+        {
+            value = kinematic_wave_cells_f32(start, extra, in, nb, dt, carry_in);
+            st.have_warm = false;
+        }
+        return KwSerialOut{value, read_lane(value, nb - 1), st.warm, st.warm_pow, st.have_warm};
+    }
 
     // Run the cell function along one block of up to 64 chain positions (lane j: position j; `nb` valid
     // lanes). Chains start where d.start is set; the first lane continues the chain of st.carry otherwise.
@@ -396,83 +565,48 @@ namespace lue_amd {
                 st.carry = out;
             }
         }
-        else if (kinematic_wave_block<T>(d, st.carry, value))
-        {
-            st.carry = read_lane(value, nb - 1);
-            st.have_warm = false;
-        }
-        else if constexpr (std::is_same_v<T, double>)
-        {
-            // plain FP64 kinematic wave: LUE's Newton per cell along the block
-            for (int j = 0; j < nb; ++j)
-            {
-                int const first = read_lane(static_cast<int>(d.start), j);
-                double const up = (first ? 0.0 : st.carry) + read_lane(d.extra, j);
-                CellInput<OP, T> cj{read_lane(d.in.q_cur, j), read_lane(d.in.q_lat, j), read_lane(d.in.alpha, j),
-                                    read_lane(d.in.beta, j), read_lane(d.in.b, j), read_lane(d.in.c1, j)};
-                double v;
-                T r{};
-                solve_cell<OP, T>(a, cj, up, isnan(up), v, r);
-                if (lane == j)
-                    value = v;
-                st.carry = v;
-            }
-        }
         else
         {
-            // FP32 kinematic wave: serial FP32 Newton (warm start from the previous cell), then the FP64
-            // correction of every cell in parallel, propagated along the chains to first order
-            double q_up_mine = 0.0;
-            float q32_mine = 0.0f;
-            float const q_lat_mine = kw_q_lat(d.in, a.dt);
-            double const b_mine = kw_b(d.in);
-            for (int j = 0; j < nb; ++j)
+            int const outcome = kinematic_wave_block<T>(d, st.carry, static_cast<double>(a.dt), value);
+            if (__builtin_expect(outcome == kw_block_done, 1))
             {
-                int const first = read_lane(static_cast<int>(d.start), j);
-                if (first)
-                    st.have_warm = false;
-                double const q_up = (first ? 0.0 : st.carry) + read_lane(d.extra, j);
-                float const q_cur = read_lane(d.in.q_cur, j);
-                float q;
-                if (isnan(q_up) || isnan(q_cur))
-                {
-                    q = __int_as_float(0x7fc00000);
-                    st.have_warm = false;
-                }
-                else
-                {
-                    float const q_lat = read_lane(q_lat_mine, j);
-                    float const alpha = read_lane(d.in.alpha, j);
-                    float const beta = read_lane(d.in.beta, j);
-                    double const b = read_lane(b_mine, j);
-                    double const c1 = read_lane(kw_c1(d.in), j);
-                    st.have_warm = st.have_warm && uniform_beta;
-                    q = kinematic_wave_root_f32(static_cast<float>(c1), alpha, beta, a.dt, q_up, q_cur, q_lat, b, c1,
-                                                st.have_warm, st.warm, st.warm_pow);
-                }
-                if (lane == j)
-                {
-                    q_up_mine = q_up;
-                    q32_mine = q;
-                }
-                st.carry = static_cast<double>(q);
+                st.carry = read_lane(value, nb - 1);
+                st.have_warm = false;
             }
-            double eps = 0.0;
-            double sens = 0.0;
-            double const q = static_cast<double>(q32_mine);
-            if (d.valid && q > 0.0 && !isnan(q))
+            else if constexpr (std::is_same_v<T, double>)
             {
-                kinematic_wave_correction(q, q_up_mine, static_cast<double>(d.in.alpha),
-                                          static_cast<double>(d.in.beta), b_mine, kw_c1(d.in), eps, sens);
+                // plain FP64 kinematic wave: LUE's Newton per cell along the block
+                for (int j = 0; j < nb; ++j)
+                {
+                    int const first = read_lane(static_cast<int>(d.start), j);
+                    double const up = (first ? 0.0 : st.carry) + read_lane(d.extra, j);
+                    CellInput<OP, T> cj{read_lane(d.in.q_cur, j), read_lane(d.in.q_lat, j), read_lane(d.in.alpha, j),
+                                        read_lane(d.in.beta, j), read_lane(d.in.b, j), read_lane(d.in.c1, j)};
+                    double v;
+                    T r{};
+                    solve_cell<OP, T>(a, cj, up, isnan(up), v, r);
+                    if (lane == j)
+                        value = v;
+                    st.carry = v;
+                }
             }
-            // affine scan: delta_k = eps_k + sens_k * delta_{k-1}; a chain start does not depend on
-            // the cell before it, and the first cell of the block starts from a corrected value
-            double mul = sens;
-            double add = eps;
-            int flag = (!d.valid || d.start) ? 1 : 0;
-            affine_scan(add, mul, flag);
-            value = isnan(q) ? q : (d.valid ? (q + add > 0.0 ? q + add : q) : 0.0);
-            st.carry = read_lane(value, nb - 1);
+            else if (outcome == kw_block_lue_far)
+            {
+                value = kinematic_wave_cells_f32(d.start, d.extra, d.in, nb, a.dt, st.carry);
+                st.carry = read_lane(value, nb - 1);
+                st.have_warm = false;
+            }
+            else
+            {
+                KwSerialOut const o = kinematic_wave_serial_f32(
+                    d.valid, d.start, d.extra, d.in.q_cur, d.in.alpha, d.in.beta, d.in.c1, d.in.b_hi, d.in.b_lo, nb,
+                    uniform_beta, a.dt, st.carry, st.warm, st.warm_pow, st.have_warm);
+                value = o.value;
+                st.carry = o.carry;
+                st.warm = o.warm;
+                st.warm_pow = o.warm_pow;
+                st.have_warm = o.have_warm;
+            }
         }
     }
 

@@ -179,10 +179,15 @@ namespace lue_amd {
         return static_cast<double>(c.c1);
     }
 
-    // Q_cur^beta of the FP32 path: the hardware FP32 power (pow for Q_cur = 0).
-    __device__ inline float kw_pow32(float q_cur, float beta)
+    // Q_cur^beta of the FP32 path's b = alpha Q_cur^beta + dt q, in FP64 (pow_positive, relative error
+    // about 1e-15): b enters every root, and where LUE's own iteration stops short of the root (see
+    // lue_guess_converges) its result amplifies input errors, so b is formed as accurately as on the FP64
+    // path. pow for Q_cur = 0 and non-finite arguments.
+    __device__ inline double kw_b_pow(float q_cur, float beta)
     {
-        return q_cur > 0.0f ? __builtin_amdgcn_exp2f(beta * __builtin_amdgcn_logf(q_cur)) : powf(q_cur, beta);
+        if (!(q_cur > 0.0f) || !isfinite(q_cur) || !isfinite(beta))
+            return static_cast<double>(powf(q_cur, beta));
+        return pow_positive(static_cast<double>(q_cur), static_cast<double>(beta));
     }
 
     // Lateral inflow of a cell: stored (FP64 path) or recovered from b = alpha Q_cur^beta + dt q (FP32 path,
@@ -194,7 +199,7 @@ namespace lue_amd {
     {
         if constexpr (std::is_same_v<T, float>)
             return dt != 0.0f ? static_cast<float>((kw_b(c) - static_cast<double>(c.alpha) *
-                                                                   static_cast<double>(kw_pow32(c.q_cur, c.beta))) /
+                                                                   kw_b_pow(c.q_cur, c.beta)) /
                                                    static_cast<double>(dt))
                               : 0.0f;
         else
@@ -269,9 +274,8 @@ namespace lue_amd {
             double const dt = static_cast<double>(a.dt);
             if constexpr (std::is_same_v<T, float>)
             {
-                // b with the hardware FP32 power (relative error about 1e-7, far inside the path's tolerance),
-                // summed in FP64 and kept as an FP32 pair
-                double const b = ok ? static_cast<double>(alpha) * static_cast<double>(kw_pow32(q_cur, beta)) +
+                // b in FP64 (kw_b_pow), kept as an FP32 pair
+                double const b = ok ? static_cast<double>(alpha) * kw_b_pow(q_cur, beta) +
                                           dt * static_cast<double>(q_lat)
                                     : 0.0;
                 c.q_cur = ok ? q_cur : quiet_nan<T>();

@@ -61,8 +61,28 @@ namespace lue_amd {
     //   atanh series to s^15 leaves 2 |s|^17 / 17 < 1.1e-14;
     //   e^z = 2^k e^r, k = rint(z / ln 2), |r| <= ln 2 / 2 (ln 2 in two parts): Taylor to r^12 leaves
     //   |r|^13 / 13! < 2e-16.
+    // The coefficients live in constant memory, read with scalar loads where they are used: as literals the
+    // compiler keeps all of them in vector registers across the kernels' loops (and spills them there).
+    // [0, 7]: 1/15, 1/13, ..., 1/3, 1 (atanh series); [8]: ln2_hi, [9]: ln2_lo (ln 2 = ln2_hi + ln2_lo,
+    // double-double; the fused products below need no exact k ln2_hi), [10]: 1/ln 2; [11, 23]: 1/12!, ...,
+    // 1/2!, 1, 1 (Taylor series of e^r).
+    inline __constant__ double pow_positive_coefficients[24] = {
+        1.0 / 15.0, 1.0 / 13.0, 1.0 / 11.0, 1.0 / 9.0, 1.0 / 7.0, 1.0 / 5.0, 1.0 / 3.0, 1.0,
+        0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56, 1.44269504088896340736,
+        1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0, 1.0 / 362880.0, 1.0 / 40320.0, 1.0 / 5040.0,
+        1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0, 0.5, 1.0, 1.0};
+
     __host__ __device__ inline double pow_positive(double x, double y)
     {
+#if defined(__HIP_DEVICE_COMPILE__)
+        double const* const k = pow_positive_coefficients;
+#else
+        static constexpr double k[24] = {
+            1.0 / 15.0, 1.0 / 13.0, 1.0 / 11.0, 1.0 / 9.0, 1.0 / 7.0, 1.0 / 5.0, 1.0 / 3.0, 1.0,
+            0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56, 1.44269504088896340736,
+            1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0, 1.0 / 362880.0, 1.0 / 40320.0, 1.0 / 5040.0,
+            1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0, 0.5, 1.0, 1.0};
+#endif
         int e;
         double m = frexp(x, &e);  // [0.5, 1)
         if (m < 0.70710678118654752440)
@@ -72,34 +92,17 @@ namespace lue_amd {
         }
         double const s = (m - 1.0) / (m + 1.0);
         double const s2 = s * s;
-        double p = 1.0 / 15.0;
-        p = fma(p, s2, 1.0 / 13.0);
-        p = fma(p, s2, 1.0 / 11.0);
-        p = fma(p, s2, 1.0 / 9.0);
-        p = fma(p, s2, 1.0 / 7.0);
-        p = fma(p, s2, 1.0 / 5.0);
-        p = fma(p, s2, 1.0 / 3.0);
-        p = fma(p, s2, 1.0);
-        constexpr double ln2_hi = 0x1.62e42fefa39efp-1;  // ln 2 = ln2_hi + ln2_lo (double-double); the fused
-        constexpr double ln2_lo = 0x1.abc9e3b39803fp-56; // products below need no exact k ln2_hi
-        double const ln_x = fma(static_cast<double>(e), ln2_hi, fma(static_cast<double>(e), ln2_lo, 2.0 * s * p));
+        double p = k[0];
+        for (int i = 1; i < 8; ++i)
+            p = fma(p, s2, k[i]);
+        double const ln_x = fma(static_cast<double>(e), k[8], fma(static_cast<double>(e), k[9], 2.0 * s * p));
         double const z = y * ln_x;
-        double const k = rint(z * 1.44269504088896340736);
-        double const r = fma(-k, ln2_lo, fma(-k, ln2_hi, z));
-        double q = 1.0 / 479001600.0;  // 1 / 12!
-        q = fma(q, r, 1.0 / 39916800.0);
-        q = fma(q, r, 1.0 / 3628800.0);
-        q = fma(q, r, 1.0 / 362880.0);
-        q = fma(q, r, 1.0 / 40320.0);
-        q = fma(q, r, 1.0 / 5040.0);
-        q = fma(q, r, 1.0 / 720.0);
-        q = fma(q, r, 1.0 / 120.0);
-        q = fma(q, r, 1.0 / 24.0);
-        q = fma(q, r, 1.0 / 6.0);
-        q = fma(q, r, 0.5);
-        q = fma(q, r, 1.0);
-        q = fma(q, r, 1.0);
-        return ldexp(q, static_cast<int>(k));
+        double const n = rint(z * k[10]);
+        double const r = fma(-n, k[9], fma(-n, k[8], z));
+        double q = k[11];
+        for (int i = 12; i < 24; ++i)
+            q = fma(q, r, k[i]);
+        return ldexp(q, static_cast<int>(n));
     }
 
     // Coefficients of one cell's equation.
@@ -269,6 +272,37 @@ namespace lue_amd {
         return x;
     }
 
+    // LUE's FP64 iteration of one cell, out of line: the rarely taken paths that need it stay out of the
+    // register budget of the kernels' main paths.
+    __device__ __noinline__ inline double lue_iteration_f64(double q_up, double q_cur, double q_lat, double alpha,
+                                                            double beta, double dt, double c1, double b)
+    {
+        KinematicWaveCell<double> cell{q_up, q_cur, q_lat, alpha, beta, dt, c1, b};
+        bool converged;
+        return solve_bracketed<double>(cell, converged);
+    }
+
+    // Whether LUE's iteration surely ends at the root `root` (alpha, beta > 0): its guess, linearised at
+    // (Q_cur + Q_up) / 2 (kinematic_wave.hpp:61-91), lies within theta root, theta = 1/2 / max(1, (beta - 1)^2).
+    // Newton's error on f obeys e' = f''(xi) / (2 f'(x)) e^2 with xi between the iterate x and the root, and
+    // with x = (1 - r) root (or (1 + r) root) that gives r' <= |beta - 1| / (2 (1 - r)) (1 + r)^|beta - 2| r^2;
+    // from r <= theta the relative step falls below LUE's 2^-30 within 7 of its 10 steps. Elsewhere (a guess
+    // far from the root, typically a tiny Q_cur under a large inflow with beta well below 1) LUE returns
+    // whatever its 10 steps reach, which only its own iteration reproduces. dtq = dt q_lat. FP32 arithmetic
+    // with the hardware power: a rounding here can only send a cell to LUE's own iteration or spare it that,
+    // and both give LUE's result.
+    __device__ inline bool lue_guess_converges(float q_up, float q_cur, float dtq, float alpha, float beta, float c1,
+                                               float root)
+    {
+        float const mean = 0.5f * (q_cur + q_up);
+        if (!(mean >= FLT_MIN && mean <= FLT_MAX) || !(root > 0.0f) || !(alpha > 0.0f) || !(beta > 0.0f))
+            return false;
+        float const a_b_pq = alpha * beta * fast_exp2((beta - 1.0f) * fast_log2(mean));
+        float const x0 = fmaxf((c1 * q_up + a_b_pq * q_cur + dtq) * fast_rcp(c1 + a_b_pq), FLT_MIN);
+        float const d = fmaxf(1.0f, (beta - 1.0f) * (beta - 1.0f));
+        return fabsf(x0 - root) * d <= 0.5f * root;
+    }
+
     // FP32 path of one cell. q_up is the compensated upstream sum (FP64), b = alpha q_cur^beta + dt q_lat
     // and c1 = dt / dx are evaluated in FP64 outside the dependency chain. The FP32 root gets one FP64
     // Newton correction, the only FP64 arithmetic on the chain.
@@ -283,11 +317,7 @@ namespace lue_amd {
             return 0.0;
         }
         if (q_up + q_cur == 0.0 && beta < 1.0f)
-        {
-            KinematicWaveCell<double> cell{q_up, q_cur, q_lat, alpha, beta, dt, c1, b};
-            bool converged;
-            return solve_bracketed<double>(cell, converged);
-        }
+            return lue_iteration_f64(q_up, q_cur, q_lat, alpha, beta, dt, c1, b);
         double const known = c1 * q_up + b;
         bool converged;
         float const x32 = solve_fast_f32(static_cast<float>(c1), alpha, beta, dt, static_cast<float>(q_up), q_cur,
@@ -297,8 +327,19 @@ namespace lue_amd {
         double const ax = static_cast<double>(alpha) * xb;
         double const f0 = (c1 * x) + ax - known;
         double const f1 = c1 + static_cast<double>(beta) * ax / x;
-        double const corrected = x - f0 / f1;
-        return corrected > 0.0 ? corrected : x;
+        double corrected = x - f0 / f1;
+        corrected = corrected > 0.0 ? corrected : x;
+        if (!lue_guess_converges(static_cast<float>(q_up), q_cur, dt * q_lat, alpha, beta, static_cast<float>(c1),
+                                 static_cast<float>(corrected)))
+            return lue_iteration_f64(q_up, q_cur, q_lat, alpha, beta, dt, c1, b);  // its result need not be the root
+        return corrected;
+    }
+
+    // new_discharge_f32_path out of line, for the rarely taken per-cell redo of a block.
+    __device__ __noinline__ inline double new_discharge_f32_path_call(double q_up, float q_cur, float q_lat, float alpha,
+                                                                      float beta, float dt, double b, double c1)
+    {
+        return new_discharge_f32_path(q_up, q_cur, q_lat, alpha, beta, dt, b, c1);
     }
 
 
@@ -361,7 +402,7 @@ namespace lue_amd {
     __device__ inline void kinematic_wave_correction(double x, double q_up, double alpha, double beta, double b,
                                                      double c1, double& eps, double& sens)
     {
-        double const xb = dpow(x, beta);
+        double const xb = pow_positive(x, beta);  // x > 0 here
         double const ax = alpha * xb;
         double const f0 = c1 * x + ax - (c1 * q_up + b);
         double const f1 = c1 + beta * ax / x;

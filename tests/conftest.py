@@ -5,6 +5,9 @@ import sys
 import numpy as np
 import pytest
 
+# beta as the FP32 path receives it (0.6f): FP64 references for FP32 results take this value
+BETA32 = float(np.float32(0.6))
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from conftest import assert_equal_nan
+from conftest import BETA32, assert_equal_nan
 
 torch = pytest.importorskip("torch")
 
@@ -71,7 +71,7 @@ def test_network_matches_oracle(fw, oracle_mod, tile_shape, schedule):
     else:
         q = net.kinematic_wave(*kw_args)
         want_q = oracle_mod.kinematic_wave(c["ldd"], *[c[k].astype(np.float64) for k in ("q_cur", "q_lat", "alpha")],
-                                           0.6, 3600.0, 100.0, None, np.float64)
+                                           BETA32, 3600.0, 100.0, None, np.float64)
         torch.cuda.synchronize()
         assert rel_err(q.numpy(), want_q).max() <= 1e-6
     out_t, rem_t = net.accu_threshold(pa(fw, c["material"]), pa(fw, c["threshold"]))

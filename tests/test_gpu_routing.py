@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 import reference_models as rm
-from conftest import assert_close_boost, assert_equal_nan, golden_arg, load_golden, raster
+from conftest import BETA32, assert_close_boost, assert_equal_nan, golden_arg, load_golden, raster
 
 torch = pytest.importorskip("torch")
 
@@ -169,11 +169,35 @@ def test_kinematic_wave_random(fw, oracle_mod, seed, shape, pshape):
     assert rel_err(got64, want).max() <= 1e-12
     # FP32 inputs: compare with the FP64 oracle on the same (FP32-representable) inputs
     f32 = [a.astype(np.float32) for a in (q_cur, q_lat, alpha)]
-    want32in = oracle_mod.kinematic_wave(ldd, *[a.astype(np.float64) for a in f32], 0.6, 3600.0, 100.0, pshape,
+    want32in = oracle_mod.kinematic_wave(ldd, *[a.astype(np.float64) for a in f32], BETA32, 3600.0, 100.0, pshape,
                                          np.float64)
     got32 = fw.kinematic_wave(pa(fw, ldd, pshape), *[pa(fw, a, pshape) for a in f32], 0.6, 3600.0,
                               100.0).numpy()
     assert rel_err(got32, want32in).max() <= 1e-6
+
+
+@pytest.mark.parametrize("beta,dt,dx", [(0.6, 3600.0, 100.0), (0.3, 60.0, 10.0), (1.4, 600.0, 250.0)])
+def test_kinematic_wave_f32_wide_range(fw, oracle_mod, beta, dt, dx):
+    """Discharges from 1e-8 to 1e3 (log-uniform): a power through the FP32 log2 of the whole argument loses
+    |beta log2 Q| 2^-24 there, and LUE's 10 Newton steps stop short of the root in some cells (a tiny
+    current discharge under a large inflow); both paths still follow LUE (FP32 within 1e-6)."""
+    rng = np.random.default_rng(int(beta * 10) + int(dt))
+    shape = (90, 110)
+    ldd = rm.random_ldd(rng, *shape, nodata_fraction=0.02, flat_fraction=0.3)
+    q_cur = np.exp(rng.uniform(np.log(1e-8), np.log(1e3), shape)).astype(np.float32)
+    q_lat = np.exp(rng.uniform(np.log(1e-9), np.log(1e-3), shape)).astype(np.float32)
+    alpha = rng.uniform(0.5, 5.0, shape).astype(np.float32)
+    beta = float(np.float32(beta))  # as the FP32 path receives it
+    want = oracle_mod.kinematic_wave(ldd, *[a.astype(np.float64) for a in (q_cur, q_lat, alpha)], beta, dt, dx,
+                                     (45, 55), np.float64)
+    got = fw.kinematic_wave(pa(fw, ldd, (45, 55)), *[pa(fw, a, (45, 55)) for a in (q_cur, q_lat, alpha)], beta, dt,
+                            dx).numpy()
+    assert rel_err(got, want).max() <= 1e-6
+    # FP64 path on the same values: LUE's iteration, including where its 10 steps stop short of the root
+    got64 = fw.kinematic_wave(pa(fw, ldd, (45, 55)),
+                              *[pa(fw, a.astype(np.float64), (45, 55)) for a in (q_cur, q_lat, alpha)], beta, dt,
+                              dx).numpy()
+    assert rel_err(got64, want).max() <= 1e-10
 
 
 def test_kinematic_wave_linear_closed_form(fw):
@@ -245,7 +269,7 @@ def test_config1_kinematic_wave_2048(fw, oracle_mod):
                               3600.0, 100.0).numpy()
     assert rel_err(got64, want).max() <= 1e-12
     f32 = [a.astype(np.float32) for a in (q_cur, q_lat, alpha)]
-    want32in = oracle_mod.kinematic_wave(ldd, *[a.astype(np.float64) for a in f32], 0.6, 3600.0, 100.0, ps,
+    want32in = oracle_mod.kinematic_wave(ldd, *[a.astype(np.float64) for a in f32], BETA32, 3600.0, 100.0, ps,
                                          np.float64)
     got32 = fw.kinematic_wave(pa(fw, ldd, ps), *[pa(fw, a, ps) for a in f32], 0.6, 3600.0, 100.0).numpy()
     err = rel_err(got32, want32in)
@@ -304,8 +328,10 @@ def test_tail_policy_split_and_kinematic_wave(fw, oracle_mod, policy_lib, policy
     alpha = synthetic.uniform_host(seed, synthetic.STREAM_ALPHA, 1.5, 2.5, shape, np.float32)
     want = oracle_mod.kinematic_wave(ldd, *[x.astype(np.float64) for x in (q_cur, q_lat, alpha)], 0.6, 3600.0,
                                      100.0, None, np.float64)
+    want32 = oracle_mod.kinematic_wave(ldd, *[x.astype(np.float64) for x in (q_cur, q_lat, alpha)], BETA32, 3600.0,
+                                       100.0, None, np.float64)
     got32 = fw.kinematic_wave(pa(fw, ldd), pa(fw, q_cur), pa(fw, q_lat), pa(fw, alpha), 0.6, 3600.0, 100.0).numpy()
-    assert rel_err(got32, want).max() <= 1e-6
+    assert rel_err(got32, want32).max() <= 1e-6
     got64 = fw.kinematic_wave(pa(fw, ldd), *[pa(fw, x.astype(np.float64)) for x in (q_cur, q_lat, alpha)], 0.6,
                               3600.0, 100.0).numpy()
     assert rel_err(got64, want).max() <= 1e-12
