@@ -380,15 +380,13 @@ namespace lue_amd {
     struct BlockState
     {
         double carry{0.0};  // outflow of the previous chain position (FP64)
-        float warm{0.0f};   // kinematic wave FP32: an iterate of the previous cell, to start Newton from
-        float warm_pow{0.0f};
-        bool have_warm{false};
     };
 
-    // FP32 kinematic wave of one block, cell after cell, each with the per-cell FP32 path
-    // (new_discharge_f32_path, which follows LUE's FP64 iteration where that stops short of the root).
-    // Returns this lane's outflow.
-    __device__ __attribute__((always_inline)) inline double kinematic_wave_cells_f32(bool start, double extra,
+    // FP32 kinematic wave of one block, cell after cell, each with the per-cell FP32 path (an out-of-line call
+    // of new_discharge_f32_path, which follows LUE's FP64 iteration where that stops short of the root): for
+    // the blocks the block Newton does not take. Kept small, so that it adds little to the register budget of
+    // the kernels' main paths. Returns this lane's outflow.
+    __device__ inline double kinematic_wave_cells_f32(bool start, double extra,
                                                       CellInput<op_kinematic_wave, float> const& in, int nb, float dt,
                                                       double carry)
     {
@@ -415,116 +413,6 @@ namespace lue_amd {
         return value;
     }
 
-    struct KwSerialOut
-    {
-        double value;  // this lane's outflow
-        double carry;  // outflow of the block's last position
-        float warm;
-        float warm_pow;
-        bool have_warm;
-    };
-
-    // FP32 kinematic wave of one block, cell after cell, for the blocks the block Newton does not take (no-data
-    // or dry cells, non-convergence, cells where LUE's own iteration stops short of the root): serial FP32
-    // Newton (warm start from the previous cell), then the FP64 correction of every cell in parallel,
-    // propagated along the chains to first order.
-    __device__ __attribute__((always_inline)) inline KwSerialOut kinematic_wave_serial_f32(
-        bool valid, bool start, double extra, float in_q_cur, float in_alpha, float in_beta, float in_c1,
-        float in_b_hi, float in_b_lo, int nb, bool uniform_beta, float dt, double carry_in, float warm_in,
-        float warm_pow_in, bool have_warm_in)
-    {
-        int const lane = threadIdx.x & 63;
-        double const nan64 = __longlong_as_double(0x7ff8000000000000ll);
-        CellInput<op_kinematic_wave, float> const in{in_q_cur, in_alpha, in_beta, in_c1, in_b_hi, in_b_lo};
-        BlockState st{carry_in, warm_in, warm_pow_in, have_warm_in};
-        double value = 0.0;
-        double q_up_mine = 0.0;
-        float q32_mine = 0.0f;
-        float const q_lat_mine = kw_q_lat(in, dt);
-        double const b_mine = kw_b(in);
-        double const c1_mine = kw_c1(in);
-        bool exact_mine = false;  // this lane's value is LUE's own FP64 iteration (no correction)
-        double exact_value = 0.0;
-        for (int j = 0; j < nb; ++j)
-        {
-            int const first = read_lane(static_cast<int>(start), j);
-            if (first)
-                st.have_warm = false;
-            double const q_up = (first ? 0.0 : st.carry) + read_lane(extra, j);
-            float const q_cur = read_lane(in.q_cur, j);
-            float q;
-            if (isnan(q_up) || isnan(q_cur))
-            {
-                q = __int_as_float(0x7fc00000);
-                st.have_warm = false;
-            }
-            else
-            {
-                float const q_lat = read_lane(q_lat_mine, j);
-                float const alpha = read_lane(in.alpha, j);
-                float const beta = read_lane(in.beta, j);
-                double const b = read_lane(b_mine, j);
-                double const c1 = read_lane(c1_mine, j);
-                if (q_up + q_cur == 0.0 && beta < 1.0f && q_lat > 0.0f)
-                {
-                    // LUE starts from numeric_limits::min() here: its own FP64 iteration, kept in FP64
-                    double const v = lue_iteration_f64(q_up, q_cur, q_lat, alpha, beta, dt, c1, b);
-                    st.have_warm = false;
-                    if (lane == j)
-                    {
-                        exact_mine = true;
-                        exact_value = v;
-                        q_up_mine = q_up;
-                    }
-                    st.carry = v;
-                    continue;
-                }
-                st.have_warm = st.have_warm && uniform_beta;
-                q = kinematic_wave_root_f32(static_cast<float>(c1), alpha, beta, dt, q_up, q_cur, q_lat, b, c1,
-                                            st.have_warm, st.warm, st.warm_pow);
-            }
-            if (lane == j)
-            {
-                q_up_mine = q_up;
-                q32_mine = q;
-            }
-            st.carry = static_cast<double>(q);
-        }
-        double eps = 0.0;
-        double sens = 0.0;
-        double const q = exact_mine ? exact_value : static_cast<double>(q32_mine);
-        if (valid && q > 0.0 && !isnan(q) && !exact_mine)
-        {
-            kinematic_wave_correction(q, q_up_mine, static_cast<double>(in.alpha), static_cast<double>(in.beta),
-                                      b_mine, c1_mine, eps, sens);
-        }
-        // affine scan: delta_k = eps_k + sens_k * delta_{k-1}; a chain start does not depend on the cell
-        // before it, and the first cell of the block starts from a corrected value
-        double mul = sens;
-        double add = eps;
-        int flag = (!valid || start) ? 1 : 0;
-        affine_scan(add, mul, flag);
-        value = isnan(q) ? q : (valid ? (q + add > 0.0 ? q + add : q) : 0.0);
-        // cells whose LUE iteration would not end at this root (lue_guess_converges): the block again, cell
-        // after cell, each with the per-cell FP32 path, which follows LUE's FP64 iteration there
-        double prev = wave_shr1(value);
-        if (lane == 0)
-            prev = carry_in;
-        if (start)
-            prev = 0.0;
-        float const dtq = static_cast<float>(b_mine - static_cast<double>(in.alpha) *
-                                                         static_cast<double>(fast_exp2(in.beta * fast_log2(in.q_cur))));
-        bool const suspect = valid && !exact_mine && value > 0.0 && !isnan(value) &&
-                             !lue_guess_converges(static_cast<float>(prev + extra), in.q_cur, dtq, in.alpha, in.beta,
-                                                  static_cast<float>(c1_mine), static_cast<float>(value));
-        if (__builtin_expect(__ballot(suspect) != 0, 0))
-        {
-            value = kinematic_wave_cells_f32(start, extra, in, nb, dt, carry_in);
-            st.have_warm = false;
-        }
-        return KwSerialOut{value, read_lane(value, nb - 1), st.warm, st.warm_pow, st.have_warm};
-    }
-
     // Run the cell function along one block of up to 64 chain positions (lane j: position j; `nb` valid
     // lanes). Chains start where d.start is set; the first lane continues the chain of st.carry otherwise.
     // Linear functions are scans; the kinematic wave is a block Newton (kinematic_wave_block) with LUE's
@@ -532,8 +420,8 @@ namespace lue_amd {
     // On return `value` (FP64 outflow) and `rem` hold this lane's results and st the state for the next
     // block.
     template<int OP, typename T>
-    __device__ inline void solve_block(SweepArgs<T> const& a, ChainLane<OP, T> const& d, int nb, bool uniform_beta,
-                                       BlockState& st, double& value, T& rem)
+    __device__ inline void solve_block(SweepArgs<T> const& a, ChainLane<OP, T> const& d, int nb, BlockState& st,
+                                       double& value, T& rem)
     {
         int const lane = threadIdx.x & 63;
         double const nan64 = __longlong_as_double(0x7ff8000000000000ll);
@@ -569,10 +457,7 @@ namespace lue_amd {
         {
             int const outcome = kinematic_wave_block<T>(d, st.carry, static_cast<double>(a.dt), value);
             if (__builtin_expect(outcome == kw_block_done, 1))
-            {
                 st.carry = read_lane(value, nb - 1);
-                st.have_warm = false;
-            }
             else if constexpr (std::is_same_v<T, double>)
             {
                 // plain FP64 kinematic wave: LUE's Newton per cell along the block
@@ -590,22 +475,11 @@ namespace lue_amd {
                     st.carry = v;
                 }
             }
-            else if (outcome == kw_block_lue_far)
-            {
-                value = kinematic_wave_cells_f32(d.start, d.extra, d.in, nb, a.dt, st.carry);
-                st.carry = read_lane(value, nb - 1);
-                st.have_warm = false;
-            }
             else
             {
-                KwSerialOut const o = kinematic_wave_serial_f32(
-                    d.valid, d.start, d.extra, d.in.q_cur, d.in.alpha, d.in.beta, d.in.c1, d.in.b_hi, d.in.b_lo, nb,
-                    uniform_beta, a.dt, st.carry, st.warm, st.warm_pow, st.have_warm);
-                value = o.value;
-                st.carry = o.carry;
-                st.warm = o.warm;
-                st.warm_pow = o.warm_pow;
-                st.have_warm = o.have_warm;
+                // FP32 kinematic wave: the per-cell FP32 path along the block
+                value = kinematic_wave_cells_f32(d.start, d.extra, d.in, nb, a.dt, st.carry);
+                st.carry = read_lane(value, nb - 1);
             }
         }
     }

@@ -503,7 +503,6 @@ namespace lue_amd {
         double* side;                     // per position: FP64 sum of the upstream cells off the chain
         double* res;                      // per position: outflow (FP64)
         T* res_rem;                       // per position: remainder (threshold, fraction)
-        int uniform_beta;                 // kinematic wave: beta is a scalar (Newton may start from the previous cell)
     };
 
     // Add the side references of positions [pos_begin, pos_end) (one chain level) to their staged side sums, in
@@ -583,7 +582,7 @@ namespace lue_amd {
             int const nb = static_cast<int>(count - block < 64 ? count - block : 64);
             double value;
             T rem{};
-            solve_block<OP, T>(a, d, nb, c.uniform_beta != 0, st, value, rem);
+            solve_block<OP, T>(a, d, nb, st, value, rem);
             if (d.valid)
             {
                 std::uint32_t const p = start + block + lane;

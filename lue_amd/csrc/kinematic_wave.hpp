@@ -343,73 +343,6 @@ namespace lue_amd {
     }
 
 
-    // FP32 root of one cell's equation, without the FP64 correction (the caller applies it, possibly
-    // later and in parallel). Dry cells give 0; cells that start from numeric_limits::min() in LUE take
-    // the FP64 iteration. When `have_warm` is set, Newton starts from (warm_x, warm_x^beta = warm_pow),
-    // an iterate of the previous cell on the same flow path (beta must be the same); on return they hold
-    // an iterate of this cell.
-    __device__ inline float kinematic_wave_root_f32(float c1f, float alpha, float beta, float dt, double q_up,
-                                                    float q_cur, float q_lat, double b, double c1, bool& have_warm,
-                                                    float& warm_x, float& warm_pow)
-    {
-        if (!(q_up + q_cur > 0 || q_lat > 0))
-        {
-            have_warm = false;
-            return 0.0f;
-        }
-        if (q_up + q_cur == 0.0 && beta < 1.0f)
-        {
-            have_warm = false;
-            return static_cast<float>(new_discharge_f32_path(q_up, q_cur, q_lat, alpha, beta, dt, b, c1));
-        }
-        float const known = static_cast<float>(c1 * q_up + b);
-        float const ab = alpha * beta;
-        float x;
-        float xb;
-        if (have_warm)
-        {
-            x = warm_x;
-            xb = warm_pow;
-        }
-        else
-        {
-            float const a_b_pq = ab * fast_exp2((beta - 1.0f) * fast_log2(0.5f * (q_cur + static_cast<float>(q_up))));
-            x = (c1f * static_cast<float>(q_up) + a_b_pq * q_cur + dt * q_lat) * fast_rcp(c1f + a_b_pq);
-            x = fmaxf(x, FLT_MIN);
-            xb = fast_exp2(beta * fast_log2(x));
-        }
-        for (int iteration = 0; iteration < 10; ++iteration)
-        {
-            float const f = fmaf(c1f, x, fmaf(alpha, xb, -known));
-            float const fp = fmaf(ab * xb, fast_rcp(x), c1f);
-            float const step = f * fast_rcp(fp);
-            float xn = x - step;
-            xn = xn > 0.0f ? xn : 0.5f * x;
-            bool const done = fabsf(step) <= fabsf(xn) * 0x1p-13f;
-            warm_x = x;
-            warm_pow = xb;
-            x = xn;
-            if (done)
-                break;
-            xb = fast_exp2(beta * fast_log2(x));
-        }
-        have_warm = true;
-        return x;
-    }
-
-    // FP64 residual correction of an FP32 root x of f(Q) = c1 Q + alpha Q^beta - (c1 q_up + b): one
-    // Newton step in FP64, eps = -f(x) / f'(x), and the sensitivity of the root to q_up, c1 / f'(x).
-    __device__ inline void kinematic_wave_correction(double x, double q_up, double alpha, double beta, double b,
-                                                     double c1, double& eps, double& sens)
-    {
-        double const xb = pow_positive(x, beta);  // x > 0 here
-        double const ax = alpha * xb;
-        double const f0 = c1 * x + ax - (c1 * q_up + b);
-        double const f1 = c1 + beta * ax / x;
-        eps = -f0 / f1;
-        sens = c1 / f1;
-    }
-
     // Elementwise FP32 solve with the corrected root, used by iterate_to_new_discharge_f32.
     __device__ inline double new_discharge_f32_corrected(
         double q_up, float q_cur, float q_lat, float alpha, float beta, float dt, float dx, double alpha_pow)

@@ -967,9 +967,6 @@ namespace lue_amd {
     {
         if (net.pending == 0)
             return status_success;
-        bool uniform_beta = true;
-        for (auto const& x : io)
-            uniform_beta = uniform_beta && x.beta == nullptr;
         ChainRun<OP, T> c{net.item_start.ptr,
                           net.item_count.ptr,
                           net.seg_start.ptr,
@@ -978,8 +975,7 @@ namespace lue_amd {
                           reinterpret_cast<CellInput<OP, T> const*>(w.staged.ptr),
                           w.side.ptr,
                           w.res.ptr,
-                          reinterpret_cast<T*>(w.res_rem.ptr),
-                          uniform_beta ? 1 : 0};
+                          reinterpret_cast<T*>(w.res_rem.ptr)};
         for (int l = 0; l < net.levels(); ++l)
         {
             std::uint32_t const pos_begin = net.level_pos[l];

@@ -282,7 +282,6 @@ namespace lue_amd {
             static_assert((G::n & (G::n - 1)) == 0, "tile cell count must be a power of two");
             int const wave = tid >> 6;
             int const lane_id = tid & 63;
-            bool const uniform_beta = io.beta == nullptr;
             for (int l = 0; l < levels; ++l)
             {
                 int const lo = split[l * (sweep_waves + 1) + wave];
@@ -323,7 +322,7 @@ namespace lue_amd {
                     }
                     double value;
                     T rem{};
-                    solve_block<OP, T>(a, d, nb, uniform_beta, st, value, rem);
+                    solve_block<OP, T>(a, d, nb, st, value, rem);
                     if (d.valid)
                     {
                         T const h = static_cast<T>(value);
