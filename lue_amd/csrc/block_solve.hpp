@@ -243,10 +243,10 @@ namespace lue_amd {
     // Returns kw_block_serial, with q untouched, when a lane needs LUE's per-cell iteration: no-data, a zero
     // current discharge (dry cells and cells whose iteration starts from numeric_limits::min()), a non-positive
     // iterate, or no convergence within 16 steps; kw_block_lue_far when LUE's own iteration would stop short
-    // of the root in some lane (lue_guess_converges).
+    // of the root in some lane (lue_guess_converges; q then holds the roots and far_mask those lanes).
     template<typename T>
     __device__ inline int kinematic_wave_block(ChainLane<op_kinematic_wave, T> const& d, double carry, double dt,
-                                                double& q)
+                                                double& q, std::uint64_t& far_mask)
     {
         int const lane = threadIdx.x & 63;
         bool const valid = d.valid;
@@ -369,10 +369,11 @@ namespace lue_amd {
             bool const far = valid && !lue_guess_converges(static_cast<float>(prev + d.extra), static_cast<float>(q_cur),
                                                            dtq, static_cast<float>(alpha), beta32, c1f,
                                                            static_cast<float>(x));
-            if (__builtin_expect(__ballot(far) != 0, 0))
+            q = valid ? x : 0.0;
+            far_mask = __ballot(far);
+            if (__builtin_expect(far_mask != 0, 0))
                 return kw_block_lue_far;
         }
-        q = valid ? x : 0.0;
         return kw_block_done;
     }
 
@@ -381,37 +382,6 @@ namespace lue_amd {
     {
         double carry{0.0};  // outflow of the previous chain position (FP64)
     };
-
-    // FP32 kinematic wave of one block, cell after cell, each with the per-cell FP32 path (an out-of-line call
-    // of new_discharge_f32_path, which follows LUE's FP64 iteration where that stops short of the root): for
-    // the blocks the block Newton does not take. Kept small, so that it adds little to the register budget of
-    // the kernels' main paths. Returns this lane's outflow.
-    __device__ inline double kinematic_wave_cells_f32(bool start, double extra,
-                                                      CellInput<op_kinematic_wave, float> const& in, int nb, float dt,
-                                                      double carry)
-    {
-        int const lane = threadIdx.x & 63;
-        float const q_lat_mine = kw_q_lat(in, dt);
-        double const b_mine = kw_b(in);
-        double value = 0.0;
-        for (int j = 0; j < nb; ++j)
-        {
-            int const first = read_lane(static_cast<int>(start), j);
-            double const up = (first ? 0.0 : carry) + read_lane(extra, j);
-            float const q_cur = read_lane(in.q_cur, j);
-            double v;
-            if (isnan(up) || isnan(q_cur))
-                v = __longlong_as_double(0x7ff8000000000000ll);
-            else
-                v = new_discharge_f32_path_call(up, q_cur, read_lane(q_lat_mine, j), read_lane(in.alpha, j),
-                                                read_lane(in.beta, j), dt, read_lane(b_mine, j),
-                                                read_lane(kw_c1(in), j));
-            if (lane == j)
-                value = v;
-            carry = v;
-        }
-        return value;
-    }
 
     // Run the cell function along one block of up to 64 chain positions (lane j: position j; `nb` valid
     // lanes). Chains start where d.start is set; the first lane continues the chain of st.carry otherwise.
@@ -455,31 +425,63 @@ namespace lue_amd {
         }
         else
         {
-            int const outcome = kinematic_wave_block<T>(d, st.carry, static_cast<double>(a.dt), value);
+            std::uint64_t far_mask = 0;
+            int const outcome = kinematic_wave_block<T>(d, st.carry, static_cast<double>(a.dt), value, far_mask);
             if (__builtin_expect(outcome == kw_block_done, 1))
-                st.carry = read_lane(value, nb - 1);
-            else if constexpr (std::is_same_v<T, double>)
             {
-                // plain FP64 kinematic wave: LUE's Newton per cell along the block
-                for (int j = 0; j < nb; ++j)
-                {
-                    int const first = read_lane(static_cast<int>(d.start), j);
-                    double const up = (first ? 0.0 : st.carry) + read_lane(d.extra, j);
-                    CellInput<OP, T> cj{read_lane(d.in.q_cur, j), read_lane(d.in.q_lat, j), read_lane(d.in.alpha, j),
-                                        read_lane(d.in.beta, j), read_lane(d.in.b, j), read_lane(d.in.c1, j)};
-                    double v;
-                    T r{};
-                    solve_cell<OP, T>(a, cj, up, isnan(up), v, r);
-                    if (lane == j)
-                        value = v;
-                    st.carry = v;
-                }
+                st.carry = read_lane(value, nb - 1);
             }
             else
             {
-                // FP32 kinematic wave: the per-cell FP32 path along the block
-                value = kinematic_wave_cells_f32(d.start, d.extra, d.in, nb, a.dt, st.carry);
-                st.carry = read_lane(value, nb - 1);
+                // per-cell path (LUE's iteration on the FP64 path, new_discharge_f32_path on the FP32 path):
+                // along the whole block, or, where only LUE's own iteration stops short of the root, from the
+                // first such lane through the positions of its chain after it (the other lanes keep the roots,
+                // which are LUE's results there)
+                bool const all = outcome == kw_block_serial;
+                int const j0 = all ? 0 : __builtin_ctzll(far_mask);
+                double carry = j0 == 0 ? st.carry : read_lane(value, j0 - 1);
+                [[maybe_unused]] float q_lat_mine = 0.0f;
+                [[maybe_unused]] double b_mine = 0.0;
+                if constexpr (std::is_same_v<T, float>)
+                {
+                    q_lat_mine = kw_q_lat(d.in, a.dt);
+                    b_mine = kw_b(d.in);
+                }
+                bool affected = false;
+                for (int j = j0; j < nb; ++j)
+                {
+                    int const first = read_lane(static_cast<int>(d.start), j);
+                    affected = all || ((far_mask >> j) & 1u) != 0 || (affected && !first);
+                    if (!affected)
+                    {
+                        carry = read_lane(value, j);
+                        continue;
+                    }
+                    double const up = (first ? 0.0 : carry) + read_lane(d.extra, j);
+                    double v;
+                    if constexpr (std::is_same_v<T, double>)
+                    {
+                        CellInput<OP, T> cj{read_lane(d.in.q_cur, j), read_lane(d.in.q_lat, j),
+                                            read_lane(d.in.alpha, j), read_lane(d.in.beta, j),
+                                            read_lane(d.in.b, j), read_lane(d.in.c1, j)};
+                        T r{};
+                        solve_cell<OP, T>(a, cj, up, isnan(up), v, r);
+                    }
+                    else
+                    {
+                        float const q_cur = read_lane(d.in.q_cur, j);
+                        if (isnan(up) || isnan(q_cur))
+                            v = nan64;
+                        else
+                            v = new_discharge_f32_path_call(up, q_cur, read_lane(q_lat_mine, j),
+                                                            read_lane(d.in.alpha, j), read_lane(d.in.beta, j), a.dt,
+                                                            read_lane(b_mine, j), read_lane(kw_c1(d.in), j));
+                    }
+                    if (lane == j)
+                        value = v;
+                    carry = v;
+                }
+                st.carry = carry;
             }
         }
     }
