@@ -135,6 +135,32 @@ int lue_amd_accu_fraction_f64(const uint8_t* ldd, int64_t rows, int64_t cols, co
                               double* outflow, double* remainder, void* stream);
 
 /*
+ * accu_capacity: at most the capacity flows on, the rest stays (remainder).
+ * Replaces lue::value_policies::accu_capacity(flow_direction, inflow, capacity), all four
+ * raster/Scalar combinations (ALG/value_policies/accu_capacity.hpp:34-88, ALG/accu_capacity.hpp:9-50,
+ * ALG/definition/accu_capacity.hpp:124-166).
+ */
+int lue_amd_accu_capacity_f32(const uint8_t* ldd, int64_t rows, int64_t cols, const float* inflow,
+                              float inflow_scalar, const float* capacity, float capacity_scalar, float* outflow,
+                              float* remainder, void* stream);
+int lue_amd_accu_capacity_f64(const uint8_t* ldd, int64_t rows, int64_t cols, const double* inflow,
+                              double inflow_scalar, const double* capacity, double capacity_scalar,
+                              double* outflow, double* remainder, void* stream);
+
+/*
+ * accu_trigger: material below the trigger stays (remainder), from the trigger on all of it flows on.
+ * Replaces lue::value_policies::accu_trigger(flow_direction, inflow, trigger), all four
+ * raster/Scalar combinations (ALG/value_policies/accu_trigger.hpp:34-88, ALG/accu_trigger.hpp:9-50,
+ * ALG/definition/accu_trigger.hpp:124-164).
+ */
+int lue_amd_accu_trigger_f32(const uint8_t* ldd, int64_t rows, int64_t cols, const float* inflow,
+                             float inflow_scalar, const float* trigger, float trigger_scalar, float* outflow,
+                             float* remainder, void* stream);
+int lue_amd_accu_trigger_f64(const uint8_t* ldd, int64_t rows, int64_t cols, const double* inflow,
+                             double inflow_scalar, const double* trigger, double trigger_scalar,
+                             double* outflow, double* remainder, void* stream);
+
+/*
  * kinematic_wave: new discharge per cell from the upstream new discharge, the current discharge and
  * the lateral inflow (non-linear kinematic wave, Newton solve per cell).
  * Replaces both lue::value_policies::kinematic_wave overloads (ALG/value_policies/kinematic_wave.hpp:43-91,
@@ -216,6 +242,18 @@ int lue_amd_network_accu_fraction_f32(lue_amd_network network, const float* cons
 int lue_amd_network_accu_fraction_f64(lue_amd_network network, const double* const* inflow, double inflow_scalar,
                                       const double* const* fraction, double fraction_scalar, double* const* outflow,
                                       double* const* remainder, void* stream);
+int lue_amd_network_accu_capacity_f32(lue_amd_network network, const float* const* inflow, float inflow_scalar,
+                                      const float* const* capacity, float capacity_scalar, float* const* outflow,
+                                      float* const* remainder, void* stream);
+int lue_amd_network_accu_capacity_f64(lue_amd_network network, const double* const* inflow, double inflow_scalar,
+                                      const double* const* capacity, double capacity_scalar, double* const* outflow,
+                                      double* const* remainder, void* stream);
+int lue_amd_network_accu_trigger_f32(lue_amd_network network, const float* const* inflow, float inflow_scalar,
+                                     const float* const* trigger, float trigger_scalar, float* const* outflow,
+                                     float* const* remainder, void* stream);
+int lue_amd_network_accu_trigger_f64(lue_amd_network network, const double* const* inflow, double inflow_scalar,
+                                     const double* const* trigger, double trigger_scalar, double* const* outflow,
+                                     double* const* remainder, void* stream);
 int lue_amd_network_kinematic_wave_f32(lue_amd_network network, const float* const* current_outflow,
                                        const float* const* inflow, const float* const* alpha, float alpha_scalar,
                                        const float* const* beta, float beta_scalar, float time_step_duration,

@@ -38,6 +38,10 @@ SIGNATURES = {
     "lue_amd_network_accu_threshold_f64": [c_p, c_p, c_d, c_p, c_d, c_p, c_p, c_p],
     "lue_amd_network_accu_fraction_f32": [c_p, c_p, c_f, c_p, c_f, c_p, c_p, c_p],
     "lue_amd_network_accu_fraction_f64": [c_p, c_p, c_d, c_p, c_d, c_p, c_p, c_p],
+    "lue_amd_network_accu_capacity_f32": [c_p, c_p, c_f, c_p, c_f, c_p, c_p, c_p],
+    "lue_amd_network_accu_capacity_f64": [c_p, c_p, c_d, c_p, c_d, c_p, c_p, c_p],
+    "lue_amd_network_accu_trigger_f32": [c_p, c_p, c_f, c_p, c_f, c_p, c_p, c_p],
+    "lue_amd_network_accu_trigger_f64": [c_p, c_p, c_d, c_p, c_d, c_p, c_p, c_p],
     "lue_amd_network_kinematic_wave_f32": [c_p, c_p, c_p, c_p, c_f, c_p, c_f, c_f, c_p, c_f, c_p, c_p],
     "lue_amd_network_kinematic_wave_f64": [c_p, c_p, c_p, c_p, c_d, c_p, c_d, c_d, c_p, c_d, c_p, c_p],
     "lue_amd_profile_enable": [c_int],
@@ -54,6 +58,10 @@ SIGNATURES = {
     "lue_amd_accu_threshold_f64": [c_p, c_i64, c_i64, c_p, c_d, c_p, c_d, c_p, c_p, c_p],
     "lue_amd_accu_fraction_f32": [c_p, c_i64, c_i64, c_p, c_f, c_p, c_f, c_p, c_p, c_p],
     "lue_amd_accu_fraction_f64": [c_p, c_i64, c_i64, c_p, c_d, c_p, c_d, c_p, c_p, c_p],
+    "lue_amd_accu_capacity_f32": [c_p, c_i64, c_i64, c_p, c_f, c_p, c_f, c_p, c_p, c_p],
+    "lue_amd_accu_capacity_f64": [c_p, c_i64, c_i64, c_p, c_d, c_p, c_d, c_p, c_p, c_p],
+    "lue_amd_accu_trigger_f32": [c_p, c_i64, c_i64, c_p, c_f, c_p, c_f, c_p, c_p, c_p],
+    "lue_amd_accu_trigger_f64": [c_p, c_i64, c_i64, c_p, c_d, c_p, c_d, c_p, c_p, c_p],
     "lue_amd_kinematic_wave_f32": [c_p, c_i64, c_i64, c_p, c_p, c_p, c_f, c_p, c_f, c_f, c_p, c_f, c_p, c_p],
     "lue_amd_kinematic_wave_f64": [c_p, c_i64, c_i64, c_p, c_p, c_p, c_d, c_p, c_d, c_d, c_p, c_d, c_p, c_p],
     "lue_amd_iterate_to_new_discharge_f32": [c_p] * 8 + [c_i64, c_p],

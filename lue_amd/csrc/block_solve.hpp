@@ -405,7 +405,7 @@ namespace lue_amd {
             value = flag ? term : st.carry + term;
             st.carry = read_lane(value, nb - 1);
         }
-        else if constexpr (OP == op_threshold || OP == op_fraction)
+        else if constexpr (is_split_op(OP))
         {
             for (int j = 0; j < nb; ++j)
             {

@@ -138,6 +138,20 @@ namespace lue_amd {
         T p;
     };
 
+    template<typename T>
+    struct CellInput<op_capacity, T>
+    {
+        T x;
+        T p;
+    };
+
+    template<typename T>
+    struct CellInput<op_trigger, T>
+    {
+        T x;
+        T p;
+    };
+
     // Kinematic wave, plain FP64 path.
     template<typename T>
     struct CellInput<op_kinematic_wave, T>
@@ -251,13 +265,13 @@ namespace lue_amd {
             // value_policies/accu.hpp:9-19: inflow >= 0
             c.x = (x >= T{0}) ? x : quiet_nan<T>();
         }
-        else if constexpr (OP == op_threshold || OP == op_fraction)
+        else if constexpr (is_split_op(OP))
         {
             T const x = r.v[0];
             T const p = r.v[1];
-            // value_policies/accu_threshold.hpp:9-20, value_policies/accu_fraction.hpp:9-20
-            bool const ok = OP == op_threshold ? (x >= T{0} && p >= T{0})
-                                               : (x >= T{0} && p >= T{0} && p <= T{1});
+            // value_policies/accu_threshold.hpp:9-20, value_policies/accu_fraction.hpp:9-20,
+            // value_policies/accu_capacity.hpp:15-19, value_policies/accu_trigger.hpp:15-19
+            bool const ok = OP == op_fraction ? (x >= T{0} && p >= T{0} && p <= T{1}) : (x >= T{0} && p >= T{0});
             c.x = ok ? x : quiet_nan<T>();
             c.p = p;
         }
@@ -338,6 +352,53 @@ namespace lue_amd {
                 {
                     rem = static_cast<T>(total);
                     out = 0.0;
+                }
+            }
+        }
+        else if constexpr (OP == op_capacity)
+        {
+            // accu_capacity.hpp:124-166: the outflow is at most the capacity, the rest remains
+            if (poisoned || isnan(in.x) || isnan(in.p))
+            {
+                out = quiet_nan<double>();
+                rem = quiet_nan<T>();
+            }
+            else
+            {
+                double const total = upstream + static_cast<double>(in.x);
+                double const cap = static_cast<double>(in.p);
+                if (total > cap)
+                {
+                    rem = static_cast<T>(total - cap);
+                    out = cap;
+                }
+                else
+                {
+                    rem = T{0};
+                    out = total;
+                }
+            }
+        }
+        else if constexpr (OP == op_trigger)
+        {
+            // accu_trigger.hpp:124-164: below the trigger all material remains, from it on all flows
+            if (poisoned || isnan(in.x) || isnan(in.p))
+            {
+                out = quiet_nan<double>();
+                rem = quiet_nan<T>();
+            }
+            else
+            {
+                double const total = upstream + static_cast<double>(in.x);
+                if (total < static_cast<double>(in.p))
+                {
+                    rem = static_cast<T>(total);
+                    out = 0.0;
+                }
+                else
+                {
+                    rem = T{0};
+                    out = total;
                 }
             }
         }

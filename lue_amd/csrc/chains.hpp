@@ -562,7 +562,7 @@ namespace lue_amd {
     __global__ void __launch_bounds__(256)
         chains_run_kernel(SweepArgs<T> a, ChainRun<OP, T> c, std::uint32_t item_begin, std::uint32_t item_end)
     {
-        constexpr bool two_outputs = OP == op_threshold || OP == op_fraction;
+        constexpr bool two_outputs = is_split_op(OP);
         int const lane = threadIdx.x & 63;
         std::uint32_t const item = item_begin + blockIdx.x * 4 + (threadIdx.x >> 6);
         if (item >= item_end)
@@ -600,7 +600,7 @@ namespace lue_amd {
                               double const* res, T const* res_rem)
     {
         using G = Tile<TW, TH>;
-        constexpr bool two_outputs = OP == op_threshold || OP == op_fraction;
+        constexpr bool two_outputs = is_split_op(OP);
         __shared__ std::uint32_t prefix[G::words + 1];
         __shared__ std::uint32_t words[G::words];
         std::int64_t const tile = blockIdx.x;

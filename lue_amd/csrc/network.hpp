@@ -949,7 +949,7 @@ namespace lue_amd {
         LUE_AMD_HIP(w.staged.allocate(n * sizeof(CellInput<OP, T>), stream));
         LUE_AMD_HIP(w.side.allocate(n, stream));
         LUE_AMD_HIP(w.res.allocate(n, stream));
-        if constexpr (OP == op_threshold || OP == op_fraction)
+        if constexpr (is_split_op(OP))
         {
             LUE_AMD_HIP(w.res_rem.allocate(n * sizeof(T), stream));
         }

@@ -551,6 +551,38 @@ int lue_amd_accu_fraction_f64(const uint8_t* ldd, int64_t rows, int64_t cols, co
                                            outflow, remainder, stream);
 }
 
+int lue_amd_accu_capacity_f32(const uint8_t* ldd, int64_t rows, int64_t cols, const float* inflow,
+                              float inflow_scalar, const float* capacity, float capacity_scalar, float* outflow,
+                              float* remainder, void* stream)
+{
+    return split_impl<op_capacity, float>(ldd, rows, cols, inflow, inflow_scalar, capacity, capacity_scalar,
+                                          outflow, remainder, stream);
+}
+
+int lue_amd_accu_capacity_f64(const uint8_t* ldd, int64_t rows, int64_t cols, const double* inflow,
+                              double inflow_scalar, const double* capacity, double capacity_scalar,
+                              double* outflow, double* remainder, void* stream)
+{
+    return split_impl<op_capacity, double>(ldd, rows, cols, inflow, inflow_scalar, capacity, capacity_scalar,
+                                           outflow, remainder, stream);
+}
+
+int lue_amd_accu_trigger_f32(const uint8_t* ldd, int64_t rows, int64_t cols, const float* inflow,
+                             float inflow_scalar, const float* trigger, float trigger_scalar, float* outflow,
+                             float* remainder, void* stream)
+{
+    return split_impl<op_trigger, float>(ldd, rows, cols, inflow, inflow_scalar, trigger, trigger_scalar, outflow,
+                                         remainder, stream);
+}
+
+int lue_amd_accu_trigger_f64(const uint8_t* ldd, int64_t rows, int64_t cols, const double* inflow,
+                             double inflow_scalar, const double* trigger, double trigger_scalar, double* outflow,
+                             double* remainder, void* stream)
+{
+    return split_impl<op_trigger, double>(ldd, rows, cols, inflow, inflow_scalar, trigger, trigger_scalar, outflow,
+                                          remainder, stream);
+}
+
 int lue_amd_kinematic_wave_f32(const uint8_t* ldd, int64_t rows, int64_t cols, const float* current_outflow,
                                const float* inflow, const float* alpha, float alpha_scalar, const float* beta,
                                float beta_scalar, float time_step_duration, const float* channel_length,
@@ -746,6 +778,38 @@ int lue_amd_network_info(lue_amd_network network, int64_t* info, int n)
             io.push_back(CatchmentIO<T>{entry(inflow, c), entry(fraction, c), nullptr, nullptr, nullptr, outflow[c], \
                                         remainder[c]});                                                         \
         return route_network<op_fraction, T>(net, sc, io, static_cast<hipStream_t>(stream));                    \
+    }                                                                                                           \
+    int lue_amd_network_accu_capacity_##SUFFIX(lue_amd_network network, const T* const* inflow, T inflow_scalar, \
+                                               const T* const* capacity, T capacity_scalar, T* const* outflow,  \
+                                               T* const* remainder, void* stream)                               \
+    {                                                                                                           \
+        if (!network || !outflow || !remainder)                                                                 \
+            return fail(status_invalid_argument, "null argument");                                              \
+        Network const& net = network->net;                                                                      \
+        Scalars<T> sc{};                                                                                        \
+        sc.in1_s = inflow_scalar;                                                                               \
+        sc.in2_s = capacity_scalar;                                                                             \
+        std::vector<CatchmentIO<T>> io;                                                                         \
+        for (int c = 0; c < static_cast<int>(net.cats_host.size()); ++c)                                        \
+            io.push_back(CatchmentIO<T>{entry(inflow, c), entry(capacity, c), nullptr, nullptr, nullptr, outflow[c], \
+                                        remainder[c]});                                                         \
+        return route_network<op_capacity, T>(net, sc, io, static_cast<hipStream_t>(stream));                    \
+    }                                                                                                           \
+    int lue_amd_network_accu_trigger_##SUFFIX(lue_amd_network network, const T* const* inflow, T inflow_scalar, \
+                                               const T* const* trigger, T trigger_scalar, T* const* outflow,    \
+                                               T* const* remainder, void* stream)                               \
+    {                                                                                                           \
+        if (!network || !outflow || !remainder)                                                                 \
+            return fail(status_invalid_argument, "null argument");                                              \
+        Network const& net = network->net;                                                                      \
+        Scalars<T> sc{};                                                                                        \
+        sc.in1_s = inflow_scalar;                                                                               \
+        sc.in2_s = trigger_scalar;                                                                              \
+        std::vector<CatchmentIO<T>> io;                                                                         \
+        for (int c = 0; c < static_cast<int>(net.cats_host.size()); ++c)                                        \
+            io.push_back(CatchmentIO<T>{entry(inflow, c), entry(trigger, c), nullptr, nullptr, nullptr, outflow[c], \
+                                        remainder[c]});                                                         \
+        return route_network<op_trigger, T>(net, sc, io, static_cast<hipStream_t>(stream));                     \
     }                                                                                                           \
     int lue_amd_network_kinematic_wave_##SUFFIX(lue_amd_network network, const T* const* current_outflow,       \
                                                 const T* const* inflow, const T* const* alpha, T alpha_scalar,  \

@@ -65,6 +65,20 @@ namespace lue_amd {
     }
 
     // Operation kinds of the accumulating router.
-    enum Op : int { op_accu = 0, op_threshold = 1, op_fraction = 2, op_kinematic_wave = 3 };
+    enum Op : int
+    {
+        op_accu = 0,
+        op_threshold = 1,
+        op_fraction = 2,
+        op_kinematic_wave = 3,
+        op_capacity = 4,
+        op_trigger = 5
+    };
+
+    // Operations with two outputs (outflow, remainder) split from the material that enters a cell.
+    constexpr bool is_split_op(int op)
+    {
+        return op == op_threshold || op == op_fraction || op == op_capacity || op == op_trigger;
+    }
 
 }  // namespace lue_amd

@@ -68,7 +68,7 @@ namespace lue_amd {
     {
         using G = Tile<TW, TH>;
         static constexpr bool compensated = std::is_same_v<T, float>;
-        static constexpr bool two_outputs = OP == op_threshold || OP == op_fraction;
+        static constexpr bool two_outputs = is_split_op(OP);
 
         CellInput<OP, T> in[G::n];
         T hi[G::n];

@@ -182,6 +182,16 @@ def accu_fraction(flow_direction, inflow, fraction):
     return _split("accu_fraction", flow_direction, inflow, fraction)
 
 
+def accu_capacity(flow_direction, inflow, capacity):
+    """lue.framework.accu_capacity -> (outflow, remainder) (value_policies/accu_capacity.hpp:34-88)."""
+    return _split("accu_capacity", flow_direction, inflow, capacity)
+
+
+def accu_trigger(flow_direction, inflow, trigger):
+    """lue.framework.accu_trigger -> (outflow, remainder) (value_policies/accu_trigger.hpp:34-88)."""
+    return _split("accu_trigger", flow_direction, inflow, trigger)
+
+
 def kinematic_wave(flow_direction, current_outflow, inflow, alpha, beta, time_step_duration, channel_length):
     """lue.framework.kinematic_wave -> new outflow (value_policies/kinematic_wave.hpp:43-91)."""
     _check_ldd(flow_direction)
@@ -342,6 +352,12 @@ class FlowNetwork:
 
     def accu_fraction(self, inflow, fraction, out=None, remainder=None):
         return self._split("fraction", inflow, fraction, out, remainder)
+
+    def accu_capacity(self, inflow, capacity, out=None, remainder=None):
+        return self._split("capacity", inflow, capacity, out, remainder)
+
+    def accu_trigger(self, inflow, trigger, out=None, remainder=None):
+        return self._split("trigger", inflow, trigger, out, remainder)
 
     def kinematic_wave(self, current_outflow, inflow, alpha, beta, time_step_duration, channel_length, out=None):
         dtype = self._dtype(current_outflow, inflow)

@@ -8,7 +8,7 @@
 //     raster in HBM;
 //   - lue::Scalar<E>  (source/framework/algorithm/include/lue/framework/algorithm/scalar.hpp:20);
 //   - lue::create_partitioned_array(shape, partition_shape, fill_value);
-//   - lue::value_policies::{accu, accu_threshold, accu_fraction, kinematic_wave} with the reference's
+//   - lue::value_policies::{accu, accu_threshold, accu_fraction, accu_capacity, accu_trigger, kinematic_wave} with the reference's
 //     overload sets (ALG/value_policies/*.hpp) and lue::inflow_count<std::uint8_t>(ldd)
 //     (ALG/inflow_count.hpp:29-42), ALG = source/framework/algorithm/include/lue/framework/algorithm.
 // Each operation calls one function of the C ABI (include/lue_amd/routing.h). Shape and partitioning
@@ -342,7 +342,10 @@ namespace lue {
 
         namespace detail2 {
 
-            template<bool threshold, std::floating_point E, typename Inflow, typename Param>
+            // the two-output operations: (outflow, remainder)
+            enum class Split { threshold, fraction, capacity, trigger };
+
+            template<Split kind, std::floating_point E, typename Inflow, typename Param>
             auto split(PartitionedArray<FlowDirectionElement, 2> const& flow_direction, Inflow const& inflow,
                        Param const& param) -> std::tuple<PartitionedArray<E, 2>, PartitionedArray<E, 2>>
             {
@@ -356,19 +359,31 @@ namespace lue {
                 auto const s1 = lue::detail::scalar(inflow);
                 auto const* p2 = lue::detail::pointer(param);
                 auto const s2 = lue::detail::scalar(param);
+                E* const out = outflow.data();
+                E* const rem = remainder.data();
                 int status;
-                if constexpr (threshold && std::is_same_v<E, float>)
-                    status = lue_amd_accu_threshold_f32(ldd, rows, cols, p1, s1, p2, s2, outflow.data(),
-                                                        remainder.data(), nullptr);
-                else if constexpr (threshold)
-                    status = lue_amd_accu_threshold_f64(ldd, rows, cols, p1, s1, p2, s2, outflow.data(),
-                                                        remainder.data(), nullptr);
-                else if constexpr (std::is_same_v<E, float>)
-                    status = lue_amd_accu_fraction_f32(ldd, rows, cols, p1, s1, p2, s2, outflow.data(),
-                                                       remainder.data(), nullptr);
+                if constexpr (std::is_same_v<E, float>)
+                {
+                    if constexpr (kind == Split::threshold)
+                        status = lue_amd_accu_threshold_f32(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                    else if constexpr (kind == Split::fraction)
+                        status = lue_amd_accu_fraction_f32(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                    else if constexpr (kind == Split::capacity)
+                        status = lue_amd_accu_capacity_f32(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                    else
+                        status = lue_amd_accu_trigger_f32(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                }
                 else
-                    status = lue_amd_accu_fraction_f64(ldd, rows, cols, p1, s1, p2, s2, outflow.data(),
-                                                       remainder.data(), nullptr);
+                {
+                    if constexpr (kind == Split::threshold)
+                        status = lue_amd_accu_threshold_f64(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                    else if constexpr (kind == Split::fraction)
+                        status = lue_amd_accu_fraction_f64(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                    else if constexpr (kind == Split::capacity)
+                        status = lue_amd_accu_capacity_f64(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                    else
+                        status = lue_amd_accu_trigger_f64(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                }
                 lue::detail::check(status);
                 return {std::move(outflow), std::move(remainder)};
             }
@@ -377,60 +392,118 @@ namespace lue {
 
         // ALG/value_policies/accu_threshold.hpp:34-87: raster|Scalar inflow x raster|Scalar threshold
         template<std::integral FlowDirection, std::floating_point E>
-        auto accu_threshold(PartitionedArray<FlowDirection, 2> const& flow_direction,
-                            PartitionedArray<E, 2> const& inflow, PartitionedArray<E, 2> const& threshold)
+        auto accu_threshold(PartitionedArray<FlowDirection, 2> const& flow_direction, PartitionedArray<E, 2> const& inflow,
+                     PartitionedArray<E, 2> const& threshold)
         {
-            return detail2::split<true, E>(flow_direction, inflow, threshold);
+            return detail2::split<detail2::Split::threshold, E>(flow_direction, inflow, threshold);
         }
 
         template<std::integral FlowDirection, std::floating_point E>
-        auto accu_threshold(PartitionedArray<FlowDirection, 2> const& flow_direction,
-                            PartitionedArray<E, 2> const& inflow, Scalar<E> const& threshold)
+        auto accu_threshold(PartitionedArray<FlowDirection, 2> const& flow_direction, PartitionedArray<E, 2> const& inflow,
+                     Scalar<E> const& threshold)
         {
-            return detail2::split<true, E>(flow_direction, inflow, threshold);
-        }
-
-        template<std::integral FlowDirection, std::floating_point E>
-        auto accu_threshold(PartitionedArray<FlowDirection, 2> const& flow_direction, Scalar<E> const& inflow,
-                            Scalar<E> const& threshold)
-        {
-            return detail2::split<true, E>(flow_direction, inflow, threshold);
+            return detail2::split<detail2::Split::threshold, E>(flow_direction, inflow, threshold);
         }
 
         template<std::integral FlowDirection, std::floating_point E>
         auto accu_threshold(PartitionedArray<FlowDirection, 2> const& flow_direction, Scalar<E> const& inflow,
-                            PartitionedArray<E, 2> const& threshold)
+                     Scalar<E> const& threshold)
         {
-            return detail2::split<true, E>(flow_direction, inflow, threshold);
+            return detail2::split<detail2::Split::threshold, E>(flow_direction, inflow, threshold);
+        }
+
+        template<std::integral FlowDirection, std::floating_point E>
+        auto accu_threshold(PartitionedArray<FlowDirection, 2> const& flow_direction, Scalar<E> const& inflow,
+                     PartitionedArray<E, 2> const& threshold)
+        {
+            return detail2::split<detail2::Split::threshold, E>(flow_direction, inflow, threshold);
         }
 
         // ALG/value_policies/accu_fraction.hpp: the same four combinations
         template<std::integral FlowDirection, std::floating_point E>
-        auto accu_fraction(PartitionedArray<FlowDirection, 2> const& flow_direction,
-                           PartitionedArray<E, 2> const& inflow, PartitionedArray<E, 2> const& fraction)
+        auto accu_fraction(PartitionedArray<FlowDirection, 2> const& flow_direction, PartitionedArray<E, 2> const& inflow,
+                     PartitionedArray<E, 2> const& fraction)
         {
-            return detail2::split<false, E>(flow_direction, inflow, fraction);
+            return detail2::split<detail2::Split::fraction, E>(flow_direction, inflow, fraction);
         }
 
         template<std::integral FlowDirection, std::floating_point E>
-        auto accu_fraction(PartitionedArray<FlowDirection, 2> const& flow_direction,
-                           PartitionedArray<E, 2> const& inflow, Scalar<E> const& fraction)
+        auto accu_fraction(PartitionedArray<FlowDirection, 2> const& flow_direction, PartitionedArray<E, 2> const& inflow,
+                     Scalar<E> const& fraction)
         {
-            return detail2::split<false, E>(flow_direction, inflow, fraction);
-        }
-
-        template<std::integral FlowDirection, std::floating_point E>
-        auto accu_fraction(PartitionedArray<FlowDirection, 2> const& flow_direction, Scalar<E> const& inflow,
-                           Scalar<E> const& fraction)
-        {
-            return detail2::split<false, E>(flow_direction, inflow, fraction);
+            return detail2::split<detail2::Split::fraction, E>(flow_direction, inflow, fraction);
         }
 
         template<std::integral FlowDirection, std::floating_point E>
         auto accu_fraction(PartitionedArray<FlowDirection, 2> const& flow_direction, Scalar<E> const& inflow,
-                           PartitionedArray<E, 2> const& fraction)
+                     Scalar<E> const& fraction)
         {
-            return detail2::split<false, E>(flow_direction, inflow, fraction);
+            return detail2::split<detail2::Split::fraction, E>(flow_direction, inflow, fraction);
+        }
+
+        template<std::integral FlowDirection, std::floating_point E>
+        auto accu_fraction(PartitionedArray<FlowDirection, 2> const& flow_direction, Scalar<E> const& inflow,
+                     PartitionedArray<E, 2> const& fraction)
+        {
+            return detail2::split<detail2::Split::fraction, E>(flow_direction, inflow, fraction);
+        }
+
+        // ALG/value_policies/accu_capacity.hpp:34-88: the same four combinations
+        template<std::integral FlowDirection, std::floating_point E>
+        auto accu_capacity(PartitionedArray<FlowDirection, 2> const& flow_direction, PartitionedArray<E, 2> const& inflow,
+                     PartitionedArray<E, 2> const& capacity)
+        {
+            return detail2::split<detail2::Split::capacity, E>(flow_direction, inflow, capacity);
+        }
+
+        template<std::integral FlowDirection, std::floating_point E>
+        auto accu_capacity(PartitionedArray<FlowDirection, 2> const& flow_direction, PartitionedArray<E, 2> const& inflow,
+                     Scalar<E> const& capacity)
+        {
+            return detail2::split<detail2::Split::capacity, E>(flow_direction, inflow, capacity);
+        }
+
+        template<std::integral FlowDirection, std::floating_point E>
+        auto accu_capacity(PartitionedArray<FlowDirection, 2> const& flow_direction, Scalar<E> const& inflow,
+                     Scalar<E> const& capacity)
+        {
+            return detail2::split<detail2::Split::capacity, E>(flow_direction, inflow, capacity);
+        }
+
+        template<std::integral FlowDirection, std::floating_point E>
+        auto accu_capacity(PartitionedArray<FlowDirection, 2> const& flow_direction, Scalar<E> const& inflow,
+                     PartitionedArray<E, 2> const& capacity)
+        {
+            return detail2::split<detail2::Split::capacity, E>(flow_direction, inflow, capacity);
+        }
+
+        // ALG/value_policies/accu_trigger.hpp:34-88: the same four combinations
+        template<std::integral FlowDirection, std::floating_point E>
+        auto accu_trigger(PartitionedArray<FlowDirection, 2> const& flow_direction, PartitionedArray<E, 2> const& inflow,
+                     PartitionedArray<E, 2> const& trigger)
+        {
+            return detail2::split<detail2::Split::trigger, E>(flow_direction, inflow, trigger);
+        }
+
+        template<std::integral FlowDirection, std::floating_point E>
+        auto accu_trigger(PartitionedArray<FlowDirection, 2> const& flow_direction, PartitionedArray<E, 2> const& inflow,
+                     Scalar<E> const& trigger)
+        {
+            return detail2::split<detail2::Split::trigger, E>(flow_direction, inflow, trigger);
+        }
+
+        template<std::integral FlowDirection, std::floating_point E>
+        auto accu_trigger(PartitionedArray<FlowDirection, 2> const& flow_direction, Scalar<E> const& inflow,
+                     Scalar<E> const& trigger)
+        {
+            return detail2::split<detail2::Split::trigger, E>(flow_direction, inflow, trigger);
+        }
+
+        template<std::integral FlowDirection, std::floating_point E>
+        auto accu_trigger(PartitionedArray<FlowDirection, 2> const& flow_direction, Scalar<E> const& inflow,
+                     PartitionedArray<E, 2> const& trigger)
+        {
+            return detail2::split<detail2::Split::trigger, E>(flow_direction, inflow, trigger);
         }
 
         namespace detail2 {

@@ -11,8 +11,10 @@
 //   - three-phase router ........................... definition/accumulating_router.hpp:525-599 (walk),
 //                                                    :866-882 (row-major ridge scan), :987-1010 (inter-partition)
 //   - cell functors ................................ definition/accu.hpp:74-153, definition/accu_threshold.hpp:124-230,
-//                                                    definition/accu_fraction.hpp:126-190, definition/kinematic_wave.hpp:416-515
-//   - value-policy domains ......................... value_policies/{accu,accu_threshold,accu_fraction,kinematic_wave}.hpp
+//                                                    definition/accu_fraction.hpp:126-190, definition/kinematic_wave.hpp:416-515,
+//                                                    definition/accu_capacity.hpp:124-166, definition/accu_trigger.hpp:124-164
+//   - value-policy domains ......................... value_policies/{accu,accu_threshold,accu_fraction,kinematic_wave,
+//                                                    accu_capacity,accu_trigger}.hpp
 //   - kinematic-wave Newton solve .................. definition/kinematic_wave.hpp:13-254
 //
 // Arithmetic order. Inside one partition the order of floating point operations is LUE's: cells are
@@ -346,22 +348,27 @@ namespace {
         return std::numeric_limits<T>::quiet_NaN();
     }
 
-    enum class Kind { accu, threshold, fraction, kinematic_wave };
+    enum class Kind { accu, threshold, fraction, kinematic_wave, capacity, trigger };
+
+    constexpr bool two_outputs(Kind kind)
+    {
+        return kind == Kind::threshold || kind == Kind::fraction || kind == Kind::capacity || kind == Kind::trigger;
+    }
 
     template<typename T, Kind kind>
     struct Cells
     {
         Input<T> in1;  // material | current outflow
-        Input<T> in2;  // threshold | fraction | lateral inflow
+        Input<T> in2;  // threshold | fraction | capacity | trigger | lateral inflow
         Input<T> alpha, beta, dx;
         T dt;
         T* out;  // outflow; holds the summed upstream outflow until the cell is entered
-        T* rem;  // remainder (threshold, fraction)
+        T* rem;  // remainder (threshold, fraction, capacity, trigger)
 
         void mark_no_data(Index i)
         {
             out[i] = nan_value<T>();
-            if constexpr (kind == Kind::threshold || kind == Kind::fraction)
+            if constexpr (two_outputs(kind))
             {
                 rem[i] = nan_value<T>();
             }
@@ -441,6 +448,48 @@ namespace {
                     T const delta = frac * out[i];
                     rem[i] = out[i] - delta;
                     out[i] = delta;
+                }
+            }
+            else if constexpr (kind == Kind::capacity)
+            {
+                // accu_capacity.hpp:124-166; domain inflow >= 0 and capacity >= 0
+                // (value_policies/accu_capacity.hpp:9-21)
+                T const ext = in1(i);
+                T const cap = in2(i);
+                if (is_nan(ext) || is_nan(cap) || !(ext >= 0 && cap >= 0))
+                {
+                    mark_no_data(i);
+                }
+                else
+                {
+                    out[i] += ext;
+                    rem[i] = 0;
+                    if (out[i] > cap)
+                    {
+                        rem[i] = out[i] - cap;
+                        out[i] = cap;
+                    }
+                }
+            }
+            else if constexpr (kind == Kind::trigger)
+            {
+                // accu_trigger.hpp:124-164; domain inflow >= 0 and trigger >= 0
+                // (value_policies/accu_trigger.hpp)
+                T const ext = in1(i);
+                T const trg = in2(i);
+                if (is_nan(ext) || is_nan(trg) || !(ext >= 0 && trg >= 0))
+                {
+                    mark_no_data(i);
+                }
+                else
+                {
+                    out[i] += ext;
+                    rem[i] = 0;
+                    if (out[i] < trg)
+                    {
+                        rem[i] = out[i];
+                        out[i] = 0;
+                    }
                 }
             }
             else
@@ -677,6 +726,32 @@ int oracle_inflow_count(
         Cells<T, Kind::fraction> cells{input(inflow, inflow_scalar), input(fraction, fraction_scalar),       \
                                        input<T>(nullptr, 0), input<T>(nullptr, 0), input<T>(nullptr, 0),     \
                                        T{0}, outflow, remainder};                                            \
+        return route<T>(g, ldd, cells, nthreads);                                                            \
+    }                                                                                                        \
+    int oracle_accu_capacity_##SUFFIX(                                                                       \
+        std::uint8_t const* ldd, Index rows, Index cols, Index prows, Index pcols,                           \
+        T const* inflow, T inflow_scalar, T const* capacity, T capacity_scalar, T* outflow, T* remainder,     \
+        int nthreads)                                                                                        \
+    {                                                                                                        \
+        Grid g;                                                                                              \
+        if (!make_grid(rows, cols, prows, pcols, g) || !ldd || !outflow || !remainder)                       \
+            return err_invalid_argument;                                                                     \
+        Cells<T, Kind::capacity> cells{input(inflow, inflow_scalar), input(capacity, capacity_scalar),       \
+                                       input<T>(nullptr, 0), input<T>(nullptr, 0), input<T>(nullptr, 0),     \
+                                       T{0}, outflow, remainder};                                            \
+        return route<T>(g, ldd, cells, nthreads);                                                            \
+    }                                                                                                        \
+    int oracle_accu_trigger_##SUFFIX(                                                                        \
+        std::uint8_t const* ldd, Index rows, Index cols, Index prows, Index pcols,                           \
+        T const* inflow, T inflow_scalar, T const* trigger, T trigger_scalar, T* outflow, T* remainder,       \
+        int nthreads)                                                                                        \
+    {                                                                                                        \
+        Grid g;                                                                                              \
+        if (!make_grid(rows, cols, prows, pcols, g) || !ldd || !outflow || !remainder)                       \
+            return err_invalid_argument;                                                                     \
+        Cells<T, Kind::trigger> cells{input(inflow, inflow_scalar), input(trigger, trigger_scalar),          \
+                                      input<T>(nullptr, 0), input<T>(nullptr, 0), input<T>(nullptr, 0),      \
+                                      T{0}, outflow, remainder};                                             \
         return route<T>(g, ldd, cells, nthreads);                                                            \
     }                                                                                                        \
     int oracle_kinematic_wave_##SUFFIX(                                                                      \

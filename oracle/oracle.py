@@ -44,7 +44,7 @@ def _load():
     ci = ctypes.c_int
     for suffix, ct in (("f32", ctypes.c_float), ("f64", ctypes.c_double)):
         getattr(lib, f"oracle_accu_{suffix}").argtypes = [p, i64, i64, i64, i64, p, ct, p, ci]
-        for op in ("threshold", "fraction"):
+        for op in ("threshold", "fraction", "capacity", "trigger"):
             getattr(lib, f"oracle_accu_{op}_{suffix}").argtypes = [p, i64, i64, i64, i64, p, ct, p, ct, p, p, ci]
         getattr(lib, f"oracle_kinematic_wave_{suffix}").argtypes = [
             p, i64, i64, i64, i64, p, p, p, ct, p, ct, ct, p, ct, p, ci]
@@ -140,6 +140,14 @@ def accu_threshold(ldd, inflow, threshold, partition_shape=None, dtype=np.float3
 
 def accu_fraction(ldd, inflow, fraction, partition_shape=None, dtype=np.float32, nthreads=None):
     return _accu2("fraction", ldd, inflow, fraction, partition_shape, dtype, nthreads)
+
+
+def accu_capacity(ldd, inflow, capacity, partition_shape=None, dtype=np.float32, nthreads=None):
+    return _accu2("capacity", ldd, inflow, capacity, partition_shape, dtype, nthreads)
+
+
+def accu_trigger(ldd, inflow, trigger, partition_shape=None, dtype=np.float32, nthreads=None):
+    return _accu2("trigger", ldd, inflow, trigger, partition_shape, dtype, nthreads)
 
 
 def kinematic_wave(ldd, current_outflow, inflow, alpha, beta, time_step_duration, channel_length,

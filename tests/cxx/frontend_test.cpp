@@ -92,11 +92,18 @@ namespace {
             auto out = c.args[0].raster ? value_policies::accu(ldd, raster(0)) : value_policies::accu(ldd, scalar(0));
             return compare(c, out, c.expected[0]);
         }
-        if (kind == "accu_threshold" || kind == "accu_fraction")
+        if (kind == "accu_threshold" || kind == "accu_fraction" || kind == "accu_capacity" || kind == "accu_trigger")
         {
-            bool const thr = kind == "accu_threshold";
             auto call = [&](auto const& a, auto const& b)
-            { return thr ? value_policies::accu_threshold(ldd, a, b) : value_policies::accu_fraction(ldd, a, b); };
+            {
+                if (kind == "accu_threshold")
+                    return value_policies::accu_threshold(ldd, a, b);
+                if (kind == "accu_fraction")
+                    return value_policies::accu_fraction(ldd, a, b);
+                if (kind == "accu_capacity")
+                    return value_policies::accu_capacity(ldd, a, b);
+                return value_policies::accu_trigger(ldd, a, b);
+            };
             std::tuple<PartitionedArray<E, 2>, PartitionedArray<E, 2>> result =
                 c.args[0].raster ? (c.args[1].raster ? call(raster(0), raster(1)) : call(raster(0), scalar(1)))
                                  : (c.args[1].raster ? call(scalar(0), raster(1)) : call(scalar(0), scalar(1)));

@@ -14,6 +14,8 @@ Files read (relative to the reference root):
     source/framework/algorithm/test/accu_test.cpp
     source/framework/algorithm/test/accu_threshold_test.cpp
     source/framework/algorithm/test/accu_fraction_test.cpp
+    source/framework/algorithm/test/accu_capacity_test.cpp
+    source/framework/algorithm/test/accu_trigger_test.cpp
     source/framework/algorithm/test/kinematic_wave_test.cpp
     source/framework/algorithm/test/inflow_count_test.cpp
 
@@ -396,6 +398,8 @@ def extract(ref_root):
         "accu": "accu_test.cpp",
         "accu_threshold": "accu_threshold_test.cpp",
         "accu_fraction": "accu_fraction_test.cpp",
+        "accu_capacity": "accu_capacity_test.cpp",
+        "accu_trigger": "accu_trigger_test.cpp",
         "kinematic_wave": "kinematic_wave_test.cpp",
     }
     for op, fname in routing_files.items():
@@ -432,6 +436,8 @@ def extract(ref_root):
                 "accu": ["inflow"],
                 "accu_threshold": ["inflow", "threshold"],
                 "accu_fraction": ["inflow", "fraction"],
+                "accu_capacity": ["inflow", "capacity"],
+                "accu_trigger": ["inflow", "trigger"],
                 "kinematic_wave": ["current_outflow", "inflow", "alpha", "beta", "time_step_duration",
                                    "channel_length"],
             }[opname]

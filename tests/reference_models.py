@@ -134,7 +134,8 @@ def _value(x, r, c):
 
 
 def route_split(ldd, material, param, kind):
-    """accu_threshold / accu_fraction over a topological order; returns (outflow, remainder)."""
+    """accu_threshold / accu_fraction / accu_capacity / accu_trigger over a topological order; returns
+    (outflow, remainder)."""
     rows, cols = ldd.shape
     inflow = np.zeros((rows, cols))
     out = np.full((rows, cols), np.nan)
@@ -150,6 +151,10 @@ def route_split(ldd, material, param, kind):
             total = inflow[r, c] + m
             if kind == "threshold":
                 o, rm = (total - p, p) if total > p else (0.0, total)
+            elif kind == "capacity":
+                o, rm = (p, total - p) if total > p else (total, 0.0)
+            elif kind == "trigger":
+                o, rm = (total, 0.0) if not total < p else (0.0, total)
             else:
                 o, rm = p * total, total - p * total
             out[r, c], rem[r, c] = o, rm

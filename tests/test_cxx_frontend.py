@@ -23,7 +23,8 @@ def _vec(raster):
 
 def write_cases(path):
     lines = ["std::vector<Case> const& cases() {", "static std::vector<Case> const all{"]
-    for op in ("inflow_count", "accu", "accu_threshold", "accu_fraction", "kinematic_wave"):
+    for op in ("inflow_count", "accu", "accu_threshold", "accu_fraction", "accu_capacity", "accu_trigger",
+               "kinematic_wave"):
         for c in load_golden(op):
             if c["kind"] == "iterate_to_new_discharge":
                 continue

@@ -136,3 +136,32 @@ def test_network_rejects_cycles(fw, schedule):
     ldd[50, 50], ldd[50, 51], ldd[51, 51], ldd[51, 50] = 6, 2, 4, 8
     with pytest.raises(_lib.InvalidNetworkError):
         fw.FlowNetwork(pa(fw, ldd), schedule=schedule)
+
+
+@pytest.mark.parametrize("kind", ["capacity", "trigger"])
+@pytest.mark.parametrize("schedule", ["chains", "rounds"])
+def test_network_capacity_trigger(fw, oracle_mod, kind, schedule):
+    """accu_capacity / accu_trigger on recorded networks (a batch of two catchments), FP32 and FP64, against
+    the oracle; the batch equals each catchment alone."""
+    cs = [catchment(400 + i, *s) for i, s in enumerate([(200, 170), (96, 260)])]
+    ldds = [pa(fw, c["ldd"]) for c in cs]
+    batch = fw.FlowNetwork(ldds, schedule=schedule)
+    for dtype in (np.float32, np.float64):
+        mats = [pa(fw, c["material"].astype(dtype)) for c in cs]
+        params = [pa(fw, (c["threshold"] * 4.0).astype(dtype)) for c in cs]  # capacity / trigger U[0, 2)
+        outs, rems = getattr(batch, f"accu_{kind}")(mats, params)
+        torch.cuda.synchronize()
+        for i, c in enumerate(cs):
+            alone = fw.FlowNetwork(ldds[i], schedule=schedule)
+            o, r = getattr(alone, f"accu_{kind}")(mats[i], params[i])
+            torch.cuda.synchronize()
+            assert np.array_equal(o.numpy(), outs[i].numpy(), equal_nan=True)
+            assert np.array_equal(r.numpy(), rems[i].numpy(), equal_nan=True)
+            w_out, w_rem = getattr(oracle_mod, f"accu_{kind}")(
+                c["ldd"], c["material"].astype(dtype).astype(np.float64),
+                (c["threshold"] * 4.0).astype(dtype).astype(np.float64), None, np.float64)
+            scale = np.abs(w_out) + np.abs(w_rem) + 1e-30
+            tol = 1e-6 if dtype == np.float32 else 1e-12
+            assert (np.isnan(outs[i].numpy()) == np.isnan(w_out)).all()
+            assert np.nanmax(np.abs(outs[i].numpy() - w_out) / scale) <= tol
+            assert np.nanmax(np.abs(rems[i].numpy() - w_rem) / scale) <= tol

@@ -37,8 +37,12 @@ def test_inflow_count_golden(fw, case):
     assert_equal_nan(got, raster(case["expected"]["count"]["raster"], np.uint8))
 
 
-ROUTING_CASES = [c for op in ("accu", "accu_threshold", "accu_fraction", "kinematic_wave")
+ROUTING_CASES = [c for op in ("accu", "accu_threshold", "accu_fraction", "accu_capacity", "accu_trigger",
+                              "kinematic_wave")
                  for c in load_golden(op) if c["kind"] != "iterate_to_new_discharge"]
+# second argument of the two-output operations
+SPLIT_SECOND = {"accu_threshold": "threshold", "accu_fraction": "fraction", "accu_capacity": "capacity",
+                "accu_trigger": "trigger"}
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
@@ -50,11 +54,8 @@ def test_routing_golden(fw, case, dtype):
     kind = case["kind"]
     if kind == "accu":
         got = {"outflow": fw.accu(ldd, args["inflow"])}
-    elif kind == "accu_threshold":
-        out, rem = fw.accu_threshold(ldd, args["inflow"], args["threshold"])
-        got = {"outflow": out, "remainder": rem}
-    elif kind == "accu_fraction":
-        out, rem = fw.accu_fraction(ldd, args["inflow"], args["fraction"])
+    elif kind in SPLIT_SECOND:
+        out, rem = getattr(fw, kind)(ldd, args["inflow"], args[SPLIT_SECOND[kind]])
         got = {"outflow": out, "remainder": rem}
     else:
         shape = ldd.shape
@@ -130,17 +131,17 @@ def test_accu_random(fw, oracle_mod, seed, shape, pshape):
     assert_equal_nan(ones, rm.accu_by_walking(ldd, 1.0).astype(np.float32))
 
 
-@pytest.mark.parametrize("kind", ["threshold", "fraction"])
+@pytest.mark.parametrize("kind", ["threshold", "fraction", "capacity", "trigger"])
 @pytest.mark.parametrize("seed", range(2))
 @pytest.mark.parametrize("shape,pshape", SHAPES)
 def test_split_random(fw, oracle_mod, kind, seed, shape, pshape):
     rng = np.random.default_rng(20 + seed)
     ldd = rm.random_ldd(rng, *shape, nodata_fraction=0.03, flat_fraction=0.3)
     material = rng.random(shape)
-    param = rng.random(shape) * (0.5 if kind == "threshold" else 1.0)
+    param = rng.random(shape) * {"threshold": 0.5, "fraction": 1.0, "capacity": 3.0, "trigger": 2.0}[kind]
     param[rng.random(shape) < 0.01] = np.nan
-    fn = fw.accu_threshold if kind == "threshold" else fw.accu_fraction
-    ofn = oracle_mod.accu_threshold if kind == "threshold" else oracle_mod.accu_fraction
+    fn = getattr(fw, f"accu_{kind}")
+    ofn = getattr(oracle_mod, f"accu_{kind}")
     want_out, want_rem = ofn(ldd, material, param, pshape, np.float64)
     out, rem = fn(pa(fw, ldd, pshape), pa(fw, material, pshape), pa(fw, param, pshape))
     # out = total - threshold cancels: compare against the magnitude of the total material

@@ -18,6 +18,11 @@ def test_inflow_count_golden(oracle_mod, case):
     assert_equal_nan(got, raster(case["expected"]["count"]["raster"], np.uint8))
 
 
+# second argument of the two-output operations
+SPLIT_SECOND = {"accu_threshold": "threshold", "accu_fraction": "fraction", "accu_capacity": "capacity",
+                "accu_trigger": "trigger"}
+
+
 def _run_golden(oracle_mod, case, dtype):
     ldd = raster(case["ldd"], np.uint8)
     args = {k: golden_arg(v, dtype) for k, v in case["args"].items()}
@@ -25,9 +30,9 @@ def _run_golden(oracle_mod, case, dtype):
     kind = case["kind"]
     if kind == "accu":
         return {"outflow": oracle_mod.accu(ldd, args["inflow"], ps, dtype)}
-    if kind in ("accu_threshold", "accu_fraction"):
+    if kind in SPLIT_SECOND:
         fn = getattr(oracle_mod, kind)
-        out, rem = fn(ldd, args["inflow"], args["threshold" if kind == "accu_threshold" else "fraction"], ps, dtype)
+        out, rem = fn(ldd, args["inflow"], args[SPLIT_SECOND[kind]], ps, dtype)
         return {"outflow": out, "remainder": rem}
     if kind == "kinematic_wave":
         shape = ldd.shape
@@ -41,7 +46,8 @@ def _run_golden(oracle_mod, case, dtype):
     raise AssertionError(kind)
 
 
-ROUTING_CASES = [c for op in ("accu", "accu_threshold", "accu_fraction", "kinematic_wave")
+ROUTING_CASES = [c for op in ("accu", "accu_threshold", "accu_fraction", "accu_capacity", "accu_trigger",
+                              "kinematic_wave")
                  for c in load_golden(op) if c["kind"] != "iterate_to_new_discharge"]
 
 
@@ -101,17 +107,17 @@ def test_accu_brute_force(oracle_mod, seed, shape, pshape):
     assert_equal_nan(oracle_mod.accu(ldd, 1.0, pshape, np.float64), ones)
 
 
-@pytest.mark.parametrize("kind", ["threshold", "fraction"])
+@pytest.mark.parametrize("kind", ["threshold", "fraction", "capacity", "trigger"])
 @pytest.mark.parametrize("seed", range(3))
 @pytest.mark.parametrize("shape,pshape", SHAPES)
 def test_split_brute_force(oracle_mod, kind, seed, shape, pshape):
     rng = np.random.default_rng(200 + seed)
     ldd = rm.random_ldd(rng, *shape, nodata_fraction=0.05, flat_fraction=0.3)
     material = rng.random(shape)
-    param = rng.random(shape) * (0.5 if kind == "threshold" else 1.0)
+    param = rng.random(shape) * {"threshold": 0.5, "fraction": 1.0, "capacity": 3.0, "trigger": 2.0}[kind]
     param[rng.random(shape) < 0.03] = np.nan
     want_out, want_rem = rm.route_split(ldd, material, param, kind)
-    fn = oracle_mod.accu_threshold if kind == "threshold" else oracle_mod.accu_fraction
+    fn = getattr(oracle_mod, f"accu_{kind}")
     out, rem = fn(ldd, material, param, pshape, np.float64)
     np.testing.assert_allclose(out, want_out, rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(rem, want_rem, rtol=1e-12, atol=1e-12)
