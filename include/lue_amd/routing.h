@@ -99,6 +99,19 @@ int lue_amd_stream_synchronize(void* stream);
 int lue_amd_inflow_count_u8(const uint8_t* ldd, int64_t rows, int64_t cols, uint8_t* inflow_count, void* stream);
 
 /*
+ * d8_flow_direction: the ldd of an elevation raster (NaN = no-data -> 255). Each cell drains to the
+ * neighbour with the largest raw drop (no distance weighting; first in the order N, W, E, S, NW, NE, SW,
+ * SE on ties; sink when no drop is positive); outside the array the elevation is numeric_limits<T>::min().
+ * Replaces lue::value_policies::d8_flow_direction<uint8_t>(elevation)
+ * (ALG/value_policies/d8_flow_direction.hpp:9-57, ALG/definition/d8_flow_direction.hpp:12-173).
+ * Asynchronous on `stream`.
+ */
+int lue_amd_d8_flow_direction_f32(const float* elevation, int64_t rows, int64_t cols, uint8_t* flow_direction,
+                                  void* stream);
+int lue_amd_d8_flow_direction_f64(const double* elevation, int64_t rows, int64_t cols, uint8_t* flow_direction,
+                                  void* stream);
+
+/*
  * accu: outflow = sum of upstream outflow + inflow.
  * Replaces lue::value_policies::accu(flow_direction, inflow | Scalar inflow)
  * (ALG/value_policies/accu.hpp:33-55, ALG/accu.hpp:12-28, ALG/definition/accu.hpp:226-264).
@@ -199,6 +212,9 @@ int lue_amd_iterate_to_new_discharge_f64(const double* upstream_discharge, const
  * generator returns when its output is written; the uniform generators return at once (stream order).
  */
 int lue_amd_synthetic_ldd(uint64_t seed, int64_t rows, int64_t cols, uint8_t* ldd, void* stream);
+/* the DEM e(r, c) itself, rounded to the output type (stream order) */
+int lue_amd_synthetic_dem_f32(uint64_t seed, int64_t rows, int64_t cols, float* out, void* stream);
+int lue_amd_synthetic_dem_f64(uint64_t seed, int64_t rows, int64_t cols, double* out, void* stream);
 int lue_amd_synthetic_uniform_f32(uint64_t seed, uint64_t stream_id, double lo, double hi, int64_t n, float* out,
                                   void* stream);
 int lue_amd_synthetic_uniform_f64(uint64_t seed, uint64_t stream_id, double lo, double hi, int64_t n, double* out,

@@ -52,6 +52,8 @@ SIGNATURES = {
     "lue_amd_copy_to_host": [c_p, c_p, c_size, c_p],
     "lue_amd_stream_synchronize": [c_p],
     "lue_amd_inflow_count_u8": [c_p, c_i64, c_i64, c_p, c_p],
+    "lue_amd_d8_flow_direction_f32": [c_p, c_i64, c_i64, c_p, c_p],
+    "lue_amd_d8_flow_direction_f64": [c_p, c_i64, c_i64, c_p, c_p],
     "lue_amd_accu_f32": [c_p, c_i64, c_i64, c_p, c_f, c_p, c_p],
     "lue_amd_accu_f64": [c_p, c_i64, c_i64, c_p, c_d, c_p, c_p],
     "lue_amd_accu_threshold_f32": [c_p, c_i64, c_i64, c_p, c_f, c_p, c_f, c_p, c_p, c_p],
@@ -67,6 +69,8 @@ SIGNATURES = {
     "lue_amd_iterate_to_new_discharge_f32": [c_p] * 8 + [c_i64, c_p],
     "lue_amd_iterate_to_new_discharge_f64": [c_p] * 8 + [c_i64, c_p],
     "lue_amd_synthetic_ldd": [c_u64, c_i64, c_i64, c_p, c_p],
+    "lue_amd_synthetic_dem_f32": [c_u64, c_i64, c_i64, c_p, c_p],
+    "lue_amd_synthetic_dem_f64": [c_u64, c_i64, c_i64, c_p, c_p],
     "lue_amd_synthetic_uniform_f32": [c_u64, c_u64, c_d, c_d, c_i64, c_p, c_p],
     "lue_amd_synthetic_uniform_f64": [c_u64, c_u64, c_d, c_d, c_i64, c_p, c_p],
 }

@@ -2,6 +2,7 @@
 // round loop of the tiled sweep, and the synthetic-input generators.
 #include "lue_amd/routing.h"
 
+#include "d8_flow_direction.hpp"
 #include "inflow_count.hpp"
 #include "kinematic_wave.hpp"
 #include "routing_common.hpp"
@@ -385,6 +386,49 @@ namespace lue_amd {
 
 using namespace lue_amd;
 
+namespace {
+
+    template<typename T>
+    int dem_impl(std::uint64_t seed, std::int64_t rows, std::int64_t cols, T* out, void* stream)
+    {
+        if (rows <= 0 || cols <= 0 || !out)
+            return fail(status_invalid_argument, "bad shape or null output");
+        hipStream_t const s = static_cast<hipStream_t>(stream);
+        double table[256];
+        for (int k = 0; k < 256; ++k)
+            table[k] = std::fabs(std::sin(M_PI * static_cast<double>(k) / 256.0));
+        ScratchArray<double> d_table;
+        LUE_AMD_HIP(d_table.allocate(256, s));
+        LUE_AMD_HIP(host_staging().copy_to_device(d_table.ptr, table, sizeof(table), s));
+        unsigned const grid = static_cast<unsigned>(std::min<std::int64_t>((rows * cols + 255) / 256, 16384));
+        hipLaunchKernelGGL(dem_kernel<T>, dim3(grid), dim3(256), 0, s, seed, rows, cols, d_table.ptr, out);
+        LUE_AMD_HIP(hipGetLastError());
+        return status_success;
+    }
+
+    template<typename T>
+    int d8_impl(T const* elevation, std::int64_t rows, std::int64_t cols, std::uint8_t* flow_direction,
+                void* stream)
+    {
+        if (rows < 0 || cols < 0)
+            return fail(status_invalid_argument, "array shape must not be negative");
+        if (rows == 0 || cols == 0)
+            return status_success;
+        if (!elevation || !flow_direction)
+            return fail(status_invalid_argument, "elevation and flow_direction must not be null");
+        std::int64_t const gx = (cols + d8_tw - 1) / d8_tw;
+        std::int64_t const gy = (rows + d8_th - 1) / d8_th;
+        if (gx > 0x7fffffff || gy > 0xffff)
+            return fail(status_invalid_argument, "array too large for one launch");
+        hipLaunchKernelGGL(d8_flow_direction_kernel<T>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy)),
+                           dim3(d8_threads), 0, static_cast<hipStream_t>(stream), elevation, rows, cols,
+                           flow_direction);
+        LUE_AMD_HIP(hipGetLastError());
+        return status_success;
+    }
+
+}  // namespace
+
 extern "C" {
 
 const char* lue_amd_status_string(int status)
@@ -505,6 +549,18 @@ int lue_amd_inflow_count_u8(const uint8_t* ldd, int64_t rows, int64_t cols, uint
     LUE_AMD_HIP(hipGetLastError());
     LUE_AMD_HIP(hipStreamSynchronize(s));
     return status_success;
+}
+
+int lue_amd_d8_flow_direction_f32(const float* elevation, int64_t rows, int64_t cols, uint8_t* flow_direction,
+                                  void* stream)
+{
+    return d8_impl<float>(elevation, rows, cols, flow_direction, stream);
+}
+
+int lue_amd_d8_flow_direction_f64(const double* elevation, int64_t rows, int64_t cols, uint8_t* flow_direction,
+                                  void* stream)
+{
+    return d8_impl<double>(elevation, rows, cols, flow_direction, stream);
 }
 
 int lue_amd_accu_f32(const uint8_t* ldd, int64_t rows, int64_t cols, const float* inflow, float inflow_scalar,
@@ -638,6 +694,16 @@ int lue_amd_synthetic_ldd(uint64_t seed, int64_t rows, int64_t cols, uint8_t* ld
     LUE_AMD_HIP(hipFreeAsync(d_table, s));
     LUE_AMD_HIP(hipStreamSynchronize(s));
     return status_success;
+}
+
+int lue_amd_synthetic_dem_f32(uint64_t seed, int64_t rows, int64_t cols, float* out, void* stream)
+{
+    return dem_impl<float>(seed, rows, cols, out, stream);
+}
+
+int lue_amd_synthetic_dem_f64(uint64_t seed, int64_t rows, int64_t cols, double* out, void* stream)
+{
+    return dem_impl<double>(seed, rows, cols, out, stream);
 }
 
 int lue_amd_synthetic_uniform_f32(uint64_t seed, uint64_t stream_id, double lo, double hi, int64_t n, float* out,

@@ -106,6 +106,18 @@ namespace lue_amd {
         }
     }
 
+    // The synthetic DEM itself (dem_value), rounded to T.
+    template<typename T>
+    __global__ void __launch_bounds__(256) dem_kernel(
+        std::uint64_t seed, std::int64_t rows, std::int64_t cols, double const* __restrict__ sin_table,
+        T* __restrict__ out)
+    {
+        std::int64_t const n = rows * cols;
+        std::int64_t const stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+        for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+            out[i] = static_cast<T>(dem_value(seed, rows, cols, i / cols, i % cols, sin_table));
+    }
+
     // lo + (hi - lo) u, rounded to T
     template<typename T>
     __global__ void __launch_bounds__(256) uniform_kernel(

@@ -142,6 +142,23 @@ def inflow_count(flow_direction):
     return out
 
 
+def d8_flow_direction(elevation):
+    """lue.framework.d8_flow_direction(elevation) -> uint8 ldd with the elevation's partitioning
+    (value_policies/d8_flow_direction.hpp:46-55, definition/d8_flow_direction.hpp:12-173). NaN elevation is
+    no-data."""
+    if not isinstance(elevation, PartitionedArray):
+        raise TypeError("elevation must be a PartitionedArray")
+    dtype = elevation.data.dtype
+    if dtype not in FLOAT_TYPES:
+        raise TypeError(f"unsupported elevation element type {dtype}")
+    lib = _lib.load()
+    out = _output(elevation, torch.uint8)
+    rows, cols = elevation.shape
+    fn = getattr(lib, f"lue_amd_d8_flow_direction_{FLOAT_TYPES[dtype]}")
+    _lib.check(fn(_ptr(elevation.data), rows, cols, _ptr(out.data), _stream()))
+    return out
+
+
 def accu(flow_direction, inflow):
     """lue.framework.accu(flow_direction, inflow) -> outflow (value_policies/accu.hpp:33-55)."""
     _check_ldd(flow_direction)

@@ -8,8 +8,9 @@
 //     raster in HBM;
 //   - lue::Scalar<E>  (source/framework/algorithm/include/lue/framework/algorithm/scalar.hpp:20);
 //   - lue::create_partitioned_array(shape, partition_shape, fill_value);
-//   - lue::value_policies::{accu, accu_threshold, accu_fraction, accu_capacity, accu_trigger, kinematic_wave} with the reference's
-//     overload sets (ALG/value_policies/*.hpp) and lue::inflow_count<std::uint8_t>(ldd)
+//   - lue::value_policies::{accu, accu_threshold, accu_fraction, accu_capacity, accu_trigger, kinematic_wave,
+//     d8_flow_direction} with the reference's overload sets (ALG/value_policies/*.hpp) and
+//     lue::inflow_count<std::uint8_t>(ldd)
 //     (ALG/inflow_count.hpp:29-42), ALG = source/framework/algorithm/include/lue/framework/algorithm.
 // Each operation calls one function of the C ABI (include/lue_amd/routing.h). Shape and partitioning
 // mismatches throw std::runtime_error as detail::verify_compatible does (ALG/detail/verify_compatible.hpp:
@@ -296,6 +297,22 @@ namespace lue {
     }
 
     namespace value_policies {
+
+        // lue::value_policies::d8_flow_direction<FlowDirection>(elevation)
+        // (ALG/value_policies/d8_flow_direction.hpp:46-55): the ldd of an elevation raster, NaN = no-data
+        template<std::integral FlowDirection, std::floating_point E>
+            requires std::same_as<FlowDirection, FlowDirectionElement>
+        auto d8_flow_direction(PartitionedArray<E, 2> const& elevation) -> PartitionedArray<FlowDirection, 2>
+        {
+            PartitionedArray<FlowDirection, 2> result{elevation.shape(), elevation.partition_shape()};
+            auto const [rows, cols] = elevation.shape();
+            if constexpr (std::is_same_v<E, float>)
+                detail::check(lue_amd_d8_flow_direction_f32(elevation.data(), rows, cols, result.data(), nullptr));
+            else
+                detail::check(lue_amd_d8_flow_direction_f64(elevation.data(), rows, cols, result.data(), nullptr));
+            detail::check(lue_amd_stream_synchronize(nullptr));
+            return result;
+        }
 
         // ALG/value_policies/accu.hpp:33-55
         template<std::integral FlowDirection, std::floating_point E, typename Inflow>

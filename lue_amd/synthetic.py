@@ -106,6 +106,15 @@ def ldd_device(seed, rows, cols, device=None):
     return out
 
 
+def dem_device(seed, rows, cols, dtype=torch.float32, device=None):
+    """dem_host(seed, rows, cols) on the device, rounded to dtype (the same values)."""
+    lib = _lib.load()
+    out = torch.empty((rows, cols), dtype=dtype, device=device or "cuda")
+    fn = lib.lue_amd_synthetic_dem_f32 if dtype == torch.float32 else lib.lue_amd_synthetic_dem_f64
+    _lib.check(fn(seed, rows, cols, out.data_ptr(), _stream()))
+    return out
+
+
 def uniform_device(seed, stream, lo, hi, shape, dtype=torch.float32, device=None):
     lib = _lib.load()
     out = torch.empty(tuple(shape), dtype=dtype, device=device or "cuda")

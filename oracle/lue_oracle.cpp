@@ -16,6 +16,8 @@
 //   - value-policy domains ......................... value_policies/{accu,accu_threshold,accu_fraction,kinematic_wave,
 //                                                    accu_capacity,accu_trigger}.hpp
 //   - kinematic-wave Newton solve .................. definition/kinematic_wave.hpp:13-254
+//   - D8 flow direction ............................ definition/d8_flow_direction.hpp:12-143,
+//                                                    value_policies/d8_flow_direction.hpp:31-41 (halo value)
 //
 // Arithmetic order. Inside one partition the order of floating point operations is LUE's: cells are
 // scanned row-major, a walk starts at every ridge cell (inflow count 0), upstream outflow is added to a
@@ -660,6 +662,49 @@ namespace {
         return Input<T>{raster, scalar};
     }
 
+    // D8 flow direction of one cell from its 3x3 window: definition/d8_flow_direction.hpp:12-143. No-data
+    // (NaN) centre -> no-data ldd; else sink, then the neighbours in the order N, W, E, S, NW, NE, SW, SE,
+    // each taken when its raw drop (centre - neighbour, no distance weighting) exceeds the best so far
+    // (initially 0); no-data neighbours are skipped. Outside the array the window holds
+    // numeric_limits<T>::min() (value_policies/d8_flow_direction.hpp:31-41: FillHaloWithConstantValue).
+    template<typename T>
+    void d8_flow_direction(T const* elevation, Index rows, Index cols, std::uint8_t* out)
+    {
+        T const halo = std::numeric_limits<T>::min();
+        auto at = [&](Index r, Index c) -> T
+        { return (r < 0 || r >= rows || c < 0 || c >= cols) ? halo : elevation[r * cols + c]; };
+        // (row offset, column offset, ldd code) in LUE's order
+        static constexpr int order[8][3] = {{-1, 0, 8}, {0, -1, 4}, {0, 1, 6}, {1, 0, 2},
+                                            {-1, -1, 7}, {-1, 1, 9}, {1, -1, 1}, {1, 1, 3}};
+        for (Index r = 0; r < rows; ++r)
+        {
+            for (Index c = 0; c < cols; ++c)
+            {
+                T const centre = elevation[r * cols + c];
+                if (is_nan(centre))
+                {
+                    out[r * cols + c] = no_data_ldd;
+                    continue;
+                }
+                std::uint8_t direction = 5;
+                T drop{0};
+                for (auto const& o : order)
+                {
+                    T const neighbour = at(r + o[0], c + o[1]);
+                    if (is_nan(neighbour))
+                        continue;
+                    T const other_drop = centre - neighbour;
+                    if (other_drop > drop)
+                    {
+                        drop = other_drop;
+                        direction = static_cast<std::uint8_t>(o[2]);
+                    }
+                }
+                out[r * cols + c] = direction;
+            }
+        }
+    }
+
 }  // namespace
 
 
@@ -776,5 +821,21 @@ int oracle_inflow_count(
 
 ORACLE_ROUTING_OPS(float, f32)
 ORACLE_ROUTING_OPS(double, f64)
+
+int oracle_d8_flow_direction_f32(float const* elevation, Index rows, Index cols, std::uint8_t* out)
+{
+    if (rows < 0 || cols < 0 || (rows * cols > 0 && (!elevation || !out)))
+        return err_invalid_argument;
+    d8_flow_direction<float>(elevation, rows, cols, out);
+    return ok;
+}
+
+int oracle_d8_flow_direction_f64(double const* elevation, Index rows, Index cols, std::uint8_t* out)
+{
+    if (rows < 0 || cols < 0 || (rows * cols > 0 && (!elevation || !out)))
+        return err_invalid_argument;
+    d8_flow_direction<double>(elevation, rows, cols, out);
+    return ok;
+}
 
 }  // extern "C"

@@ -52,6 +52,8 @@ def _load():
         f.argtypes = [ct] * 7
         f.restype = ct
     lib.oracle_inflow_count.argtypes = [p, i64, i64, i64, i64, p, ci]
+    for suffix in ("f32", "f64"):
+        getattr(lib, f"oracle_d8_flow_direction_{suffix}").argtypes = [p, i64, i64, p]
     _lib = lib
     return lib
 
@@ -173,3 +175,14 @@ def iterate_to_new_discharge(upstream, current, lateral, alpha, beta, dt, dx, dt
     lib = _load()
     sfx, _ = _dtype_suffix(dtype)
     return getattr(lib, f"oracle_iterate_to_new_discharge_{sfx}")(upstream, current, lateral, alpha, beta, dt, dx)
+
+
+def d8_flow_direction(elevation, dtype=np.float32):
+    """LUE's d8_flow_direction (value policies): uint8 ldd from an elevation raster (NaN = no-data)."""
+    lib = _load()
+    sfx, _ = _dtype_suffix(dtype)
+    elevation = np.ascontiguousarray(elevation, dtype=dtype)
+    rows, cols = elevation.shape
+    out = np.empty((rows, cols), dtype=np.uint8)
+    _check(getattr(lib, f"oracle_d8_flow_direction_{sfx}")(elevation.ctypes.data, rows, cols, out.ctypes.data))
+    return out

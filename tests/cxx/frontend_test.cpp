@@ -27,7 +27,7 @@ namespace {
         char const* kind;
         Shape shape;
         Shape partition_shape;
-        std::vector<double> ldd;
+        std::vector<double> ldd;  // the elevation for d8_flow_direction
         std::vector<Arg> args;
         std::vector<std::vector<double>> expected;  // outflow [, remainder] | count
         bool close;                                  // Boost tolerance 1e-6 instead of exact
@@ -78,8 +78,14 @@ namespace {
     bool run(Case const& c)
     {
         using namespace lue;
-        auto const ldd = make_raster<std::uint8_t>(c, c.ldd);
         std::string const kind = c.kind;
+        if (kind == "d8_flow_direction")
+        {
+            // c.ldd holds the elevation here
+            auto const elevation = make_raster<E>(c, c.ldd);
+            return compare(c, value_policies::d8_flow_direction<std::uint8_t>(elevation), c.expected[0]);
+        }
+        auto const ldd = make_raster<std::uint8_t>(c, c.ldd);
         if (kind == "inflow_count")
         {
             auto const count = inflow_count<std::uint8_t>(ldd);

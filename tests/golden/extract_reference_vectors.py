@@ -18,6 +18,7 @@ Files read (relative to the reference root):
     source/framework/algorithm/test/accu_trigger_test.cpp
     source/framework/algorithm/test/kinematic_wave_test.cpp
     source/framework/algorithm/test/inflow_count_test.cpp
+    source/framework/algorithm/test/d8_flow_direction_test.cpp
 
 A partitioned literal lists one initializer per partition, partitions in row-major partition order
 (source/framework/test/include/lue/framework/test/array.hpp:76-105), each partition's cells row-major,
@@ -33,6 +34,9 @@ import re
 import sys
 
 LDD_NAMES = {"n": 8, "ne": 9, "e": 6, "se": 3, "s": 2, "sw": 1, "w": 4, "nw": 7, "p": 5, "nd": 255}
+# lue::north<FlowDirection> ... (source/framework/algorithm/include/lue/framework/algorithm/flow_direction.hpp)
+DIRECTION_NAMES = {"north": 8, "north_east": 9, "east": 6, "south_east": 3, "south": 2, "south_west": 1, "west": 4,
+                   "north_west": 7, "sink": 5}
 NAN = float("nan")
 
 TOKEN_RE = re.compile(r"""
@@ -202,6 +206,8 @@ class Parser:
             raise SyntaxError(member)
         if short in self.env:
             return self.env[short]
+        if name.startswith("lue::") and short in DIRECTION_NAMES:
+            return float(DIRECTION_NAMES[short])
         if short in LDD_NAMES:
             return float(LDD_NAMES[short])
         raise KeyError(f"unknown identifier {name} near {self.context()}")
@@ -453,6 +459,35 @@ def extract(ref_root):
             case["expected"] = expected
             cases.append(case)
         out[op] = cases
+
+    # d8_flow_direction: every (elevation, expected) pair a case checks with check_arrays_are_equal (nested
+    # blocks may hold several); cases without a check (barnes_2017: LUE does not match it) are left out
+    path = os.path.join(test_dir, "d8_flow_direction_test.cpp")
+    cases = []
+    for name, body in case_bodies(path):
+        env = {}
+        record = run_statements(split_statements(body), env, [])
+        last = {}
+        k = 0
+        for key, value in record:
+            if key == "equal":
+                got_name, want_name = value
+                call = last[got_name]
+                elevation = call[2][0]
+                want = last[want_name]
+                k += 1
+                cases.append({
+                    "name": name if k == 1 else f"{name}_{k}",
+                    "source": "source/framework/algorithm/test/d8_flow_direction_test.cpp",
+                    "kind": "d8_flow_direction",
+                    "array_shape": list(elevation.array_shape),
+                    "partition_shape": list(elevation.partition_shape),
+                    "elevation": to_json_raster(elevation),
+                    "expected": {"flow_direction": {"raster": to_json_raster(want), "compare": "equal"}},
+                })
+            elif isinstance(key, str):
+                last[key] = value
+    out["d8_flow_direction"] = cases
 
     # inflow count: test_inflow_count([policies,] flow_direction, expected)
     path = os.path.join(test_dir, "inflow_count_test.cpp")

@@ -187,3 +187,26 @@ def kinematic_residual(q_new, q_up, q_cur, q_lat, alpha, beta, dt, dx):
     """f(Q) of the kinematic-wave equation, relative to its known terms."""
     known = (dt / dx) * q_up + alpha * q_cur ** beta + dt * q_lat
     return ((dt / dx) * q_new + alpha * q_new ** beta - known) / np.maximum(known, 1e-300)
+
+
+def d8_flow_direction(elevation):
+    """LUE's D8 flow direction, vectorised: pad with numeric_limits<T>::min(), then the eight raw drops in the
+    order N, W, E, S, NW, NE, SW, SE, each taken where it exceeds the best so far (start: sink, drop 0);
+    NaN (no-data) neighbours are skipped, a NaN centre gives 255."""
+    elevation = np.asarray(elevation)
+    t = elevation.dtype
+    padded = np.pad(elevation, 1, constant_values=np.finfo(t).tiny)
+    rows, cols = elevation.shape
+    centre = padded[1:-1, 1:-1]
+    out = np.full((rows, cols), 5, dtype=np.uint8)
+    best = np.zeros((rows, cols), dtype=t)
+    for dr, dc, code in ((-1, 0, 8), (0, -1, 4), (0, 1, 6), (1, 0, 2), (-1, -1, 7), (-1, 1, 9), (1, -1, 1),
+                         (1, 1, 3)):
+        neighbour = padded[1 + dr:1 + dr + rows, 1 + dc:1 + dc + cols]
+        with np.errstate(invalid="ignore"):
+            drop = (centre - neighbour).astype(t)
+            take = ~np.isnan(neighbour) & (drop > best)
+        out[take] = code
+        best = np.where(take, drop, best)
+    out[np.isnan(centre)] = 255
+    return out

@@ -43,6 +43,12 @@ def write_cases(path):
                 f'Case{{"{c["name"]}", "{c["kind"]}", {{{rows}, {cols}}}, {{{prows}, {pcols}}}, {_vec(c["ldd"])}, '
                 f'{{{", ".join(args)}}}, {{{", ".join(_vec(e["raster"]) for e in exp)}}}, '
                 f'{"true" if close else "false"}, {"true" if c.get("property") == "all_positive" else "false"}}},')
+    for c in load_golden("d8_flow_direction"):
+        rows, cols = c["array_shape"]
+        prows, pcols = c["partition_shape"]
+        lines.append(
+            f'Case{{"{c["name"]}", "d8_flow_direction", {{{rows}, {cols}}}, {{{prows}, {pcols}}}, '
+            f'{_vec(c["elevation"])}, {{}}, {{{_vec(c["expected"]["flow_direction"]["raster"])}}}, false, false}},')
     lines += ["};", "return all;", "}"]
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")

@@ -1,0 +1,60 @@
+"""d8_flow_direction on the GPU (lue_amd_d8_flow_direction_*) against LUE's test vectors, the oracle and the
+vectorised model of the reference's functor."""
+import numpy as np
+import pytest
+
+import reference_models as rm
+from conftest import load_golden, raster
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def fw():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lue_amd import framework
+
+    return framework
+
+
+def pa(fw, array, partition_shape=None):
+    return fw.PartitionedArray.from_numpy(array, partition_shape)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("case", load_golden("d8_flow_direction"), ids=lambda c: c["name"])
+def test_d8_golden(fw, case, dtype):
+    elevation = pa(fw, raster(case["elevation"], dtype), case["partition_shape"])
+    got = fw.d8_flow_direction(elevation)
+    assert got.partition_shape == elevation.partition_shape
+    assert np.array_equal(got.numpy(), raster(case["expected"]["flow_direction"]["raster"], np.uint8))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(1, 1), (1, 70), (33, 5), (32, 64), (65, 129), (300, 257), (1000, 777)])
+def test_d8_random_against_oracle(fw, oracle_mod, dtype, shape):
+    """Ties (integer heights), no-data, negative heights, odd shapes (partial tiles, unaligned rows)."""
+    rng = np.random.default_rng(shape[0] * 7 + shape[1])
+    dem = rng.integers(-3, 6, shape).astype(dtype)
+    dem[rng.random(shape) < 0.05] = np.nan
+    dem[:, shape[1] // 2:] += rng.random((shape[0], shape[1] - shape[1] // 2)).astype(dtype)
+    got = fw.d8_flow_direction(pa(fw, dem)).numpy()
+    assert np.array_equal(got, oracle_mod.d8_flow_direction(dem, dtype))
+    assert np.array_equal(got, rm.d8_flow_direction(dem))
+
+
+def test_d8_synthetic_dem(fw, oracle_mod):
+    """The benchmark DEM (4099 x 3001, FP32): device generator equals the host one, and the ldd equals the
+    oracle's; every cell of this DEM drains (no sinks but the ones the edges make)."""
+    from lue_amd import synthetic as syn
+
+    dem_d = syn.dem_device(20261208, 4099, 3001, torch.float32)
+    dem_h = syn.dem_host(20261208, 4099, 3001).astype(np.float32)
+    assert np.array_equal(dem_d.cpu().numpy(), dem_h)
+    got = fw.d8_flow_direction(fw.PartitionedArray(dem_d)).numpy()
+    assert np.array_equal(got, oracle_mod.d8_flow_direction(dem_h, np.float32))
+    # the DEM falls by at least 0.1 per row: interior cells never are sinks
+    assert (got[1:-1, 1:-1] != 5).all()

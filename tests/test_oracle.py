@@ -195,3 +195,25 @@ def test_planted_mistake_is_caught(oracle_mod):
     if (ldd == 3).any():
         assert not np.array_equal(np.nan_to_num(wrong), np.nan_to_num(want))
     assert_equal_nan(oracle_mod.accu(ldd, 1.0, (4, 4), np.float64), want)
+
+
+D8_CASES = load_golden("d8_flow_direction")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("case", D8_CASES, ids=lambda c: c["name"])
+def test_d8_flow_direction_golden(oracle_mod, case, dtype):
+    got = oracle_mod.d8_flow_direction(raster(case["elevation"], dtype), dtype)
+    assert np.array_equal(got, raster(case["expected"]["flow_direction"]["raster"], np.uint8))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("seed", range(3))
+def test_d8_flow_direction_brute_force(oracle_mod, dtype, seed):
+    """Against the vectorised model on DEMs with ties (integer heights), flats, no-data and negative heights."""
+    rng = np.random.default_rng(700 + seed)
+    shape = (57, 83)
+    dem = rng.integers(-3, 6, shape).astype(dtype)
+    dem[rng.random(shape) < 0.05] = np.nan
+    dem[:, 40:] += rng.random((shape[0], shape[1] - 40)).astype(dtype)
+    assert np.array_equal(oracle_mod.d8_flow_direction(dem, dtype), rm.d8_flow_direction(dem))
