@@ -335,6 +335,9 @@ def main():
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
         extras = {}
+        log("[bench] extras: d8_flow_direction 16384^2")
+        extras["d8_flow_direction"] = workloads.d8_flow_direction()
+        torch.cuda.empty_cache()
         log("[bench] extras: batch of 512 catchments")
         extras["batch_accu_threshold"] = workloads.batch_threshold()
         torch.cuda.empty_cache()

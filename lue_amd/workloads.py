@@ -24,6 +24,31 @@ def _sync():
     torch.cuda.synchronize()
 
 
+def d8_flow_direction(seed=20261208, side=16384, repeats=5, device=None):
+    """d8_flow_direction on the synthetic DEM (side x side), FP32 and FP64: time per call (HIP events) and
+    the HBM rate of its algorithmic bytes (elevation read + ldd written: 5 B/cell FP32, 9 B/cell FP64)."""
+    dev = torch.device(device or "cuda")
+    result = {"workload": f"d8_flow_direction, {side}x{side} synthetic DEM", "cells": side * side}
+    for dtype, name in ((torch.float32, "f32"), (torch.float64, "f64")):
+        elevation = fw.PartitionedArray(syn.dem_device(seed, side, side, dtype, dev))
+        out = fw.d8_flow_direction(elevation)
+        _sync()
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(repeats):
+            out = fw.d8_flow_direction(elevation)
+        stop.record()
+        _sync()
+        ms = start.elapsed_time(stop) / repeats
+        nbytes = side * side * (elevation.data.element_size() + 1)
+        result[name] = {"ms": round(ms, 4), "mcells_per_s": round(side * side / ms / 1e3, 1),
+                        "algorithmic_bytes": nbytes, "gb_per_s": round(nbytes / ms / 1e6, 1),
+                        "hbm_fraction": round(nbytes / ms / 1e6 / 8000.0, 3)}
+        del elevation, out
+    return result
+
+
 def batch_threshold(seed=20261205, count=512, side_lo=64, side_hi=2048, repeats=3, device=None):
     dev = torch.device(device or "cuda")
     shapes = syn.batch_shapes(seed, count, side_lo, side_hi)
