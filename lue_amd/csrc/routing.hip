@@ -416,13 +416,28 @@ namespace {
             return status_success;
         if (!elevation || !flow_direction)
             return fail(status_invalid_argument, "elevation and flow_direction must not be null");
-        std::int64_t const gx = (cols + d8_tw - 1) / d8_tw;
-        std::int64_t const gy = (rows + d8_th - 1) / d8_th;
-        if (gx > 0x7fffffff || gy > 0xffff)
-            return fail(status_invalid_argument, "array too large for one launch");
-        hipLaunchKernelGGL(d8_flow_direction_kernel<T>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy)),
-                           dim3(d8_threads), 0, static_cast<hipStream_t>(stream), elevation, rows, cols,
-                           flow_direction);
+        hipStream_t const s = static_cast<hipStream_t>(stream);
+        bool const aligned = cols % 4 == 0 && reinterpret_cast<std::uintptr_t>(elevation) % 16 == 0 &&
+                             reinterpret_cast<std::uintptr_t>(flow_direction) % 4 == 0;
+        if (aligned)
+        {
+            std::int64_t const gx = (cols + d8_strip_cols - 1) / d8_strip_cols;
+            std::int64_t const gy = (rows + d8_strip_rows - 1) / d8_strip_rows;
+            if (gx > 0x7fffffff || gy > 0xffff)
+                return fail(status_invalid_argument, "array too large for one launch");
+            hipLaunchKernelGGL(d8_strip_kernel<T>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy)),
+                               dim3(d8_threads), 0, s, elevation, rows, cols, flow_direction);
+        }
+        else
+        {
+            std::int64_t const gx = (cols + d8_tw - 1) / d8_tw;
+            std::int64_t const gy = (rows + d8_th - 1) / d8_th;
+            if (gx > 0x7fffffff || gy > 0xffff)
+                return fail(status_invalid_argument, "array too large for one launch");
+            hipLaunchKernelGGL(d8_flow_direction_kernel<T>,
+                               dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy)), dim3(d8_threads), 0, s,
+                               elevation, rows, cols, flow_direction);
+        }
         LUE_AMD_HIP(hipGetLastError());
         return status_success;
     }

@@ -34,9 +34,12 @@ def test_d8_golden(fw, case, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("shape", [(1, 1), (1, 70), (33, 5), (32, 64), (65, 129), (300, 257), (1000, 777)])
+@pytest.mark.parametrize("shape", [(1, 1), (1, 70), (33, 5), (32, 64), (65, 129), (300, 257), (1000, 777),
+                                   (1, 4), (5, 8), (64, 256), (65, 260), (130, 1028), (129, 2048), (3, 4100)])
 def test_d8_random_against_oracle(fw, oracle_mod, dtype, shape):
-    """Ties (integer heights), no-data, negative heights, odd shapes (partial tiles, unaligned rows)."""
+    """Ties (integer heights), no-data, negative heights, odd shapes: partial tiles and unaligned rows (the
+    LDS-tile kernel), rows a multiple of 4 long (the strip kernel: partial strips, wavefront and workgroup
+    borders inside a row)."""
     rng = np.random.default_rng(shape[0] * 7 + shape[1])
     dem = rng.integers(-3, 6, shape).astype(dtype)
     dem[rng.random(shape) < 0.05] = np.nan
@@ -46,15 +49,32 @@ def test_d8_random_against_oracle(fw, oracle_mod, dtype, shape):
     assert np.array_equal(got, rm.d8_flow_direction(dem))
 
 
-def test_d8_synthetic_dem(fw, oracle_mod):
-    """The benchmark DEM (4099 x 3001, FP32): device generator equals the host one, and the ldd equals the
-    oracle's; every cell of this DEM drains (no sinks but the ones the edges make)."""
+@pytest.mark.parametrize("shape", [(4099, 3001), (2051, 4100)])
+def test_d8_synthetic_dem(fw, oracle_mod, shape):
+    """The benchmark DEM (FP32): device generator equals the host one, and the ldd equals the oracle's; every
+    cell of this DEM drains (no sinks but the ones the edges make)."""
     from lue_amd import synthetic as syn
 
-    dem_d = syn.dem_device(20261208, 4099, 3001, torch.float32)
-    dem_h = syn.dem_host(20261208, 4099, 3001).astype(np.float32)
+    dem_d = syn.dem_device(20261208, *shape, torch.float32)
+    dem_h = syn.dem_host(20261208, *shape).astype(np.float32)
     assert np.array_equal(dem_d.cpu().numpy(), dem_h)
     got = fw.d8_flow_direction(fw.PartitionedArray(dem_d)).numpy()
     assert np.array_equal(got, oracle_mod.d8_flow_direction(dem_h, np.float32))
     # the DEM falls by at least 0.1 per row: interior cells never are sinks
     assert (got[1:-1, 1:-1] != 5).all()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(37, 64), (37, 63)])
+def test_d8_special_values(fw, oracle_mod, dtype, shape):
+    """Infinities (inf - inf is NaN), signed zeros, subnormals and the halo value itself: the kernels reuse each
+    difference negated for the opposite neighbour, which must give the drops the reference computes."""
+    rng = np.random.default_rng(shape[1])
+    info = np.finfo(dtype)
+    pool = np.array([np.inf, -np.inf, 0.0, -0.0, info.tiny, info.smallest_subnormal, -info.smallest_subnormal,
+                     1.0, 1.0 + info.eps, np.nan, info.max, -info.max], dtype=dtype)
+    dem = pool[rng.integers(0, len(pool), shape)]
+    got = fw.d8_flow_direction(pa(fw, dem)).numpy()
+    assert np.array_equal(got, oracle_mod.d8_flow_direction(dem, dtype))
+    with np.errstate(invalid="ignore", over="ignore"):
+        assert np.array_equal(got, rm.d8_flow_direction(dem))
