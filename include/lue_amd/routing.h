@@ -112,6 +112,65 @@ int lue_amd_d8_flow_direction_f64(const double* elevation, int64_t rows, int64_t
                                   void* stream);
 
 /*
+ * downstream_distance: per cell, the distance to its downstream cell: cell_size for N, W, E, S,
+ * sqrt(2) * cell_size (rounded in the element type) for NW, NE, SW, SE, 0 for a pit or unknown code. With
+ * mark_no_data != 0 (LUE's value policies, as lue.framework.downstream_distance uses them) a no-data ldd
+ * (255) gives NaN; with 0 (lue::downstream_distance's default policies) it gives 0. flow_direction must be
+ * 4-byte aligned. Replaces lue::downstream_distance (ALG/downstream_distance.hpp:24-39,
+ * ALG/definition/downstream_distance.hpp:13-75; Python binding
+ * source/framework/python/source/algorithm/routing_operation/downstream_distance.cpp:14-23).
+ * Asynchronous on `stream`.
+ */
+int lue_amd_downstream_distance_f32(const uint8_t* flow_direction, int64_t rows, int64_t cols, float cell_size,
+                                   int mark_no_data, float* distance, void* stream);
+int lue_amd_downstream_distance_f64(const uint8_t* flow_direction, int64_t rows, int64_t cols, double cell_size,
+                                   int mark_no_data, double* distance, void* stream);
+
+/*
+ * upstream: per cell, the sum (from 0, in the window's row-major order: NW, N, NE, W, E, SW, S, SE
+ * neighbour) of the material of the neighbours that drain into it; downstream: per cell, the material of
+ * its downstream neighbour (0 beyond the array's edge; its own for a pit). A no-data ldd (255) or material
+ * in the cell, or a no-data material taken from a neighbour, gives no-data: NaN for floating point,
+ * the minimum of a signed and the maximum of an unsigned integer type. Integer sums wrap.
+ * Replaces lue::value_policies::upstream / downstream (ALG/value_policies/upstream.hpp:16-30,
+ * ALG/definition/upstream.hpp:36-190; ALG/value_policies/downstream.hpp:16-30,
+ * ALG/definition/downstream.hpp:36-186) for LUE's default material element types
+ * (environment/cmake/LueConfiguration.cmake:171-228). Asynchronous on `stream`.
+ */
+int lue_amd_upstream_u8(const uint8_t* flow_direction, const uint8_t* material, int64_t rows, int64_t cols,
+                        uint8_t* out, void* stream);
+int lue_amd_upstream_u32(const uint8_t* flow_direction, const uint32_t* material, int64_t rows, int64_t cols,
+                         uint32_t* out, void* stream);
+int lue_amd_upstream_u64(const uint8_t* flow_direction, const uint64_t* material, int64_t rows, int64_t cols,
+                         uint64_t* out, void* stream);
+int lue_amd_upstream_i8(const uint8_t* flow_direction, const int8_t* material, int64_t rows, int64_t cols,
+                        int8_t* out, void* stream);
+int lue_amd_upstream_i32(const uint8_t* flow_direction, const int32_t* material, int64_t rows, int64_t cols,
+                         int32_t* out, void* stream);
+int lue_amd_upstream_i64(const uint8_t* flow_direction, const int64_t* material, int64_t rows, int64_t cols,
+                         int64_t* out, void* stream);
+int lue_amd_upstream_f32(const uint8_t* flow_direction, const float* material, int64_t rows, int64_t cols,
+                         float* out, void* stream);
+int lue_amd_upstream_f64(const uint8_t* flow_direction, const double* material, int64_t rows, int64_t cols,
+                         double* out, void* stream);
+int lue_amd_downstream_u8(const uint8_t* flow_direction, const uint8_t* material, int64_t rows, int64_t cols,
+                          uint8_t* out, void* stream);
+int lue_amd_downstream_u32(const uint8_t* flow_direction, const uint32_t* material, int64_t rows, int64_t cols,
+                           uint32_t* out, void* stream);
+int lue_amd_downstream_u64(const uint8_t* flow_direction, const uint64_t* material, int64_t rows, int64_t cols,
+                           uint64_t* out, void* stream);
+int lue_amd_downstream_i8(const uint8_t* flow_direction, const int8_t* material, int64_t rows, int64_t cols,
+                          int8_t* out, void* stream);
+int lue_amd_downstream_i32(const uint8_t* flow_direction, const int32_t* material, int64_t rows, int64_t cols,
+                           int32_t* out, void* stream);
+int lue_amd_downstream_i64(const uint8_t* flow_direction, const int64_t* material, int64_t rows, int64_t cols,
+                           int64_t* out, void* stream);
+int lue_amd_downstream_f32(const uint8_t* flow_direction, const float* material, int64_t rows, int64_t cols,
+                           float* out, void* stream);
+int lue_amd_downstream_f64(const uint8_t* flow_direction, const double* material, int64_t rows, int64_t cols,
+                           double* out, void* stream);
+
+/*
  * accu: outflow = sum of upstream outflow + inflow.
  * Replaces lue::value_policies::accu(flow_direction, inflow | Scalar inflow)
  * (ALG/value_policies/accu.hpp:33-55, ALG/accu.hpp:12-28, ALG/definition/accu.hpp:226-264).

@@ -74,6 +74,11 @@ SIGNATURES = {
     "lue_amd_synthetic_uniform_f32": [c_u64, c_u64, c_d, c_d, c_i64, c_p, c_p],
     "lue_amd_synthetic_uniform_f64": [c_u64, c_u64, c_d, c_d, c_i64, c_p, c_p],
 }
+SIGNATURES["lue_amd_downstream_distance_f32"] = [c_p, c_i64, c_i64, c_f, c_int, c_p, c_p]
+SIGNATURES["lue_amd_downstream_distance_f64"] = [c_p, c_i64, c_i64, c_d, c_int, c_p, c_p]
+for _op in ("upstream", "downstream"):
+    for _sfx in ("u8", "u32", "u64", "i8", "i32", "i64", "f32", "f64"):
+        SIGNATURES[f"lue_amd_{_op}_{_sfx}"] = [c_p, c_p, c_i64, c_i64, c_p, c_p]
 
 _RESTYPES = {
     "lue_amd_status_string": ctypes.c_char_p,

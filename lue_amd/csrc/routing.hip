@@ -3,6 +3,7 @@
 #include "lue_amd/routing.h"
 
 #include "d8_flow_direction.hpp"
+#include "flow_neighbours.hpp"
 #include "inflow_count.hpp"
 #include "kinematic_wave.hpp"
 #include "routing_common.hpp"
@@ -442,6 +443,48 @@ namespace {
         return status_success;
     }
 
+    template<typename T>
+    int downstream_distance_impl(std::uint8_t const* ldd, std::int64_t rows, std::int64_t cols, T cell_size,
+                                 int mark_no_data, T* out, void* stream)
+    {
+        if (rows < 0 || cols < 0)
+            return fail(status_invalid_argument, "array shape must not be negative");
+        if (rows == 0 || cols == 0)
+            return status_success;
+        if (!ldd || !out)
+            return fail(status_invalid_argument, "flow_direction and distance must not be null");
+        if (reinterpret_cast<std::uintptr_t>(ldd) % 4 != 0)
+            return fail(status_invalid_argument, "flow_direction must be 4-byte aligned");
+        std::int64_t const n = rows * cols;
+        T const diagonal = std::sqrt(T{2}) * cell_size;  // as definition/downstream_distance.hpp:31-35 does
+        std::int64_t const blocks = std::clamp<std::int64_t>((n / 4 + 255) / 256, 1, 8192);
+        hipLaunchKernelGGL(downstream_distance_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), ldd, n, cell_size, diagonal, mark_no_data, out);
+        LUE_AMD_HIP(hipGetLastError());
+        return status_success;
+    }
+
+    template<typename T, bool Upstream>
+    int flow_neighbour_impl(std::uint8_t const* ldd, T const* material, std::int64_t rows, std::int64_t cols,
+                            T* out, void* stream)
+    {
+        if (rows < 0 || cols < 0)
+            return fail(status_invalid_argument, "array shape must not be negative");
+        if (rows == 0 || cols == 0)
+            return status_success;
+        if (!ldd || !material || !out)
+            return fail(status_invalid_argument, "flow_direction, material and output must not be null");
+        std::int64_t const gx = (cols + nb_tw - 1) / nb_tw;
+        std::int64_t const gy = (rows + nb_th - 1) / nb_th;
+        if (gx > 0x7fffffff || gy > 0xffff)
+            return fail(status_invalid_argument, "array too large for one launch");
+        hipLaunchKernelGGL((flow_neighbour_kernel<T, Upstream>),
+                           dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy)), dim3(nb_threads), 0,
+                           static_cast<hipStream_t>(stream), ldd, material, rows, cols, out);
+        LUE_AMD_HIP(hipGetLastError());
+        return status_success;
+    }
+
 }  // namespace
 
 extern "C" {
@@ -710,6 +753,40 @@ int lue_amd_synthetic_ldd(uint64_t seed, int64_t rows, int64_t cols, uint8_t* ld
     LUE_AMD_HIP(hipStreamSynchronize(s));
     return status_success;
 }
+
+int lue_amd_downstream_distance_f32(const uint8_t* flow_direction, int64_t rows, int64_t cols, float cell_size,
+                                   int mark_no_data, float* distance, void* stream)
+{
+    return downstream_distance_impl<float>(flow_direction, rows, cols, cell_size, mark_no_data, distance, stream);
+}
+
+int lue_amd_downstream_distance_f64(const uint8_t* flow_direction, int64_t rows, int64_t cols, double cell_size,
+                                   int mark_no_data, double* distance, void* stream)
+{
+    return downstream_distance_impl<double>(flow_direction, rows, cols, cell_size, mark_no_data, distance,
+                                            stream);
+}
+
+#define LUE_AMD_FLOW_NEIGHBOUR_OPS(T, SUFFIX)                                                                 \
+    int lue_amd_upstream_##SUFFIX(const uint8_t* flow_direction, const T* material, int64_t rows, int64_t cols, \
+                                  T* out, void* stream)                                                      \
+    {                                                                                                        \
+        return flow_neighbour_impl<T, true>(flow_direction, material, rows, cols, out, stream);              \
+    }                                                                                                        \
+    int lue_amd_downstream_##SUFFIX(const uint8_t* flow_direction, const T* material, int64_t rows,         \
+                                    int64_t cols, T* out, void* stream)                                      \
+    {                                                                                                        \
+        return flow_neighbour_impl<T, false>(flow_direction, material, rows, cols, out, stream);             \
+    }
+
+LUE_AMD_FLOW_NEIGHBOUR_OPS(uint8_t, u8)
+LUE_AMD_FLOW_NEIGHBOUR_OPS(uint32_t, u32)
+LUE_AMD_FLOW_NEIGHBOUR_OPS(uint64_t, u64)
+LUE_AMD_FLOW_NEIGHBOUR_OPS(int8_t, i8)
+LUE_AMD_FLOW_NEIGHBOUR_OPS(int32_t, i32)
+LUE_AMD_FLOW_NEIGHBOUR_OPS(int64_t, i64)
+LUE_AMD_FLOW_NEIGHBOUR_OPS(float, f32)
+LUE_AMD_FLOW_NEIGHBOUR_OPS(double, f64)
 
 int lue_amd_synthetic_dem_f32(uint64_t seed, int64_t rows, int64_t cols, float* out, void* stream)
 {

@@ -1,7 +1,8 @@
 """Python front end of the routing path, mirroring lue.framework's routing operations.
 
 Reference bindings: source/framework/python/source/algorithm/routing_operation/{accu,accu_threshold,
-accu_fraction,kinematic_wave,inflow_count}.cpp, which call lue::value_policies::* on PartitionedArray.
+accu_fraction,accu_capacity,accu_trigger,kinematic_wave,inflow_count,d8_flow_direction,upstream,
+downstream}.cpp, which call lue::value_policies::* on PartitionedArray.
 
 PartitionedArray here is device resident: one contiguous row-major raster in HBM (a torch tensor used as
 device memory) plus LUE's partition shape, which is kept as metadata (results do not depend on it). A
@@ -156,6 +157,56 @@ def d8_flow_direction(elevation):
     rows, cols = elevation.shape
     fn = getattr(lib, f"lue_amd_d8_flow_direction_{FLOAT_TYPES[dtype]}")
     _lib.check(fn(_ptr(elevation.data), rows, cols, _ptr(out.data), _stream()))
+    return out
+
+
+# material element types of LUE's default build (environment/cmake/LueConfiguration.cmake:171-228)
+MATERIAL_TYPES = {torch.uint8: "u8", torch.uint32: "u32", torch.uint64: "u64", torch.int8: "i8",
+                  torch.int32: "i32", torch.int64: "i64", torch.float32: "f32", torch.float64: "f64"}
+
+
+def _flow_neighbour(op, flow_direction, material):
+    _check_ldd(flow_direction)
+    if not isinstance(material, PartitionedArray):
+        raise TypeError("material must be a PartitionedArray")
+    _verify_compatible(flow_direction, material)
+    if material.dtype not in MATERIAL_TYPES:
+        raise TypeError(f"unsupported material element type {material.dtype}")
+    lib = _lib.load()
+    out = _output(flow_direction, material.dtype)
+    rows, cols = flow_direction.shape
+    fn = getattr(lib, f"lue_amd_{op}_{MATERIAL_TYPES[material.dtype]}")
+    _lib.check(fn(_ptr(flow_direction.data), _ptr(material.data), rows, cols, _ptr(out.data), _stream()))
+    return out
+
+
+def upstream(flow_direction, material):
+    """lue.framework.upstream(flow_direction, material): per cell, the sum of the material of the neighbours
+    that drain into it (value_policies/upstream.hpp:16-30, definition/upstream.hpp:36-190)."""
+    return _flow_neighbour("upstream", flow_direction, material)
+
+
+def downstream(flow_direction, material):
+    """lue.framework.downstream(flow_direction, material): per cell, the material of its downstream
+    neighbour (value_policies/downstream.hpp:16-30, definition/downstream.hpp:36-186)."""
+    return _flow_neighbour("downstream", flow_direction, material)
+
+
+def downstream_distance(flow_direction, cell_size, dtype=np.float32):
+    """lue.framework.downstream_distance(flow_direction, cell_size, dtype): per cell, the distance to its
+    downstream cell (python/source/algorithm/routing_operation/downstream_distance.cpp:14-23, value policies:
+    a no-data ldd gives NaN; definition/downstream_distance.hpp:13-75). dtype selects float32 or float64 by
+    its item size, as the binding does."""
+    _check_ldd(flow_direction)
+    size = np.dtype(dtype).itemsize
+    if size not in (4, 8):
+        raise TypeError(f"unsupported distance element type {dtype}")
+    torch_dtype = torch.float32 if size == 4 else torch.float64
+    lib = _lib.load()
+    out = _output(flow_direction, torch_dtype)
+    rows, cols = flow_direction.shape
+    fn = getattr(lib, f"lue_amd_downstream_distance_{FLOAT_TYPES[torch_dtype]}")
+    _lib.check(fn(_ptr(flow_direction.data), rows, cols, float(cell_size), 1, _ptr(out.data), _stream()))
     return out
 
 

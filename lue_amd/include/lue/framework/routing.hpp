@@ -9,8 +9,8 @@
 //   - lue::Scalar<E>  (source/framework/algorithm/include/lue/framework/algorithm/scalar.hpp:20);
 //   - lue::create_partitioned_array(shape, partition_shape, fill_value);
 //   - lue::value_policies::{accu, accu_threshold, accu_fraction, accu_capacity, accu_trigger, kinematic_wave,
-//     d8_flow_direction} with the reference's overload sets (ALG/value_policies/*.hpp) and
-//     lue::inflow_count<std::uint8_t>(ldd)
+//     d8_flow_direction, upstream, downstream} with the reference's overload sets (ALG/value_policies/*.hpp) and
+//     lue::inflow_count<std::uint8_t>(ldd) and lue::downstream_distance(ldd, cell_size)
 //     (ALG/inflow_count.hpp:29-42), ALG = source/framework/algorithm/include/lue/framework/algorithm.
 // Each operation calls one function of the C ABI (include/lue_amd/routing.h). Shape and partitioning
 // mismatches throw std::runtime_error as detail::verify_compatible does (ALG/detail/verify_compatible.hpp:
@@ -296,7 +296,90 @@ namespace lue {
         return result;
     }
 
+    namespace detail {
+
+        // the C ABI's upstream / downstream per material element type (LUE's default material elements,
+        // environment/cmake/LueConfiguration.cmake:171-228)
+        template<typename E>
+        struct FlowNeighbourOps;
+
+#define LUE_AMD_FLOW_NEIGHBOUR_OPS(T, SUFFIX)                                                                 \
+    template<>                                                                                               \
+    struct FlowNeighbourOps<T>                                                                               \
+    {                                                                                                        \
+        static constexpr auto upstream = &lue_amd_upstream_##SUFFIX;                                        \
+        static constexpr auto downstream = &lue_amd_downstream_##SUFFIX;                                    \
+    };
+        LUE_AMD_FLOW_NEIGHBOUR_OPS(std::uint8_t, u8)
+        LUE_AMD_FLOW_NEIGHBOUR_OPS(std::uint32_t, u32)
+        LUE_AMD_FLOW_NEIGHBOUR_OPS(std::uint64_t, u64)
+        LUE_AMD_FLOW_NEIGHBOUR_OPS(std::int8_t, i8)
+        LUE_AMD_FLOW_NEIGHBOUR_OPS(std::int32_t, i32)
+        LUE_AMD_FLOW_NEIGHBOUR_OPS(std::int64_t, i64)
+        LUE_AMD_FLOW_NEIGHBOUR_OPS(float, f32)
+        LUE_AMD_FLOW_NEIGHBOUR_OPS(double, f64)
+#undef LUE_AMD_FLOW_NEIGHBOUR_OPS
+
+        template<typename E>
+        concept MaterialElement = requires { FlowNeighbourOps<E>::upstream; };
+
+        template<bool upstream, typename E>
+        auto flow_neighbour(PartitionedArray<FlowDirectionElement, 2> const& flow_direction,
+                            PartitionedArray<E, 2> const& material) -> PartitionedArray<E, 2>
+        {
+            verify_compatible(flow_direction, material);
+            auto result = output_like<E>(flow_direction);
+            auto const [rows, cols] = flow_direction.shape();
+            auto const fn = upstream ? FlowNeighbourOps<E>::upstream : FlowNeighbourOps<E>::downstream;
+            check(fn(flow_direction.data(), material.data(), rows, cols, result.data(), nullptr));
+            check(lue_amd_stream_synchronize(nullptr));
+            return result;
+        }
+
+    }  // namespace detail
+
+    // lue::downstream_distance(flow_direction, cell_size) (ALG/downstream_distance.hpp:31-39, default policies:
+    // a no-data ldd is an unknown code, distance 0; ALG/definition/downstream_distance.hpp:13-75): per cell,
+    // the distance to its downstream cell
+    template<std::integral FlowDirection, std::floating_point Distance>
+        requires std::same_as<FlowDirection, FlowDirectionElement>
+    auto downstream_distance(PartitionedArray<FlowDirection, 2> const& flow_direction, Distance const cell_size)
+        -> PartitionedArray<Distance, 2>
+    {
+        auto result = detail::output_like<Distance>(flow_direction);
+        auto const [rows, cols] = flow_direction.shape();
+        if constexpr (std::is_same_v<Distance, float>)
+            detail::check(lue_amd_downstream_distance_f32(flow_direction.data(), rows, cols, cell_size, 0,
+                                                          result.data(), nullptr));
+        else
+            detail::check(lue_amd_downstream_distance_f64(flow_direction.data(), rows, cols, cell_size, 0,
+                                                          result.data(), nullptr));
+        detail::check(lue_amd_stream_synchronize(nullptr));
+        return result;
+    }
+
     namespace value_policies {
+
+        // lue::value_policies::upstream(flow_direction, material) (ALG/value_policies/upstream.hpp:16-30,
+        // ALG/definition/upstream.hpp:36-190): per cell, the sum of the material of the neighbours that
+        // drain into it
+        template<std::integral FlowDirection, lue::detail::MaterialElement E>
+            requires std::same_as<FlowDirection, FlowDirectionElement>
+        auto upstream(PartitionedArray<FlowDirection, 2> const& flow_direction, PartitionedArray<E, 2> const& material)
+            -> PartitionedArray<E, 2>
+        {
+            return lue::detail::flow_neighbour<true>(flow_direction, material);
+        }
+
+        // lue::value_policies::downstream(flow_direction, material) (ALG/value_policies/downstream.hpp:16-30,
+        // ALG/definition/downstream.hpp:36-186): per cell, the material of its downstream neighbour
+        template<std::integral FlowDirection, lue::detail::MaterialElement E>
+            requires std::same_as<FlowDirection, FlowDirectionElement>
+        auto downstream(PartitionedArray<FlowDirection, 2> const& flow_direction,
+                        PartitionedArray<E, 2> const& material) -> PartitionedArray<E, 2>
+        {
+            return lue::detail::flow_neighbour<false>(flow_direction, material);
+        }
 
         // lue::value_policies::d8_flow_direction<FlowDirection>(elevation)
         // (ALG/value_policies/d8_flow_direction.hpp:46-55): the ldd of an elevation raster, NaN = no-data

@@ -18,6 +18,13 @@
 //   - kinematic-wave Newton solve .................. definition/kinematic_wave.hpp:13-254
 //   - D8 flow direction ............................ definition/d8_flow_direction.hpp:12-143,
 //                                                    value_policies/d8_flow_direction.hpp:31-41 (halo value)
+//   - downstream_distance .......................... definition/downstream_distance.hpp:13-75, with the value
+//                                                    policies of python/source/algorithm/routing_operation/
+//                                                    downstream_distance.cpp:14-23 (no-data ldd -> NaN) or without
+//   - upstream / downstream ........................ definition/upstream.hpp:36-150, definition/downstream.hpp:36-155,
+//                                                    flow_accumulation.hpp:47-84 (policies: halo ldd drains away,
+//                                                    halo material 0), policy/default_value_policies.hpp:19-29 and
+//                                                    policy/detect_no_data_by_value.hpp:25-50 (no-data values)
 //
 // Arithmetic order. Inside one partition the order of floating point operations is LUE's: cells are
 // scanned row-major, a walk starts at every ridge cell (inflow count 0), upstream outflow is added to a
@@ -47,6 +54,7 @@
 #include <cstring>
 #include <limits>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -705,6 +713,144 @@ namespace {
         }
     }
 
+    // Material no-data under the default value policies: NaN for floating point, numeric_limits<T>::min()
+    // for signed and ::max() for unsigned integers (policy/default_value_policies.hpp:19-29,
+    // policy/detect_no_data_by_value.hpp:25-50).
+    template<typename T>
+    constexpr auto material_no_data() -> T
+    {
+        if constexpr (std::is_floating_point_v<T>)
+            return std::numeric_limits<T>::quiet_NaN();
+        else if constexpr (std::is_signed_v<T>)
+            return std::numeric_limits<T>::min();
+        else
+            return std::numeric_limits<T>::max();
+    }
+
+    template<typename T>
+    auto material_is_no_data(T v) -> bool
+    {
+        if constexpr (std::is_floating_point_v<T>)
+            return is_nan(v);
+        else
+            return v == material_no_data<T>();
+    }
+
+    // a + b in T; integers wrap (two's complement), as the reference's `material += ...` does for the
+    // narrow types and its hardware does for the wide ones
+    template<typename T>
+    auto material_add(T a, T b) -> T
+    {
+        if constexpr (std::is_floating_point_v<T>)
+            return a + b;
+        else
+        {
+            using U = std::make_unsigned_t<T>;
+            return static_cast<T>(static_cast<U>(static_cast<U>(a) + static_cast<U>(b)));
+        }
+    }
+
+    // upstream: definition/upstream.hpp:36-150. A no-data ldd or material in the cell gives no-data; else the
+    // sum, starting from 0, of the material of the neighbours that drain into the cell, taken in the window's
+    // row-major order (NW, N, NE, W, E, SW, S, SE neighbour); a no-data material among them gives no-data.
+    // Cells outside the array never drain into it (FlowDirectionHalo: the halo points away).
+    template<typename T>
+    void upstream(std::uint8_t const* ldd, T const* material, Index rows, Index cols, T* out)
+    {
+        // (row offset, column offset, code that drains into the centre) in window order
+        static constexpr int window[8][3] = {{-1, -1, 3}, {-1, 0, 2}, {-1, 1, 1}, {0, -1, 6},
+                                             {0, 1, 4},   {1, -1, 9}, {1, 0, 8},  {1, 1, 7}};
+        for (Index r = 0; r < rows; ++r)
+        {
+            for (Index c = 0; c < cols; ++c)
+            {
+                Index const i = r * cols + c;
+                if (ldd[i] == no_data_ldd || material_is_no_data(material[i]))
+                {
+                    out[i] = material_no_data<T>();
+                    continue;
+                }
+                T sum{0};
+                for (auto const& w : window)
+                {
+                    Index const rr = r + w[0];
+                    Index const cc = c + w[1];
+                    if (rr < 0 || rr >= rows || cc < 0 || cc >= cols)
+                        continue;
+                    Index const j = rr * cols + cc;
+                    if (ldd[j] != w[2])
+                        continue;
+                    if (material_is_no_data(material[j]))
+                    {
+                        sum = material_no_data<T>();
+                        break;
+                    }
+                    sum = material_add(sum, material[j]);
+                }
+                out[i] = sum;
+            }
+        }
+    }
+
+    // downstream: definition/downstream.hpp:36-155. A no-data ldd or material in the cell gives no-data; else
+    // the material of the downstream neighbour (no-data if that is no-data; 0 where the ldd leaves the
+    // array: FillHaloWithConstantValue{0}), the cell's own material for a sink (or any code that is no
+    // direction).
+    template<typename T>
+    void downstream(std::uint8_t const* ldd, T const* material, Index rows, Index cols, T* out)
+    {
+        for (Index r = 0; r < rows; ++r)
+        {
+            for (Index c = 0; c < cols; ++c)
+            {
+                Index const i = r * cols + c;
+                std::uint8_t const code = ldd[i];
+                if (code == no_data_ldd || material_is_no_data(material[i]))
+                {
+                    out[i] = material_no_data<T>();
+                    continue;
+                }
+                if (code < 1 || code > 9 || code == 5)
+                {
+                    out[i] = material[i];
+                    continue;
+                }
+                int const dr = code <= 3 ? 1 : (code >= 7 ? -1 : 0);
+                int const dc = (code % 3 == 1) ? -1 : (code % 3 == 0 ? 1 : 0);
+                Index const rr = r + dr;
+                Index const cc = c + dc;
+                if (rr < 0 || rr >= rows || cc < 0 || cc >= cols)
+                {
+                    out[i] = T{0};
+                    continue;
+                }
+                T const m = material[rr * cols + cc];
+                out[i] = material_is_no_data(m) ? material_no_data<T>() : m;
+            }
+        }
+    }
+
+    // downstream_distance: definition/downstream_distance.hpp:13-75. cell_size for N, W, E, S;
+    // sqrt(2) * cell_size (in T) for NW, NE, SW, SE; 0 otherwise (pit, unknown code). Under value policies a
+    // no-data ldd gives NaN; under the default policies it is an unknown code (0).
+    template<typename T>
+    void downstream_distance(std::uint8_t const* ldd, Index n, T cell_size, bool mark_no_data, T* out)
+    {
+        T const diagonal = std::sqrt(T{2}) * cell_size;
+        for (Index i = 0; i < n; ++i)
+        {
+            std::uint8_t const code = ldd[i];
+            if (mark_no_data && code == no_data_ldd)
+                out[i] = std::numeric_limits<T>::quiet_NaN();
+            else if (code == 7 || code == 9 || code == 1 || code == 3)
+                out[i] = diagonal;
+            else if (code == 8 || code == 4 || code == 6 || code == 2)
+                out[i] = cell_size;
+            else
+                out[i] = T{0};
+        }
+    }
+
 }  // namespace
 
 
@@ -837,5 +983,48 @@ int oracle_d8_flow_direction_f64(double const* elevation, Index rows, Index cols
     d8_flow_direction<double>(elevation, rows, cols, out);
     return ok;
 }
+
+#define ORACLE_NEIGHBOUR_OPS(T, SUFFIX)                                                                       \
+    int oracle_upstream_##SUFFIX(std::uint8_t const* ldd, T const* material, Index rows, Index cols, T* out)  \
+    {                                                                                                        \
+        if (rows < 0 || cols < 0 || (rows * cols > 0 && (!ldd || !material || !out)))                      \
+            return err_invalid_argument;                                                                     \
+        upstream<T>(ldd, material, rows, cols, out);                                                         \
+        return ok;                                                                                           \
+    }                                                                                                        \
+    int oracle_downstream_##SUFFIX(std::uint8_t const* ldd, T const* material, Index rows, Index cols, T* out) \
+    {                                                                                                        \
+        if (rows < 0 || cols < 0 || (rows * cols > 0 && (!ldd || !material || !out)))                      \
+            return err_invalid_argument;                                                                     \
+        downstream<T>(ldd, material, rows, cols, out);                                                       \
+        return ok;                                                                                           \
+    }
+
+int oracle_downstream_distance_f32(std::uint8_t const* ldd, Index rows, Index cols, float cell_size,
+                                   int mark_no_data, float* out)
+{
+    if (rows < 0 || cols < 0 || (rows * cols > 0 && (!ldd || !out)))
+        return err_invalid_argument;
+    downstream_distance<float>(ldd, rows * cols, cell_size, mark_no_data != 0, out);
+    return ok;
+}
+
+int oracle_downstream_distance_f64(std::uint8_t const* ldd, Index rows, Index cols, double cell_size,
+                                   int mark_no_data, double* out)
+{
+    if (rows < 0 || cols < 0 || (rows * cols > 0 && (!ldd || !out)))
+        return err_invalid_argument;
+    downstream_distance<double>(ldd, rows * cols, cell_size, mark_no_data != 0, out);
+    return ok;
+}
+
+ORACLE_NEIGHBOUR_OPS(std::uint8_t, u8)
+ORACLE_NEIGHBOUR_OPS(std::uint32_t, u32)
+ORACLE_NEIGHBOUR_OPS(std::uint64_t, u64)
+ORACLE_NEIGHBOUR_OPS(std::int8_t, i8)
+ORACLE_NEIGHBOUR_OPS(std::int32_t, i32)
+ORACLE_NEIGHBOUR_OPS(std::int64_t, i64)
+ORACLE_NEIGHBOUR_OPS(float, f32)
+ORACLE_NEIGHBOUR_OPS(double, f64)
 
 }  // extern "C"

@@ -54,6 +54,11 @@ def _load():
     lib.oracle_inflow_count.argtypes = [p, i64, i64, i64, i64, p, ci]
     for suffix in ("f32", "f64"):
         getattr(lib, f"oracle_d8_flow_direction_{suffix}").argtypes = [p, i64, i64, p]
+    lib.oracle_downstream_distance_f32.argtypes = [p, i64, i64, ctypes.c_float, ci, p]
+    lib.oracle_downstream_distance_f64.argtypes = [p, i64, i64, ctypes.c_double, ci, p]
+    for suffix in MATERIAL_SUFFIX.values():
+        for op in ("upstream", "downstream"):
+            getattr(lib, f"oracle_{op}_{suffix}").argtypes = [p, p, i64, i64, p]
     _lib = lib
     return lib
 
@@ -175,6 +180,47 @@ def iterate_to_new_discharge(upstream, current, lateral, alpha, beta, dt, dx, dt
     lib = _load()
     sfx, _ = _dtype_suffix(dtype)
     return getattr(lib, f"oracle_iterate_to_new_discharge_{sfx}")(upstream, current, lateral, alpha, beta, dt, dx)
+
+
+# material element types of LUE's default build (environment/cmake/LueConfiguration.cmake:171-228)
+MATERIAL_SUFFIX = {np.dtype(np.uint8): "u8", np.dtype(np.uint32): "u32", np.dtype(np.uint64): "u64",
+                   np.dtype(np.int8): "i8", np.dtype(np.int32): "i32", np.dtype(np.int64): "i64",
+                   np.dtype(np.float32): "f32", np.dtype(np.float64): "f64"}
+
+
+def _neighbour_op(op, ldd, material):
+    lib = _load()
+    material = np.ascontiguousarray(material)
+    sfx = MATERIAL_SUFFIX[material.dtype]
+    ldd = np.ascontiguousarray(ldd, dtype=np.uint8)
+    if ldd.shape != material.shape:
+        raise ValueError("shapes differ")
+    rows, cols = material.shape
+    out = np.empty_like(material)
+    _check(getattr(lib, f"oracle_{op}_{sfx}")(ldd.ctypes.data, material.ctypes.data, rows, cols, out.ctypes.data))
+    return out
+
+
+def upstream(ldd, material):
+    """LUE's upstream (value policies): per cell, the sum of the material of the neighbours draining into it."""
+    return _neighbour_op("upstream", ldd, material)
+
+
+def downstream(ldd, material):
+    """LUE's downstream (value policies): per cell, the material of its downstream neighbour."""
+    return _neighbour_op("downstream", ldd, material)
+
+
+def downstream_distance(ldd, cell_size, dtype=np.float32, value_policies=True):
+    """LUE's downstream_distance: the distance to the downstream cell (value policies: no-data ldd -> NaN)."""
+    lib = _load()
+    sfx, _ = _dtype_suffix(dtype)
+    ldd = np.ascontiguousarray(ldd, dtype=np.uint8)
+    rows, cols = ldd.shape
+    out = np.empty((rows, cols), dtype=dtype)
+    _check(getattr(lib, f"oracle_downstream_distance_{sfx}")(ldd.ctypes.data, rows, cols, cell_size,
+                                                             int(value_policies), out.ctypes.data))
+    return out
 
 
 def d8_flow_direction(elevation, dtype=np.float32):

@@ -29,6 +29,20 @@ def raster(values, dtype):
     return np.array([[np.nan if v == "nan" else v for v in row] for row in values], dtype=dtype)
 
 
+def material_raster(values, dtype):
+    """JSON raster -> numpy array of a material element type; 'nan' is that type's no-data (NaN, or the
+    minimum of a signed / maximum of an unsigned integer type)."""
+    dtype = np.dtype(dtype)
+    if dtype.kind == "f":
+        return raster(values, dtype)
+    info = np.iinfo(dtype)
+    nd = info.min if dtype.kind == "i" else info.max
+    return np.array([[nd if v == "nan" else v for v in row] for row in values], dtype=dtype)
+
+
+MATERIAL_DTYPES = [np.uint8, np.uint32, np.uint64, np.int8, np.int32, np.int64, np.float32, np.float64]
+
+
 def golden_arg(arg, dtype):
     if "raster" in arg:
         return raster(arg["raster"], dtype)

@@ -8,6 +8,8 @@
 #include <functional>
 #include <limits>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -74,6 +76,57 @@ namespace {
         return true;
     }
 
+    // no-data of a material element: NaN, or the minimum (signed) / maximum (unsigned) integer; the cases
+    // write no-data as NaN whatever the element type
+    template<typename E>
+    E no_data_of()
+    {
+        if constexpr (std::is_floating_point_v<E>)
+            return std::numeric_limits<E>::quiet_NaN();
+        else if constexpr (std::is_signed_v<E>)
+            return std::numeric_limits<E>::min();
+        else
+            return std::numeric_limits<E>::max();
+    }
+
+    template<typename E>
+    E material_value(double v)
+    {
+        return std::isnan(v) ? no_data_of<E>() : static_cast<E>(v);
+    }
+
+    // upstream / downstream for one material element type
+    template<typename E>
+    bool run_flow_neighbour(Case const& c)
+    {
+        using namespace lue;
+        auto const ldd = make_raster<std::uint8_t>(c, c.ldd);
+        PartitionedArray<E, 2> material{c.shape, c.partition_shape};
+        std::vector<E> host;
+        for (double v : c.args[0].values)
+            host.push_back(material_value<E>(v));
+        material.from_host(host);
+        auto const got = std::string(c.kind) == "upstream" ? value_policies::upstream(ldd, material)
+                                                           : value_policies::downstream(ldd, material);
+        auto const values = got.to_host();
+        for (std::size_t i = 0; i < values.size(); ++i)
+        {
+            E const want = material_value<E>(c.expected[0][i]);
+            bool const same = std::is_floating_point_v<E>
+                                  ? ((std::isnan(static_cast<double>(values[i])) &&
+                                      std::isnan(static_cast<double>(want))) ||
+                                     values[i] == want)
+                                  : values[i] == want;
+            if (!same)
+            {
+                std::printf("  %s: cell %zu got %.17g want %.17g\n", c.name, i, static_cast<double>(values[i]),
+                            static_cast<double>(want));
+                return false;
+            }
+        }
+        return true;
+    }
+
     template<typename E>
     bool run(Case const& c)
     {
@@ -86,6 +139,26 @@ namespace {
             return compare(c, value_policies::d8_flow_direction<std::uint8_t>(elevation), c.expected[0]);
         }
         auto const ldd = make_raster<std::uint8_t>(c, c.ldd);
+        if (kind == "downstream_distance")
+        {
+            // the reference case holds no no-data ldd, so default and value policies agree. Its diagonal
+            // distance q = sqrt(2) * cell_size is written in float; in E it is sqrt(E{2}) * cell_size
+            E const cell_size = static_cast<E>(c.args[0].scalar);
+            float const q_float = std::sqrt(2.0f) * static_cast<float>(c.args[0].scalar);
+            auto const got = downstream_distance(ldd, cell_size).to_host();
+            for (std::size_t i = 0; i < got.size(); ++i)
+            {
+                double const v = c.expected[0][i];
+                E const want = static_cast<float>(v) == q_float ? std::sqrt(E{2}) * cell_size : static_cast<E>(v);
+                if (got[i] != want)
+                {
+                    std::printf("  %s: cell %zu got %.9g want %.9g\n", c.name, i, static_cast<double>(got[i]),
+                                static_cast<double>(want));
+                    return false;
+                }
+            }
+            return true;
+        }
         if (kind == "inflow_count")
         {
             auto const count = inflow_count<std::uint8_t>(ldd);
@@ -168,6 +241,32 @@ int main()
     int passed = 0;
     for (Case const& c : cases())
     {
+        std::string const kind = c.kind;
+        if (kind == "upstream" || kind == "downstream")
+        {
+            // LUE's cases through every default material element type
+            std::pair<char const*, std::function<bool(Case const&)>> const element_runs[] = {
+                {"uint8", run_flow_neighbour<std::uint8_t>},   {"uint32", run_flow_neighbour<std::uint32_t>},
+                {"uint64", run_flow_neighbour<std::uint64_t>}, {"int8", run_flow_neighbour<std::int8_t>},
+                {"int32", run_flow_neighbour<std::int32_t>},   {"int64", run_flow_neighbour<std::int64_t>},
+                {"float", run_flow_neighbour<float>},          {"double", run_flow_neighbour<double>}};
+            for (auto const& [element, fn] : element_runs)
+            {
+                bool ok;
+                try
+                {
+                    ok = fn(c);
+                }
+                catch (std::exception const& e)
+                {
+                    std::printf("  %s: exception %s\n", c.name, e.what());
+                    ok = false;
+                }
+                std::printf("%s %s %s %s\n", ok ? "PASS" : "FAIL", c.kind, c.name, element);
+                (ok ? passed : failed) += 1;
+            }
+            continue;
+        }
         for (int precision = 0; precision < 2; ++precision)
         {
             if (std::string(c.kind) == "inflow_count" && precision == 1)

@@ -19,6 +19,9 @@ Files read (relative to the reference root):
     source/framework/algorithm/test/kinematic_wave_test.cpp
     source/framework/algorithm/test/inflow_count_test.cpp
     source/framework/algorithm/test/d8_flow_direction_test.cpp
+    source/framework/algorithm/test/upstream_test.cpp
+    source/framework/algorithm/test/downstream_test.cpp
+    source/framework/algorithm/test/downstream_distance_test.cpp
 
 A partitioned literal lists one initializer per partition, partitions in row-major partition order
 (source/framework/test/include/lue/framework/test/array.hpp:76-105), each partition's cells row-major,
@@ -258,7 +261,7 @@ def split_statements(toks):
     return stmts
 
 
-DECL_RE = re.compile(r"^(?:auto const(?: &)?|auto|lue::Scalar < \w+ > const|Shape const|FloatingPoint const|"
+DECL_RE = re.compile(r"^(?:auto const(?: &)?|auto|lue::Scalar < \w+ > const|Shape const|Shape|FloatingPoint const|"
                      r"lue::ShapeT < [\w:]+ > const|InflowCount|FlowDirection)$")
 
 
@@ -488,6 +491,82 @@ def extract(ref_root):
             elif isinstance(key, str):
                 last[key] = value
     out["d8_flow_direction"] = cases
+
+    # upstream / downstream: (flow_direction, material, expected) of every check; the material element
+    # type is the case's `using MaterialElement = ...` (the no-data value x depends on it: NaN for floating
+    # point, the largest value for unsigned integers)
+    element_names = {"FloatingPointElement": "float32", "LargestUnsignedIntegralElement": "uint64"}
+    for op in ("upstream", "downstream"):
+        fname = f"{op}_test.cpp"
+        cases = []
+        for name, body in case_bodies(os.path.join(test_dir, fname)):
+            vals = [t[1] for t in body]
+            at = next(i for i in range(len(vals) - 2) if vals[i] == "using" and vals[i + 1] == "MaterialElement")
+            element_tokens = vals[at + 3:vals.index(";", at)]
+            element = next(element_names[t.split("::")[-1]] for t in element_tokens
+                           if t.split("::")[-1] in element_names)
+            env = {}
+            record = run_statements(split_statements(body), env, [])
+            last = {}
+            k = 0
+            for key, value in record:
+                if key == "equal":
+                    got_name, want_name = value
+                    _, opname, args = last[got_name]
+                    assert opname == op
+                    flow_direction, material = args
+                    k += 1
+                    cases.append({
+                        "name": name if k == 1 else f"{name}_{k}",
+                        "source": f"source/framework/algorithm/test/{fname}",
+                        "kind": op,
+                        "element": element,
+                        "array_shape": list(flow_direction.array_shape),
+                        "partition_shape": list(flow_direction.partition_shape),
+                        "ldd": to_json_raster(flow_direction),
+                        "material": to_json_raster(material),
+                        "expected": {"material": {"raster": to_json_raster(last[want_name]), "compare": "equal"}},
+                    })
+                elif isinstance(key, str):
+                    last[key] = value
+        out[op] = cases
+
+    # downstream_distance: the case declares `DistanceElement const cell_size{N}` and
+    # `q{std::sqrt(DistanceElement{2}) * cell_size}` with DistanceElement = float; q is evaluated here as the
+    # test defines it (float32 arithmetic), the rasters come from the literals
+    fname = "downstream_distance_test.cpp"
+    cases = []
+    for name, body in case_bodies(os.path.join(test_dir, fname)):
+        stmts = split_statements(body)
+        env = {}
+        rest = []
+        cell_size = None
+        for stmt in stmts:
+            vals = [t[1] for t in stmt]
+            if "cell_size" in vals[:4] and vals[:2] == ["DistanceElement", "const"]:
+                cell_size = float(vals[vals.index("{") + 1])
+            elif vals[:3] == ["auto", "const", "q"]:
+                continue
+            else:
+                rest.append(stmt)
+        import numpy as np  # float32 arithmetic of the test's own expression
+
+        env["q"] = float(np.sqrt(np.float32(2)) * np.float32(cell_size))
+        run_statements(rest, env, [])
+        flow_direction = env["flow_direction"]
+        cases.append({
+            "name": name,
+            "source": f"source/framework/algorithm/test/{fname}",
+            "kind": "downstream_distance",
+            "element": "float32",
+            "policies": "value",
+            "cell_size": cell_size,
+            "array_shape": list(flow_direction.array_shape),
+            "partition_shape": list(flow_direction.partition_shape),
+            "ldd": to_json_raster(flow_direction),
+            "expected": {"distance": {"raster": to_json_raster(env["result_we_want"]), "compare": "equal"}},
+        })
+    out["downstream_distance"] = cases
 
     # inflow count: test_inflow_count([policies,] flow_direction, expected)
     path = os.path.join(test_dir, "inflow_count_test.cpp")

@@ -210,3 +210,62 @@ def d8_flow_direction(elevation):
         best = np.where(take, drop, best)
     out[np.isnan(centre)] = 255
     return out
+
+
+def material_no_data(dtype):
+    """No-data of a material element: NaN for floating point, the minimum of a signed and the maximum of an
+    unsigned integer type."""
+    dtype = np.dtype(dtype)
+    if dtype.kind == "f":
+        return dtype.type(np.nan)
+    info = np.iinfo(dtype)
+    return dtype.type(info.min if dtype.kind == "i" else info.max)
+
+
+def _is_no_data(a):
+    return np.isnan(a) if a.dtype.kind == "f" else a == material_no_data(a.dtype)
+
+
+def upstream_by_pushing(ldd, material):
+    """Each cell pushes its material to its downstream neighbour (vectorised, one direction at a time, in the
+    order of the receiving cell's window: from NW, N, NE, W, E, SW, S, SE); a cell is no-data where its ldd or
+    material is, or where a no-data material arrives."""
+    ldd = np.asarray(ldd)
+    material = np.asarray(material)
+    rows, cols = material.shape
+    total = np.zeros_like(material)
+    poisoned = np.zeros(material.shape, dtype=bool)
+    source_nd = _is_no_data(material)
+    for code in (3, 2, 1, 6, 4, 9, 8, 7):
+        dr, dc = OFFSETS[code]
+        src = np.zeros((rows + 2, cols + 2), dtype=bool)
+        src[1:-1, 1:-1] = ldd == code
+        # cells (r, c) whose neighbour (r - dr, c - dc) drains into them
+        sent = src[1 - dr:1 - dr + rows, 1 - dc:1 - dc + cols]
+        value = np.zeros((rows + 2, cols + 2), dtype=material.dtype)
+        value[1:-1, 1:-1] = material
+        arriving = value[1 - dr:1 - dr + rows, 1 - dc:1 - dc + cols]
+        nd = np.zeros((rows + 2, cols + 2), dtype=bool)
+        nd[1:-1, 1:-1] = source_nd
+        poisoned |= sent & nd[1 - dr:1 - dr + rows, 1 - dc:1 - dc + cols]
+        with np.errstate(over="ignore", invalid="ignore"):
+            total = np.where(sent & ~poisoned, total + arriving, total)
+    out = total.astype(material.dtype)
+    out[poisoned | (ldd == ND) | source_nd] = material_no_data(material.dtype)
+    return out
+
+
+def downstream_by_pulling(ldd, material):
+    """Each cell takes the material of its downstream neighbour (0 beyond the array's edge), its own for a pit
+    or an unknown code; no-data where its ldd or material is, or where the neighbour's material is."""
+    ldd = np.asarray(ldd)
+    material = np.asarray(material)
+    rows, cols = material.shape
+    padded = np.zeros((rows + 2, cols + 2), dtype=material.dtype)
+    padded[1:-1, 1:-1] = material
+    out = material.copy()
+    for code, (dr, dc) in OFFSETS.items():
+        target = padded[1 + dr:1 + dr + rows, 1 + dc:1 + dc + cols]
+        out = np.where(ldd == code, target, out)
+    out[_is_no_data(out) | (ldd == ND) | _is_no_data(material)] = material_no_data(material.dtype)
+    return out

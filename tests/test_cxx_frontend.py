@@ -43,6 +43,21 @@ def write_cases(path):
                 f'Case{{"{c["name"]}", "{c["kind"]}", {{{rows}, {cols}}}, {{{prows}, {pcols}}}, {_vec(c["ldd"])}, '
                 f'{{{", ".join(args)}}}, {{{", ".join(_vec(e["raster"]) for e in exp)}}}, '
                 f'{"true" if close else "false"}, {"true" if c.get("property") == "all_positive" else "false"}}},')
+    for op in ("upstream", "downstream"):
+        for c in load_golden(op):
+            rows, cols = c["array_shape"]
+            prows, pcols = c["partition_shape"]
+            lines.append(
+                f'Case{{"{c["name"]}", "{op}", {{{rows}, {cols}}}, {{{prows}, {pcols}}}, {_vec(c["ldd"])}, '
+                f'{{Arg{{true, 0.0, {_vec(c["material"])}}}}}, {{{_vec(c["expected"]["material"]["raster"])}}}, '
+                f'false, false}},')
+    for c in load_golden("downstream_distance"):
+        rows, cols = c["array_shape"]
+        prows, pcols = c["partition_shape"]
+        lines.append(
+            f'Case{{"{c["name"]}", "downstream_distance", {{{rows}, {cols}}}, {{{prows}, {pcols}}}, '
+            f'{_vec(c["ldd"])}, {{Arg{{false, {_num(c["cell_size"])}, {{}}}}}}, '
+            f'{{{_vec(c["expected"]["distance"]["raster"])}}}, false, false}},')
     for c in load_golden("d8_flow_direction"):
         rows, cols = c["array_shape"]
         prows, pcols = c["partition_shape"]

@@ -1,0 +1,113 @@
+"""upstream / downstream on the GPU (lue_amd_upstream_*, lue_amd_downstream_*) against LUE's test vectors, the
+oracle and the push / pull models, for every default material element type."""
+import numpy as np
+import pytest
+
+import reference_models as rm
+from conftest import MATERIAL_DTYPES, load_golden, material_raster, raster
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def fw():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lue_amd import framework
+
+    return framework
+
+
+def pa(fw, array, partition_shape=None):
+    return fw.PartitionedArray.from_numpy(array, partition_shape)
+
+
+CASES = [c for op in ("upstream", "downstream") for c in load_golden(op)]
+
+
+@pytest.mark.parametrize("dtype", MATERIAL_DTYPES, ids=lambda d: np.dtype(d).name)
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['kind']}-{c['name']}")
+def test_flow_neighbour_golden(fw, case, dtype):
+    ldd = pa(fw, raster(case["ldd"], np.uint8), case["partition_shape"])
+    material = pa(fw, material_raster(case["material"], dtype), case["partition_shape"])
+    got = getattr(fw, case["kind"])(ldd, material)
+    assert got.partition_shape == ldd.partition_shape
+    want = material_raster(case["expected"]["material"]["raster"], dtype)
+    assert np.array_equal(got.numpy(), want, equal_nan=np.dtype(dtype).kind == "f")
+
+
+def _random_material(rng, shape, dtype):
+    dtype = np.dtype(dtype)
+    if dtype.kind == "f":
+        m = (rng.random(shape) * 10 - 2).astype(dtype)
+    else:
+        info = np.iinfo(dtype)
+        m = rng.integers(info.min, info.max, size=shape, dtype=dtype, endpoint=True)
+    m[rng.random(shape) < 0.05] = rm.material_no_data(dtype)
+    return m
+
+
+@pytest.mark.parametrize("dtype", MATERIAL_DTYPES, ids=lambda d: np.dtype(d).name)
+@pytest.mark.parametrize("shape", [(1, 1), (1, 70), (17, 5), (16, 64), (33, 129), (300, 257)])
+def test_flow_neighbour_random(fw, oracle_mod, dtype, shape):
+    """Random ldds (no-data, pits, unknown codes, flow off the edges) and material (no-data, wrapping integer
+    sums, float order of addition); partial tiles in both directions."""
+    rng = np.random.default_rng(shape[0] * 131 + shape[1])
+    ldd = rm.random_ldd(rng, *shape, nodata_fraction=0.05, pit_fraction=0.05, flat_fraction=0.3)
+    ldd[rng.random(shape) < 0.02] = 0
+    material = _random_material(rng, shape, dtype)
+    exact_nan = np.dtype(dtype).kind == "f"
+    d_ldd = pa(fw, ldd)
+    d_material = pa(fw, material)
+    up = fw.upstream(d_ldd, d_material).numpy()
+    assert np.array_equal(up, oracle_mod.upstream(ldd, material), equal_nan=exact_nan)
+    with np.errstate(over="ignore", invalid="ignore"):
+        assert np.array_equal(up, rm.upstream_by_pushing(ldd, material), equal_nan=exact_nan)
+    down = fw.downstream(d_ldd, d_material).numpy()
+    assert np.array_equal(down, oracle_mod.downstream(ldd, material), equal_nan=exact_nan)
+    assert np.array_equal(down, rm.downstream_by_pulling(ldd, material), equal_nan=exact_nan)
+
+
+def test_flow_neighbour_synthetic_network(fw, oracle_mod):
+    """The benchmark network generator (2051 x 3001): upstream of ones is the inflow count, downstream of the
+    cell index is the index of the downstream cell."""
+    from lue_amd import synthetic as syn
+
+    d_ldd = fw.PartitionedArray(syn.ldd_device(20261208, 2051, 3001))
+    ldd = d_ldd.numpy()
+    assert np.array_equal(ldd, syn.ldd_host(20261208, 2051, 3001))
+    ones = pa(fw, np.ones(ldd.shape, np.int32))
+    count = fw.upstream(d_ldd, ones).numpy()
+    assert np.array_equal(count, oracle_mod.upstream(ldd, np.ones(ldd.shape, np.int32)))
+    index = np.arange(ldd.size, dtype=np.int64).reshape(ldd.shape)
+    down = fw.downstream(d_ldd, pa(fw, index)).numpy()
+    assert np.array_equal(down, oracle_mod.downstream(ldd, index))
+
+
+def test_flow_neighbour_errors(fw):
+    ldd = pa(fw, np.full((8, 8), 2, np.uint8))
+    with pytest.raises(ValueError, match="Shapes differ"):
+        fw.upstream(ldd, pa(fw, np.ones((8, 9), np.float32)))
+    with pytest.raises(TypeError):
+        fw.downstream(ldd, pa(fw, np.ones((8, 8), np.float16)))
+    with pytest.raises(TypeError):
+        fw.upstream(ldd, 1.0)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(1, 1), (1, 3), (5, 7), (64, 64), (301, 257)])
+def test_downstream_distance(fw, oracle_mod, dtype, shape):
+    """Every code (no-data, pits, unknown codes), ragged tails (cells % 4), both policies."""
+    from conftest import load_golden, raster
+
+    rng = np.random.default_rng(shape[0] * 7 + shape[1])
+    ldd = rng.integers(0, 12, shape).astype(np.uint8)
+    ldd[rng.random(shape) < 0.1] = 255
+    got = fw.downstream_distance(pa(fw, ldd), 2.5, dtype).numpy()
+    assert got.dtype == dtype
+    assert np.array_equal(got, oracle_mod.downstream_distance(ldd, 2.5, dtype, True), equal_nan=True)
+    case = load_golden("downstream_distance")[0]
+    golden = fw.downstream_distance(pa(fw, raster(case["ldd"], np.uint8)), case["cell_size"], np.float32).numpy()
+    assert np.array_equal(golden, raster(case["expected"]["distance"]["raster"], np.float32))

@@ -5,7 +5,8 @@ import numpy as np
 import pytest
 
 import reference_models as rm
-from conftest import assert_close_boost, assert_equal_nan, golden_arg, load_golden, raster
+from conftest import (MATERIAL_DTYPES, assert_close_boost, assert_equal_nan, golden_arg, load_golden, material_raster,
+                      raster)
 
 
 # ---------------------------------------------------------------------------------------------------
@@ -217,3 +218,94 @@ def test_d8_flow_direction_brute_force(oracle_mod, dtype, seed):
     dem[rng.random(shape) < 0.05] = np.nan
     dem[:, 40:] += rng.random((shape[0], shape[1] - 40)).astype(dtype)
     assert np.array_equal(oracle_mod.d8_flow_direction(dem, dtype), rm.d8_flow_direction(dem))
+
+
+FLOW_NEIGHBOUR_CASES = [c for op in ("upstream", "downstream") for c in load_golden(op)]
+
+
+@pytest.mark.parametrize("dtype", MATERIAL_DTYPES, ids=lambda d: np.dtype(d).name)
+@pytest.mark.parametrize("case", FLOW_NEIGHBOUR_CASES, ids=lambda c: f"{c['kind']}-{c['name']}")
+def test_flow_neighbour_golden(oracle_mod, case, dtype):
+    """LUE's upstream / downstream cases, through every default material element type (the cases' values are
+    small integers; their no-data follows the element type)."""
+    ldd = raster(case["ldd"], np.uint8)
+    got = getattr(oracle_mod, case["kind"])(ldd, material_raster(case["material"], dtype))
+    want = material_raster(case["expected"]["material"]["raster"], dtype)
+    assert np.array_equal(got, want, equal_nan=np.dtype(dtype).kind == "f")
+
+
+def _random_material(rng, shape, dtype):
+    dtype = np.dtype(dtype)
+    if dtype.kind == "f":
+        m = (rng.random(shape) * 10 - 2).astype(dtype)  # negative values too: no domain check here
+    else:
+        info = np.iinfo(dtype)
+        m = rng.integers(info.min, info.max, size=shape, dtype=dtype, endpoint=True)  # sums wrap
+    m[rng.random(shape) < 0.05] = rm.material_no_data(dtype)
+    return m
+
+
+@pytest.mark.parametrize("dtype", MATERIAL_DTYPES, ids=lambda d: np.dtype(d).name)
+@pytest.mark.parametrize("seed", range(3))
+def test_flow_neighbour_brute_force(oracle_mod, dtype, seed):
+    """upstream / downstream against the push / pull models on random ldds (no-data, pits, codes that leave the
+    array, unknown codes) and random material (no-data, wrapping integer sums, float order of addition)."""
+    rng = np.random.default_rng(800 + seed)
+    shape = (23, 31)
+    ldd = rm.random_ldd(rng, *shape, nodata_fraction=0.05, pit_fraction=0.05, flat_fraction=0.3)
+    ldd[rng.random(shape) < 0.02] = 0  # no direction: downstream takes the cell's own material
+    material = _random_material(rng, shape, dtype)
+    exact = np.dtype(dtype).kind != "f"
+    up = oracle_mod.upstream(ldd, material)
+    with np.errstate(over="ignore", invalid="ignore"):
+        want_up = rm.upstream_by_pushing(ldd, material)
+    if exact:
+        assert np.array_equal(up, want_up)
+    else:
+        assert np.array_equal(up, want_up, equal_nan=True)  # same order of addition
+    down = oracle_mod.downstream(ldd, material)
+    assert np.array_equal(down, rm.downstream_by_pulling(ldd, material), equal_nan=not exact)
+
+
+def test_upstream_of_ones_is_the_inflow_count(oracle_mod):
+    """upstream(ldd, 1) counts the upstream neighbours: inflow_count (definition/inflow_count.hpp:670-672)."""
+    rng = np.random.default_rng(17)
+    ldd = rm.random_ldd(rng, 40, 37, nodata_fraction=0.05, flat_fraction=0.3)
+    count = oracle_mod.upstream(ldd, np.ones(ldd.shape, np.int32))
+    want = rm.inflow_count(ldd)
+    valid = ldd != 255
+    assert np.array_equal(count[valid], want[valid].astype(np.int32))
+
+
+@pytest.mark.parametrize("case", load_golden("downstream_distance"), ids=lambda c: c["name"])
+def test_downstream_distance_golden(oracle_mod, case):
+    ldd = raster(case["ldd"], np.uint8)
+    want = raster(case["expected"]["distance"]["raster"], np.float32)
+    for value_policies in (True, False):  # the case has no no-data ldd: both policies agree
+        got = oracle_mod.downstream_distance(ldd, case["cell_size"], np.float32, value_policies)
+        assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_downstream_distance_geometry(oracle_mod, dtype):
+    """The distance to the downstream cell is the length of the ldd's offset times the cell size (Euclidean,
+    rounded in the element type); no-data ldd: NaN under value policies, 0 otherwise."""
+    rng = np.random.default_rng(31)
+    ldd = rng.integers(0, 12, (19, 23)).astype(np.uint8)
+    ldd[rng.random(ldd.shape) < 0.1] = 255
+    cell_size = 2.5
+    got = oracle_mod.downstream_distance(ldd, cell_size, dtype, True)
+    for code in range(256):
+        sel = ldd == code
+        if not sel.any():
+            continue
+        if code == 255:
+            assert np.isnan(got[sel]).all()
+        elif code in rm.OFFSETS:
+            dr, dc = rm.OFFSETS[code]
+            length = np.sqrt(dtype(dr * dr + dc * dc)) * dtype(cell_size) if dr and dc else dtype(cell_size)
+            assert (got[sel] == length).all()
+        else:
+            assert (got[sel] == 0).all()
+    plain = oracle_mod.downstream_distance(ldd, cell_size, dtype, False)
+    assert (plain[ldd == 255] == 0).all()
