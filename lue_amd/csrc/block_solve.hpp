@@ -220,6 +220,74 @@ namespace lue_amd {
         }
     }
 
+    // Clamped sums: lane k holds x_k = ext(d, x_{k-1} + c), ext = max (Max) or min. A composition of such
+    // maps is one of the same form: (d, c) after (d_prev, c_prev) is (ext(d, d_prev + c), c_prev + c). A NaN
+    // in d or c gives NaN from there on (extremum_nan returns NaN for a NaN operand). On return
+    // x_k = ext(d, x_in + c), or ext(d, c) where flag is set (a chain start has no upstream value).
+    template<bool Max>
+    __device__ inline double extremum_nan(double a, double b)
+    {
+        double const nan = __longlong_as_double(0x7ff8000000000000ll);
+        if constexpr (Max)
+            return a >= b ? a : (b > a ? b : nan);
+        else
+            return a <= b ? a : (b < a ? b : nan);
+    }
+
+    template<bool Max>
+    __device__ inline void clamp_combine(double d_prev, double c_prev, int f_prev, double& d, double& c, int& flag)
+    {
+        if (!flag)
+        {
+            d = extremum_nan<Max>(d, d_prev + c);
+            c = c_prev + c;
+            flag = f_prev;
+        }
+    }
+
+    template<bool Max>
+    constexpr double clamp_identity()
+    {
+        return Max ? -__builtin_huge_val() : __builtin_huge_val();
+    }
+
+    template<bool Max, int N>
+    __device__ inline void clamp_step(double& d, double& c, int& flag)
+    {
+        double const d_prev = row_shr<N>(d, clamp_identity<Max>());
+        double const c_prev = row_shr<N>(c, 0.0);
+        int const f_prev = row_shr<N>(flag, 0);
+        clamp_combine<Max>(d_prev, c_prev, f_prev, d, c, flag);
+    }
+
+    template<bool Max>
+    __device__ inline void clamp_scan(double& d, double& c, int& flag)
+    {
+        clamp_step<Max, 1>(d, c, flag);
+        clamp_step<Max, 2>(d, c, flag);
+        clamp_step<Max, 4>(d, c, flag);
+        clamp_step<Max, 8>(d, c, flag);
+        double const e1_d = read_lane(d, 15);
+        double const e1_c = read_lane(c, 15);
+        int const e1_f = read_lane(flag, 15);
+        double e2_d = read_lane(d, 31);
+        double e2_c = read_lane(c, 31);
+        int e2_f = read_lane(flag, 31);
+        clamp_combine<Max>(e1_d, e1_c, e1_f, e2_d, e2_c, e2_f);
+        double e3_d = read_lane(d, 47);
+        double e3_c = read_lane(c, 47);
+        int e3_f = read_lane(flag, 47);
+        clamp_combine<Max>(e2_d, e2_c, e2_f, e3_d, e3_c, e3_f);
+        int const row = (threadIdx.x & 63) >> 4;
+        if (row > 0)
+        {
+            double const p_d = row == 1 ? e1_d : (row == 2 ? e2_d : e3_d);
+            double const p_c = row == 1 ? e1_c : (row == 2 ? e2_c : e3_c);
+            int const p_f = row == 1 ? e1_f : (row == 2 ? e2_f : e3_f);
+            clamp_combine<Max>(p_d, p_c, p_f, d, c, flag);
+        }
+    }
+
     // Outcomes of kinematic_wave_block.
     enum KwBlockResult : int
     {
@@ -386,7 +454,8 @@ namespace lue_amd {
     // Run the cell function along one block of up to 64 chain positions (lane j: position j; `nb` valid
     // lanes). Chains start where d.start is set; the first lane continues the chain of st.carry otherwise.
     // Linear functions are scans; the kinematic wave is a block Newton (kinematic_wave_block) with LUE's
-    // per-cell iteration for blocks that need it; threshold and fraction run serially with lane broadcasts.
+    // per-cell iteration for blocks that need it; threshold, capacity and fraction are scans of their cell maps;
+    // trigger (a step function) runs serially with lane broadcasts.
     // On return `value` (FP64 outflow) and `rem` hold this lane's results and st the state for the next
     // block.
     template<int OP, typename T>
@@ -405,8 +474,49 @@ namespace lue_amd {
             value = flag ? term : st.carry + term;
             st.carry = read_lane(value, nb - 1);
         }
-        else if constexpr (is_split_op(OP))
+        else if constexpr (OP == op_threshold || OP == op_capacity || OP == op_fraction)
         {
+            // The outflow as a function of the outflow u of the previous chain position is a clamped sum
+            // (threshold: max(0, u + extra + x - p), capacity: min(p, u + extra + x)) or an affine map
+            // (fraction: p (u + extra + x)). Compositions keep the form, so one segmented scan gives every
+            // lane its upstream outflow; the lane then runs the cell function on it (the reference's
+            // formula from there on; the scan only reorders the sums that lead to the upstream value).
+            int flag = d.valid && d.start ? 1 : 0;
+            double const ex = d.extra + static_cast<double>(d.in.x);  // NaN (no-data, domain) propagates
+            double const p = static_cast<double>(d.in.p);
+            double x;
+            if constexpr (OP == op_fraction)
+            {
+                double add = d.valid ? p * ex : 0.0;
+                double mul = d.valid ? p : 1.0;
+                affine_scan(add, mul, flag);
+                x = flag ? add : fma(mul, st.carry, add);
+            }
+            else
+            {
+                constexpr bool is_max = OP == op_threshold;
+                double dd = !d.valid ? clamp_identity<is_max>() : (is_max ? 0.0 : p);
+                double c = !d.valid ? 0.0 : (is_max ? ex - p : ex);
+                clamp_scan<is_max>(dd, c, flag);
+                x = extremum_nan<is_max>(dd, (flag ? 0.0 : st.carry) + c);
+            }
+            double prev = wave_shr1(x);
+            if (lane == 0)
+                prev = st.carry;
+            if (d.start)
+                prev = 0.0;
+            double const up = prev + d.extra;
+            double out = 0.0;
+            T r{};
+            if (d.valid)
+                solve_cell<OP, T>(a, d.in, up, isnan(up), out, r);
+            value = out;
+            rem = r;
+            st.carry = read_lane(value, nb - 1);
+        }
+        else if constexpr (OP == op_trigger)
+        {
+            // a step function of the upstream outflow: serial, with lane broadcasts
             for (int j = 0; j < nb; ++j)
             {
                 int const first = read_lane(static_cast<int>(d.start), j);
