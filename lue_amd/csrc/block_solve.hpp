@@ -74,9 +74,36 @@ namespace lue_amd {
         return __hiloint2double(hi, lo);
     }
 
+    // DPP row broadcasts across rows of 16 lanes: Ctrl 0x142 (row_bcast:15) with RowMask 0xA gives rows 1 and
+    // 3 the value of lane 15 of the row before them; Ctrl 0x143 (row_bcast:31) with RowMask 0xC gives rows 2
+    // and 3 the value of lane 31. Lanes of the other rows get `identity`.
+    template<int Ctrl, int RowMask>
+    __device__ inline double row_bcast(double v, double identity)
+    {
+        int const lo = __builtin_amdgcn_update_dpp(__double2loint(identity), __double2loint(v), Ctrl, RowMask, 0xF, false);
+        int const hi = __builtin_amdgcn_update_dpp(__double2hiint(identity), __double2hiint(v), Ctrl, RowMask, 0xF, false);
+        return __hiloint2double(hi, lo);
+    }
+
+    template<int Ctrl, int RowMask>
+    __device__ inline float row_bcast(float v, float identity)
+    {
+        return __int_as_float(
+            __builtin_amdgcn_update_dpp(__float_as_int(identity), __float_as_int(v), Ctrl, RowMask, 0xF, false));
+    }
+
+    template<int Ctrl, int RowMask>
+    __device__ inline int row_bcast(int v, int identity)
+    {
+        return __builtin_amdgcn_update_dpp(identity, v, Ctrl, RowMask, 0xF, false);
+    }
+
+    constexpr int dpp_row_bcast15 = 0x142;
+    constexpr int dpp_row_bcast31 = 0x143;
+
     // Segmented inclusive scans along the 64 lanes of a wavefront: Hillis-Steele inside rows of 16 lanes
-    // (DPP shifts by 1, 2, 4, 8), then the totals of the rows before (lanes 15, 31, 47, read into scalars)
-    // are applied to rows 1-3. `flag` marks lanes whose element does not combine with the lanes before
+    // (DPP shifts by 1, 2, 4, 8), then rows 1 and 3 take the prefix of the row before them (row_bcast:15)
+    // and rows 2 and 3 that of rows 0-1 (row_bcast:31); the values stay in vector registers. `flag` marks lanes whose element does not combine with the lanes before
     // them (chain starts); on return it is set where the result does not depend on the value before lane 0.
 
     // Affine maps: lane k holds x_k = add + mul x_{k-1}. On return x_k = add + mul x_in, x_in being the
@@ -106,29 +133,17 @@ namespace lue_amd {
         affine_step<2>(add, mul, flag);
         affine_step<4>(add, mul, flag);
         affine_step<8>(add, mul, flag);
-        // prefixes of rows 1, 2, 3: E1 = R0, E2 = R1 o E1, E3 = R2 o E2
-        double e_a = read_lane(add, 15);
-        double e_m = read_lane(mul, 15);
-        int e_f = read_lane(flag, 15);
-        double r_a = read_lane(add, 31);
-        double r_m = read_lane(mul, 31);
-        int r_f = read_lane(flag, 31);
-        double const e1_a = e_a, e1_m = e_m;
-        int const e1_f = e_f;
-        affine_combine(e1_a, e1_m, e1_f, r_a, r_m, r_f);
-        double const e2_a = r_a, e2_m = r_m;
-        int const e2_f = r_f;
-        double e3_a = read_lane(add, 47);
-        double e3_m = read_lane(mul, 47);
-        int e3_f = read_lane(flag, 47);
-        affine_combine(e2_a, e2_m, e2_f, e3_a, e3_m, e3_f);
-        int const row = (threadIdx.x & 63) >> 4;
-        if (row > 0)
         {
-            double const p_a = row == 1 ? e1_a : (row == 2 ? e2_a : e3_a);
-            double const p_m = row == 1 ? e1_m : (row == 2 ? e2_m : e3_m);
-            int const p_f = row == 1 ? e1_f : (row == 2 ? e2_f : e3_f);
-            affine_combine(p_a, p_m, p_f, add, mul, flag);
+            double const a_prev = row_bcast<dpp_row_bcast15, 0xA>(add, 0.0);
+            double const m_prev = row_bcast<dpp_row_bcast15, 0xA>(mul, 1.0);
+            int const f_prev = row_bcast<dpp_row_bcast15, 0xA>(flag, 0);
+            affine_combine(a_prev, m_prev, f_prev, add, mul, flag);
+        }
+        {
+            double const a_prev = row_bcast<dpp_row_bcast31, 0xC>(add, 0.0);
+            double const m_prev = row_bcast<dpp_row_bcast31, 0xC>(mul, 1.0);
+            int const f_prev = row_bcast<dpp_row_bcast31, 0xC>(flag, 0);
+            affine_combine(a_prev, m_prev, f_prev, add, mul, flag);
         }
     }
 
@@ -158,24 +173,17 @@ namespace lue_amd {
         affine_step_f32<2>(add, mul, flag);
         affine_step_f32<4>(add, mul, flag);
         affine_step_f32<8>(add, mul, flag);
-        float const e1_a = read_lane(add, 15);
-        float const e1_m = read_lane(mul, 15);
-        int const e1_f = read_lane(flag, 15);
-        float e2_a = read_lane(add, 31);
-        float e2_m = read_lane(mul, 31);
-        int e2_f = read_lane(flag, 31);
-        affine_combine_f32(e1_a, e1_m, e1_f, e2_a, e2_m, e2_f);
-        float e3_a = read_lane(add, 47);
-        float e3_m = read_lane(mul, 47);
-        int e3_f = read_lane(flag, 47);
-        affine_combine_f32(e2_a, e2_m, e2_f, e3_a, e3_m, e3_f);
-        int const row = (threadIdx.x & 63) >> 4;
-        if (row > 0)
         {
-            float const p_a = row == 1 ? e1_a : (row == 2 ? e2_a : e3_a);
-            float const p_m = row == 1 ? e1_m : (row == 2 ? e2_m : e3_m);
-            int const p_f = row == 1 ? e1_f : (row == 2 ? e2_f : e3_f);
-            affine_combine_f32(p_a, p_m, p_f, add, mul, flag);
+            float const a_prev = row_bcast<dpp_row_bcast15, 0xA>(add, 0.0f);
+            float const m_prev = row_bcast<dpp_row_bcast15, 0xA>(mul, 1.0f);
+            int const f_prev = row_bcast<dpp_row_bcast15, 0xA>(flag, 0);
+            affine_combine_f32(a_prev, m_prev, f_prev, add, mul, flag);
+        }
+        {
+            float const a_prev = row_bcast<dpp_row_bcast31, 0xC>(add, 0.0f);
+            float const m_prev = row_bcast<dpp_row_bcast31, 0xC>(mul, 1.0f);
+            int const f_prev = row_bcast<dpp_row_bcast31, 0xC>(flag, 0);
+            affine_combine_f32(a_prev, m_prev, f_prev, add, mul, flag);
         }
     }
 
@@ -203,20 +211,15 @@ namespace lue_amd {
         sum_step<2>(term, flag);
         sum_step<4>(term, flag);
         sum_step<8>(term, flag);
-        double const e1_t = read_lane(term, 15);
-        int const e1_f = read_lane(flag, 15);
-        double e2_t = read_lane(term, 31);
-        int e2_f = read_lane(flag, 31);
-        sum_combine(e1_t, e1_f, e2_t, e2_f);
-        double e3_t = read_lane(term, 47);
-        int e3_f = read_lane(flag, 47);
-        sum_combine(e2_t, e2_f, e3_t, e3_f);
-        int const row = (threadIdx.x & 63) >> 4;
-        if (row > 0)
         {
-            double const p_t = row == 1 ? e1_t : (row == 2 ? e2_t : e3_t);
-            int const p_f = row == 1 ? e1_f : (row == 2 ? e2_f : e3_f);
-            sum_combine(p_t, p_f, term, flag);
+            double const t_prev = row_bcast<dpp_row_bcast15, 0xA>(term, 0.0);
+            int const f_prev = row_bcast<dpp_row_bcast15, 0xA>(flag, 0);
+            sum_combine(t_prev, f_prev, term, flag);
+        }
+        {
+            double const t_prev = row_bcast<dpp_row_bcast31, 0xC>(term, 0.0);
+            int const f_prev = row_bcast<dpp_row_bcast31, 0xC>(flag, 0);
+            sum_combine(t_prev, f_prev, term, flag);
         }
     }
 
@@ -267,24 +270,17 @@ namespace lue_amd {
         clamp_step<Max, 2>(d, c, flag);
         clamp_step<Max, 4>(d, c, flag);
         clamp_step<Max, 8>(d, c, flag);
-        double const e1_d = read_lane(d, 15);
-        double const e1_c = read_lane(c, 15);
-        int const e1_f = read_lane(flag, 15);
-        double e2_d = read_lane(d, 31);
-        double e2_c = read_lane(c, 31);
-        int e2_f = read_lane(flag, 31);
-        clamp_combine<Max>(e1_d, e1_c, e1_f, e2_d, e2_c, e2_f);
-        double e3_d = read_lane(d, 47);
-        double e3_c = read_lane(c, 47);
-        int e3_f = read_lane(flag, 47);
-        clamp_combine<Max>(e2_d, e2_c, e2_f, e3_d, e3_c, e3_f);
-        int const row = (threadIdx.x & 63) >> 4;
-        if (row > 0)
         {
-            double const p_d = row == 1 ? e1_d : (row == 2 ? e2_d : e3_d);
-            double const p_c = row == 1 ? e1_c : (row == 2 ? e2_c : e3_c);
-            int const p_f = row == 1 ? e1_f : (row == 2 ? e2_f : e3_f);
-            clamp_combine<Max>(p_d, p_c, p_f, d, c, flag);
+            double const d_prev = row_bcast<dpp_row_bcast15, 0xA>(d, clamp_identity<Max>());
+            double const c_prev = row_bcast<dpp_row_bcast15, 0xA>(c, 0.0);
+            int const f_prev = row_bcast<dpp_row_bcast15, 0xA>(flag, 0);
+            clamp_combine<Max>(d_prev, c_prev, f_prev, d, c, flag);
+        }
+        {
+            double const d_prev = row_bcast<dpp_row_bcast31, 0xC>(d, clamp_identity<Max>());
+            double const c_prev = row_bcast<dpp_row_bcast31, 0xC>(c, 0.0);
+            int const f_prev = row_bcast<dpp_row_bcast31, 0xC>(flag, 0);
+            clamp_combine<Max>(d_prev, c_prev, f_prev, d, c, flag);
         }
     }
 
