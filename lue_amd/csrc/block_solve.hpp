@@ -363,7 +363,7 @@ namespace lue_amd {
             double step;
             if constexpr (std::is_same_v<T, double>)
             {
-                double const pw1 = pow_positive(x, beta - 1.0);  // x^(beta-1)
+                double const pw1 = pow_positive<false>(x, beta - 1.0);  // x^(beta-1)
                 double const f = c1 * (x - prev - d.extra) + alpha * x * pw1 - b;
                 double const fp = c1 + alpha * beta * pw1;
                 double inv = static_cast<double>(fast_rcp(static_cast<float>(fp)));

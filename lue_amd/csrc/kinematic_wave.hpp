@@ -61,7 +61,7 @@ namespace lue_amd {
     //   atanh series to s^15 leaves 2 |s|^17 / 17 < 1.1e-14;
     //   e^z = 2^k e^r, k = rint(z / ln 2), |r| <= ln 2 / 2 (ln 2 in two parts): Taylor to r^12 leaves
     //   |r|^13 / 13! < 2e-16.
-    // The coefficients live in constant memory, read with scalar loads where they are used: as literals the
+    // The coefficients live in constant memory, read with scalar loads in every call: as literals the
     // compiler keeps all of them in vector registers across the kernels' loops (and spills them there).
     // [0, 7]: 1/15, 1/13, ..., 1/3, 1 (atanh series); [8]: ln2_hi, [9]: ln2_lo (ln 2 = ln2_hi + ln2_lo,
     // double-double; the fused products below need no exact k ln2_hi), [10]: 1/ln 2; [11, 23]: 1/12!, ...,
@@ -72,10 +72,16 @@ namespace lue_amd {
         1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0, 1.0 / 362880.0, 1.0 / 40320.0, 1.0 / 5040.0,
         1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0, 0.5, 1.0, 1.0};
 
+    // Reload: read the table in every call (an opaque copy of its address keeps the scalar loads in place);
+    // otherwise the compiler hoists the loads out of the kernels' loops into scalar registers, which pays
+    // where the power runs in a dependent chain of Newton steps and the wave has registers to spare.
+    template<bool Reload = true>
     __host__ __device__ inline double pow_positive(double x, double y)
     {
 #if defined(__HIP_DEVICE_COMPILE__)
-        double const* const k = pow_positive_coefficients;
+        auto k = (__attribute__((address_space(4))) double const*)pow_positive_coefficients;
+        if constexpr (Reload)
+            asm volatile("" : "+s"(k));
 #else
         static constexpr double k[24] = {
             1.0 / 15.0, 1.0 / 13.0, 1.0 / 11.0, 1.0 / 9.0, 1.0 / 7.0, 1.0 / 5.0, 1.0 / 3.0, 1.0,
