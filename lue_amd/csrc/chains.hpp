@@ -505,20 +505,39 @@ namespace lue_amd {
         T* res_rem;                       // per position: remainder (threshold, fraction)
     };
 
-    // Add the side references of positions [pos_begin, pos_end) (one chain level) to their staged side sums, in
-    // reference order (neighbour order). The referenced values are final: raster cells solved by round 0,
-    // and chain cells of lower levels.
+    // has_ref[p] = 1 where position p has side references (p < n), has_ref[n] = 0.
+    __global__ inline void chains_has_ref_kernel(std::uint32_t const* ref_off, std::uint32_t n, std::uint32_t* has_ref)
+    {
+        std::uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+        if (p < n)
+            has_ref[p] = ref_off[p + 1] > ref_off[p] ? 1u : 0u;
+        else if (p == n)
+            has_ref[p] = 0u;
+    }
+
+    // ref_pos[rank[p]] = p for the positions with side references.
+    __global__ inline void chains_ref_pos_kernel(std::uint32_t const* has_ref, std::uint32_t const* rank, std::uint32_t n,
+                                                 std::uint32_t* ref_pos)
+    {
+        std::uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+        if (p < n && has_ref[p])
+            ref_pos[rank[p]] = p;
+    }
+
+    // Add the side references of the positions ref_pos[i_begin, i_end) (those of one chain level that have
+    // any) to their staged side sums, in reference order (neighbour order). The referenced values are final:
+    // raster cells solved by round 0, and chain cells of lower levels.
     template<int OP, typename T>
     __global__ void __launch_bounds__(256)
-        chains_gather_kernel(SweepArgs<T> a, ChainRun<OP, T> c, std::uint32_t pos_begin, std::uint32_t pos_end)
+        chains_gather_kernel(SweepArgs<T> a, ChainRun<OP, T> c, std::uint32_t const* ref_pos, std::uint32_t i_begin,
+                             std::uint32_t i_end)
     {
-        std::uint32_t const p = pos_begin + blockIdx.x * blockDim.x + threadIdx.x;
-        if (p >= pos_end)
+        std::uint32_t const i = i_begin + blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= i_end)
             return;
+        std::uint32_t const p = as_global(ref_pos)[i];
         std::uint32_t const begin = as_global(c.ref_off)[p];
         std::uint32_t const end = as_global(c.ref_off)[p + 1];
-        if (begin == end)
-            return;
         double sum = as_global(c.side)[p];
         for (std::uint32_t j = begin; j < end; ++j)
         {
@@ -557,12 +576,16 @@ namespace lue_amd {
         return d;
     }
 
-    // One level of chains. A wavefront per work item (4 per workgroup).
+    // One level of chains. A wavefront per work item (4 per workgroup). The data of the next `ahead` blocks is
+    // in flight while a block runs: 2 for the kinematic wave, whose block solve is long enough to cover the
+    // memory latency, 6 for the scans of the other operations. The ring of blocks rotates through a loop
+    // unrolled over it, so no load is waited for before its block is due.
     template<int OP, typename T>
     __global__ void __launch_bounds__(256)
         chains_run_kernel(SweepArgs<T> a, ChainRun<OP, T> c, std::uint32_t item_begin, std::uint32_t item_end)
     {
         constexpr bool two_outputs = is_split_op(OP);
+        constexpr int ahead = OP == op_kinematic_wave ? 2 : 6;
         int const lane = threadIdx.x & 63;
         std::uint32_t const item = item_begin + blockIdx.x * 4 + (threadIdx.x >> 6);
         if (item >= item_end)
@@ -570,25 +593,32 @@ namespace lue_amd {
         std::uint32_t const start = as_global(c.item_start)[item];
         std::uint32_t const count = as_global(c.item_count)[item];
         BlockState st;
-        // data of the next two blocks is in flight while a block runs
-        ChainLane<OP, T> next = load_chain_lane<OP, T>(a, c, start, count, lane);
-        ChainLane<OP, T> next2 = load_chain_lane<OP, T>(a, c, start, count, 64 + lane);
-        for (std::uint32_t block = 0; block < count; block += 64)
+        ChainLane<OP, T> ring[ahead];
+#pragma unroll
+        for (int k = 0; k < ahead; ++k)
+            ring[k] = load_chain_lane<OP, T>(a, c, start, count, 64 * k + lane);
+        for (std::uint32_t base = 0; base < count; base += 64 * ahead)
         {
-            ChainLane<OP, T> const d = next;
-            next = next2;
-            if (block + 128 < count)
-                next2 = load_chain_lane<OP, T>(a, c, start, count, block + 128 + lane);
-            int const nb = static_cast<int>(count - block < 64 ? count - block : 64);
-            double value;
-            T rem{};
-            solve_block<OP, T>(a, d, nb, st, value, rem);
-            if (d.valid)
+#pragma unroll
+            for (int k = 0; k < ahead; ++k)
             {
-                std::uint32_t const p = start + block + lane;
-                as_global(c.res)[p] = value;
-                if constexpr (two_outputs)
-                    as_global(c.res_rem)[p] = rem;
+                std::uint32_t const block = base + 64 * k;
+                if (block >= count)
+                    break;
+                ChainLane<OP, T> const d = ring[k];
+                if (block + 64 * ahead < count)
+                    ring[k] = load_chain_lane<OP, T>(a, c, start, count, block + 64 * ahead + lane);
+                int const nb = static_cast<int>(count - block < 64 ? count - block : 64);
+                double value;
+                T rem{};
+                solve_block<OP, T>(a, d, nb, st, value, rem);
+                if (d.valid)
+                {
+                    std::uint32_t const p = start + block + lane;
+                    as_global(c.res)[p] = value;
+                    if constexpr (two_outputs)
+                        as_global(c.res_rem)[p] = rem;
+                }
             }
         }
     }

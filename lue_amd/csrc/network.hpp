@@ -433,6 +433,8 @@ namespace lue_amd {
         DeviceArray<std::uint8_t> seg_start;     // per position
         DeviceArray<std::uint32_t> ref_off;      // per position (+1)
         DeviceArray<std::uint64_t> refs;
+        DeviceArray<std::uint32_t> ref_pos;      // the positions that have side references, level after level
+        std::vector<std::uint32_t> level_ref;    // those of level l: ref_pos[level_ref[l] .. level_ref[l+1])
         DeviceArray<std::uint32_t> item_start;
         DeviceArray<std::uint32_t> item_count;
         // in-tile chain layout of round 0 (tile_layout.hpp)
@@ -456,6 +458,7 @@ namespace lue_amd {
             seg_start.stream = nullptr;
             ref_off.stream = nullptr;
             refs.stream = nullptr;
+            ref_pos.stream = nullptr;
             item_start.stream = nullptr;
             item_count.stream = nullptr;
             tile_order.stream = nullptr;
@@ -913,6 +916,29 @@ namespace lue_amd {
                            pending_bits, cells.ptr, index_of.ptr, heavy.ptr, net.pos_of.ptr, n, true, nullptr,
                            net.ref_off.ptr, net.refs.ptr);
         LUE_AMD_HIP(hipGetLastError());
+
+        // the positions with side references, in position order (so level after level): the gather of a level
+        // visits only those
+        {
+            DeviceArray<std::uint32_t> has_ref;
+            DeviceArray<std::uint32_t> ref_rank;
+            LUE_AMD_HIP(has_ref.allocate(n + 1, stream));
+            LUE_AMD_HIP(ref_rank.allocate(n + 1, stream));
+            hipLaunchKernelGGL(chains_has_ref_kernel, dim3(blocks_for(n + 1)), dim3(256), 0, stream, net.ref_off.ptr, n,
+                               has_ref.ptr);
+            LUE_AMD_HIP(hipGetLastError());
+            LUE_AMD_HIP(exclusive_scan(has_ref.ptr, ref_rank.ptr, n + 1, stream));
+            std::uint32_t nref_pos = 0;
+            LUE_AMD_HIP(copy_to_host(&nref_pos, ref_rank.ptr + n, 1, stream));
+            LUE_AMD_HIP(net.ref_pos.allocate(std::max<std::uint32_t>(nref_pos, 1), stream));
+            hipLaunchKernelGGL(chains_ref_pos_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, has_ref.ptr,
+                               ref_rank.ptr, n, net.ref_pos.ptr);
+            LUE_AMD_HIP(hipGetLastError());
+            net.level_ref.assign(net.level_pos.size(), 0);
+            for (std::size_t l = 0; l < net.level_pos.size(); ++l)
+                LUE_AMD_HIP(copy_to_host(&net.level_ref[l], ref_rank.ptr + net.level_pos[l], 1, stream));
+            LUE_AMD_HIP(hipStreamSynchronize(stream));  // before the temporaries go
+        }
         LUE_AMD_HIP(hipStreamSynchronize(stream));  // host vectors go out of scope
         return status_success;
     }
@@ -978,12 +1004,12 @@ namespace lue_amd {
                           reinterpret_cast<T*>(w.res_rem.ptr)};
         for (int l = 0; l < net.levels(); ++l)
         {
-            std::uint32_t const pos_begin = net.level_pos[l];
-            std::uint32_t const pos_end = net.level_pos[l + 1];
-            if (pos_end > pos_begin)
+            std::uint32_t const ref_begin = net.level_ref[l];
+            std::uint32_t const ref_end = net.level_ref[l + 1];
+            if (ref_end > ref_begin)
             {
-                hipLaunchKernelGGL((chains_gather_kernel<OP, T>), dim3(blocks_for(pos_end - pos_begin)), dim3(256), 0,
-                                   stream, a, c, pos_begin, pos_end);
+                hipLaunchKernelGGL((chains_gather_kernel<OP, T>), dim3(blocks_for(ref_end - ref_begin)), dim3(256), 0,
+                                   stream, a, c, net.ref_pos.ptr, ref_begin, ref_end);
             }
             std::uint32_t const begin = net.level_items[l];
             std::uint32_t const end = net.level_items[l + 1];
