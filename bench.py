@@ -61,6 +61,9 @@ def parse():
     p.add_argument("--kw-tile", default="32x32", help="sweep tile of the kinematic-wave network (WxH)")
     p.add_argument("--schedule", default="chains", choices=["chains", "rounds"])
     p.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic runs")
+    p.add_argument("--serving-seconds", type=float, default=5.0,
+                   help="length of the serving run behind p99_request_ms (200 resident catchments, Poisson "
+                        "1000 requests/s; 0: skip; --extras runs it for 10 s)")
     p.add_argument("--extras", action="store_true",
                    help="also run the batch (512 catchments, accu_threshold, autotuned), large-domain (32768^2, "
                         "24 resident timesteps) and serving (200 catchments, Poisson 1000/s) workloads")
@@ -326,7 +329,8 @@ def main():
         del host, ldd_h
 
     extras = None
-    if args.extras:
+    serving_result = None
+    if args.extras or args.serving_seconds > 0:
         from lue_amd import workloads
 
         del f64, out64, out32
@@ -334,6 +338,11 @@ def main():
         net_kw.close()
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
+    if args.serving_seconds > 0 and not args.extras:
+        log(f"[bench] serving {args.serving_seconds:g} s")
+        serving_result = workloads.serving_run(duration=args.serving_seconds)
+        torch.cuda.empty_cache()
+    if args.extras:
         extras = {}
         log("[bench] extras: d8_flow_direction 16384^2")
         extras["d8_flow_direction"] = workloads.d8_flow_direction()
@@ -345,7 +354,7 @@ def main():
         extras["large_domain"] = workloads.large_domain()
         torch.cuda.empty_cache()
         log("[bench] extras: serving")
-        extras["serving"] = workloads.serving_run()
+        extras["serving"] = serving_result = workloads.serving_run()
         torch.cuda.empty_cache()
 
     result = {
@@ -382,6 +391,10 @@ def main():
         "traffic": traffic,
         "step_hbm_gbs": round(cells * (BYTES_ACCU_F32 + BYTES_KW_F32) / t32 / 1e9, 1),
     }
+    if serving_result is not None:
+        result["p99_request_ms"] = round(serving_result["p99_ms"], 3)
+        result["serving"] = {k: serving_result[k] for k in ("workload", "requests", "p50_ms", "p99_ms", "mean_ms",
+                                                            "mcells_per_s", "resident_cells", "wall_s")}
     if extras is not None:
         result["extras"] = extras
     print(json.dumps(result), flush=True)
