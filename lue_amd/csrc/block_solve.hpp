@@ -303,7 +303,7 @@ namespace lue_amd {
     // so the iterates approach the root from below after the first step. The iteration stops when every
     // |dQ_k| <= |Q_k| 2^(1 - digits), LUE's criterion (digits 31 for double, 14 for float).
     // T = float: the iterations use the hardware FP32 power (log2, exp2) and approximate reciprocals; one
-    // FP64 Newton step follows (the FP64 residual correction).
+    // Newton step on the FP64 residual follows (the FP64 residual correction; its scan in FP32).
     // Returns kw_block_serial, with q untouched, when a lane needs LUE's per-cell iteration: no-data, a zero
     // current discharge (dry cells and cells whose iteration starts from numeric_limits::min()), a non-positive
     // iterate, or no convergence within 16 steps; kw_block_lue_far when LUE's own iteration would stop short
@@ -396,7 +396,9 @@ namespace lue_amd {
             return kw_block_serial;
         if constexpr (std::is_same_v<T, float>)
         {
-            // FP64 residual correction: one Newton step with the FP64 power and Jacobian
+            // FP64 residual correction: one Newton step on the FP64 residual (FP64 power). The step is about
+            // 1e-7 of x, so its Jacobian and scan in FP32 (relative error 1e-7 of the step) leave x within
+            // about 1e-14 of the FP64 step's result
             double prev = wave_shr1(x);
             if (lane == 0)
                 prev = carry;
@@ -404,15 +406,12 @@ namespace lue_amd {
                 prev = 0.0;
             double const pw1 = pow_positive(x, beta - 1.0);
             double const f = c1 * (x - prev - d.extra) + alpha * x * pw1 - b;
-            double const fp = c1 + alpha * beta * pw1;
-            double inv = static_cast<double>(fast_rcp(static_cast<float>(fp)));
-            inv = inv * fma(-fp, inv, 2.0);
-            inv = inv * fma(-fp, inv, 2.0);
-            double add = valid ? -f * inv : 0.0;
-            double mul = valid ? c1 * inv : 0.0;
+            float const inv = fast_rcp(c1f + ab32 * static_cast<float>(pw1));
+            float add = valid ? static_cast<float>(-f) * inv : 0.0f;
+            float mul = valid ? c1f * inv : 0.0f;
             int flag = (start || !valid) ? 1 : 0;
-            affine_scan(add, mul, flag);
-            double const xn = x + add;
+            affine_scan_f32(add, mul, flag);
+            double const xn = x + static_cast<double>(add);
             if (__builtin_expect(__ballot(valid && !(xn > 0.0)) != 0, 0))
                 return kw_block_serial;
             x = xn;
