@@ -74,6 +74,11 @@ namespace lue_amd {
         return __hiloint2double(hi, lo);
     }
 
+    __device__ inline float wave_shr1(float v)
+    {
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+    }
+
     // DPP row broadcasts across rows of 16 lanes: Ctrl 0x142 (row_bcast:15) with RowMask 0xA gives rows 1 and
     // 3 the value of lane 15 of the row before them; Ctrl 0x143 (row_bcast:31) with RowMask 0xC gives rows 2
     // and 3 the value of lane 31. Lanes of the other rows get `identity`.
@@ -353,16 +358,23 @@ namespace lue_amd {
         // step scan in FP32 for T = float, FP64 for T = double
         constexpr double tol = std::is_same_v<T, double> ? 0x1p-30 : 0x1p-13;
         bool converged = false;
+        // T = float: the residual in FP32 as well; its rounding (2^-24 of the largest term) moves the iterate by
+        // at most about 2^-24 of Q (the Jacobian is at least c1), far below the stop criterion, and the FP64
+        // residual correction below removes it
+        float const carry32 = static_cast<float>(carry);
+        float const extra32 = static_cast<float>(d.extra);
+        float const alpha32 = static_cast<float>(alpha);
+        float const b32 = static_cast<float>(b);
         for (int iteration = 0; iteration < 16 && !converged; ++iteration)
         {
-            double prev = wave_shr1(x);
-            if (lane == 0)
-                prev = carry;
-            if (start)
-                prev = 0.0;
             double step;
             if constexpr (std::is_same_v<T, double>)
             {
+                double prev = wave_shr1(x);
+                if (lane == 0)
+                    prev = carry;
+                if (start)
+                    prev = 0.0;
                 double const pw1 = pow_positive<false>(x, beta - 1.0);  // x^(beta-1)
                 double const f = c1 * (x - prev - d.extra) + alpha * x * pw1 - b;
                 double const fp = c1 + alpha * beta * pw1;
@@ -377,10 +389,16 @@ namespace lue_amd {
             }
             else
             {
-                float const pw1 = fast_exp2((beta32 - 1.0f) * fast_log2(static_cast<float>(x)));
-                double const f = c1 * (x - prev - d.extra) + alpha * x * static_cast<double>(pw1) - b;
+                float const x32 = static_cast<float>(x);
+                float prev32 = wave_shr1(x32);
+                if (lane == 0)
+                    prev32 = carry32;
+                if (start)
+                    prev32 = 0.0f;
+                float const pw1 = fast_exp2((beta32 - 1.0f) * fast_log2(x32));
+                float const f = c1f * (x32 - prev32 - extra32) + alpha32 * x32 * pw1 - b32;
                 float const inv = fast_rcp(c1f + ab32 * pw1);
-                float add = valid ? static_cast<float>(-f) * inv : 0.0f;
+                float add = valid ? -f * inv : 0.0f;
                 float mul = valid ? c1f * inv : 0.0f;
                 int flag = (start || !valid) ? 1 : 0;
                 affine_scan_f32(add, mul, flag);
