@@ -302,13 +302,19 @@ namespace lue_amd {
                         int const heavy = (f & 16u) ? ((f >> 1) & 7) : -1;
                         int const lr = i / TW;
                         int const lc = i % TW;
-                        std::uint8_t const mask = s.upmask[i];
+                        // the light upstream neighbours, in neighbour order (only the set bits: most cells
+                        // have one or two)
+                        unsigned m = s.upmask[i];
+                        if (heavy >= 0)
+                            m &= ~(1u << heavy);
                         double extra = 0.0;
-                        for (int k = 0; k < 8; ++k)
+                        while (m != 0u)
                         {
+                            int const k = __builtin_ctz(m);
+                            m &= m - 1u;
                             int const ur = lr + neighbour_dr(k);
                             int const uc = lc + neighbour_dc(k);
-                            if (((mask >> k) & 1u) && k != heavy && ur >= 0 && ur < TH && uc >= 0 && uc < TW)
+                            if (ur >= 0 && ur < TH && uc >= 0 && uc < TW)
                             {
                                 int const u = ur * TW + uc;
                                 double v = static_cast<double>(s.hi[u]);
