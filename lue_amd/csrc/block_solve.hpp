@@ -331,21 +331,22 @@ namespace lue_amd {
         float const beta32 = static_cast<float>(d.in.beta);
         float const ab32 = static_cast<float>(d.in.alpha) * beta32;
 
-        // start: F linearised at Q_cur gives Q_k = w_k (Q_{k-1} + extra_k) + g_k
+        // start: F linearised at Q_cur gives Q_k = w_k (Q_{k-1} + extra_k) + g_k, in FP32 (a starting point:
+        // b - (1 - beta) alpha Q_cur^beta = beta alpha Q_cur^beta + dt q_lat does not cancel)
         double x;
+        float pc1 = 1.0f;  // Q_cur^(beta-1)
         {
             float add = 0.0f;
             float mul = 0.0f;
             int flag = (start || !valid) ? 1 : 0;
-            double g = 0.0;
             if (valid)
             {
-                float const pc1 = fast_exp2((beta32 - 1.0f) * fast_log2(static_cast<float>(q_cur)));  // Q_cur^(beta-1)
+                float const q_cur32 = static_cast<float>(q_cur);
+                pc1 = fast_exp2((beta32 - 1.0f) * fast_log2(q_cur32));
                 float const inv = fast_rcp(c1f + ab32 * pc1);
                 mul = c1f * inv;
-                g = static_cast<double>(mul) * d.extra +
-                    (b - (1.0 - beta) * alpha * q_cur * static_cast<double>(pc1)) * static_cast<double>(inv);
-                add = static_cast<float>(g);
+                add = mul * static_cast<float>(d.extra) +
+                      (static_cast<float>(b) - (1.0f - beta32) * static_cast<float>(alpha) * q_cur32 * pc1) * inv;
             }
             affine_scan_f32(add, mul, flag);
             x = valid ? (flag ? static_cast<double>(add) : fma(static_cast<double>(mul), carry, static_cast<double>(add)))
@@ -444,7 +445,7 @@ namespace lue_amd {
                 prev = 0.0;
             float dtq;  // dt q_lat; for T = float recovered from b with the hardware power (enough for a check)
             if constexpr (std::is_same_v<T, float>)
-                dtq = static_cast<float>(b - alpha * static_cast<double>(fast_exp2(beta32 * fast_log2(d.in.q_cur))));
+                dtq = static_cast<float>(b - alpha * q_cur * static_cast<double>(pc1));
             else
                 dtq = static_cast<float>(dt * d.in.q_lat);
             bool const far = valid && !lue_guess_converges(static_cast<float>(prev + d.extra), static_cast<float>(q_cur),
