@@ -76,7 +76,7 @@ namespace lue_amd {
         T rem[two_outputs ? G::n : 1];
         TileScheduleLds<LAYOUT, G::n> sched;
         std::uint8_t ldd[G::hn];
-        std::uint8_t upmask[G::n];
+        alignas(4) std::uint8_t upmask[G::n];  // bit k: neighbour k drains into the cell (set by word-wide ORs)
         std::uint8_t state[G::n];  // see state_* below
         std::uint32_t pword[G::words];   // pending bitmap of the tile
         std::uint32_t pprefix[G::words]; // pending cells before each bitmap word
@@ -86,6 +86,25 @@ namespace lue_amd {
         unsigned int taken;
     };
 
+
+    // The cell at tile coordinates (lr, lc) (the halo ring included, -1 .. TW / TH) with ldd `code` drains into
+    // its downstream neighbour t: if t is a cell of the tile and not no-data, set bit opposite(direction) of
+    // t's upstream mask (an LDS OR on the word holding four masks).
+    template<int TW, int TH>
+    __device__ inline void mark_upstream(std::uint8_t const* ldd, std::uint8_t* upmask, int lr, int lc,
+                                         std::uint8_t code)
+    {
+        constexpr int hw = TW + 2;
+        int const dir = ldd_direction(code);
+        if (dir < 0)
+            return;
+        int const tr = lr + neighbour_dr(dir);
+        int const tc = lc + neighbour_dc(dir);
+        if (tr < 0 || tr >= TH || tc < 0 || tc >= TW || ldd[(tr + 1) * hw + (tc + 1)] == no_data_ldd)
+            return;
+        int const t = tr * TW + tc;
+        atomicOr(reinterpret_cast<unsigned int*>(upmask) + (t >> 2), (1u << opposite(dir)) << (8 * (t & 3)));
+    }
 
     // Cell states inside a round.
     constexpr std::uint8_t state_absent = 0;   // outside the array or ldd no-data
@@ -196,6 +215,8 @@ namespace lue_amd {
             if (i < G::hn)
                 s.ldd[i] = halo[j];
         }
+        for (int w = tid; w < G::n / 4; w += nthreads)
+            reinterpret_cast<std::uint32_t*>(s.upmask)[w] = 0u;
         __syncthreads();
 
         // 2. state, upstream mask and staged inputs of pending cells
@@ -210,7 +231,6 @@ namespace lue_amd {
             int const h = (lr + 1) * G::hw + (lc + 1);
             std::uint8_t const code = s.ldd[h];
             std::uint8_t st;
-            std::uint8_t mask = 0;
             if (code == no_data_ldd)
             {
                 st = state_absent;
@@ -232,14 +252,7 @@ namespace lue_amd {
             {
                 bool const pending = round == 0 || ((pbits[i >> 5] >> (i & 31)) & 1u);
                 st = pending ? state_pending : state_solved;
-                for (int k = 0; k < 8; ++k)
-                {
-                    std::uint8_t const ucode = s.ldd[h + neighbour_dr(k) * G::hw + neighbour_dc(k)];
-                    if (ucode != no_data_ldd && ldd_direction(ucode) == opposite(k))
-                    {
-                        mask |= static_cast<std::uint8_t>(1u << k);
-                    }
-                }
+                mark_upstream<TW, TH>(s.ldd, s.upmask, lr, lc, code);
                 if (round == 0)
                 {
                     stage_raw<OP, T>(a, raw[j], s.in[i]);
@@ -251,7 +264,6 @@ namespace lue_amd {
                 }
             }
             s.state[i] = st;
-            s.upmask[i] = mask;
             if constexpr (!layout)
             {
                 int next = -1;
@@ -269,6 +281,30 @@ namespace lue_amd {
                 }
                 s.sched.next[i] = static_cast<std::int16_t>(next);
             }
+        }
+        // cells of the halo ring (neighbouring tiles) that drain into this tile
+        constexpr int ring = 2 * (TW + 2) + 2 * TH;
+        for (int q = tid; q < ring; q += nthreads)
+        {
+            int hr;
+            int hc;
+            if (q < TW + 2)
+            {
+                hr = 0;
+                hc = q;
+            }
+            else if (q < 2 * (TW + 2))
+            {
+                hr = TH + 1;
+                hc = q - (TW + 2);
+            }
+            else
+            {
+                int const side = q - 2 * (TW + 2);
+                hr = 1 + (side >> 1);
+                hc = (side & 1) ? TW + 1 : 0;
+            }
+            mark_upstream<TW, TH>(s.ldd, s.upmask, hr - 1, hc - 1, s.ldd[hr * G::hw + hc]);
         }
         __syncthreads();
 
@@ -608,14 +644,15 @@ namespace lue_amd {
                 std::uint32_t const pos = pos_of[j];
                 int const lr = i / TW;
                 int const lc = i % TW;
-                std::uint8_t const mask = s.upmask[i];
+                unsigned m = s.upmask[i];
                 double side = 0.0;
-                for (int k = 0; k < 8; ++k)
+                while (m != 0u)  // the set bits only, in neighbour order
                 {
+                    int const k = __builtin_ctz(m);
+                    m &= m - 1u;
                     int const ur = lr + neighbour_dr(k);
                     int const uc = lc + neighbour_dc(k);
-                    if (((mask >> k) & 1u) && ur >= 0 && ur < TH && uc >= 0 && uc < TW &&
-                        s.state[ur * TW + uc] == state_new)
+                    if (ur >= 0 && ur < TH && uc >= 0 && uc < TW && s.state[ur * TW + uc] == state_new)
                     {
                         int const u = ur * TW + uc;
                         double v = static_cast<double>(s.hi[u]);
