@@ -658,18 +658,44 @@ namespace lue_amd {
         std::int64_t const local = tile - cat.tile_base;
         std::int64_t const r0 = (local / cat.tiles_x) * TH;
         std::int64_t const c0 = (local % cat.tiles_x) * TW;
-        for (int i = threadIdx.x; i < G::n; i += blockDim.x)
+        // all loads of a thread's cells in flight together: positions, then results (random reads), then stores
+        constexpr int per_thread = G::n / 256;
+        static_assert(G::n % 256 == 0, "the scatter runs 256 threads per tile");
+        std::uint32_t pos[per_thread];
+        bool pending[per_thread];
+#pragma unroll
+        for (int j = 0; j < per_thread; ++j)
         {
+            int const i = static_cast<int>(threadIdx.x) + 256 * j;
             std::uint32_t const word = words[i >> 5];
             std::uint32_t const b = static_cast<std::uint32_t>(i & 31);
-            if (!((word >> b) & 1u))
+            pending[j] = ((word >> b) & 1u) != 0u;
+            pos[j] = 0;
+            if (pending[j])
+                pos[j] = as_global(pos_of)[begin + prefix[i >> 5] + __popc(word & ((1u << b) - 1u))];
+        }
+        double value[per_thread];
+        T remainder[per_thread];
+#pragma unroll
+        for (int j = 0; j < per_thread; ++j)
+        {
+            if (pending[j])
+            {
+                value[j] = as_global(res)[pos[j]];
+                if constexpr (two_outputs)
+                    remainder[j] = as_global(res_rem)[pos[j]];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < per_thread; ++j)
+        {
+            if (!pending[j])
                 continue;
-            std::uint32_t const rank = prefix[i >> 5] + __popc(word & ((1u << b) - 1u));
-            std::uint32_t const p = as_global(pos_of)[begin + rank];
+            int const i = static_cast<int>(threadIdx.x) + 256 * j;
             std::int64_t const g = (r0 + i / TW) * cat.cols + (c0 + i % TW);
-            as_global(io.out)[g] = static_cast<T>(as_global(res)[p]);
+            as_global(io.out)[g] = static_cast<T>(value[j]);
             if constexpr (two_outputs)
-                as_global(io.rem)[g] = as_global(res_rem)[p];
+                as_global(io.rem)[g] = remainder[j];
         }
     }
 
