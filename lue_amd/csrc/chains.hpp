@@ -524,39 +524,65 @@ namespace lue_amd {
             ref_pos[rank[p]] = p;
     }
 
+    // ref_first[i] = ref_off[ref_pos[i]] for the nref_pos positions with references, ref_first[nref_pos] =
+    // ref_off[n]: the positions between two of them have none, so entry i's references end where entry i + 1's
+    // begin.
+    __global__ inline void chains_ref_first_kernel(std::uint32_t const* ref_pos, std::uint32_t const* ref_off,
+                                                   std::uint32_t nref_pos, std::uint32_t n, std::uint32_t* ref_first)
+    {
+        std::uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < nref_pos)
+            ref_first[i] = ref_off[ref_pos[i]];
+        else if (i == nref_pos)
+            ref_first[i] = ref_off[n];
+    }
+
     // Add the side references of the positions ref_pos[i_begin, i_end) (those of one chain level that have
     // any) to their staged side sums, in reference order (neighbour order). The referenced values are final:
     // raster cells solved by round 0, and chain cells of lower levels.
     template<int OP, typename T>
     __global__ void __launch_bounds__(256)
-        chains_gather_kernel(SweepArgs<T> a, ChainRun<OP, T> c, std::uint32_t const* ref_pos, std::uint32_t i_begin,
-                             std::uint32_t i_end)
+        chains_gather_kernel(SweepArgs<T> a, ChainRun<OP, T> c, std::uint32_t const* ref_pos,
+                             std::uint32_t const* ref_first, std::uint32_t i_begin, std::uint32_t i_end)
     {
         std::uint32_t const i = i_begin + blockIdx.x * blockDim.x + threadIdx.x;
         if (i >= i_end)
             return;
         std::uint32_t const p = as_global(ref_pos)[i];
-        std::uint32_t const begin = as_global(c.ref_off)[p];
-        std::uint32_t const end = as_global(c.ref_off)[p + 1];
-        double sum = as_global(c.side)[p];
-        for (std::uint32_t j = begin; j < end; ++j)
+        std::uint32_t const begin = as_global(ref_first)[i];
+        std::uint32_t const end = as_global(ref_first)[i + 1];
+        auto value_of = [&](std::uint64_t ref) -> double
         {
-            std::uint64_t const ref = as_global(c.refs)[j];
-            double v;
             if (ref & ref_position_tag)
-            {
-                v = as_global(c.res)[static_cast<std::uint32_t>(ref)];
-            }
-            else
-            {
-                std::uint32_t const ci = static_cast<std::uint32_t>(ref >> 32);
-                std::uint32_t const u = static_cast<std::uint32_t>(ref);
-                v = static_cast<double>(as_global(a.io[ci].out)[u]);
-                if constexpr (std::is_same_v<T, float>)
-                    v += static_cast<double>(as_global(a.out_lo)[a.cats[ci].cell_base + u]);
-            }
-            sum += v;
+                return as_global(c.res)[static_cast<std::uint32_t>(ref)];
+            std::uint32_t const ci = static_cast<std::uint32_t>(ref >> 32);
+            std::uint32_t const u = static_cast<std::uint32_t>(ref);
+            double v = static_cast<double>(as_global(a.io[ci].out)[u]);
+            if constexpr (std::is_same_v<T, float>)
+                v += static_cast<double>(as_global(a.out_lo)[a.cats[ci].cell_base + u]);
+            return v;
+        };
+        // a cell has at most 7 light upstream neighbours: their references, then their values, all in flight
+        // together; the sum in reference order
+        constexpr int most = 8;
+        std::uint32_t const nref = end - begin;
+        std::uint64_t ref[most];
+#pragma unroll
+        for (int k = 0; k < most; ++k)
+            ref[k] = static_cast<std::uint32_t>(k) < nref ? as_global(c.refs)[begin + k] : 0ull;
+        double v[most];
+#pragma unroll
+        for (int k = 0; k < most; ++k)
+            v[k] = static_cast<std::uint32_t>(k) < nref ? value_of(ref[k]) : 0.0;
+        double sum = as_global(c.side)[p];
+#pragma unroll
+        for (int k = 0; k < most; ++k)
+        {
+            if (static_cast<std::uint32_t>(k) < nref)
+                sum += v[k];
         }
+        for (std::uint32_t j = begin + most; j < end; ++j)
+            sum += value_of(as_global(c.refs)[j]);
         as_global(c.side)[p] = sum;
     }
 

@@ -434,6 +434,7 @@ namespace lue_amd {
         DeviceArray<std::uint32_t> ref_off;      // per position (+1)
         DeviceArray<std::uint64_t> refs;
         DeviceArray<std::uint32_t> ref_pos;      // the positions that have side references, level after level
+        DeviceArray<std::uint32_t> ref_first;    // per entry of ref_pos (+1): its first reference (ref_off there)
         std::vector<std::uint32_t> level_ref;    // those of level l: ref_pos[level_ref[l] .. level_ref[l+1])
         DeviceArray<std::uint32_t> item_start;
         DeviceArray<std::uint32_t> item_count;
@@ -459,6 +460,7 @@ namespace lue_amd {
             ref_off.stream = nullptr;
             refs.stream = nullptr;
             ref_pos.stream = nullptr;
+            ref_first.stream = nullptr;
             item_start.stream = nullptr;
             item_count.stream = nullptr;
             tile_order.stream = nullptr;
@@ -934,6 +936,10 @@ namespace lue_amd {
             hipLaunchKernelGGL(chains_ref_pos_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, has_ref.ptr,
                                ref_rank.ptr, n, net.ref_pos.ptr);
             LUE_AMD_HIP(hipGetLastError());
+            LUE_AMD_HIP(net.ref_first.allocate(nref_pos + 1, stream));
+            hipLaunchKernelGGL(chains_ref_first_kernel, dim3(blocks_for(nref_pos + 1)), dim3(256), 0, stream,
+                               net.ref_pos.ptr, net.ref_off.ptr, nref_pos, n, net.ref_first.ptr);
+            LUE_AMD_HIP(hipGetLastError());
             net.level_ref.assign(net.level_pos.size(), 0);
             for (std::size_t l = 0; l < net.level_pos.size(); ++l)
                 LUE_AMD_HIP(copy_to_host(&net.level_ref[l], ref_rank.ptr + net.level_pos[l], 1, stream));
@@ -1009,7 +1015,7 @@ namespace lue_amd {
             if (ref_end > ref_begin)
             {
                 hipLaunchKernelGGL((chains_gather_kernel<OP, T>), dim3(blocks_for(ref_end - ref_begin)), dim3(256), 0,
-                                   stream, a, c, net.ref_pos.ptr, ref_begin, ref_end);
+                                   stream, a, c, net.ref_pos.ptr, net.ref_first.ptr, ref_begin, ref_end);
             }
             std::uint32_t const begin = net.level_items[l];
             std::uint32_t const end = net.level_items[l + 1];
