@@ -527,6 +527,7 @@ namespace lue_amd {
         // rounds read, pending bitmap, border stamps, and queue the tiles that receive flow
         std::uint32_t const next_slot = (round + 1) % 3;
         unsigned int solved = 0;
+#pragma unroll
         for (int i = tid; i < G::n; i += nthreads)
         {
             std::uint8_t const st = s.state[i];
@@ -534,7 +535,8 @@ namespace lue_amd {
             int const lc = i % TW;
             std::int64_t const r = r0 + lr;
             std::int64_t const c = c0 + lc;
-            int const p = G::perimeter_index(lr, lc);
+            // border stamps serve the later rounds of the rounds schedule only
+            int const p = layout ? -1 : G::perimeter_index(lr, lc);
             if (st == state_new)
             {
                 ++solved;
