@@ -177,7 +177,8 @@ def measure_traffic(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; sharding across GPUs is a design only",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
     if args.probe:
         probe(args)
