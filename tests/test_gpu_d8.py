@@ -78,3 +78,20 @@ def test_d8_special_values(fw, oracle_mod, dtype, shape):
     assert np.array_equal(got, oracle_mod.d8_flow_direction(dem, dtype))
     with np.errstate(invalid="ignore", over="ignore"):
         assert np.array_equal(got, rm.d8_flow_direction(dem))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_d8_unaligned_rows_of_four(fw, oracle_mod, dtype):
+    """Rows a multiple of 4 long but an elevation pointer off the 16-byte grid: the LDS-tile kernel takes over
+    and gives the same ldd."""
+    rows, cols = 37, 128
+    rng = np.random.default_rng(3)
+    dem = rng.integers(-3, 6, (rows, cols)).astype(dtype)
+    dem[rng.random(dem.shape) < 0.05] = np.nan
+    flat = torch.empty(rows * cols + 1, dtype=torch.float32 if dtype == np.float32 else torch.float64,
+                       device="cuda")
+    view = flat[1:].view(rows, cols)
+    view.copy_(torch.from_numpy(dem))
+    assert view.data_ptr() % 16 != 0
+    got = fw.d8_flow_direction(fw.PartitionedArray(view)).numpy()
+    assert np.array_equal(got, oracle_mod.d8_flow_direction(dem, dtype))

@@ -111,3 +111,14 @@ def test_downstream_distance(fw, oracle_mod, dtype, shape):
     case = load_golden("downstream_distance")[0]
     golden = fw.downstream_distance(pa(fw, raster(case["ldd"], np.uint8)), case["cell_size"], np.float32).numpy()
     assert np.array_equal(golden, raster(case["expected"]["distance"]["raster"], np.float32))
+
+
+def test_downstream_distance_rejects_unaligned_ldd(fw):
+    """The C ABI reads the ldd 4 bytes at a time: an ldd pointer off that grid is an invalid argument."""
+    from lue_amd import _lib
+
+    flat = torch.full((8 * 8 + 1,), 2, dtype=torch.uint8, device="cuda")
+    ldd = fw.PartitionedArray(flat[1:].view(8, 8))
+    assert ldd.data.data_ptr() % 4 != 0
+    with pytest.raises(_lib.RoutingError, match="aligned"):
+        fw.downstream_distance(ldd, 1.0)
