@@ -299,7 +299,8 @@ namespace lue_amd {
             std::int64_t const c = c0 + lc;
             std::int64_t const at = r * cols + c;
             auto const g_out = as_global(out);
-            if (c + 4 <= cols && (at & 3) == 0)
+            // whole-word store where the four bytes lie in the row and at a 4-byte aligned address
+            if (c + 4 <= cols && ((reinterpret_cast<std::uintptr_t>(out) + static_cast<std::uintptr_t>(at)) & 3u) == 0)
             {
                 *reinterpret_cast<global_ptr<std::uint32_t>>(g_out + at) = packed;
             }

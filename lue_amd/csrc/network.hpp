@@ -444,6 +444,39 @@ namespace lue_amd {
         DeviceArray<std::uint16_t> tile_split;
         DeviceArray<std::uint8_t> tile_nlevels;
 
+        Network() = default;
+        Network(Network const&) = delete;
+        auto operator=(Network const&) -> Network& = delete;
+
+        // A replay enqueued on any stream may still read the buffers: wait for the network's device before
+        // freeing them (hipFreeAsync on the null stream, which is not ordered with non-blocking streams).
+        ~Network()
+        {
+            int current = 0;
+            bool const switched = hipGetDevice(&current) == hipSuccess && current != device &&
+                                  hipSetDevice(device) == hipSuccess;
+            (void)hipDeviceSynchronize();
+            detach_stream();
+            cats.reset();
+            tile_cat.reset();
+            round_tiles.reset();
+            tile_off.reset();
+            pos_of.reset();
+            seg_start.reset();
+            ref_off.reset();
+            refs.reset();
+            ref_pos.reset();
+            ref_first.reset();
+            item_start.reset();
+            item_count.reset();
+            tile_order.reset();
+            tile_flags.reset();
+            tile_split.reset();
+            tile_nlevels.reset();
+            if (switched)
+                (void)hipSetDevice(current);
+        }
+
         auto rounds() const -> int { return static_cast<int>(round_offset.empty() ? 0 : round_offset.size() - 1); }
         auto levels() const -> int { return static_cast<int>(level_items.empty() ? 0 : level_items.size() - 1); }
 
@@ -471,24 +504,26 @@ namespace lue_amd {
     };
 
     // 64-bit fingerprint of n bytes: sum over 32-bit words of splitmix64(word + index * golden ratio)
-    // (order-sensitive, reduction-friendly). `bytes` must be 4-byte aligned; the tail is zero-padded.
+    // (order-sensitive, reduction-friendly); word i holds bytes 4i .. 4i + 3 (little endian), the tail
+    // zero-padded. Whole-word loads where `bytes` is 4-byte aligned, byte loads otherwise.
     __global__ inline void fingerprint_kernel(std::uint8_t const* bytes, std::uint64_t n, unsigned long long* out)
     {
         std::uint64_t const words = (n + 3) / 4;
+        bool const aligned = (reinterpret_cast<std::uintptr_t>(bytes) & 3u) == 0;
         std::uint64_t sum = 0;
         for (std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < words;
              i += static_cast<std::uint64_t>(gridDim.x) * blockDim.x)
         {
             std::uint32_t w;
-            if (4 * i + 4 <= n)
+            if (aligned && 4 * i + 4 <= n)
             {
                 w = as_global(reinterpret_cast<std::uint32_t const*>(bytes))[i];
             }
             else
             {
                 w = 0;
-                for (std::uint64_t j = 4 * i; j < n; ++j)
-                    w |= static_cast<std::uint32_t>(bytes[j]) << (8 * (j - 4 * i));
+                for (std::uint64_t j = 4 * i; j < n && j < 4 * i + 4; ++j)
+                    w |= static_cast<std::uint32_t>(as_global(bytes)[j]) << (8 * (j - 4 * i));
             }
             std::uint64_t z = static_cast<std::uint64_t>(w) + (i + 1) * 0x9E3779B97F4A7C15ull;
             z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -577,8 +612,46 @@ namespace lue_amd {
         return status_success;
     }
 
-    // Scratch of one operation on a network.
-    struct Workspace
+    // Per-catchment input/output tables of up to io_inline_max catchments travel to the device as a kernel
+    // argument (no host staging buffer, so the launch sequence needs no host memory that outlives the call).
+    constexpr int io_inline_max = 32;
+
+    template<typename T>
+    struct IoTable
+    {
+        CatchmentIO<T> entries[io_inline_max];
+        int n;
+    };
+
+    template<typename T>
+    __global__ void write_io_kernel(IoTable<T> table, CatchmentIO<T>* out)
+    {
+        int const i = static_cast<int>(threadIdx.x);
+        if (i < table.n)
+            out[i] = table.entries[i];
+    }
+
+    // Write the IO table of `n` catchments to `dst` in stream order.
+    template<typename T>
+    int write_io(CatchmentIO<T> const* io, int n, CatchmentIO<T>* dst, hipStream_t stream)
+    {
+        if (n <= io_inline_max)
+        {
+            IoTable<T> table{};
+            for (int c = 0; c < n; ++c)
+                table.entries[c] = io[c];
+            table.n = n;
+            hipLaunchKernelGGL((write_io_kernel<T>), dim3(1), dim3(64), 0, stream, table, dst);
+            LUE_AMD_HIP(hipGetLastError());
+            return status_success;
+        }
+        LUE_AMD_HIP(host_staging().copy_to_device(dst, io, sizeof(CatchmentIO<T>) * n, stream));
+        return status_success;
+    }
+
+    // Scratch of recording an operation on a network (round 0 with walks, and the rounds of the rounds
+    // schedule).
+    struct RecordWorkspace
     {
         ScratchArray<std::uint32_t> pending_bits;
         ScratchArray<std::uint32_t> border_stamp;
@@ -588,14 +661,11 @@ namespace lue_amd {
         ScratchArray<std::uint32_t> work_count;
         ScratchArray<unsigned long long> counters;
         ScratchArray<unsigned char> io;
-        ScratchArray<unsigned char> staged;
-        ScratchArray<double> side;
-        ScratchArray<double> res;
-        ScratchArray<unsigned char> res_rem;
     };
 
     template<typename T, int TW, int TH>
-    int workspace_init(Network const& net, Workspace& w, SweepArgs<T>& a, bool recording, hipStream_t stream)
+    int record_workspace_init(Network const& net, RecordWorkspace& w, std::vector<CatchmentIO<T>> const& io,
+                              SweepArgs<T>& a, hipStream_t stream)
     {
         using G = Tile<TW, TH>;
         LUE_AMD_HIP(w.pending_bits.allocate(static_cast<std::size_t>(net.ntiles * G::words), stream));
@@ -604,20 +674,22 @@ namespace lue_amd {
         {
             LUE_AMD_HIP(w.out_lo.allocate(static_cast<std::size_t>(net.ncells), stream));
         }
-        if (recording)
-        {
-            LUE_AMD_HIP(w.tile_flag.allocate(static_cast<std::size_t>(3 * net.ntiles), stream));
-            LUE_AMD_HIP(w.worklist.allocate(static_cast<std::size_t>(3 * net.ntiles), stream));
-            LUE_AMD_HIP(w.work_count.allocate(4, stream));
-            LUE_AMD_HIP(w.counters.allocate(2, stream));
-            LUE_AMD_HIP(hipMemsetAsync(w.tile_flag.ptr, 0, 3 * net.ntiles * sizeof(std::uint32_t), stream));
-            LUE_AMD_HIP(hipMemsetAsync(w.work_count.ptr, 0, 4 * sizeof(std::uint32_t), stream));
-            LUE_AMD_HIP(hipMemsetAsync(w.counters.ptr, 0, 2 * sizeof(unsigned long long), stream));
-        }
+        LUE_AMD_HIP(w.tile_flag.allocate(static_cast<std::size_t>(3 * net.ntiles), stream));
+        LUE_AMD_HIP(w.worklist.allocate(static_cast<std::size_t>(3 * net.ntiles), stream));
+        LUE_AMD_HIP(w.work_count.allocate(4, stream));
+        LUE_AMD_HIP(w.counters.allocate(2, stream));
+        LUE_AMD_HIP(hipMemsetAsync(w.tile_flag.ptr, 0, 3 * net.ntiles * sizeof(std::uint32_t), stream));
+        LUE_AMD_HIP(hipMemsetAsync(w.work_count.ptr, 0, 4 * sizeof(std::uint32_t), stream));
+        LUE_AMD_HIP(hipMemsetAsync(w.counters.ptr, 0, 2 * sizeof(unsigned long long), stream));
+        LUE_AMD_HIP(w.io.allocate(sizeof(CatchmentIO<T>) * io.size(), stream));
+        if (int const st = write_io<T>(io.data(), static_cast<int>(io.size()),
+                                       reinterpret_cast<CatchmentIO<T>*>(w.io.ptr), stream))
+            return st;
         a.cats = net.cats.ptr;
         a.tile_cat = net.tile_cat.ptr;
         a.ncats = static_cast<int>(net.cats_host.size());
         a.ntiles = net.ntiles;
+        a.io = reinterpret_cast<CatchmentIO<T> const*>(w.io.ptr);
         a.out_lo = w.out_lo.ptr;
         a.pending_bits = w.pending_bits.ptr;
         a.border_stamp = w.border_stamp.ptr;
@@ -625,7 +697,7 @@ namespace lue_amd {
         a.worklist = w.worklist.ptr;
         a.work_count = w.work_count.ptr;
         a.counters = w.counters.ptr;
-        a.record = recording ? 1 : 0;
+        a.record = 1;
         a.stage_off = nullptr;
         a.stage_pos = nullptr;
         a.staged = nullptr;
@@ -634,16 +706,6 @@ namespace lue_amd {
         a.tile_flags = nullptr;
         a.tile_split = nullptr;
         a.tile_nlevels = nullptr;
-        return status_success;
-    }
-
-    // Copy the per-catchment inputs and outputs to the device (the host table may go out of scope after).
-    template<typename T>
-    int upload_io(Workspace& w, std::vector<CatchmentIO<T>> const& io, SweepArgs<T>& a, hipStream_t stream)
-    {
-        LUE_AMD_HIP(w.io.allocate(sizeof(CatchmentIO<T>) * io.size(), stream));
-        LUE_AMD_HIP(host_staging().copy_to_device(w.io.ptr, io.data(), sizeof(CatchmentIO<T>) * io.size(), stream));
-        a.io = reinterpret_cast<CatchmentIO<T> const*>(w.io.ptr);
         return status_success;
     }
 
@@ -973,41 +1035,12 @@ namespace lue_amd {
         return status_success;
     }
 
-    // Per-call buffers of the chain schedule and the staging arguments of round 0.
-    template<int OP, typename T>
-    int chains_workspace(Network const& net, Workspace& w, SweepArgs<T>& a, hipStream_t stream)
-    {
-        std::size_t const n = std::max<std::uint32_t>(net.pending, 1);
-        LUE_AMD_HIP(w.staged.allocate(n * sizeof(CellInput<OP, T>), stream));
-        LUE_AMD_HIP(w.side.allocate(n, stream));
-        LUE_AMD_HIP(w.res.allocate(n, stream));
-        if constexpr (is_split_op(OP))
-        {
-            LUE_AMD_HIP(w.res_rem.allocate(n * sizeof(T), stream));
-        }
-        a.stage_off = net.tile_off.ptr;
-        a.stage_pos = net.pos_of.ptr;
-        a.staged = w.staged.ptr;
-        a.stage_side = w.side.ptr;
-        return status_success;
-    }
-
     // Levels and the final scatter of the chain schedule.
     template<int OP, typename T, int TW, int TH>
-    int launch_chains(Network const& net, Workspace& w, SweepArgs<T> const& a, std::vector<CatchmentIO<T>> const& io,
-                      hipStream_t stream)
+    int launch_chains(Network const& net, SweepArgs<T> const& a, ChainRun<OP, T> const& c, hipStream_t stream)
     {
         if (net.pending == 0)
             return status_success;
-        ChainRun<OP, T> c{net.item_start.ptr,
-                          net.item_count.ptr,
-                          net.seg_start.ptr,
-                          net.ref_off.ptr,
-                          net.refs.ptr,
-                          reinterpret_cast<CellInput<OP, T> const*>(w.staged.ptr),
-                          w.side.ptr,
-                          w.res.ptr,
-                          reinterpret_cast<T*>(w.res_rem.ptr)};
         for (int l = 0; l < net.levels(); ++l)
         {
             std::uint32_t const ref_begin = net.level_ref[l];
@@ -1025,8 +1058,7 @@ namespace lue_amd {
                                end);
         }
         hipLaunchKernelGGL((chains_scatter_kernel<OP, T, TW, TH>), dim3(static_cast<unsigned>(net.ntiles)), dim3(256), 0,
-                           stream, a, net.tile_off.ptr, net.pos_of.ptr, w.res.ptr,
-                           reinterpret_cast<T const*>(w.res_rem.ptr));
+                           stream, a, net.tile_off.ptr, net.pos_of.ptr, c.res, c.res_rem);
         LUE_AMD_HIP(hipGetLastError());
         return status_success;
     }
@@ -1080,8 +1112,149 @@ namespace lue_amd {
         }
     };
 
+    // Per-call device buffers of a replay, carved out of one block at these byte offsets (each 256-byte
+    // aligned). The sizes depend on the network and the operation only.
+    struct ReplayLayout
+    {
+        std::size_t pending_bits{};  // per tile: pending bitmap (written by round 0, read by the scatter)
+        std::size_t border_stamp{};  // per tile perimeter cell (rounds schedule only)
+        std::size_t out_lo{};        // FP32 path: compensation term per cell
+        std::size_t io{};            // per catchment: CatchmentIO<T>
+        std::size_t staged{};        // chain schedule, per pending cell: CellInput<OP, T>
+        std::size_t side{};          //   FP64 side sum
+        std::size_t res{};           //   FP64 result
+        std::size_t res_rem{};       //   remainder (two-output operations)
+        std::size_t total{};
+    };
+
     template<int OP, typename T, int TW, int TH>
-    int replay_run(Network const& net, SweepArgs<T> a, std::vector<CatchmentIO<T>> const& io, hipStream_t stream);
+    ReplayLayout replay_layout(Network const& net)
+    {
+        using G = Tile<TW, TH>;
+        ReplayLayout l;
+        std::size_t at = 0;
+        auto take = [&at](std::size_t bytes)
+        {
+            std::size_t const offset = at;
+            at += (std::max<std::size_t>(bytes, 1) + 255) / 256 * 256;
+            return offset;
+        };
+        std::size_t const ntiles = static_cast<std::size_t>(net.ntiles);
+        bool const chains = net.schedule == schedule_chains;
+        std::size_t const pending = chains ? net.pending : 0;
+        l.pending_bits = take(ntiles * G::words * sizeof(std::uint32_t));
+        l.border_stamp = take(chains ? 0 : ntiles * G::perimeter * sizeof(std::uint32_t));
+        l.out_lo = take(std::is_same_v<T, float> ? static_cast<std::size_t>(net.ncells) * sizeof(float) : 0);
+        l.io = take(net.cats_host.size() * sizeof(CatchmentIO<T>));
+        l.staged = take(pending * sizeof(CellInput<OP, T>));
+        l.side = take(pending * sizeof(double));
+        l.res = take(pending * sizeof(double));
+        l.res_rem = take(is_split_op(OP) ? pending * sizeof(T) : 0);
+        l.total = at;
+        return l;
+    }
+
+    // The launches of a replay on the per-call buffers at `base` (laid out by replay_layout; the IO table
+    // already written). No allocation and no host synchronisation.
+    template<int OP, typename T, int TW, int TH>
+    int replay_launch(Network const& net, SweepArgs<T> a, unsigned char* base, ReplayLayout const& l,
+                      hipStream_t stream)
+    {
+        bool const chains = net.schedule == schedule_chains;
+        a.cats = net.cats.ptr;
+        a.tile_cat = net.tile_cat.ptr;
+        a.ncats = static_cast<int>(net.cats_host.size());
+        a.ntiles = net.ntiles;
+        a.io = reinterpret_cast<CatchmentIO<T> const*>(base + l.io);
+        a.out_lo = std::is_same_v<T, float> ? reinterpret_cast<float*>(base + l.out_lo) : nullptr;
+        a.pending_bits = reinterpret_cast<std::uint32_t*>(base + l.pending_bits);
+        a.border_stamp = chains ? nullptr : reinterpret_cast<std::uint32_t*>(base + l.border_stamp);
+        a.tile_flag = nullptr;
+        a.worklist = nullptr;
+        a.work_count = nullptr;
+        a.counters = nullptr;
+        a.record = 0;
+        a.stage_off = nullptr;
+        a.stage_pos = nullptr;
+        a.staged = nullptr;
+        a.stage_side = nullptr;
+        a.tile_order = nullptr;
+        a.tile_flags = nullptr;
+        a.tile_split = nullptr;
+        a.tile_nlevels = nullptr;
+        ChainRun<OP, T> c{};
+        if (chains)
+        {
+            a.tile_order = net.tile_order.ptr;
+            a.tile_flags = net.tile_flags.ptr;
+            a.tile_split = net.tile_split.ptr;
+            a.tile_nlevels = net.tile_nlevels.ptr;
+            if (net.pending > 0)
+            {
+                a.stage_off = net.tile_off.ptr;
+                a.stage_pos = net.pos_of.ptr;
+                a.staged = base + l.staged;
+                a.stage_side = reinterpret_cast<double*>(base + l.side);
+                c = ChainRun<OP, T>{net.item_start.ptr,
+                                    net.item_count.ptr,
+                                    net.seg_start.ptr,
+                                    net.ref_off.ptr,
+                                    net.refs.ptr,
+                                    reinterpret_cast<CellInput<OP, T> const*>(base + l.staged),
+                                    reinterpret_cast<double*>(base + l.side),
+                                    reinterpret_cast<double*>(base + l.res),
+                                    is_split_op(OP) ? reinterpret_cast<T*>(base + l.res_rem) : nullptr};
+            }
+        }
+        OpTimer timer;
+        if (int const st = timer.start(stream))
+            return st;
+        if (chains)
+            hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH, true>), dim3(static_cast<unsigned>(net.ntiles)),
+                               dim3(sweep_threads), 0, stream, a);
+        else
+            hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH, false>), dim3(static_cast<unsigned>(net.ntiles)),
+                               dim3(sweep_threads), 0, stream, a);
+        LUE_AMD_HIP(hipGetLastError());
+        if (int const st = timer.mark(stream))
+            return st;
+        if (chains)
+        {
+            if (int const st = launch_chains<OP, T, TW, TH>(net, a, c, stream))
+                return st;
+        }
+        else
+        {
+            for (int r = 1; r <= net.rounds(); ++r)
+            {
+                std::uint64_t const begin = net.round_offset[r - 1];
+                std::uint32_t const count = static_cast<std::uint32_t>(net.round_offset[r] - begin);
+                unsigned const grid = std::min<unsigned>(count, 2048);
+                hipLaunchKernelGGL((sweep_list_round<OP, T, TW, TH>), dim3(grid), dim3(sweep_threads), 0, stream, a,
+                                   net.round_tiles.ptr + begin, count, static_cast<std::uint32_t>(r));
+            }
+            LUE_AMD_HIP(hipGetLastError());
+        }
+        return timer.stop(stream, chains ? net.levels() + 1 : net.rounds() + 1, net.ncells);
+    }
+
+    // Replay: the operation as a fixed launch sequence from the recorded schedule, its per-call buffers one
+    // block of the scratch cache (no host synchronisation).
+    template<int OP, typename T, int TW, int TH>
+    int replay_run(Network const& net, SweepArgs<T> a, std::vector<CatchmentIO<T>> const& io, hipStream_t stream)
+    {
+        if (!net.recorded)
+            return fail(status_invalid_argument, "network has no recorded schedule");
+        if (io.size() != net.cats_host.size())
+            return fail(status_invalid_argument, "one input/output set per catchment");
+        ReplayLayout const l = replay_layout<OP, T, TW, TH>(net);
+        ScratchArray<unsigned char> block;
+        LUE_AMD_HIP(block.allocate(l.total, stream));
+        if (int const st = write_io<T>(io.data(), static_cast<int>(io.size()),
+                                       reinterpret_cast<CatchmentIO<T>*>(block.ptr + l.io), stream))
+            return st;
+        return replay_launch<OP, T, TW, TH>(net, a, block.ptr, l, stream);
+    }
 
     // Record: run round 0 (and, for the rounds schedule, the later rounds) while recording the schedule into
     // `net`; the chain schedule is then built and the operation replayed.
@@ -1089,10 +1262,8 @@ namespace lue_amd {
     int record_run(Network& net, SweepArgs<T> a, std::vector<CatchmentIO<T>> const& io, hipStream_t stream,
                    bool produce_output = true)
     {
-        Workspace w;
-        if (int const st = workspace_init<T, TW, TH>(net, w, a, true, stream))
-            return st;
-        if (int const st = upload_io<T>(w, io, a, stream))
+        RecordWorkspace w;
+        if (int const st = record_workspace_init<T, TW, TH>(net, w, io, a, stream))
             return st;
         OpTimer timer;
         bool const chains = net.schedule == schedule_chains;
@@ -1170,63 +1341,6 @@ namespace lue_amd {
         }
         net.recorded = true;
         return status_success;
-    }
-
-    // Replay: the operation as a fixed launch sequence from the recorded schedule (no host synchronisation).
-    template<int OP, typename T, int TW, int TH>
-    int replay_run(Network const& net, SweepArgs<T> a, std::vector<CatchmentIO<T>> const& io, hipStream_t stream)
-    {
-        if (!net.recorded)
-            return fail(status_invalid_argument, "network has no recorded schedule");
-        Workspace w;
-        if (int const st = workspace_init<T, TW, TH>(net, w, a, false, stream))
-            return st;
-        if (int const st = upload_io<T>(w, io, a, stream))
-            return st;
-        bool const chains = net.schedule == schedule_chains;
-        if (chains && net.pending > 0)
-        {
-            if (int const st = chains_workspace<OP, T>(net, w, a, stream))
-                return st;
-        }
-        if (chains)
-        {
-            a.tile_order = net.tile_order.ptr;
-            a.tile_flags = net.tile_flags.ptr;
-            a.tile_split = net.tile_split.ptr;
-            a.tile_nlevels = net.tile_nlevels.ptr;
-        }
-        OpTimer timer;
-        if (int const st = timer.start(stream))
-            return st;
-        if (chains)
-            hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH, true>), dim3(static_cast<unsigned>(net.ntiles)),
-                               dim3(sweep_threads), 0, stream, a);
-        else
-            hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH, false>),
-                           dim3(static_cast<unsigned>(net.ntiles)), dim3(sweep_threads), 0,
-                           stream, a);
-        LUE_AMD_HIP(hipGetLastError());
-        if (int const st = timer.mark(stream))
-            return st;
-        if (chains)
-        {
-            if (int const st = launch_chains<OP, T, TW, TH>(net, w, a, io, stream))
-                return st;
-        }
-        else
-        {
-            for (int r = 1; r <= net.rounds(); ++r)
-            {
-                std::uint64_t const begin = net.round_offset[r - 1];
-                std::uint32_t const count = static_cast<std::uint32_t>(net.round_offset[r] - begin);
-                unsigned const grid = std::min<unsigned>(count, 2048);
-                hipLaunchKernelGGL((sweep_list_round<OP, T, TW, TH>), dim3(grid), dim3(sweep_threads), 0, stream, a,
-                                   net.round_tiles.ptr + begin, count, static_cast<std::uint32_t>(r));
-            }
-            LUE_AMD_HIP(hipGetLastError());
-        }
-        return timer.stop(stream, chains ? net.levels() + 1 : net.rounds() + 1, net.ncells);
     }
 
 }  // namespace lue_amd

@@ -1,0 +1,102 @@
+// C ABI of accu (per call and on flow networks) and of flow-network creation, which records a network's
+// schedule with accu(ldd, 1).
+#include "api.hpp"
+
+using namespace lue_amd;
+
+extern "C" {
+
+int lue_amd_accu_f32(const uint8_t* ldd, int64_t rows, int64_t cols, const float* inflow, float inflow_scalar,
+                     float* outflow, void* stream)
+{
+    return accu_impl<float>(ldd, rows, cols, inflow, inflow_scalar, outflow, stream);
+}
+
+int lue_amd_accu_f64(const uint8_t* ldd, int64_t rows, int64_t cols, const double* inflow, double inflow_scalar,
+                     double* outflow, void* stream)
+{
+    return accu_impl<double>(ldd, rows, cols, inflow, inflow_scalar, outflow, stream);
+}
+
+int lue_amd_network_create(int n, const uint8_t* const* ldd, const int64_t* rows, const int64_t* cols, int tile_shape,
+                           int schedule, void* stream, lue_amd_network* network)
+{
+    if (!network)
+        return fail(status_invalid_argument, "network must not be null");
+    *network = nullptr;
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    auto handle = std::make_unique<lue_amd_network_s>();
+    Network& net = handle->net;
+    if (int const st = network_init(net, n, ldd, reinterpret_cast<std::int64_t const*>(rows),
+                                    reinterpret_cast<std::int64_t const*>(cols), tile_shape, schedule, s))
+        return st;
+    // record the schedule with accu(ldd, 1) into scratch (every operation fits every tile shape of accu f32)
+    DeviceArray<float> scratch;
+    LUE_AMD_HIP(scratch.allocate(static_cast<std::size_t>(net.ncells), s));
+    std::vector<CatchmentIO<float>> io;
+    for (auto const& cat : net.cats_host)
+        io.push_back(CatchmentIO<float>{nullptr, nullptr, nullptr, nullptr, nullptr, scratch.ptr + cat.cell_base, nullptr});
+    Scalars<float> sc{};
+    sc.in1_s = 1.0f;
+    int const st = with_tile_shape<op_accu, float>(net.tile_shape, [&]<int TW, int TH>()
+                                                   { return record_run<op_accu, float, TW, TH>(net, args_from(sc), io, s,
+                                                                                               false); });
+    if (st)
+        return st;
+    LUE_AMD_HIP(hipStreamSynchronize(s));
+    *network = handle.release();
+    return status_success;
+}
+
+int lue_amd_network_destroy(lue_amd_network network)
+{
+    // ~Network waits for the device before it frees the buffers
+    delete network;
+    return status_success;
+}
+
+int lue_amd_network_info(lue_amd_network network, int64_t* info, int n)
+{
+    if (!network || !info || n < 0)
+        return fail(status_invalid_argument, "null argument");
+    Network const& net = network->net;
+    int64_t const values[] = {static_cast<int64_t>(net.cats_host.size()),
+                              net.ncells,
+                              net.ntiles,
+                              net.rounds(),
+                              net.pending,
+                              net.nchains,
+                              net.levels(),
+                              net.nodata_cells,
+                              net.tile_shape,
+                              net.schedule,
+                              net.longest_chain,
+                              static_cast<int64_t>(net.nrefs)};
+    for (int i = 0; i < n && i < static_cast<int>(sizeof(values) / sizeof(values[0])); ++i)
+        info[i] = values[i];
+    return status_success;
+}
+
+#define LUE_AMD_NETWORK_ACCU(T, SUFFIX)                                                                        \
+    int lue_amd_network_accu_##SUFFIX(lue_amd_network network, const T* const* inflow, T inflow_scalar,         \
+                                      T* const* outflow, void* stream)                                          \
+    {                                                                                                           \
+        if (!network || !outflow)                                                                               \
+            return fail(status_invalid_argument, "null argument");                                              \
+        Network const& net = network->net;                                                                      \
+        Scalars<T> sc{};                                                                                        \
+        sc.in1_s = inflow_scalar;                                                                               \
+        std::vector<CatchmentIO<T>> io;                                                                         \
+        for (int c = 0; c < static_cast<int>(net.cats_host.size()); ++c)                                        \
+        {                                                                                                       \
+            if (!outflow[c])                                                                                    \
+                return fail(status_invalid_argument, "outflow must not be null");                              \
+            io.push_back(CatchmentIO<T>{entry(inflow, c), nullptr, nullptr, nullptr, nullptr, outflow[c], nullptr}); \
+        }                                                                                                       \
+        return route_network<op_accu, T>(net, sc, io, static_cast<hipStream_t>(stream));                        \
+    }
+
+LUE_AMD_NETWORK_ACCU(float, f32)
+LUE_AMD_NETWORK_ACCU(double, f64)
+
+}  // extern "C"

@@ -376,10 +376,25 @@ class FlowNetwork:
         return None, float(value), None
 
     def _outputs(self, dtype, out=None):
+        """Per-catchment output pointer array: new rasters, or the caller's, which must match each
+        catchment's ldd in shape and device and have the operation's element type."""
         if out is None:
             out = [_output(ldd, dtype) for ldd in self.flow_directions]
-        elif isinstance(out, PartitionedArray):
-            out = [out]
+        else:
+            if isinstance(out, PartitionedArray):
+                out = [out]
+            out = list(out)
+            if len(out) != len(self):
+                raise ValueError(f"one output raster per catchment: {len(out)} given for {len(self)}")
+            for o, ldd in zip(out, self.flow_directions):
+                if not isinstance(o, PartitionedArray):
+                    raise TypeError("outputs must be PartitionedArrays")
+                if o.shape != ldd.shape:
+                    raise ValueError(f"Shapes differ: {o.shape} != {ldd.shape}")
+                if o.dtype != dtype:
+                    raise TypeError(f"output element type {o.dtype} differs from the operation's {dtype}")
+                if o.data.device != ldd.data.device:
+                    raise ValueError("output lives on another device than the ldd")
         return _pointer_array([_ptr(o.data) for o in out]), out
 
     @staticmethod
