@@ -13,9 +13,15 @@
  *     (the partition grid does not change results; partitions are contiguous sub-blocks in LUE).
  *   - A raster argument may be NULL; the argument is then the Scalar that follows it, broadcast to
  *     every cell (LUE's Scalar<E> overloads, accumulating_router.hpp:727-786).
- *   - `stream` is a hipStream_t (NULL: the default stream). The call enqueues its work on `stream`
- *     and returns when the outputs are complete (it waits for the stream, so the status can report
- *     invalid networks).
+ *   - `stream` is a hipStream_t (NULL: the default stream). Every call enqueues its work on `stream` and
+ *     returns without waiting for it: the outputs are complete when the stream reaches that point
+ *     (synchronise the stream, or make other streams wait for an event recorded after the call, before
+ *     reading them). The per-call routing entry points (accu, accu_threshold, accu_fraction,
+ *     accu_capacity, accu_trigger, kinematic_wave) first compute a fingerprint of the ldd bytes on
+ *     `stream` and wait for it, which also waits for the work queued on `stream` before the call; when
+ *     they record a new schedule for the ldd they wait for the recording as well (so an invalid network
+ *     is reported by the call that meets it). Callers that know when their ldd changes avoid that wait
+ *     with a flow network (lue_amd_network_*), which never waits.
  *   - No-data: ldd 255 (numeric_limits<uint8>::max()), float NaN. Cells whose inputs are no-data or
  *     outside the operation's domain produce NaN, as do all cells downstream of them.
  *   - Return value: LUE_AMD_SUCCESS or an error status; lue_amd_last_error() gives a message.
@@ -48,12 +54,23 @@ const char* lue_amd_version(void);
 /*
  * Timing of the routing sweeps with HIP events on the caller's stream (no reference counterpart; it plays
  * the role of LUE's QA stopwatch instrumentation, source/quality_assurance/cxx). enable != 0 resets and
- * starts collection on this process; read returns the summed time of the first (all-tile) rounds, of the
- * later rounds, the number of routing calls, of rounds and of cells.
+ * starts collection on this process (enable == 0 resets and stops it); the operations record their events
+ * without waiting for them. read waits for the events recorded so far and returns the summed time of the
+ * first (all-tile) rounds, of the later rounds, the number of routing calls, of rounds and of cells;
+ * read_op the same for one operation (LUE_AMD_OP_*) and element type (f64 != 0: double), with the
+ * shortest and longest first round.
  */
+#define LUE_AMD_OP_ACCU 0
+#define LUE_AMD_OP_ACCU_THRESHOLD 1
+#define LUE_AMD_OP_ACCU_FRACTION 2
+#define LUE_AMD_OP_KINEMATIC_WAVE 3
+#define LUE_AMD_OP_ACCU_CAPACITY 4
+#define LUE_AMD_OP_ACCU_TRIGGER 5
 int lue_amd_profile_enable(int enable);
 int lue_amd_profile_read(double* first_round_ms, double* next_rounds_ms, long long* calls, long long* rounds,
                          long long* cells);
+int lue_amd_profile_read_op(int op, int f64, double* first_round_ms, double* next_rounds_ms,
+                            double* first_round_min_ms, double* first_round_max_ms, long long* calls);
 
 /*
  * Schedules of the routing sweep after its first round over all tiles (no reference counterpart):
@@ -83,12 +100,26 @@ int lue_amd_set_network_cache(int entries);
  */
 int lue_amd_trim_scratch(void);
 
-/* Device memory used by the C++ front end's device-resident partitions. */
+/* Device memory used by the C++ front end's device-resident partitions. The copies wait for `stream`. */
 int lue_amd_device_malloc(void** ptr, size_t bytes);
 int lue_amd_device_free(void* ptr);
 int lue_amd_copy_to_device(void* dst, const void* src, size_t bytes, void* stream);
 int lue_amd_copy_to_host(void* dst, const void* src, size_t bytes, void* stream);
 int lue_amd_stream_synchronize(void* stream);
+
+/*
+ * Completion events (hipEvent_t) of the C++ front end's partitioned arrays, which stand in for the futures
+ * of LUE's partitions (source/framework/partitioned_array/include/lue/framework/partitioned_array_decl.hpp:
+ * 23-98, partitions are hpx::future-backed components): an operation records one after its launches, later
+ * operations make their stream wait for it, host reads wait for it. query sets *complete to 1 when the
+ * event has completed, 0 otherwise.
+ */
+int lue_amd_event_create(void** event);
+int lue_amd_event_destroy(void* event);
+int lue_amd_event_record(void* event, void* stream);
+int lue_amd_stream_wait_event(void* stream, void* event);
+int lue_amd_event_synchronize(void* event);
+int lue_amd_event_query(void* event, int* complete);
 
 /*
  * Inflow count: number of upstream neighbours per cell, 255 where the ldd is no-data.
@@ -268,7 +299,7 @@ int lue_amd_iterate_to_new_discharge_f64(const double* upstream_discharge, const
  * Seeded synthetic inputs of the benchmark workloads (no reference counterpart: LUE's own tests and
  * benchmarks build inputs in C++). ldd from the DEM e(r,c) = (rows - r) + 50 |sin(pi c / 256)| + 0.9 u,
  * sqrt(2)-weighted steepest descent, bottom row outlets; uniform fields lo + (hi - lo) u. The ldd
- * generator returns when its output is written; the uniform generators return at once (stream order).
+ * generators return at once (stream order).
  */
 int lue_amd_synthetic_ldd(uint64_t seed, int64_t rows, int64_t cols, uint8_t* ldd, void* stream);
 /* the DEM e(r, c) itself, rounded to the output type (stream order) */

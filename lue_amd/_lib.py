@@ -46,11 +46,18 @@ SIGNATURES = {
     "lue_amd_network_kinematic_wave_f64": [c_p, c_p, c_p, c_p, c_d, c_p, c_d, c_d, c_p, c_d, c_p, c_p],
     "lue_amd_profile_enable": [c_int],
     "lue_amd_profile_read": [ctypes.POINTER(c_d), ctypes.POINTER(c_d)] + [ctypes.POINTER(ctypes.c_longlong)] * 3,
+    "lue_amd_profile_read_op": [c_int, c_int] + [ctypes.POINTER(c_d)] * 4 + [ctypes.POINTER(ctypes.c_longlong)],
     "lue_amd_device_malloc": [ctypes.POINTER(c_p), c_size],
     "lue_amd_device_free": [c_p],
     "lue_amd_copy_to_device": [c_p, c_p, c_size, c_p],
     "lue_amd_copy_to_host": [c_p, c_p, c_size, c_p],
     "lue_amd_stream_synchronize": [c_p],
+    "lue_amd_event_create": [ctypes.POINTER(c_p)],
+    "lue_amd_event_destroy": [c_p],
+    "lue_amd_event_record": [c_p, c_p],
+    "lue_amd_stream_wait_event": [c_p, c_p],
+    "lue_amd_event_synchronize": [c_p],
+    "lue_amd_event_query": [c_p, ctypes.POINTER(c_int)],
     "lue_amd_inflow_count_u8": [c_p, c_i64, c_i64, c_p, c_p],
     "lue_amd_d8_flow_direction_f32": [c_p, c_i64, c_i64, c_p, c_p],
     "lue_amd_d8_flow_direction_f64": [c_p, c_i64, c_i64, c_p, c_p],
@@ -149,6 +156,22 @@ def profile_read():
     check(load().lue_amd_profile_read(ctypes.byref(first), ctypes.byref(rest), ctypes.byref(calls),
                                       ctypes.byref(rounds), ctypes.byref(cells)))
     return first.value, rest.value, calls.value, rounds.value, cells.value
+
+
+OPS = {"accu": 0, "accu_threshold": 1, "accu_fraction": 2, "kinematic_wave": 3, "accu_capacity": 4,
+       "accu_trigger": 5}
+
+
+def profile_read_op(op, f64=False):
+    """Per-operation profile since profile_enable(): dict of summed first-round and later-rounds ms, the
+    shortest and longest first round, and the number of calls."""
+    vals = [c_d() for _ in range(4)]
+    calls = ctypes.c_longlong()
+    check(load().lue_amd_profile_read_op(OPS[op], 1 if f64 else 0, *[ctypes.byref(v) for v in vals],
+                                         ctypes.byref(calls)))
+    first, rest, lo, hi = (v.value for v in vals)
+    return dict(first_round_ms=first, after_first_round_ms=rest, first_round_min_ms=lo, first_round_max_ms=hi,
+                calls=calls.value)
 
 
 def check(status):

@@ -59,15 +59,146 @@ namespace lue_amd {
         th = shape == tile_64x64 ? 64 : 32;
     }
 
-    // Timing of the sweeps with HIP events (lue_amd_profile_*).
-    struct Profile
+    // Timing of the sweeps with HIP events (lue_amd_profile_*). A profiled operation records three events on
+    // its stream (start, end of round 0, end) without waiting for them; reading the profile waits for the
+    // events recorded so far and sums their times, so profiling adds no host synchronisation to the calls.
+    class Profile
     {
-        bool enabled{false};
-        double first_round_ms{0.0};
-        double next_rounds_ms{0.0};
-        long long calls{0};
-        long long rounds{0};
-        long long cells{0};
+    public:
+        struct Totals
+        {
+            double first_round_ms{0.0};
+            double next_rounds_ms{0.0};
+            double first_round_min_ms{0.0};
+            double first_round_max_ms{0.0};
+            long long calls{0};
+            long long rounds{0};
+            long long cells{0};
+        };
+
+        bool enabled()
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            return enabled_;
+        }
+
+        // Reset the totals and start (or stop) collecting.
+        hipError_t reset(bool enable)
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            hipError_t const e = resolve_locked();
+            totals_ = Totals{};
+            for (auto& t : by_kind_)
+                t = Totals{};
+            enabled_ = enable;
+            return e;
+        }
+
+        // Three events for one operation (from the pool).
+        hipError_t take(hipEvent_t (&ev)[3])
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            for (auto& e : ev)
+            {
+                if (!pool_.empty())
+                {
+                    e = pool_.back();
+                    pool_.pop_back();
+                }
+                else if (hipError_t const err = hipEventCreate(&e); err != hipSuccess)
+                {
+                    e = nullptr;
+                    return err;
+                }
+            }
+            return hipSuccess;
+        }
+
+        void give_back(hipEvent_t (&ev)[3])
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            for (auto& e : ev)
+            {
+                if (e)
+                    pool_.push_back(e);
+                e = nullptr;
+            }
+        }
+
+        // The events of one operation (kind = operation * 2 + 1 for FP64), all recorded.
+        void finish(hipEvent_t (&ev)[3], int kind, long long rounds, std::int64_t cells)
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            pending_.push_back(Pending{{ev[0], ev[1], ev[2]}, kind, rounds, cells});
+            for (auto& e : ev)
+                e = nullptr;
+        }
+
+        // Totals of all operations (kind < 0) or of one kind.
+        hipError_t read(int kind, Totals& out)
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            hipError_t const e = resolve_locked();
+            out = kind < 0 ? totals_ : (kind < kinds ? by_kind_[kind] : Totals{});
+            return e;
+        }
+
+        static constexpr int kinds = 12;
+
+    private:
+        struct Pending
+        {
+            hipEvent_t ev[3];
+            int kind;
+            long long rounds;
+            std::int64_t cells;
+        };
+
+        static void add(Totals& t, float first, float rest, Pending const& p)
+        {
+            t.first_round_min_ms = t.calls == 0 ? first : std::min<double>(t.first_round_min_ms, first);
+            t.first_round_max_ms = std::max<double>(t.first_round_max_ms, first);
+            t.first_round_ms += first;
+            t.next_rounds_ms += rest;
+            t.calls += 1;
+            t.rounds += p.rounds;
+            t.cells += p.cells;
+        }
+
+        hipError_t resolve_locked()
+        {
+            hipError_t result = hipSuccess;
+            for (auto& p : pending_)
+            {
+                float first = 0.0f, rest = 0.0f;
+                hipError_t e = hipEventSynchronize(p.ev[2]);
+                if (e == hipSuccess)
+                    e = hipEventElapsedTime(&first, p.ev[0], p.ev[1]);
+                if (e == hipSuccess)
+                    e = hipEventElapsedTime(&rest, p.ev[1], p.ev[2]);
+                if (e == hipSuccess)
+                {
+                    add(totals_, first, rest, p);
+                    if (p.kind >= 0 && p.kind < kinds)
+                        add(by_kind_[p.kind], first, rest, p);
+                }
+                else if (result == hipSuccess)
+                {
+                    result = e;
+                }
+                for (auto& ev : p.ev)
+                    pool_.push_back(ev);
+            }
+            pending_.clear();
+            return result;
+        }
+
+        std::mutex mutex_;
+        bool enabled_{false};
+        Totals totals_;
+        Totals by_kind_[kinds];
+        std::vector<Pending> pending_;
+        std::vector<hipEvent_t> pool_;
     };
     Profile& profile();
 
@@ -1063,19 +1194,20 @@ namespace lue_amd {
         return status_success;
     }
 
-    // Timing of one operation (lue_amd_profile_*): events around round 0 and the rest.
+    // Timing of one operation (lue_amd_profile_*): events around round 0 and the rest, queued in the
+    // profile without waiting.
     struct OpTimer
     {
         hipEvent_t ev[3]{nullptr, nullptr, nullptr};
         bool on{false};
+        int kind{-1};  // operation * 2 + 1 for FP64
 
         int start(hipStream_t stream)
         {
-            on = profile().enabled;
+            on = profile().enabled();
             if (!on)
                 return status_success;
-            for (auto& e : ev)
-                LUE_AMD_HIP(hipEventCreate(&e));
+            LUE_AMD_HIP(profile().take(ev));
             LUE_AMD_HIP(hipEventRecord(ev[0], stream));
             return status_success;
         }
@@ -1092,23 +1224,14 @@ namespace lue_amd {
             if (!on)
                 return status_success;
             LUE_AMD_HIP(hipEventRecord(ev[2], stream));
-            LUE_AMD_HIP(hipEventSynchronize(ev[2]));
-            float first = 0.0f, rest = 0.0f;
-            LUE_AMD_HIP(hipEventElapsedTime(&first, ev[0], ev[1]));
-            LUE_AMD_HIP(hipEventElapsedTime(&rest, ev[1], ev[2]));
-            profile().first_round_ms += first;
-            profile().next_rounds_ms += rest;
-            profile().calls += 1;
-            profile().rounds += rounds;
-            profile().cells += cells;
+            profile().finish(ev, kind, rounds, cells);
             return status_success;
         }
 
         ~OpTimer()
         {
-            for (auto& e : ev)
-                if (e)
-                    (void)hipEventDestroy(e);
+            if (on)
+                profile().give_back(ev);
         }
     };
 
@@ -1155,10 +1278,11 @@ namespace lue_amd {
     }
 
     // The launches of a replay on the per-call buffers at `base` (laid out by replay_layout; the IO table
-    // already written). No allocation and no host synchronisation.
+    // already written). No device allocation and no host synchronisation; with `timed` false not even the
+    // profiler's events are taken.
     template<int OP, typename T, int TW, int TH>
     int replay_launch(Network const& net, SweepArgs<T> a, unsigned char* base, ReplayLayout const& l,
-                      hipStream_t stream)
+                      hipStream_t stream, bool timed = true)
     {
         bool const chains = net.schedule == schedule_chains;
         a.cats = net.cats.ptr;
@@ -1207,8 +1331,12 @@ namespace lue_amd {
             }
         }
         OpTimer timer;
-        if (int const st = timer.start(stream))
-            return st;
+        timer.kind = OP * 2 + (std::is_same_v<T, double> ? 1 : 0);
+        if (timed)
+        {
+            if (int const st = timer.start(stream))
+                return st;
+        }
         if (chains)
             hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH, true>), dim3(static_cast<unsigned>(net.ntiles)),
                                dim3(sweep_threads), 0, stream, a);
@@ -1266,6 +1394,7 @@ namespace lue_amd {
         if (int const st = record_workspace_init<T, TW, TH>(net, w, io, a, stream))
             return st;
         OpTimer timer;
+        timer.kind = OP * 2 + (std::is_same_v<T, double> ? 1 : 0);
         bool const chains = net.schedule == schedule_chains;
         if (!chains)
         {

@@ -134,7 +134,6 @@ namespace lue_amd {
             hipLaunchKernelGGL((iterate_kernel<T>), dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s,
                                up, cur, lat, alpha, beta, dt, dx, out, n);
             LUE_AMD_HIP(hipGetLastError());
-            LUE_AMD_HIP(hipStreamSynchronize(s));
             return status_success;
         }
 
@@ -309,8 +308,7 @@ int lue_amd_trim_scratch(void)
 
 int lue_amd_profile_enable(int enable)
 {
-    profile() = Profile{};
-    profile().enabled = enable != 0;
+    LUE_AMD_HIP(profile().reset(enable != 0));
     return status_success;
 }
 
@@ -319,11 +317,30 @@ int lue_amd_profile_read(double* first_round_ms, double* next_rounds_ms, long lo
 {
     if (!first_round_ms || !next_rounds_ms || !calls || !rounds || !cells)
         return fail(status_invalid_argument, "null pointer");
-    *first_round_ms = profile().first_round_ms;
-    *next_rounds_ms = profile().next_rounds_ms;
-    *calls = profile().calls;
-    *rounds = profile().rounds;
-    *cells = profile().cells;
+    Profile::Totals t;
+    LUE_AMD_HIP(profile().read(-1, t));
+    *first_round_ms = t.first_round_ms;
+    *next_rounds_ms = t.next_rounds_ms;
+    *calls = t.calls;
+    *rounds = t.rounds;
+    *cells = t.cells;
+    return status_success;
+}
+
+int lue_amd_profile_read_op(int op, int f64, double* first_round_ms, double* next_rounds_ms,
+                            double* first_round_min_ms, double* first_round_max_ms, long long* calls)
+{
+    if (!first_round_ms || !next_rounds_ms || !first_round_min_ms || !first_round_max_ms || !calls)
+        return fail(status_invalid_argument, "null pointer");
+    if (op < 0 || op >= Profile::kinds / 2)
+        return fail(status_invalid_argument, "unknown operation");
+    Profile::Totals t;
+    LUE_AMD_HIP(profile().read(2 * op + (f64 ? 1 : 0), t));
+    *first_round_ms = t.first_round_ms;
+    *next_rounds_ms = t.next_rounds_ms;
+    *first_round_min_ms = t.first_round_min_ms;
+    *first_round_max_ms = t.first_round_max_ms;
+    *calls = t.calls;
     return status_success;
 }
 
@@ -364,6 +381,63 @@ int lue_amd_stream_synchronize(void* stream)
     return status_success;
 }
 
+int lue_amd_event_create(void** event)
+{
+    if (!event)
+        return fail(status_invalid_argument, "null pointer");
+    hipEvent_t e = nullptr;
+    LUE_AMD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    *event = e;
+    return status_success;
+}
+
+int lue_amd_event_destroy(void* event)
+{
+    if (event)
+        LUE_AMD_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+    return status_success;
+}
+
+int lue_amd_event_record(void* event, void* stream)
+{
+    if (!event)
+        return fail(status_invalid_argument, "null event");
+    LUE_AMD_HIP(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)));
+    return status_success;
+}
+
+int lue_amd_stream_wait_event(void* stream, void* event)
+{
+    if (!event)
+        return fail(status_invalid_argument, "null event");
+    LUE_AMD_HIP(hipStreamWaitEvent(static_cast<hipStream_t>(stream), static_cast<hipEvent_t>(event), 0));
+    return status_success;
+}
+
+int lue_amd_event_synchronize(void* event)
+{
+    if (!event)
+        return fail(status_invalid_argument, "null event");
+    LUE_AMD_HIP(hipEventSynchronize(static_cast<hipEvent_t>(event)));
+    return status_success;
+}
+
+int lue_amd_event_query(void* event, int* complete)
+{
+    if (!event || !complete)
+        return fail(status_invalid_argument, "null argument");
+    hipError_t const e = hipEventQuery(static_cast<hipEvent_t>(event));
+    if (e == hipErrorNotReady)
+    {
+        (void)hipGetLastError();
+        *complete = 0;
+        return status_success;
+    }
+    LUE_AMD_HIP(e);
+    *complete = 1;
+    return status_success;
+}
+
 int lue_amd_inflow_count_u8(const uint8_t* ldd, int64_t rows, int64_t cols, uint8_t* inflow_count, void* stream)
 {
     if (int const st = check_shape(ldd, rows, cols))
@@ -374,7 +448,6 @@ int lue_amd_inflow_count_u8(const uint8_t* ldd, int64_t rows, int64_t cols, uint
     dim3 const grid(static_cast<unsigned>((cols + ic_tw - 1) / ic_tw), static_cast<unsigned>((rows + ic_th - 1) / ic_th));
     hipLaunchKernelGGL(inflow_count_kernel, grid, dim3(256), 0, s, ldd, rows, cols, inflow_count);
     LUE_AMD_HIP(hipGetLastError());
-    LUE_AMD_HIP(hipStreamSynchronize(s));
     return status_success;
 }
 
@@ -416,14 +489,12 @@ int lue_amd_synthetic_ldd(uint64_t seed, int64_t rows, int64_t cols, uint8_t* ld
     double table[256];
     for (int k = 0; k < 256; ++k)
         table[k] = std::fabs(std::sin(M_PI * static_cast<double>(k) / 256.0));
-    double* d_table = nullptr;
-    LUE_AMD_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_table), sizeof(table), s));
-    LUE_AMD_HIP(hipMemcpyAsync(d_table, table, sizeof(table), hipMemcpyHostToDevice, s));
+    ScratchArray<double> d_table;
+    LUE_AMD_HIP(d_table.allocate(256, s));
+    LUE_AMD_HIP(host_staging().copy_to_device(d_table.ptr, table, sizeof(table), s));
     dim3 const grid(static_cast<unsigned>((cols + dem_tw - 1) / dem_tw), static_cast<unsigned>((rows + dem_th - 1) / dem_th));
-    hipLaunchKernelGGL(dem_ldd_kernel, grid, dim3(256), 0, s, seed, rows, cols, d_table, ldd);
+    hipLaunchKernelGGL(dem_ldd_kernel, grid, dim3(256), 0, s, seed, rows, cols, d_table.ptr, ldd);
     LUE_AMD_HIP(hipGetLastError());
-    LUE_AMD_HIP(hipFreeAsync(d_table, s));
-    LUE_AMD_HIP(hipStreamSynchronize(s));
     return status_success;
 }
 

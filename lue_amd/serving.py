@@ -59,7 +59,8 @@ class AdmissionError(RuntimeError):
 
 
 class Server:
-    """Resident catchments and the request loop. beta, dt and dx are the scalars of every timestep."""
+    """Resident catchments and the request loop. beta, dt and dx are the scalars of every timestep.
+    keep_results: keep every request's result (True), none (False), or those of the named catchments."""
 
     def __init__(self, hbm_budget_bytes, slots=16, beta=0.6, dt=3600.0, dx=100.0, tile_shape=(32, 32),
                  device=None, keep_results=False):
@@ -154,7 +155,8 @@ class Server:
         # the timestep has run: its inputs can go (the result is the catchment's state now)
         request.lateral = None
         request.previous_state = None
-        if not self.keep_results:
+        keep = self.keep_results
+        if not (keep is True or (keep and request.catchment in keep)):
             request.result = None
         self.completed.append(request)
 

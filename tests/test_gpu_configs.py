@@ -1,0 +1,265 @@
+"""Every benchmark configuration at its own size, on the GPU, against the oracle or full-size invariants:
+  - kinematic_wave on 8192^2 (1024^2 partitions): FP64 path and FP32 path against the FP64 oracle;
+  - the headline step, accu + kinematic_wave on 16384^2, both paths against the oracle;
+  - accu_threshold over 512 catchments (sides log-uniform 64..2048, aspect U[0.5, 2]) in one batch: the batch
+    equals every catchment routed alone, bit for bit, every catchment matches the oracle and conserves
+    material;
+  - the serving loop with 200 resident catchments (256^2..4096^2) for 2 s of Poisson 1000 requests/s: a
+    sample of 10 catchments re-run alone gives the same bits;
+  - 32768^2 with 24 resident kinematic-wave timesteps: the int64 upstream cell count satisfies the neighbour
+    recurrence at full size, the bottom row holds every cell once, and every resident step equals the
+    per-call step bit for bit.
+Tolerances: kinematic_wave_test.cpp:61-118 checks LUE's own results at 1e-6 relative; the FP32 path is held
+to 1e-5 of the FP64 oracle (the stated bound) and the FP64 path to 1e-12 (accu sums: 1e-11, their summation
+order differs).
+"""
+import concurrent.futures
+
+import numpy as np
+import pytest
+
+from conftest import BETA32
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261203
+
+
+@pytest.fixture(scope="module")
+def fw():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lue_amd import framework
+
+    return framework
+
+
+def rel_stats(got, want):
+    """(max, mean) relative error; no-data (NaN) must agree."""
+    got = np.asarray(got, np.float64)
+    want = np.asarray(want, np.float64)
+    assert (np.isnan(got) == np.isnan(want)).all(), "no-data cells differ"
+    err = np.abs(got - want) / np.maximum(np.abs(want), 1e-30)
+    err = np.where(np.isnan(want), 0.0, err)
+    return float(err.max()), float(err.mean())
+
+
+def report(name, stats):
+    print(f"[configs] {name}: max rel {stats[0]:.3e}, mean rel {stats[1]:.3e}", flush=True)
+    return stats
+
+
+def device_inputs(fw, n, ps, dtype):
+    from lue_amd import synthetic as syn
+
+    dev = torch.device("cuda")
+    ldd = fw.PartitionedArray(syn.ldd_device(SEED, n, n, dev), ps)
+
+    def field(stream, lo, hi):
+        return fw.PartitionedArray(syn.uniform_device(SEED, stream, lo, hi, (n, n), dtype, dev), ps)
+
+    return ldd, {"material": field(syn.STREAM_MATERIAL, 0.0, 1.0), "q_cur": field(syn.STREAM_CURRENT, 0.0, 5.0),
+                 "q_lat": field(syn.STREAM_LATERAL, 0.0, 1e-4), "alpha": field(syn.STREAM_ALPHA, 1.5, 2.5)}
+
+
+def test_kinematic_wave_8192(fw, oracle_mod):
+    """kinematic_wave on 8192^2 with 1024^2 partitions: upstream discharge U[0,5), lateral inflow U[0,1e-4),
+    alpha 1.5 + U[0,1), beta 0.6, dt 3600 s, channel length 100 m."""
+    n, ps = 8192, (1024, 1024)
+    ldd, f64 = device_inputs(fw, n, ps, torch.float64)
+    host = {k: v.numpy() for k, v in f64.items()}
+    ldd_h = ldd.numpy()
+    want = oracle_mod.kinematic_wave(ldd_h, host["q_cur"], host["q_lat"], host["alpha"], 0.6, 3600.0, 100.0, ps,
+                                     np.float64)
+    got64 = fw.kinematic_wave(ldd, f64["q_cur"], f64["q_lat"], f64["alpha"], 0.6, 3600.0, 100.0).numpy()
+    assert report("kinematic_wave 8192^2 fp64 path vs oracle", rel_stats(got64, want))[0] <= 1e-12
+    f32 = {k: fw.PartitionedArray(v.data.float(), ps) for k, v in f64.items()}
+    want32in = oracle_mod.kinematic_wave(ldd_h, *[f32[k].numpy().astype(np.float64) for k in ("q_cur", "q_lat",
+                                                                                             "alpha")],
+                                         BETA32, 3600.0, 100.0, ps, np.float64)
+    got32 = fw.kinematic_wave(ldd, f32["q_cur"], f32["q_lat"], f32["alpha"], 0.6, 3600.0, 100.0).numpy()
+    assert report("kinematic_wave 8192^2 fp32 path vs fp64 oracle", rel_stats(got32, want32in))[0] <= 1e-5
+
+
+def test_headline_accu_kinematic_wave_16384(fw, oracle_mod):
+    """The headline step on 16384^2 (1024^2 partitions): accu and kinematic_wave, FP32 and FP64 paths."""
+    n, ps = 16384, (1024, 1024)
+    ldd, f32 = device_inputs(fw, n, ps, torch.float32)
+    ldd_h = ldd.numpy()
+    host = {k: v.numpy().astype(np.float64) for k, v in f32.items()}  # FP32 values, exactly
+    # accu
+    want = oracle_mod.accu(ldd_h, host["material"], ps, np.float64)
+    got32 = fw.accu(ldd, f32["material"]).numpy()
+    assert report("accu 16384^2 fp32 path vs fp64 oracle", rel_stats(got32, want))[0] <= 1e-5
+    got64 = fw.accu(ldd, fw.PartitionedArray(f32["material"].data.double(), ps)).numpy()
+    assert report("accu 16384^2 fp64 path vs oracle", rel_stats(got64, want))[0] <= 1e-11
+    del got32, got64, want
+    # kinematic wave (beta as the FP32 path receives it)
+    want = oracle_mod.kinematic_wave(ldd_h, host["q_cur"], host["q_lat"], host["alpha"], BETA32, 3600.0, 100.0,
+                                     ps, np.float64)
+    got32 = fw.kinematic_wave(ldd, f32["q_cur"], f32["q_lat"], f32["alpha"], 0.6, 3600.0, 100.0).numpy()
+    assert report("kinematic_wave 16384^2 fp32 path vs fp64 oracle", rel_stats(got32, want))[0] <= 1e-5
+    f64 = {k: fw.PartitionedArray(f32[k].data.double(), ps) for k in ("q_cur", "q_lat", "alpha")}
+    got64 = fw.kinematic_wave(ldd, f64["q_cur"], f64["q_lat"], f64["alpha"], BETA32, 3600.0, 100.0).numpy()
+    assert report("kinematic_wave 16384^2 fp64 path vs oracle", rel_stats(got64, want))[0] <= 1e-12
+
+
+def outlet_mask(ldd):
+    """Cells whose flow leaves the catchment: pits (5) and cells draining across the raster's edge."""
+    rows, cols = ldd.shape
+    dr = {1: 1, 2: 1, 3: 1, 7: -1, 8: -1, 9: -1}
+    dc = {1: -1, 4: -1, 7: -1, 3: 1, 6: 1, 9: 1}
+    r = np.arange(rows)[:, None]
+    c = np.arange(cols)[None, :]
+    tr = r + np.vectorize(lambda v: dr.get(int(v), 0), otypes=[np.int64])(np.arange(256))[ldd]
+    tc = c + np.vectorize(lambda v: dc.get(int(v), 0), otypes=[np.int64])(np.arange(256))[ldd]
+    outside = (tr < 0) | (tr >= rows) | (tc < 0) | (tc >= cols)
+    return (ldd == 5) | (outside & (ldd != 255))
+
+
+def test_batch_512_catchments_accu_threshold(fw, oracle_mod):
+    """accu_threshold (float) over 512 catchments in one batch, tile shape autotuned; material U[0,1),
+    threshold U[0,0.5)."""
+    from lue_amd import autotune
+    from lue_amd import synthetic as syn
+
+    seed = 20261205
+    dev = torch.device("cuda")
+    shapes = syn.batch_shapes(seed, 512, 64, 2048)
+    ldds, mats, thrs = [], [], []
+    for i, (r, c) in enumerate(shapes):
+        s = seed + 1 + i
+        ldds.append(fw.PartitionedArray(syn.ldd_device(s, r, c, dev)))
+        mats.append(fw.PartitionedArray(syn.uniform_device(s, syn.STREAM_MATERIAL, 0.0, 1.0, (r, c), torch.float32,
+                                                           dev)))
+        thrs.append(fw.PartitionedArray(syn.uniform_device(s, syn.STREAM_THRESHOLD, 0.0, 0.5, (r, c),
+                                                           torch.float32, dev)))
+    net, _ = autotune.tune(ldds, lambda n: n.accu_threshold(mats, thrs), repeats=1)
+    assert net.info()["catchments"] == 512
+    outs, rems = net.accu_threshold(mats, thrs)
+    torch.cuda.synchronize()
+    shape = {0: (32, 32), 1: (64, 32), 2: (64, 64)}[net.info()["tile_shape"]]
+    print(f"[configs] batch of 512: {sum(r * c for r, c in shapes)} cells, tile shape {shape}", flush=True)
+    # every catchment alone gives the same bits
+    for i in range(len(shapes)):
+        alone = fw.FlowNetwork(ldds[i], tile_shape=shape)
+        o, r = alone.accu_threshold(mats[i], thrs[i])
+        torch.cuda.synchronize()
+        assert torch.equal(torch.nan_to_num(o.data, nan=-1.0), torch.nan_to_num(outs[i].data, nan=-1.0)), i
+        assert torch.equal(torch.nan_to_num(r.data, nan=-1.0), torch.nan_to_num(rems[i].data, nan=-1.0)), i
+        alone.close()
+        del o, r
+    # every catchment against the oracle, and conservation of material
+    worst = [0.0, 0.0]
+
+    def check(i):
+        ldd = ldds[i].numpy()
+        mat = mats[i].numpy().astype(np.float64)
+        thr = thrs[i].numpy().astype(np.float64)
+        w_out, w_rem = oracle_mod.accu_threshold(ldd, mat, thr, None, np.float64, 1)
+        out = outs[i].numpy().astype(np.float64)
+        rem = rems[i].numpy().astype(np.float64)
+        scale = np.abs(w_out) + np.abs(w_rem) + 1e-30
+        err = max(np.max(np.abs(out - w_out) / scale), np.max(np.abs(rem - w_rem) / scale))
+        total = mat.sum()
+        leaving = rem.sum() + out[outlet_mask(ldd)].sum()
+        return err, abs(leaving - total) / total
+
+    with concurrent.futures.ThreadPoolExecutor(16) as pool:
+        for err, cons in pool.map(check, range(len(shapes))):
+            worst[0] = max(worst[0], err)
+            worst[1] = max(worst[1], cons)
+    print(f"[configs] batch of 512 vs oracle: max rel {worst[0]:.3e}; conservation max rel {worst[1]:.3e}",
+          flush=True)
+    assert worst[0] <= 1e-5
+    assert worst[1] <= 1e-5
+
+
+def test_serving_200_catchments(fw):
+    """200 resident catchments of sides log-uniform 256..4096, HBM budget 200 GB, Poisson 1000 requests/s for
+    2 s: the timesteps of a sample of 10 catchments, re-run alone in order, give the same bits."""
+    from lue_amd import serving
+    from lue_amd import synthetic as syn
+
+    seed = 20261207
+    dev = torch.device("cuda")
+    shapes = syn.batch_shapes(seed, 200, 256, 4096)
+    rng = np.random.default_rng(seed)
+    sample = {f"c{i}" for i in rng.choice(200, 10, replace=False)}
+    server = serving.Server(200e9, slots=16, device=dev, keep_results=sample)
+    initial = {}
+    for i, (r, c) in enumerate(shapes):
+        s = seed + 1 + i
+        ldd = fw.PartitionedArray(syn.ldd_device(s, r, c, dev))
+        alpha = fw.PartitionedArray(syn.uniform_device(s, syn.STREAM_ALPHA, 1.5, 2.5, (r, c), torch.float32, dev))
+        q0 = fw.PartitionedArray(syn.uniform_device(s, syn.STREAM_CURRENT, 0.0, 5.0, (r, c), torch.float32, dev))
+        server.add_catchment(f"c{i}", ldd, alpha, q0)
+        if f"c{i}" in sample:
+            initial[f"c{i}"] = (ldd, alpha, q0.data.clone())
+    assert len(server.catchments) == 200
+    requests = serving.poisson_requests(seed, 1000.0, 2.0, sorted(server.catchments))
+    done = server.run(requests)
+    torch.cuda.synchronize()
+    assert len(done) == len(requests)
+    stats = server.statistics(2.0)
+    print(f"[configs] serving 200 catchments: {stats['requests']} requests, p50 {stats['p50_ms']:.3f} ms, "
+          f"p99 {stats['p99_ms']:.3f} ms", flush=True)
+    checked = 0
+    for name, (ldd, alpha, q0) in initial.items():
+        alone = fw.FlowNetwork(ldd)
+        q = fw.PartitionedArray(q0.clone())
+        for r in sorted((r for r in done if r.catchment == name), key=lambda r: r.seq):
+            lateral = serving.Server.lateral_inflow(r.seed, ldd.shape, dev)
+            q = alone.kinematic_wave(q, lateral, alpha, 0.6, 3600.0, 100.0)
+            torch.cuda.synchronize()
+            assert torch.equal(q.data, r.result.data), name
+            checked += 1
+        alone.close()
+    assert checked > 0
+
+
+def test_large_domain_32768_24_steps(fw):
+    """32768^2 (1024^2 partitions), 24 kinematic-wave timesteps with the state resident: int64 upstream cell
+    counts exact by the neighbour recurrence at full size, every resident step equal to the per-call step."""
+    from lue_amd import synthetic as syn
+    from lue_amd import workloads
+
+    n, ps, seed = 32768, (1024, 1024), 20261206
+    dev = torch.device("cuda")
+    ldd = fw.PartitionedArray(syn.ldd_device(seed, n, n, dev), ps)
+    net = fw.FlowNetwork(ldd)
+    counts = workloads.upstream_cell_count(ldd, net)
+    assert int(counts[-1].sum()) == n * n == 2 ** 30
+    # recurrence, in bands of rows: count = 1 + the counts of the neighbours draining into the cell
+    codes = {(-1, 0): 2, (-1, 1): 1, (0, 1): 4, (1, 1): 7, (1, 0): 8, (1, -1): 9, (0, -1): 6, (-1, -1): 3}
+    band = 2048
+    for r0 in range(0, n, band):
+        lo, hi = max(r0 - 1, 0), min(r0 + band + 1, n)
+        c = torch.nn.functional.pad(counts[lo:hi], (1, 1, 1 if r0 == 0 else 0, 1 if hi == n else 0))
+        l = torch.nn.functional.pad(ldd.data[lo:hi].to(torch.int32), (1, 1, 1 if r0 == 0 else 0,
+                                                                      1 if hi == n else 0), value=255)
+        rows = min(band, n - r0)
+        expect = torch.ones((rows, n), dtype=torch.int64, device=dev)
+        for (dr, dc), code in codes.items():
+            nb = c[1 + dr:1 + dr + rows, 1 + dc:1 + dc + n]
+            nl = l[1 + dr:1 + dr + rows, 1 + dc:1 + dc + n]
+            expect += torch.where(nl == code, nb, torch.zeros_like(nb))
+        assert torch.equal(counts[r0:r0 + rows], expect), r0
+        del c, l, expect
+    alpha = fw.PartitionedArray(syn.uniform_device(seed, syn.STREAM_ALPHA, 1.5, 2.5, (n, n), torch.float32, dev), ps)
+    q_res = fw.PartitionedArray(counts.to(torch.float32) * 1e-3, ps)
+    q_call = fw.PartitionedArray(q_res.data.clone(), ps)
+    del counts
+    for step in range(24):
+        lateral = fw.PartitionedArray(syn.uniform_device(seed + 1 + step, syn.STREAM_LATERAL, 0.0, 1e-4, (n, n),
+                                                         torch.float32, dev), ps)
+        q_res = net.kinematic_wave(q_res, lateral, alpha, 0.6, 3600.0, 100.0)
+        q_call = fw.kinematic_wave(ldd, q_call, lateral, alpha, 0.6, 3600.0, 100.0)
+        assert torch.equal(q_res.data, q_call.data), step
+        del lateral
+    assert bool(torch.isfinite(q_res.data).all())
+    print(f"[configs] 32768^2: 24 steps, resident == per call; discharge sum {float(q_res.data.double().sum()):.6e}",
+          flush=True)
+    net.close()
