@@ -371,6 +371,23 @@ int lue_amd_network_kinematic_wave_f64(lue_amd_network network, const double* co
                                        const double* const* channel_length, double channel_length_scalar,
                                        double* const* outflow, void* stream);
 
+/*
+ * lue_amd_network_kinematic_wave_f32 for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation). No counterpart in the reference. It
+ * allocates nothing: its per-call buffers (the replay's pending bitmaps, compensation terms, staged chain
+ * inputs, side sums and chain results) live in one fixed 64 MiB device array per GPU, held until the process
+ * ends, which fits networks of up to about 4 million cells on the synthetic DEM (16.5 bytes per cell; a
+ * larger network returns LUE_AMD_ERROR_INVALID_ARGUMENT with the bytes it needs) and of at most 32
+ * catchments. The profiler (lue_amd_profile_*) does not time these calls. Every call on one GPU shares that
+ * array: calls that overlap on one GPU (on different streams, or from different threads) race; calls on one
+ * stream are ordered. Arguments and results as lue_amd_network_kinematic_wave_f32; asynchronous on `stream`.
+ */
+int lue_amd_network_kinematic_wave_f32_noalloc(lue_amd_network network, const float* const* current_outflow,
+                                               const float* const* inflow, const float* const* alpha,
+                                               float alpha_scalar, const float* const* beta, float beta_scalar,
+                                               float time_step_duration, const float* const* channel_length,
+                                               float channel_length_scalar, float* const* outflow, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

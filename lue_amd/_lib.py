@@ -44,6 +44,7 @@ SIGNATURES = {
     "lue_amd_network_accu_trigger_f64": [c_p, c_p, c_d, c_p, c_d, c_p, c_p, c_p],
     "lue_amd_network_kinematic_wave_f32": [c_p, c_p, c_p, c_p, c_f, c_p, c_f, c_f, c_p, c_f, c_p, c_p],
     "lue_amd_network_kinematic_wave_f64": [c_p, c_p, c_p, c_p, c_d, c_p, c_d, c_d, c_p, c_d, c_p, c_p],
+    "lue_amd_network_kinematic_wave_f32_noalloc": [c_p, c_p, c_p, c_p, c_f, c_p, c_f, c_f, c_p, c_f, c_p, c_p],
     "lue_amd_profile_enable": [c_int],
     "lue_amd_profile_read": [ctypes.POINTER(c_d), ctypes.POINTER(c_d)] + [ctypes.POINTER(ctypes.c_longlong)] * 3,
     "lue_amd_profile_read_op": [c_int, c_int] + [ctypes.POINTER(c_d)] * 4 + [ctypes.POINTER(ctypes.c_longlong)],
