@@ -152,43 +152,65 @@ namespace lue_amd {
         }
     }
 
-    // The same in FP32.
-    __device__ inline void affine_combine_f32(float a_prev, float m_prev, int f_prev, float& add, float& mul, int& flag)
+    // Affine maps without start flags: a lane whose element does not depend on the lanes before it (a chain
+    // start) has mul = 0, and compositions keep that (mul of the composition = product of the muls). On return
+    // x_k = add + mul x_in for every lane. Needs finite add and mul (0 * inf or NaN would leak across a start):
+    // the kinematic-wave block sends any block with a non-finite value to its per-cell path anyway.
+    template<int N>
+    __device__ inline void affine_step_nf(double& add, double& mul)
     {
-        if (!flag)
+        double const a_prev = row_shr<N>(add, 0.0);
+        double const m_prev = row_shr<N>(mul, 1.0);
+        add = fma(mul, a_prev, add);
+        mul = mul * m_prev;
+    }
+
+    __device__ inline void affine_scan_nf(double& add, double& mul)
+    {
+        affine_step_nf<1>(add, mul);
+        affine_step_nf<2>(add, mul);
+        affine_step_nf<4>(add, mul);
+        affine_step_nf<8>(add, mul);
         {
-            add = fmaf(mul, a_prev, add);
+            double const a_prev = row_bcast<dpp_row_bcast15, 0xA>(add, 0.0);
+            double const m_prev = row_bcast<dpp_row_bcast15, 0xA>(mul, 1.0);
+            add = fma(mul, a_prev, add);
             mul = mul * m_prev;
-            flag = f_prev;
+        }
+        {
+            double const a_prev = row_bcast<dpp_row_bcast31, 0xC>(add, 0.0);
+            double const m_prev = row_bcast<dpp_row_bcast31, 0xC>(mul, 1.0);
+            add = fma(mul, a_prev, add);
+            mul = mul * m_prev;
         }
     }
 
     template<int N>
-    __device__ inline void affine_step_f32(float& add, float& mul, int& flag)
+    __device__ inline void affine_step_f32_nf(float& add, float& mul)
     {
         float const a_prev = row_shr<N>(add, 0.0f);
         float const m_prev = row_shr<N>(mul, 1.0f);
-        int const f_prev = row_shr<N>(flag, 0);
-        affine_combine_f32(a_prev, m_prev, f_prev, add, mul, flag);
+        add = fmaf(mul, a_prev, add);
+        mul = mul * m_prev;
     }
 
-    __device__ inline void affine_scan_f32(float& add, float& mul, int& flag)
+    __device__ inline void affine_scan_f32_nf(float& add, float& mul)
     {
-        affine_step_f32<1>(add, mul, flag);
-        affine_step_f32<2>(add, mul, flag);
-        affine_step_f32<4>(add, mul, flag);
-        affine_step_f32<8>(add, mul, flag);
+        affine_step_f32_nf<1>(add, mul);
+        affine_step_f32_nf<2>(add, mul);
+        affine_step_f32_nf<4>(add, mul);
+        affine_step_f32_nf<8>(add, mul);
         {
             float const a_prev = row_bcast<dpp_row_bcast15, 0xA>(add, 0.0f);
             float const m_prev = row_bcast<dpp_row_bcast15, 0xA>(mul, 1.0f);
-            int const f_prev = row_bcast<dpp_row_bcast15, 0xA>(flag, 0);
-            affine_combine_f32(a_prev, m_prev, f_prev, add, mul, flag);
+            add = fmaf(mul, a_prev, add);
+            mul = mul * m_prev;
         }
         {
             float const a_prev = row_bcast<dpp_row_bcast31, 0xC>(add, 0.0f);
             float const m_prev = row_bcast<dpp_row_bcast31, 0xC>(mul, 1.0f);
-            int const f_prev = row_bcast<dpp_row_bcast31, 0xC>(flag, 0);
-            affine_combine_f32(a_prev, m_prev, f_prev, add, mul, flag);
+            add = fmaf(mul, a_prev, add);
+            mul = mul * m_prev;
         }
     }
 
@@ -303,16 +325,21 @@ namespace lue_amd {
     // Q_{k-1} being the previous position on the chain (`carry` before the first lane, nothing at a chain
     // start); per cell this is LUE's equation with Q_up = Q_{k-1} + extra_k. The Jacobian is lower
     // bidiagonal (d_k = c1_k + alpha_k beta_k Q_k^(beta_k - 1) on the diagonal, -c1_k below it), so a
-    // Newton step dQ_k = (-F_k + c1_k dQ_{k-1}) / d_k is one affine scan. The start is the system linearised
-    // at Q_cur, an affine recurrence as well. F is concave in every Q_k and the Jacobian inverse-positive,
-    // so the iterates approach the root from below after the first step. The iteration stops when every
-    // |dQ_k| <= |Q_k| 2^(1 - digits), LUE's criterion (digits 31 for double, 14 for float).
-    // T = float: the iterations use the hardware FP32 power (log2, exp2) and approximate reciprocals; one
-    // Newton step on the FP64 residual follows (the FP64 residual correction; its scan in FP32).
+    // Newton step dQ_k = (-F_k + c1_k dQ_{k-1}) / d_k is one affine scan (a chain start multiplies the value
+    // before it by 0, so the scans need no start flags). The start is the system linearised at Q_cur, an
+    // affine recurrence as well. F is concave in every Q_k and the Jacobian inverse-positive, so the iterates
+    // approach the root from below after the first step.
+    // T = double: FP64 steps until every |dQ_k| <= |Q_k| 2^-30, LUE's FP64 criterion (digits 31).
+    // T = float: FP32 steps with the hardware power (log2, exp2) and reciprocal until every
+    // |dQ_k| <= |Q_k| 2^-10, then one Newton step on the FP64 residual (the FP64 residual correction; its
+    // scan in FP32). Newton's relative error obeys r' <= |beta - 1| / (2 (1 - r)) (1 + r)^|beta - 2| r^2, so
+    // after the step that meets 2^-10 the error is below about 4e-7 (beta in [0.3, 3]) and after the FP64
+    // step below 1e-12: the correction, not more FP32 steps, sets the accuracy.
     // Returns kw_block_serial, with q untouched, when a lane needs LUE's per-cell iteration: no-data, a zero
     // current discharge (dry cells and cells whose iteration starts from numeric_limits::min()), a non-positive
-    // iterate, or no convergence within 16 steps; kw_block_lue_far when LUE's own iteration would stop short
-    // of the root in some lane (lue_guess_converges; q then holds the roots and far_mask those lanes).
+    // or non-finite iterate, or no convergence within 16 steps; kw_block_lue_far when LUE's own iteration
+    // would stop short of the root in some lane (lue_guess_converges; q then holds the roots and far_mask
+    // those lanes).
     template<typename T>
     __device__ inline int kinematic_wave_block(ChainLane<op_kinematic_wave, T> const& d, double carry, double dt,
                                                 double& q, std::uint64_t& far_mask)
@@ -330,6 +357,9 @@ namespace lue_amd {
         float const c1f = static_cast<float>(c1);
         float const beta32 = static_cast<float>(d.in.beta);
         float const ab32 = static_cast<float>(d.in.alpha) * beta32;
+        // the value before lane k is multiplied by c1 (its upstream neighbour on the chain), except at a chain
+        // start and past the block's positions
+        bool const chained = valid && !start;
 
         // start: F linearised at Q_cur gives Q_k = w_k (Q_{k-1} + extra_k) + g_k, in FP32 (a starting point:
         // b - (1 - beta) alpha Q_cur^beta = beta alpha Q_cur^beta + dt q_lat does not cancel)
@@ -338,30 +368,28 @@ namespace lue_amd {
         {
             float add = 0.0f;
             float mul = 0.0f;
-            int flag = (start || !valid) ? 1 : 0;
             if (valid)
             {
                 float const q_cur32 = static_cast<float>(q_cur);
                 pc1 = fast_exp2((beta32 - 1.0f) * fast_log2(q_cur32));
                 float const inv = fast_rcp(c1f + ab32 * pc1);
-                mul = c1f * inv;
-                add = mul * static_cast<float>(d.extra) +
+                float const w = c1f * inv;
+                mul = chained ? w : 0.0f;
+                add = w * static_cast<float>(d.extra) +
                       (static_cast<float>(b) - (1.0f - beta32) * static_cast<float>(alpha) * q_cur32 * pc1) * inv;
             }
-            affine_scan_f32(add, mul, flag);
-            x = valid ? (flag ? static_cast<double>(add) : fma(static_cast<double>(mul), carry, static_cast<double>(add)))
-                      : 1.0;
+            affine_scan_f32_nf(add, mul);
+            x = valid ? fma(static_cast<double>(mul), carry, static_cast<double>(add)) : 1.0;
         }
-        if (__builtin_expect(__ballot(valid && !(x > 0.0)) != 0, 0))
+        if (__builtin_expect(__ballot(valid && !(x > 0.0 && x < __builtin_huge_val())) != 0, 0))
             return kw_block_serial;
 
-        // Newton steps: F in FP64; the power (T = float: FP32 hardware log2/exp2), the Jacobian and the
-        // step scan in FP32 for T = float, FP64 for T = double
-        constexpr double tol = std::is_same_v<T, double> ? 0x1p-30 : 0x1p-13;
+        // Newton steps: T = float: FP32 residual, power (hardware log2/exp2), Jacobian and step scan; its
+        // rounding (2^-24 of the largest term) moves the iterate by about 2^-24 of Q (the Jacobian is at least
+        // c1), far below the stop criterion, and the FP64 residual correction below removes it. T = double:
+        // all in FP64
+        constexpr double tol = std::is_same_v<T, double> ? 0x1p-30 : 0x1p-10;
         bool converged = false;
-        // T = float: the residual in FP32 as well; its rounding (2^-24 of the largest term) moves the iterate by
-        // at most about 2^-24 of Q (the Jacobian is at least c1), far below the stop criterion, and the FP64
-        // residual correction below removes it
         float const carry32 = static_cast<float>(carry);
         float const extra32 = static_cast<float>(d.extra);
         float const alpha32 = static_cast<float>(alpha);
@@ -383,9 +411,8 @@ namespace lue_amd {
                 inv = inv * fma(-fp, inv, 2.0);
                 inv = inv * fma(-fp, inv, 2.0);
                 double add = valid ? -f * inv : 0.0;
-                double mul = valid ? c1 * inv : 0.0;
-                int flag = (start || !valid) ? 1 : 0;
-                affine_scan(add, mul, flag);
+                double mul = chained ? c1 * inv : 0.0;
+                affine_scan_nf(add, mul);
                 step = add;  // the carry is exact: its step is 0
             }
             else
@@ -400,13 +427,12 @@ namespace lue_amd {
                 float const f = c1f * (x32 - prev32 - extra32) + alpha32 * x32 * pw1 - b32;
                 float const inv = fast_rcp(c1f + ab32 * pw1);
                 float add = valid ? -f * inv : 0.0f;
-                float mul = valid ? c1f * inv : 0.0f;
-                int flag = (start || !valid) ? 1 : 0;
-                affine_scan_f32(add, mul, flag);
+                float mul = chained ? c1f * inv : 0.0f;
+                affine_scan_f32_nf(add, mul);
                 step = static_cast<double>(add);
             }
             double const xn = x + step;
-            if (__builtin_expect(__ballot(valid && !(xn > 0.0)) != 0, 0))
+            if (__builtin_expect(__ballot(valid && !(xn > 0.0 && xn < __builtin_huge_val())) != 0, 0))
                 return kw_block_serial;
             converged = __ballot(valid && !(fabs(step) <= fabs(xn) * tol)) == 0;
             x = xn;
@@ -415,23 +441,22 @@ namespace lue_amd {
             return kw_block_serial;
         if constexpr (std::is_same_v<T, float>)
         {
-            // FP64 residual correction: one Newton step on the FP64 residual (FP64 power). The step is about
-            // 1e-7 of x, so its Jacobian and scan in FP32 (relative error 1e-7 of the step) leave x within
-            // about 1e-14 of the FP64 step's result
+            // FP64 residual correction: one Newton step on the FP64 residual (the short FP64 power, relative
+            // error about 1e-12). The step is below about 1e-6 of x, so its Jacobian and scan in FP32
+            // (relative error 1e-7 of the step) leave x within about 1e-13 of the FP64 step's result
             double prev = wave_shr1(x);
             if (lane == 0)
                 prev = carry;
             if (start)
                 prev = 0.0;
-            double const pw1 = pow_positive(x, beta - 1.0);
+            double const pw1 = pow_positive<true, true>(x, beta - 1.0);
             double const f = c1 * (x - prev - d.extra) + alpha * x * pw1 - b;
             float const inv = fast_rcp(c1f + ab32 * static_cast<float>(pw1));
             float add = valid ? static_cast<float>(-f) * inv : 0.0f;
-            float mul = valid ? c1f * inv : 0.0f;
-            int flag = (start || !valid) ? 1 : 0;
-            affine_scan_f32(add, mul, flag);
+            float mul = chained ? c1f * inv : 0.0f;
+            affine_scan_f32_nf(add, mul);
             double const xn = x + static_cast<double>(add);
-            if (__builtin_expect(__ballot(valid && !(xn > 0.0)) != 0, 0))
+            if (__builtin_expect(__ballot(valid && !(xn > 0.0 && xn < __builtin_huge_val())) != 0, 0))
                 return kw_block_serial;
             x = xn;
         }

@@ -193,15 +193,16 @@ namespace lue_amd {
         return static_cast<double>(c.c1);
     }
 
-    // Q_cur^beta of the FP32 path's b = alpha Q_cur^beta + dt q, in FP64 (pow_positive, relative error
-    // about 1e-15): b enters every root, and where LUE's own iteration stops short of the root (see
-    // lue_guess_converges) its result amplifies input errors, so b is formed as accurately as on the FP64
-    // path. pow for Q_cur = 0 and non-finite arguments.
+    // Q_cur^beta of the FP32 path's b = alpha Q_cur^beta + dt q, in FP64 (the short pow_positive, relative
+    // error below about 1e-12): b enters every root, and where LUE's own iteration stops short of the root
+    // (see lue_guess_converges) its result amplifies input errors about 20-fold, so b is formed far more
+    // accurately than FP32 (the hardware FP32 power loses |beta log2 Q_cur| 2^-24). pow for Q_cur = 0 and
+    // non-finite arguments.
     __device__ inline double kw_b_pow(float q_cur, float beta)
     {
         if (!(q_cur > 0.0f) || !isfinite(q_cur) || !isfinite(beta))
             return static_cast<double>(powf(q_cur, beta));
-        return pow_positive(static_cast<double>(q_cur), static_cast<double>(beta));
+        return pow_positive<true, true>(static_cast<double>(q_cur), static_cast<double>(beta));
     }
 
     // Lateral inflow of a cell: stored (FP64 path) or recovered from b = alpha Q_cur^beta + dt q (FP32 path,
@@ -295,7 +296,7 @@ namespace lue_amd {
                 c.q_cur = ok ? q_cur : quiet_nan<T>();
                 c.alpha = alpha;
                 c.beta = beta;
-                c.c1 = static_cast<float>(dt / static_cast<double>(dx));
+                c.c1 = a.dt / dx;  // correctly rounded FP32 division: the FP64 quotient rounded to FP32
                 c.b_hi = static_cast<float>(b);
                 c.b_lo = static_cast<float>(b - static_cast<double>(c.b_hi));
             }

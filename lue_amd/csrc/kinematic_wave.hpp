@@ -54,13 +54,29 @@ namespace lue_amd {
         return exp(y * log(x));
     }
 
-    // x^y for finite x > 0 and |y ln x| < 700, relative error below 2^-52 |y ln x| + 1e-15 (the roundings of
-    // ln x and y ln x; about 40 FP64 operations, a third of exp(y log x) with the library's full-precision
-    // exp and log). Used by the block Newton iterations, whose roots it moves by about its own error.
-    //   ln x = e ln 2 + 2 atanh(s), s = (m - 1) / (m + 1), m in [sqrt(1/2), sqrt(2)), |s| < 0.1716: the
-    //   atanh series to s^15 leaves 2 |s|^17 / 17 < 1.1e-14;
-    //   e^z = 2^k e^r, k = rint(z / ln 2), |r| <= ln 2 / 2 (ln 2 in two parts): Taylor to r^12 leaves
-    //   |r|^13 / 13! < 2e-16.
+    // a * b + c with c in scalar registers: VOP3 v_fma_f64 takes one SGPR operand, where the compiler's own
+    // choice (v_fmac_f64, accumulator = addend) first copies every coefficient into vector registers.
+    __host__ __device__ inline double fma_scalar_addend(double a, double b, double c)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        double r;
+        asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+        return r;
+#else
+        return std::fma(a, b, c);
+#endif
+    }
+
+    // x^y for finite x > 0 and |y ln x| < 700.
+    //   ln x = e ln 2 + 2 atanh(s), s = (m - 1) / (m + 1), m in [sqrt(1/2), sqrt(2)), |s| < 0.1716;
+    //   e^z = 2^k e^r, k = rint(z / ln 2), |r| <= ln 2 / 2 (ln 2 in two parts, double-double).
+    // Full (Short = false): the atanh series to s^15 leaves 2 |s|^17 / 17 < 1.1e-14, Taylor to r^12 leaves
+    // |r|^13 / 13! < 2e-16: relative error below 2^-52 |y ln x| + 1e-15. Used by the FP64 path.
+    // Short: s from the FP32 hardware reciprocal of m + 1 refined by one Newton step (relative 2^-46, no FP64
+    // division), the atanh series to s^13 (2 |s|^15 / 15 < 4.4e-13) and Taylor to r^10 (|r|^11 / 11! <
+    // 2.2e-13): relative error below 4.4e-13 |y| + 2.2e-13 + 2^-52 |y ln x|, which moves a kinematic-wave
+    // root by at most that over beta, far below the FP32 path's output rounding (6e-8). Used by the FP32
+    // path's b and its FP64 residual correction.
     // The coefficients live in constant memory, read with scalar loads in every call: as literals the
     // compiler keeps all of them in vector registers across the kernels' loops (and spills them there).
     // [0, 7]: 1/15, 1/13, ..., 1/3, 1 (atanh series); [8]: ln2_hi, [9]: ln2_lo (ln 2 = ln2_hi + ln2_lo,
@@ -75,7 +91,7 @@ namespace lue_amd {
     // Reload: read the table in every call (an opaque copy of its address keeps the scalar loads in place);
     // otherwise the compiler hoists the loads out of the kernels' loops into scalar registers, which pays
     // where the power runs in a dependent chain of Newton steps and the wave has registers to spare.
-    template<bool Reload = true>
+    template<bool Reload = true, bool Short = false>
     __host__ __device__ inline double pow_positive(double x, double y)
     {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -96,18 +112,35 @@ namespace lue_amd {
             m *= 2.0;
             e -= 1;
         }
-        double const s = (m - 1.0) / (m + 1.0);
+        double s;
+        if constexpr (Short)
+        {
+            double const t = m + 1.0;
+#if defined(__HIP_DEVICE_COMPILE__)
+            double r = static_cast<double>(__builtin_amdgcn_rcpf(static_cast<float>(t)));
+#else
+            double r = static_cast<double>(1.0f / static_cast<float>(t));
+#endif
+            r = fma(r, fma(-t, r, 1.0), r);
+            s = (m - 1.0) * r;
+        }
+        else
+        {
+            s = (m - 1.0) / (m + 1.0);
+        }
         double const s2 = s * s;
-        double p = k[0];
-        for (int i = 1; i < 8; ++i)
-            p = fma(p, s2, k[i]);
+        constexpr int atanh_first = Short ? 1 : 0;
+        double p = k[atanh_first];
+        for (int i = atanh_first + 1; i < 8; ++i)
+            p = fma_scalar_addend(p, s2, k[i]);
         double const ln_x = fma(static_cast<double>(e), k[8], fma(static_cast<double>(e), k[9], 2.0 * s * p));
         double const z = y * ln_x;
         double const n = rint(z * k[10]);
         double const r = fma(-n, k[9], fma(-n, k[8], z));
-        double q = k[11];
-        for (int i = 12; i < 24; ++i)
-            q = fma(q, r, k[i]);
+        constexpr int taylor_first = Short ? 13 : 11;
+        double q = k[taylor_first];
+        for (int i = taylor_first + 1; i < 24; ++i)
+            q = fma_scalar_addend(q, r, k[i]);
         return ldexp(q, static_cast<int>(n));
     }
 
