@@ -435,7 +435,7 @@ def main():
                         "fp32_speedup": round(value / value64, 3),
                         "per_call_ms_per_step": round(t64_call * 1e3, 3)},
         "fp32_per_call": {"value": round(cells / t32_call / 1e6, 1), "ms_per_step": round(t32_call * 1e3, 3),
-                          "note": "drop-in entry points (schedule kept per ldd fingerprint)"},
+                          "note": "per-call operations of the Python front end (each enqueues its kernels on the flow network the ldd array keeps)"},
         "network": {"record_ms": round(t_network * 1e3, 1), "kinematic_wave": info_kw, "accu": info_accu},
         "cold_call_ms": round(cold_ms, 1),
         "accuracy_vs_fp64": {"accu_max_rel": acc_err[0], "accu_mean_rel": acc_err[1],

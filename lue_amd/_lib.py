@@ -131,14 +131,32 @@ def load():
 SCHEDULES = {"chains": 0, "rounds": 1}
 
 
+# mirrors of the per-call settings, read by the Python front end
+_schedule = "chains"
+_network_cache_entries = 4
+
+
 def set_sweep_options(schedule="chains"):
     """Schedule of the per-call operations after the first round of tiles: "chains" or "rounds"."""
+    global _schedule
     check(load().lue_amd_set_sweep_options(SCHEDULES[schedule]))
+    _schedule = schedule
+
+
+def sweep_schedule():
+    return _schedule
 
 
 def set_network_cache(entries=4):
-    """Schedules the per-call operations keep (0: record in every call)."""
+    """Schedules the per-call operations keep (0: record in every call). The C ABI keeps up to `entries`
+    schedules found by an ldd fingerprint; the Python front end keeps one per ldd array (0: none)."""
+    global _network_cache_entries
     check(load().lue_amd_set_network_cache(entries))
+    _network_cache_entries = entries
+
+
+def network_cache_entries():
+    return _network_cache_entries
 
 
 def trim_scratch():

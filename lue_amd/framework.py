@@ -7,6 +7,12 @@ downstream}.cpp, which call lue::value_policies::* on PartitionedArray.
 PartitionedArray here is device resident: one contiguous row-major raster in HBM (a torch tensor used as
 device memory) plus LUE's partition shape, which is kept as metadata (results do not depend on it). A
 Scalar argument is a Python number or numpy scalar (LUE's Scalar<E> and plain-value overloads).
+
+Operations are asynchronous on torch's current stream, like LUE's operations, which return future-backed
+partitions at once (partitioned_array_decl.hpp:23-98): they enqueue their kernels and return; reading a
+result (numpy(), torch ops) on the same stream is ordered after them. The routing operations run on the
+flow network (recorded schedule) of their ldd, which a PartitionedArray of uint8 keeps after its first
+routing operation, until its data change (torch's version counter of the tensor) or it goes away.
 """
 from __future__ import annotations
 
@@ -66,6 +72,24 @@ class PartitionedArray:
     def numpy(self):
         return self.data.cpu().numpy()
 
+    def _routing_network(self, tile_shape=(32, 32)):
+        """The flow network of this ldd array for the per-call routing operations: recorded at the first
+        call (which waits for the recording), kept while the tensor and its contents are unchanged, so later
+        calls only enqueue kernels. None when the per-call schedules are not kept (set_network_cache(0))."""
+        if _lib.network_cache_entries() == 0:
+            return None
+        key = (self.data.data_ptr(), tuple(self.data.shape), self.data._version, _lib.sweep_schedule(),
+               tuple(tile_shape), self.data.device)
+        cached = self.__dict__.get("_network")
+        if cached is not None and cached[0] == key:
+            return cached[1]
+        self.__dict__.pop("_network", None)
+        if cached is not None:
+            cached[1].close()
+        net = FlowNetwork(self, tile_shape=tile_shape, schedule=_lib.sweep_schedule())
+        self._network = (key, net)
+        return net
+
     @classmethod
     def from_numpy(cls, array, partition_shape=None, device=None):
         device = device or _default_device()
@@ -112,6 +136,11 @@ def _element_type(*args):
     if dtype not in FLOAT_TYPES:
         raise TypeError(f"material element type must be float32 or float64, not {dtype}")
     return dtype
+
+
+def _scalar(value, dtype):
+    """A raster as it is; a Scalar as a float (the network entry points take it in the operation's type)."""
+    return value if isinstance(value, PartitionedArray) else float(value)
 
 
 def _arg(value, dtype):
@@ -215,6 +244,9 @@ def accu(flow_direction, inflow):
     _check_ldd(flow_direction)
     _verify_compatible(flow_direction, inflow)
     dtype = _element_type(inflow)
+    net = flow_direction._routing_network()
+    if net is not None:
+        return net.accu(inflow if isinstance(inflow, PartitionedArray) else _scalar(inflow, dtype), dtype=dtype)
     lib = _lib.load()
     out = _output(flow_direction, dtype)
     p, s = _arg(inflow, dtype)
@@ -228,6 +260,9 @@ def _split(name, flow_direction, inflow, param):
     _check_ldd(flow_direction)
     _verify_compatible(flow_direction, inflow, param)
     dtype = _element_type(inflow, param)
+    net = flow_direction._routing_network()
+    if net is not None:
+        return net._split(name[len("accu_"):], _scalar(inflow, dtype), _scalar(param, dtype), dtype=dtype)
     lib = _lib.load()
     out = _output(flow_direction, dtype)
     rem = _output(flow_direction, dtype)
@@ -269,6 +304,10 @@ def kinematic_wave(flow_direction, current_outflow, inflow, alpha, beta, time_st
     dtype = _element_type(current_outflow, inflow, alpha, beta, channel_length)
     if isinstance(time_step_duration, PartitionedArray):
         raise TypeError("time_step_duration is a Scalar")
+    net = flow_direction._routing_network()
+    if net is not None:
+        return net.kinematic_wave(current_outflow, inflow, _scalar(alpha, dtype), _scalar(beta, dtype),
+                                  float(time_step_duration), _scalar(channel_length, dtype), dtype=dtype)
     lib = _lib.load()
     out = _output(flow_direction, dtype)
     pa, sa = _arg(alpha, dtype)
@@ -320,9 +359,11 @@ class FlowNetwork:
 
         if isinstance(flow_directions, PartitionedArray):
             flow_directions = [flow_directions]
-        self.flow_directions = list(flow_directions)
-        for ldd in self.flow_directions:
+        for ldd in flow_directions:
             _check_ldd(ldd)
+        # views of the same tensors: the network keeps the ldd data alive (it borrows the device pointers)
+        # without referring to the arrays themselves, which may hold this network
+        self.flow_directions = [PartitionedArray(ldd.data, ldd.partition_shape) for ldd in flow_directions]
         lib = _lib.load()
         n = len(self.flow_directions)
         ptrs = _pointer_array([_ptr(a.data) for a in self.flow_directions])
@@ -398,7 +439,9 @@ class FlowNetwork:
         return _pointer_array([_ptr(o.data) for o in out]), out
 
     @staticmethod
-    def _dtype(*values):
+    def _dtype(*values, dtype=None):
+        if dtype is not None:
+            return dtype
         for v in values:
             if isinstance(v, (list, tuple)) and v:
                 return v[0].dtype
@@ -410,8 +453,8 @@ class FlowNetwork:
         return outs[0] if len(self) == 1 else outs
 
     # -- operations (asynchronous on the current stream) --------------------------------------------
-    def accu(self, inflow, out=None):
-        dtype = self._dtype(inflow)
+    def accu(self, inflow, out=None, dtype=None):
+        dtype = self._dtype(inflow, dtype=dtype)
         p, s, keep = self._rasters(inflow, dtype, "inflow")
         po, outs = self._outputs(dtype, out)
         fn = getattr(_lib.load(), f"lue_amd_network_accu_{FLOAT_TYPES[dtype]}")
@@ -419,8 +462,8 @@ class FlowNetwork:
         del keep
         return self._unwrap(outs)
 
-    def _split(self, name, inflow, param, out=None, remainder=None):
-        dtype = self._dtype(inflow, param)
+    def _split(self, name, inflow, param, out=None, remainder=None, dtype=None):
+        dtype = self._dtype(inflow, param, dtype=dtype)
         p1, s1, k1 = self._rasters(inflow, dtype, "inflow")
         p2, s2, k2 = self._rasters(param, dtype, name)
         po, outs = self._outputs(dtype, out)
@@ -442,8 +485,9 @@ class FlowNetwork:
     def accu_trigger(self, inflow, trigger, out=None, remainder=None):
         return self._split("trigger", inflow, trigger, out, remainder)
 
-    def kinematic_wave(self, current_outflow, inflow, alpha, beta, time_step_duration, channel_length, out=None):
-        dtype = self._dtype(current_outflow, inflow)
+    def kinematic_wave(self, current_outflow, inflow, alpha, beta, time_step_duration, channel_length, out=None,
+                       dtype=None):
+        dtype = self._dtype(current_outflow, inflow, dtype=dtype)
         pc, _, kc = self._rasters(current_outflow, dtype, "current_outflow")
         pq, _, kq = self._rasters(inflow, dtype, "inflow")
         if pc is None or pq is None:

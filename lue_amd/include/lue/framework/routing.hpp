@@ -12,10 +12,17 @@
 //     d8_flow_direction, upstream, downstream} with the reference's overload sets (ALG/value_policies/*.hpp) and
 //     lue::inflow_count<std::uint8_t>(ldd) and lue::downstream_distance(ldd, cell_size)
 //     (ALG/inflow_count.hpp:29-42), ALG = source/framework/algorithm/include/lue/framework/algorithm.
-// Each operation calls one function of the C ABI (include/lue_amd/routing.h). Shape and partitioning
-// mismatches throw std::runtime_error as detail::verify_compatible does (ALG/detail/verify_compatible.hpp:
-// 22-41); other failures throw std::runtime_error with the library's message. Results are complete when a
-// call returns.
+// Each operation calls the C ABI (include/lue_amd/routing.h). Shape and partitioning mismatches throw
+// std::runtime_error as detail::verify_compatible does (ALG/detail/verify_compatible.hpp:22-41); other
+// failures throw std::runtime_error with the library's message.
+//
+// Operations are asynchronous, like LUE's, whose partitions are hpx::future-backed components
+// (partitioned_array_decl.hpp:23-98): an operation makes the front end's stream (lue::set_stream, default the
+// null stream) wait for the completion events of its input arrays, enqueues its kernels, records a new
+// completion event into its output arrays and returns. to_host() waits for the array's event; ready() tells
+// whether it has completed. The routing operations run on the flow network (recorded schedule) of their ldd
+// array, which the array keeps after its first routing operation and drops when its contents change
+// (from_host, set_partitions, mutable_data); recording it is the only step that waits for the device.
 #pragma once
 
 #include "lue_amd/routing.h"
@@ -60,7 +67,62 @@ namespace lue {
             return "(" + std::to_string(shape[0]) + ", " + std::to_string(shape[1]) + ")";
         }
 
+        // The stream the front end enqueues its operations on (per thread; null: the default stream).
+        inline auto front_end_stream() -> void*&
+        {
+            thread_local void* stream = nullptr;
+            return stream;
+        }
+
+        // Completion event of an array's contents (a hipEvent_t behind the C ABI).
+        class Event
+        {
+            public:
+
+                Event() { check(lue_amd_event_create(&_event)); }
+
+                Event(Event const&) = delete;
+                auto operator=(Event const&) -> Event& = delete;
+
+                ~Event() { (void)lue_amd_event_destroy(_event); }
+
+                auto handle() const -> void* { return _event; }
+
+            private:
+
+                void* _event{nullptr};
+        };
+
+        // A recorded flow network (lue_amd_network).
+        struct NetworkHandle
+        {
+            lue_amd_network network{nullptr};
+
+            NetworkHandle() = default;
+            NetworkHandle(NetworkHandle const&) = delete;
+            auto operator=(NetworkHandle const&) -> NetworkHandle& = delete;
+
+            ~NetworkHandle()
+            {
+                if (network)
+                {
+                    (void)lue_amd_network_destroy(network);
+                }
+            }
+        };
+
     }  // namespace detail
+
+    // Stream of the front end's operations on this thread (a hipStream_t; nullptr: the default stream).
+    inline void set_stream(void* stream)
+    {
+        detail::front_end_stream() = stream;
+    }
+
+    inline auto stream() -> void*
+    {
+        return detail::front_end_stream();
+    }
 
     // A value broadcast to every cell (LUE's Scalar<E>).
     template<typename Element>
@@ -154,26 +216,101 @@ namespace lue {
                 return {e0, e1};
             }
 
-            auto data() const -> Element* { return _data.get(); }
+            // Device pointer to read the raster (row-major); mutable_data() to write it.
+            auto data() const -> Element const* { return _data.get(); }
+
+            // Device pointer to write the raster: the array's contents count as changed (a flow network kept
+            // for it is recorded anew by the next routing operation).
+            auto mutable_data() -> Element*
+            {
+                ++_version;
+                return _data.get();
+            }
 
             auto valid() const -> bool { return static_cast<bool>(_data); }
 
-            // Whole raster, row-major.
+            // Whether the operation that wrote the array has completed.
+            auto ready() const -> bool
+            {
+                if (!_ready)
+                {
+                    return true;
+                }
+                int complete = 0;
+                detail::check(lue_amd_event_query(_ready->handle(), &complete));
+                return complete != 0;
+            }
+
+            // Wait (host) for the operation that wrote the array.
+            void wait() const
+            {
+                if (_ready)
+                {
+                    detail::check(lue_amd_event_synchronize(_ready->handle()));
+                }
+            }
+
+            // Make `stream` wait for the operation that wrote the array.
+            void order_before(void* stream) const
+            {
+                if (_ready)
+                {
+                    detail::check(lue_amd_stream_wait_event(stream, _ready->handle()));
+                }
+            }
+
+            // The array was written by work enqueued on `stream` up to now.
+            void written_on(void* stream)
+            {
+                auto event = std::make_shared<detail::Event>();
+                detail::check(lue_amd_event_record(event->handle(), stream));
+                _ready = std::move(event);
+            }
+
+            // Contents version: changes with every write through from_host, set_partitions or mutable_data.
+            auto version() const -> std::uint64_t { return _version; }
+
+            // Whole raster, row-major (waits for the operation that wrote it).
             auto to_host() const -> std::vector<Element>
             {
                 std::vector<Element> host(static_cast<std::size_t>(nr_elements()));
-                detail::check(lue_amd_copy_to_host(host.data(), _data.get(), sizeof(Element) * host.size(), nullptr));
+                void* const s = lue::stream();
+                order_before(s);
+                detail::check(lue_amd_copy_to_host(host.data(), _data.get(), sizeof(Element) * host.size(), s));
                 return host;
             }
 
+            // Overwrite the raster (in the front end's stream order, after the operations enqueued before).
             void from_host(std::vector<Element> const& host)
             {
                 if (static_cast<Count>(host.size()) != nr_elements())
                 {
                     throw std::runtime_error("host buffer size differs from the array size");
                 }
+                void* const s = lue::stream();
+                order_before(s);
                 detail::check(
-                    lue_amd_copy_to_device(_data.get(), host.data(), sizeof(Element) * host.size(), nullptr));
+                    lue_amd_copy_to_device(mutable_data(), host.data(), sizeof(Element) * host.size(), s));
+                _ready.reset();
+            }
+
+            // The flow network of this (ldd) array for the routing operations, recorded at the first call.
+            auto network() const -> lue_amd_network
+            {
+                if (!_network || _network_version != _version)
+                {
+                    void* const s = lue::stream();
+                    order_before(s);
+                    auto handle = std::make_shared<detail::NetworkHandle>();
+                    std::uint8_t const* ldd = reinterpret_cast<std::uint8_t const*>(_data.get());
+                    std::int64_t const rows = _shape[0];
+                    std::int64_t const cols = _shape[1];
+                    detail::check(lue_amd_network_create(1, &ldd, &rows, &cols, 0, LUE_AMD_SCHEDULE_CHAINS, s,
+                                                         &handle->network));
+                    _network = std::move(handle);
+                    _network_version = _version;
+                }
+                return _network->network;
             }
 
             // Set partition by partition from per-partition row-major values, partitions in row-major
@@ -215,6 +352,10 @@ namespace lue {
             Shape _shape{0, 0};
             Shape _partition_shape{1, 1};
             std::shared_ptr<Element> _data;
+            std::shared_ptr<detail::Event> _ready;  // completion of the operation that wrote the array
+            std::uint64_t _version{0};
+            mutable std::shared_ptr<detail::NetworkHandle> _network;  // ldd arrays: recorded flow network
+            mutable std::uint64_t _network_version{0};
     };
 
     template<typename Element>
@@ -279,6 +420,18 @@ namespace lue {
             return PartitionedArray<E, 2>{ldd.shape(), ldd.partition_shape()};
         }
 
+        // Make the stream wait for an input array (a Scalar needs nothing).
+        template<typename E>
+        void order_before(PartitionedArray<E, 2> const& a, void* stream)
+        {
+            a.order_before(stream);
+        }
+
+        template<typename E>
+        void order_before(Scalar<E> const&, void*)
+        {
+        }
+
         template<typename T>
         concept RasterOrScalar = requires(T const& t) { pointer(t); };
 
@@ -291,8 +444,11 @@ namespace lue {
         -> PartitionedArray<CountElement, 2>
     {
         auto result = detail::output_like<CountElement>(flow_direction);
-        detail::check(lue_amd_inflow_count_u8(
-            flow_direction.data(), flow_direction.shape()[0], flow_direction.shape()[1], result.data(), nullptr));
+        void* const s = lue::stream();
+        flow_direction.order_before(s);
+        detail::check(lue_amd_inflow_count_u8(flow_direction.data(), flow_direction.shape()[0],
+                                              flow_direction.shape()[1], result.mutable_data(), s));
+        result.written_on(s);
         return result;
     }
 
@@ -331,8 +487,11 @@ namespace lue {
             auto result = output_like<E>(flow_direction);
             auto const [rows, cols] = flow_direction.shape();
             auto const fn = upstream ? FlowNeighbourOps<E>::upstream : FlowNeighbourOps<E>::downstream;
-            check(fn(flow_direction.data(), material.data(), rows, cols, result.data(), nullptr));
-            check(lue_amd_stream_synchronize(nullptr));
+            void* const s = lue::stream();
+            flow_direction.order_before(s);
+            material.order_before(s);
+            check(fn(flow_direction.data(), material.data(), rows, cols, result.mutable_data(), s));
+            result.written_on(s);
             return result;
         }
 
@@ -348,13 +507,15 @@ namespace lue {
     {
         auto result = detail::output_like<Distance>(flow_direction);
         auto const [rows, cols] = flow_direction.shape();
+        void* const s = lue::stream();
+        flow_direction.order_before(s);
         if constexpr (std::is_same_v<Distance, float>)
             detail::check(lue_amd_downstream_distance_f32(flow_direction.data(), rows, cols, cell_size, 0,
-                                                          result.data(), nullptr));
+                                                          result.mutable_data(), s));
         else
             detail::check(lue_amd_downstream_distance_f64(flow_direction.data(), rows, cols, cell_size, 0,
-                                                          result.data(), nullptr));
-        detail::check(lue_amd_stream_synchronize(nullptr));
+                                                          result.mutable_data(), s));
+        result.written_on(s);
         return result;
     }
 
@@ -389,11 +550,13 @@ namespace lue {
         {
             PartitionedArray<FlowDirection, 2> result{elevation.shape(), elevation.partition_shape()};
             auto const [rows, cols] = elevation.shape();
+            void* const s = lue::stream();
+            elevation.order_before(s);
             if constexpr (std::is_same_v<E, float>)
-                detail::check(lue_amd_d8_flow_direction_f32(elevation.data(), rows, cols, result.data(), nullptr));
+                detail::check(lue_amd_d8_flow_direction_f32(elevation.data(), rows, cols, result.mutable_data(), s));
             else
-                detail::check(lue_amd_d8_flow_direction_f64(elevation.data(), rows, cols, result.data(), nullptr));
-            detail::check(lue_amd_stream_synchronize(nullptr));
+                detail::check(lue_amd_d8_flow_direction_f64(elevation.data(), rows, cols, result.mutable_data(), s));
+            result.written_on(s);
             return result;
         }
 
@@ -406,17 +569,20 @@ namespace lue {
         {
             detail::verify_compatible(flow_direction, inflow);
             auto outflow = detail::output_like<E>(flow_direction);
-            auto const [rows, cols] = flow_direction.shape();
+            lue_amd_network const network = flow_direction.network();
+            void* const s = lue::stream();
+            detail::order_before(inflow, s);
+            E const* const in[1] = {detail::pointer(inflow)};
+            E* const out[1] = {outflow.mutable_data()};
             if constexpr (std::is_same_v<E, float>)
             {
-                detail::check(lue_amd_accu_f32(flow_direction.data(), rows, cols, detail::pointer(inflow),
-                                               detail::scalar(inflow), outflow.data(), nullptr));
+                detail::check(lue_amd_network_accu_f32(network, in, detail::scalar(inflow), out, s));
             }
             else
             {
-                detail::check(lue_amd_accu_f64(flow_direction.data(), rows, cols, detail::pointer(inflow),
-                                               detail::scalar(inflow), outflow.data(), nullptr));
+                detail::check(lue_amd_network_accu_f64(network, in, detail::scalar(inflow), out, s));
             }
+            outflow.written_on(s);
             return outflow;
         }
 
@@ -453,38 +619,42 @@ namespace lue {
                 lue::detail::verify_compatible(flow_direction, param);
                 auto outflow = lue::detail::output_like<E>(flow_direction);
                 auto remainder = lue::detail::output_like<E>(flow_direction);
-                auto const [rows, cols] = flow_direction.shape();
-                auto const* ldd = flow_direction.data();
-                auto const* p1 = lue::detail::pointer(inflow);
+                lue_amd_network const net = flow_direction.network();
+                void* const s = lue::stream();
+                lue::detail::order_before(inflow, s);
+                lue::detail::order_before(param, s);
+                E const* const p1[1] = {lue::detail::pointer(inflow)};
                 auto const s1 = lue::detail::scalar(inflow);
-                auto const* p2 = lue::detail::pointer(param);
+                E const* const p2[1] = {lue::detail::pointer(param)};
                 auto const s2 = lue::detail::scalar(param);
-                E* const out = outflow.data();
-                E* const rem = remainder.data();
+                E* const out[1] = {outflow.mutable_data()};
+                E* const rem[1] = {remainder.mutable_data()};
                 int status;
                 if constexpr (std::is_same_v<E, float>)
                 {
                     if constexpr (kind == Split::threshold)
-                        status = lue_amd_accu_threshold_f32(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                        status = lue_amd_network_accu_threshold_f32(net, p1, s1, p2, s2, out, rem, s);
                     else if constexpr (kind == Split::fraction)
-                        status = lue_amd_accu_fraction_f32(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                        status = lue_amd_network_accu_fraction_f32(net, p1, s1, p2, s2, out, rem, s);
                     else if constexpr (kind == Split::capacity)
-                        status = lue_amd_accu_capacity_f32(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                        status = lue_amd_network_accu_capacity_f32(net, p1, s1, p2, s2, out, rem, s);
                     else
-                        status = lue_amd_accu_trigger_f32(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                        status = lue_amd_network_accu_trigger_f32(net, p1, s1, p2, s2, out, rem, s);
                 }
                 else
                 {
                     if constexpr (kind == Split::threshold)
-                        status = lue_amd_accu_threshold_f64(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                        status = lue_amd_network_accu_threshold_f64(net, p1, s1, p2, s2, out, rem, s);
                     else if constexpr (kind == Split::fraction)
-                        status = lue_amd_accu_fraction_f64(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                        status = lue_amd_network_accu_fraction_f64(net, p1, s1, p2, s2, out, rem, s);
                     else if constexpr (kind == Split::capacity)
-                        status = lue_amd_accu_capacity_f64(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                        status = lue_amd_network_accu_capacity_f64(net, p1, s1, p2, s2, out, rem, s);
                     else
-                        status = lue_amd_accu_trigger_f64(ldd, rows, cols, p1, s1, p2, s2, out, rem, nullptr);
+                        status = lue_amd_network_accu_trigger_f64(net, p1, s1, p2, s2, out, rem, s);
                 }
                 lue::detail::check(status);
+                outflow.written_on(s);
+                remainder.written_on(s);
                 return {std::move(outflow), std::move(remainder)};
             }
 
@@ -620,21 +790,32 @@ namespace lue {
                 lue::detail::verify_compatible(flow_direction, beta);
                 lue::detail::verify_compatible(flow_direction, channel_length);
                 auto outflow = lue::detail::output_like<E>(flow_direction);
-                auto const [rows, cols] = flow_direction.shape();
+                lue_amd_network const net = flow_direction.network();
+                void* const s = lue::stream();
+                current_outflow.order_before(s);
+                inflow.order_before(s);
+                lue::detail::order_before(alpha, s);
+                lue::detail::order_before(beta, s);
+                lue::detail::order_before(channel_length, s);
                 using lue::detail::pointer;
                 using lue::detail::scalar;
+                E const* const cur[1] = {current_outflow.data()};
+                E const* const lat[1] = {inflow.data()};
+                E const* const a[1] = {pointer(alpha)};
+                E const* const b[1] = {pointer(beta)};
+                E const* const dx[1] = {pointer(channel_length)};
+                E* const out[1] = {outflow.mutable_data()};
                 int status;
                 if constexpr (std::is_same_v<E, float>)
-                    status = lue_amd_kinematic_wave_f32(
-                        flow_direction.data(), rows, cols, current_outflow.data(), inflow.data(), pointer(alpha),
-                        scalar(alpha), pointer(beta), scalar(beta), time_step_duration.value(),
-                        pointer(channel_length), scalar(channel_length), outflow.data(), nullptr);
+                    status = lue_amd_network_kinematic_wave_f32(net, cur, lat, a, scalar(alpha), b, scalar(beta),
+                                                                time_step_duration.value(), dx,
+                                                                scalar(channel_length), out, s);
                 else
-                    status = lue_amd_kinematic_wave_f64(
-                        flow_direction.data(), rows, cols, current_outflow.data(), inflow.data(), pointer(alpha),
-                        scalar(alpha), pointer(beta), scalar(beta), time_step_duration.value(),
-                        pointer(channel_length), scalar(channel_length), outflow.data(), nullptr);
+                    status = lue_amd_network_kinematic_wave_f64(net, cur, lat, a, scalar(alpha), b, scalar(beta),
+                                                                time_step_duration.value(), dx,
+                                                                scalar(channel_length), out, s);
                 lue::detail::check(status);
+                outflow.written_on(s);
                 return outflow;
             }
 

@@ -3,6 +3,8 @@
 // from tests/golden/*.json. Prints one line per case and exits non-zero if any case fails.
 #include "lue/framework/routing.hpp"
 
+#include <hip/hip_runtime_api.h>
+
 #include <cmath>
 #include <cstdio>
 #include <functional>
@@ -233,6 +235,52 @@ namespace {
         return false;
     }
 
+    // Asynchrony: a kinematic-wave timestep returns before its kernels finish (its output is not ready yet),
+    // and a second timestep on another stream, ordered only by the first one's completion event, gives the
+    // same bits as the two timesteps run one after the other with the host waiting in between.
+    bool asynchronous_timesteps()
+    {
+        std::int64_t const n = 8192;
+        Shape const shape{n, n};
+        Shape const ps{1024, 1024};
+        lue::PartitionedArray<std::uint8_t, 2> ldd{shape, ps};
+        lue::PartitionedArray<float, 2> q0{shape, ps};
+        lue::PartitionedArray<float, 2> lateral{shape, ps};
+        lue::detail::check(lue_amd_synthetic_ldd(20261209, n, n, ldd.mutable_data(), nullptr));
+        lue::detail::check(lue_amd_synthetic_uniform_f32(20261209, 1, 0.0, 5.0, n * n, q0.mutable_data(), nullptr));
+        lue::detail::check(
+            lue_amd_synthetic_uniform_f32(20261209, 2, 0.0, 1e-4, n * n, lateral.mutable_data(), nullptr));
+        lue::detail::check(lue_amd_stream_synchronize(nullptr));
+        lue::Scalar<float> const alpha{2.0f}, beta{0.6f}, dt{3600.0f}, dx{100.0f};
+        using lue::value_policies::kinematic_wave;
+
+        auto r1 = kinematic_wave(ldd, q0, lateral, alpha, beta, dt, dx);  // records the ldd's network
+        r1.wait();
+        auto r2 = kinematic_wave(ldd, r1, lateral, alpha, beta, dt, dx);
+        auto const want = r2.to_host();
+
+        hipStream_t a = nullptr, b = nullptr;
+        if (hipStreamCreateWithFlags(&a, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&b, hipStreamNonBlocking) != hipSuccess)
+        {
+            std::printf("  stream creation failed\n");
+            return false;
+        }
+        lue::set_stream(a);
+        auto q1 = kinematic_wave(ldd, q0, lateral, alpha, beta, dt, dx);
+        bool const pending = !q1.ready();
+        lue::set_stream(b);
+        auto q2 = kinematic_wave(ldd, q1, lateral, alpha, beta, dt, dx);
+        auto const got = q2.to_host();
+        lue::set_stream(nullptr);
+        (void)hipStreamDestroy(a);
+        (void)hipStreamDestroy(b);
+        bool const same = got == want;
+        std::printf("  asynchronous_timesteps: returned before completion %d, same bits %d\n", pending ? 1 : 0,
+                    same ? 1 : 0);
+        return pending && same;
+    }
+
 }  // namespace
 
 int main()
@@ -287,7 +335,18 @@ int main()
     }
     bool const ok = shape_mismatch_throws();
     std::printf("%s shape_mismatch_throws\n", ok ? "PASS" : "FAIL");
-    failed += ok ? 0 : 1;
-    std::printf("%d passed, %d failed\n", passed + (ok ? 1 : 0), failed);
+    (ok ? passed : failed) += 1;
+    bool async_ok = false;
+    try
+    {
+        async_ok = asynchronous_timesteps();
+    }
+    catch (std::exception const& e)
+    {
+        std::printf("  asynchronous_timesteps: exception %s\n", e.what());
+    }
+    std::printf("%s asynchronous_timesteps\n", async_ok ? "PASS" : "FAIL");
+    (async_ok ? passed : failed) += 1;
+    std::printf("%d passed, %d failed\n", passed, failed);
     return failed == 0 ? 0 : 1;
 }

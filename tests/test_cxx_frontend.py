@@ -75,8 +75,9 @@ def build_driver(tmp_path):
     exe = tmp_path / "frontend_test"
     lib_dir = os.path.join(ROOT, "lue_amd")
     cmd = ["g++", "-std=c++20", "-O1", "-Wall", "-I", str(tmp_path), "-I", os.path.join(ROOT, "include"),
-           "-I", os.path.join(ROOT, "lue_amd", "include"), os.path.join(CXX_DIR, "frontend_test.cpp"),
-           "-L", lib_dir, "-llue_amd", f"-Wl,-rpath,{lib_dir}", "-o", str(exe)]
+           "-I", os.path.join(ROOT, "lue_amd", "include"), "-I", "/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
+           os.path.join(CXX_DIR, "frontend_test.cpp"), "-L", lib_dir, "-llue_amd", f"-Wl,-rpath,{lib_dir}",
+           "-L", "/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", "-o", str(exe)]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     assert proc.returncode == 0, proc.stderr[-4000:]
     return exe

@@ -405,3 +405,54 @@ def test_scratch_blocks_reused_across_streams_and_trimmed(fw):
         assert np.array_equal(o.numpy(), want)
     _lib.trim_scratch()
     assert np.array_equal(net.kinematic_wave(*args, 0.6, 3600.0, 100.0).numpy(), want)
+
+
+def test_per_call_operations_are_asynchronous(fw):
+    """A per-call kinematic_wave returns before its kernels finish (the stream is still busy), and the next
+    operation, which reads its output, is ordered after it: the same bits as with the host waiting between."""
+    from lue_amd import synthetic as syn
+
+    n = 8192
+    dev = torch.device("cuda")
+    ldd = fw.PartitionedArray(syn.ldd_device(20261209, n, n, dev), (1024, 1024))
+    q0, lateral, alpha = [fw.PartitionedArray(syn.uniform_device(20261209, s, lo, hi, (n, n)), (1024, 1024))
+                          for s, lo, hi in ((1, 0.0, 5.0), (2, 0.0, 1e-4), (3, 1.5, 2.5))]
+    r1 = fw.kinematic_wave(ldd, q0, lateral, alpha, 0.6, 3600.0, 100.0)  # records the ldd's network
+    torch.cuda.synchronize()
+    r2 = fw.kinematic_wave(ldd, r1, lateral, alpha, 0.6, 3600.0, 100.0)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    q1 = fw.kinematic_wave(ldd, q0, lateral, alpha, 0.6, 3600.0, 100.0)
+    pending = not stream.query()
+    q2 = fw.kinematic_wave(ldd, q1, lateral, alpha, 0.6, 3600.0, 100.0)
+    torch.cuda.synchronize()
+    assert pending
+    assert torch.equal(q1.data, r1.data) and torch.equal(q2.data, r2.data)
+
+
+def test_per_call_accu_on_an_unaligned_ldd_view(fw, oracle_mod):
+    """An ldd view one byte off the 4-byte grid (rows not a multiple of 4 long) works per call, through the
+    Python front end (network of the array) and through the C ABI (fingerprint cache), as in LUE."""
+    import ctypes
+
+    from lue_amd import _lib, synthetic
+
+    rows, cols = 37, 41
+    ldd = synthetic.ldd_host(44, rows, cols)
+    material = synthetic.uniform_host(44, synthetic.STREAM_MATERIAL, 0.0, 1.0, (rows, cols))
+    flat = torch.empty(rows * cols + 1, dtype=torch.uint8, device="cuda")
+    view = flat[1:].view(rows, cols)
+    view.copy_(torch.from_numpy(ldd))
+    arr = fw.PartitionedArray(view)
+    assert arr.data.data_ptr() % 4 != 0
+    want = oracle_mod.accu(ldd, material.astype(np.float64), None, np.float64)
+    got = fw.accu(arr, pa(fw, material)).numpy()
+    assert rel_err(got, want).max() <= 1e-6
+    out = torch.empty((rows, cols), dtype=torch.float32, device="cuda")
+    mat = torch.from_numpy(material).cuda()
+    lib = _lib.load()
+    for _ in range(2):  # record, then replay from the fingerprint cache
+        _lib.check(lib.lue_amd_accu_f32(arr.data.data_ptr(), rows, cols, mat.data_ptr(), ctypes.c_float(0.0),
+                                        out.data_ptr(), torch.cuda.current_stream().cuda_stream))
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), got)
