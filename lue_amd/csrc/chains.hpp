@@ -5,8 +5,9 @@
 // has at most one pending downstream cell) whose long paths are the main streams. They are solved by
 // a heavy-path decomposition of that forest, recorded once per flow network:
 //   1. compact the pending cells (tile after tile), link each to its pending downstream cell;
-//   2. Euler tour of the forest (children in neighbour order) and list ranking by pointer jumping give
-//      every pending cell the number of pending cells upstream of it (its subtree);
+//   2. Euler tour of the forest (children in neighbour order) and list ranking (a sparse ruling set: rulers
+//      walk their sublists, the rulers are ranked by pointer jumping) give every pending cell the number of
+//      pending cells upstream of it (its subtree);
 //   3. the heavy child of a cell is its pending upstream neighbour with the largest subtree; a second
 //      Euler tour with heavy children first numbers the cells in preorder, in which every heavy path
 //      (a chain) is one contiguous run;
@@ -234,21 +235,9 @@ namespace lue_amd {
         succ[2 * q + 1] = s;
     }
 
-    // List ranking: value = number of enter elements from the element to the end of its list (inclusive),
-    // last = the list's final element.
-    __global__ inline void chains_rank_init_kernel(std::int32_t const* succ, std::uint32_t n2, std::int32_t* next,
-                                                   std::int32_t* value, std::int32_t* last)
-    {
-        std::uint32_t const e = blockIdx.x * blockDim.x + threadIdx.x;
-        if (e >= n2)
-            return;
-        std::int32_t const s = succ[e];
-        next[e] = s;
-        value[e] = (e & 1u) == 0 ? 1 : 0;
-        last[e] = s < 0 ? static_cast<std::int32_t>(e) : s;
-    }
-
-    // One pointer-jumping step (double buffered); counts the elements still linked.
+    // One pointer-jumping step (double buffered) over linked elements with values, next (-1 at the end) and
+    // the list's last element (known at the end, carried back): value becomes the sum from the element to the
+    // list's end, last the list's last element. Counts the elements still linked.
     __global__ inline void chains_rank_step_kernel(std::int32_t const* next, std::int32_t const* value,
                                                    std::int32_t const* last, std::uint32_t n2, std::int32_t* next2,
                                                    std::int32_t* value2, std::int32_t* last2, std::uint32_t* linked)
@@ -276,6 +265,119 @@ namespace lue_amd {
         std::uint64_t const ballot = __ballot(still);
         if ((threadIdx.x & 63) == 0 && ballot)
             atomicAdd(linked, static_cast<std::uint32_t>(__popcll(ballot)));
+    }
+
+    // List ranking by a sparse ruling set (work O(n) instead of pointer jumping's O(n log n)). Rulers are the
+    // list heads (elements no other element links to) and a pseudo-random 1/64 of the others; each ruler
+    // walks its sublist up to the next ruler, recording for every element its ruler and the inclusive prefix
+    // of the values from the ruler; the rulers, linked to their next rulers with their sublists' sums, are
+    // ranked by pointer jumping (few: about n / 64 plus the heads); every element's rank follows from its
+    // ruler's. value(e) = 1 for enter elements (even e), 0 for exit elements.
+    constexpr std::uint32_t ruler_sample = 64;
+    constexpr std::uint32_t ruler_walk_limit = 1u << 20;  // a longer walk can only circle on a cycle
+
+    __device__ inline bool sampled_ruler(std::uint32_t e)
+    {
+        std::uint64_t z = static_cast<std::uint64_t>(e) + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return ((z ^ (z >> 31)) & (ruler_sample - 1)) == 0;
+    }
+
+    // has_pred[succ[e]] = 1 (has_pred zeroed before).
+    __global__ inline void rank_mark_successors_kernel(std::int32_t const* succ, std::uint32_t n2,
+                                                       std::uint32_t* has_pred)
+    {
+        std::uint32_t const e = blockIdx.x * blockDim.x + threadIdx.x;
+        if (e >= n2)
+            return;
+        std::int32_t const s = succ[e];
+        if (s >= 0 && static_cast<std::uint32_t>(s) < n2)
+            has_pred[s] = 1u;
+    }
+
+    // is_ruler[e] (0/1) for the scan that numbers the rulers; owner[e] = -1 (no ruler reached it yet).
+    __global__ inline void rank_rulers_kernel(std::uint32_t const* has_pred, std::uint32_t n2, std::uint32_t* is_ruler,
+                                              std::int32_t* owner)
+    {
+        std::uint32_t const e = blockIdx.x * blockDim.x + threadIdx.x;
+        if (e >= n2)
+            return;
+        is_ruler[e] = (has_pred[e] == 0u || sampled_ruler(e)) ? 1u : 0u;
+        owner[e] = -1;
+    }
+
+    // Walk of ruler r (element ruler_elem[r]) to the next ruler or the list's end. `bad` counts walks that
+    // reached the limit (cycles).
+    __global__ inline void rank_walk_kernel(std::int32_t const* succ, std::uint32_t const* is_ruler,
+                                            std::uint32_t const* ruler_rank, std::uint32_t const* ruler_elem,
+                                            std::uint32_t nrulers, std::int32_t* owner, std::int32_t* prefix,
+                                            std::int32_t* sub_sum, std::int32_t* next_ruler, std::int32_t* last_elem,
+                                            std::uint32_t* bad)
+    {
+        std::uint32_t const r = blockIdx.x * blockDim.x + threadIdx.x;
+        if (r >= nrulers)
+            return;
+        std::uint32_t x = ruler_elem[r];
+        std::int32_t acc = 0;
+        std::int32_t next = -1;
+        std::int32_t last = -1;
+        for (std::uint32_t step = 0;; ++step)
+        {
+            if (step == ruler_walk_limit)
+            {
+                atomicAdd(bad, 1u);
+                break;
+            }
+            acc += (x & 1u) == 0u ? 1 : 0;
+            owner[x] = static_cast<std::int32_t>(r);
+            prefix[x] = acc;
+            std::int32_t const s = succ[x];
+            if (s < 0)
+            {
+                last = static_cast<std::int32_t>(x);
+                break;
+            }
+            if (is_ruler[s])
+            {
+                next = static_cast<std::int32_t>(ruler_rank[s]);
+                break;
+            }
+            x = static_cast<std::uint32_t>(s);
+        }
+        sub_sum[r] = acc;
+        next_ruler[r] = next;
+        last_elem[r] = last;
+    }
+
+    // ruler_elem[ruler_rank[e]] = e for the rulers.
+    __global__ inline void rank_ruler_elements_kernel(std::uint32_t const* is_ruler, std::uint32_t const* ruler_rank,
+                                                      std::uint32_t n2, std::uint32_t* ruler_elem)
+    {
+        std::uint32_t const e = blockIdx.x * blockDim.x + threadIdx.x;
+        if (e < n2 && is_ruler[e])
+            ruler_elem[ruler_rank[e]] = e;
+    }
+
+    // Element ranks from their rulers' (rv: sum from the ruler to the list's end, rlast: the list's last
+    // element); elements no ruler reached lie on a cycle (`bad`).
+    __global__ inline void rank_elements_kernel(std::int32_t const* owner, std::int32_t const* prefix,
+                                                std::int32_t const* rv, std::int32_t const* rlast, std::uint32_t n2,
+                                                std::int32_t* value, std::int32_t* last, std::uint32_t* bad)
+    {
+        std::uint32_t const e = blockIdx.x * blockDim.x + threadIdx.x;
+        if (e >= n2)
+            return;
+        std::int32_t const r = owner[e];
+        if (r < 0)
+        {
+            atomicAdd(bad, 1u);
+            value[e] = 0;
+            last[e] = static_cast<std::int32_t>(e);
+            return;
+        }
+        value[e] = rv[r] - prefix[e] + ((e & 1u) == 0u ? 1 : 0);
+        last[e] = rlast[r];
     }
 
     // Counts entries of `perm` that are out of range or repeated (perm must be a permutation of [0, n)).
@@ -339,7 +441,8 @@ namespace lue_amd {
     __global__ inline void chains_preorder_kernel(std::int32_t const* parent, std::int32_t const* heavy,
                                                   std::int32_t const* value, std::int32_t const* last,
                                                   std::uint32_t const* root_index, std::uint32_t const* tree_base,
-                                                  std::uint32_t n, std::uint32_t* pre, std::uint32_t* head_flag)
+                                                  std::uint32_t n, std::uint32_t* pre, std::uint32_t* head_flag,
+                                                  std::uint32_t* at_pre)
     {
         std::uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
         if (q >= n)
@@ -351,7 +454,10 @@ namespace lue_amd {
         pre[q] = p;
         std::int32_t const d = parent[q];
         if (p < n)
+        {
             head_flag[p] = (d < 0 || heavy[d] != static_cast<std::int32_t>(q)) ? 1u : 0u;
+            at_pre[p] = q;
+        }
     }
 
     // Chains are numbered by preorder of their heads: chain_pre[c] = preorder of head of chain c.
@@ -371,20 +477,20 @@ namespace lue_amd {
             length[c] = (c + 1 < nchains ? chain_pre[c + 1] : n) - chain_pre[c];
     }
 
-    // One relaxation of the levels: level(chain of parent(h)) >= level(chain of h) + 1 for every head h
-    // with a parent.
-    __global__ inline void chains_level_kernel(std::int32_t const* parent, std::uint32_t const* pre,
-                                               std::uint32_t const* head_flag, std::uint32_t const* chain_incl,
-                                               std::uint32_t n, std::uint32_t* level, std::uint32_t* changed)
+    // One relaxation of the levels over the chains: level(chain of parent(head)) >= level(chain) + 1 for every
+    // chain whose head has a parent.
+    __global__ inline void chains_level_by_chain_kernel(std::int32_t const* parent, std::uint32_t const* pre,
+                                                        std::uint32_t const* at_pre, std::uint32_t const* chain_pre,
+                                                        std::uint32_t const* chain_incl, std::uint32_t nchains,
+                                                        std::uint32_t* level, std::uint32_t* changed)
     {
-        std::uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
-        if (q >= n)
+        std::uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
+        if (c >= nchains)
             return;
-        std::uint32_t const p = pre[q];
+        std::uint32_t const q = at_pre[chain_pre[c]];
         std::int32_t const d = parent[q];
-        if (!head_flag[p] || d < 0)
+        if (d < 0)
             return;
-        std::uint32_t const c = chain_incl[p] - 1;
         std::uint32_t const cd = chain_incl[pre[d]] - 1;
         std::uint32_t const want = level[c] + 1;
         if (level[cd] < want)
@@ -394,22 +500,101 @@ namespace lue_amd {
         }
     }
 
-    // Lengths of the chains of one level (others 0), for the per-level exclusive scan.
-    __global__ inline void chains_level_mask_kernel(std::uint32_t const* level, std::uint32_t const* length,
-                                                    std::uint32_t nchains, std::uint32_t which, std::uint32_t* out)
+    // Chain layout (device). Chains sorted by level (stable, so in preorder of their heads inside a level) lie
+    // one after the other: their positions are the exclusive scan of their lengths in that order. `order` is
+    // the sorted chain indices, `level_sorted` their levels.
+    constexpr int chains_max_levels = 64;
+
+    __global__ inline void chains_sorted_lengths_kernel(std::uint32_t const* order, std::uint32_t const* length,
+                                                        std::uint32_t const* level_sorted, std::uint32_t nchains,
+                                                        std::uint32_t* len_sorted, std::uint32_t* level_cells,
+                                                        std::uint32_t* longest)
     {
-        std::uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
-        if (c < nchains)
-            out[c] = level[c] == which ? length[c] : 0u;
+        std::uint32_t const k = blockIdx.x * blockDim.x + threadIdx.x;
+        if (k >= nchains)
+            return;
+        std::uint32_t const len = length[order[k]];
+        len_sorted[k] = len;
+        std::uint32_t const l = level_sorted[k];
+        if (l < static_cast<std::uint32_t>(chains_max_levels))
+            atomicAdd(&level_cells[l], len);
+        atomicMax(longest, len);
     }
 
-    __global__ inline void chains_level_base_kernel(std::uint32_t const* level, std::uint32_t const* scan,
-                                                    std::uint32_t nchains, std::uint32_t which, std::uint32_t offset,
-                                                    std::uint32_t* base)
+    // Base position of every chain, and the work-item starts: a chain longer than 64 cells is an item of its
+    // own; shorter ones go together while they are consecutive, of one level and start in one 64-position
+    // window (an item then spans at most two 64-position blocks).
+    __global__ inline void chains_items_kernel(std::uint32_t const* order, std::uint32_t const* level_sorted,
+                                               std::uint32_t const* len_sorted, std::uint32_t const* base_sorted,
+                                               std::uint32_t nchains, std::uint32_t* base, std::uint32_t* item_flag)
     {
-        std::uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
-        if (c < nchains && level[c] == which)
-            base[c] = offset + scan[c];
+        std::uint32_t const k = blockIdx.x * blockDim.x + threadIdx.x;
+        if (k >= nchains)
+            return;
+        std::uint32_t const b = base_sorted[k];
+        base[order[k]] = b;
+        bool start = k == 0;
+        if (!start)
+        {
+            bool const long_here = len_sorted[k] > 64;
+            bool const long_before = len_sorted[k - 1] > 64;
+            start = long_here || long_before || level_sorted[k] != level_sorted[k - 1] ||
+                    (b >> 6) != (base_sorted[k - 1] >> 6);
+        }
+        item_flag[k] = start ? 1u : 0u;
+    }
+
+    // Items (unsorted): start, end, level; `key` orders them by level, then the longest first.
+    __global__ inline void chains_item_fill_kernel(std::uint32_t const* level_sorted, std::uint32_t const* len_sorted,
+                                                   std::uint32_t const* base_sorted, std::uint32_t const* item_flag,
+                                                   std::uint32_t const* item_incl, std::uint32_t nchains,
+                                                   std::uint32_t* item_begin, std::uint32_t* item_end,
+                                                   std::uint32_t* item_level, std::uint32_t* level_items)
+    {
+        std::uint32_t const k = blockIdx.x * blockDim.x + threadIdx.x;
+        if (k >= nchains)
+            return;
+        std::uint32_t const id = item_incl[k] - 1;
+        if (item_flag[k])
+        {
+            item_begin[id] = base_sorted[k];
+            item_level[id] = level_sorted[k];
+            if (level_sorted[k] < static_cast<std::uint32_t>(chains_max_levels))
+                atomicAdd(&level_items[level_sorted[k]], 1u);
+        }
+        if (k + 1 == nchains || item_flag[k + 1])
+            item_end[id] = base_sorted[k] + len_sorted[k];
+    }
+
+    __global__ inline void chains_item_key_kernel(std::uint32_t const* item_begin, std::uint32_t const* item_end,
+                                                  std::uint32_t const* item_level, std::uint32_t nitems,
+                                                  std::uint64_t* key, std::uint32_t* index)
+    {
+        std::uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= nitems)
+            return;
+        std::uint32_t const count = item_end[i] - item_begin[i];
+        key[i] = (static_cast<std::uint64_t>(item_level[i]) << 32) | (0xFFFFFFFFu - count);
+        index[i] = i;
+    }
+
+    __global__ inline void chains_item_gather_kernel(std::uint32_t const* sorted_index, std::uint32_t const* item_begin,
+                                                     std::uint32_t const* item_end, std::uint32_t nitems,
+                                                     std::uint32_t* item_start, std::uint32_t* item_count)
+    {
+        std::uint32_t const j = blockIdx.x * blockDim.x + threadIdx.x;
+        if (j >= nitems)
+            return;
+        std::uint32_t const i = sorted_index[j];
+        item_start[j] = item_begin[i];
+        item_count[j] = item_end[i] - item_begin[i];
+    }
+
+    __global__ inline void iota_u32_kernel(std::uint32_t* p, std::uint32_t n)
+    {
+        std::uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < n)
+            p[i] = i;
     }
 
     // Position of every pending cell: chains in flow order (the most upstream cell first).

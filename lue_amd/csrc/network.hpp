@@ -507,6 +507,26 @@ namespace lue_amd {
         return e != hipSuccess ? e : f;
     }
 
+    // Stable radix sort of (key, value) pairs on the low end_bit bits of the keys.
+    template<typename Key, typename Value>
+    hipError_t sort_pairs(Key const* keys_in, Key* keys_out, Value const* values_in, Value* values_out, std::size_t n,
+                          int end_bit, hipStream_t stream)
+    {
+        std::size_t bytes = 0;
+        hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys_in, keys_out, values_in, values_out,
+                                                          static_cast<int>(n), 0, end_bit, stream);
+        if (e != hipSuccess)
+            return e;
+        void* temp = nullptr;
+        e = hipMallocAsync(&temp, std::max<std::size_t>(bytes, 1), stream);
+        if (e != hipSuccess)
+            return e;
+        e = hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys_in, keys_out, values_in, values_out,
+                                               static_cast<int>(n), 0, end_bit, stream);
+        hipError_t const f = hipFreeAsync(temp, stream);
+        return e != hipSuccess ? e : f;
+    }
+
     inline unsigned blocks_for(std::uint64_t n, unsigned threads = 256)
     {
         return static_cast<unsigned>(std::max<std::uint64_t>((n + threads - 1) / threads, 1));
@@ -847,38 +867,81 @@ namespace lue_amd {
         return e != hipSuccess ? e : hipStreamSynchronize(stream);
     }
 
-    // List ranking of the tour elements (pointer jumping, double buffered). `cur` names the buffers that
-    // hold the result. Lists that do not end (cycles in the flow network) leave `cyclic` set.
-    inline int rank_lists(std::int32_t const* succ, std::uint32_t n2, DeviceArray<std::int32_t> (&next)[2],
-                          DeviceArray<std::int32_t> (&value)[2], DeviceArray<std::int32_t> (&last)[2], int& cur,
-                          bool& cyclic, hipStream_t stream)
+    // List ranking of the tour elements by a sparse ruling set (chains.hpp: rank_* kernels): value[e] = number
+    // of enter elements from e to the end of its list (inclusive), last[e] = the list's final element. Lists
+    // that do not end (cycles in the flow network) leave `cyclic` set. The rulers (about n2 / 64 plus the list
+    // heads) are ranked by pointer jumping with a fixed number of steps and one readback at the end.
+    inline int rank_lists(std::int32_t const* succ, std::uint32_t n2, DeviceArray<std::int32_t>& value,
+                          DeviceArray<std::int32_t>& last, bool& cyclic, hipStream_t stream)
     {
+        DeviceArray<std::uint32_t> has_pred, is_ruler, ruler_rank;
+        DeviceArray<std::int32_t> owner, prefix;
+        DeviceArray<std::uint32_t> bad;
+        LUE_AMD_HIP(has_pred.allocate(n2, stream));
+        LUE_AMD_HIP(is_ruler.allocate(n2 + 1, stream));
+        LUE_AMD_HIP(ruler_rank.allocate(n2 + 1, stream));
+        LUE_AMD_HIP(owner.allocate(n2, stream));
+        LUE_AMD_HIP(prefix.allocate(n2, stream));
+        LUE_AMD_HIP(bad.allocate(1, stream));
+        LUE_AMD_HIP(value.allocate(n2, stream));
+        LUE_AMD_HIP(last.allocate(n2, stream));
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, has_pred.ptr, 0u,
+                           static_cast<std::uint64_t>(n2));
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, bad.ptr, 0u, 1ull);
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, is_ruler.ptr + n2, 0u, 1ull);
+        hipLaunchKernelGGL(rank_mark_successors_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, succ, n2,
+                           has_pred.ptr);
+        hipLaunchKernelGGL(rank_rulers_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, has_pred.ptr, n2,
+                           is_ruler.ptr, owner.ptr);
+        LUE_AMD_HIP(hipGetLastError());
+        LUE_AMD_HIP(exclusive_scan(is_ruler.ptr, ruler_rank.ptr, static_cast<std::size_t>(n2) + 1, stream));
+        std::uint32_t nrulers = 0;
+        LUE_AMD_HIP(copy_to_host(&nrulers, ruler_rank.ptr + n2, 1, stream));
+        if (nrulers == 0)
+        {
+            cyclic = true;  // every list circles
+            return status_success;
+        }
+        DeviceArray<std::uint32_t> ruler_elem;
+        DeviceArray<std::int32_t> rnext[2], rvalue[2], rlast[2];  // [0] starts as the walks' results
+        LUE_AMD_HIP(ruler_elem.allocate(nrulers, stream));
         for (int b = 0; b < 2; ++b)
         {
-            LUE_AMD_HIP(next[b].allocate(n2, stream));
-            LUE_AMD_HIP(value[b].allocate(n2, stream));
-            LUE_AMD_HIP(last[b].allocate(n2, stream));
+            LUE_AMD_HIP(rnext[b].allocate(nrulers, stream));
+            LUE_AMD_HIP(rvalue[b].allocate(nrulers, stream));
+            LUE_AMD_HIP(rlast[b].allocate(nrulers, stream));
         }
+        hipLaunchKernelGGL(rank_ruler_elements_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, is_ruler.ptr,
+                           ruler_rank.ptr, n2, ruler_elem.ptr);
+        hipLaunchKernelGGL(rank_walk_kernel, dim3(blocks_for(nrulers)), dim3(256), 0, stream, succ, is_ruler.ptr,
+                           ruler_rank.ptr, ruler_elem.ptr, nrulers, owner.ptr, prefix.ptr, rvalue[0].ptr, rnext[0].ptr,
+                           rlast[0].ptr, bad.ptr);
+        LUE_AMD_HIP(hipGetLastError());
+        // pointer jumping over the rulers: value = sum of the sublists from the ruler to the list's end,
+        // last = the list's last element (known at the list's final ruler, carried back)
+        int cur = 0;
+        int steps = 2;
+        for (std::uint64_t span = 1; span < nrulers; span <<= 1)
+            ++steps;
         DeviceArray<std::uint32_t> linked;
         LUE_AMD_HIP(linked.allocate(1, stream));
-        hipLaunchKernelGGL(chains_rank_init_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, succ, n2, next[0].ptr,
-                           value[0].ptr, last[0].ptr);
-        cur = 0;
-        int steps = 2;
-        for (std::uint64_t span = 1; span < n2; span <<= 1)
-            ++steps;
-        std::uint32_t still = 1;
-        for (int step = 0; step < steps && still > 0; ++step)
+        for (int step = 0; step < steps; ++step)
         {
             hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, linked.ptr, 0u, 1ull);
-            hipLaunchKernelGGL(chains_rank_step_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, next[cur].ptr,
-                               value[cur].ptr, last[cur].ptr, n2, next[1 - cur].ptr, value[1 - cur].ptr,
-                               last[1 - cur].ptr, linked.ptr);
-            LUE_AMD_HIP(hipGetLastError());
+            hipLaunchKernelGGL(chains_rank_step_kernel, dim3(blocks_for(nrulers)), dim3(256), 0, stream,
+                               rnext[cur].ptr, rvalue[cur].ptr, rlast[cur].ptr, nrulers, rnext[1 - cur].ptr,
+                               rvalue[1 - cur].ptr, rlast[1 - cur].ptr, linked.ptr);
             cur = 1 - cur;
-            LUE_AMD_HIP(copy_to_host(&still, linked.ptr, 1, stream));
         }
-        cyclic = still > 0;
+        LUE_AMD_HIP(hipGetLastError());
+        hipLaunchKernelGGL(rank_elements_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, owner.ptr, prefix.ptr,
+                           rvalue[cur].ptr, rlast[cur].ptr, n2, value.ptr, last.ptr, bad.ptr);
+        LUE_AMD_HIP(hipGetLastError());
+        std::uint32_t still = 0;
+        std::uint32_t nbad = 0;
+        LUE_AMD_HIP(copy_to_host(&still, linked.ptr, 1, stream));
+        LUE_AMD_HIP(copy_to_host(&nbad, bad.ptr, 1, stream));
+        cyclic = still > 0 || nbad > 0;
         return status_success;
     }
 
@@ -942,29 +1005,28 @@ namespace lue_amd {
         LUE_AMD_HIP(hipGetLastError());
 
         // subtree sizes (Euler tour, children in neighbour order), heavy children
-        DeviceArray<std::int32_t> next[2], value[2], last[2];
-        int cur = 0;
+        DeviceArray<std::int32_t> value, last;
         bool cyclic = false;
         hipLaunchKernelGGL((chains_tour_kernel<TW, TH>), dim3(blocks_for(n)), dim3(256), 0, stream, net.cats.ptr, ncats,
                            pending_bits, cells.ptr, index_of.ptr, parent.ptr, nullptr, n, succ.ptr);
         LUE_AMD_HIP(hipGetLastError());
-        if (int const st = rank_lists(succ.ptr, 2 * n, next, value, last, cur, cyclic, stream))
+        if (int const st = rank_lists(succ.ptr, 2 * n, value, last, cyclic, stream))
             return st;
         if (cyclic)
             return fail(status_invalid_network, "invalid flow direction network: cells on or downstream of a cycle");
         hipLaunchKernelGGL((chains_heavy_kernel<TW, TH>), dim3(blocks_for(n)), dim3(256), 0, stream, net.cats.ptr,
-                           ncats, pending_bits, cells.ptr, index_of.ptr, value[cur].ptr, n, heavy.ptr);
+                           ncats, pending_bits, cells.ptr, index_of.ptr, value.ptr, n, heavy.ptr);
         LUE_AMD_HIP(hipGetLastError());
 
         // preorder with heavy children first: every chain is a contiguous run
         hipLaunchKernelGGL((chains_tour_kernel<TW, TH>), dim3(blocks_for(n)), dim3(256), 0, stream, net.cats.ptr, ncats,
                            pending_bits, cells.ptr, index_of.ptr, parent.ptr, heavy.ptr, n, succ.ptr);
         LUE_AMD_HIP(hipGetLastError());
-        if (int const st = rank_lists(succ.ptr, 2 * n, next, value, last, cur, cyclic, stream))
+        if (int const st = rank_lists(succ.ptr, 2 * n, value, last, cyclic, stream))
             return st;
         if (cyclic)
             return fail(status_invalid_network, "invalid flow direction network: cells on or downstream of a cycle");
-        DeviceArray<std::uint32_t> root_flag, root_index, tree_size, tree_base, pre, head_flag, chain_incl;
+        DeviceArray<std::uint32_t> root_flag, root_index, tree_size, tree_base, pre, head_flag, chain_incl, at_pre;
         LUE_AMD_HIP(root_flag.allocate(n + 1, stream));
         LUE_AMD_HIP(root_index.allocate(n + 1, stream));
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, root_flag.ptr + n, 0u, 1ull);
@@ -977,13 +1039,14 @@ namespace lue_amd {
         LUE_AMD_HIP(tree_base.allocate(nroots + 1, stream));
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, tree_size.ptr + nroots, 0u, 1ull);
         hipLaunchKernelGGL(chains_tree_size_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, parent.ptr,
-                           value[cur].ptr, root_index.ptr, n, tree_size.ptr);
+                           value.ptr, root_index.ptr, n, tree_size.ptr);
         LUE_AMD_HIP(exclusive_scan(tree_size.ptr, tree_base.ptr, nroots + 1, stream));
         LUE_AMD_HIP(pre.allocate(n, stream));
         LUE_AMD_HIP(head_flag.allocate(n, stream));
         LUE_AMD_HIP(chain_incl.allocate(n, stream));
+        LUE_AMD_HIP(at_pre.allocate(n, stream));
         hipLaunchKernelGGL(chains_preorder_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, parent.ptr, heavy.ptr,
-                           value[cur].ptr, last[cur].ptr, root_index.ptr, tree_base.ptr, n, pre.ptr, head_flag.ptr);
+                           value.ptr, last.ptr, root_index.ptr, tree_base.ptr, n, pre.ptr, head_flag.ptr, at_pre.ptr);
         LUE_AMD_HIP(hipGetLastError());
         if (int const st = check_permutation(pre.ptr, n, "preorder", stream))
             return st;
@@ -1005,86 +1068,91 @@ namespace lue_amd {
                            n, length.ptr);
         hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, level.ptr, 0u,
                            static_cast<std::uint64_t>(nchains));
-        for (int iteration = 0; iteration < 64; ++iteration)
+        for (int iteration = 0; iteration < chains_max_levels; ++iteration)
         {
             std::uint32_t count = 0;
             hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, changed.ptr, 0u, 1ull);
-            hipLaunchKernelGGL(chains_level_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, parent.ptr, pre.ptr,
-                               head_flag.ptr, chain_incl.ptr, n, level.ptr, changed.ptr);
+            hipLaunchKernelGGL(chains_level_by_chain_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, parent.ptr,
+                               pre.ptr, at_pre.ptr, chain_pre.ptr, chain_incl.ptr, nchains, level.ptr, changed.ptr);
             LUE_AMD_HIP(hipGetLastError());
             LUE_AMD_HIP(copy_to_host(&count, changed.ptr, 1, stream));
             if (count == 0)
                 break;
-            if (iteration == 63)
+            if (iteration == chains_max_levels - 1)
                 return fail(status_invalid_network, "chain levels did not converge");
         }
 
-        // layout (host): chains level after level, in preorder of their heads inside a level; work items are
-        // one long chain, or consecutive short chains of at most 64 cells together
-        std::vector<std::uint32_t> h_level(nchains), h_length(nchains), h_base(nchains);
-        LUE_AMD_HIP(copy_to_host(h_level.data(), level.ptr, nchains, stream));
-        LUE_AMD_HIP(copy_to_host(h_length.data(), length.ptr, nchains, stream));
-        std::uint32_t nlevels = 0;
-        for (std::uint32_t c = 0; c < nchains; ++c)
-            nlevels = std::max(nlevels, h_level[c] + 1);
-        std::vector<std::uint64_t> level_cells(nlevels + 1, 0);
-        for (std::uint32_t c = 0; c < nchains; ++c)
-            level_cells[h_level[c] + 1] += h_length[c];
-        for (std::uint32_t l = 0; l < nlevels; ++l)
-            level_cells[l + 1] += level_cells[l];
-        std::vector<std::uint64_t> at(level_cells.begin(), level_cells.end() - 1);
-        std::vector<std::vector<std::pair<std::uint32_t, std::uint32_t>>> items(nlevels);
-        std::vector<std::pair<std::uint32_t, std::uint32_t>> open(nlevels, {0u, 0u});
-        std::uint32_t longest = 0;
-        for (std::uint32_t c = 0; c < nchains; ++c)
+        // layout (device): chains level after level, in preorder of their heads inside a level; work items are
+        // one long chain, or consecutive short chains of one level starting in one 64-position window; inside
+        // a level the longest items come first
         {
-            std::uint32_t const l = h_level[c];
-            std::uint32_t const len = h_length[c];
-            std::uint32_t const b = static_cast<std::uint32_t>(at[l]);
-            h_base[c] = b;
-            at[l] += len;
-            longest = std::max(longest, len);
-            auto& cur_item = open[l];
-            if (len > 64)
+            DeviceArray<std::uint32_t> order_in, order, level_sorted, len_sorted, base_sorted, stats, item_flag,
+                item_incl;
+            LUE_AMD_HIP(order_in.allocate(nchains, stream));
+            LUE_AMD_HIP(order.allocate(nchains, stream));
+            LUE_AMD_HIP(level_sorted.allocate(nchains, stream));
+            LUE_AMD_HIP(len_sorted.allocate(nchains, stream));
+            LUE_AMD_HIP(base_sorted.allocate(nchains, stream));
+            LUE_AMD_HIP(item_flag.allocate(nchains, stream));
+            LUE_AMD_HIP(item_incl.allocate(nchains, stream));
+            // stats: [0, 64) cells per level, [64, 128) items per level, [128] longest chain
+            constexpr int nstats = 2 * chains_max_levels + 1;
+            LUE_AMD_HIP(stats.allocate(nstats, stream));
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, stream, stats.ptr, 0u,
+                               static_cast<std::uint64_t>(nstats));
+            hipLaunchKernelGGL(iota_u32_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, order_in.ptr, nchains);
+            LUE_AMD_HIP(sort_pairs(level.ptr, level_sorted.ptr, order_in.ptr, order.ptr, nchains, 7, stream));
+            hipLaunchKernelGGL(chains_sorted_lengths_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, order.ptr,
+                               length.ptr, level_sorted.ptr, nchains, len_sorted.ptr, stats.ptr,
+                               stats.ptr + 2 * chains_max_levels);
+            LUE_AMD_HIP(exclusive_scan(len_sorted.ptr, base_sorted.ptr, nchains, stream));
+            hipLaunchKernelGGL(chains_items_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, order.ptr,
+                               level_sorted.ptr, len_sorted.ptr, base_sorted.ptr, nchains, base.ptr, item_flag.ptr);
+            LUE_AMD_HIP(inclusive_scan(item_flag.ptr, item_incl.ptr, nchains, stream));
+            std::uint32_t nitems = 0;
+            LUE_AMD_HIP(copy_to_host(&nitems, item_incl.ptr + nchains - 1, 1, stream));
+            DeviceArray<std::uint32_t> item_begin, item_end, item_level, index_in, index;
+            DeviceArray<std::uint64_t> key_in, key;
+            LUE_AMD_HIP(item_begin.allocate(nitems, stream));
+            LUE_AMD_HIP(item_end.allocate(nitems, stream));
+            LUE_AMD_HIP(item_level.allocate(nitems, stream));
+            LUE_AMD_HIP(index_in.allocate(nitems, stream));
+            LUE_AMD_HIP(index.allocate(nitems, stream));
+            LUE_AMD_HIP(key_in.allocate(nitems, stream));
+            LUE_AMD_HIP(key.allocate(nitems, stream));
+            hipLaunchKernelGGL(chains_item_fill_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream,
+                               level_sorted.ptr, len_sorted.ptr, base_sorted.ptr, item_flag.ptr, item_incl.ptr, nchains,
+                               item_begin.ptr, item_end.ptr, item_level.ptr, stats.ptr + chains_max_levels);
+            hipLaunchKernelGGL(chains_item_key_kernel, dim3(blocks_for(nitems)), dim3(256), 0, stream, item_begin.ptr,
+                               item_end.ptr, item_level.ptr, nitems, key_in.ptr, index_in.ptr);
+            LUE_AMD_HIP(sort_pairs(key_in.ptr, key.ptr, index_in.ptr, index.ptr, nitems, 32 + 7, stream));
+            LUE_AMD_HIP(net.item_start.allocate(nitems, stream));
+            LUE_AMD_HIP(net.item_count.allocate(nitems, stream));
+            hipLaunchKernelGGL(chains_item_gather_kernel, dim3(blocks_for(nitems)), dim3(256), 0, stream, index.ptr,
+                               item_begin.ptr, item_end.ptr, nitems, net.item_start.ptr, net.item_count.ptr);
+            LUE_AMD_HIP(hipGetLastError());
+            std::uint32_t h_stats[nstats];
+            LUE_AMD_HIP(copy_to_host(h_stats, stats.ptr, nstats, stream));
+            std::uint32_t nlevels = 0;
+            std::uint64_t cells = 0;
+            for (int l = 0; l < chains_max_levels; ++l)
             {
-                items[l].emplace_back(b, len);
-                continue;
+                if (h_stats[l] > 0)
+                    nlevels = static_cast<std::uint32_t>(l) + 1;
+                cells += h_stats[l];
             }
-            if (cur_item.second > 0 && (cur_item.first + cur_item.second != b || cur_item.second + len > 64))
+            if (cells != n)
+                return fail(status_invalid_network, "chain layout: cells not accounted for");
+            net.level_pos.assign(1, 0);
+            net.level_items.assign(1, 0);
+            for (std::uint32_t l = 0; l < nlevels; ++l)
             {
-                items[l].push_back(cur_item);
-                cur_item = {0u, 0u};
+                net.level_pos.push_back(net.level_pos.back() + h_stats[l]);
+                net.level_items.push_back(net.level_items.back() + h_stats[chains_max_levels + l]);
             }
-            if (cur_item.second == 0)
-                cur_item.first = b;
-            cur_item.second += len;
+            net.longest_chain = h_stats[2 * chains_max_levels];
+            LUE_AMD_HIP(hipStreamSynchronize(stream));  // before the temporaries go
         }
-        std::vector<std::uint32_t> h_item_start, h_item_count;
-        net.level_items.assign(1, 0);
-        net.level_pos.assign(level_cells.begin(), level_cells.end());
-        for (std::uint32_t l = 0; l < nlevels; ++l)
-        {
-            if (open[l].second > 0)
-                items[l].push_back(open[l]);
-            // long chains first
-            std::stable_sort(items[l].begin(), items[l].end(),
-                             [](auto const& x, auto const& y) { return x.second > y.second; });
-            for (auto const& it : items[l])
-            {
-                h_item_start.push_back(it.first);
-                h_item_count.push_back(it.second);
-            }
-            net.level_items.push_back(static_cast<std::uint32_t>(h_item_start.size()));
-        }
-        net.longest_chain = longest;
-        LUE_AMD_HIP(hipMemcpyAsync(base.ptr, h_base.data(), sizeof(std::uint32_t) * nchains, hipMemcpyHostToDevice,
-                                   stream));
-        LUE_AMD_HIP(net.item_start.allocate(h_item_start.size(), stream));
-        LUE_AMD_HIP(net.item_count.allocate(h_item_count.size(), stream));
-        LUE_AMD_HIP(hipMemcpyAsync(net.item_start.ptr, h_item_start.data(), sizeof(std::uint32_t) * h_item_start.size(),
-                                   hipMemcpyHostToDevice, stream));
-        LUE_AMD_HIP(hipMemcpyAsync(net.item_count.ptr, h_item_count.data(), sizeof(std::uint32_t) * h_item_count.size(),
-                                   hipMemcpyHostToDevice, stream));
 
         // positions, chain starts, side references
         LUE_AMD_HIP(net.pos_of.allocate(n, stream));

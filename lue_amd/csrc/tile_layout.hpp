@@ -38,6 +38,9 @@ namespace lue_amd {
         std::uint8_t heavy_dir[G::n];
         std::uint32_t count[G::n];       // children still to be sized
         std::uint32_t size[G::n];        // local cells upstream, including the cell
+        alignas(4) std::uint8_t children[G::n];  // bit k: neighbour k is a local child (set by word-wide ORs)
+        std::uint16_t heads[G::n];       // chain heads in cell order
+        std::uint32_t wave_heads[G::n / 64];  // chain heads per 64 cells (prefix: offsets into heads)
         std::uint16_t queue[G::n];
         std::int16_t head_of[G::n];
         std::uint16_t depth[G::n];       // distance from the chain head (its most downstream cell)
@@ -50,6 +53,7 @@ namespace lue_amd {
         unsigned int taken;
         unsigned int changed;
         unsigned int levels;
+        unsigned int nheads;
     };
 
     // One workgroup per tile. `pending_bits`: the cells round 0 leaves pending. On a tile whose levels
@@ -100,6 +104,8 @@ namespace lue_amd {
             s.heavy[i] = -1;
             s.head_of[i] = -1;
         }
+        for (int w = tid; w < G::n / 4; w += nthreads)
+            reinterpret_cast<std::uint32_t*>(s.children)[w] = 0u;
         __syncthreads();
         for (int i = tid; i < G::n; i += nthreads)
         {
@@ -117,6 +123,9 @@ namespace lue_amd {
                     {
                         p = static_cast<std::int16_t>(dr * TW + dc);
                         atomicAdd(&s.count[p], 1u);
+                        // the parent sees this cell as its neighbour opposite(dir)
+                        atomicOr(reinterpret_cast<unsigned int*>(s.children) + (p >> 2),
+                                 (1u << opposite(dir)) << (8 * (p & 3)));
                     }
                 }
             }
@@ -146,15 +155,12 @@ namespace lue_amd {
                 std::uint32_t best_size = 0;
                 int best = -1;
                 int best_dir = 0;
-                for (int k = 0; k < 8; ++k)
+                unsigned m = s.children[i];
+                while (m != 0u)  // the local children only, in neighbour order
                 {
-                    int const ur = lr + neighbour_dr(k);
-                    int const uc = lc + neighbour_dc(k);
-                    if (ur < 0 || ur >= TH || uc < 0 || uc >= TW)
-                        continue;
-                    int const u = ur * TW + uc;
-                    if (!s.local[u] || s.parent[u] != i)
-                        continue;
+                    int const k = __builtin_ctz(m);
+                    m &= m - 1u;
+                    int const u = (lr + neighbour_dr(k)) * TW + (lc + neighbour_dc(k));
                     std::uint32_t const su = __hip_atomic_load(&s.size[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     size += su;
                     if (su > best_size)
@@ -198,6 +204,7 @@ namespace lue_amd {
             }
             s.length[i] = d;
             s.level[i] = 0;
+            s.base[i] = 0xffff;  // no position unless its level is packed
         }
         __syncthreads();
 
@@ -244,74 +251,134 @@ namespace lue_amd {
         // level costs the sum over the parts of ceil(part / 64) blocks. The level's ceil(cells / 64) blocks
         // are dealt out to the wavefronts as capacities (multiples of 64) and the chains, in cell order, go
         // first-fit into the parts (a chain that fits nowhere goes to the part with the most room left).
-        if (tid == 0)
+        // The heads are first listed in cell order (ballots); levels are independent, so lane l of the first
+        // wavefront packs level l, its four parts in registers.
+        static_assert(G::n % 64 == 0 && sweep_waves == 4, "tile layout packing assumes 64-cell groups, 4 parts");
+        int const lane = tid & 63;
+        for (int g = tid >> 6; g < G::n / 64; g += nthreads >> 6)
+        {
+            int const i = g * 64 + lane;
+            bool const head = s.local[i] && s.head_of[i] == i;
+            std::uint64_t const ballot = __ballot(head);
+            if (lane == 0)
+                s.wave_heads[g] = static_cast<std::uint32_t>(__popcll(ballot));
+        }
+        __syncthreads();
+        if (tid < 64)
+        {
+            // offsets of the 64-cell groups (at most G::n / 64 = 64 groups: one per lane)
+            std::uint32_t count = lane < G::n / 64 ? s.wave_heads[lane] : 0u;
+            std::uint32_t incl = count;
+            for (int d = 1; d < 64; d <<= 1)
+            {
+                std::uint32_t const up = __shfl_up(incl, d, 64);
+                if (lane >= d)
+                    incl += up;
+            }
+            if (lane < G::n / 64)
+                s.wave_heads[lane] = incl - count;
+            if (lane == G::n / 64 - 1)
+                s.nheads = incl;
+        }
+        __syncthreads();
+        for (int g = tid >> 6; g < G::n / 64; g += nthreads >> 6)
+        {
+            int const i = g * 64 + lane;
+            bool const head = s.local[i] && s.head_of[i] == i;
+            std::uint64_t const ballot = __ballot(head);
+            if (head)
+            {
+                std::uint64_t const below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+                s.heads[s.wave_heads[g] + __popcll(ballot & below)] = static_cast<std::uint16_t>(i);
+            }
+        }
+        __syncthreads();
+        if (tid < 64)
         {
             unsigned int const levels = s.levels;
-            std::uint32_t cap[tile_max_levels][sweep_waves];
-            std::uint32_t part[tile_max_levels][sweep_waves];
-            for (unsigned int l = 0; l < levels; ++l)
+            std::uint32_t const nheads = s.nheads;
+            unsigned int const l = static_cast<unsigned int>(lane);
+            if (l < levels)
             {
                 std::uint32_t const blocks = (s.level_cells[l] + 63) / 64;
+                std::uint32_t cap[sweep_waves];
+                std::uint32_t part[sweep_waves];
+#pragma unroll
                 for (int w = 0; w < sweep_waves; ++w)
                 {
-                    cap[l][w] = 64 * ((blocks + sweep_waves - 1 - w) / sweep_waves);
-                    part[l][w] = 0;
+                    cap[w] = 64 * ((blocks + sweep_waves - 1 - w) / sweep_waves);
+                    part[w] = 0;
                 }
-            }
-            // pass 1: the part of every chain (kept in base[] until pass 2)
-            for (int i = 0; i < G::n; ++i)
-            {
-                if (!s.local[i] || s.head_of[i] != i)
-                    continue;
-                std::uint32_t const l = s.level[i];
-                if (l >= levels)
+                // pass 1: the part of every chain of level l (kept in base[] until pass 2)
+                for (std::uint32_t h = 0; h < nheads; ++h)
                 {
-                    s.base[i] = 0xffff;  // too many levels (error set above): no position
-                    continue;
+                    int const i = s.heads[h];
+                    if (s.level[i] != l)
+                        continue;
+                    std::uint32_t const len = s.length[i];
+                    int chosen = -1;
+                    int roomiest = 0;
+                    std::int64_t room_best = static_cast<std::int64_t>(cap[0]) - part[0];
+#pragma unroll
+                    for (int w = 0; w < sweep_waves; ++w)
+                    {
+                        std::int64_t const room = static_cast<std::int64_t>(cap[w]) - part[w];
+                        if (chosen < 0 && room >= static_cast<std::int64_t>(len))
+                            chosen = w;
+                        if (room > room_best)
+                        {
+                            roomiest = w;
+                            room_best = room;
+                        }
+                    }
+                    if (chosen < 0)
+                        chosen = roomiest;
+#pragma unroll
+                    for (int w = 0; w < sweep_waves; ++w)
+                        part[w] += w == chosen ? len : 0u;
+                    s.base[i] = static_cast<std::uint16_t>(chosen);
                 }
-                std::uint32_t const len = s.length[i];
-                int chosen = -1;
-                int roomiest = 0;
-                for (int w = 0; w < sweep_waves; ++w)
-                {
-                    std::int64_t const room = static_cast<std::int64_t>(cap[l][w]) - part[l][w];
-                    if (chosen < 0 && room >= static_cast<std::int64_t>(len))
-                        chosen = w;
-                    if (room > static_cast<std::int64_t>(cap[l][roomiest]) - part[l][roomiest])
-                        roomiest = w;
-                }
-                if (chosen < 0)
-                    chosen = roomiest;
-                part[l][chosen] += len;
-                s.base[i] = static_cast<std::uint16_t>(chosen);
-            }
-            // part boundaries, then pass 2: positions
-            std::uint32_t at = 0;
-            for (unsigned int l = 0; l < levels; ++l)
-            {
+                // part boundaries: level l starts after the cells of the levels below it
+                std::uint32_t at = 0;
+                for (unsigned int k = 0; k < l; ++k)
+                    at += s.level_cells[k];
+#pragma unroll
                 for (int w = 0; w < sweep_waves; ++w)
                 {
                     s.split[l * (sweep_waves + 1) + w] = static_cast<std::uint16_t>(at);
-                    std::uint32_t const size = part[l][w];
-                    part[l][w] = at;  // cursor from here on
+                    std::uint32_t const size = part[w];
+                    part[w] = at;  // cursor from here on
                     at += size;
                 }
                 s.split[l * (sweep_waves + 1) + sweep_waves] = static_cast<std::uint16_t>(at);
+                // pass 2: positions
+                for (std::uint32_t h = 0; h < nheads; ++h)
+                {
+                    int const i = s.heads[h];
+                    if (s.level[i] != l)
+                        continue;
+                    int const w = s.base[i];
+                    std::uint32_t cursor = 0;
+#pragma unroll
+                    for (int k = 0; k < sweep_waves; ++k)
+                    {
+                        if (k == w)
+                        {
+                            cursor = part[k];
+                            part[k] += s.length[i];
+                        }
+                    }
+                    s.base[i] = static_cast<std::uint16_t>(cursor);
+                }
             }
-            for (int i = 0; i < G::n; ++i)
+            else if (l < static_cast<unsigned int>(tile_max_levels))
             {
-                if (!s.local[i] || s.head_of[i] != i)
-                    continue;
-                std::uint32_t const l = s.level[i];
-                if (l >= levels)
-                    continue;
-                int const w = s.base[i];
-                s.base[i] = static_cast<std::uint16_t>(part[l][w]);
-                part[l][w] += s.length[i];
-            }
-            for (unsigned int l = levels; l < static_cast<unsigned int>(tile_max_levels); ++l)
+                std::uint32_t total = 0;
+                for (unsigned int k = 0; k < levels; ++k)
+                    total += s.level_cells[k];
                 for (int w = 0; w <= sweep_waves; ++w)
-                    s.split[l * (sweep_waves + 1) + w] = static_cast<std::uint16_t>(at);
+                    s.split[l * (sweep_waves + 1) + w] = static_cast<std::uint16_t>(total);
+            }
         }
         __syncthreads();
 
