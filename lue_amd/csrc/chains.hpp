@@ -505,20 +505,42 @@ namespace lue_amd {
     // the sorted chain indices, `level_sorted` their levels.
     constexpr int chains_max_levels = 64;
 
+    // Lengths in sorted order, the first sorted chain of every level (level_first, set to ~0 before) and the
+    // longest chain (a wavefront maximum, one atomic per wavefront).
     __global__ inline void chains_sorted_lengths_kernel(std::uint32_t const* order, std::uint32_t const* length,
                                                         std::uint32_t const* level_sorted, std::uint32_t nchains,
-                                                        std::uint32_t* len_sorted, std::uint32_t* level_cells,
+                                                        std::uint32_t* len_sorted, std::uint32_t* level_first,
                                                         std::uint32_t* longest)
     {
         std::uint32_t const k = blockIdx.x * blockDim.x + threadIdx.x;
-        if (k >= nchains)
+        std::uint32_t len = 0;
+        if (k < nchains)
+        {
+            len = length[order[k]];
+            len_sorted[k] = len;
+            std::uint32_t const l = level_sorted[k];
+            if ((k == 0 || level_sorted[k - 1] != l) && l < static_cast<std::uint32_t>(chains_max_levels))
+                level_first[l] = k;
+        }
+        for (int offset = 32; offset > 0; offset >>= 1)
+        {
+            std::uint32_t const other = __shfl_xor(len, offset, 64);
+            len = other > len ? other : len;
+        }
+        if ((threadIdx.x & 63) == 0 && len > 0)
+            atomicMax(longest, len);
+    }
+
+    // Per level present: its first position and its first work item (a level's first chain starts an item).
+    __global__ inline void chains_level_stats_kernel(std::uint32_t const* level_first, std::uint32_t const* base_sorted,
+                                                     std::uint32_t const* item_incl, std::uint32_t* stats)
+    {
+        int const l = static_cast<int>(threadIdx.x);
+        if (l >= chains_max_levels)
             return;
-        std::uint32_t const len = length[order[k]];
-        len_sorted[k] = len;
-        std::uint32_t const l = level_sorted[k];
-        if (l < static_cast<std::uint32_t>(chains_max_levels))
-            atomicAdd(&level_cells[l], len);
-        atomicMax(longest, len);
+        std::uint32_t const k = level_first[l];
+        stats[l] = k == ~0u ? ~0u : base_sorted[k];
+        stats[chains_max_levels + l] = k == ~0u ? ~0u : item_incl[k] - 1;
     }
 
     // Base position of every chain, and the work-item starts: a chain longer than 64 cells is an item of its
@@ -549,7 +571,7 @@ namespace lue_amd {
                                                    std::uint32_t const* base_sorted, std::uint32_t const* item_flag,
                                                    std::uint32_t const* item_incl, std::uint32_t nchains,
                                                    std::uint32_t* item_begin, std::uint32_t* item_end,
-                                                   std::uint32_t* item_level, std::uint32_t* level_items)
+                                                   std::uint32_t* item_level)
     {
         std::uint32_t const k = blockIdx.x * blockDim.x + threadIdx.x;
         if (k >= nchains)
@@ -559,8 +581,6 @@ namespace lue_amd {
         {
             item_begin[id] = base_sorted[k];
             item_level[id] = level_sorted[k];
-            if (level_sorted[k] < static_cast<std::uint32_t>(chains_max_levels))
-                atomicAdd(&level_items[level_sorted[k]], 1u);
         }
         if (k + 1 == nchains || item_flag[k + 1])
             item_end[id] = base_sorted[k] + len_sorted[k];

@@ -1095,15 +1095,20 @@ namespace lue_amd {
             LUE_AMD_HIP(base_sorted.allocate(nchains, stream));
             LUE_AMD_HIP(item_flag.allocate(nchains, stream));
             LUE_AMD_HIP(item_incl.allocate(nchains, stream));
-            // stats: [0, 64) cells per level, [64, 128) items per level, [128] longest chain
+            // stats: [0, 64) first position per level, [64, 128) first item per level (~0: no such level),
+            // [128] longest chain; level_first: the first sorted chain of each level
             constexpr int nstats = 2 * chains_max_levels + 1;
+            DeviceArray<std::uint32_t> level_first;
             LUE_AMD_HIP(stats.allocate(nstats, stream));
+            LUE_AMD_HIP(level_first.allocate(chains_max_levels, stream));
             hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, stream, stats.ptr, 0u,
                                static_cast<std::uint64_t>(nstats));
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, level_first.ptr, ~0u,
+                               static_cast<std::uint64_t>(chains_max_levels));
             hipLaunchKernelGGL(iota_u32_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, order_in.ptr, nchains);
             LUE_AMD_HIP(sort_pairs(level.ptr, level_sorted.ptr, order_in.ptr, order.ptr, nchains, 7, stream));
             hipLaunchKernelGGL(chains_sorted_lengths_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, order.ptr,
-                               length.ptr, level_sorted.ptr, nchains, len_sorted.ptr, stats.ptr,
+                               length.ptr, level_sorted.ptr, nchains, len_sorted.ptr, level_first.ptr,
                                stats.ptr + 2 * chains_max_levels);
             LUE_AMD_HIP(exclusive_scan(len_sorted.ptr, base_sorted.ptr, nchains, stream));
             hipLaunchKernelGGL(chains_items_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, order.ptr,
@@ -1122,7 +1127,9 @@ namespace lue_amd {
             LUE_AMD_HIP(key.allocate(nitems, stream));
             hipLaunchKernelGGL(chains_item_fill_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream,
                                level_sorted.ptr, len_sorted.ptr, base_sorted.ptr, item_flag.ptr, item_incl.ptr, nchains,
-                               item_begin.ptr, item_end.ptr, item_level.ptr, stats.ptr + chains_max_levels);
+                               item_begin.ptr, item_end.ptr, item_level.ptr);
+            hipLaunchKernelGGL(chains_level_stats_kernel, dim3(1), dim3(chains_max_levels), 0, stream, level_first.ptr,
+                               base_sorted.ptr, item_incl.ptr, stats.ptr);
             hipLaunchKernelGGL(chains_item_key_kernel, dim3(blocks_for(nitems)), dim3(256), 0, stream, item_begin.ptr,
                                item_end.ptr, item_level.ptr, nitems, key_in.ptr, index_in.ptr);
             LUE_AMD_HIP(sort_pairs(key_in.ptr, key.ptr, index_in.ptr, index.ptr, nitems, 32 + 7, stream));
@@ -1133,22 +1140,18 @@ namespace lue_amd {
             LUE_AMD_HIP(hipGetLastError());
             std::uint32_t h_stats[nstats];
             LUE_AMD_HIP(copy_to_host(h_stats, stats.ptr, nstats, stream));
+            // levels are 0 .. nlevels - 1 without gaps (a chain's level is one more than one of its side chains')
             std::uint32_t nlevels = 0;
-            std::uint64_t cells = 0;
-            for (int l = 0; l < chains_max_levels; ++l)
-            {
-                if (h_stats[l] > 0)
-                    nlevels = static_cast<std::uint32_t>(l) + 1;
-                cells += h_stats[l];
-            }
-            if (cells != n)
-                return fail(status_invalid_network, "chain layout: cells not accounted for");
+            while (nlevels < static_cast<std::uint32_t>(chains_max_levels) && h_stats[nlevels] != ~0u)
+                ++nlevels;
+            if (nlevels == 0 || h_stats[0] != 0 || h_stats[chains_max_levels] != 0)
+                return fail(status_invalid_network, "chain layout: levels not accounted for");
             net.level_pos.assign(1, 0);
             net.level_items.assign(1, 0);
-            for (std::uint32_t l = 0; l < nlevels; ++l)
+            for (std::uint32_t l = 1; l <= nlevels; ++l)
             {
-                net.level_pos.push_back(net.level_pos.back() + h_stats[l]);
-                net.level_items.push_back(net.level_items.back() + h_stats[chains_max_levels + l]);
+                net.level_pos.push_back(l < nlevels ? h_stats[l] : n);
+                net.level_items.push_back(l < nlevels ? h_stats[chains_max_levels + l] : nitems);
             }
             net.longest_chain = h_stats[2 * chains_max_levels];
             LUE_AMD_HIP(hipStreamSynchronize(stream));  // before the temporaries go
