@@ -372,6 +372,62 @@ int lue_amd_network_kinematic_wave_f64(lue_amd_network network, const double* co
                                        double* const* outflow, void* stream);
 
 /*
+ * Plans: one operation on a network with fixed arguments, captured into a HIP graph (no reference
+ * counterpart; LUE's timestep loops re-issue the operation's tasks, source/framework/model/include/lue/
+ * framework/model/simulate.hpp:60-113). Creating a plan allocates the operation's per-call buffers, writes
+ * the per-catchment argument table once and captures the launch sequence (waits for the device); a launch
+ * replays the whole sequence as one graph launch on `stream` (asynchronous), reading and writing the rasters
+ * given at creation (their contents may change between launches, the pointers may not). The network and the
+ * rasters must outlive the plan; launches of one plan must not overlap (they share its buffers). The plan
+ * functions take the arguments of the corresponding lue_amd_network_* functions, without the stream.
+ * info[0..1]: kernel launches per replay (graph nodes), bytes of per-call buffers.
+ */
+typedef struct lue_amd_plan_s* lue_amd_plan;
+
+int lue_amd_plan_launch(lue_amd_plan plan, void* stream);
+int lue_amd_plan_destroy(lue_amd_plan plan);
+int lue_amd_plan_info(lue_amd_plan plan, int64_t* info, int n);
+
+int lue_amd_plan_accu_f32(lue_amd_network network, const float* const* inflow, float inflow_scalar,
+                          float* const* outflow, lue_amd_plan* plan);
+int lue_amd_plan_accu_f64(lue_amd_network network, const double* const* inflow, double inflow_scalar,
+                          double* const* outflow, lue_amd_plan* plan);
+int lue_amd_plan_accu_threshold_f32(lue_amd_network network, const float* const* inflow, float inflow_scalar,
+                                    const float* const* threshold, float threshold_scalar, float* const* outflow,
+                                    float* const* remainder, lue_amd_plan* plan);
+int lue_amd_plan_accu_threshold_f64(lue_amd_network network, const double* const* inflow, double inflow_scalar,
+                                    const double* const* threshold, double threshold_scalar, double* const* outflow,
+                                    double* const* remainder, lue_amd_plan* plan);
+int lue_amd_plan_accu_fraction_f32(lue_amd_network network, const float* const* inflow, float inflow_scalar,
+                                   const float* const* fraction, float fraction_scalar, float* const* outflow,
+                                   float* const* remainder, lue_amd_plan* plan);
+int lue_amd_plan_accu_fraction_f64(lue_amd_network network, const double* const* inflow, double inflow_scalar,
+                                   const double* const* fraction, double fraction_scalar, double* const* outflow,
+                                   double* const* remainder, lue_amd_plan* plan);
+int lue_amd_plan_accu_capacity_f32(lue_amd_network network, const float* const* inflow, float inflow_scalar,
+                                   const float* const* capacity, float capacity_scalar, float* const* outflow,
+                                   float* const* remainder, lue_amd_plan* plan);
+int lue_amd_plan_accu_capacity_f64(lue_amd_network network, const double* const* inflow, double inflow_scalar,
+                                   const double* const* capacity, double capacity_scalar, double* const* outflow,
+                                   double* const* remainder, lue_amd_plan* plan);
+int lue_amd_plan_accu_trigger_f32(lue_amd_network network, const float* const* inflow, float inflow_scalar,
+                                  const float* const* trigger, float trigger_scalar, float* const* outflow,
+                                  float* const* remainder, lue_amd_plan* plan);
+int lue_amd_plan_accu_trigger_f64(lue_amd_network network, const double* const* inflow, double inflow_scalar,
+                                  const double* const* trigger, double trigger_scalar, double* const* outflow,
+                                  double* const* remainder, lue_amd_plan* plan);
+int lue_amd_plan_kinematic_wave_f32(lue_amd_network network, const float* const* current_outflow,
+                                    const float* const* inflow, const float* const* alpha, float alpha_scalar,
+                                    const float* const* beta, float beta_scalar, float time_step_duration,
+                                    const float* const* channel_length, float channel_length_scalar,
+                                    float* const* outflow, lue_amd_plan* plan);
+int lue_amd_plan_kinematic_wave_f64(lue_amd_network network, const double* const* current_outflow,
+                                    const double* const* inflow, const double* const* alpha, double alpha_scalar,
+                                    const double* const* beta, double beta_scalar, double time_step_duration,
+                                    const double* const* channel_length, double channel_length_scalar,
+                                    double* const* outflow, lue_amd_plan* plan);
+
+/*
  * lue_amd_network_kinematic_wave_f32 for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid allocation). No counterpart in the reference. It
  * allocates nothing: its per-call buffers (the replay's pending bitmaps, compensation terms, staged chain

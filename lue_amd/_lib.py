@@ -45,6 +45,13 @@ SIGNATURES = {
     "lue_amd_network_kinematic_wave_f32": [c_p, c_p, c_p, c_p, c_f, c_p, c_f, c_f, c_p, c_f, c_p, c_p],
     "lue_amd_network_kinematic_wave_f64": [c_p, c_p, c_p, c_p, c_d, c_p, c_d, c_d, c_p, c_d, c_p, c_p],
     "lue_amd_network_kinematic_wave_f32_noalloc": [c_p, c_p, c_p, c_p, c_f, c_p, c_f, c_f, c_p, c_f, c_p, c_p],
+    "lue_amd_plan_launch": [c_p, c_p],
+    "lue_amd_plan_destroy": [c_p],
+    "lue_amd_plan_info": [c_p, c_p, c_int],
+    "lue_amd_plan_accu_f32": [c_p, c_p, c_f, c_p, ctypes.POINTER(c_p)],
+    "lue_amd_plan_accu_f64": [c_p, c_p, c_d, c_p, ctypes.POINTER(c_p)],
+    "lue_amd_plan_kinematic_wave_f32": [c_p, c_p, c_p, c_p, c_f, c_p, c_f, c_f, c_p, c_f, c_p, ctypes.POINTER(c_p)],
+    "lue_amd_plan_kinematic_wave_f64": [c_p, c_p, c_p, c_p, c_d, c_p, c_d, c_d, c_p, c_d, c_p, ctypes.POINTER(c_p)],
     "lue_amd_profile_enable": [c_int],
     "lue_amd_profile_read": [ctypes.POINTER(c_d), ctypes.POINTER(c_d)] + [ctypes.POINTER(ctypes.c_longlong)] * 3,
     "lue_amd_profile_read_op": [c_int, c_int] + [ctypes.POINTER(c_d)] * 4 + [ctypes.POINTER(ctypes.c_longlong)],
@@ -82,6 +89,9 @@ SIGNATURES = {
     "lue_amd_synthetic_uniform_f32": [c_u64, c_u64, c_d, c_d, c_i64, c_p, c_p],
     "lue_amd_synthetic_uniform_f64": [c_u64, c_u64, c_d, c_d, c_i64, c_p, c_p],
 }
+for _op in ("threshold", "fraction", "capacity", "trigger"):
+    SIGNATURES[f"lue_amd_plan_accu_{_op}_f32"] = [c_p, c_p, c_f, c_p, c_f, c_p, c_p, ctypes.POINTER(c_p)]
+    SIGNATURES[f"lue_amd_plan_accu_{_op}_f64"] = [c_p, c_p, c_d, c_p, c_d, c_p, c_p, ctypes.POINTER(c_p)]
 SIGNATURES["lue_amd_downstream_distance_f32"] = [c_p, c_i64, c_i64, c_f, c_int, c_p, c_p]
 SIGNATURES["lue_amd_downstream_distance_f64"] = [c_p, c_i64, c_i64, c_d, c_int, c_p, c_p]
 for _op in ("upstream", "downstream"):

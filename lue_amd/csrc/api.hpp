@@ -281,10 +281,19 @@ namespace lue_amd {
         return table ? table[c] : nullptr;
     }
 
+    // Capture one operation on a network into a plan (network.hpp: Plan).
+    template<int OP, typename T>
+    int plan_network(Network const& net, Scalars<T> const& sc, std::vector<CatchmentIO<T>> const& io, Plan& plan)
+    {
+        return with_tile_shape<OP, T>(net.tile_shape, [&]<int TW, int TH>()
+                                      { return plan_capture<OP, T, TW, TH>(net, args_from(sc), io, plan); });
+    }
+
     // Network operations with two outputs: every catchment needs both output rasters.
+    // plan == nullptr: run on `stream`; else capture into *plan.
     template<int OP, typename T>
     int network_split(Network const& net, T const* const* inflow, T inflow_s, T const* const* param, T param_s,
-                      T* const* outflow, T* const* remainder, hipStream_t stream)
+                      T* const* outflow, T* const* remainder, hipStream_t stream, Plan* plan = nullptr)
     {
         if (!outflow || !remainder)
             return fail(status_invalid_argument, "outflow and remainder must not be null");
@@ -299,8 +308,12 @@ namespace lue_amd {
             io.push_back(CatchmentIO<T>{entry(inflow, c), entry(param, c), nullptr, nullptr, nullptr, outflow[c],
                                         remainder[c]});
         }
-        return route_network<OP, T>(net, sc, io, stream);
+        return plan ? plan_network<OP, T>(net, sc, io, *plan) : route_network<OP, T>(net, sc, io, stream);
     }
+
+    // Wrap a capture into a new C ABI plan handle.
+    template<typename F>
+    int make_plan(lue_amd_plan* out, F&& capture);
 
 }  // namespace lue_amd
 
@@ -308,3 +321,21 @@ struct lue_amd_network_s
 {
     lue_amd::Network net;
 };
+
+struct lue_amd_plan_s
+{
+    lue_amd::Plan plan;
+};
+
+template<typename F>
+int lue_amd::make_plan(lue_amd_plan* out, F&& capture)
+{
+    if (!out)
+        return fail(status_invalid_argument, "plan must not be null");
+    *out = nullptr;
+    auto handle = std::make_unique<lue_amd_plan_s>();
+    if (int const st = capture(handle->plan))
+        return st;
+    *out = handle.release();
+    return status_success;
+}

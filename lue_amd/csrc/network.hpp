@@ -1353,7 +1353,7 @@ namespace lue_amd {
     // profiler's events are taken.
     template<int OP, typename T, int TW, int TH>
     int replay_launch(Network const& net, SweepArgs<T> a, unsigned char* base, ReplayLayout const& l,
-                      hipStream_t stream, bool timed = true)
+                      hipStream_t stream, bool timed)
     {
         bool const chains = net.schedule == schedule_chains;
         a.cats = net.cats.ptr;
@@ -1437,6 +1437,71 @@ namespace lue_amd {
         return timer.stop(stream, chains ? net.levels() + 1 : net.rounds() + 1, net.ncells);
     }
 
+    // A captured operation on a network: its own per-call buffers (replay_layout) with the IO table written
+    // once, and the HIP graph of its launch sequence. A launch replays the graph on the caller's stream; the
+    // arguments (raster pointers and scalars) are those given at capture. The network and the rasters must
+    // outlive the plan.
+    struct Plan
+    {
+        Network const* net{};
+        unsigned char* buffer{};
+        hipGraph_t graph{};
+        hipGraphExec_t exec{};
+        std::size_t nodes{};
+        std::size_t bytes{};
+
+        Plan() = default;
+        Plan(Plan const&) = delete;
+        auto operator=(Plan const&) -> Plan& = delete;
+
+        // a launch may still use the buffers: wait for the device first
+        ~Plan()
+        {
+            int current = 0;
+            bool const switched = net && hipGetDevice(&current) == hipSuccess && current != net->device &&
+                                  hipSetDevice(net->device) == hipSuccess;
+            (void)hipDeviceSynchronize();
+            if (exec)
+                (void)hipGraphExecDestroy(exec);
+            if (graph)
+                (void)hipGraphDestroy(graph);
+            if (buffer)
+                (void)hipFree(buffer);
+            if (switched)
+                (void)hipSetDevice(current);
+        }
+    };
+
+    template<int OP, typename T, int TW, int TH>
+    int plan_capture(Network const& net, SweepArgs<T> a, std::vector<CatchmentIO<T>> const& io, Plan& plan)
+    {
+        if (!net.recorded)
+            return fail(status_invalid_argument, "network has no recorded schedule");
+        if (io.size() != net.cats_host.size())
+            return fail(status_invalid_argument, "one input/output set per catchment");
+        ReplayLayout const l = replay_layout<OP, T, TW, TH>(net);
+        plan.net = &net;
+        plan.bytes = l.total;
+        LUE_AMD_HIP(hipMalloc(reinterpret_cast<void**>(&plan.buffer), l.total));
+        LUE_AMD_HIP(hipMemcpy(plan.buffer + l.io, io.data(), sizeof(CatchmentIO<T>) * io.size(), hipMemcpyHostToDevice));
+        hipStream_t s = nullptr;
+        LUE_AMD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        int st = status_success;
+        if (e == hipSuccess)
+        {
+            st = replay_launch<OP, T, TW, TH>(net, a, plan.buffer, l, s, false);
+            e = hipStreamEndCapture(s, &plan.graph);  // ends the capture whatever the launches returned
+        }
+        (void)hipStreamDestroy(s);
+        if (st != status_success)
+            return st;
+        LUE_AMD_HIP(e);
+        LUE_AMD_HIP(hipGraphGetNodes(plan.graph, nullptr, &plan.nodes));
+        LUE_AMD_HIP(hipGraphInstantiate(&plan.exec, plan.graph, nullptr, nullptr, 0));
+        return status_success;
+    }
+
     // Replay: the operation as a fixed launch sequence from the recorded schedule, its per-call buffers one
     // block of the scratch cache (no host synchronisation).
     template<int OP, typename T, int TW, int TH>
@@ -1452,7 +1517,7 @@ namespace lue_amd {
         if (int const st = write_io<T>(io.data(), static_cast<int>(io.size()),
                                        reinterpret_cast<CatchmentIO<T>*>(block.ptr + l.io), stream))
             return st;
-        return replay_launch<OP, T, TW, TH>(net, a, block.ptr, l, stream);
+        return replay_launch<OP, T, TW, TH>(net, a, block.ptr, l, stream, true);
     }
 
     // Record: run round 0 (and, for the rounds schedule, the later rounds) while recording the schedule into

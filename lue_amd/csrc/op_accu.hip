@@ -77,26 +77,55 @@ int lue_amd_network_info(lue_amd_network network, int64_t* info, int n)
     return status_success;
 }
 
-#define LUE_AMD_NETWORK_ACCU(T, SUFFIX)                                                                        \
-    int lue_amd_network_accu_##SUFFIX(lue_amd_network network, const T* const* inflow, T inflow_scalar,         \
-                                      T* const* outflow, void* stream)                                          \
-    {                                                                                                           \
-        if (!network || !outflow)                                                                               \
-            return fail(status_invalid_argument, "null argument");                                              \
-        Network const& net = network->net;                                                                      \
-        Scalars<T> sc{};                                                                                        \
-        sc.in1_s = inflow_scalar;                                                                               \
-        std::vector<CatchmentIO<T>> io;                                                                         \
-        for (int c = 0; c < static_cast<int>(net.cats_host.size()); ++c)                                        \
-        {                                                                                                       \
-            if (!outflow[c])                                                                                    \
-                return fail(status_invalid_argument, "outflow must not be null");                              \
-            io.push_back(CatchmentIO<T>{entry(inflow, c), nullptr, nullptr, nullptr, nullptr, outflow[c], nullptr}); \
-        }                                                                                                       \
-        return route_network<op_accu, T>(net, sc, io, static_cast<hipStream_t>(stream));                        \
+}  // extern "C"
+
+namespace {
+
+    template<typename T>
+    int network_accu(lue_amd_network network, T const* const* inflow, T inflow_scalar, T* const* outflow,
+                     hipStream_t stream, Plan* plan)
+    {
+        if (!network || !outflow)
+            return fail(status_invalid_argument, "null argument");
+        Network const& net = network->net;
+        Scalars<T> sc{};
+        sc.in1_s = inflow_scalar;
+        std::vector<CatchmentIO<T>> io;
+        for (int c = 0; c < static_cast<int>(net.cats_host.size()); ++c)
+        {
+            if (!outflow[c])
+                return fail(status_invalid_argument, "outflow must not be null");
+            io.push_back(CatchmentIO<T>{entry(inflow, c), nullptr, nullptr, nullptr, nullptr, outflow[c], nullptr});
+        }
+        return plan ? plan_network<op_accu, T>(net, sc, io, *plan) : route_network<op_accu, T>(net, sc, io, stream);
     }
 
-LUE_AMD_NETWORK_ACCU(float, f32)
-LUE_AMD_NETWORK_ACCU(double, f64)
+}  // namespace
+
+extern "C" {
+
+int lue_amd_network_accu_f32(lue_amd_network network, const float* const* inflow, float inflow_scalar,
+                             float* const* outflow, void* stream)
+{
+    return network_accu<float>(network, inflow, inflow_scalar, outflow, static_cast<hipStream_t>(stream), nullptr);
+}
+
+int lue_amd_network_accu_f64(lue_amd_network network, const double* const* inflow, double inflow_scalar,
+                             double* const* outflow, void* stream)
+{
+    return network_accu<double>(network, inflow, inflow_scalar, outflow, static_cast<hipStream_t>(stream), nullptr);
+}
+
+int lue_amd_plan_accu_f32(lue_amd_network network, const float* const* inflow, float inflow_scalar,
+                          float* const* outflow, lue_amd_plan* plan)
+{
+    return make_plan(plan, [&](Plan& p) { return network_accu<float>(network, inflow, inflow_scalar, outflow, nullptr, &p); });
+}
+
+int lue_amd_plan_accu_f64(lue_amd_network network, const double* const* inflow, double inflow_scalar,
+                          double* const* outflow, lue_amd_plan* plan)
+{
+    return make_plan(plan, [&](Plan& p) { return network_accu<double>(network, inflow, inflow_scalar, outflow, nullptr, &p); });
+}
 
 }  // extern "C"

@@ -40,4 +40,28 @@ int lue_amd_network_accu_trigger_f64(lue_amd_network network, const double* cons
                                          remainder, static_cast<hipStream_t>(stream));
 }
 
+int lue_amd_plan_accu_trigger_f32(lue_amd_network network, const float* const* inflow, float inflow_scalar,
+                             const float* const* trigger, float trigger_scalar, float* const* outflow, float* const* remainder,
+                             lue_amd_plan* plan)
+{
+    if (!network)
+        return fail(status_invalid_argument, "null argument");
+    return make_plan(plan, [&](Plan& p) {
+        return network_split<op_trigger, float>(network->net, inflow, inflow_scalar, trigger, trigger_scalar, outflow,
+                                          remainder, nullptr, &p);
+    });
+}
+
+int lue_amd_plan_accu_trigger_f64(lue_amd_network network, const double* const* inflow, double inflow_scalar,
+                             const double* const* trigger, double trigger_scalar, double* const* outflow, double* const* remainder,
+                             lue_amd_plan* plan)
+{
+    if (!network)
+        return fail(status_invalid_argument, "null argument");
+    return make_plan(plan, [&](Plan& p) {
+        return network_split<op_trigger, double>(network->net, inflow, inflow_scalar, trigger, trigger_scalar, outflow,
+                                          remainder, nullptr, &p);
+    });
+}
+
 }  // extern "C"

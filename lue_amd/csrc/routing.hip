@@ -344,6 +344,31 @@ int lue_amd_profile_read_op(int op, int f64, double* first_round_ms, double* nex
     return status_success;
 }
 
+int lue_amd_plan_launch(lue_amd_plan plan, void* stream)
+{
+    if (!plan || !plan->plan.exec)
+        return fail(status_invalid_argument, "null plan");
+    LUE_AMD_HIP(hipGraphLaunch(plan->plan.exec, static_cast<hipStream_t>(stream)));
+    return status_success;
+}
+
+int lue_amd_plan_destroy(lue_amd_plan plan)
+{
+    // ~Plan waits for the device before it frees the graph and the buffers
+    delete plan;
+    return status_success;
+}
+
+int lue_amd_plan_info(lue_amd_plan plan, int64_t* info, int n)
+{
+    if (!plan || !info || n < 0)
+        return fail(status_invalid_argument, "null argument");
+    int64_t const values[] = {static_cast<int64_t>(plan->plan.nodes), static_cast<int64_t>(plan->plan.bytes)};
+    for (int i = 0; i < n && i < static_cast<int>(sizeof(values) / sizeof(values[0])); ++i)
+        info[i] = values[i];
+    return status_success;
+}
+
 int lue_amd_device_malloc(void** ptr, size_t bytes)
 {
     if (!ptr)

@@ -452,6 +452,57 @@ class FlowNetwork:
     def _unwrap(self, outs):
         return outs[0] if len(self) == 1 else outs
 
+    # -- plans: one operation with fixed rasters, captured into a HIP graph ---------------------------
+    def _plan(self, name, fn_args, outs, keep):
+        import ctypes
+
+        handle = ctypes.c_void_p()
+        _lib.check(getattr(_lib.load(), name)(self._handle, *fn_args, ctypes.byref(handle)))
+        return Plan(self, handle, outs, keep)
+
+    def plan_accu(self, inflow, out=None, dtype=None):
+        """accu with these rasters as a Plan (one graph launch per run); outputs in plan.outputs."""
+        dtype = self._dtype(inflow, dtype=dtype)
+        p, s, keep = self._rasters(inflow, dtype, "inflow")
+        po, outs = self._outputs(dtype, out)
+        return self._plan(f"lue_amd_plan_accu_{FLOAT_TYPES[dtype]}", (p, s, po), [self._unwrap(outs)], (keep, outs))
+
+    def _plan_split(self, name, inflow, param, out=None, remainder=None, dtype=None):
+        dtype = self._dtype(inflow, param, dtype=dtype)
+        p1, s1, k1 = self._rasters(inflow, dtype, "inflow")
+        p2, s2, k2 = self._rasters(param, dtype, name)
+        po, outs = self._outputs(dtype, out)
+        pr, rems = self._outputs(dtype, remainder)
+        return self._plan(f"lue_amd_plan_accu_{name}_{FLOAT_TYPES[dtype]}", (p1, s1, p2, s2, po, pr),
+                          [self._unwrap(outs), self._unwrap(rems)], (k1, k2, outs, rems))
+
+    def plan_accu_threshold(self, inflow, threshold, out=None, remainder=None, dtype=None):
+        return self._plan_split("threshold", inflow, threshold, out, remainder, dtype)
+
+    def plan_accu_fraction(self, inflow, fraction, out=None, remainder=None, dtype=None):
+        return self._plan_split("fraction", inflow, fraction, out, remainder, dtype)
+
+    def plan_accu_capacity(self, inflow, capacity, out=None, remainder=None, dtype=None):
+        return self._plan_split("capacity", inflow, capacity, out, remainder, dtype)
+
+    def plan_accu_trigger(self, inflow, trigger, out=None, remainder=None, dtype=None):
+        return self._plan_split("trigger", inflow, trigger, out, remainder, dtype)
+
+    def plan_kinematic_wave(self, current_outflow, inflow, alpha, beta, time_step_duration, channel_length,
+                            out=None, dtype=None):
+        dtype = self._dtype(current_outflow, inflow, dtype=dtype)
+        pc, _, kc = self._rasters(current_outflow, dtype, "current_outflow")
+        pq, _, kq = self._rasters(inflow, dtype, "inflow")
+        if pc is None or pq is None:
+            raise TypeError("current_outflow and inflow are rasters")
+        pa, sa, ka = self._rasters(alpha, dtype, "alpha")
+        pb, sb, kb = self._rasters(beta, dtype, "beta")
+        pl, sl, kl = self._rasters(channel_length, dtype, "channel_length")
+        po, outs = self._outputs(dtype, out)
+        return self._plan(f"lue_amd_plan_kinematic_wave_{FLOAT_TYPES[dtype]}",
+                          (pc, pq, pa, sa, pb, sb, float(time_step_duration), pl, sl, po), [self._unwrap(outs)],
+                          (kc, kq, ka, kb, kl, outs))
+
     # -- operations (asynchronous on the current stream) --------------------------------------------
     def accu(self, inflow, out=None, dtype=None):
         dtype = self._dtype(inflow, dtype=dtype)
@@ -500,3 +551,38 @@ class FlowNetwork:
         _lib.check(fn(self._handle, pc, pq, pa, sa, pb, sb, float(time_step_duration), pl, sl, po, _stream()))
         del kc, kq, ka, kb, kl
         return self._unwrap(outs)
+
+
+class Plan:
+    """One operation of a FlowNetwork with fixed rasters, its launch sequence captured into a HIP graph
+    (C ABI lue_amd_plan_*): run() replays it as one graph launch on the current stream. The rasters' contents
+    may change between runs (e.g. a new lateral inflow), not the rasters themselves; the plan keeps them and
+    its network alive. outputs: the output raster(s), as the network operation returns them."""
+
+    def __init__(self, network, handle, outputs, keep):
+        self.network = network
+        self._handle = handle
+        self.outputs = outputs[0] if len(outputs) == 1 else tuple(outputs)
+        self._keep = keep
+
+    def run(self):
+        _lib.check(_lib.load().lue_amd_plan_launch(self._handle, _stream()))
+        return self.outputs
+
+    def info(self):
+        import ctypes
+
+        values = (ctypes.c_int64 * 2)()
+        _lib.check(_lib.load().lue_amd_plan_info(self._handle, values, 2))
+        return {"launches": int(values[0]), "buffer_bytes": int(values[1])}
+
+    def close(self):
+        if getattr(self, "_handle", None):
+            _lib.load().lue_amd_plan_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
