@@ -249,6 +249,29 @@ def main():
     in_steps = {"accu_f32": _lib.profile_read_op("accu"), "kinematic_wave_f32": _lib.profile_read_op("kinematic_wave")}
     _lib.profile_enable(False)
 
+    # the same step as two graph launches: each operation's launch sequence captured once (plans)
+    log("[bench] fp32 graph steps")
+    plan_accu = net_accu.plan_accu(f32["material"], out=out32[0])
+    plan_kw = net_kw.plan_kinematic_wave(f32["q_cur"], f32["q_lat"], f32["alpha"], 0.6, 3600.0, 100.0, out=out32[1])
+
+    def graph_step():
+        with torch.cuda.stream(s_accu):
+            plan_accu.run()
+        with torch.cuda.stream(s_kw):
+            plan_kw.run()
+
+    for _ in range(args.warmup):
+        graph_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph_step()
+    torch.cuda.synchronize()
+    t32_graph = (time.perf_counter() - t0) / args.steps
+    launches = {"accu": plan_accu.info()["launches"], "kinematic_wave": plan_kw.info()["launches"]}
+    t32_eager = t32
+    t32 = min(t32_eager, t32_graph)
+
     # each operation alone (nothing else on the device): its uncontended first round and rest
     kernel_ms = {}
     for name, op, fn in (("accu_f32", "accu", lambda: net_accu.accu(f32["material"], out=out32[0])),
@@ -412,6 +435,10 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(t32 * 1e3, 3),
+        "step_paths_ms": {"graph": round(t32_graph * 1e3, 3), "eager": round(t32_eager * 1e3, 3),
+                          "note": "graph: each operation's launch sequence replayed as one HIP graph launch (plan); "
+                                  "eager: the same launches issued from the host; value = the faster"},
+        "launches_per_step": launches,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

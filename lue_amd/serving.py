@@ -1,12 +1,15 @@
 """Serving kinematic-wave timesteps for resident catchments.
 
-A Server holds catchments on the device: ldd, recorded flow network, alpha and the discharge state. A
-request asks for one timestep of one catchment with fresh lateral inflow. Requests are admitted under an
-HBM budget (the bytes of the resident catchments plus the transient bytes of the requests in flight must
-stay below it) and a number of slots (HIP streams). Admitted requests run as soon as a slot is free, on
-that slot's stream, so requests that arrive while others are on the GPU overlap with them (continuous
-batching); timesteps of one catchment stay in order because a catchment has at most one request in flight.
-A request's latency is from its arrival to the completion of its timestep (HIP event on its stream).
+A Server holds catchments on the device: ldd, recorded flow network, alpha, the discharge state (two
+buffers, the current one and the next), a lateral-inflow buffer and the catchment's timestep captured twice
+as HIP graphs (plans: current -> next, and back). Catchments are admitted under an HBM budget for all of
+that. A request asks for one timestep of one catchment with fresh lateral inflow: it writes the inflow into
+the catchment's buffer and launches the catchment's graph, so it allocates nothing and costs two launches.
+Requests run as soon as a slot (HIP stream) is free, on that slot's stream, so requests that arrive while
+others are on the GPU overlap with them (continuous batching: each dispatch launches every waiting request
+whose catchment is idle); timesteps of one catchment stay in order because a catchment has at most one
+request in flight. A request's latency is from its arrival to the completion of its timestep (HIP event on
+its stream).
 
 Each timestep runs the catchment's recorded network on its own buffers, so a result does not depend on
 which other requests share the GPU (tests check that it equals the timestep run alone).
@@ -31,11 +34,18 @@ class Catchment:
     name: str
     ldd: fw.PartitionedArray
     alpha: fw.PartitionedArray
-    state: fw.PartitionedArray        # current discharge, replaced by every timestep
+    states: list                      # two discharge buffers: states[current] holds the current discharge
+    lateral: fw.PartitionedArray      # lateral inflow of the next timestep
     network: fw.FlowNetwork
+    plans: list                       # plans[i]: timestep states[i] -> states[1 - i]
     resident_bytes: int
+    current: int = 0
     busy: bool = False
     steps: int = 0
+
+    @property
+    def state(self):
+        return self.states[self.current]
 
 
 @dataclasses.dataclass(order=True)
@@ -44,14 +54,12 @@ class Request:
     seq: int
     catchment: str = dataclasses.field(compare=False)
     seed: int = dataclasses.field(compare=False)
-    lateral: fw.PartitionedArray | None = dataclasses.field(default=None, compare=False)
     # filled when it runs
     admitted: float | None = dataclasses.field(default=None, compare=False)
     done_event: torch.cuda.Event | None = dataclasses.field(default=None, compare=False)
     start_event: torch.cuda.Event | None = dataclasses.field(default=None, compare=False)
     result: fw.PartitionedArray | None = dataclasses.field(default=None, compare=False)
     latency: float | None = dataclasses.field(default=None, compare=False)
-    transient_bytes: int = dataclasses.field(default=0, compare=False)
 
 
 class AdmissionError(RuntimeError):
@@ -61,6 +69,10 @@ class AdmissionError(RuntimeError):
 class Server:
     """Resident catchments and the request loop. beta, dt and dx are the scalars of every timestep.
     keep_results: keep every request's result (True), none (False), or those of the named catchments."""
+
+    # resident bytes per cell: ldd 1, alpha 4, two states 8, lateral inflow 4, recorded schedule (at most 24),
+    # the two plans' per-call buffers (at most 2 x 17)
+    RESIDENT_BYTES_PER_CELL = 1 + 4 + 8 + 4 + 24 + 34
 
     def __init__(self, hbm_budget_bytes, slots=16, beta=0.6, dt=3600.0, dx=100.0, tile_shape=(32, 32),
                  device=None, keep_results=False):
@@ -73,52 +85,49 @@ class Server:
         self.free_slots = list(range(slots))
         self.catchments: dict[str, Catchment] = {}
         self.resident_bytes = 0
-        self.transient_bytes = 0
         self.completed: list[Request] = []
         self.cells_done = 0
         self._warm = False
 
     # -- resident catchments --------------------------------------------------------------------------
     def add_catchment(self, name, ldd, alpha, initial_state):
-        """Make a catchment resident: record its network. Raises AdmissionError beyond the HBM budget."""
+        """Make a catchment resident: record its network and capture its timestep. Raises AdmissionError
+        beyond the HBM budget."""
         rows, cols = ldd.shape
-        estimate = rows * cols * (1 + 4 + 4 + 24)  # ldd, alpha, state, recorded schedule (upper estimate)
+        estimate = rows * cols * self.RESIDENT_BYTES_PER_CELL
         if self.resident_bytes + estimate > self.budget:
             raise AdmissionError(f"catchment {name}: {estimate} B exceed the HBM budget")
         network = fw.FlowNetwork(ldd, tile_shape=self.tile_shape)
-        self.catchments[name] = Catchment(name, ldd, alpha, initial_state, network, estimate)
+        states = [fw.PartitionedArray(initial_state.data.clone(), initial_state.partition_shape),
+                  fw.PartitionedArray(torch.zeros_like(initial_state.data), initial_state.partition_shape)]
+        lateral = fw.PartitionedArray(torch.zeros_like(initial_state.data), initial_state.partition_shape)
+        plans = [network.plan_kinematic_wave(states[i], lateral, alpha, self.beta, self.dt, self.dx, out=states[1 - i])
+                 for i in range(2)]
+        self.catchments[name] = Catchment(name, ldd, alpha, states, lateral, network, plans, estimate)
         self.resident_bytes += estimate
         self._warm = False
         return self.catchments[name]
 
     def remove_catchment(self, name):
-        c = self.catchments.pop(name)
+        c = self.catchments[name]
         if c.busy:
             raise RuntimeError(f"catchment {name} has a request in flight")
+        del self.catchments[name]
+        for plan in c.plans:
+            plan.close()
         c.network.close()
         self.resident_bytes -= c.resident_bytes
 
     def warm_up(self):
-        """Run one discarded timestep of the largest resident catchment on every slot at once (as many slots
-        as the HBM budget admits), so that the memory pools behind the slots' streams hold a request's
-        transient buffers before the first request (growing a pool inside the request loop stalls every slot
-        for milliseconds). States are unchanged."""
-        if not self.catchments:
-            return
-        c = max(self.catchments.values(), key=lambda x: x.ldd.shape[0] * x.ldd.shape[1])
-        need = c.ldd.shape[0] * c.ldd.shape[1] * (4 + 4 + 24)  # as a request: lateral inflow, output, scratch
-        slots = min(len(self.streams), max(0, self.budget - self.resident_bytes) // need)
-        if slots == 0:
-            return
+        """Launch every catchment's current timestep graph once on every slot's stream (the first launch of a
+        graph on a stream carries one-time costs) into the catchment's next state buffer, which the next
+        request overwrites: states are unchanged."""
         torch.cuda.synchronize(self.device)
-        outs = []
-        for stream in self.streams[:slots]:
+        for stream in self.streams:
             with torch.cuda.stream(stream):
-                lateral = self.lateral_inflow(0, c.ldd.shape, self.device)
-                outs.append((lateral, c.network.kinematic_wave(c.state, lateral, c.alpha, self.beta, self.dt,
-                                                               self.dx)))
-        torch.cuda.synchronize(self.device)
-        del outs
+                for c in self.catchments.values():
+                    c.plans[c.current].run()
+            torch.cuda.synchronize(self.device)
         self._warm = True
 
     # -- one timestep ------------------------------------------------------------------------------------
@@ -130,18 +139,17 @@ class Server:
     def _launch(self, request, slot):
         c = self.catchments[request.catchment]
         stream = self.streams[slot]
+        keep = self.keep_results is True or (self.keep_results and request.catchment in self.keep_results)
         with torch.cuda.stream(stream):
             request.start_event = torch.cuda.Event(enable_timing=False)
             request.start_event.record(stream)
-            if request.lateral is None:
-                request.lateral = self.lateral_inflow(request.seed, c.ldd.shape, self.device)
-            out = c.network.kinematic_wave(c.state, request.lateral, c.alpha, self.beta, self.dt, self.dx)
+            syn.uniform_into(request.seed, syn.STREAM_LATERAL, 0.0, 1e-4, c.lateral.data)
+            c.plans[c.current].run()
+            c.current = 1 - c.current
+            if keep:
+                request.result = fw.PartitionedArray(c.state.data.clone(), c.state.partition_shape)
             request.done_event = torch.cuda.Event(enable_timing=False)
             request.done_event.record(stream)
-        # the state tensor is read on this stream; keep the old one alive until the step has run
-        request.previous_state = c.state
-        c.state = out
-        request.result = out
         c.busy = True
         c.steps += 1
 
@@ -149,15 +157,8 @@ class Server:
         c = self.catchments[request.catchment]
         c.busy = False
         request.latency = now - request.arrival
-        self.transient_bytes -= request.transient_bytes
         self.free_slots.append(slot)
         self.cells_done += c.ldd.shape[0] * c.ldd.shape[1]
-        # the timestep has run: its inputs can go (the result is the catchment's state now)
-        request.lateral = None
-        request.previous_state = None
-        keep = self.keep_results
-        if not (keep is True or (keep and request.catchment in keep)):
-            request.result = None
         self.completed.append(request)
 
     def run(self, requests, clock=time.perf_counter, poll_sleep=50e-6):
@@ -195,25 +196,22 @@ class Server:
                 else:
                     still.append((request, slot))
             in_flight = still
-            # admission: FIFO over the waiting requests whose catchment is idle, while slots and memory last
+            # dispatch: FIFO over the waiting requests whose catchment is idle, while slots last (a request
+            # needs no memory beyond its catchment's resident buffers)
             kept = collections.deque()
             while waiting:
                 request = waiting.popleft()
-                c = self.catchments[request.catchment]
-                need = c.ldd.shape[0] * c.ldd.shape[1] * (4 + 4 + 24)  # lateral inflow, output, scratch
-                if c.busy or not self.free_slots or self.resident_bytes + self.transient_bytes + need > self.budget:
+                c = self.catchments.get(request.catchment)
+                if c is None:
+                    raise AdmissionError(f"request for {request.catchment}: no such resident catchment")
+                if c.busy or not self.free_slots:
                     kept.append(request)
                     continue
-                request.transient_bytes = need
-                self.transient_bytes += need
                 request.admitted = clock() - t0
                 slot = self.free_slots.pop()
                 self._launch(request, slot)
                 in_flight.append((request, slot))
             waiting = kept
-            if not in_flight and waiting:
-                # every catchment is idle and every slot free: the request alone exceeds the budget
-                raise AdmissionError(f"request for {waiting[0].catchment} does not fit the HBM budget")
             if in_flight or (waiting and not pending):
                 time.sleep(poll_sleep)
             elif pending:

@@ -123,6 +123,14 @@ def uniform_device(seed, stream, lo, hi, shape, dtype=torch.float32, device=None
     return out
 
 
+def uniform_into(seed, stream, lo, hi, out):
+    """uniform_device into an existing contiguous float tensor (stream order)."""
+    lib = _lib.load()
+    fn = lib.lue_amd_synthetic_uniform_f32 if out.dtype == torch.float32 else lib.lue_amd_synthetic_uniform_f64
+    _lib.check(fn(seed, stream, lo, hi, out.numel(), out.data_ptr(), _stream()))
+    return out
+
+
 # ---- batches of catchments --------------------------------------------------------------------------
 
 def batch_shapes(seed, count, side_lo, side_hi, aspect_lo=0.5, aspect_hi=2.0):

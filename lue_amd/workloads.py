@@ -70,12 +70,24 @@ def batch_threshold(seed=20261205, count=512, side_lo=64, side_hi=2048, repeats=
     for _ in range(repeats):
         net.accu_threshold(mats, thrs, out=outs[0], remainder=outs[1])
     _sync()
+    eager_ms = (time.perf_counter() - t0) / repeats * 1e3
+    # the whole batch as one graph launch
+    plan = net.plan_accu_threshold(mats, thrs, out=outs[0], remainder=outs[1])
+    plan.run()
+    _sync()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        plan.run()
+    _sync()
     ms = (time.perf_counter() - t0) / repeats * 1e3
+    launches = plan.info()["launches"]
+    plan.close()
     info = net.info()
     net.close()
     return {"workload": f"accu_threshold (float) over {count} catchments, sides log-uniform {side_lo}..{side_hi}, "
                         "aspect U[0.5,2], material U[0,1), threshold U[0,0.5)",
             "catchments": count, "cells": cells, "ms": round(ms, 3), "mcells_per_s": round(cells / ms / 1e3, 1),
+            "graph": f"one graph launch of {launches} kernel launches", "eager_ms": round(eager_ms, 3),
             "tile_shape_ms": {f"{k[0]}x{k[1]}": round(v, 3) for k, v in table.items()},
             "tile_shape": info["tile_shape"], "tune_s": round(tune_s, 2), "levels": info["levels"],
             "pending_cells": info["pending_cells"]}

@@ -75,18 +75,20 @@ def test_results_are_the_same_alone_and_in_the_mix(fw):
 
 @pytest.mark.gpu
 def test_admission_by_hbm_budget(fw):
+    """Catchments are admitted while their resident bytes (ldd, alpha, two states, lateral inflow, schedule,
+    captured timesteps) fit the budget; requests then need no memory of their own."""
     from lue_amd import serving
 
     cats = _catchments(fw, 91, [(256, 256), (512, 512), (256, 300)])
-    one = 256 * 256 * (1 + 4 + 4 + 24)
+    one = 256 * 256 * serving.Server.RESIDENT_BYTES_PER_CELL
     server = serving.Server(one * 3, slots=2)
     server.add_catchment(*cats[0])
     with pytest.raises(serving.AdmissionError):
         server.add_catchment(*cats[1])  # 4x larger: beyond the budget
     server.add_catchment(*cats[2])
-    # transient bytes of a request (32 B/cell) do not fit next to both residents: requests are rejected
     with pytest.raises(serving.AdmissionError):
-        server.run(serving.poisson_requests(1, 1000.0, 0.005, ["c0"]))
+        server.run(serving.poisson_requests(1, 1000.0, 0.005, ["c9"]))  # not resident
     server.remove_catchment("c2")
+    assert server.resident_bytes == one
     done = server.run(serving.poisson_requests(2, 2000.0, 0.01, ["c0"]))
-    assert len(done) > 0 and server.transient_bytes == 0
+    assert len(done) > 0 and all(r.latency is not None for r in done)
