@@ -130,6 +130,22 @@ int lue_amd_event_query(void* event, int* complete);
 int lue_amd_inflow_count_u8(const uint8_t* ldd, int64_t rows, int64_t cols, uint8_t* inflow_count, void* stream);
 
 /*
+ * accu_info: the class LUE's accumulating router leaves in each cell when the array is split into
+ * partitions of partition_rows x partition_cols cells (floor(rows / partition_rows) partitions per column,
+ * the last one absorbing the remainder; the same along rows): 11 ridge, 12 intra-partition stream,
+ * 13 partition output, 14 partition input, 15 partition input and output, 16 inter-partition stream,
+ * 17 sink, 255 where the ldd is no-data. A partition input cell that also has an upstream neighbour in
+ * its own partition waiting for another partition gets 14/15; in LUE it can also get 16/13/17, depending on
+ * which arrives last. Runs accu (float) of a crossing indicator on the per-call network of the ldd.
+ * Replaces lue::value_policies::accu_info(flow_direction) (ALG/value_policies/accu_info.hpp:19-28,
+ * ALG/accu_info.hpp:22-78, ALG/definition/accu_info.hpp:10-188; Python binding
+ * source/framework/python/source/algorithm/routing_operation/accu_info.cpp:10-17).
+ * Asynchronous on `stream`.
+ */
+int lue_amd_accu_info_u8(const uint8_t* ldd, int64_t rows, int64_t cols, int64_t partition_rows,
+                         int64_t partition_cols, uint8_t* cell_class, void* stream);
+
+/*
  * d8_flow_direction: the ldd of an elevation raster (NaN = no-data -> 255). Each cell drains to the
  * neighbour with the largest raw drop (no distance weighting; first in the order N, W, E, S, NW, NE, SW,
  * SE on ties; sink when no drop is positive); outside the array the elevation is numeric_limits<T>::min().

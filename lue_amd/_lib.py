@@ -67,6 +67,7 @@ SIGNATURES = {
     "lue_amd_event_synchronize": [c_p],
     "lue_amd_event_query": [c_p, ctypes.POINTER(c_int)],
     "lue_amd_inflow_count_u8": [c_p, c_i64, c_i64, c_p, c_p],
+    "lue_amd_accu_info_u8": [c_p, c_i64, c_i64, c_i64, c_i64, c_p, c_p],
     "lue_amd_d8_flow_direction_f32": [c_p, c_i64, c_i64, c_p, c_p],
     "lue_amd_d8_flow_direction_f64": [c_p, c_i64, c_i64, c_p, c_p],
     "lue_amd_accu_f32": [c_p, c_i64, c_i64, c_p, c_f, c_p, c_p],

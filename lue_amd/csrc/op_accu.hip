@@ -1,5 +1,6 @@
-// C ABI of accu (per call and on flow networks) and of flow-network creation, which records a network's
-// schedule with accu(ldd, 1).
+// C ABI of accu (per call and on flow networks), of accu_info (which runs accu of a crossing indicator) and
+// of flow-network creation, which records a network's schedule with accu(ldd, 1).
+#include "accu_info.hpp"
 #include "api.hpp"
 
 using namespace lue_amd;
@@ -10,6 +11,33 @@ int lue_amd_accu_f32(const uint8_t* ldd, int64_t rows, int64_t cols, const float
                      float* outflow, void* stream)
 {
     return accu_impl<float>(ldd, rows, cols, inflow, inflow_scalar, outflow, stream);
+}
+
+int lue_amd_accu_info_u8(const uint8_t* ldd, int64_t rows, int64_t cols, int64_t partition_rows,
+                         int64_t partition_cols, uint8_t* cell_class, void* stream)
+{
+    if (int const st = check_shape(ldd, rows, cols))
+        return st;
+    if (!cell_class)
+        return fail(status_invalid_argument, "cell_class must not be null");
+    if (partition_rows <= 0 || partition_cols <= 0 || partition_rows > rows || partition_cols > cols)
+        return fail(status_invalid_argument, "partition shape must be positive and fit in the array shape");
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    PartitionGrid const g{partition_rows, partition_cols, rows / partition_rows, cols / partition_cols};
+    std::size_t const n = static_cast<std::size_t>(rows * cols);
+    ScratchArray<float> crossing;
+    ScratchArray<float> crossing_accu;
+    LUE_AMD_HIP(crossing.allocate(n, s));
+    LUE_AMD_HIP(crossing_accu.allocate(n, s));
+    dim3 const grid(static_cast<unsigned>((cols + ai_tw - 1) / ai_tw), static_cast<unsigned>((rows + ai_th - 1) / ai_th));
+    hipLaunchKernelGGL(accu_info_crossing_kernel, grid, dim3(256), 0, s, ldd, rows, cols, g, crossing.ptr);
+    LUE_AMD_HIP(hipGetLastError());
+    if (int const st = accu_impl<float>(ldd, rows, cols, crossing.ptr, 0.0f, crossing_accu.ptr, stream))
+        return st;
+    hipLaunchKernelGGL(accu_info_class_kernel, grid, dim3(256), 0, s, ldd, rows, cols, g, crossing_accu.ptr,
+                       cell_class);
+    LUE_AMD_HIP(hipGetLastError());
+    return status_success;
 }
 
 int lue_amd_accu_f64(const uint8_t* ldd, int64_t rows, int64_t cols, const double* inflow, double inflow_scalar,

@@ -1,12 +1,12 @@
 """Python front end of the routing path, mirroring lue.framework's routing operations.
 
 Reference bindings: source/framework/python/source/algorithm/routing_operation/{accu,accu_threshold,
-accu_fraction,accu_capacity,accu_trigger,kinematic_wave,inflow_count,d8_flow_direction,upstream,
+accu_fraction,accu_capacity,accu_trigger,accu_info,kinematic_wave,inflow_count,d8_flow_direction,upstream,
 downstream}.cpp, which call lue::value_policies::* on PartitionedArray.
 
 PartitionedArray here is device resident: one contiguous row-major raster in HBM (a torch tensor used as
-device memory) plus LUE's partition shape, which is kept as metadata (results do not depend on it). A
-Scalar argument is a Python number or numpy scalar (LUE's Scalar<E> and plain-value overloads).
+device memory) plus LUE's partition shape, which is kept as metadata: only accu_info's classes depend on
+it. A Scalar argument is a Python number or numpy scalar (LUE's Scalar<E> and plain-value overloads).
 
 Operations are asynchronous on torch's current stream, like LUE's operations, which return future-backed
 partitions at once (partitioned_array_decl.hpp:23-98): they enqueue their kernels and return; reading a
@@ -169,6 +169,33 @@ def inflow_count(flow_direction):
     out = _output(flow_direction, torch.uint8)
     rows, cols = flow_direction.shape
     _lib.check(lib.lue_amd_inflow_count_u8(_ptr(flow_direction.data), rows, cols, _ptr(out.data), _stream()))
+    return out
+
+
+# Cell classes of accu_info (ALG/accu_info.hpp:27-63); no-data is 255.
+ridge_cell = 11
+intra_partition_stream_cell = 12
+partition_output_cell = 13
+partition_input_cell = 14
+partition_input_output_cell = 15
+inter_partition_stream_cell = 16
+sink_cell = 17
+confluence_cell = 18
+
+
+def accu_info(flow_direction):
+    """lue.framework.accu_info(flow_direction) (routing_operation/accu_info.cpp:10-17,
+    ALG/value_policies/accu_info.hpp:19-28): the class the accumulating router leaves in each cell for the
+    array's partition shape, uint8. A partition input cell that also has an upstream neighbour in its own
+    partition waiting for another partition is a partition input cell here; in LUE it can also end up as
+    the class of a cell without inflow from other partitions, depending on which arrives last."""
+    _check_ldd(flow_direction)
+    lib = _lib.load()
+    out = _output(flow_direction, torch.uint8)
+    rows, cols = flow_direction.shape
+    prows, pcols = flow_direction.partition_shape
+    _lib.check(lib.lue_amd_accu_info_u8(_ptr(flow_direction.data), rows, cols, prows, pcols, _ptr(out.data),
+                                        _stream()))
     return out
 
 

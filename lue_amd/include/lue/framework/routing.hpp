@@ -9,7 +9,8 @@
 //   - lue::Scalar<E>  (source/framework/algorithm/include/lue/framework/algorithm/scalar.hpp:20);
 //   - lue::create_partitioned_array(shape, partition_shape, fill_value);
 //   - lue::value_policies::{accu, accu_threshold, accu_fraction, accu_capacity, accu_trigger, kinematic_wave,
-//     d8_flow_direction, upstream, downstream} with the reference's overload sets (ALG/value_policies/*.hpp) and
+//     d8_flow_direction, upstream, downstream, accu_info} with the reference's overload sets
+//     (ALG/value_policies/*.hpp), the cell classes of accu_info (ALG/accu_info.hpp:22-63) and
 //     lue::inflow_count<std::uint8_t>(ldd) and lue::downstream_distance(ldd, cell_size)
 //     (ALG/inflow_count.hpp:29-42), ALG = source/framework/algorithm/include/lue/framework/algorithm.
 // Each operation calls the C ABI (include/lue_amd/routing.h). Shape and partitioning mismatches throw
@@ -451,6 +452,36 @@ namespace lue {
         result.written_on(s);
         return result;
     }
+
+    // Cell classes of accu_info (ALG/accu_info.hpp:22-63)
+    using CellClass = std::uint8_t;
+    inline constexpr CellClass ridge_cell = 11;
+    inline constexpr CellClass intra_partition_stream_cell = 12;
+    inline constexpr CellClass partition_output_cell = 13;
+    inline constexpr CellClass partition_input_cell = 14;
+    inline constexpr CellClass partition_input_output_cell = 15;
+    inline constexpr CellClass inter_partition_stream_cell = 16;
+    inline constexpr CellClass sink_cell = 17;
+    inline constexpr CellClass confluence_cell = 18;
+
+    namespace value_policies {
+
+        // lue::value_policies::accu_info(flow_direction) (ALG/value_policies/accu_info.hpp:19-28): the
+        // class the accumulating router leaves in each cell for the array's partition shape
+        inline auto accu_info(PartitionedArray<FlowDirectionElement, 2> const& flow_direction)
+            -> PartitionedArray<CellClass, 2>
+        {
+            auto result = detail::output_like<CellClass>(flow_direction);
+            void* const s = lue::stream();
+            flow_direction.order_before(s);
+            detail::check(lue_amd_accu_info_u8(flow_direction.data(), flow_direction.shape()[0],
+                                               flow_direction.shape()[1], flow_direction.partition_shape()[0],
+                                               flow_direction.partition_shape()[1], result.mutable_data(), s));
+            result.written_on(s);
+            return result;
+        }
+
+    }  // namespace value_policies
 
     namespace detail {
 

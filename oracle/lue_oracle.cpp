@@ -10,9 +10,10 @@
 //   - inflow count (intra + cross-partition) ...... detail/inflow_count.hpp:9-352, definition/inflow_count.hpp:527-577
 //   - three-phase router ........................... definition/accumulating_router.hpp:525-599 (walk),
 //                                                    :866-882 (row-major ridge scan), :987-1010 (inter-partition)
-//   - cell functors ................................ definition/accu.hpp:74-153, definition/accu_threshold.hpp:124-230,
+//   - cell functors ................................ definition/accu.hpp:62-170, definition/accu_threshold.hpp:124-230,
 //                                                    definition/accu_fraction.hpp:126-190, definition/kinematic_wave.hpp:416-515,
-//                                                    definition/accu_capacity.hpp:124-166, definition/accu_trigger.hpp:124-164
+//                                                    definition/accu_capacity.hpp:124-166, definition/accu_trigger.hpp:124-164,
+//                                                    definition/accu_info.hpp:53-145 (cell classes, accu_info.hpp:27-63)
 //   - value-policy domains ......................... value_policies/{accu,accu_threshold,accu_fraction,kinematic_wave,
 //                                                    accu_capacity,accu_trigger}.hpp
 //   - kinematic-wave Newton solve .................. definition/kinematic_wave.hpp:13-254
@@ -384,6 +385,20 @@ namespace {
             }
         }
 
+        // Router events with empty bodies in the material functors (accu.hpp:62-72,96-99,156-170)
+        void enter_intra_partition_stream(Index) {}
+        void leave_intra_partition_stream(Index) {}
+        void leave_inter_partition_stream(Index) {}
+        void stop_at_sink_cell(Index) {}
+        void stop_at_confluence_cell(Index) {}
+        void stop_at_partition_output_cell(Index) {}
+
+        auto outflow(Index i) const -> T { return out[i]; }
+
+        void leave_cell(Index from, Index to) { add_upstream(out[from], to); }
+
+        void enter_inter_partition_stream(T upstream_outflow, Index to) { add_upstream(upstream_outflow, to); }
+
         // leave_cell / enter_inter_partition_stream (accu.hpp:74-93,126-153)
         void add_upstream(T upstream_outflow, Index to)
         {
@@ -523,6 +538,46 @@ namespace {
         }
     };
 
+    // Cell classes of accu_info (accu_info.hpp:27-63); the no-data value is the uint8 maximum.
+    constexpr std::uint8_t ridge_cell = 11;
+    constexpr std::uint8_t intra_partition_stream_cell = 12;
+    constexpr std::uint8_t partition_output_cell = 13;
+    constexpr std::uint8_t partition_input_cell = 14;
+    constexpr std::uint8_t partition_input_output_cell = 15;
+    constexpr std::uint8_t inter_partition_stream_cell = 16;
+    constexpr std::uint8_t sink_cell = 17;
+    constexpr std::uint8_t confluence_cell = 18;
+    constexpr std::uint8_t no_data_class = 255;
+
+    // accu_info's functor (definition/accu_info.hpp:53-145): each router event writes a class, leave_cell
+    // passes the class of the cell left to the cell entered.
+    struct CellInfo
+    {
+        std::uint8_t* out;
+        std::uint8_t* rem;  // unused (one output)
+
+        void mark_no_data(Index i) { out[i] = no_data_class; }
+        void enter_intra_partition_stream(Index i) { out[i] = intra_partition_stream_cell; }
+        void leave_intra_partition_stream(Index i) { out[i] = ridge_cell; }
+        void enter_inter_partition_stream(std::uint8_t, Index i) { out[i] = inter_partition_stream_cell; }
+        void leave_inter_partition_stream(Index i)
+        {
+            out[i] = out[i] != partition_output_cell ? partition_input_cell : partition_input_output_cell;
+        }
+        void enter_cell(Index) {}
+        void leave_cell(Index from, Index to)
+        {
+            if (out[to] != no_data_class)
+            {
+                out[to] = out[from] == no_data_class ? no_data_class : out[from];
+            }
+        }
+        void stop_at_sink_cell(Index i) { out[i] = sink_cell; }
+        void stop_at_confluence_cell(Index i) { out[i] = confluence_cell; }
+        void stop_at_partition_output_cell(Index i) { out[i] = partition_output_cell; }
+        auto outflow(Index i) const -> std::uint8_t { return out[i]; }
+    };
+
     struct Message
     {
         Index dest;
@@ -562,17 +617,20 @@ namespace {
                 ldd_offset(ldd[idx], d0, d1);
                 if (d0 == 0 && d1 == 0)
                 {
-                    break;  // sink
+                    cells.stop_at_sink_cell(idx);
+                    break;
                 }
                 Index const r2 = r + d0, c2 = c + d1;
                 bool const in_array = r2 >= 0 && r2 < g.rows && c2 >= 0 && c2 < g.cols;
                 bool const in_part = r2 >= rb && r2 < re && c2 >= cb && c2 < ce;
                 if (!in_part)
                 {
-                    // Partition output cell: send to the neighbour, or drop at the array edge.
+                    // Partition output cell (also at the array edge): send to the neighbour, or drop at
+                    // the array edge (accumulating_router.hpp:467-510).
+                    cells.stop_at_partition_output_cell(idx);
                     if (in_array && ldd[r2 * g.cols + c2] != no_data_ldd)
                     {
-                        sent.push_back(Message{r2 * g.cols + c2, static_cast<double>(cells.out[idx])});
+                        sent.push_back(Message{r2 * g.cols + c2, static_cast<double>(cells.outflow(idx))});
                     }
                     break;
                 }
@@ -581,10 +639,11 @@ namespace {
                 {
                     break;  // flow into no-data leaves the network
                 }
-                cells.add_upstream(cells.out[idx], to);
+                cells.leave_cell(idx, to);
                 if (--count[to] > 0)
                 {
-                    break;  // confluence
+                    cells.stop_at_confluence_cell(to);
+                    break;
                 }
                 idx = to;
             }
@@ -608,7 +667,9 @@ namespace {
                         }
                         else if (count0[idx] == 0)
                         {
+                            cells.enter_intra_partition_stream(idx);
                             walk(p, idx, outbox[static_cast<std::size_t>(p)]);
+                            cells.leave_intra_partition_stream(idx);
                         }
                     }
                 }
@@ -644,10 +705,11 @@ namespace {
                 {
                     for (Message const& m : inbox[static_cast<std::size_t>(p)])
                     {
-                        cells.add_upstream(static_cast<T>(m.value_f64), m.dest);
+                        cells.enter_inter_partition_stream(static_cast<T>(m.value_f64), m.dest);
                         if (--count[m.dest] == 0)
                         {
                             walk(p, m.dest, outbox[static_cast<std::size_t>(p)]);
+                            cells.leave_inter_partition_stream(m.dest);
                         }
                     }
                 });
@@ -967,6 +1029,20 @@ int oracle_inflow_count(
 
 ORACLE_ROUTING_OPS(float, f32)
 ORACLE_ROUTING_OPS(double, f64)
+
+// Cell classes of accu_info (uint8): the router's events replayed with accu_info's functor
+// (definition/accu_info.hpp:53-145, value policies: a no-data ldd gives class 255).
+int oracle_accu_info(std::uint8_t const* ldd, Index rows, Index cols, Index prows, Index pcols,
+                     std::uint8_t* cell_class, int nthreads)
+{
+    Grid g;
+    if (!make_grid(rows, cols, prows, pcols, g) || !ldd || !cell_class)
+    {
+        return err_invalid_argument;
+    }
+    CellInfo cells{cell_class, nullptr};
+    return route<std::uint8_t>(g, ldd, cells, nthreads);
+}
 
 int oracle_d8_flow_direction_f32(float const* elevation, Index rows, Index cols, std::uint8_t* out)
 {

@@ -52,6 +52,7 @@ def _load():
         f.argtypes = [ct] * 7
         f.restype = ct
     lib.oracle_inflow_count.argtypes = [p, i64, i64, i64, i64, p, ci]
+    lib.oracle_accu_info.argtypes = [p, i64, i64, i64, i64, p, ci]
     for suffix in ("f32", "f64"):
         getattr(lib, f"oracle_d8_flow_direction_{suffix}").argtypes = [p, i64, i64, p]
     lib.oracle_downstream_distance_f32.argtypes = [p, i64, i64, ctypes.c_float, ci, p]
@@ -111,6 +112,16 @@ def inflow_count(ldd, partition_shape=None, nthreads=None):
     out = np.empty((rows, cols), dtype=np.uint8)
     _check(lib.oracle_inflow_count(ldd.ctypes.data, rows, cols, prows, pcols, out.ctypes.data,
                                    nthreads or default_threads()))
+    return out
+
+
+def accu_info(ldd, partition_shape=None, nthreads=None):
+    """Cell classes of accu_info (uint8, 11..18, 255 = no-data) for the given partition shape."""
+    lib = _load()
+    ldd, rows, cols, prows, pcols = _prep(ldd, partition_shape)
+    out = np.empty((rows, cols), dtype=np.uint8)
+    _check(lib.oracle_accu_info(ldd.ctypes.data, rows, cols, prows, pcols, out.ctypes.data,
+                                nthreads or default_threads()))
     return out
 
 

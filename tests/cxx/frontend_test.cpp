@@ -166,6 +166,11 @@ namespace {
             auto const count = inflow_count<std::uint8_t>(ldd);
             return compare(c, count, c.expected[0]);
         }
+        if (kind == "accu_info")
+        {
+            PartitionedArray<CellClass, 2> const cell_class = value_policies::accu_info(ldd);
+            return compare(c, cell_class, c.expected[0]);
+        }
         auto raster = [&](std::size_t k) { return make_raster<E>(c, c.args[k].values); };
         auto scalar = [&](std::size_t k) { return Scalar<E>{static_cast<E>(c.args[k].scalar)}; };
         if (kind == "accu")
@@ -317,7 +322,7 @@ int main()
         }
         for (int precision = 0; precision < 2; ++precision)
         {
-            if (std::string(c.kind) == "inflow_count" && precision == 1)
+            if ((kind == "inflow_count" || kind == "accu_info") && precision == 1)
                 continue;
             bool ok;
             try

@@ -22,6 +22,7 @@ Files read (relative to the reference root):
     source/framework/algorithm/test/upstream_test.cpp
     source/framework/algorithm/test/downstream_test.cpp
     source/framework/algorithm/test/downstream_distance_test.cpp
+    source/framework/algorithm/test/accu_info_test.cpp
 
 A partitioned literal lists one initializer per partition, partitions in row-major partition order
 (source/framework/test/include/lue/framework/test/array.hpp:76-105), each partition's cells row-major,
@@ -40,6 +41,10 @@ LDD_NAMES = {"n": 8, "ne": 9, "e": 6, "se": 3, "s": 2, "sw": 1, "w": 4, "nw": 7,
 # lue::north<FlowDirection> ... (source/framework/algorithm/include/lue/framework/algorithm/flow_direction.hpp)
 DIRECTION_NAMES = {"north": 8, "north_east": 9, "east": 6, "south_east": 3, "south": 2, "south_west": 1, "west": 4,
                    "north_west": 7, "sink": 5}
+# lue::ridge_cell ... (source/framework/algorithm/include/lue/framework/algorithm/accu_info.hpp:27-63)
+CELL_CLASS_NAMES = {"ridge_cell": 11, "intra_partition_stream_cell": 12, "partition_output_cell": 13,
+                    "partition_input_cell": 14, "partition_input_output_cell": 15,
+                    "inter_partition_stream_cell": 16, "sink_cell": 17, "confluence_cell": 18}
 NAN = float("nan")
 
 TOKEN_RE = re.compile(r"""
@@ -188,7 +193,7 @@ class Parser:
             name = name + "::" + self.take()[1]
         short = name.split("::")[-1]
         if short == "no_data_value":
-            return 255.0 if ("FlowDirection" in targs or "Count" in targs) else NAN
+            return 255.0 if ("FlowDirection" in targs or "Count" in targs or "CellClass" in targs) else NAN
         if short == "min" and "numeric_limits" in name:
             self.call_args()
             return sys.float_info.min
@@ -209,6 +214,8 @@ class Parser:
             raise SyntaxError(member)
         if short in self.env:
             return self.env[short]
+        if name.startswith("lue::") and short in CELL_CLASS_NAMES:
+            return float(CELL_CLASS_NAMES[short])
         if name.startswith("lue::") and short in DIRECTION_NAMES:
             return float(DIRECTION_NAMES[short])
         if short in LDD_NAMES:
@@ -262,7 +269,8 @@ def split_statements(toks):
 
 
 DECL_RE = re.compile(r"^(?:auto const(?: &)?|auto|lue::Scalar < \w+ > const|Shape const|Shape|FloatingPoint const|"
-                     r"lue::ShapeT < [\w:]+ > const|InflowCount|FlowDirection)$")
+                     r"lue::ShapeT < [\w:]+ > const|InflowCount|FlowDirection|"
+                     r"lue::PartitionedArray < lue::CellClass , 2 >(?: const)?)$")
 
 
 def run_statements(stmts, env, record):
@@ -594,6 +602,31 @@ def extract(ref_root):
                 })
     del text_toks
     out["inflow_count"] = cases
+
+    # accu_info: the ldd fixture and the expected cell classes of each case
+    fname = "accu_info_test.cpp"
+    cases = []
+    for name, body in case_bodies(os.path.join(test_dir, fname)):
+        if name == "overloads":
+            continue
+        env = {"array_shape": [9, 9], "partition_shape": [3, 3]}
+        record = run_statements(split_statements(body), env, [])
+        checks = [v for k, v in record if k == "equal"]
+        assert len(checks) == 1, name
+        got_name, want_name = checks[0]
+        _, opname, args = env[got_name]
+        assert opname == "accu_info"
+        ldd = args[0]
+        cases.append({
+            "name": name,
+            "source": f"source/framework/algorithm/test/{fname}",
+            "kind": "accu_info",
+            "array_shape": list(ldd.array_shape),
+            "partition_shape": list(ldd.partition_shape),
+            "ldd": to_json_raster(ldd),
+            "expected": {"cell_class": {"raster": to_json_raster(env[want_name]), "compare": "equal"}},
+        })
+    out["accu_info"] = cases
     return out
 
 

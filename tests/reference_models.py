@@ -269,3 +269,89 @@ def downstream_by_pulling(ldd, material):
         out = np.where(ldd == code, target, out)
     out[_is_no_data(out) | (ldd == ND) | _is_no_data(material)] = material_no_data(material.dtype)
     return out
+
+
+def partition_index(extent, part):
+    """Partition of each row (or column): extent // part partitions, the last one absorbs the remainder
+    (source/framework/core/include/lue/framework/core/domain_decomposition.hpp:91-183)."""
+    return np.minimum(np.arange(extent) // part, extent // part - 1)
+
+
+def accu_info_by_rule(ldd, partition_shape):
+    """accu_info's cell classes from each cell's place in the network and in the partition grid, without
+    replaying the router's events (definition/accu_info.hpp names the classes):
+
+      - no upstream neighbour: ridge (11);
+      - an upstream neighbour in another partition: partition input (14), or partition input and output
+        (15) when the downstream cell is outside the partition or the array;
+      - otherwise sink (17), partition output (13), or a stream cell: inter-partition (16) when an edge that
+        crosses partitions lies upstream, else intra-partition (12).
+
+    A partition input cell that also has an upstream neighbour in its own partition with a crossing edge
+    upstream of it gets 14/15 when the message arrives after that neighbour's walk and 17/13/16 (the rule for
+    a cell without upstream neighbours in other partitions) otherwise; LUE reads the channels concurrently, so
+    either can happen. Returns (classes, alternative), alternative = that second class for such cells, 0
+    elsewhere. Pure Python loops: small rasters only.
+    """
+    ldd = np.asarray(ldd)
+    rows, cols = ldd.shape
+    prow = partition_index(rows, partition_shape[0])
+    pcol = partition_index(cols, partition_shape[1])
+
+    def part(r, c):
+        return prow[r], pcol[c]
+
+    def target(r, c):
+        """(r2, c2, in_array) of the downstream cell, None for a sink or an unknown code."""
+        code = int(ldd[r, c])
+        if code not in OFFSETS:
+            return None
+        dr, dc = OFFSETS[code]
+        r2, c2 = r + dr, c + dc
+        return r2, c2, 0 <= r2 < rows and 0 <= c2 < cols
+
+    # pending: an edge that crosses partitions lies upstream (mark from each crossing edge down to the outlet)
+    pending = np.zeros((rows, cols), dtype=bool)
+    upstream = [[[] for _ in range(cols)] for _ in range(rows)]
+    for r in range(rows):
+        for c in range(cols):
+            if ldd[r, c] == ND:
+                continue
+            t = target(r, c)
+            if t is None or not t[2] or ldd[t[0], t[1]] == ND:
+                continue
+            r2, c2 = t[0], t[1]
+            upstream[r2][c2].append((r, c))
+            if part(r, c) != part(r2, c2):
+                cell = (r2, c2)
+                while cell is not None and not pending[cell]:
+                    pending[cell] = True
+                    t2 = target(*cell)
+                    cell = (t2[0], t2[1]) if (t2 is not None and t2[2] and ldd[t2[0], t2[1]] != ND) else None
+
+    classes = np.full((rows, cols), 255, dtype=np.uint8)
+    alternative = np.zeros((rows, cols), dtype=np.uint8)
+    for r in range(rows):
+        for c in range(cols):
+            if ldd[r, c] == ND:
+                continue
+            t = target(r, c)
+            sink = t is None
+            out = not sink and (not t[2] or part(t[0], t[1]) != part(r, c))
+            if sink:
+                plain = 17
+            elif out:
+                plain = 13
+            else:
+                plain = 16 if pending[r, c] else 12
+            ups = upstream[r][c]
+            ext = [u for u in ups if part(*u) != part(r, c)]
+            if not ups:
+                classes[r, c] = 11
+            elif ext:
+                classes[r, c] = 15 if out else 14
+                if any(pending[u] for u in ups if part(*u) == part(r, c)):
+                    alternative[r, c] = plain
+            else:
+                classes[r, c] = plain
+    return classes, alternative

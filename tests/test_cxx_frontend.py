@@ -58,6 +58,12 @@ def write_cases(path):
             f'Case{{"{c["name"]}", "downstream_distance", {{{rows}, {cols}}}, {{{prows}, {pcols}}}, '
             f'{_vec(c["ldd"])}, {{Arg{{false, {_num(c["cell_size"])}, {{}}}}}}, '
             f'{{{_vec(c["expected"]["distance"]["raster"])}}}, false, false}},')
+    for c in load_golden("accu_info"):
+        rows, cols = c["array_shape"]
+        prows, pcols = c["partition_shape"]
+        lines.append(
+            f'Case{{"{c["name"]}", "accu_info", {{{rows}, {cols}}}, {{{prows}, {pcols}}}, {_vec(c["ldd"])}, {{}}, '
+            f'{{{_vec(c["expected"]["cell_class"]["raster"])}}}, false, false}},')
     for c in load_golden("d8_flow_direction"):
         rows, cols = c["array_shape"]
         prows, pcols = c["partition_shape"]

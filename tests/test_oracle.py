@@ -309,3 +309,89 @@ def test_downstream_distance_geometry(oracle_mod, dtype):
             assert (got[sel] == 0).all()
     plain = oracle_mod.downstream_distance(ldd, cell_size, dtype, False)
     assert (plain[ldd == 255] == 0).all()
+
+
+# ---------------------------------------------------------------------------------------------------
+# accu_info: LUE's golden classes, a closed-form model of the classes (reference_models.accu_info_by_rule)
+# and what a single partition reduces the classes to
+
+@pytest.mark.parametrize("case", load_golden("accu_info"), ids=lambda c: c["name"])
+def test_accu_info_golden(oracle_mod, case):
+    ldd = raster(case["ldd"], np.uint8)
+    got = oracle_mod.accu_info(ldd, case["partition_shape"])
+    np.testing.assert_array_equal(got, raster(case["expected"]["cell_class"]["raster"], np.uint8))
+
+
+def test_accu_info_golden_matches_the_rule():
+    for case in load_golden("accu_info"):
+        ldd = raster(case["ldd"], np.uint8)
+        classes, alternative = rm.accu_info_by_rule(ldd, case["partition_shape"])
+        np.testing.assert_array_equal(classes, raster(case["expected"]["cell_class"]["raster"], np.uint8))
+        assert not alternative.any()
+
+
+def _accu_info_cases(n, seed):
+    rng = np.random.default_rng(seed)
+    for _ in range(n):
+        rows, cols = (int(v) for v in rng.integers(1, 36, size=2))
+        ldd = rm.random_ldd(rng, rows, cols, nodata_fraction=float(rng.choice([0.0, 0.05])),
+                            pit_fraction=float(rng.choice([0.0, 0.03])), flat_fraction=0.3)
+        yield ldd, (int(rng.integers(1, rows + 1)), int(rng.integers(1, cols + 1)))
+
+
+def test_accu_info_against_the_rule(oracle_mod):
+    """The oracle replays the router's events; the rule derives each class from the cell's place. They agree
+    on every cell whose last arrival LUE fixes; elsewhere the oracle's class is one of the two LUE can give."""
+    ambiguous = 0
+    for ldd, ps in _accu_info_cases(150, 21):
+        got = oracle_mod.accu_info(ldd, ps)
+        classes, alternative = rm.accu_info_by_rule(ldd, ps)
+        fixed = alternative == 0
+        np.testing.assert_array_equal(got[fixed], classes[fixed], err_msg=f"partition shape {ps}")
+        free = ~fixed
+        assert ((got[free] == classes[free]) | (got[free] == alternative[free])).all()
+        ambiguous += int(free.sum())
+    assert ambiguous > 0  # the case LUE leaves open does occur in these networks
+
+
+def test_accu_info_single_partition(oracle_mod):
+    """With one partition nothing crosses partitions: ridges (no upstream neighbour), sinks, cells that
+    flow off the array (partition output) and intra-partition stream cells only."""
+    rng = np.random.default_rng(4)
+    ldd = rm.random_ldd(rng, 30, 23, nodata_fraction=0.03, pit_fraction=0.02, flat_fraction=0.3)
+    got = oracle_mod.accu_info(ldd, ldd.shape)
+    count = rm.inflow_count(ldd)
+    want = np.full(ldd.shape, 12, dtype=np.uint8)
+    rows, cols = ldd.shape
+    for r in range(rows):
+        for c in range(cols):
+            code = int(ldd[r, c])
+            if code == rm.ND:
+                want[r, c] = 255
+            elif count[r, c] == 0:
+                want[r, c] = 11
+            elif code not in rm.OFFSETS:
+                want[r, c] = 17
+            else:
+                dr, dc = rm.OFFSETS[code]
+                if not (0 <= r + dr < rows and 0 <= c + dc < cols):
+                    want[r, c] = 13
+    np.testing.assert_array_equal(got, want)
+
+
+def test_accu_info_planted_mistake_is_caught(oracle_mod):
+    """A rule that forgets the last-partition-absorbs-the-remainder grid differs from the oracle."""
+    rng = np.random.default_rng(8)
+    ldd = rm.random_ldd(rng, 23, 17, flat_fraction=0.3)
+    ps = (5, 4)  # 23 = 4 * 5 + 3, 17 = 4 * 4 + 1: last partitions absorb the remainder
+    saved = rm.partition_index
+    try:
+        rm.partition_index = lambda extent, part: np.arange(extent) // part
+        wrong, _ = rm.accu_info_by_rule(ldd, ps)
+    finally:
+        rm.partition_index = saved
+    right, alternative = rm.accu_info_by_rule(ldd, ps)
+    got = oracle_mod.accu_info(ldd, ps)
+    fixed = alternative == 0
+    np.testing.assert_array_equal(got[fixed], right[fixed])
+    assert not np.array_equal(wrong, right)
