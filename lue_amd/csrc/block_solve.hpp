@@ -66,17 +66,40 @@ namespace lue_amd {
         return __builtin_amdgcn_update_dpp(identity, v, 0x110 + N, 0xF, 0xF, false);
     }
 
-    // Lane shift by one across the wavefront (DPP wave_shr:1): lane j gets the value of lane j - 1.
+    // The same shift with 0 shifted in: bound_ctrl makes the lanes whose source lies before their row read 0,
+    // so no register has to hold the identity first.
+    template<int N>
+    __device__ inline double row_shr0(double v)
+    {
+        int const lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x110 + N, 0xF, 0xF, true);
+        int const hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x110 + N, 0xF, 0xF, true);
+        return __hiloint2double(hi, lo);
+    }
+
+    template<int N>
+    __device__ inline float row_shr0(float v)
+    {
+        return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x110 + N, 0xF, 0xF, true));
+    }
+
+    template<int N>
+    __device__ inline int row_shr0(int v)
+    {
+        return __builtin_amdgcn_mov_dpp(v, 0x110 + N, 0xF, 0xF, true);
+    }
+
+    // Lane shift by one across the wavefront (DPP wave_shr:1): lane j gets the value of lane j - 1, lane 0
+    // gets 0.
     __device__ inline double wave_shr1(double v)
     {
-        int const lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xF, 0xF, false);
-        int const hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xF, 0xF, false);
+        int const lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xF, 0xF, true);
+        int const hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xF, 0xF, true);
         return __hiloint2double(hi, lo);
     }
 
     __device__ inline float wave_shr1(float v)
     {
-        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+        return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
     }
 
     // DPP row broadcasts across rows of 16 lanes: Ctrl 0x142 (row_bcast:15) with RowMask 0xA gives rows 1 and
@@ -111,24 +134,27 @@ namespace lue_amd {
     // and rows 2 and 3 that of rows 0-1 (row_bcast:31); the values stay in vector registers. `flag` marks lanes whose element does not combine with the lanes before
     // them (chain starts); on return it is set where the result does not depend on the value before lane 0.
 
+    // The combine steps below select rather than branch: a lane whose flag is set keeps its element (the
+    // combined value is computed and dropped), so the scans compile to straight-line DPP, FP64 and
+    // v_cndmask instructions without exec-mask changes.
+
     // Affine maps: lane k holds x_k = add + mul x_{k-1}. On return x_k = add + mul x_in, x_in being the
     // value before lane 0 (or x_k = add where flag is set).
     __device__ inline void affine_combine(double a_prev, double m_prev, int f_prev, double& add, double& mul, int& flag)
     {
-        if (!flag)
-        {
-            add = fma(mul, a_prev, add);
-            mul = mul * m_prev;
-            flag = f_prev;
-        }
+        double const a = fma(mul, a_prev, add);
+        double const m = mul * m_prev;
+        add = flag ? add : a;
+        mul = flag ? mul : m;
+        flag |= f_prev;
     }
 
     template<int N>
     __device__ inline void affine_step(double& add, double& mul, int& flag)
     {
-        double const a_prev = row_shr<N>(add, 0.0);
+        double const a_prev = row_shr0<N>(add);
         double const m_prev = row_shr<N>(mul, 1.0);
-        int const f_prev = row_shr<N>(flag, 0);
+        int const f_prev = row_shr0<N>(flag);
         affine_combine(a_prev, m_prev, f_prev, add, mul, flag);
     }
 
@@ -152,6 +178,18 @@ namespace lue_amd {
         }
     }
 
+    // Inclusive sum over the 64 lanes of a wavefront (all lanes active).
+    __device__ inline int wave_inclusive_sum(int x)
+    {
+        x += row_shr0<1>(x);
+        x += row_shr0<2>(x);
+        x += row_shr0<4>(x);
+        x += row_shr0<8>(x);
+        x += row_bcast<dpp_row_bcast15, 0xA>(x, 0);
+        x += row_bcast<dpp_row_bcast31, 0xC>(x, 0);
+        return x;
+    }
+
     // Affine maps without start flags: a lane whose element does not depend on the lanes before it (a chain
     // start) has mul = 0, and compositions keep that (mul of the composition = product of the muls). On return
     // x_k = add + mul x_in for every lane. Needs finite add and mul (0 * inf or NaN would leak across a start):
@@ -159,7 +197,7 @@ namespace lue_amd {
     template<int N>
     __device__ inline void affine_step_nf(double& add, double& mul)
     {
-        double const a_prev = row_shr<N>(add, 0.0);
+        double const a_prev = row_shr0<N>(add);
         double const m_prev = row_shr<N>(mul, 1.0);
         add = fma(mul, a_prev, add);
         mul = mul * m_prev;
@@ -188,7 +226,7 @@ namespace lue_amd {
     template<int N>
     __device__ inline void affine_step_f32_nf(float& add, float& mul)
     {
-        float const a_prev = row_shr<N>(add, 0.0f);
+        float const a_prev = row_shr0<N>(add);
         float const m_prev = row_shr<N>(mul, 1.0f);
         add = fmaf(mul, a_prev, add);
         mul = mul * m_prev;
@@ -217,18 +255,16 @@ namespace lue_amd {
     // Sums: lane k holds x_k = term + x_{k-1}. On return x_k = term + x_in (or term where flag is set).
     __device__ inline void sum_combine(double t_prev, int f_prev, double& term, int& flag)
     {
-        if (!flag)
-        {
-            term += t_prev;
-            flag = f_prev;
-        }
+        double const t = term + t_prev;
+        term = flag ? term : t;
+        flag |= f_prev;
     }
 
     template<int N>
     __device__ inline void sum_step(double& term, int& flag)
     {
-        double const t_prev = row_shr<N>(term, 0.0);
-        int const f_prev = row_shr<N>(flag, 0);
+        double const t_prev = row_shr0<N>(term);
+        int const f_prev = row_shr0<N>(flag);
         sum_combine(t_prev, f_prev, term, flag);
     }
 
@@ -267,12 +303,11 @@ namespace lue_amd {
     template<bool Max>
     __device__ inline void clamp_combine(double d_prev, double c_prev, int f_prev, double& d, double& c, int& flag)
     {
-        if (!flag)
-        {
-            d = extremum_nan<Max>(d, d_prev + c);
-            c = c_prev + c;
-            flag = f_prev;
-        }
+        double const dn = extremum_nan<Max>(d, d_prev + c);
+        double const cn = c_prev + c;
+        d = flag ? d : dn;
+        c = flag ? c : cn;
+        flag |= f_prev;
     }
 
     template<bool Max>
@@ -285,8 +320,8 @@ namespace lue_amd {
     __device__ inline void clamp_step(double& d, double& c, int& flag)
     {
         double const d_prev = row_shr<N>(d, clamp_identity<Max>());
-        double const c_prev = row_shr<N>(c, 0.0);
-        int const f_prev = row_shr<N>(flag, 0);
+        double const c_prev = row_shr0<N>(c);
+        int const f_prev = row_shr0<N>(flag);
         clamp_combine<Max>(d_prev, c_prev, f_prev, d, c, flag);
     }
 

@@ -12,16 +12,24 @@ namespace lue_amd {
     constexpr std::uint8_t no_data_ldd = 255;
 
     // Neighbour k of a cell, clockwise from north: N, NE, E, SE, S, SW, W, NW.
-    // idx0 is the row (north = -1), idx1 the column.
+    // idx0 is the row (north = -1), idx1 the column. The offsets + 1 are 2-bit fields of a constant (bit
+    // extract and subtract for a k known only at run time, where a chain of comparisons compiles to
+    // branches): rows 0 0 1 2 2 2 1 0, columns 1 2 2 2 1 0 0 0.
     __host__ __device__ constexpr int neighbour_dr(int k)
     {
-        return (k == 0 || k == 1 || k == 7) ? -1 : ((k == 3 || k == 4 || k == 5) ? 1 : 0);
+        return static_cast<int>((0x1A90u >> (2 * k)) & 3u) - 1;
     }
 
     __host__ __device__ constexpr int neighbour_dc(int k)
     {
-        return (k == 1 || k == 2 || k == 3) ? 1 : ((k == 5 || k == 6 || k == 7) ? -1 : 0);
+        return static_cast<int>((0x01A9u >> (2 * k)) & 3u) - 1;
     }
+
+    static_assert(neighbour_dr(0) == -1 && neighbour_dc(0) == 0 && neighbour_dr(1) == -1 && neighbour_dc(1) == 1 &&
+                      neighbour_dr(2) == 0 && neighbour_dc(2) == 1 && neighbour_dr(3) == 1 && neighbour_dc(3) == 1 &&
+                      neighbour_dr(4) == 1 && neighbour_dc(4) == 0 && neighbour_dr(5) == 1 && neighbour_dc(5) == -1 &&
+                      neighbour_dr(6) == 0 && neighbour_dc(6) == -1 && neighbour_dr(7) == -1 && neighbour_dc(7) == -1,
+                  "neighbour offsets N, NE, E, SE, S, SW, W, NW");
 
     // ldd keypad code (7 8 9 / 4 5 6 / 1 2 3) -> neighbour index of the downstream cell, or -1 for a
     // sink (5) and for any code that is not a direction, which LUE also treats as a sink

@@ -106,6 +106,13 @@ namespace lue_amd {
         atomicOr(reinterpret_cast<unsigned int*>(upmask) + (t >> 2), (1u << opposite(dir)) << (8 * (t & 3)));
     }
 
+    // Offset of element (lr, lc) of a tile from the tile's first element, in a row-major raster of `cols`
+    // columns: one 32 x 32 -> 64-bit multiply-add per lane.
+    __device__ inline std::int64_t tile_offset(std::uint32_t lr, std::uint32_t cols, std::uint32_t lc)
+    {
+        return static_cast<std::int64_t>(static_cast<std::uint64_t>(lr) * cols + lc);
+    }
+
     // Cell states inside a round.
     constexpr std::uint8_t state_absent = 0;   // outside the array or ldd no-data
     constexpr std::uint8_t state_pending = 1;  // not solved yet
@@ -123,20 +130,30 @@ namespace lue_amd {
         constexpr int cells_per_thread = (G::n + sweep_threads - 1) / sweep_threads;
 
         int const tid = threadIdx.x;
-        int const nthreads = blockDim.x;
+        constexpr int nthreads = sweep_threads;  // every tile kernel is launched with sweep_threads threads
         std::uint32_t const cat_index = as_global(a.tile_cat)[tile];
         Catchment const cat = a.cats[cat_index];
         CatchmentIO<T> const io = a.io[cat_index];
         auto const g_ldd = as_global(cat.ldd);
         auto const g_out = as_global(io.out);
         auto const g_rem = as_global(io.rem);
-        std::int64_t const local = tile - cat.tile_base;
-        int const ty = static_cast<int>(local / cat.tiles_x);
-        int const tx = static_cast<int>(local % cat.tiles_x);
+        // tile coordinates in the catchment (32-bit division: a catchment has fewer than 2^32 tiles)
+        std::uint32_t const local = static_cast<std::uint32_t>(tile - cat.tile_base);
+        std::uint32_t const tiles_x = static_cast<std::uint32_t>(cat.tiles_x);
+        int const ty = static_cast<int>(local / tiles_x);
+        int const tx = static_cast<int>(local - static_cast<std::uint32_t>(ty) * tiles_x);
         std::int64_t const rows = cat.rows;
         std::int64_t const cols = cat.cols;
         std::int64_t const r0 = static_cast<std::int64_t>(ty) * TH;
         std::int64_t const c0 = static_cast<std::int64_t>(tx) * TW;
+        // 32-bit coordinates (a network's rasters have at most 2^31 rows and columns and 2^32 cells): a row
+        // above the array wraps to a large unsigned value, so one unsigned compare per axis tests bounds; lanes
+        // address the tile's elements from its first one (`origin`) with tile_offset
+        std::uint32_t const rows32 = static_cast<std::uint32_t>(rows);
+        std::uint32_t const cols32 = static_cast<std::uint32_t>(cols);
+        std::uint32_t const r0u = static_cast<std::uint32_t>(ty) * TH;
+        std::uint32_t const c0u = static_cast<std::uint32_t>(tx) * TW;
+        std::int64_t const origin = r0 * cols + c0;
         auto const pbits = as_global(a.pending_bits + tile * G::words);
         auto const stamps = as_global(a.border_stamp + tile * G::perimeter);
         auto const out_lo = as_global(compensated ? a.out_lo + cat.cell_base : nullptr);
@@ -164,22 +181,37 @@ namespace lue_amd {
             for (int j = 0; j < cells_per_thread; ++j)
             {
                 int const i = tid + j * nthreads;
-                std::int64_t const r = r0 + i / TW;
-                std::int64_t const c = c0 + i % TW;
-                if (i < G::n && r < rows && c < cols)
-                    load_raw<OP, T>(a, io, r * cols + c, raw[j]);
+                std::uint32_t const lr = static_cast<std::uint32_t>(i / TW);
+                std::uint32_t const lc = static_cast<std::uint32_t>(i % TW);
+                if (i < G::n && r0u + lr < rows32 && c0u + lc < cols32)
+                    load_raw<OP, T>(a, io, origin + tile_offset(lr, cols32, lc), raw[j]);
             }
         }
         std::uint8_t halo[halo_per_thread];
+        std::int64_t const halo_origin = origin - cols - 1;  // element (-1, -1) of the tile (may lie outside)
 #pragma unroll
         for (int j = 0; j < halo_per_thread; ++j)
         {
             int const i = tid + j * nthreads;
-            int const hr = i / G::hw;
-            int const hc = i % G::hw;
-            std::int64_t const r = r0 + hr - 1;
-            std::int64_t const c = c0 + hc - 1;
-            halo[j] = (i < G::hn && r >= 0 && r < rows && c >= 0 && c < cols) ? g_ldd[r * cols + c] : no_data_ldd;
+            std::uint32_t const hr = static_cast<std::uint32_t>(i / G::hw);
+            std::uint32_t const hc = static_cast<std::uint32_t>(i % G::hw);
+            halo[j] = (i < G::hn && r0u + hr - 1u < rows32 && c0u + hc - 1u < cols32)
+                          ? g_ldd[halo_origin + tile_offset(hr, cols32, hc)]
+                          : no_data_ldd;
+        }
+        // this wavefront's part of every level of the in-tile layout: lane l holds the bounds of level l
+        [[maybe_unused]] int split_lo = 0;
+        [[maybe_unused]] int split_hi = 0;
+        if constexpr (layout)
+        {
+            int const lane_id = tid & 63;
+            int const wave = tid >> 6;
+            auto const split = as_global(a.tile_split + tile * tile_split_entries);
+            if (lane_id < tile_max_levels)
+            {
+                split_lo = split[lane_id * (sweep_waves + 1) + wave];
+                split_hi = split[lane_id * (sweep_waves + 1) + wave + 1];
+            }
         }
         if constexpr (layout)
         {
@@ -234,12 +266,11 @@ namespace lue_amd {
             if (code == no_data_ldd)
             {
                 st = state_absent;
-                std::int64_t const r = r0 + lr;
-                std::int64_t const c = c0 + lc;
-                if (round == 0 && r < rows && c < cols)
+                if (round == 0 && r0u + static_cast<std::uint32_t>(lr) < rows32 &&
+                    c0u + static_cast<std::uint32_t>(lc) < cols32)
                 {
                     // mark_no_data (accumulating_router.hpp:866-872)
-                    std::int64_t const g = r * cols + c;
+                    std::int64_t const g = origin + tile_offset(lr, cols32, lc);
                     g_out[g] = quiet_nan<T>();
                     if constexpr (two_outputs)
                     {
@@ -313,15 +344,13 @@ namespace lue_amd {
             // 3-4 (recorded in-tile chains, tile_layout.hpp): level after level; wavefront w solves the
             // positions [split[l][w], split[l][w + 1]) in blocks of 64. The light upstream cells of a chain
             // lie in lower levels, the heavy one is the previous position of the chain.
-            auto const split = as_global(a.tile_split + tile * tile_split_entries);
             int const levels = min(static_cast<int>(as_global(a.tile_nlevels)[tile]), tile_max_levels);
             static_assert((G::n & (G::n - 1)) == 0, "tile cell count must be a power of two");
-            int const wave = tid >> 6;
             int const lane_id = tid & 63;
             for (int l = 0; l < levels; ++l)
             {
-                int const lo = split[l * (sweep_waves + 1) + wave];
-                int const hi = split[l * (sweep_waves + 1) + wave + 1];
+                int const lo = __builtin_amdgcn_readlane(split_lo, l);
+                int const hi = __builtin_amdgcn_readlane(split_hi, l);
                 BlockState st;
                 for (int blk = lo; blk < hi; blk += 64)
                 {
@@ -533,14 +562,12 @@ namespace lue_amd {
             std::uint8_t const st = s.state[i];
             int const lr = i / TW;
             int const lc = i % TW;
-            std::int64_t const r = r0 + lr;
-            std::int64_t const c = c0 + lc;
             // border stamps serve the later rounds of the rounds schedule only
             int const p = layout ? -1 : G::perimeter_index(lr, lc);
             if (st == state_new)
             {
                 ++solved;
-                std::int64_t const g = r * cols + c;
+                std::int64_t const g = origin + tile_offset(lr, cols32, lc);
                 g_out[g] = s.hi[i];
                 if constexpr (two_outputs)
                     g_rem[g] = s.rem[i];
@@ -611,13 +638,18 @@ namespace lue_amd {
         // sum (neighbour order) of their upstream cells in this tile solved in this round
         if (round == 0 && a.stage_off != nullptr)
         {
-            if (tid == 0)
+            // pending cells before each bitmap word: a scan over the words by the first wavefront
+            if (tid < 64)
             {
-                std::uint32_t sum = 0;
-                for (int w = 0; w < G::words; ++w)
+                int carry = 0;
+                for (int w0 = 0; w0 < G::words; w0 += 64)
                 {
-                    s.pprefix[w] = sum;
-                    sum += __popc(s.pword[w]);
+                    int const w = w0 + tid;
+                    int const count = w < G::words ? __popc(s.pword[w]) : 0;
+                    int const inclusive = wave_inclusive_sum(count);
+                    if (w < G::words)
+                        s.pprefix[w] = static_cast<std::uint32_t>(carry + inclusive - count);
+                    carry += __builtin_amdgcn_readlane(inclusive, 63);
                 }
             }
             __syncthreads();
