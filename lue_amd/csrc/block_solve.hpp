@@ -366,10 +366,11 @@ namespace lue_amd {
     // approach the root from below after the first step.
     // T = double: FP64 steps until every |dQ_k| <= |Q_k| 2^-30, LUE's FP64 criterion (digits 31).
     // T = float: FP32 steps with the hardware power (log2, exp2) and reciprocal until every
-    // |dQ_k| <= |Q_k| 2^-10, then one Newton step on the FP64 residual (the FP64 residual correction; its
+    // |dQ_k| <= |Q_k| 2^-8, then one Newton step on the FP64 residual (the FP64 residual correction; its
     // scan in FP32). Newton's relative error obeys r' <= |beta - 1| / (2 (1 - r)) (1 + r)^|beta - 2| r^2, so
-    // after the step that meets 2^-10 the error is below about 4e-7 (beta in [0.3, 3]) and after the FP64
-    // step below 1e-12: the correction, not more FP32 steps, sets the accuracy.
+    // after the step that meets 2^-8 the error is below about 1.5e-5 (beta in [0.3, 3]) and after the FP64
+    // step below about 3e-10, far below the FP32 output rounding (6e-8): the correction, not more FP32
+    // steps, sets the accuracy.
     // Returns kw_block_serial, with q untouched, when a lane needs LUE's per-cell iteration: no-data, a zero
     // current discharge (dry cells and cells whose iteration starts from numeric_limits::min()), a non-positive
     // or non-finite iterate, or no convergence within 16 steps; kw_block_lue_far when LUE's own iteration
@@ -423,7 +424,7 @@ namespace lue_amd {
         // rounding (2^-24 of the largest term) moves the iterate by about 2^-24 of Q (the Jacobian is at least
         // c1), far below the stop criterion, and the FP64 residual correction below removes it. T = double:
         // all in FP64
-        constexpr double tol = std::is_same_v<T, double> ? 0x1p-30 : 0x1p-10;
+        constexpr double tol = std::is_same_v<T, double> ? 0x1p-30 : 0x1p-8;
         bool converged = false;
         float const carry32 = static_cast<float>(carry);
         float const extra32 = static_cast<float>(d.extra);

@@ -634,71 +634,85 @@ namespace lue_amd {
         }
         __syncthreads();
 
-        // 6. chain schedule: stage the inputs of the pending cells at their chain positions, with the FP64
-        // sum (neighbour order) of their upstream cells in this tile solved in this round
-        if (round == 0 && a.stage_off != nullptr)
+        // 6. chain schedule (in-tile layout replays only): stage the inputs of the pending cells at their chain
+        // positions, with the FP64 sum (neighbour order) of their upstream cells in this tile solved in this
+        // round. The pending cells are listed in rank order first, so that thread t stages ranks t, t + 256, ...:
+        // dense work per wavefront and contiguous loads of the chain positions.
+        if constexpr (layout)
         {
-            // pending cells before each bitmap word: a scan over the words by the first wavefront
-            if (tid < 64)
+            if (round == 0 && a.stage_off != nullptr)
             {
-                int carry = 0;
-                for (int w0 = 0; w0 < G::words; w0 += 64)
+                // pending cells before each bitmap word: a scan over the words by the first wavefront
+                if (tid < 64)
                 {
-                    int const w = w0 + tid;
-                    int const count = w < G::words ? __popc(s.pword[w]) : 0;
-                    int const inclusive = wave_inclusive_sum(count);
-                    if (w < G::words)
-                        s.pprefix[w] = static_cast<std::uint32_t>(carry + inclusive - count);
-                    carry += __builtin_amdgcn_readlane(inclusive, 63);
-                }
-            }
-            __syncthreads();
-            CellInput<OP, T>* const staged = static_cast<CellInput<OP, T>*>(a.staged);
-            // chain positions of this thread's pending cells, all loads in flight together
-            std::uint32_t pos_of[cells_per_thread];
-#pragma unroll
-            for (int j = 0; j < cells_per_thread; ++j)
-            {
-                int const i = tid + j * nthreads;
-                pos_of[j] = 0;
-                if (i < G::n && s.state[i] == state_pending)
-                {
-                    std::uint32_t const word = s.pword[i >> 5];
-                    std::uint32_t const b = static_cast<std::uint32_t>(i & 31);
-                    std::uint32_t const rank = s.pprefix[i >> 5] + __popc(word & ((1u << b) - 1u));
-                    pos_of[j] = as_global(a.stage_pos)[stage_begin + rank];
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < cells_per_thread; ++j)
-            {
-                int const i = tid + j * nthreads;
-                if (i >= G::n || s.state[i] != state_pending)
-                    continue;
-                std::uint32_t const pos = pos_of[j];
-                int const lr = i / TW;
-                int const lc = i % TW;
-                unsigned m = s.upmask[i];
-                double side = 0.0;
-                while (m != 0u)  // the set bits only, in neighbour order
-                {
-                    int const k = __builtin_ctz(m);
-                    m &= m - 1u;
-                    int const ur = lr + neighbour_dr(k);
-                    int const uc = lc + neighbour_dc(k);
-                    if (ur >= 0 && ur < TH && uc >= 0 && uc < TW && s.state[ur * TW + uc] == state_new)
+                    int carry = 0;
+                    for (int w0 = 0; w0 < G::words; w0 += 64)
                     {
-                        int const u = ur * TW + uc;
-                        double v = static_cast<double>(s.hi[u]);
-                        if constexpr (compensated)
-                            v += static_cast<double>(s.lo[u]);
-                        side += v;
+                        int const w = w0 + tid;
+                        int const count = w < G::words ? __popc(s.pword[w]) : 0;
+                        int const inclusive = wave_inclusive_sum(count);
+                        if (w < G::words)
+                            s.pprefix[w] = static_cast<std::uint32_t>(carry + inclusive - count);
+                        carry += __builtin_amdgcn_readlane(inclusive, 63);
                     }
                 }
-                staged[pos] = s.in[i];
-                as_global(a.stage_side)[pos] = side;
+                __syncthreads();
+                // the pending cells by rank, in the layout's order array (not read after the levels)
+                std::uint16_t* const list = s.sched.order;
+#pragma unroll
+                for (int j = 0; j < cells_per_thread; ++j)
+                {
+                    int const i = tid + j * nthreads;
+                    if (i < G::n && s.state[i] == state_pending)
+                    {
+                        std::uint32_t const word = s.pword[i >> 5];
+                        std::uint32_t const b = static_cast<std::uint32_t>(i & 31);
+                        list[s.pprefix[i >> 5] + __popc(word & ((1u << b) - 1u))] = static_cast<std::uint16_t>(i);
+                    }
+                }
+                __syncthreads();
+                int const npending = static_cast<int>(s.pprefix[G::words - 1] + __popc(s.pword[G::words - 1]));
+                CellInput<OP, T>* const staged = static_cast<CellInput<OP, T>*>(a.staged);
+                // chain positions of this thread's ranks, all loads in flight together
+                std::uint32_t pos_of[cells_per_thread];
+#pragma unroll
+                for (int j = 0; j < cells_per_thread; ++j)
+                {
+                    int const rank = tid + j * nthreads;
+                    pos_of[j] = rank < npending ? as_global(a.stage_pos)[stage_begin + rank] : 0u;
+                }
+#pragma unroll
+                for (int j = 0; j < cells_per_thread; ++j)
+                {
+                    int const rank = tid + j * nthreads;
+                    if (rank >= npending)
+                        break;
+                    int const i = list[rank] & (G::n - 1);  // in bounds whatever the list holds
+                    std::uint32_t const pos = pos_of[j];
+                    int const lr = i / TW;
+                    int const lc = i % TW;
+                    unsigned m = s.upmask[i];
+                    double side = 0.0;
+                    while (m != 0u)  // the set bits only, in neighbour order
+                    {
+                        int const k = __builtin_ctz(m);
+                        m &= m - 1u;
+                        int const ur = lr + neighbour_dr(k);
+                        int const uc = lc + neighbour_dc(k);
+                        if (ur >= 0 && ur < TH && uc >= 0 && uc < TW && s.state[ur * TW + uc] == state_new)
+                        {
+                            int const u = ur * TW + uc;
+                            double v = static_cast<double>(s.hi[u]);
+                            if constexpr (compensated)
+                                v += static_cast<double>(s.lo[u]);
+                            side += v;
+                        }
+                    }
+                    staged[pos] = s.in[i];
+                    as_global(a.stage_side)[pos] = side;
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
 
