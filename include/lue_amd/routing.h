@@ -10,7 +10,8 @@
  *
  * Conventions
  *   - All raster pointers are DEVICE pointers to whole arrays, row-major, rows x cols cells
- *     (the partition grid does not change results; partitions are contiguous sub-blocks in LUE).
+ *     (partitions are contiguous sub-blocks in LUE; the partition grid changes no result except the cell
+ *     classes of accu_info, which takes the partition shape as arguments).
  *   - A raster argument may be NULL; the argument is then the Scalar that follows it, broadcast to
  *     every cell (LUE's Scalar<E> overloads, accumulating_router.hpp:727-786).
  *   - `stream` is a hipStream_t (NULL: the default stream). Every call enqueues its work on `stream` and
@@ -88,7 +89,7 @@ int lue_amd_set_sweep_options(int schedule);
  * Per-call entry points keep the schedules they record (no reference counterpart; LUE recomputes the
  * inflow counts in every call): up to `entries` of them (default 4, 0 disables), found again by device,
  * ldd pointer, shape, tile shape, schedule and a 64-bit fingerprint of the ldd bytes computed in every
- * call, so an ldd changed in place is recorded anew. The ldd must be 4-byte aligned while caching.
+ * call, so an ldd changed in place is recorded anew.
  */
 int lue_amd_set_network_cache(int entries);
 
