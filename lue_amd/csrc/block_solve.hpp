@@ -223,33 +223,44 @@ namespace lue_amd {
         }
     }
 
-    template<int N>
-    __device__ inline void affine_step_f32_nf(float& add, float& mul)
-    {
-        float const a_prev = row_shr0<N>(add);
-        float const m_prev = row_shr<N>(mul, 1.0f);
-        add = fmaf(mul, a_prev, add);
-        mul = mul * m_prev;
-    }
-
+    // The FP32 flagless affine scan with the lane shifts folded into the arithmetic (DPP source operands), two
+    // instructions per step where separate moves take five: v_fmac_f32_dpp with bound_ctrl reads 0 from a
+    // source lane outside the row (add unchanged there), v_mul_f32_dpp without bound_ctrl does not write a
+    // lane whose source lies outside the row or whose row the row mask leaves out (mul unchanged there: the
+    // identity 1 without a register holding it). Same operations as the step
+    //     add = fma(mul, add[lane - N], add); mul = mul * mul[lane - N]
+    // (products commute exactly), so the results are the same bits. Every DPP instruction is preceded by
+    // s_nop 1: DPP must not read a VGPR written by the VALU instruction just before it (two wait states).
+    // All 64 lanes are active where the scans run.
     __device__ inline void affine_scan_f32_nf(float& add, float& mul)
     {
-        affine_step_f32_nf<1>(add, mul);
-        affine_step_f32_nf<2>(add, mul);
-        affine_step_f32_nf<4>(add, mul);
-        affine_step_f32_nf<8>(add, mul);
-        {
-            float const a_prev = row_bcast<dpp_row_bcast15, 0xA>(add, 0.0f);
-            float const m_prev = row_bcast<dpp_row_bcast15, 0xA>(mul, 1.0f);
-            add = fmaf(mul, a_prev, add);
-            mul = mul * m_prev;
-        }
-        {
-            float const a_prev = row_bcast<dpp_row_bcast31, 0xC>(add, 0.0f);
-            float const m_prev = row_bcast<dpp_row_bcast31, 0xC>(mul, 1.0f);
-            add = fmaf(mul, a_prev, add);
-            mul = mul * m_prev;
-        }
+        asm volatile(
+            "s_nop 1\n\t"
+            "v_fmac_f32_dpp %0, %0, %1 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+            "s_nop 1\n\t"
+            "v_mul_f32_dpp %1, %1, %1 row_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+            "s_nop 1\n\t"
+            "v_fmac_f32_dpp %0, %0, %1 row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+            "s_nop 1\n\t"
+            "v_mul_f32_dpp %1, %1, %1 row_shr:2 row_mask:0xf bank_mask:0xf\n\t"
+            "s_nop 1\n\t"
+            "v_fmac_f32_dpp %0, %0, %1 row_shr:4 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+            "s_nop 1\n\t"
+            "v_mul_f32_dpp %1, %1, %1 row_shr:4 row_mask:0xf bank_mask:0xf\n\t"
+            "s_nop 1\n\t"
+            "v_fmac_f32_dpp %0, %0, %1 row_shr:8 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+            "s_nop 1\n\t"
+            "v_mul_f32_dpp %1, %1, %1 row_shr:8 row_mask:0xf bank_mask:0xf\n\t"
+            "s_nop 1\n\t"
+            "v_fmac_f32_dpp %0, %0, %1 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+            "s_nop 1\n\t"
+            "v_mul_f32_dpp %1, %1, %1 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+            "s_nop 1\n\t"
+            "v_fmac_f32_dpp %0, %0, %1 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+            "s_nop 1\n\t"
+            "v_mul_f32_dpp %1, %1, %1 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+            "s_nop 1"
+            : "+v"(add), "+v"(mul));
     }
 
     // Sums: lane k holds x_k = term + x_{k-1}. On return x_k = term + x_in (or term where flag is set).
