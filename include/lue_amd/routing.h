@@ -349,6 +349,13 @@ int lue_amd_network_destroy(lue_amd_network network);
  * chain levels, no-data cells, tile shape, schedule, longest chain, side references */
 int lue_amd_network_info(lue_amd_network network, int64_t* info, int n);
 
+/*
+ * One chain level of a network (chain schedule; no reference counterpart): info[0] chain positions,
+ * [1] work items, [2] positions of the longest work item, [3] side references gathered before the level.
+ * level is 0 .. levels - 1 (lue_amd_network_info: info[6]).
+ */
+int lue_amd_network_level_info(lue_amd_network network, int level, int64_t* info, int n);
+
 int lue_amd_network_accu_f32(lue_amd_network network, const float* const* inflow, float inflow_scalar,
                              float* const* outflow, void* stream);
 int lue_amd_network_accu_f64(lue_amd_network network, const double* const* inflow, double inflow_scalar,

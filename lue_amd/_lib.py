@@ -32,6 +32,7 @@ SIGNATURES = {
     "lue_amd_network_create": [c_int, c_p, c_p, c_p, c_int, c_int, c_p, ctypes.POINTER(c_p)],
     "lue_amd_network_destroy": [c_p],
     "lue_amd_network_info": [c_p, c_p, c_int],
+    "lue_amd_network_level_info": [c_p, c_int, c_p, c_int],
     "lue_amd_network_accu_f32": [c_p, c_p, c_f, c_p, c_p],
     "lue_amd_network_accu_f64": [c_p, c_p, c_d, c_p, c_p],
     "lue_amd_network_accu_threshold_f32": [c_p, c_p, c_f, c_p, c_f, c_p, c_p, c_p],

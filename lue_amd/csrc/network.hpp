@@ -12,6 +12,7 @@
 #include "routing_common.hpp"
 #include "sweep.hpp"
 #include "chains.hpp"
+#include "chains_wide.hpp"
 
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -578,6 +579,7 @@ namespace lue_amd {
         std::uint32_t longest_chain{0};
         std::uint64_t nrefs{0};
         std::vector<std::uint32_t> level_items;  // work items of level l: [level_items[l], level_items[l+1])
+        std::vector<std::uint32_t> level_longest;  // positions of the longest work item of level l
         std::vector<std::uint32_t> level_pos;    // positions of level l: [level_pos[l], level_pos[l+1])
         DeviceArray<std::uint32_t> tile_off;     // per tile (+1): first compact index of its pending cells
         DeviceArray<std::uint32_t> pos_of;       // per compact index: chain position
@@ -1154,6 +1156,13 @@ namespace lue_amd {
                 net.level_items.push_back(l < nlevels ? h_stats[chains_max_levels + l] : nitems);
             }
             net.longest_chain = h_stats[2 * chains_max_levels];
+            // inside a level the longest work item comes first
+            net.level_longest.assign(nlevels, 0u);
+            for (std::uint32_t l = 0; l < nlevels; ++l)
+            {
+                if (net.level_items[l + 1] > net.level_items[l])
+                    LUE_AMD_HIP(copy_to_host(&net.level_longest[l], net.item_count.ptr + net.level_items[l], 1, stream));
+            }
             LUE_AMD_HIP(hipStreamSynchronize(stream));  // before the temporaries go
         }
 
@@ -1256,8 +1265,19 @@ namespace lue_amd {
             std::uint32_t const end = net.level_items[l + 1];
             if (end == begin)
                 continue;
-            hipLaunchKernelGGL((chains_run_kernel<OP, T>), dim3((end - begin + 3) / 4), dim3(256), 0, stream, a, c, begin,
-                               end);
+            bool const wide = has_wide_path<OP>() && static_cast<std::size_t>(l) < net.level_longest.size() &&
+                              net.level_longest[static_cast<std::size_t>(l)] >= wide_min_positions;
+            if (wide)
+            {
+                if constexpr (has_wide_path<OP>())
+                    hipLaunchKernelGGL((chains_run_wide_kernel<OP, T>), dim3(end - begin), dim3(wide_threads), 0, stream,
+                                       a, c, begin);
+            }
+            else
+            {
+                hipLaunchKernelGGL((chains_run_kernel<OP, T>), dim3((end - begin + 3) / 4), dim3(256), 0, stream, a, c,
+                                   begin, end);
+            }
         }
         hipLaunchKernelGGL((chains_scatter_kernel<OP, T, TW, TH>), dim3(static_cast<unsigned>(net.ntiles)), dim3(256), 0,
                            stream, a, net.tile_off.ptr, net.pos_of.ptr, c.res, c.res_rem);

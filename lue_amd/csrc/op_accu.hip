@@ -156,4 +156,22 @@ int lue_amd_plan_accu_f64(lue_amd_network network, const double* const* inflow, 
     return make_plan(plan, [&](Plan& p) { return network_accu<double>(network, inflow, inflow_scalar, outflow, nullptr, &p); });
 }
 
+int lue_amd_network_level_info(lue_amd_network network, int level, int64_t* info, int n)
+{
+    if (!network || !info || n < 0)
+        return fail(status_invalid_argument, "null argument");
+    Network const& net = network->net;
+    if (level < 0 || level >= net.levels())
+        return fail(status_invalid_argument, "no such chain level");
+    std::size_t const l = static_cast<std::size_t>(level);
+    int64_t const values[] = {static_cast<int64_t>(net.level_pos[l + 1] - net.level_pos[l]),
+                              static_cast<int64_t>(net.level_items[l + 1] - net.level_items[l]),
+                              l < net.level_longest.size() ? static_cast<int64_t>(net.level_longest[l]) : 0,
+                              l + 1 < net.level_ref.size() ? static_cast<int64_t>(net.level_ref[l + 1] - net.level_ref[l])
+                                                           : 0};
+    for (int i = 0; i < n && i < static_cast<int>(sizeof(values) / sizeof(values[0])); ++i)
+        info[i] = values[i];
+    return status_success;
+}
+
 }  // extern "C"

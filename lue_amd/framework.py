@@ -426,6 +426,18 @@ class FlowNetwork:
         _lib.check(_lib.load().lue_amd_network_info(self._handle, values, len(keys)))
         return dict(zip(keys, list(values)))
 
+    def level_info(self):
+        """Per chain level: positions, work items, positions of the longest work item, side references."""
+        import ctypes
+
+        keys = ["positions", "items", "longest_item", "side_refs"]
+        values = (ctypes.c_int64 * len(keys))()
+        levels = []
+        for level in range(self.info()["levels"]):
+            _lib.check(_lib.load().lue_amd_network_level_info(self._handle, level, values, len(keys)))
+            levels.append(dict(zip(keys, list(values))))
+        return levels
+
     # -- argument helpers -------------------------------------------------------------------------
     def _rasters(self, value, dtype, name):
         """Per-catchment pointer array (or None and a scalar)."""
