@@ -1,0 +1,102 @@
+"""Long chains: levels whose longest work item has at least 2048 positions run one work item per workgroup
+(chains_wide.hpp). Networks made of a few very long flow paths, against exact counts and the oracle."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+WIDE_MIN_POSITIONS = 2048
+
+
+@pytest.fixture(scope="module")
+def fw():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lue_amd import framework
+
+    return framework
+
+
+def row_ldd(cols):
+    """One row flowing east, off the array at its end: one chain of almost every cell."""
+    return np.full((1, cols), 6, dtype=np.uint8)
+
+
+def serpentine_ldd(rows, cols):
+    """Rows flowing east and west in turn, joined at the ends: one flow path through every cell, leaving
+    the array from the last row."""
+    ldd = np.empty((rows, cols), dtype=np.uint8)
+    for r in range(rows):
+        east = r % 2 == 0
+        ldd[r, :] = 6 if east else 4
+        if r < rows - 1:
+            ldd[r, cols - 1 if east else 0] = 2  # down to the next row
+    return ldd
+
+
+def path_order(ldd):
+    """Cells in flow order along the single path (start: the cell nothing drains into)."""
+    rows, cols = ldd.shape
+    order = []
+    for r in range(rows):
+        cs = range(cols) if r % 2 == 0 else range(cols - 1, -1, -1)
+        order.extend((r, c) for c in cs)
+    return order
+
+
+def has_wide_level(net):
+    return any(level["longest_item"] >= WIDE_MIN_POSITIONS for level in net.level_info())
+
+
+@pytest.mark.parametrize("make", [lambda: row_ldd(12000), lambda: serpentine_ldd(64, 4096)], ids=["row", "serpentine"])
+def test_long_chain_accu_counts(fw, make):
+    ldd_np = make()
+    ldd = fw.PartitionedArray.from_numpy(ldd_np)
+    net = fw.FlowNetwork(ldd)
+    assert has_wide_level(net)
+    got = net.accu(1.0, dtype=torch.float64).numpy()
+    want = np.zeros(ldd_np.shape)
+    if ldd_np.shape[0] == 1:
+        want[0, :] = np.arange(1, ldd_np.shape[1] + 1)
+    else:
+        for k, cell in enumerate(path_order(ldd_np)):
+            want[cell] = k + 1
+    np.testing.assert_array_equal(got, want)
+    got32 = fw.accu(ldd, 1.0).numpy()  # per call: the same network schedule
+    np.testing.assert_array_equal(got32, want.astype(np.float32))
+
+
+@pytest.mark.parametrize("dry", [False, True], ids=["wet", "with_dry_cells"])
+def test_long_chain_kinematic_wave_against_oracle(fw, oracle_mod, dry):
+    rng = np.random.default_rng(41)
+    ldd_np = serpentine_ldd(48, 2048)
+    shape = ldd_np.shape
+    q_cur = rng.uniform(0.0, 5.0, shape)
+    q_lat = rng.uniform(0.0, 1e-4, shape)
+    alpha = rng.uniform(1.5, 2.5, shape)
+    if dry:
+        # a few dry cells: their groups take the one-wavefront solver (LUE's per-cell iteration)
+        q_cur[rng.random(shape) < 1e-4] = 0.0
+    ldd = fw.PartitionedArray.from_numpy(ldd_np)
+    net = fw.FlowNetwork(ldd)
+    assert has_wide_level(net)
+    for dtype, tol in ((np.float64, 1e-12), (np.float32, 1e-5)):
+        args = [fw.PartitionedArray.from_numpy(v.astype(dtype)) for v in (q_cur, q_lat, alpha)]
+        got = net.kinematic_wave(*args, dtype(0.6), dtype(3600.0), dtype(100.0)).numpy().astype(np.float64)
+        want = oracle_mod.kinematic_wave(ldd_np, q_cur.astype(dtype), q_lat.astype(dtype), alpha.astype(dtype),
+                                         dtype(0.6), dtype(3600.0), dtype(100.0), None, np.float64)
+        rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-30)
+        print(f"long chain kinematic wave {np.dtype(dtype).name} dry={dry}: max rel {rel.max():.3e}", flush=True)
+        assert rel.max() <= tol
+
+
+def test_long_chain_accu_against_oracle(fw, oracle_mod):
+    rng = np.random.default_rng(43)
+    ldd_np = serpentine_ldd(32, 4096)
+    material = rng.random(ldd_np.shape)
+    ldd = fw.PartitionedArray.from_numpy(ldd_np)
+    got = fw.accu(ldd, fw.PartitionedArray.from_numpy(material)).numpy()
+    want = oracle_mod.accu(ldd_np, material, None, np.float64)
+    assert (np.abs(got - want) / np.maximum(np.abs(want), 1e-30)).max() <= 1e-12
