@@ -84,6 +84,7 @@ namespace lue_amd {
         a.beta_s = sc.beta_s;
         a.dx_s = sc.dx_s;
         a.dt = sc.dt;
+        a.c1_s = sc.dt / sc.dx_s;  // IEEE division in T, as the device's a.dt / dx
         return a;
     }
 

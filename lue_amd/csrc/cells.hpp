@@ -63,6 +63,7 @@ namespace lue_amd {
         T beta_s;
         T dx_s;
         T dt;
+        T c1_s;  // dt / dx_s: the kinematic wave's c1 of every cell where the channel length is the scalar
 
         float* out_lo;  // compensation of `out` (float path) by global cell, for cells read across rounds
 
@@ -256,9 +257,11 @@ namespace lue_amd {
         }
     }
 
-    // Domain checks and derived coefficients of one cell from its raw inputs.
+    // Domain checks and derived coefficients of one cell from its raw inputs. dx_scalar: the channel length
+    // is the scalar of the arguments (its c1 = dt / dx is then a.c1_s, the same quotient computed once).
     template<int OP, typename T>
-    __device__ inline void stage_raw(SweepArgs<T> const& a, RawInput<OP, T> const& r, CellInput<OP, T>& c)
+    __device__ inline void stage_raw(SweepArgs<T> const& a, RawInput<OP, T> const& r, CellInput<OP, T>& c,
+                                     bool dx_scalar = false)
     {
         if constexpr (OP == op_accu)
         {
@@ -296,7 +299,7 @@ namespace lue_amd {
                 c.q_cur = ok ? q_cur : quiet_nan<T>();
                 c.alpha = alpha;
                 c.beta = beta;
-                c.c1 = a.dt / dx;  // correctly rounded FP32 division: the FP64 quotient rounded to FP32
+                c.c1 = dx_scalar ? a.c1_s : a.dt / dx;  // correctly rounded FP32 division
                 c.b_hi = static_cast<float>(b);
                 c.b_lo = static_cast<float>(b - static_cast<double>(c.b_hi));
             }
@@ -306,7 +309,7 @@ namespace lue_amd {
                 c.q_lat = q_lat;
                 c.alpha = alpha;
                 c.beta = beta;
-                c.c1 = dt / dx;
+                c.c1 = dx_scalar ? static_cast<double>(a.c1_s) : dt / dx;
                 double const qb = q_cur > T{0} ? dpow(q_cur, beta) : pow(q_cur, beta);
                 c.b = ok ? alpha * qb + dt * q_lat : 0.0;
             }

@@ -59,8 +59,8 @@ namespace lue_amd {
     template<int N>
     struct TileScheduleLds<true, N>
     {
-        std::uint16_t order[N];  // local cell at each position
-        std::uint8_t flags[N];   // chain start, heavy child direction
+        alignas(8) std::uint16_t order[N];  // local cell at each position
+        alignas(4) std::uint8_t flags[N];   // chain start, heavy child direction
     };
 
     template<int OP, typename T, int TW, int TH, bool LAYOUT = false>
@@ -217,26 +217,38 @@ namespace lue_amd {
         {
             auto const order = as_global(a.tile_order + tile * G::n);
             auto const flags = as_global(a.tile_flags + tile * G::n);
-            std::uint16_t o[cells_per_thread];
-            std::uint8_t f[cells_per_thread];
-#pragma unroll
-            for (int j = 0; j < cells_per_thread; ++j)
+            if constexpr (cells_per_thread == 4 && G::n == 4 * nthreads)
             {
-                int const p = tid + j * nthreads;
-                if (p < G::n)
-                {
-                    o[j] = order[p];
-                    f[j] = flags[p];
-                }
+                // positions 4 tid .. 4 tid + 3: one 8-byte and one 4-byte load (a tile's layout starts at a
+                // multiple of 2 n bytes, n a multiple of 1024)
+                std::uint64_t const o4 = reinterpret_cast<global_ptr<std::uint64_t const>>(order)[tid];
+                std::uint32_t const f4 = reinterpret_cast<global_ptr<std::uint32_t const>>(flags)[tid];
+                reinterpret_cast<std::uint64_t*>(s.sched.order)[tid] = o4;
+                reinterpret_cast<std::uint32_t*>(s.sched.flags)[tid] = f4;
             }
-#pragma unroll
-            for (int j = 0; j < cells_per_thread; ++j)
+            else
             {
-                int const p = tid + j * nthreads;
-                if (p < G::n)
+                std::uint16_t o[cells_per_thread];
+                std::uint8_t f[cells_per_thread];
+#pragma unroll
+                for (int j = 0; j < cells_per_thread; ++j)
                 {
-                    s.sched.order[p] = o[j];
-                    s.sched.flags[p] = f[j];
+                    int const p = tid + j * nthreads;
+                    if (p < G::n)
+                    {
+                        o[j] = order[p];
+                        f[j] = flags[p];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < cells_per_thread; ++j)
+                {
+                    int const p = tid + j * nthreads;
+                    if (p < G::n)
+                    {
+                        s.sched.order[p] = o[j];
+                        s.sched.flags[p] = f[j];
+                    }
                 }
             }
         }
@@ -286,7 +298,7 @@ namespace lue_amd {
                 mark_upstream<TW, TH>(s.ldd, s.upmask, lr, lc, code);
                 if (round == 0)
                 {
-                    stage_raw<OP, T>(a, raw[j], s.in[i]);
+                    stage_raw<OP, T>(a, raw[j], s.in[i], io.dx == nullptr);
                 }
                 else if (pending)
                 {
