@@ -343,6 +343,22 @@ def main():
     del kw_ref
     info_kw = net_kw.info()
     info_accu = net_accu.info()
+    levels_kw = net_kw.level_info()
+
+    # accu_info on the same ldd and partition grid (per call: the ldd's network is recorded by the first call)
+    log("[bench] accu_info")
+    classes = fw.accu_info(ldd)
+    torch.cuda.synchronize()
+    info_reps = max(3, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(info_reps):
+        classes = fw.accu_info(ldd)
+    torch.cuda.synchronize()
+    t_info = (time.perf_counter() - t0) / info_reps
+    del classes
+    accu_info_result = {"ms_per_call": round(t_info * 1e3, 3), "mcells_per_s": round(cells / t_info / 1e6, 1),
+                        "partition_shape": list(ps),
+                        "note": "per-call accu_info (crossing indicator, accu of it, classes) on the bench's ldd"}
 
     value = cells / t32 / 1e6
     value64 = cells / t64 / 1e6
@@ -463,7 +479,9 @@ def main():
                         "per_call_ms_per_step": round(t64_call * 1e3, 3)},
         "fp32_per_call": {"value": round(cells / t32_call / 1e6, 1), "ms_per_step": round(t32_call * 1e3, 3),
                           "note": "per-call operations of the Python front end (each enqueues its kernels on the flow network the ldd array keeps)"},
-        "network": {"record_ms": round(t_network * 1e3, 1), "kinematic_wave": info_kw, "accu": info_accu},
+        "network": {"record_ms": round(t_network * 1e3, 1), "kinematic_wave": info_kw, "accu": info_accu,
+                    "chain_levels": levels_kw},
+        "accu_info": accu_info_result,
         "cold_call_ms": round(cold_ms, 1),
         "accuracy_vs_fp64": {"accu_max_rel": acc_err[0], "accu_mean_rel": acc_err[1],
                              "kinematic_wave_max_rel": kw_err[0], "kinematic_wave_mean_rel": kw_err[1]},
