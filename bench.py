@@ -10,7 +10,7 @@ whole raster, on one GPU, inputs resident in HBM:
     plus one FP64 correction) on flow networks recorded once; the plain FP64 path and the per-call
     entry points are timed beside it.
 
-Prints one JSON line. The roofline is that of the dominant kernel (the first round of the tiled sweep of
+Prints one JSON line. The roofline is that of the dominant kernel (the tile pass, round 0 of the sweep, of
 the kinematic wave): algorithmic bytes per launch / its HIP-event time, and its HBM traffic per launch
 from two rocprofv3 --pmc runs of this script (FETCH_SIZE, WRITE_SIZE), calibrated with a copy kernel of
 known size. The CPU baseline is the oracle (C++ restatement of LUE's algorithm) on the same inputs on the
@@ -43,7 +43,7 @@ BYTES_KW_F32 = 1 + 4 + 4 + 4 + 4
 BYTES_ACCU_F64 = 1 + 8 + 8
 BYTES_KW_F64 = 1 + 8 + 8 + 8 + 8
 
-KERNELS = {"kinematic_wave_f32": "sweep_first_round<3, float", "accu_f32": "sweep_first_round<0, float"}
+KERNELS = {"kinematic_wave_f32": "tile_pass_kernel<3, float", "accu_f32": "tile_pass_kernel<0, float"}
 COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read, 1 GiB written
 
 

@@ -27,7 +27,7 @@ namespace lue_amd {
     template<int OP, typename T, int TW, int TH>
     constexpr bool fits()
     {
-        return sizeof(SweepLds<OP, T, TW, TH>) <= 160 * 1024;
+        return sizeof(SweepLds<OP, T, TW, TH>) <= 160 * 1024 && sizeof(TilePassLds<OP, T, TW, TH>) <= 160 * 1024;
     }
 
     // Call f.template operator()<TW, TH>() for a tile shape the operation fits in.

@@ -82,11 +82,14 @@ namespace lue_amd {
         void* staged;                    // per chain position: CellInput<OP, T>
         double* stage_side;              // per chain position: FP64 sum of the same-tile upstream cells solved
 
-        // Chain schedule, round 0 of a run: in-tile chain layout (tile_layout.hpp; null tile_order: walks)
-        std::uint16_t const* tile_order;
-        std::uint8_t const* tile_flags;
+        // Chain schedule, round 0 of a run (tile_pass.hpp): in-tile chain layout and the rest of the tiles'
+        // recorded programs (tile_layout.hpp)
+        std::uint16_t const* tile_order;   // per tile and position: local cell | order_start_bit
+        std::uint8_t const* tile_flags;    // per tile and position: light children
         std::uint16_t const* tile_split;
         std::uint8_t const* tile_nlevels;
+        std::uint32_t const* lo_bits;      // per tile: local cells read by other tiles
+        std::uint32_t const* stage_cell;   // per compact index: local cell | upstream mask solved in the tile << 16
     };
 
     template<int TW, int TH>

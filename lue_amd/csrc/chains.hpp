@@ -854,7 +854,9 @@ namespace lue_amd {
         }
     }
 
-    // Write the chain results of the pending cells of every tile to the output rasters.
+    // Write the chain results of the pending cells of every tile to the output rasters: thread r of a tile's
+    // workgroup takes the tile's pending ranks r, r + 256, ... (cell order), their local cells from the staging
+    // entries of the tile program (stage_cell) and their chain positions.
     template<int OP, typename T, int TW, int TH>
     __global__ void __launch_bounds__(256)
         chains_scatter_kernel(SweepArgs<T> a, std::uint32_t const* tile_off, std::uint32_t const* pos_of,
@@ -862,55 +864,39 @@ namespace lue_amd {
     {
         using G = Tile<TW, TH>;
         constexpr bool two_outputs = is_split_op(OP);
-        __shared__ std::uint32_t prefix[G::words + 1];
-        __shared__ std::uint32_t words[G::words];
         std::int64_t const tile = blockIdx.x;
         std::uint32_t const begin = as_global(tile_off)[tile];
-        if (as_global(tile_off)[tile + 1] == begin)
+        std::uint32_t const npending = as_global(tile_off)[tile + 1] - begin;
+        if (npending == 0)
             return;
-        auto const bits = as_global(a.pending_bits + tile * G::words);
-        for (int w = threadIdx.x; w < G::words; w += blockDim.x)
-            words[w] = bits[w];
-        __syncthreads();
-        if (threadIdx.x == 0)
-        {
-            std::uint32_t sum = 0;
-            for (int w = 0; w < G::words; ++w)
-            {
-                prefix[w] = sum;
-                sum += __popc(words[w]);
-            }
-            prefix[G::words] = sum;
-        }
-        __syncthreads();
         std::uint32_t const cat_index = as_global(a.tile_cat)[tile];
         Catchment const cat = a.cats[cat_index];
         CatchmentIO<T> const io = a.io[cat_index];
-        std::int64_t const local = tile - cat.tile_base;
-        std::int64_t const r0 = (local / cat.tiles_x) * TH;
-        std::int64_t const c0 = (local % cat.tiles_x) * TW;
-        // all loads of a thread's cells in flight together: positions, then results (random reads), then stores
+        std::uint32_t const local = static_cast<std::uint32_t>(tile - cat.tile_base);
+        std::uint32_t const tiles_x = static_cast<std::uint32_t>(cat.tiles_x);
+        std::uint32_t const ty = local / tiles_x;
+        std::uint32_t const tx = local - ty * tiles_x;
+        std::uint32_t const cols32 = static_cast<std::uint32_t>(cat.cols);
+        std::int64_t const origin = static_cast<std::int64_t>(ty * TH) * cat.cols + tx * TW;
+        // all loads of a thread's ranks in flight together: entries and positions, then results (random
+        // reads), then stores
         constexpr int per_thread = G::n / 256;
         static_assert(G::n % 256 == 0, "the scatter runs 256 threads per tile");
+        std::uint32_t cell[per_thread];
         std::uint32_t pos[per_thread];
-        bool pending[per_thread];
 #pragma unroll
         for (int j = 0; j < per_thread; ++j)
         {
-            int const i = static_cast<int>(threadIdx.x) + 256 * j;
-            std::uint32_t const word = words[i >> 5];
-            std::uint32_t const b = static_cast<std::uint32_t>(i & 31);
-            pending[j] = ((word >> b) & 1u) != 0u;
-            pos[j] = 0;
-            if (pending[j])
-                pos[j] = as_global(pos_of)[begin + prefix[i >> 5] + __popc(word & ((1u << b) - 1u))];
+            std::uint32_t const rank = threadIdx.x + 256u * j;
+            cell[j] = rank < npending ? as_global(a.stage_cell)[begin + rank] & (G::n - 1) : 0u;
+            pos[j] = rank < npending ? as_global(pos_of)[begin + rank] : 0u;
         }
         double value[per_thread];
         T remainder[per_thread];
 #pragma unroll
         for (int j = 0; j < per_thread; ++j)
         {
-            if (pending[j])
+            if (threadIdx.x + 256u * j < npending)
             {
                 value[j] = as_global(res)[pos[j]];
                 if constexpr (two_outputs)
@@ -920,10 +906,10 @@ namespace lue_amd {
 #pragma unroll
         for (int j = 0; j < per_thread; ++j)
         {
-            if (!pending[j])
-                continue;
-            int const i = static_cast<int>(threadIdx.x) + 256 * j;
-            std::int64_t const g = (r0 + i / TW) * cat.cols + (c0 + i % TW);
+            if (threadIdx.x + 256u * j >= npending)
+                break;
+            std::int64_t const g = origin + static_cast<std::int64_t>(
+                                                static_cast<std::uint64_t>(cell[j] / TW) * cols32 + cell[j] % TW);
             as_global(io.out)[g] = static_cast<T>(value[j]);
             if constexpr (two_outputs)
                 as_global(io.rem)[g] = remainder[j];

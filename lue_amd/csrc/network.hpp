@@ -11,6 +11,7 @@
 
 #include "routing_common.hpp"
 #include "sweep.hpp"
+#include "tile_pass.hpp"
 #include "chains.hpp"
 #include "chains_wide.hpp"
 
@@ -596,6 +597,10 @@ namespace lue_amd {
         DeviceArray<std::uint8_t> tile_flags;
         DeviceArray<std::uint16_t> tile_split;
         DeviceArray<std::uint8_t> tile_nlevels;
+        // the rest of the tiles' replay programs (tile_program_kernel)
+        DeviceArray<std::uint32_t> pending_bits;  // per tile: cells left pending by round 0
+        DeviceArray<std::uint32_t> lo_bits;       // per tile: local cells read by other tiles
+        DeviceArray<std::uint32_t> stage_cell;    // per compact index: local cell | solved upstream mask << 16
 
         Network() = default;
         Network(Network const&) = delete;
@@ -626,6 +631,9 @@ namespace lue_amd {
             tile_flags.reset();
             tile_split.reset();
             tile_nlevels.reset();
+            pending_bits.reset();
+            lo_bits.reset();
+            stage_cell.reset();
             if (switched)
                 (void)hipSetDevice(current);
         }
@@ -653,6 +661,9 @@ namespace lue_amd {
             tile_flags.stream = nullptr;
             tile_split.stream = nullptr;
             tile_nlevels.stream = nullptr;
+            pending_bits.stream = nullptr;
+            lo_bits.stream = nullptr;
+            stage_cell.stream = nullptr;
         }
     };
 
@@ -859,6 +870,8 @@ namespace lue_amd {
         a.tile_flags = nullptr;
         a.tile_split = nullptr;
         a.tile_nlevels = nullptr;
+        a.lo_bits = nullptr;
+        a.stage_cell = nullptr;
         return status_success;
     }
 
@@ -1239,6 +1252,17 @@ namespace lue_amd {
                            net.cats.ptr, net.tile_cat.ptr, pending_bits, net.tile_order.ptr, net.tile_flags.ptr,
                            net.tile_split.ptr, net.tile_nlevels.ptr, error.ptr);
         LUE_AMD_HIP(hipGetLastError());
+        // the pending bitmaps stay with the network (the tile pass and the scatter read them), beside the lo
+        // bitmaps and the staging entries of the pending cells
+        LUE_AMD_HIP(net.pending_bits.allocate(ntiles * G::words, stream));
+        LUE_AMD_HIP(net.lo_bits.allocate(ntiles * G::words, stream));
+        LUE_AMD_HIP(net.stage_cell.allocate(std::max<std::size_t>(net.pending, 1), stream));
+        LUE_AMD_HIP(hipMemcpyAsync(net.pending_bits.ptr, pending_bits, ntiles * G::words * sizeof(std::uint32_t),
+                                   hipMemcpyDeviceToDevice, stream));
+        hipLaunchKernelGGL((tile_program_kernel<TW, TH>), dim3(static_cast<unsigned>(ntiles)), dim3(256), 0, stream,
+                           net.cats.ptr, net.tile_cat.ptr, pending_bits, net.tile_off.ptr, net.lo_bits.ptr,
+                           net.stage_cell.ptr);
+        LUE_AMD_HIP(hipGetLastError());
         std::uint32_t errors = 0;
         LUE_AMD_HIP(copy_to_host(&errors, error.ptr, 1, stream));
         if (errors != 0)
@@ -1330,7 +1354,7 @@ namespace lue_amd {
     // aligned). The sizes depend on the network and the operation only.
     struct ReplayLayout
     {
-        std::size_t pending_bits{};  // per tile: pending bitmap (written by round 0, read by the scatter)
+        std::size_t pending_bits{};  // rounds schedule, per tile: pending bitmap (written by each round)
         std::size_t border_stamp{};  // per tile perimeter cell (rounds schedule only)
         std::size_t out_lo{};        // FP32 path: compensation term per cell
         std::size_t io{};            // per catchment: CatchmentIO<T>
@@ -1356,7 +1380,7 @@ namespace lue_amd {
         std::size_t const ntiles = static_cast<std::size_t>(net.ntiles);
         bool const chains = net.schedule == schedule_chains;
         std::size_t const pending = chains ? net.pending : 0;
-        l.pending_bits = take(ntiles * G::words * sizeof(std::uint32_t));
+        l.pending_bits = take(chains ? 0 : ntiles * G::words * sizeof(std::uint32_t));  // chains: the network's
         l.border_stamp = take(chains ? 0 : ntiles * G::perimeter * sizeof(std::uint32_t));
         l.out_lo = take(std::is_same_v<T, float> ? static_cast<std::size_t>(net.ncells) * sizeof(float) : 0);
         l.io = take(net.cats_host.size() * sizeof(CatchmentIO<T>));
@@ -1397,6 +1421,8 @@ namespace lue_amd {
         a.tile_flags = nullptr;
         a.tile_split = nullptr;
         a.tile_nlevels = nullptr;
+        a.lo_bits = nullptr;
+        a.stage_cell = nullptr;
         ChainRun<OP, T> c{};
         if (chains)
         {
@@ -1404,10 +1430,13 @@ namespace lue_amd {
             a.tile_flags = net.tile_flags.ptr;
             a.tile_split = net.tile_split.ptr;
             a.tile_nlevels = net.tile_nlevels.ptr;
+            a.pending_bits = net.pending_bits.ptr;
+            a.lo_bits = net.lo_bits.ptr;
+            a.stage_cell = net.stage_cell.ptr;
+            a.stage_off = net.tile_off.ptr;
+            a.stage_pos = net.pos_of.ptr;
             if (net.pending > 0)
             {
-                a.stage_off = net.tile_off.ptr;
-                a.stage_pos = net.pos_of.ptr;
                 a.staged = base + l.staged;
                 a.stage_side = reinterpret_cast<double*>(base + l.side);
                 c = ChainRun<OP, T>{net.item_start.ptr,
@@ -1429,10 +1458,10 @@ namespace lue_amd {
                 return st;
         }
         if (chains)
-            hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH, true>), dim3(static_cast<unsigned>(net.ntiles)),
+            hipLaunchKernelGGL((tile_pass_kernel<OP, T, TW, TH>), dim3(static_cast<unsigned>(net.ntiles)),
                                dim3(sweep_threads), 0, stream, a);
         else
-            hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH, false>), dim3(static_cast<unsigned>(net.ntiles)),
+            hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH>), dim3(static_cast<unsigned>(net.ntiles)),
                                dim3(sweep_threads), 0, stream, a);
         LUE_AMD_HIP(hipGetLastError());
         if (int const st = timer.mark(stream))
@@ -1557,7 +1586,7 @@ namespace lue_amd {
             if (int const st = timer.start(stream))
                 return st;
         }
-        hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH, false>),
+        hipLaunchKernelGGL((sweep_first_round<OP, T, TW, TH>),
                            dim3(static_cast<unsigned>(net.ntiles)), dim3(sweep_threads), 0,
                            stream, a);
         LUE_AMD_HIP(hipGetLastError());

@@ -30,7 +30,6 @@
 
 #include "block_solve.hpp"
 #include "cells.hpp"
-#include "tile_layout.hpp"
 #include "kinematic_wave.hpp"
 #include "routing_common.hpp"
 
@@ -43,27 +42,16 @@
 
 namespace lue_amd {
 
-    // LDS of the schedule inside a tile: the walks' counters, links and ready queue, or the recorded in-tile
-    // chain layout (positions and flags).
-    template<bool LAYOUT, int N>
-    struct TileScheduleLds;
-
+    // LDS of the schedule inside a tile: the walks' counters, links and ready queue.
     template<int N>
-    struct TileScheduleLds<false, N>
+    struct TileScheduleLds
     {
         std::uint32_t count[N / 4];  // four 8-bit counters per word
         std::int16_t next[N];        // downstream cell inside the tile, -1 if none
         std::uint16_t queue[N];      // initially ready cells
     };
 
-    template<int N>
-    struct TileScheduleLds<true, N>
-    {
-        alignas(8) std::uint16_t order[N];  // local cell at each position
-        alignas(4) std::uint8_t flags[N];   // chain start, heavy child direction
-    };
-
-    template<int OP, typename T, int TW, int TH, bool LAYOUT = false>
+    template<int OP, typename T, int TW, int TH>
     struct SweepLds
     {
         using G = Tile<TW, TH>;
@@ -74,12 +62,10 @@ namespace lue_amd {
         T hi[G::n];
         float lo[compensated ? G::n : 1];
         T rem[two_outputs ? G::n : 1];
-        TileScheduleLds<LAYOUT, G::n> sched;
+        TileScheduleLds<G::n> sched;
         std::uint8_t ldd[G::hn];
         alignas(4) std::uint8_t upmask[G::n];  // bit k: neighbour k drains into the cell (set by word-wide ORs)
         std::uint8_t state[G::n];  // see state_* below
-        std::uint32_t pword[G::words];   // pending bitmap of the tile
-        std::uint32_t pprefix[G::words]; // pending cells before each bitmap word
         unsigned int solved;
         unsigned int no_data;
         unsigned int queued;
@@ -120,13 +106,13 @@ namespace lue_amd {
     constexpr std::uint8_t state_new = 3;      // solved in this round (value in LDS)
 
     // One round over one tile. All threads of the workgroup call it.
-    template<int OP, typename T, int TW, int TH, bool LAYOUT>
-    __device__ void sweep_tile(SweepArgs<T> const& a, SweepLds<OP, T, TW, TH, LAYOUT>& s, std::int64_t tile,
+    template<int OP, typename T, int TW, int TH>
+    __device__ void sweep_tile(SweepArgs<T> const& a, SweepLds<OP, T, TW, TH>& s, std::int64_t tile,
                                std::uint32_t round)
     {
         using G = Tile<TW, TH>;
-        constexpr bool compensated = SweepLds<OP, T, TW, TH, LAYOUT>::compensated;
-        constexpr bool two_outputs = SweepLds<OP, T, TW, TH, LAYOUT>::two_outputs;
+        constexpr bool compensated = SweepLds<OP, T, TW, TH>::compensated;
+        constexpr bool two_outputs = SweepLds<OP, T, TW, TH>::two_outputs;
         constexpr int cells_per_thread = (G::n + sweep_threads - 1) / sweep_threads;
 
         int const tid = threadIdx.x;
@@ -165,15 +151,10 @@ namespace lue_amd {
             s.queued = 0;
             s.taken = 0;
         }
-        // first chain staging index of the tile, loaded early (used at the end)
-        std::uint32_t const stage_begin = (round == 0 && a.stage_off != nullptr) ? as_global(a.stage_off)[tile] : 0u;
 
         // 1. ldd with halo (cells outside the array read as no-data); in round 0 also the raw inputs of every
-        // cell, and with a recorded in-tile layout its order and flags. All loads of a thread are issued
-        // before the first one is used.
+        // cell. All loads of a thread are issued before the first one is used.
         constexpr int halo_per_thread = (G::hn + sweep_threads - 1) / sweep_threads;
-        // the recorded in-tile layout replaces the walks (round 0 of the chain schedule)
-        constexpr bool layout = LAYOUT;
         RawInput<OP, T> raw[cells_per_thread];
         if (round == 0)
         {
@@ -198,59 +179,6 @@ namespace lue_amd {
             halo[j] = (i < G::hn && r0u + hr - 1u < rows32 && c0u + hc - 1u < cols32)
                           ? g_ldd[halo_origin + tile_offset(hr, cols32, hc)]
                           : no_data_ldd;
-        }
-        // this wavefront's part of every level of the in-tile layout: lane l holds the bounds of level l
-        [[maybe_unused]] int split_lo = 0;
-        [[maybe_unused]] int split_hi = 0;
-        if constexpr (layout)
-        {
-            int const lane_id = tid & 63;
-            int const wave = tid >> 6;
-            auto const split = as_global(a.tile_split + tile * tile_split_entries);
-            if (lane_id < tile_max_levels)
-            {
-                split_lo = split[lane_id * (sweep_waves + 1) + wave];
-                split_hi = split[lane_id * (sweep_waves + 1) + wave + 1];
-            }
-        }
-        if constexpr (layout)
-        {
-            auto const order = as_global(a.tile_order + tile * G::n);
-            auto const flags = as_global(a.tile_flags + tile * G::n);
-            if constexpr (cells_per_thread == 4 && G::n == 4 * nthreads)
-            {
-                // positions 4 tid .. 4 tid + 3: one 8-byte and one 4-byte load (a tile's layout starts at a
-                // multiple of 2 n bytes, n a multiple of 1024)
-                std::uint64_t const o4 = reinterpret_cast<global_ptr<std::uint64_t const>>(order)[tid];
-                std::uint32_t const f4 = reinterpret_cast<global_ptr<std::uint32_t const>>(flags)[tid];
-                reinterpret_cast<std::uint64_t*>(s.sched.order)[tid] = o4;
-                reinterpret_cast<std::uint32_t*>(s.sched.flags)[tid] = f4;
-            }
-            else
-            {
-                std::uint16_t o[cells_per_thread];
-                std::uint8_t f[cells_per_thread];
-#pragma unroll
-                for (int j = 0; j < cells_per_thread; ++j)
-                {
-                    int const p = tid + j * nthreads;
-                    if (p < G::n)
-                    {
-                        o[j] = order[p];
-                        f[j] = flags[p];
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < cells_per_thread; ++j)
-                {
-                    int const p = tid + j * nthreads;
-                    if (p < G::n)
-                    {
-                        s.sched.order[p] = o[j];
-                        s.sched.flags[p] = f[j];
-                    }
-                }
-            }
         }
 #pragma unroll
         for (int j = 0; j < halo_per_thread; ++j)
@@ -307,23 +235,20 @@ namespace lue_amd {
                 }
             }
             s.state[i] = st;
-            if constexpr (!layout)
+            int next = -1;
+            if (st != state_absent)
             {
-                int next = -1;
-                if (st != state_absent)
+                int const dir = ldd_direction(code);
+                if (dir >= 0)
                 {
-                    int const dir = ldd_direction(code);
-                    if (dir >= 0)
-                    {
-                        int const dr = lr + neighbour_dr(dir);
-                        int const dc = lc + neighbour_dc(dir);
-                        if (dr >= 0 && dr < TH && dc >= 0 && dc < TW &&
-                            s.ldd[(dr + 1) * G::hw + (dc + 1)] != no_data_ldd)
-                            next = dr * TW + dc;
-                    }
+                    int const dr = lr + neighbour_dr(dir);
+                    int const dc = lc + neighbour_dc(dir);
+                    if (dr >= 0 && dr < TH && dc >= 0 && dc < TW &&
+                        s.ldd[(dr + 1) * G::hw + (dc + 1)] != no_data_ldd)
+                        next = dr * TW + dc;
                 }
-                s.sched.next[i] = static_cast<std::int16_t>(next);
             }
+            s.sched.next[i] = static_cast<std::int16_t>(next);
         }
         // cells of the halo ring (neighbouring tiles) that drain into this tile
         constexpr int ring = 2 * (TW + 2) + 2 * TH;
@@ -351,218 +276,146 @@ namespace lue_amd {
         }
         __syncthreads();
 
-        if constexpr (layout)
+        // 3. counts of unsolved upstream neighbours; the initially ready cells are compacted into an LDS
+        // queue (wavefront ballot + prefix count, one LDS atomic per wavefront)
+        int const lane = tid & 63;
+        std::uint64_t const lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        for (int j = 0; j < cells_per_thread; ++j)
         {
-            // 3-4 (recorded in-tile chains, tile_layout.hpp): level after level; wavefront w solves the
-            // positions [split[l][w], split[l][w + 1]) in blocks of 64. The light upstream cells of a chain
-            // lie in lower levels, the heavy one is the previous position of the chain.
-            int const levels = min(static_cast<int>(as_global(a.tile_nlevels)[tile]), tile_max_levels);
-            static_assert((G::n & (G::n - 1)) == 0, "tile cell count must be a power of two");
-            int const lane_id = tid & 63;
-            for (int l = 0; l < levels; ++l)
+            int const i = tid + j * nthreads;
+            bool is_ready = false;
+            if (i < G::n)
             {
-                int const lo = __builtin_amdgcn_readlane(split_lo, l);
-                int const hi = __builtin_amdgcn_readlane(split_hi, l);
-                BlockState st;
-                for (int blk = lo; blk < hi; blk += 64)
+                std::uint32_t n = 0;
+                if (s.state[i] == state_pending)
                 {
-                    int const p = blk + lane_id;
-                    int const nb = hi - blk < 64 ? hi - blk : 64;
-                    ChainLane<OP, T> d{};
-                    int i = 0;
-                    d.valid = p < hi;
-                    if (d.valid)
+                    int const lr = i / TW;
+                    int const lc = i % TW;
+                    std::uint8_t const mask = s.upmask[i];
+                    for (int k = 0; k < 8; ++k)
                     {
-                        i = s.sched.order[p] & (G::n - 1);  // in bounds whatever the recorded data
-                        std::uint8_t const f = s.sched.flags[p];
-                        d.start = (f & 1u) != 0;
-                        int const heavy = (f & 16u) ? ((f >> 1) & 7) : -1;
-                        int const lr = i / TW;
-                        int const lc = i % TW;
-                        // the light upstream neighbours, in neighbour order (only the set bits: most cells
-                        // have one or two)
-                        unsigned m = s.upmask[i];
-                        if (heavy >= 0)
-                            m &= ~(1u << heavy);
-                        double extra = 0.0;
-                        while (m != 0u)
+                        if (!((mask >> k) & 1u))
+                            continue;
+                        int const ur = lr + neighbour_dr(k);
+                        int const uc = lc + neighbour_dc(k);
+                        if (ur >= 0 && ur < TH && uc >= 0 && uc < TW)
                         {
-                            int const k = __builtin_ctz(m);
-                            m &= m - 1u;
-                            int const ur = lr + neighbour_dr(k);
-                            int const uc = lc + neighbour_dc(k);
-                            if (ur >= 0 && ur < TH && uc >= 0 && uc < TW)
-                            {
-                                int const u = ur * TW + uc;
-                                double v = static_cast<double>(s.hi[u]);
-                                if constexpr (compensated)
-                                    v += static_cast<double>(s.lo[u]);
-                                extra += v;
-                            }
+                            n += s.state[ur * TW + uc] == state_pending ? 1u : 0u;
                         }
-                        d.extra = extra;
-                        d.in = s.in[i];
+                        else if (round == 0)
+                        {
+                            ++n;
+                        }
+                        else
+                        {
+                            // upstream neighbour in another tile: solved when stamped in an earlier round
+                            int const nty = ty + (ur < 0 ? -1 : (ur >= TH ? 1 : 0));
+                            int const ntx = tx + (uc < 0 ? -1 : (uc >= TW ? 1 : 0));
+                            int const nlr = ur < 0 ? TH - 1 : (ur >= TH ? 0 : ur);
+                            int const nlc = uc < 0 ? TW - 1 : (uc >= TW ? 0 : uc);
+                            std::int64_t const ntile =
+                                cat.tile_base + static_cast<std::int64_t>(nty) * cat.tiles_x + ntx;
+                            std::uint32_t const st =
+                                as_global(a.border_stamp)[ntile * G::perimeter + G::perimeter_index(nlr, nlc)];
+                            n += st < round ? 0u : 1u;
+                        }
                     }
-                    double value;
-                    T rem{};
-                    solve_block<OP, T>(a, d, nb, st, value, rem);
-                    if (d.valid)
-                    {
-                        T const h = static_cast<T>(value);
-                        s.hi[i] = h;
-                        if constexpr (compensated)
-                            s.lo[i] = isnan(value) ? 0.0f : static_cast<float>(value - static_cast<double>(h));
-                        if constexpr (two_outputs)
-                            s.rem[i] = rem;
-                        s.state[i] = state_new;
-                    }
+                    is_ready = n == 0;
                 }
-                __syncthreads();
+                // 8-bit counter, written as a byte of the packed word
+                reinterpret_cast<std::uint8_t*>(s.sched.count)[i] = static_cast<std::uint8_t>(n);
+            }
+            std::uint64_t const ballot = __ballot(is_ready);
+            if (ballot)
+            {
+                unsigned int base = 0;
+                if (lane == 0)
+                    base = atomicAdd(&s.queued, static_cast<unsigned int>(__popcll(ballot)));
+                base = __shfl(base, 0, 64);
+                if (is_ready)
+                    s.sched.queue[base + __popcll(ballot & lanes_below)] = static_cast<std::uint16_t>(i);
             }
         }
-        else
+        __syncthreads();
+
+        // Sum (FP64, neighbour order) of the outflow of the upstream neighbours of tile cell i: solved in
+        // this round (LDS) or earlier (HBM).
+        auto gather = [&](int i, double& upstream, bool& poisoned)
         {
-            // 3. counts of unsolved upstream neighbours; the initially ready cells are compacted into an LDS
-            // queue (wavefront ballot + prefix count, one LDS atomic per wavefront)
-            int const lane = tid & 63;
-            std::uint64_t const lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-            for (int j = 0; j < cells_per_thread; ++j)
+            int const lr = i / TW;
+            int const lc = i % TW;
+            std::uint8_t const mask = s.upmask[i];
+            upstream = 0.0;
+            poisoned = false;
+            for (int k = 0; k < 8; ++k)
             {
-                int const i = tid + j * nthreads;
-                bool is_ready = false;
-                if (i < G::n)
+                if (!((mask >> k) & 1u))
+                    continue;
+                int const ur = lr + neighbour_dr(k);
+                int const uc = lc + neighbour_dc(k);
+                bool const inside = ur >= 0 && ur < TH && uc >= 0 && uc < TW;
+                double v;
+                // in round 0 every upstream cell of a ready cell lies in the tile and was solved now
+                if (inside && (round == 0 || s.state[ur * TW + uc] == state_new))
                 {
-                    std::uint32_t n = 0;
-                    if (s.state[i] == state_pending)
-                    {
-                        int const lr = i / TW;
-                        int const lc = i % TW;
-                        std::uint8_t const mask = s.upmask[i];
-                        for (int k = 0; k < 8; ++k)
-                        {
-                            if (!((mask >> k) & 1u))
-                                continue;
-                            int const ur = lr + neighbour_dr(k);
-                            int const uc = lc + neighbour_dc(k);
-                            if (ur >= 0 && ur < TH && uc >= 0 && uc < TW)
-                            {
-                                n += s.state[ur * TW + uc] == state_pending ? 1u : 0u;
-                            }
-                            else if (round == 0)
-                            {
-                                ++n;
-                            }
-                            else
-                            {
-                                // upstream neighbour in another tile: solved when stamped in an earlier round
-                                int const nty = ty + (ur < 0 ? -1 : (ur >= TH ? 1 : 0));
-                                int const ntx = tx + (uc < 0 ? -1 : (uc >= TW ? 1 : 0));
-                                int const nlr = ur < 0 ? TH - 1 : (ur >= TH ? 0 : ur);
-                                int const nlc = uc < 0 ? TW - 1 : (uc >= TW ? 0 : uc);
-                                std::int64_t const ntile =
-                                    cat.tile_base + static_cast<std::int64_t>(nty) * cat.tiles_x + ntx;
-                                std::uint32_t const st =
-                                    as_global(a.border_stamp)[ntile * G::perimeter + G::perimeter_index(nlr, nlc)];
-                                n += st < round ? 0u : 1u;
-                            }
-                        }
-                        is_ready = n == 0;
-                    }
-                    // 8-bit counter, written as a byte of the packed word
-                    reinterpret_cast<std::uint8_t*>(s.sched.count)[i] = static_cast<std::uint8_t>(n);
+                    int const u = ur * TW + uc;
+                    v = static_cast<double>(s.hi[u]);
+                    if constexpr (compensated)
+                        v += static_cast<double>(s.lo[u]);
                 }
-                std::uint64_t const ballot = __ballot(is_ready);
-                if (ballot)
+                else
                 {
-                    unsigned int base = 0;
-                    if (lane == 0)
-                        base = atomicAdd(&s.queued, static_cast<unsigned int>(__popcll(ballot)));
-                    base = __shfl(base, 0, 64);
-                    if (is_ready)
-                        s.sched.queue[base + __popcll(ballot & lanes_below)] = static_cast<std::uint16_t>(i);
+                    std::int64_t const ug = (r0 + ur) * cols + (c0 + uc);
+                    v = static_cast<double>(g_out[ug]);
+                    if constexpr (compensated)
+                        v += static_cast<double>(out_lo[ug]);
                 }
+                if (isnan(v))
+                    poisoned = true;
+                else
+                    upstream += v;
             }
-            __syncthreads();
+        };
 
-            // Sum (FP64, neighbour order) of the outflow of the upstream neighbours of tile cell i: solved in
-            // this round (LDS) or earlier (HBM).
-            auto gather = [&](int i, double& upstream, bool& poisoned)
-            {
-                int const lr = i / TW;
-                int const lc = i % TW;
-                std::uint8_t const mask = s.upmask[i];
-                upstream = 0.0;
-                poisoned = false;
-                for (int k = 0; k < 8; ++k)
-                {
-                    if (!((mask >> k) & 1u))
-                        continue;
-                    int const ur = lr + neighbour_dr(k);
-                    int const uc = lc + neighbour_dc(k);
-                    bool const inside = ur >= 0 && ur < TH && uc >= 0 && uc < TW;
-                    double v;
-                    // in round 0 every upstream cell of a ready cell lies in the tile and was solved now
-                    if (inside && (round == 0 || s.state[ur * TW + uc] == state_new))
-                    {
-                        int const u = ur * TW + uc;
-                        v = static_cast<double>(s.hi[u]);
-                        if constexpr (compensated)
-                            v += static_cast<double>(s.lo[u]);
-                    }
-                    else
-                    {
-                        std::int64_t const ug = (r0 + ur) * cols + (c0 + uc);
-                        v = static_cast<double>(g_out[ug]);
-                        if constexpr (compensated)
-                            v += static_cast<double>(out_lo[ug]);
-                    }
-                    if (isnan(v))
-                        poisoned = true;
-                    else
-                        upstream += v;
-                }
-            };
-
-            // 4. walks: lanes take initially ready cells from the queue and walk downstream while they are the
-            // last upstream contributor of the next cell
-            unsigned int const queued = s.queued;
+        // 4. walks: lanes take initially ready cells from the queue and walk downstream while they are the
+        // last upstream contributor of the next cell
+        unsigned int const queued = s.queued;
+        while (true)
+        {
+            unsigned int const item = atomicAdd(&s.taken, 1u);
+            if (item >= queued)
+                break;
+            int i = s.sched.queue[item];
             while (true)
             {
-                unsigned int const item = atomicAdd(&s.taken, 1u);
-                if (item >= queued)
+                int const next = s.sched.next[i];
+                double upstream;
+                bool poisoned;
+                gather(i, upstream, poisoned);
+                double out;
+                T rem{};
+                solve_cell<OP, T>(a, s.in[i], upstream, poisoned, out, rem);
+                T const hi = static_cast<T>(out);
+                s.hi[i] = hi;
+                if constexpr (compensated)
+                    s.lo[i] = isnan(out) ? 0.0f : static_cast<float>(out - static_cast<double>(hi));
+                if constexpr (two_outputs)
+                    s.rem[i] = rem;
+
+                // publish: the state write is ordered before the counter update (release)
+                __hip_atomic_store(&s.state[i], state_new, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (next < 0)
                     break;
-                int i = s.sched.queue[item];
-                while (true)
-                {
-                    int const next = s.sched.next[i];
-                    double upstream;
-                    bool poisoned;
-                    gather(i, upstream, poisoned);
-                    double out;
-                    T rem{};
-                    solve_cell<OP, T>(a, s.in[i], upstream, poisoned, out, rem);
-                    T const hi = static_cast<T>(out);
-                    s.hi[i] = hi;
-                    if constexpr (compensated)
-                        s.lo[i] = isnan(out) ? 0.0f : static_cast<float>(out - static_cast<double>(hi));
-                    if constexpr (two_outputs)
-                        s.rem[i] = rem;
-
-                    // publish: the state write is ordered before the counter update (release)
-                    __hip_atomic_store(&s.state[i], state_new, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (next < 0)
-                        break;
-                    std::uint32_t const shift = 8u * static_cast<std::uint32_t>(next & 3);
-                    std::uint32_t const old = __hip_atomic_fetch_sub(
-                        &s.sched.count[next >> 2], 1u << shift, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (((old >> shift) & 0xFFu) != 1u)
-                        break;  // confluence: another stream still has to arrive
-                    i = next;
-                }
+                std::uint32_t const shift = 8u * static_cast<std::uint32_t>(next & 3);
+                std::uint32_t const old = __hip_atomic_fetch_sub(
+                    &s.sched.count[next >> 2], 1u << shift, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (((old >> shift) & 0xFFu) != 1u)
+                    break;  // confluence: another stream still has to arrive
+                i = next;
             }
-            __syncthreads();
-
         }
+        __syncthreads();
+
 
         // 5. write back: outputs of cells solved in this round (coalesced), compensation terms that later
         // rounds read, pending bitmap, border stamps, and queue the tiles that receive flow
@@ -575,7 +428,7 @@ namespace lue_amd {
             int const lr = i / TW;
             int const lc = i % TW;
             // border stamps serve the later rounds of the rounds schedule only
-            int const p = layout ? -1 : G::perimeter_index(lr, lc);
+            int const p = G::perimeter_index(lr, lc);
             if (st == state_new)
             {
                 ++solved;
@@ -629,8 +482,6 @@ namespace lue_amd {
                 int const w = i >> 5;
                 pbits[w] = static_cast<std::uint32_t>(bits);
                 pbits[w + 1] = static_cast<std::uint32_t>(bits >> 32);
-                s.pword[w] = static_cast<std::uint32_t>(bits);
-                s.pword[w + 1] = static_cast<std::uint32_t>(bits >> 32);
             }
         }
         if (a.record)
@@ -645,121 +496,41 @@ namespace lue_amd {
             }
         }
         __syncthreads();
-
-        // 6. chain schedule (in-tile layout replays only): stage the inputs of the pending cells at their chain
-        // positions, with the FP64 sum (neighbour order) of their upstream cells in this tile solved in this
-        // round. The pending cells are listed in rank order first, so that thread t stages ranks t, t + 256, ...:
-        // dense work per wavefront and contiguous loads of the chain positions.
-        if constexpr (layout)
-        {
-            if (round == 0 && a.stage_off != nullptr)
-            {
-                // pending cells before each bitmap word: a scan over the words by the first wavefront
-                if (tid < 64)
-                {
-                    int carry = 0;
-                    for (int w0 = 0; w0 < G::words; w0 += 64)
-                    {
-                        int const w = w0 + tid;
-                        int const count = w < G::words ? __popc(s.pword[w]) : 0;
-                        int const inclusive = wave_inclusive_sum(count);
-                        if (w < G::words)
-                            s.pprefix[w] = static_cast<std::uint32_t>(carry + inclusive - count);
-                        carry += __builtin_amdgcn_readlane(inclusive, 63);
-                    }
-                }
-                __syncthreads();
-                // the pending cells by rank, in the layout's order array (not read after the levels)
-                std::uint16_t* const list = s.sched.order;
-#pragma unroll
-                for (int j = 0; j < cells_per_thread; ++j)
-                {
-                    int const i = tid + j * nthreads;
-                    if (i < G::n && s.state[i] == state_pending)
-                    {
-                        std::uint32_t const word = s.pword[i >> 5];
-                        std::uint32_t const b = static_cast<std::uint32_t>(i & 31);
-                        list[s.pprefix[i >> 5] + __popc(word & ((1u << b) - 1u))] = static_cast<std::uint16_t>(i);
-                    }
-                }
-                __syncthreads();
-                int const npending = static_cast<int>(s.pprefix[G::words - 1] + __popc(s.pword[G::words - 1]));
-                CellInput<OP, T>* const staged = static_cast<CellInput<OP, T>*>(a.staged);
-                // chain positions of this thread's ranks, all loads in flight together
-                std::uint32_t pos_of[cells_per_thread];
-#pragma unroll
-                for (int j = 0; j < cells_per_thread; ++j)
-                {
-                    int const rank = tid + j * nthreads;
-                    pos_of[j] = rank < npending ? as_global(a.stage_pos)[stage_begin + rank] : 0u;
-                }
-#pragma unroll
-                for (int j = 0; j < cells_per_thread; ++j)
-                {
-                    int const rank = tid + j * nthreads;
-                    if (rank >= npending)
-                        break;
-                    int const i = list[rank] & (G::n - 1);  // in bounds whatever the list holds
-                    std::uint32_t const pos = pos_of[j];
-                    int const lr = i / TW;
-                    int const lc = i % TW;
-                    unsigned m = s.upmask[i];
-                    double side = 0.0;
-                    while (m != 0u)  // the set bits only, in neighbour order
-                    {
-                        int const k = __builtin_ctz(m);
-                        m &= m - 1u;
-                        int const ur = lr + neighbour_dr(k);
-                        int const uc = lc + neighbour_dc(k);
-                        if (ur >= 0 && ur < TH && uc >= 0 && uc < TW && s.state[ur * TW + uc] == state_new)
-                        {
-                            int const u = ur * TW + uc;
-                            double v = static_cast<double>(s.hi[u]);
-                            if constexpr (compensated)
-                                v += static_cast<double>(s.lo[u]);
-                            side += v;
-                        }
-                    }
-                    staged[pos] = s.in[i];
-                    as_global(a.stage_side)[pos] = side;
-                }
-                __syncthreads();
-            }
-        }
     }
 
     // Workgroups of a tile kernel that fit one CU (160 KB of LDS, 32 wavefront slots), and the wavefronts per
     // SIMD that follow: the register budget the kernels are compiled for.
-    template<int OP, typename T, int TW, int TH, bool LAYOUT = false>
+    template<int OP, typename T, int TW, int TH>
     constexpr int sweep_workgroups_per_cu()
     {
-        int const by_lds = static_cast<int>((160 * 1024) / sizeof(SweepLds<OP, T, TW, TH, LAYOUT>));
+        int const by_lds = static_cast<int>((160 * 1024) / sizeof(SweepLds<OP, T, TW, TH>));
         int const by_waves = 32 / sweep_waves;
         int const n = by_lds < by_waves ? by_lds : by_waves;
         return n < 1 ? 1 : n;
     }
 
-    template<int OP, typename T, int TW, int TH, bool LAYOUT = false>
+    template<int OP, typename T, int TW, int TH>
     constexpr int sweep_waves_per_simd()
     {
-        return sweep_workgroups_per_cu<OP, T, TW, TH, LAYOUT>() * sweep_waves / 4;
+        return sweep_workgroups_per_cu<OP, T, TW, TH>() * sweep_waves / 4;
     }
 
-    // Round 0: every tile of the batch, one workgroup per tile (grid = number of tiles). LAYOUT: with the
-    // recorded in-tile chain layout (chain schedule replays), else with walks (recording, rounds schedule).
-    template<int OP, typename T, int TW, int TH, bool LAYOUT>
-    __global__ void __launch_bounds__(sweep_threads, (sweep_waves_per_simd<OP, T, TW, TH, LAYOUT>()))
+    // Round 0 with walks: every tile of the batch, one workgroup per tile (grid = number of tiles). It records
+    // the schedule, and runs round 0 of the rounds schedule; replays of the chain schedule run the recorded
+    // tile programs instead (tile_pass.hpp).
+    template<int OP, typename T, int TW, int TH>
+    __global__ void __launch_bounds__(sweep_threads, (sweep_waves_per_simd<OP, T, TW, TH>()))
         sweep_first_round(SweepArgs<T> a)
     {
-        __shared__ SweepLds<OP, T, TW, TH, LAYOUT> s;
-        sweep_tile<OP, T, TW, TH, LAYOUT>(a, s, blockIdx.x, 0u);
+        __shared__ SweepLds<OP, T, TW, TH> s;
+        sweep_tile<OP, T, TW, TH>(a, s, blockIdx.x, 0u);
     }
 
     // Round r >= 1 while recording: the tiles queued by round r - 1. Workgroup b takes items b, b + grid, ...
     template<int OP, typename T, int TW, int TH>
     __global__ void __launch_bounds__(sweep_threads, (sweep_waves_per_simd<OP, T, TW, TH>())) sweep_next_round(SweepArgs<T> a, std::uint32_t round)
     {
-        __shared__ SweepLds<OP, T, TW, TH, false> s;
+        __shared__ SweepLds<OP, T, TW, TH> s;
         std::uint32_t const slot = round % 3;
         if (blockIdx.x == 0 && threadIdx.x == 0)
         {
@@ -772,7 +543,7 @@ namespace lue_amd {
             std::int64_t const tile = a.worklist[slot * a.ntiles + item];
             if (threadIdx.x == 0)
                 a.tile_flag[slot * a.ntiles + tile] = 0u;
-            sweep_tile<OP, T, TW, TH, false>(a, s, tile, round);
+            sweep_tile<OP, T, TW, TH>(a, s, tile, round);
         }
     }
 
@@ -781,10 +552,10 @@ namespace lue_amd {
     __global__ void __launch_bounds__(sweep_threads, (sweep_waves_per_simd<OP, T, TW, TH>())) sweep_list_round(SweepArgs<T> a, std::uint32_t const* tiles,
                                                             std::uint32_t count, std::uint32_t round)
     {
-        __shared__ SweepLds<OP, T, TW, TH, false> s;
+        __shared__ SweepLds<OP, T, TW, TH> s;
         for (std::uint32_t item = blockIdx.x; item < count; item += gridDim.x)
         {
-            sweep_tile<OP, T, TW, TH, false>(a, s, tiles[item], round);
+            sweep_tile<OP, T, TW, TH>(a, s, tiles[item], round);
         }
     }
 

@@ -10,9 +10,11 @@
 // wavefront taking a contiguous part of the level in blocks of 64 positions (solve_block, block_solve.hpp).
 //
 // Recorded per tile: order[p] = local cell index at position p (chains level after level, each in flow
-// order), flags[p] = chain start (bit 0) | heavy child direction (bits 1-3) | has heavy child (bit 4),
-// and per level the positions of the sweep_waves wavefronts: split[l][w], w = 0..sweep_waves (parts made of
-// whole chains, packed so that the level needs few 64-position blocks).
+// order) | order_start_bit at the first (most upstream) cell of a chain, flags[p] = the light children of
+// that cell (bit k: neighbour k is a local upstream cell other than the chain predecessor), and per level the
+// positions of the sweep_waves wavefronts: split[l][w], w = 0..sweep_waves (parts made of whole chains,
+// packed so that the level needs few 64-position blocks). tile_program_kernel records the rest of what a
+// replay's tile pass needs (tile_pass.hpp), so that a replay reads no ldd.
 #pragma once
 
 #include "cells.hpp"
@@ -25,6 +27,7 @@
 namespace lue_amd {
 
     constexpr int tile_max_levels = 16;
+    constexpr std::uint16_t order_start_bit = 0x8000;  // tiles hold at most 4096 cells (12 index bits)
     constexpr int tile_split_entries = tile_max_levels * (sweep_waves + 1);
 
     template<int TW, int TH>
@@ -394,16 +397,108 @@ namespace lue_amd {
             int const pos = s.base[h] + len - 1 - d;
             if (pos >= G::n)
                 continue;
-            tile_order[pos] = static_cast<std::uint16_t>(i);
-            std::uint8_t f = (d == len - 1) ? 1 : 0;
+            // the chain start (its most upstream cell) in bit 15 of the order entry; the flags byte holds the
+            // light children: the local upstream neighbours other than the chain predecessor (heavy child)
+            tile_order[pos] = static_cast<std::uint16_t>(i | ((d == len - 1) ? order_start_bit : 0));
+            unsigned light = s.children[i];
             if (s.heavy[i] >= 0)
-                f |= static_cast<std::uint8_t>(16 | (s.heavy_dir[i] << 1));
-            tile_flags[pos] = f;
+                light &= ~(1u << s.heavy_dir[i]);
+            tile_flags[pos] = static_cast<std::uint8_t>(light);
         }
         for (int e = tid; e < tile_split_entries; e += nthreads)
             split[tile * tile_split_entries + e] = s.split[e];
         if (tid == 0)
             nlevels[tile] = static_cast<std::uint8_t>(s.levels);
+    }
+
+    // The rest of a tile's replay program, recorded once per flow network after the chain layout (tile_off: the
+    // first compact index of each tile's pending cells, cell order):
+    //   lo_bits[tile][w]: bit set for a local cell whose downstream cell is a data cell in another tile; such a
+    //     cell is a side reference of a chain position there, read with its FP32 compensation term;
+    //   stage_cell[tile_off[tile] + rank]: the rank-th pending cell of the tile (cell order) as its local index
+    //     | (mask of its upstream neighbours that are local cells of the tile, bit k = neighbour k) << 16: the
+    //     tile pass adds their outflows (neighbour order) to the cell's side sum.
+    template<int TW, int TH>
+    __global__ void __launch_bounds__(256)
+        tile_program_kernel(Catchment const* cats, std::uint32_t const* tile_cat, std::uint32_t const* pending_bits,
+                            std::uint32_t const* tile_off, std::uint32_t* lo_bits, std::uint32_t* stage_cell)
+    {
+        using G = Tile<TW, TH>;
+        static_assert(G::n % 256 == 0 && G::n <= 4096, "tile programs assume 256-cell groups of at most 4096 cells");
+        __shared__ std::uint8_t ldd[G::hn];
+        __shared__ std::uint32_t words[G::words];
+        __shared__ std::uint32_t prefix[G::words];
+        __shared__ alignas(4) std::uint8_t side[G::n];
+        std::int64_t const tile = blockIdx.x;
+        int const tid = threadIdx.x;
+        Catchment const cat = cats[tile_cat[tile]];
+        std::int64_t const local_tile = tile - cat.tile_base;
+        std::int64_t const r0 = (local_tile / cat.tiles_x) * TH;
+        std::int64_t const c0 = (local_tile % cat.tiles_x) * TW;
+        auto const g_ldd = as_global(cat.ldd);
+        for (int i = tid; i < G::hn; i += 256)
+        {
+            std::int64_t const r = r0 + i / G::hw - 1;
+            std::int64_t const c = c0 + i % G::hw - 1;
+            ldd[i] = (r >= 0 && r < cat.rows && c >= 0 && c < cat.cols) ? g_ldd[r * cat.cols + c] : no_data_ldd;
+        }
+        for (int w = tid; w < G::words; w += 256)
+            words[w] = as_global(pending_bits)[tile * G::words + w];
+        for (int w = tid; w < G::n / 4; w += 256)
+            reinterpret_cast<std::uint32_t*>(side)[w] = 0u;
+        __syncthreads();
+        auto pending = [&](int i) { return ((words[i >> 5] >> (i & 31)) & 1u) != 0u; };
+        for (int i = tid; i < G::n; i += 256)
+        {
+            int const lr = i / TW;
+            int const lc = i % TW;
+            std::uint8_t const code = ldd[(lr + 1) * G::hw + (lc + 1)];
+            bool lo = false;
+            int const dir = ldd_direction(code);
+            if (code != no_data_ldd && !pending(i) && dir >= 0)
+            {
+                int const dr = lr + neighbour_dr(dir);
+                int const dc = lc + neighbour_dc(dir);
+                if (ldd[(dr + 1) * G::hw + (dc + 1)] != no_data_ldd)
+                {
+                    if (dr < 0 || dr >= TH || dc < 0 || dc >= TW)
+                        lo = true;
+                    else if (pending(dr * TW + dc))
+                    {
+                        int const t = dr * TW + dc;
+                        atomicOr(reinterpret_cast<unsigned int*>(side) + (t >> 2),
+                                 (1u << opposite(dir)) << (8 * (t & 3)));
+                    }
+                }
+            }
+            // a wavefront's 64 consecutive cells are two bitmap words
+            std::uint64_t const bits = __ballot(lo);
+            if ((tid & 63) == 0)
+            {
+                as_global(lo_bits)[tile * G::words + (i >> 5)] = static_cast<std::uint32_t>(bits);
+                as_global(lo_bits)[tile * G::words + (i >> 5) + 1] = static_cast<std::uint32_t>(bits >> 32);
+            }
+        }
+        if (tid == 0)
+        {
+            std::uint32_t sum = 0;
+            for (int w = 0; w < G::words; ++w)
+            {
+                prefix[w] = sum;
+                sum += __popc(words[w]);
+            }
+        }
+        __syncthreads();
+        std::uint32_t const begin = as_global(tile_off)[tile];
+        for (int i = tid; i < G::n; i += 256)
+        {
+            if (!pending(i))
+                continue;
+            std::uint32_t const word = words[i >> 5];
+            std::uint32_t const b = static_cast<std::uint32_t>(i & 31);
+            std::uint32_t const rank = prefix[i >> 5] + __popc(word & ((1u << b) - 1u));
+            as_global(stage_cell)[begin + rank] = static_cast<std::uint32_t>(i) | (static_cast<std::uint32_t>(side[i]) << 16);
+        }
     }
 
 }  // namespace lue_amd
