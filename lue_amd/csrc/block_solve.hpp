@@ -41,6 +41,7 @@ namespace lue_amd {
         bool start;    // first cell of a chain
         double extra;  // side inflow (FP64)
         CellInput<OP, T> in;
+        std::uint32_t push;  // chain phase: side slot the position's outflow feeds (0xFFFFFFFF: none)
     };
 
     // Lane shift inside rows of 16 lanes (DPP row_shr:N): lane j gets the value of lane j - N of its row;

@@ -88,8 +88,10 @@ namespace lue_amd {
         std::uint8_t const* tile_flags;    // per tile and position: light children
         std::uint16_t const* tile_split;
         std::uint8_t const* tile_nlevels;
-        std::uint32_t const* lo_bits;      // per tile: local cells read by other tiles
         std::uint32_t const* stage_cell;   // per compact index: local cell | upstream mask solved in the tile << 16
+        std::uint32_t const* lo_off;       // per tile (+1): first entry of its cells that feed side slots
+        std::uint64_t const* lo_entry;     // per such cell: side slot << 16 | local cell
+        double* slot;                      // per side slot: the source's outflow (FP64)
     };
 
     template<int TW, int TH>

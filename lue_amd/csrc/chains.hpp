@@ -14,11 +14,14 @@
 //   4. the level of a chain is one more than the highest level of the chains that join it from the
 //      side (light children); 0 when none do. Any path crosses at most log2(n) light edges, so there
 //      are few levels. Chains are laid out level after level, each in flow order (upstream first);
-//   5. per chain cell, the references to the values it receives from the side: light children (their
-//      chain positions) and upstream cells in other tiles that round 0 solved (raster cells).
+//   5. per chain cell, the slots of the values it receives from the side, in neighbour order: light
+//      children (the last position of a chain of a lower level) and upstream cells in other tiles that
+//      round 0 solves (raster cells). Every source feeds exactly one slot, which it writes itself once its
+//      value is final (push): round 0 for raster cells, the chain run for chain ends.
 // Running an operation: round 0 solves the non-pending cells and stages the inputs of the pending ones
 // at their chain positions, together with the FP64 sum of their upstream cells in the same tile that
-// round 0 solved. Then one launch per level: a wavefront takes a work item (one long chain, or several
+// round 0 solved. Then per level a gather (each position with side slots adds them, contiguous, to its
+// staged sum) and a run: a wavefront takes a work item (one long chain, or several
 // short consecutive chains of at most 64 cells), 64 positions at a time, and runs the cell function
 // along the chains (scans where the function allows it, serially with lane broadcasts where it does
 // not). A final pass writes the chain results to the output rasters. Levels only exchange data across
@@ -35,9 +38,8 @@
 
 namespace lue_amd {
 
-    // A side reference: bit 63 set -> chain position in the low 32 bits; else catchment in bits 32-62 and
-    // the local cell index in the low 32 bits.
-    constexpr std::uint64_t ref_position_tag = 1ull << 63;
+    // No side slot (a position that is not the last of a light chain).
+    constexpr std::uint32_t no_slot = 0xFFFFFFFFu;
 
     template<int TW, int TH>
     __device__ inline bool cell_pending(std::uint32_t const* pending_bits, Catchment const& cat, std::int64_t r,
@@ -635,15 +637,16 @@ namespace lue_amd {
             seg_start[at] = offset + 1 == length[c] ? 1 : 0;
     }
 
-    // Side references of every pending cell, in neighbour order: pending upstream cells other than the
-    // heavy child (chain positions) and upstream cells in other tiles that round 0 solves (raster cells).
-    // Pass 1 counts (fill == false), pass 2 writes.
+    // Side slots of every pending cell, in neighbour order: pending upstream cells other than the heavy child
+    // (the last positions of light chains) and upstream cells in other tiles that round 0 solves (raster
+    // cells). Pass 1 counts (fill == false); pass 2 assigns the slots [offset[p], offset[p + 1]) of position p
+    // to its sources: push_slot[light child's position] and lo_slot[raster cell's index in the batch].
     template<int TW, int TH>
     __global__ void chains_refs_kernel(Catchment const* cats, int ncats, std::uint32_t const* pending_bits,
                                        std::uint32_t const* cells, std::uint32_t const* index_of,
                                        std::int32_t const* heavy, std::uint32_t const* pos_of, std::uint32_t n,
                                        bool fill, std::uint32_t* count, std::uint32_t const* offset,
-                                       std::uint64_t* refs)
+                                       std::uint32_t* push_slot, std::uint32_t* lo_slot)
     {
         std::uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
         if (q >= n)
@@ -668,24 +671,23 @@ namespace lue_amd {
             std::uint8_t const code = cat.ldd[u];
             if (code == no_data_ldd || ldd_direction(code) != opposite(k))
                 continue;
-            std::uint64_t ref;
             if (cell_pending<TW, TH>(pending_bits, cat, r2, c2))
             {
                 std::uint32_t const uq = index_of[cat.cell_base + u];
                 if (static_cast<std::int32_t>(uq) == h)
                     continue;  // the chain predecessor
-                ref = ref_position_tag | pos_of[uq];
+                if (fill)
+                    push_slot[pos_of[uq]] = k_out++;
             }
             else if (r2 / TH != r / TH || c2 / TW != c / TW)
             {
-                ref = (static_cast<std::uint64_t>(ci) << 32) | static_cast<std::uint64_t>(u);
+                if (fill)
+                    lo_slot[cat.cell_base + u] = k_out++;
             }
             else
             {
                 continue;  // same tile, solved by round 0: in the staged side sum
             }
-            if (fill)
-                refs[k_out++] = ref;
             ++m;
         }
         if (!fill)
@@ -702,8 +704,8 @@ namespace lue_amd {
         std::uint32_t const* item_start;  // per work item: first position
         std::uint32_t const* item_count;  // per work item: positions
         std::uint8_t const* seg_start;    // per position: 1 at the first (most upstream) cell of a chain
-        std::uint32_t const* ref_off;     // per position (+1): side references
-        std::uint64_t const* refs;
+        std::uint32_t const* push_slot;   // per position: side slot its value feeds (no_slot: none)
+        double* slot;                     // per side slot: the source's outflow (FP64), written by the source
         CellInput<OP, T> const* staged;   // per position: inputs (staged by round 0)
         double* side;                     // per position: FP64 sum of the upstream cells off the chain
         double* res;                      // per position: outflow (FP64)
@@ -742,53 +744,55 @@ namespace lue_amd {
             ref_first[i] = ref_off[n];
     }
 
-    // Add the side references of the positions ref_pos[i_begin, i_end) (those of one chain level that have
-    // any) to their staged side sums, in reference order (neighbour order). The referenced values are final:
-    // raster cells solved by round 0, and chain cells of lower levels.
+    // Add the side slots of the positions ref_pos[i_begin, i_end) (those of one chain level that have any) to
+    // their staged side sums, in slot order (neighbour order). The slots of a position are contiguous and
+    // final: written by round 0 (raster cells) and by the runs of lower levels (chain ends). Thread t of a
+    // workgroup takes entries base + t and base + t + 256, all loads in flight together.
+    constexpr std::uint32_t gather_entries_per_thread = 2;
+
     template<int OP, typename T>
     __global__ void __launch_bounds__(256)
-        chains_gather_kernel(SweepArgs<T> a, ChainRun<OP, T> c, std::uint32_t const* ref_pos,
-                             std::uint32_t const* ref_first, std::uint32_t i_begin, std::uint32_t i_end)
+        chains_gather_kernel(ChainRun<OP, T> c, std::uint32_t const* ref_pos, std::uint32_t const* ref_first,
+                             std::uint32_t i_begin, std::uint32_t i_end)
     {
-        std::uint32_t const i = i_begin + blockIdx.x * blockDim.x + threadIdx.x;
-        if (i >= i_end)
-            return;
-        std::uint32_t const p = as_global(ref_pos)[i];
-        std::uint32_t const begin = as_global(ref_first)[i];
-        std::uint32_t const end = as_global(ref_first)[i + 1];
-        auto value_of = [&](std::uint64_t ref) -> double
+        constexpr int per_thread = static_cast<int>(gather_entries_per_thread);
+        constexpr int most = 8;  // a cell has at most 8 upstream neighbours
+        std::uint32_t const base = i_begin + blockIdx.x * (256u * per_thread) + threadIdx.x;
+        std::uint32_t p[per_thread];
+        std::uint32_t begin[per_thread];
+        std::uint32_t count[per_thread];
+#pragma unroll
+        for (int j = 0; j < per_thread; ++j)
         {
-            if (ref & ref_position_tag)
-                return as_global(c.res)[static_cast<std::uint32_t>(ref)];
-            std::uint32_t const ci = static_cast<std::uint32_t>(ref >> 32);
-            std::uint32_t const u = static_cast<std::uint32_t>(ref);
-            double v = static_cast<double>(as_global(a.io[ci].out)[u]);
-            if constexpr (std::is_same_v<T, float>)
-                v += static_cast<double>(as_global(a.out_lo)[a.cats[ci].cell_base + u]);
-            return v;
-        };
-        // a cell has at most 7 light upstream neighbours: their references, then their values, all in flight
-        // together; the sum in reference order
-        constexpr int most = 8;
-        std::uint32_t const nref = end - begin;
-        std::uint64_t ref[most];
-#pragma unroll
-        for (int k = 0; k < most; ++k)
-            ref[k] = static_cast<std::uint32_t>(k) < nref ? as_global(c.refs)[begin + k] : 0ull;
-        double v[most];
-#pragma unroll
-        for (int k = 0; k < most; ++k)
-            v[k] = static_cast<std::uint32_t>(k) < nref ? value_of(ref[k]) : 0.0;
-        double sum = as_global(c.side)[p];
-#pragma unroll
-        for (int k = 0; k < most; ++k)
-        {
-            if (static_cast<std::uint32_t>(k) < nref)
-                sum += v[k];
+            std::uint32_t const i = base + 256u * j;
+            bool const ok = i < i_end;
+            p[j] = ok ? as_global(ref_pos)[i] : 0u;
+            begin[j] = ok ? as_global(ref_first)[i] : 0u;
+            count[j] = ok ? as_global(ref_first)[i + 1] - begin[j] : 0u;
         }
-        for (std::uint32_t j = begin + most; j < end; ++j)
-            sum += value_of(as_global(c.refs)[j]);
-        as_global(c.side)[p] = sum;
+        double sum[per_thread];
+        double v[per_thread][most];
+#pragma unroll
+        for (int j = 0; j < per_thread; ++j)
+        {
+            sum[j] = count[j] != 0u ? as_global(c.side)[p[j]] : 0.0;
+#pragma unroll
+            for (int k = 0; k < most; ++k)
+                v[j][k] = static_cast<std::uint32_t>(k) < count[j] ? as_global(c.slot)[begin[j] + k] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < per_thread; ++j)
+        {
+            if (count[j] == 0u)
+                continue;
+#pragma unroll
+            for (int k = 0; k < most; ++k)
+            {
+                if (static_cast<std::uint32_t>(k) < count[j])
+                    sum[j] += v[j][k];
+            }
+            as_global(c.side)[p[j]] = sum[j];
+        }
     }
 
     template<int OP, typename T>
@@ -802,6 +806,7 @@ namespace lue_amd {
             std::uint32_t const p = start + k;
             d.start = as_global(c.seg_start)[p] != 0;
             d.extra = as_global(c.side)[p];
+            d.push = as_global(c.push_slot)[p];
             d.in = c.staged[p];
         }
         return d;
@@ -849,54 +854,42 @@ namespace lue_amd {
                     as_global(c.res)[p] = value;
                     if constexpr (two_outputs)
                         as_global(c.res_rem)[p] = rem;
+                    if (d.push != no_slot)
+                        as_global(c.slot)[d.push] = value;
                 }
             }
         }
     }
 
-    // Write the chain results of the pending cells of every tile to the output rasters: thread r of a tile's
-    // workgroup takes the tile's pending ranks r, r + 256, ... (cell order), their local cells from the staging
-    // entries of the tile program (stage_cell) and their chain positions.
-    template<int OP, typename T, int TW, int TH>
+    // Write the chain results of the pending cells to the output rasters. A flat grid over the compact indices
+    // (tile after tile, cell order): thread t of a workgroup takes indices base + t, base + t + 256, ...; each
+    // index's cell in the batch and chain position are loaded, then all its results, then the stores.
+    constexpr std::uint32_t scatter_cells_per_thread = 4;
+    constexpr std::uint32_t scatter_cells_per_block = 256 * scatter_cells_per_thread;
+
+    template<int OP, typename T>
     __global__ void __launch_bounds__(256)
-        chains_scatter_kernel(SweepArgs<T> a, std::uint32_t const* tile_off, std::uint32_t const* pos_of,
-                              double const* res, T const* res_rem)
+        chains_scatter_kernel(SweepArgs<T> a, std::uint32_t const* cell_of, std::uint32_t const* pos_of,
+                              double const* res, T const* res_rem, std::uint32_t n)
     {
-        using G = Tile<TW, TH>;
         constexpr bool two_outputs = is_split_op(OP);
-        std::int64_t const tile = blockIdx.x;
-        std::uint32_t const begin = as_global(tile_off)[tile];
-        std::uint32_t const npending = as_global(tile_off)[tile + 1] - begin;
-        if (npending == 0)
-            return;
-        std::uint32_t const cat_index = as_global(a.tile_cat)[tile];
-        Catchment const cat = a.cats[cat_index];
-        CatchmentIO<T> const io = a.io[cat_index];
-        std::uint32_t const local = static_cast<std::uint32_t>(tile - cat.tile_base);
-        std::uint32_t const tiles_x = static_cast<std::uint32_t>(cat.tiles_x);
-        std::uint32_t const ty = local / tiles_x;
-        std::uint32_t const tx = local - ty * tiles_x;
-        std::uint32_t const cols32 = static_cast<std::uint32_t>(cat.cols);
-        std::int64_t const origin = static_cast<std::int64_t>(ty * TH) * cat.cols + tx * TW;
-        // all loads of a thread's ranks in flight together: entries and positions, then results (random
-        // reads), then stores
-        constexpr int per_thread = G::n / 256;
-        static_assert(G::n % 256 == 0, "the scatter runs 256 threads per tile");
+        constexpr int per_thread = static_cast<int>(scatter_cells_per_thread);
+        std::uint32_t const base = blockIdx.x * scatter_cells_per_block + threadIdx.x;
         std::uint32_t cell[per_thread];
         std::uint32_t pos[per_thread];
 #pragma unroll
         for (int j = 0; j < per_thread; ++j)
         {
-            std::uint32_t const rank = threadIdx.x + 256u * j;
-            cell[j] = rank < npending ? as_global(a.stage_cell)[begin + rank] & (G::n - 1) : 0u;
-            pos[j] = rank < npending ? as_global(pos_of)[begin + rank] : 0u;
+            std::uint32_t const q = base + 256u * j;
+            cell[j] = q < n ? as_global(cell_of)[q] : 0u;
+            pos[j] = q < n ? as_global(pos_of)[q] : 0u;
         }
         double value[per_thread];
         T remainder[per_thread];
 #pragma unroll
         for (int j = 0; j < per_thread; ++j)
         {
-            if (threadIdx.x + 256u * j < npending)
+            if (base + 256u * j < n)
             {
                 value[j] = as_global(res)[pos[j]];
                 if constexpr (two_outputs)
@@ -906,13 +899,14 @@ namespace lue_amd {
 #pragma unroll
         for (int j = 0; j < per_thread; ++j)
         {
-            if (threadIdx.x + 256u * j >= npending)
+            if (base + 256u * j >= n)
                 break;
-            std::int64_t const g = origin + static_cast<std::int64_t>(
-                                                static_cast<std::uint64_t>(cell[j] / TW) * cols32 + cell[j] % TW);
-            as_global(io.out)[g] = static_cast<T>(value[j]);
+            int const ci = a.ncats == 1 ? 0 : find_catchment(a.cats, a.ncats, cell[j]);
+            std::uint64_t const u = cell[j] - static_cast<std::uint64_t>(a.cats[ci].cell_base);
+            CatchmentIO<T> const& io = a.io[ci];
+            as_global(io.out)[u] = static_cast<T>(value[j]);
             if constexpr (two_outputs)
-                as_global(io.rem)[g] = remainder[j];
+                as_global(io.rem)[u] = remainder[j];
         }
     }
 

@@ -294,7 +294,11 @@ namespace lue_amd {
                 __syncthreads();
                 carry = next_carry;
                 if (d.valid)
+                {
                     as_global(c.res)[p] = value;
+                    if (d.push != no_slot)
+                        as_global(c.slot)[d.push] = value;
+                }
             }
             else
             {
@@ -303,7 +307,11 @@ namespace lue_amd {
                 if (outcome == kw_block_done)
                 {
                     if (d.valid)
+                    {
                         as_global(c.res)[p] = value;
+                        if (d.push != no_slot)
+                            as_global(c.slot)[d.push] = value;
+                    }
                     std::uint32_t const last = (count - base < wide_threads ? count - base : wide_threads) - 1u;
                     if (mine == last)
                         s.carry = value;
@@ -326,7 +334,11 @@ namespace lue_amd {
                             T r{};
                             solve_block<OP, T>(a, dk, nb, st, v, r);
                             if (dk.valid)
+                            {
                                 as_global(c.res)[start + block + lane] = v;
+                                if (dk.push != no_slot)
+                                    as_global(c.slot)[dk.push] = v;
+                            }
                         }
                         if (lane == 0)
                             s.carry = st.carry;

@@ -585,10 +585,9 @@ namespace lue_amd {
         DeviceArray<std::uint32_t> tile_off;     // per tile (+1): first compact index of its pending cells
         DeviceArray<std::uint32_t> pos_of;       // per compact index: chain position
         DeviceArray<std::uint8_t> seg_start;     // per position
-        DeviceArray<std::uint32_t> ref_off;      // per position (+1)
-        DeviceArray<std::uint64_t> refs;
-        DeviceArray<std::uint32_t> ref_pos;      // the positions that have side references, level after level
-        DeviceArray<std::uint32_t> ref_first;    // per entry of ref_pos (+1): its first reference (ref_off there)
+        DeviceArray<std::uint32_t> push_slot;    // per position: the side slot its outflow feeds (no_slot: none)
+        DeviceArray<std::uint32_t> ref_pos;      // the positions that have side slots, level after level
+        DeviceArray<std::uint32_t> ref_first;    // per entry of ref_pos (+1): its first side slot
         std::vector<std::uint32_t> level_ref;    // those of level l: ref_pos[level_ref[l] .. level_ref[l+1])
         DeviceArray<std::uint32_t> item_start;
         DeviceArray<std::uint32_t> item_count;
@@ -599,8 +598,10 @@ namespace lue_amd {
         DeviceArray<std::uint8_t> tile_nlevels;
         // the rest of the tiles' replay programs (tile_program_kernel)
         DeviceArray<std::uint32_t> pending_bits;  // per tile: cells left pending by round 0
-        DeviceArray<std::uint32_t> lo_bits;       // per tile: local cells read by other tiles
+        DeviceArray<std::uint32_t> lo_off;        // per tile (+1): its local cells that feed side slots
+        DeviceArray<std::uint64_t> lo_entry;      // per such cell: side slot << 16 | local cell
         DeviceArray<std::uint32_t> stage_cell;    // per compact index: local cell | solved upstream mask << 16
+        DeviceArray<std::uint32_t> scatter_cell;  // per compact index: cell index in the batch
 
         Network() = default;
         Network(Network const&) = delete;
@@ -621,8 +622,7 @@ namespace lue_amd {
             tile_off.reset();
             pos_of.reset();
             seg_start.reset();
-            ref_off.reset();
-            refs.reset();
+            push_slot.reset();
             ref_pos.reset();
             ref_first.reset();
             item_start.reset();
@@ -632,8 +632,10 @@ namespace lue_amd {
             tile_split.reset();
             tile_nlevels.reset();
             pending_bits.reset();
-            lo_bits.reset();
+            lo_off.reset();
+            lo_entry.reset();
             stage_cell.reset();
+            scatter_cell.reset();
             if (switched)
                 (void)hipSetDevice(current);
         }
@@ -651,8 +653,7 @@ namespace lue_amd {
             tile_off.stream = nullptr;
             pos_of.stream = nullptr;
             seg_start.stream = nullptr;
-            ref_off.stream = nullptr;
-            refs.stream = nullptr;
+            push_slot.stream = nullptr;
             ref_pos.stream = nullptr;
             ref_first.stream = nullptr;
             item_start.stream = nullptr;
@@ -662,8 +663,10 @@ namespace lue_amd {
             tile_split.stream = nullptr;
             tile_nlevels.stream = nullptr;
             pending_bits.stream = nullptr;
-            lo_bits.stream = nullptr;
+            lo_off.stream = nullptr;
+            lo_entry.stream = nullptr;
             stage_cell.stream = nullptr;
+            scatter_cell.stream = nullptr;
         }
     };
 
@@ -870,8 +873,10 @@ namespace lue_amd {
         a.tile_flags = nullptr;
         a.tile_split = nullptr;
         a.tile_nlevels = nullptr;
-        a.lo_bits = nullptr;
         a.stage_cell = nullptr;
+        a.lo_off = nullptr;
+        a.lo_entry = nullptr;
+        a.slot = nullptr;
         return status_success;
     }
 
@@ -982,7 +987,7 @@ namespace lue_amd {
 
     // Record the chain schedule from the pending bitmaps of round 0.
     template<int TW, int TH>
-    int build_chains(Network& net, std::uint32_t const* pending_bits, hipStream_t stream)
+    int build_chains(Network& net, std::uint32_t const* pending_bits, std::uint32_t* lo_slot, hipStream_t stream)
     {
         std::int64_t const ntiles = net.ntiles;
         int const ncats = static_cast<int>(net.cats_host.size());
@@ -1187,22 +1192,25 @@ namespace lue_amd {
         LUE_AMD_HIP(hipGetLastError());
         if (int const st = check_permutation(net.pos_of.ptr, n, "chain positions", stream))
             return st;
-        DeviceArray<std::uint32_t> ref_count;
+        DeviceArray<std::uint32_t> ref_count, ref_off;
         LUE_AMD_HIP(ref_count.allocate(n + 1, stream));
-        LUE_AMD_HIP(net.ref_off.allocate(n + 1, stream));
+        LUE_AMD_HIP(ref_off.allocate(n + 1, stream));
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, ref_count.ptr + n, 0u, 1ull);
         hipLaunchKernelGGL((chains_refs_kernel<TW, TH>), dim3(blocks_for(n)), dim3(256), 0, stream, net.cats.ptr, ncats,
                            pending_bits, cells.ptr, index_of.ptr, heavy.ptr, net.pos_of.ptr, n, false, ref_count.ptr,
-                           nullptr, nullptr);
+                           nullptr, nullptr, nullptr);
         LUE_AMD_HIP(hipGetLastError());
-        LUE_AMD_HIP(exclusive_scan(ref_count.ptr, net.ref_off.ptr, n + 1, stream));
+        LUE_AMD_HIP(exclusive_scan(ref_count.ptr, ref_off.ptr, n + 1, stream));
         std::uint32_t nrefs = 0;
-        LUE_AMD_HIP(copy_to_host(&nrefs, net.ref_off.ptr + n, 1, stream));
+        LUE_AMD_HIP(copy_to_host(&nrefs, ref_off.ptr + n, 1, stream));
         net.nrefs = nrefs;
-        LUE_AMD_HIP(net.refs.allocate(std::max<std::uint32_t>(nrefs, 1), stream));
+        // side slots: each source (a light chain's last position, a raster cell of round 0) feeds one
+        LUE_AMD_HIP(net.push_slot.allocate(n, stream));
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(n)), dim3(256), 0, stream, net.push_slot.ptr, no_slot,
+                           static_cast<std::uint64_t>(n));
         hipLaunchKernelGGL((chains_refs_kernel<TW, TH>), dim3(blocks_for(n)), dim3(256), 0, stream, net.cats.ptr, ncats,
                            pending_bits, cells.ptr, index_of.ptr, heavy.ptr, net.pos_of.ptr, n, true, nullptr,
-                           net.ref_off.ptr, net.refs.ptr);
+                           ref_off.ptr, net.push_slot.ptr, lo_slot);
         LUE_AMD_HIP(hipGetLastError());
 
         // the positions with side references, in position order (so level after level): the gather of a level
@@ -1212,7 +1220,7 @@ namespace lue_amd {
             DeviceArray<std::uint32_t> ref_rank;
             LUE_AMD_HIP(has_ref.allocate(n + 1, stream));
             LUE_AMD_HIP(ref_rank.allocate(n + 1, stream));
-            hipLaunchKernelGGL(chains_has_ref_kernel, dim3(blocks_for(n + 1)), dim3(256), 0, stream, net.ref_off.ptr, n,
+            hipLaunchKernelGGL(chains_has_ref_kernel, dim3(blocks_for(n + 1)), dim3(256), 0, stream, ref_off.ptr, n,
                                has_ref.ptr);
             LUE_AMD_HIP(hipGetLastError());
             LUE_AMD_HIP(exclusive_scan(has_ref.ptr, ref_rank.ptr, n + 1, stream));
@@ -1224,7 +1232,7 @@ namespace lue_amd {
             LUE_AMD_HIP(hipGetLastError());
             LUE_AMD_HIP(net.ref_first.allocate(nref_pos + 1, stream));
             hipLaunchKernelGGL(chains_ref_first_kernel, dim3(blocks_for(nref_pos + 1)), dim3(256), 0, stream,
-                               net.ref_pos.ptr, net.ref_off.ptr, nref_pos, n, net.ref_first.ptr);
+                               net.ref_pos.ptr, ref_off.ptr, nref_pos, n, net.ref_first.ptr);
             LUE_AMD_HIP(hipGetLastError());
             net.level_ref.assign(net.level_pos.size(), 0);
             for (std::size_t l = 0; l < net.level_pos.size(); ++l)
@@ -1237,7 +1245,8 @@ namespace lue_amd {
 
     // Record the in-tile chain layout of round 0 from the pending bitmaps.
     template<int TW, int TH>
-    int build_tile_layout(Network& net, std::uint32_t const* pending_bits, hipStream_t stream)
+    int build_tile_layout(Network& net, std::uint32_t const* pending_bits, std::uint32_t const* lo_slot,
+                          hipStream_t stream)
     {
         using G = Tile<TW, TH>;
         std::size_t const ntiles = static_cast<std::size_t>(net.ntiles);
@@ -1252,17 +1261,43 @@ namespace lue_amd {
                            net.cats.ptr, net.tile_cat.ptr, pending_bits, net.tile_order.ptr, net.tile_flags.ptr,
                            net.tile_split.ptr, net.tile_nlevels.ptr, error.ptr);
         LUE_AMD_HIP(hipGetLastError());
-        // the pending bitmaps stay with the network (the tile pass and the scatter read them), beside the lo
-        // bitmaps and the staging entries of the pending cells
+        // the pending bitmaps stay with the network (the tile pass reads them), beside the staging entries of
+        // the pending cells and the list of the local cells that feed side slots
         LUE_AMD_HIP(net.pending_bits.allocate(ntiles * G::words, stream));
-        LUE_AMD_HIP(net.lo_bits.allocate(ntiles * G::words, stream));
         LUE_AMD_HIP(net.stage_cell.allocate(std::max<std::size_t>(net.pending, 1), stream));
+        LUE_AMD_HIP(net.scatter_cell.allocate(std::max<std::size_t>(net.pending, 1), stream));
         LUE_AMD_HIP(hipMemcpyAsync(net.pending_bits.ptr, pending_bits, ntiles * G::words * sizeof(std::uint32_t),
                                    hipMemcpyDeviceToDevice, stream));
+        DeviceArray<std::uint32_t> lo_bits, lo_count;
+        LUE_AMD_HIP(lo_bits.allocate(ntiles * G::words, stream));
+        LUE_AMD_HIP(lo_count.allocate(ntiles + 1, stream));
+        LUE_AMD_HIP(net.lo_off.allocate(ntiles + 1, stream));
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, lo_count.ptr + ntiles, 0u, 1ull);
         hipLaunchKernelGGL((tile_program_kernel<TW, TH>), dim3(static_cast<unsigned>(ntiles)), dim3(256), 0, stream,
-                           net.cats.ptr, net.tile_cat.ptr, pending_bits, net.tile_off.ptr, net.lo_bits.ptr,
-                           net.stage_cell.ptr);
+                           net.cats.ptr, net.tile_cat.ptr, pending_bits, net.tile_off.ptr, lo_bits.ptr, lo_count.ptr,
+                           net.stage_cell.ptr, net.scatter_cell.ptr);
         LUE_AMD_HIP(hipGetLastError());
+        LUE_AMD_HIP(exclusive_scan(lo_count.ptr, net.lo_off.ptr, ntiles + 1, stream));
+        std::uint32_t nlo = 0;
+        LUE_AMD_HIP(copy_to_host(&nlo, net.lo_off.ptr + ntiles, 1, stream));
+        if (nlo > net.nrefs)
+            return fail(status_invalid_network, "tile programs: more side-slot sources than side slots");
+        LUE_AMD_HIP(net.lo_entry.allocate(std::max<std::uint32_t>(nlo, 1), stream));
+        if (nlo > 0)
+        {
+            DeviceArray<std::uint32_t> unassigned;
+            LUE_AMD_HIP(unassigned.allocate(1, stream));
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, unassigned.ptr, 0u, 1ull);
+            hipLaunchKernelGGL((tile_lo_list_kernel<TW, TH>), dim3(static_cast<unsigned>(ntiles)), dim3(256), 0, stream,
+                               net.cats.ptr, net.tile_cat.ptr, lo_bits.ptr, net.lo_off.ptr, lo_slot,
+                               static_cast<std::uint32_t>(net.nrefs), net.lo_entry.ptr, unassigned.ptr);
+            LUE_AMD_HIP(hipGetLastError());
+            std::uint32_t bad = 0;
+            LUE_AMD_HIP(copy_to_host(&bad, unassigned.ptr, 1, stream));
+            if (bad != 0)
+                return fail(status_invalid_network, "tile programs: a raster cell feeds no side slot");
+        }
+        LUE_AMD_HIP(hipStreamSynchronize(stream));  // before the temporaries go
         std::uint32_t errors = 0;
         LUE_AMD_HIP(copy_to_host(&errors, error.ptr, 1, stream));
         if (errors != 0)
@@ -1282,8 +1317,9 @@ namespace lue_amd {
             std::uint32_t const ref_end = net.level_ref[l + 1];
             if (ref_end > ref_begin)
             {
-                hipLaunchKernelGGL((chains_gather_kernel<OP, T>), dim3(blocks_for(ref_end - ref_begin)), dim3(256), 0,
-                                   stream, a, c, net.ref_pos.ptr, net.ref_first.ptr, ref_begin, ref_end);
+                std::uint32_t const per_block = 256u * gather_entries_per_thread;
+                hipLaunchKernelGGL((chains_gather_kernel<OP, T>), dim3((ref_end - ref_begin + per_block - 1) / per_block),
+                                   dim3(256), 0, stream, c, net.ref_pos.ptr, net.ref_first.ptr, ref_begin, ref_end);
             }
             std::uint32_t const begin = net.level_items[l];
             std::uint32_t const end = net.level_items[l + 1];
@@ -1303,8 +1339,8 @@ namespace lue_amd {
                                    begin, end);
             }
         }
-        hipLaunchKernelGGL((chains_scatter_kernel<OP, T, TW, TH>), dim3(static_cast<unsigned>(net.ntiles)), dim3(256), 0,
-                           stream, a, net.tile_off.ptr, net.pos_of.ptr, c.res, c.res_rem);
+        hipLaunchKernelGGL((chains_scatter_kernel<OP, T>), dim3((net.pending + scatter_cells_per_block - 1) / scatter_cells_per_block),
+                           dim3(256), 0, stream, a, net.scatter_cell.ptr, net.pos_of.ptr, c.res, c.res_rem, net.pending);
         LUE_AMD_HIP(hipGetLastError());
         return status_success;
     }
@@ -1356,12 +1392,13 @@ namespace lue_amd {
     {
         std::size_t pending_bits{};  // rounds schedule, per tile: pending bitmap (written by each round)
         std::size_t border_stamp{};  // per tile perimeter cell (rounds schedule only)
-        std::size_t out_lo{};        // FP32 path: compensation term per cell
+        std::size_t out_lo{};        // rounds schedule, FP32 path: compensation term per cell
         std::size_t io{};            // per catchment: CatchmentIO<T>
         std::size_t staged{};        // chain schedule, per pending cell: CellInput<OP, T>
         std::size_t side{};          //   FP64 side sum
         std::size_t res{};           //   FP64 result
         std::size_t res_rem{};       //   remainder (two-output operations)
+        std::size_t slot{};          //   per side slot: the source's FP64 outflow
         std::size_t total{};
     };
 
@@ -1382,12 +1419,13 @@ namespace lue_amd {
         std::size_t const pending = chains ? net.pending : 0;
         l.pending_bits = take(chains ? 0 : ntiles * G::words * sizeof(std::uint32_t));  // chains: the network's
         l.border_stamp = take(chains ? 0 : ntiles * G::perimeter * sizeof(std::uint32_t));
-        l.out_lo = take(std::is_same_v<T, float> ? static_cast<std::size_t>(net.ncells) * sizeof(float) : 0);
+        l.out_lo = take(!chains && std::is_same_v<T, float> ? static_cast<std::size_t>(net.ncells) * sizeof(float) : 0);
         l.io = take(net.cats_host.size() * sizeof(CatchmentIO<T>));
         l.staged = take(pending * sizeof(CellInput<OP, T>));
         l.side = take(pending * sizeof(double));
         l.res = take(pending * sizeof(double));
         l.res_rem = take(is_split_op(OP) ? pending * sizeof(T) : 0);
+        l.slot = take(chains ? static_cast<std::size_t>(net.nrefs) * sizeof(double) : 0);
         l.total = at;
         return l;
     }
@@ -1405,7 +1443,7 @@ namespace lue_amd {
         a.ncats = static_cast<int>(net.cats_host.size());
         a.ntiles = net.ntiles;
         a.io = reinterpret_cast<CatchmentIO<T> const*>(base + l.io);
-        a.out_lo = std::is_same_v<T, float> ? reinterpret_cast<float*>(base + l.out_lo) : nullptr;
+        a.out_lo = !chains && std::is_same_v<T, float> ? reinterpret_cast<float*>(base + l.out_lo) : nullptr;
         a.pending_bits = reinterpret_cast<std::uint32_t*>(base + l.pending_bits);
         a.border_stamp = chains ? nullptr : reinterpret_cast<std::uint32_t*>(base + l.border_stamp);
         a.tile_flag = nullptr;
@@ -1421,8 +1459,10 @@ namespace lue_amd {
         a.tile_flags = nullptr;
         a.tile_split = nullptr;
         a.tile_nlevels = nullptr;
-        a.lo_bits = nullptr;
         a.stage_cell = nullptr;
+        a.lo_off = nullptr;
+        a.lo_entry = nullptr;
+        a.slot = nullptr;
         ChainRun<OP, T> c{};
         if (chains)
         {
@@ -1431,8 +1471,10 @@ namespace lue_amd {
             a.tile_split = net.tile_split.ptr;
             a.tile_nlevels = net.tile_nlevels.ptr;
             a.pending_bits = net.pending_bits.ptr;
-            a.lo_bits = net.lo_bits.ptr;
             a.stage_cell = net.stage_cell.ptr;
+            a.lo_off = net.lo_off.ptr;
+            a.lo_entry = net.lo_entry.ptr;
+            a.slot = reinterpret_cast<double*>(base + l.slot);
             a.stage_off = net.tile_off.ptr;
             a.stage_pos = net.pos_of.ptr;
             if (net.pending > 0)
@@ -1442,8 +1484,8 @@ namespace lue_amd {
                 c = ChainRun<OP, T>{net.item_start.ptr,
                                     net.item_count.ptr,
                                     net.seg_start.ptr,
-                                    net.ref_off.ptr,
-                                    net.refs.ptr,
+                                    net.push_slot.ptr,
+                                    reinterpret_cast<double*>(base + l.slot),
                                     reinterpret_cast<CellInput<OP, T> const*>(base + l.staged),
                                     reinterpret_cast<double*>(base + l.side),
                                     reinterpret_cast<double*>(base + l.res),
@@ -1636,9 +1678,15 @@ namespace lue_amd {
         net.nodata_cells = static_cast<std::int64_t>(counters[1]);
         if (chains)
         {
-            if (int const st = build_chains<TW, TH>(net, w.pending_bits.ptr, stream))
+            // side slot of every raster cell that feeds one (by cell index in the batch), from the chain layout
+            // to the tile programs
+            DeviceArray<std::uint32_t> lo_slot;
+            LUE_AMD_HIP(lo_slot.allocate(static_cast<std::size_t>(net.ncells), stream));
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(static_cast<std::uint64_t>(net.ncells))), dim3(256), 0,
+                               stream, lo_slot.ptr, no_slot, static_cast<std::uint64_t>(net.ncells));
+            if (int const st = build_chains<TW, TH>(net, w.pending_bits.ptr, lo_slot.ptr, stream))
                 return st;
-            if (int const st = build_tile_layout<TW, TH>(net, w.pending_bits.ptr, stream))
+            if (int const st = build_tile_layout<TW, TH>(net, w.pending_bits.ptr, lo_slot.ptr, stream))
                 return st;
             if (counters[0] + counters[1] + net.pending != static_cast<std::uint64_t>(net.ncells))
                 return fail(status_invalid_network, "invalid flow direction network: cells not accounted for");

@@ -413,15 +413,19 @@ namespace lue_amd {
 
     // The rest of a tile's replay program, recorded once per flow network after the chain layout (tile_off: the
     // first compact index of each tile's pending cells, cell order):
-    //   lo_bits[tile][w]: bit set for a local cell whose downstream cell is a data cell in another tile; such a
-    //     cell is a side reference of a chain position there, read with its FP32 compensation term;
+    //   lo_bits[tile][w]: bit set for a local cell whose downstream cell is a data cell in another tile (that
+    //     cell is pending: such a local cell feeds one of its side slots); lo_count[tile]: their number (the
+    //     list of them, with their slots, is tile_lo_list_kernel's);
     //   stage_cell[tile_off[tile] + rank]: the rank-th pending cell of the tile (cell order) as its local index
     //     | (mask of its upstream neighbours that are local cells of the tile, bit k = neighbour k) << 16: the
-    //     tile pass adds their outflows (neighbour order) to the cell's side sum.
+    //     tile pass adds their outflows (neighbour order) to the cell's side sum;
+    //   scatter_cell[tile_off[tile] + rank]: the same cell's index in the batch (catchment cell base + row-major
+    //     index; a network has fewer than 2^32 cells), where the scatter writes its chain result.
     template<int TW, int TH>
     __global__ void __launch_bounds__(256)
         tile_program_kernel(Catchment const* cats, std::uint32_t const* tile_cat, std::uint32_t const* pending_bits,
-                            std::uint32_t const* tile_off, std::uint32_t* lo_bits, std::uint32_t* stage_cell)
+                            std::uint32_t const* tile_off, std::uint32_t* lo_bits, std::uint32_t* lo_count,
+                            std::uint32_t* stage_cell, std::uint32_t* scatter_cell)
     {
         using G = Tile<TW, TH>;
         static_assert(G::n % 256 == 0 && G::n <= 4096, "tile programs assume 256-cell groups of at most 4096 cells");
@@ -429,6 +433,7 @@ namespace lue_amd {
         __shared__ std::uint32_t words[G::words];
         __shared__ std::uint32_t prefix[G::words];
         __shared__ alignas(4) std::uint8_t side[G::n];
+        __shared__ unsigned int nlo;
         std::int64_t const tile = blockIdx.x;
         int const tid = threadIdx.x;
         Catchment const cat = cats[tile_cat[tile]];
@@ -446,6 +451,8 @@ namespace lue_amd {
             words[w] = as_global(pending_bits)[tile * G::words + w];
         for (int w = tid; w < G::n / 4; w += 256)
             reinterpret_cast<std::uint32_t*>(side)[w] = 0u;
+        if (tid == 0)
+            nlo = 0;
         __syncthreads();
         auto pending = [&](int i) { return ((words[i >> 5] >> (i & 31)) & 1u) != 0u; };
         for (int i = tid; i < G::n; i += 256)
@@ -477,6 +484,7 @@ namespace lue_amd {
             {
                 as_global(lo_bits)[tile * G::words + (i >> 5)] = static_cast<std::uint32_t>(bits);
                 as_global(lo_bits)[tile * G::words + (i >> 5) + 1] = static_cast<std::uint32_t>(bits >> 32);
+                atomicAdd(&nlo, static_cast<unsigned int>(__popcll(bits)));
             }
         }
         if (tid == 0)
@@ -489,6 +497,8 @@ namespace lue_amd {
             }
         }
         __syncthreads();
+        if (tid == 0)
+            as_global(lo_count)[tile] = nlo;
         std::uint32_t const begin = as_global(tile_off)[tile];
         for (int i = tid; i < G::n; i += 256)
         {
@@ -498,6 +508,61 @@ namespace lue_amd {
             std::uint32_t const b = static_cast<std::uint32_t>(i & 31);
             std::uint32_t const rank = prefix[i >> 5] + __popc(word & ((1u << b) - 1u));
             as_global(stage_cell)[begin + rank] = static_cast<std::uint32_t>(i) | (static_cast<std::uint32_t>(side[i]) << 16);
+            as_global(scatter_cell)[begin + rank] =
+                static_cast<std::uint32_t>(cat.cell_base + (r0 + i / TW) * cat.cols + (c0 + i % TW));
+        }
+    }
+
+    // The local cells of each tile that feed side slots (lo_bits), in cell order from lo_off[tile]: their side
+    // slot (lo_slot, by cell index in the batch, assigned with the chain layout's side slots) << 16 | local cell.
+    // A cell without a slot below nslots (none when the record is consistent) is counted in `unassigned`.
+    template<int TW, int TH>
+    __global__ void __launch_bounds__(256)
+        tile_lo_list_kernel(Catchment const* cats, std::uint32_t const* tile_cat, std::uint32_t const* lo_bits,
+                            std::uint32_t const* lo_off, std::uint32_t const* lo_slot, std::uint32_t nslots,
+                            std::uint64_t* lo_entry, std::uint32_t* unassigned)
+    {
+        using G = Tile<TW, TH>;
+        __shared__ std::uint32_t words[G::words];
+        __shared__ std::uint32_t prefix[G::words];
+        std::int64_t const tile = blockIdx.x;
+        int const tid = threadIdx.x;
+        std::uint32_t const begin = as_global(lo_off)[tile];
+        if (as_global(lo_off)[tile + 1] == begin)
+            return;
+        Catchment const cat = cats[tile_cat[tile]];
+        std::int64_t const local_tile = tile - cat.tile_base;
+        std::int64_t const r0 = (local_tile / cat.tiles_x) * TH;
+        std::int64_t const c0 = (local_tile % cat.tiles_x) * TW;
+        for (int w = tid; w < G::words; w += 256)
+            words[w] = as_global(lo_bits)[tile * G::words + w];
+        __syncthreads();
+        if (tid == 0)
+        {
+            std::uint32_t sum = 0;
+            for (int w = 0; w < G::words; ++w)
+            {
+                prefix[w] = sum;
+                sum += __popc(words[w]);
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < G::n; i += 256)
+        {
+            std::uint32_t const word = words[i >> 5];
+            std::uint32_t const b = static_cast<std::uint32_t>(i & 31);
+            if (((word >> b) & 1u) == 0u)
+                continue;
+            std::uint32_t const rank = prefix[i >> 5] + __popc(word & ((1u << b) - 1u));
+            std::int64_t const cell = cat.cell_base + (r0 + i / TW) * cat.cols + (c0 + i % TW);
+            std::uint32_t slot = as_global(lo_slot)[cell];
+            if (slot >= nslots)
+            {
+                // recorded for every source; the replay's stores stay in bounds whatever the record
+                atomicAdd(unassigned, 1u);
+                slot = 0;
+            }
+            as_global(lo_entry)[begin + rank] = (static_cast<std::uint64_t>(slot) << 16) | static_cast<std::uint64_t>(i);
         }
     }
 

@@ -9,10 +9,11 @@
 //
 // Everything that depends on the ldd alone is part of the recorded program (tile_layout.hpp): per layout
 // position the local cell, its chain start and its light children (upstream neighbours other than the chain
-// predecessor); per tile the pending bitmap and the bitmap of local cells read by other tiles; per pending cell
-// its local index and the mask of its upstream neighbours that the tile solves. A replay therefore reads no ldd
-// and derives nothing from it: per cell it loads the raw inputs and 3 bytes of layout, per pending cell 8
-// bytes of staging entries, and spends its instructions on the cell functions.
+// predecessor); per tile the pending bitmap and the list of its local cells that drain into another tile, each
+// with the side slot it feeds there; per pending cell its local index and the mask of its upstream neighbours
+// that the tile solves. A replay therefore reads no ldd and derives nothing from it: per cell it loads the raw
+// inputs and 3 bytes of layout, per pending cell 8 bytes of staging entries, and spends its instructions on
+// the cell functions.
 #pragma once
 
 #include "block_solve.hpp"
@@ -116,11 +117,12 @@ namespace lue_amd {
         std::uint32_t const r0u = ty * TH;
         std::uint32_t const c0u = tx * TW;
         std::int64_t const origin = static_cast<std::int64_t>(r0u) * cat.cols + c0u;
-        auto const out_lo = as_global(compensated ? a.out_lo + cat.cell_base : nullptr);
 
         // 1. loads: staging range, level bounds (lane l: level l of this wavefront), layout, raw inputs
         std::uint32_t const stage_begin = as_global(a.stage_off)[tile];
         std::uint32_t const stage_end = as_global(a.stage_off)[tile + 1];
+        std::uint32_t const lo_begin = as_global(a.lo_off)[tile];
+        std::uint32_t const lo_end = as_global(a.lo_off)[tile + 1];
         int split_lo = 0;
         int split_hi = 0;
         {
@@ -172,6 +174,22 @@ namespace lue_amd {
                 s.order[tid + j * nthreads] = o[j];
                 s.light[tid + j * nthreads] = f[j];
             }
+        }
+        // the tile's pending bitmap words of this thread's cells, its staging entries and its side-slot entry,
+        // all in flight with the inputs (used after the levels)
+        std::uint32_t const npending = stage_end - stage_begin;
+        bool const feeds = lo_begin + static_cast<std::uint32_t>(tid) < lo_end;
+        std::uint64_t const lo_entry = feeds ? as_global(a.lo_entry)[lo_begin + tid] : 0ull;
+        std::uint32_t cell_of[per_thread];
+        std::uint32_t pos_of[per_thread];
+        std::uint32_t pword[per_thread];
+#pragma unroll
+        for (int j = 0; j < per_thread; ++j)
+        {
+            std::uint32_t const rank = static_cast<std::uint32_t>(tid + j * nthreads);
+            cell_of[j] = rank < npending ? as_global(a.stage_cell)[stage_begin + rank] : 0u;
+            pos_of[j] = rank < npending ? as_global(a.stage_pos)[stage_begin + rank] : 0u;
+            pword[j] = as_global(a.pending_bits)[tile * G::words + ((tid + j * nthreads) >> 5)];
         }
         bool const dx_scalar = io.dx == nullptr;
 #pragma unroll
@@ -227,20 +245,9 @@ namespace lue_amd {
         }
 
         // 3. write-back of every cell of the array that is not pending (local cells, and NaN for no-data
-        // ldd cells), coalesced; compensation terms of the local cells read by other tiles. The staging
-        // entries of this thread's pending ranks are loaded first, so that they arrive meanwhile.
-        std::uint32_t const npending = stage_end - stage_begin;
-        std::uint32_t cell_of[per_thread];
-        std::uint32_t pos_of[per_thread];
-#pragma unroll
-        for (int j = 0; j < per_thread; ++j)
-        {
-            std::uint32_t const rank = static_cast<std::uint32_t>(tid + j * nthreads);
-            cell_of[j] = rank < npending ? as_global(a.stage_cell)[stage_begin + rank] : 0u;
-            pos_of[j] = rank < npending ? as_global(a.stage_pos)[stage_begin + rank] : 0u;
-        }
-        auto const pbits = as_global(a.pending_bits + tile * G::words);
-        auto const lbits = as_global(a.lo_bits + tile * G::words);
+        // ldd cells), coalesced; the FP64 outflows of the local cells that feed side slots in other tiles (at
+        // most one per perimeter cell: one per thread)
+        static_assert(G::perimeter <= nthreads, "one side-slot source per thread");
 #pragma unroll
         for (int j = 0; j < per_thread; ++j)
         {
@@ -248,19 +255,21 @@ namespace lue_amd {
             std::uint32_t const lr = static_cast<std::uint32_t>(i / TW);
             std::uint32_t const lc = static_cast<std::uint32_t>(i % TW);
             std::uint32_t const bit = 1u << (i & 31);
-            std::uint32_t const pw = pbits[i >> 5];
-            if (r0u + lr < rows32 && c0u + lc < cols32 && (pw & bit) == 0u)
+            if (r0u + lr < rows32 && c0u + lc < cols32 && (pword[j] & bit) == 0u)
             {
                 std::int64_t const g = origin + tile_offset(lr, cols32, lc);
                 g_out[g] = s.hi[i];
                 if constexpr (two_outputs)
                     g_rem[g] = s.rem[i];
-                if constexpr (compensated)
-                {
-                    if (lbits[i >> 5] & bit)
-                        out_lo[g] = s.lo[i];
-                }
             }
+        }
+        if (feeds)
+        {
+            int const i = static_cast<int>(lo_entry & (G::n - 1));
+            double v = static_cast<double>(s.hi[i]);
+            if constexpr (compensated)
+                v += static_cast<double>(s.lo[i]);
+            as_global(a.slot)[lo_entry >> 16] = v;
         }
 
         // 4. staging of the pending cells, rank r by thread r mod nthreads: inputs and the FP64 sum (neighbour
