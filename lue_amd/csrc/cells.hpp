@@ -78,7 +78,7 @@ namespace lue_amd {
         // Chain schedule, round 0 of a run: the inputs of the cells left pending are staged at their
         // chain positions (null stage_off: no staging).
         std::uint32_t const* stage_off;  // per tile (+1): first compact index of its pending cells
-        std::uint32_t const* stage_pos;  // per compact index (tile after tile, cell order): chain position
+        std::uint32_t const* stage_pos;  // per staging entry (tile after tile): chain position
         void* staged;                    // per chain position: CellInput<OP, T>
         double* stage_side;              // per chain position: FP64 sum of the same-tile upstream cells solved
 
@@ -88,7 +88,7 @@ namespace lue_amd {
         std::uint8_t const* tile_flags;    // per tile and position: light children
         std::uint16_t const* tile_split;
         std::uint8_t const* tile_nlevels;
-        std::uint32_t const* stage_cell;   // per compact index: local cell | upstream mask solved in the tile << 16
+        std::uint32_t const* stage_cell;   // per staging entry: local cell | upstream mask solved in the tile << 16
         std::uint32_t const* lo_off;       // per tile (+1): first entry of its cells that feed side slots
         std::uint64_t const* lo_entry;     // per such cell: side slot << 16 | local cell
         double* slot;                      // per side slot: the source's outflow (FP64)

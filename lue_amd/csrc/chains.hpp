@@ -708,7 +708,7 @@ namespace lue_amd {
         double* slot;                     // per side slot: the source's outflow (FP64), written by the source
         CellInput<OP, T> const* staged;   // per position: inputs (staged by round 0)
         double* side;                     // per position: FP64 sum of the upstream cells off the chain
-        double* res;                      // per position: outflow (FP64)
+        T* res;                           // per position: outflow, in the output type (read by the scatter)
         T* res_rem;                       // per position: remainder (threshold, fraction)
     };
 
@@ -851,7 +851,7 @@ namespace lue_amd {
                 if (d.valid)
                 {
                     std::uint32_t const p = start + block + lane;
-                    as_global(c.res)[p] = value;
+                    as_global(c.res)[p] = static_cast<T>(value);
                     if constexpr (two_outputs)
                         as_global(c.res_rem)[p] = rem;
                     if (d.push != no_slot)
@@ -861,52 +861,91 @@ namespace lue_amd {
         }
     }
 
-    // Write the chain results of the pending cells to the output rasters. A flat grid over the compact indices
-    // (tile after tile, cell order): thread t of a workgroup takes indices base + t, base + t + 256, ...; each
-    // index's cell in the batch and chain position are loaded, then all its results, then the stores.
-    constexpr std::uint32_t scatter_cells_per_thread = 4;
-    constexpr std::uint32_t scatter_cells_per_block = 256 * scatter_cells_per_thread;
-
-    template<int OP, typename T>
+    // Write the chain results of the pending cells of every tile to the output rasters, one workgroup per tile:
+    // the results are read in chain-position order (sorted_pos: runs of consecutive positions, so whole lines)
+    // into LDS, then written in cell order (the staging entries in cell order give the cell, sorted_rank the
+    // result's place in LDS), so that the stores of a row of the tile are one line.
+    template<int OP, typename T, int TW, int TH>
     __global__ void __launch_bounds__(256)
-        chains_scatter_kernel(SweepArgs<T> a, std::uint32_t const* cell_of, std::uint32_t const* pos_of,
-                              double const* res, T const* res_rem, std::uint32_t n)
+        chains_scatter_kernel(SweepArgs<T> a, std::uint32_t const* tile_off, std::uint32_t const* stage_cell,
+                              std::uint32_t const* sorted_pos, std::uint16_t const* sorted_rank, T const* res,
+                              T const* res_rem)
     {
+        using G = Tile<TW, TH>;
         constexpr bool two_outputs = is_split_op(OP);
-        constexpr int per_thread = static_cast<int>(scatter_cells_per_thread);
-        std::uint32_t const base = blockIdx.x * scatter_cells_per_block + threadIdx.x;
-        std::uint32_t cell[per_thread];
+        constexpr int per_thread = G::n / 256;
+        static_assert(G::n % 256 == 0, "the scatter runs 256 threads per tile");
+        __shared__ T value[G::n];
+        __shared__ T remainder[two_outputs ? G::n : 1];
+        std::int64_t const tile = blockIdx.x;
+        int const tid = static_cast<int>(threadIdx.x);
+        std::uint32_t const begin = as_global(tile_off)[tile];
+        std::uint32_t const npending = as_global(tile_off)[tile + 1] - begin;
+        if (npending == 0)
+            return;
+        // chain-position order: positions, then results, all in flight together
         std::uint32_t pos[per_thread];
 #pragma unroll
         for (int j = 0; j < per_thread; ++j)
         {
-            std::uint32_t const q = base + 256u * j;
-            cell[j] = q < n ? as_global(cell_of)[q] : 0u;
-            pos[j] = q < n ? as_global(pos_of)[q] : 0u;
+            std::uint32_t const k = static_cast<std::uint32_t>(tid + 256 * j);
+            pos[j] = k < npending ? as_global(sorted_pos)[begin + k] : 0u;
         }
-        double value[per_thread];
-        T remainder[per_thread];
+        T v[per_thread];
+        T r[per_thread];
 #pragma unroll
         for (int j = 0; j < per_thread; ++j)
         {
-            if (base + 256u * j < n)
+            std::uint32_t const k = static_cast<std::uint32_t>(tid + 256 * j);
+            if (k < npending)
             {
-                value[j] = as_global(res)[pos[j]];
+                v[j] = as_global(res)[pos[j]];
                 if constexpr (two_outputs)
-                    remainder[j] = as_global(res_rem)[pos[j]];
+                    r[j] = as_global(res_rem)[pos[j]];
             }
         }
+        // cell order: the entries of this thread's ranks (loaded while the results arrive)
+        std::uint32_t cell[per_thread];
+        std::uint32_t at[per_thread];
 #pragma unroll
         for (int j = 0; j < per_thread; ++j)
         {
-            if (base + 256u * j >= n)
+            std::uint32_t const rank = static_cast<std::uint32_t>(tid + 256 * j);
+            cell[j] = rank < npending ? as_global(stage_cell)[begin + rank] & (G::n - 1) : 0u;
+            at[j] = rank < npending ? as_global(sorted_rank)[begin + rank] & (G::n - 1) : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < per_thread; ++j)
+        {
+            std::uint32_t const k = static_cast<std::uint32_t>(tid + 256 * j);
+            if (k < npending)
+            {
+                value[k] = v[j];
+                if constexpr (two_outputs)
+                    remainder[k] = r[j];
+            }
+        }
+        __syncthreads();
+        std::uint32_t const cat_index = as_global(a.tile_cat)[tile];
+        Catchment const cat = a.cats[cat_index];
+        CatchmentIO<T> const io = a.io[cat_index];
+        std::uint32_t const local = static_cast<std::uint32_t>(tile - cat.tile_base);
+        std::uint32_t const tiles_x = static_cast<std::uint32_t>(cat.tiles_x);
+        std::uint32_t const ty = local / tiles_x;
+        std::uint32_t const tx = local - ty * tiles_x;
+        std::uint32_t const cols32 = static_cast<std::uint32_t>(cat.cols);
+        std::int64_t const origin = static_cast<std::int64_t>(ty * TH) * cat.cols + tx * TW;
+#pragma unroll
+        for (int j = 0; j < per_thread; ++j)
+        {
+            std::uint32_t const rank = static_cast<std::uint32_t>(tid + 256 * j);
+            if (rank >= npending)
                 break;
-            int const ci = a.ncats == 1 ? 0 : find_catchment(a.cats, a.ncats, cell[j]);
-            std::uint64_t const u = cell[j] - static_cast<std::uint64_t>(a.cats[ci].cell_base);
-            CatchmentIO<T> const& io = a.io[ci];
-            as_global(io.out)[u] = static_cast<T>(value[j]);
+            std::int64_t const g = origin + static_cast<std::int64_t>(
+                                                static_cast<std::uint64_t>(cell[j] / TW) * cols32 + cell[j] % TW);
+            as_global(io.out)[g] = value[at[j]];
             if constexpr (two_outputs)
-                as_global(io.rem)[u] = remainder[j];
+                as_global(io.rem)[g] = remainder[at[j]];
         }
     }
 

@@ -295,7 +295,7 @@ namespace lue_amd {
                 carry = next_carry;
                 if (d.valid)
                 {
-                    as_global(c.res)[p] = value;
+                    as_global(c.res)[p] = static_cast<T>(value);
                     if (d.push != no_slot)
                         as_global(c.slot)[d.push] = value;
                 }
@@ -308,7 +308,7 @@ namespace lue_amd {
                 {
                     if (d.valid)
                     {
-                        as_global(c.res)[p] = value;
+                        as_global(c.res)[p] = static_cast<T>(value);
                         if (d.push != no_slot)
                             as_global(c.slot)[d.push] = value;
                     }
@@ -335,7 +335,7 @@ namespace lue_amd {
                             solve_block<OP, T>(a, dk, nb, st, v, r);
                             if (dk.valid)
                             {
-                                as_global(c.res)[start + block + lane] = v;
+                                as_global(c.res)[start + block + lane] = static_cast<T>(v);
                                 if (dk.push != no_slot)
                                     as_global(c.slot)[dk.push] = v;
                             }

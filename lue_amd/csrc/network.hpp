@@ -601,7 +601,9 @@ namespace lue_amd {
         DeviceArray<std::uint32_t> lo_off;        // per tile (+1): its local cells that feed side slots
         DeviceArray<std::uint64_t> lo_entry;      // per such cell: side slot << 16 | local cell
         DeviceArray<std::uint32_t> stage_cell;    // per compact index: local cell | solved upstream mask << 16
-        DeviceArray<std::uint32_t> scatter_cell;  // per compact index: cell index in the batch
+        DeviceArray<std::uint32_t> sorted_pos;    // per tile from tile_off: its pending cells' chain positions, sorted
+        DeviceArray<std::uint32_t> sorted_cell;   //   and their staging entries in that order
+        DeviceArray<std::uint16_t> sorted_rank;   // per compact index: its place in the tile's sorted order
 
         Network() = default;
         Network(Network const&) = delete;
@@ -635,7 +637,9 @@ namespace lue_amd {
             lo_off.reset();
             lo_entry.reset();
             stage_cell.reset();
-            scatter_cell.reset();
+            sorted_pos.reset();
+            sorted_cell.reset();
+            sorted_rank.reset();
             if (switched)
                 (void)hipSetDevice(current);
         }
@@ -666,7 +670,9 @@ namespace lue_amd {
             lo_off.stream = nullptr;
             lo_entry.stream = nullptr;
             stage_cell.stream = nullptr;
-            scatter_cell.stream = nullptr;
+            sorted_pos.stream = nullptr;
+            sorted_cell.stream = nullptr;
+            sorted_rank.stream = nullptr;
         }
     };
 
@@ -1265,7 +1271,9 @@ namespace lue_amd {
         // the pending cells and the list of the local cells that feed side slots
         LUE_AMD_HIP(net.pending_bits.allocate(ntiles * G::words, stream));
         LUE_AMD_HIP(net.stage_cell.allocate(std::max<std::size_t>(net.pending, 1), stream));
-        LUE_AMD_HIP(net.scatter_cell.allocate(std::max<std::size_t>(net.pending, 1), stream));
+        LUE_AMD_HIP(net.sorted_pos.allocate(std::max<std::size_t>(net.pending, 1), stream));
+        LUE_AMD_HIP(net.sorted_cell.allocate(std::max<std::size_t>(net.pending, 1), stream));
+        LUE_AMD_HIP(net.sorted_rank.allocate(std::max<std::size_t>(net.pending, 1), stream));
         LUE_AMD_HIP(hipMemcpyAsync(net.pending_bits.ptr, pending_bits, ntiles * G::words * sizeof(std::uint32_t),
                                    hipMemcpyDeviceToDevice, stream));
         DeviceArray<std::uint32_t> lo_bits, lo_count;
@@ -1275,8 +1283,15 @@ namespace lue_amd {
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, lo_count.ptr + ntiles, 0u, 1ull);
         hipLaunchKernelGGL((tile_program_kernel<TW, TH>), dim3(static_cast<unsigned>(ntiles)), dim3(256), 0, stream,
                            net.cats.ptr, net.tile_cat.ptr, pending_bits, net.tile_off.ptr, lo_bits.ptr, lo_count.ptr,
-                           net.stage_cell.ptr, net.scatter_cell.ptr);
+                           net.stage_cell.ptr);
         LUE_AMD_HIP(hipGetLastError());
+        if (net.pending > 0)
+        {
+            hipLaunchKernelGGL((tile_sort_kernel<TW, TH>), dim3(static_cast<unsigned>(ntiles)), dim3(256), 0, stream,
+                               net.tile_off.ptr, net.pos_of.ptr, net.stage_cell.ptr, net.sorted_pos.ptr,
+                               net.sorted_cell.ptr, net.sorted_rank.ptr);
+            LUE_AMD_HIP(hipGetLastError());
+        }
         LUE_AMD_HIP(exclusive_scan(lo_count.ptr, net.lo_off.ptr, ntiles + 1, stream));
         std::uint32_t nlo = 0;
         LUE_AMD_HIP(copy_to_host(&nlo, net.lo_off.ptr + ntiles, 1, stream));
@@ -1339,8 +1354,9 @@ namespace lue_amd {
                                    begin, end);
             }
         }
-        hipLaunchKernelGGL((chains_scatter_kernel<OP, T>), dim3((net.pending + scatter_cells_per_block - 1) / scatter_cells_per_block),
-                           dim3(256), 0, stream, a, net.scatter_cell.ptr, net.pos_of.ptr, c.res, c.res_rem, net.pending);
+        hipLaunchKernelGGL((chains_scatter_kernel<OP, T, TW, TH>), dim3(static_cast<unsigned>(net.ntiles)), dim3(256), 0,
+                           stream, a, net.tile_off.ptr, net.stage_cell.ptr, net.sorted_pos.ptr, net.sorted_rank.ptr,
+                           c.res, c.res_rem);
         LUE_AMD_HIP(hipGetLastError());
         return status_success;
     }
@@ -1396,7 +1412,7 @@ namespace lue_amd {
         std::size_t io{};            // per catchment: CatchmentIO<T>
         std::size_t staged{};        // chain schedule, per pending cell: CellInput<OP, T>
         std::size_t side{};          //   FP64 side sum
-        std::size_t res{};           //   FP64 result
+        std::size_t res{};           //   result in the output type
         std::size_t res_rem{};       //   remainder (two-output operations)
         std::size_t slot{};          //   per side slot: the source's FP64 outflow
         std::size_t total{};
@@ -1423,7 +1439,7 @@ namespace lue_amd {
         l.io = take(net.cats_host.size() * sizeof(CatchmentIO<T>));
         l.staged = take(pending * sizeof(CellInput<OP, T>));
         l.side = take(pending * sizeof(double));
-        l.res = take(pending * sizeof(double));
+        l.res = take(pending * sizeof(T));
         l.res_rem = take(is_split_op(OP) ? pending * sizeof(T) : 0);
         l.slot = take(chains ? static_cast<std::size_t>(net.nrefs) * sizeof(double) : 0);
         l.total = at;
@@ -1471,12 +1487,12 @@ namespace lue_amd {
             a.tile_split = net.tile_split.ptr;
             a.tile_nlevels = net.tile_nlevels.ptr;
             a.pending_bits = net.pending_bits.ptr;
-            a.stage_cell = net.stage_cell.ptr;
+            a.stage_cell = net.sorted_cell.ptr;  // the tile pass stages in chain-position order
             a.lo_off = net.lo_off.ptr;
             a.lo_entry = net.lo_entry.ptr;
             a.slot = reinterpret_cast<double*>(base + l.slot);
             a.stage_off = net.tile_off.ptr;
-            a.stage_pos = net.pos_of.ptr;
+            a.stage_pos = net.sorted_pos.ptr;
             if (net.pending > 0)
             {
                 a.staged = base + l.staged;
@@ -1488,7 +1504,7 @@ namespace lue_amd {
                                     reinterpret_cast<double*>(base + l.slot),
                                     reinterpret_cast<CellInput<OP, T> const*>(base + l.staged),
                                     reinterpret_cast<double*>(base + l.side),
-                                    reinterpret_cast<double*>(base + l.res),
+                                    reinterpret_cast<T*>(base + l.res),
                                     is_split_op(OP) ? reinterpret_cast<T*>(base + l.res_rem) : nullptr};
             }
         }

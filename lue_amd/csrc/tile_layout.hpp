@@ -418,14 +418,13 @@ namespace lue_amd {
     //     list of them, with their slots, is tile_lo_list_kernel's);
     //   stage_cell[tile_off[tile] + rank]: the rank-th pending cell of the tile (cell order) as its local index
     //     | (mask of its upstream neighbours that are local cells of the tile, bit k = neighbour k) << 16: the
-    //     tile pass adds their outflows (neighbour order) to the cell's side sum;
-    //   scatter_cell[tile_off[tile] + rank]: the same cell's index in the batch (catchment cell base + row-major
-    //     index; a network has fewer than 2^32 cells), where the scatter writes its chain result.
+    //     tile pass adds their outflows (neighbour order) to the cell's side sum (tile_sort_kernel then lists
+    //     the same entries in chain-position order).
     template<int TW, int TH>
     __global__ void __launch_bounds__(256)
         tile_program_kernel(Catchment const* cats, std::uint32_t const* tile_cat, std::uint32_t const* pending_bits,
                             std::uint32_t const* tile_off, std::uint32_t* lo_bits, std::uint32_t* lo_count,
-                            std::uint32_t* stage_cell, std::uint32_t* scatter_cell)
+                            std::uint32_t* stage_cell)
     {
         using G = Tile<TW, TH>;
         static_assert(G::n % 256 == 0 && G::n <= 4096, "tile programs assume 256-cell groups of at most 4096 cells");
@@ -508,8 +507,69 @@ namespace lue_amd {
             std::uint32_t const b = static_cast<std::uint32_t>(i & 31);
             std::uint32_t const rank = prefix[i >> 5] + __popc(word & ((1u << b) - 1u));
             as_global(stage_cell)[begin + rank] = static_cast<std::uint32_t>(i) | (static_cast<std::uint32_t>(side[i]) << 16);
-            as_global(scatter_cell)[begin + rank] =
-                static_cast<std::uint32_t>(cat.cell_base + (r0 + i / TW) * cat.cols + (c0 + i % TW));
+        }
+    }
+
+    // The pending cells of each tile in chain-position order (a chain crossing the tile is a run of consecutive
+    // positions there), so that the tile pass stages and the scatter reads whole runs of positions: from
+    // tile_off[tile], sorted_pos[k] = the k-th smallest chain position of the tile's pending cells,
+    // sorted_cell[k] = that cell's staging entry (stage_cell), and sorted_rank[rank] = k for the cell of
+    // that rank (cell order). A bitonic sort in LDS, one workgroup per tile.
+    template<int TW, int TH>
+    __global__ void __launch_bounds__(256)
+        tile_sort_kernel(std::uint32_t const* tile_off, std::uint32_t const* pos_of, std::uint32_t const* stage_cell,
+                         std::uint32_t* sorted_pos, std::uint32_t* sorted_cell, std::uint16_t* sorted_rank)
+    {
+        using G = Tile<TW, TH>;
+        static_assert(G::n <= 65536, "ranks fit 16 bits");
+        __shared__ std::uint32_t key[G::n];
+        __shared__ std::uint16_t val[G::n];
+        std::int64_t const tile = blockIdx.x;
+        int const tid = threadIdx.x;
+        std::uint32_t const begin = as_global(tile_off)[tile];
+        int const npending = static_cast<int>(as_global(tile_off)[tile + 1] - begin);
+        if (npending == 0)
+            return;
+        int width = 1;  // the sort's width: a power of two at least npending (at most G::n)
+        while (width < npending)
+            width <<= 1;
+        for (int i = tid; i < width; i += 256)
+        {
+            key[i] = i < npending ? as_global(pos_of)[begin + i] : 0xFFFFFFFFu;
+            val[i] = static_cast<std::uint16_t>(i);
+        }
+        __syncthreads();
+        for (int k = 2; k <= width; k <<= 1)
+        {
+            for (int j = k >> 1; j > 0; j >>= 1)
+            {
+                for (int i = tid; i < width; i += 256)
+                {
+                    int const partner = i ^ j;
+                    if (partner > i)
+                    {
+                        bool const ascending = (i & k) == 0;
+                        std::uint32_t const a = key[i];
+                        std::uint32_t const b = key[partner];
+                        if ((a > b) == ascending)
+                        {
+                            key[i] = b;
+                            key[partner] = a;
+                            std::uint16_t const t = val[i];
+                            val[i] = val[partner];
+                            val[partner] = t;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int i = tid; i < npending; i += 256)
+        {
+            std::uint32_t const rank = val[i];
+            as_global(sorted_pos)[begin + i] = key[i];
+            as_global(sorted_cell)[begin + i] = as_global(stage_cell)[begin + rank];
+            as_global(sorted_rank)[begin + rank] = static_cast<std::uint16_t>(i);
         }
     }
 

@@ -272,7 +272,8 @@ namespace lue_amd {
             as_global(a.slot)[lo_entry >> 16] = v;
         }
 
-        // 4. staging of the pending cells, rank r by thread r mod nthreads: inputs and the FP64 sum (neighbour
+        // 4. staging of the pending cells, entry k by thread k mod nthreads (entries in chain-position order, so
+        // that a wavefront's stores cover runs of consecutive positions): inputs and the FP64 sum (neighbour
         // order) of their upstream cells that this tile solved, at their chain positions
         CellInput<OP, T>* const staged = static_cast<CellInput<OP, T>*>(a.staged);
 #pragma unroll
