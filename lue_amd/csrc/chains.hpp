@@ -861,91 +861,115 @@ namespace lue_amd {
         }
     }
 
-    // Write the chain results of the pending cells of every tile to the output rasters, one workgroup per tile:
-    // the results are read in chain-position order (sorted_pos: runs of consecutive positions, so whole lines)
-    // into LDS, then written in cell order (the staging entries in cell order give the cell, sorted_rank the
-    // result's place in LDS), so that the stores of a row of the tile are one line.
+    // Write the chain results of the pending cells of every tile to the output rasters. One wavefront per tile
+    // (a tile has about a third of its cells pending), scatter_tiles_per_block<...>() tiles per workgroup: the
+    // results are read in chain-position order (sorted_pos: runs of consecutive positions, so whole lines) into
+    // the wavefront's part of LDS, then written in cell order (the staging entries in cell order give the cell,
+    // sorted_rank the result's place in LDS), so that the stores of a row of the tile are one line. The tile's
+    // catchment and IO entry are loaded with its bounds, so a wavefront waits for three rounds of loads.
     template<int OP, typename T, int TW, int TH>
-    __global__ void __launch_bounds__(256)
+    constexpr int scatter_tiles_per_block()
+    {
+        constexpr int bytes = TW * TH * static_cast<int>(sizeof(T)) * (is_split_op(OP) ? 2 : 1);
+        constexpr int fit = 16384 / bytes;
+        return fit < 1 ? 1 : (fit > 4 ? 4 : fit);
+    }
+
+    template<int OP, typename T, int TW, int TH>
+    __global__ void __launch_bounds__((64 * scatter_tiles_per_block<OP, T, TW, TH>()))
         chains_scatter_kernel(SweepArgs<T> a, std::uint32_t const* tile_off, std::uint32_t const* stage_cell,
                               std::uint32_t const* sorted_pos, std::uint16_t const* sorted_rank, T const* res,
-                              T const* res_rem)
+                              T const* res_rem, std::int64_t ntiles)
     {
         using G = Tile<TW, TH>;
         constexpr bool two_outputs = is_split_op(OP);
-        constexpr int per_thread = G::n / 256;
-        static_assert(G::n % 256 == 0, "the scatter runs 256 threads per tile");
-        __shared__ T value[G::n];
-        __shared__ T remainder[two_outputs ? G::n : 1];
-        std::int64_t const tile = blockIdx.x;
-        int const tid = static_cast<int>(threadIdx.x);
+        constexpr int waves = scatter_tiles_per_block<OP, T, TW, TH>();
+        constexpr int chunk = 8;  // entries per lane with loads in flight together
+        static_assert(G::n % (64 * chunk) == 0, "tile cells a multiple of 512");
+        __shared__ T value[waves][G::n];
+        __shared__ T remainder[waves][two_outputs ? G::n : 1];
+        int const lane = static_cast<int>(threadIdx.x & 63);
+        int const wave = static_cast<int>(threadIdx.x >> 6);
+        std::int64_t const tile = static_cast<std::int64_t>(blockIdx.x) * waves + wave;
+        if (tile >= ntiles)
+            return;
         std::uint32_t const begin = as_global(tile_off)[tile];
         std::uint32_t const npending = as_global(tile_off)[tile + 1] - begin;
+        std::uint32_t const cat_index = as_global(a.tile_cat)[tile];
         if (npending == 0)
             return;
-        // chain-position order: positions, then results, all in flight together
-        std::uint32_t pos[per_thread];
-#pragma unroll
-        for (int j = 0; j < per_thread; ++j)
-        {
-            std::uint32_t const k = static_cast<std::uint32_t>(tid + 256 * j);
-            pos[j] = k < npending ? as_global(sorted_pos)[begin + k] : 0u;
-        }
-        T v[per_thread];
-        T r[per_thread];
-#pragma unroll
-        for (int j = 0; j < per_thread; ++j)
-        {
-            std::uint32_t const k = static_cast<std::uint32_t>(tid + 256 * j);
-            if (k < npending)
-            {
-                v[j] = as_global(res)[pos[j]];
-                if constexpr (two_outputs)
-                    r[j] = as_global(res_rem)[pos[j]];
-            }
-        }
-        // cell order: the entries of this thread's ranks (loaded while the results arrive)
-        std::uint32_t cell[per_thread];
-        std::uint32_t at[per_thread];
-#pragma unroll
-        for (int j = 0; j < per_thread; ++j)
-        {
-            std::uint32_t const rank = static_cast<std::uint32_t>(tid + 256 * j);
-            cell[j] = rank < npending ? as_global(stage_cell)[begin + rank] & (G::n - 1) : 0u;
-            at[j] = rank < npending ? as_global(sorted_rank)[begin + rank] & (G::n - 1) : 0u;
-        }
-#pragma unroll
-        for (int j = 0; j < per_thread; ++j)
-        {
-            std::uint32_t const k = static_cast<std::uint32_t>(tid + 256 * j);
-            if (k < npending)
-            {
-                value[k] = v[j];
-                if constexpr (two_outputs)
-                    remainder[k] = r[j];
-            }
-        }
-        __syncthreads();
-        std::uint32_t const cat_index = as_global(a.tile_cat)[tile];
         Catchment const cat = a.cats[cat_index];
         CatchmentIO<T> const io = a.io[cat_index];
+        T* const v_lds = value[wave];
+        T* const r_lds = remainder[wave];
+        // chain-position order: positions, then results, a chunk of entries per lane at a time
+        for (std::uint32_t base = 0; base < npending; base += 64 * chunk)
+        {
+            std::uint32_t pos[chunk];
+#pragma unroll
+            for (int j = 0; j < chunk; ++j)
+            {
+                std::uint32_t const k = base + static_cast<std::uint32_t>(64 * j + lane);
+                pos[j] = k < npending ? as_global(sorted_pos)[begin + k] : 0u;
+            }
+            T v[chunk];
+            T r[chunk];
+#pragma unroll
+            for (int j = 0; j < chunk; ++j)
+            {
+                std::uint32_t const k = base + static_cast<std::uint32_t>(64 * j + lane);
+                if (k < npending)
+                {
+                    v[j] = as_global(res)[pos[j]];
+                    if constexpr (two_outputs)
+                        r[j] = as_global(res_rem)[pos[j]];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < chunk; ++j)
+            {
+                std::uint32_t const k = base + static_cast<std::uint32_t>(64 * j + lane);
+                if (k < npending)
+                {
+                    v_lds[k] = v[j];
+                    if constexpr (two_outputs)
+                        r_lds[k] = r[j];
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         std::uint32_t const local = static_cast<std::uint32_t>(tile - cat.tile_base);
         std::uint32_t const tiles_x = static_cast<std::uint32_t>(cat.tiles_x);
         std::uint32_t const ty = local / tiles_x;
         std::uint32_t const tx = local - ty * tiles_x;
         std::uint32_t const cols32 = static_cast<std::uint32_t>(cat.cols);
         std::int64_t const origin = static_cast<std::int64_t>(ty * TH) * cat.cols + tx * TW;
-#pragma unroll
-        for (int j = 0; j < per_thread; ++j)
+        // cell order: entries of a chunk of ranks per lane, then the stores
+        for (std::uint32_t base = 0; base < npending; base += 64 * chunk)
         {
-            std::uint32_t const rank = static_cast<std::uint32_t>(tid + 256 * j);
-            if (rank >= npending)
-                break;
-            std::int64_t const g = origin + static_cast<std::int64_t>(
-                                                static_cast<std::uint64_t>(cell[j] / TW) * cols32 + cell[j] % TW);
-            as_global(io.out)[g] = value[at[j]];
-            if constexpr (two_outputs)
-                as_global(io.rem)[g] = remainder[at[j]];
+            std::uint32_t cell[chunk];
+            std::uint32_t at[chunk];
+#pragma unroll
+            for (int j = 0; j < chunk; ++j)
+            {
+                std::uint32_t const rank = base + static_cast<std::uint32_t>(64 * j + lane);
+                cell[j] = rank < npending ? as_global(stage_cell)[begin + rank] & (G::n - 1) : 0u;
+                at[j] = rank < npending ? as_global(sorted_rank)[begin + rank] & (G::n - 1) : 0u;
+            }
+#pragma unroll
+            for (int j = 0; j < chunk; ++j)
+            {
+                std::uint32_t const rank = base + static_cast<std::uint32_t>(64 * j + lane);
+                if (rank >= npending)
+                    break;
+                std::int64_t const g = origin + static_cast<std::int64_t>(
+                                                    static_cast<std::uint64_t>(cell[j] / TW) * cols32 + cell[j] % TW);
+                as_global(io.out)[g] = v_lds[at[j]];
+                if constexpr (two_outputs)
+                    as_global(io.rem)[g] = r_lds[at[j]];
+            }
         }
     }
 

@@ -583,7 +583,7 @@ namespace lue_amd {
         std::vector<std::uint32_t> level_longest;  // positions of the longest work item of level l
         std::vector<std::uint32_t> level_pos;    // positions of level l: [level_pos[l], level_pos[l+1])
         DeviceArray<std::uint32_t> tile_off;     // per tile (+1): first compact index of its pending cells
-        DeviceArray<std::uint32_t> pos_of;       // per compact index: chain position
+        DeviceArray<std::uint32_t> pos_of;       // per compact index: chain position (recording only)
         DeviceArray<std::uint8_t> seg_start;     // per position
         DeviceArray<std::uint32_t> push_slot;    // per position: the side slot its outflow feeds (no_slot: none)
         DeviceArray<std::uint32_t> ref_pos;      // the positions that have side slots, level after level
@@ -1292,6 +1292,7 @@ namespace lue_amd {
                                net.sorted_cell.ptr, net.sorted_rank.ptr);
             LUE_AMD_HIP(hipGetLastError());
         }
+        net.pos_of.reset();  // replays use the sorted positions
         LUE_AMD_HIP(exclusive_scan(lo_count.ptr, net.lo_off.ptr, ntiles + 1, stream));
         std::uint32_t nlo = 0;
         LUE_AMD_HIP(copy_to_host(&nlo, net.lo_off.ptr + ntiles, 1, stream));
@@ -1354,9 +1355,11 @@ namespace lue_amd {
                                    begin, end);
             }
         }
-        hipLaunchKernelGGL((chains_scatter_kernel<OP, T, TW, TH>), dim3(static_cast<unsigned>(net.ntiles)), dim3(256), 0,
-                           stream, a, net.tile_off.ptr, net.stage_cell.ptr, net.sorted_pos.ptr, net.sorted_rank.ptr,
-                           c.res, c.res_rem);
+        constexpr int tiles_per_block = scatter_tiles_per_block<OP, T, TW, TH>();
+        hipLaunchKernelGGL((chains_scatter_kernel<OP, T, TW, TH>),
+                           dim3(static_cast<unsigned>((net.ntiles + tiles_per_block - 1) / tiles_per_block)),
+                           dim3(64 * tiles_per_block), 0, stream, a, net.tile_off.ptr, net.stage_cell.ptr,
+                           net.sorted_pos.ptr, net.sorted_rank.ptr, c.res, c.res_rem, net.ntiles);
         LUE_AMD_HIP(hipGetLastError());
         return status_success;
     }
