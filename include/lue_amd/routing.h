@@ -454,9 +454,9 @@ int lue_amd_plan_kinematic_wave_f64(lue_amd_network network, const double* const
 /*
  * lue_amd_network_kinematic_wave_f32 for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid allocation). No counterpart in the reference. It
- * allocates nothing: its per-call buffers (the replay's pending bitmaps, compensation terms, staged chain
- * inputs, side sums and chain results) live in one fixed 64 MiB device array per GPU, held until the process
- * ends, which fits networks of up to about 4 million cells on the synthetic DEM (16.5 bytes per cell; a
+ * allocates nothing: its per-call buffers (the replay's staged chain inputs, side sums, chain results and
+ * side slots) live in one fixed 64 MiB device array per GPU, held until the process ends, which fits
+ * networks of up to about 5.5 million cells on the synthetic DEM (about 11.6 bytes per cell; a
  * larger network returns LUE_AMD_ERROR_INVALID_ARGUMENT with the bytes it needs) and of at most 32
  * catchments. The profiler (lue_amd_profile_*) does not time these calls. Every call on one GPU shares that
  * array: calls that overlap on one GPU (on different streams, or from different threads) race; calls on one
