@@ -345,8 +345,9 @@ typedef struct lue_amd_network_s* lue_amd_network;
 int lue_amd_network_create(int n, const uint8_t* const* ldd, const int64_t* rows, const int64_t* cols, int tile_shape,
                            int schedule, void* stream, lue_amd_network* network);
 int lue_amd_network_destroy(lue_amd_network network);
-/* info[0..11]: catchments, cells, tiles, recorded rounds, cells left pending by the first round, chains,
- * chain levels, no-data cells, tile shape, schedule, longest chain, side references */
+/* info[0..13]: catchments, cells, tiles, recorded rounds, cells left pending by the first round, chains,
+ * chain levels, no-data cells, tile shape, schedule, longest chain, side references, 64-position blocks
+ * the first round runs on the in-tile layouts, and of them the blocks on the tiles' critical paths */
 int lue_amd_network_info(lue_amd_network network, int64_t* info, int n);
 
 /*

@@ -579,6 +579,8 @@ namespace lue_amd {
         std::uint32_t nchains{0};
         std::uint32_t longest_chain{0};
         std::uint64_t nrefs{0};
+        std::uint64_t tile_blocks{0};           // 64-position blocks of the tile pass (in-tile layout)
+        std::uint64_t tile_critical_blocks{0};  // of them on the tiles' critical paths (per level the most)
         std::vector<std::uint32_t> level_items;  // work items of level l: [level_items[l], level_items[l+1])
         std::vector<std::uint32_t> level_longest;  // positions of the longest work item of level l
         std::vector<std::uint32_t> level_pos;    // positions of level l: [level_pos[l], level_pos[l+1])
@@ -1318,6 +1320,18 @@ namespace lue_amd {
         LUE_AMD_HIP(copy_to_host(&errors, error.ptr, 1, stream));
         if (errors != 0)
             return fail(status_hip_error, "in-tile chain layout: too many levels");
+        {
+            DeviceArray<unsigned long long> counts;
+            LUE_AMD_HIP(counts.allocate(2, stream));
+            LUE_AMD_HIP(hipMemsetAsync(counts.ptr, 0, 2 * sizeof(unsigned long long), stream));
+            hipLaunchKernelGGL(tile_block_stats_kernel, dim3(blocks_for(ntiles)), dim3(256), 0, stream,
+                               net.tile_split.ptr, net.tile_nlevels.ptr, static_cast<std::int64_t>(ntiles), counts.ptr);
+            LUE_AMD_HIP(hipGetLastError());
+            unsigned long long h[2] = {0, 0};
+            LUE_AMD_HIP(copy_to_host(h, counts.ptr, 2, stream));
+            net.tile_blocks = h[0];
+            net.tile_critical_blocks = h[1];
+        }
         return status_success;
     }
 

@@ -99,7 +99,9 @@ int lue_amd_network_info(lue_amd_network network, int64_t* info, int n)
                               net.tile_shape,
                               net.schedule,
                               net.longest_chain,
-                              static_cast<int64_t>(net.nrefs)};
+                              static_cast<int64_t>(net.nrefs),
+                              static_cast<int64_t>(net.tile_blocks),
+                              static_cast<int64_t>(net.tile_critical_blocks)};
     for (int i = 0; i < n && i < static_cast<int>(sizeof(values) / sizeof(values[0])); ++i)
         info[i] = values[i];
     return status_success;

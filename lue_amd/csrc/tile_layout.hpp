@@ -626,4 +626,43 @@ namespace lue_amd {
         }
     }
 
+    // Block statistics of the in-tile layouts (one thread per tile): counts[0] += the 64-position blocks the
+    // tile pass runs (over levels and wavefronts), counts[1] += the blocks on the tile's critical path (per
+    // level the most of any wavefront).
+    __global__ inline void tile_block_stats_kernel(std::uint16_t const* split, std::uint8_t const* nlevels,
+                                                   std::int64_t ntiles, unsigned long long* counts)
+    {
+        std::int64_t const tile = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+        unsigned long long blocks = 0;
+        unsigned long long critical = 0;
+        if (tile < ntiles)
+        {
+            int const levels = min(static_cast<int>(nlevels[tile]), tile_max_levels);
+            std::uint16_t const* t = split + tile * tile_split_entries;
+            for (int l = 0; l < levels; ++l)
+            {
+                unsigned most = 0;
+                for (int w = 0; w < sweep_waves; ++w)
+                {
+                    unsigned const part = static_cast<unsigned>(t[l * (sweep_waves + 1) + w + 1]) -
+                                          static_cast<unsigned>(t[l * (sweep_waves + 1) + w]);
+                    unsigned const b = (part + 63u) / 64u;
+                    blocks += b;
+                    most = b > most ? b : most;
+                }
+                critical += most;
+            }
+        }
+        for (int offset = 32; offset > 0; offset >>= 1)
+        {
+            blocks += __shfl_down(blocks, offset, 64);
+            critical += __shfl_down(critical, offset, 64);
+        }
+        if ((threadIdx.x & 63) == 0)
+        {
+            atomicAdd(&counts[0], blocks);
+            atomicAdd(&counts[1], critical);
+        }
+    }
+
 }  // namespace lue_amd

@@ -421,7 +421,7 @@ class FlowNetwork:
         import ctypes
 
         keys = ["catchments", "cells", "tiles", "rounds", "pending_cells", "chains", "levels", "no_data_cells",
-                "tile_shape", "schedule", "longest_chain", "side_refs"]
+                "tile_shape", "schedule", "longest_chain", "side_refs", "tile_blocks", "tile_critical_blocks"]
         values = (ctypes.c_int64 * len(keys))()
         _lib.check(_lib.load().lue_amd_network_info(self._handle, values, len(keys)))
         return dict(zip(keys, list(values)))
