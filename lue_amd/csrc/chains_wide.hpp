@@ -8,6 +8,10 @@
 //   - accu: each wavefront scans its 64 positions; the wavefronts' aggregates (sum, chain-start flag) are
 //     composed in order through LDS, starting from the carry. This is exactly the composition the one-wavefront
 //     kernel performs block after block, so the results are the same bits.
+//   - threshold, capacity, fraction: each wavefront scans its 64 positions (the maps of solve_block); the value
+//     entering wavefront w + 1 is the cell function of wavefront w's last position applied to the map of the
+//     positions before it and the value entering w, one after the other by one thread (16 short steps). That
+//     is the arithmetic of the one-wavefront kernel block after block, so the results are the same bits;
 //   - kinematic wave: the block Newton of kinematic_wave_block over all 64 * wide_waves positions at once (the
 //     Jacobian of the chain stays lower bidiagonal; every affine scan is a wavefront scan followed by the
 //     composition of the wavefront aggregates, and the value before each wavefront's first lane comes from the
@@ -36,7 +40,8 @@ namespace lue_amd {
     template<int OP>
     constexpr bool has_wide_path()
     {
-        return OP == op_accu || OP == op_kinematic_wave;
+        return OP == op_accu || OP == op_kinematic_wave || OP == op_threshold || OP == op_capacity ||
+               OP == op_fraction;
     }
 
     // Exchange slots of the wavefronts. Every exchange writes one buffer of a pair and is read after one
@@ -50,6 +55,13 @@ namespace lue_amd {
         double mul[2][wide_waves];
         int flag[wide_waves];        // accu: chain-start flag of the wavefront aggregate
         double carry;                // value of the last position of a group, for the next group
+        // threshold, capacity, fraction: per wavefront its last position's cell and the value entering it
+        int cell_valid[wide_waves];
+        int cell_start[wide_waves];
+        double cell_extra[wide_waves];
+        double cell_x[wide_waves];
+        double cell_p[wide_waves];
+        double entering[wide_waves];
     };
 
     constexpr int wide_not_converged = 1;
@@ -255,7 +267,7 @@ namespace lue_amd {
     __global__ void __launch_bounds__(wide_threads)
         chains_run_wide_kernel(SweepArgs<T> a, ChainRun<OP, T> c, std::uint32_t item_begin)
     {
-        static_assert(has_wide_path<OP>(), "the wide path serves accu and the kinematic wave");
+        static_assert(has_wide_path<OP>(), "the wide path serves accu, threshold, capacity, fraction and the kinematic wave");
         __shared__ WideLds s;
         int const lane = static_cast<int>(threadIdx.x & 63);
         int const wave = wide_wave();
@@ -299,6 +311,90 @@ namespace lue_amd {
                     if (d.push != no_slot)
                         as_global(c.slot)[d.push] = value;
                 }
+            }
+            else if constexpr (OP == op_threshold || OP == op_capacity || OP == op_fraction)
+            {
+                // the maps of solve_block (value before this position -> value at it) scanned per wavefront
+                int flag = d.valid && d.start ? 1 : 0;
+                double const ex = d.extra + static_cast<double>(d.in.x);
+                double const pp = static_cast<double>(d.in.p);
+                double ma;  // fraction: add, clamped sums: c
+                double mb;  // fraction: mul, clamped sums: d
+                if constexpr (OP == op_fraction)
+                {
+                    ma = d.valid ? pp * ex : 0.0;
+                    mb = d.valid ? pp : 1.0;
+                    affine_scan(ma, mb, flag);
+                }
+                else
+                {
+                    constexpr bool is_max = OP == op_threshold;
+                    mb = !d.valid ? clamp_identity<is_max>() : (is_max ? 0.0 : pp);
+                    ma = !d.valid ? 0.0 : (is_max ? ex - pp : ex);
+                    clamp_scan<is_max>(mb, ma, flag);
+                }
+                auto apply = [](double a_, double b_, int f_, double before) -> double
+                {
+                    if constexpr (OP == op_fraction)
+                        return f_ ? a_ : fma(b_, before, a_);
+                    else
+                        return extremum_nan<OP == op_threshold>(b_, (f_ ? 0.0 : before) + a_);
+                };
+                if (lane == 62)
+                {
+                    s.add[0][wave] = ma;
+                    s.mul[0][wave] = mb;
+                    s.flag[wave] = flag;
+                }
+                if (lane == 63)
+                {
+                    s.cell_valid[wave] = d.valid ? 1 : 0;
+                    s.cell_start[wave] = d.valid && d.start ? 1 : 0;
+                    s.cell_extra[wave] = d.extra;
+                    s.cell_x[wave] = static_cast<double>(d.in.x);
+                    s.cell_p[wave] = static_cast<double>(d.in.p);
+                }
+                __syncthreads();
+                if (threadIdx.x == 0)
+                {
+                    // the value entering each wavefront: its predecessor's last cell, block after block
+                    double before = carry;
+                    for (int v = 0; v < wide_waves; ++v)
+                    {
+                        s.entering[v] = before;
+                        if (!s.cell_valid[v])
+                            continue;  // the work item ends in this wavefront
+                        double const x62 = apply(s.add[0][v], s.mul[0][v], s.flag[v], before);
+                        double const up = (s.cell_start[v] ? 0.0 : x62) + s.cell_extra[v];
+                        CellInput<OP, T> const in{static_cast<T>(s.cell_x[v]), static_cast<T>(s.cell_p[v])};
+                        double out = 0.0;
+                        T r{};
+                        solve_cell<OP, T>(a, in, up, isnan(up), out, r);
+                        before = out;
+                    }
+                    s.carry = before;
+                }
+                __syncthreads();
+                double const before = s.entering[wave];
+                double const x = apply(ma, mb, flag, before);
+                double prev = wave_shr1(x);
+                if (lane == 0)
+                    prev = before;
+                if (d.start)
+                    prev = 0.0;
+                double const up = prev + d.extra;
+                if (d.valid)
+                {
+                    double out = 0.0;
+                    T r{};
+                    solve_cell<OP, T>(a, d.in, up, isnan(up), out, r);
+                    as_global(c.res)[p] = static_cast<T>(out);
+                    as_global(c.res_rem)[p] = r;
+                    if (d.push != no_slot)
+                        as_global(c.slot)[d.push] = out;
+                }
+                carry = s.carry;
+                __syncthreads();  // s is written again by the next group
             }
             else
             {

@@ -100,3 +100,60 @@ def test_long_chain_accu_against_oracle(fw, oracle_mod):
     got = fw.accu(ldd, fw.PartitionedArray.from_numpy(material)).numpy()
     want = oracle_mod.accu(ldd_np, material, None, np.float64)
     assert (np.abs(got - want) / np.maximum(np.abs(want), 1e-30)).max() <= 1e-12
+
+
+SPLIT_OPS = {
+    "threshold": (0.0, 0.05),  # inflow U[0,1]; the parameter range of each operation
+    "capacity": (0.0, 2.0),
+    "fraction": (0.0, 1.0),
+}
+
+
+@pytest.mark.parametrize("op", sorted(SPLIT_OPS))
+def test_long_chain_split_ops_against_oracle(fw, oracle_mod, op):
+    """Threshold, capacity and fraction on one flow path of 131 072 cells (its chain runs on the wide path):
+    FP64 within 1e-12 and FP32 within 1e-6 of the material entering each cell, against the FP64 oracle."""
+    rng = np.random.default_rng(47)
+    ldd_np = serpentine_ldd(32, 4096)
+    material = rng.random(ldd_np.shape)
+    lo, hi = SPLIT_OPS[op]
+    param = rng.uniform(lo, hi, ldd_np.shape)
+    ldd = fw.PartitionedArray.from_numpy(ldd_np)
+    net = fw.FlowNetwork(ldd)
+    assert has_wide_level(net)
+    w_out, w_rem = getattr(oracle_mod, f"accu_{op}")(ldd_np, material, param, None, np.float64)
+    scale = np.abs(w_out) + np.abs(w_rem) + 1e-30
+    for dtype, tol in ((np.float64, 1e-12), (np.float32, 1e-6)):
+        args = [fw.PartitionedArray.from_numpy(v.astype(dtype)) for v in (material, param)]
+        out, rem = getattr(net, f"accu_{op}")(*args)
+        err = max(np.max(np.abs(out.numpy() - w_out) / scale), np.max(np.abs(rem.numpy() - w_rem) / scale))
+        print(f"long chain {op} {np.dtype(dtype).name}: max error {err:.3e} of the entering material", flush=True)
+        assert err <= tol
+
+
+@pytest.mark.parametrize("op", sorted(SPLIT_OPS) + ["accu"])
+def test_wide_path_equals_one_wavefront_path(fw, op):
+    """A path of 1 500 cells runs on the one-wavefront kernel alone and on the wide path in a batch with a
+    path of 3 000 cells (its chain is in the same level): the same bits either way."""
+    rng = np.random.default_rng(53)
+    short_np = row_ldd(1500)
+    long_np = row_ldd(3000)
+    short = fw.PartitionedArray.from_numpy(short_np)
+    long_ = fw.PartitionedArray.from_numpy(long_np)
+    alone = fw.FlowNetwork(short)
+    batch = fw.FlowNetwork([short, long_])
+    assert not has_wide_level(alone) and has_wide_level(batch)
+    for dtype in (np.float64, np.float32):
+        mats = [fw.PartitionedArray.from_numpy(rng.random(a.shape).astype(dtype)) for a in (short_np, long_np)]
+        if op == "accu":
+            got = batch.accu(mats)[0].numpy()
+            want = alone.accu(mats[0]).numpy()
+            assert np.array_equal(got, want, equal_nan=True)
+            continue
+        lo, hi = SPLIT_OPS[op]
+        params = [fw.PartitionedArray.from_numpy(rng.uniform(lo, hi, a.shape).astype(dtype))
+                  for a in (short_np, long_np)]
+        outs, rems = getattr(batch, f"accu_{op}")(mats, params)
+        out, rem = getattr(alone, f"accu_{op}")(mats[0], params[0])
+        assert np.array_equal(outs[0].numpy(), out.numpy(), equal_nan=True)
+        assert np.array_equal(rems[0].numpy(), rem.numpy(), equal_nan=True)
