@@ -345,10 +345,30 @@ typedef struct lue_amd_network_s* lue_amd_network;
 int lue_amd_network_create(int n, const uint8_t* const* ldd, const int64_t* rows, const int64_t* cols, int tile_shape,
                            int schedule, void* stream, lue_amd_network* network);
 int lue_amd_network_destroy(lue_amd_network network);
-/* info[0..13]: catchments, cells, tiles, recorded rounds, cells left pending by the first round, chains,
+/* info[0..15]: catchments, cells, tiles, recorded rounds, cells left pending by the first round, chains,
  * chain levels, no-data cells, tile shape, schedule, longest chain, side references, 64-position blocks
- * the first round runs on the in-tile layouts, and of them the blocks on the tiles' critical paths */
+ * the first round runs on the in-tile layouts, and of them the blocks on the tiles' critical paths, the
+ * wide threshold (lue_amd_network_set_wide_threshold), the tile queue (lue_amd_network_set_tile_queue) */
 int lue_amd_network_info(lue_amd_network network, int64_t* info, int n);
+
+/*
+ * The wave schedule of the chain levels (no reference counterpart; a tuning knob of the batching addition):
+ * a level whose longest work item has at least `positions` chain positions runs one work item per workgroup
+ * of 16 wavefronts, 1024 positions at a time (accu, threshold, capacity, fraction, kinematic wave; the same
+ * results as one wavefront per work item); 0 runs every level one wavefront per work item. Default 2048.
+ * Applies to operations enqueued from then on (plans keep the schedule they captured); not while another
+ * thread enqueues operations on the network.
+ */
+int lue_amd_network_set_wide_threshold(lue_amd_network network, int64_t positions);
+
+/*
+ * The launch of the chain schedule's first round (no reference counterpart; a tuning knob of the batching
+ * addition): on = 0 launches one workgroup per tile of the batch (default), on != 0 a persistent launch of
+ * as many workgroups as the device holds at once, fed by a tile work queue (one atomic counter). The results
+ * are the same. Applies to operations enqueued from then on (plans keep the launch they captured); not while
+ * another thread enqueues operations on the network.
+ */
+int lue_amd_network_set_tile_queue(lue_amd_network network, int on);
 
 /*
  * One chain level of a network (chain schedule; no reference counterpart): info[0] chain positions,

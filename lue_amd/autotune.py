@@ -1,8 +1,12 @@
-"""Tile-shape tuning of flow networks: record one network per candidate tile shape and keep the fastest.
+"""Tuning of flow networks: tile shape, wave schedule and the launch of the first round.
 
 The schedule of a network depends on the ldd and on the tile shape: small tiles leave more cells to the
-chain phase, large ones hold fewer tiles per CU. Which is faster depends on the catchments (their sizes,
-the length of their flow paths), so a batch is timed on each candidate with the operation it will run.
+chain phase, large ones hold fewer tiles per CU. The wave schedule of the chain levels (which levels run one
+work item per workgroup of 16 wavefronts instead of one per wavefront) is a threshold on a level's longest
+work item, and the first round runs one workgroup per tile or as a persistent grid fed by a tile work queue.
+Which is faster depends on the catchments (their sizes, the length of their flow paths), so a batch is timed
+on each candidate with the operation it will run: one network is recorded per tile shape and replayed with
+every threshold and launch (they need no new recording).
 """
 from __future__ import annotations
 
@@ -14,14 +18,30 @@ from . import _lib
 from .framework import TILE_SHAPES, FlowNetwork
 
 CANDIDATES = ((32, 32), (64, 32), (64, 64))
+WIDE_THRESHOLDS = (1024, 2048, 4096, 0)
+TILE_QUEUE = (False, True)
 
 
-def tune(flow_directions, run, candidates=CANDIDATES, schedule="chains", repeats=3):
-    """Record a network per candidate tile shape, time `run(network)` (after one untimed call) and return
-    (best network, table of {tile_shape: milliseconds}). Shapes whose operation does not fit the tile in
-    LDS are skipped; the other networks are released."""
+def _time(run, net, repeats):
+    run(net)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        run(net)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / repeats * 1e3
+
+
+def tune(flow_directions, run, candidates=CANDIDATES, schedule="chains", repeats=3, wide_thresholds=WIDE_THRESHOLDS,
+         tile_queue=TILE_QUEUE):
+    """Record a network per candidate tile shape, time `run(network)` (after one untimed call) for each wide
+    threshold and first-round launch, and return (best network, set to its best threshold and launch; table
+    of {tile_shape: milliseconds} with each shape's best time; table of {(tile_shape, threshold, tile_queue):
+    milliseconds}). Shapes whose operation does not fit the tile in LDS are skipped; the other networks are
+    released."""
     best = None
     table = {}
+    schedule_table = {}
     for shape in candidates:
         if tuple(shape) not in TILE_SHAPES:
             raise ValueError(f"unknown tile shape {shape}")
@@ -33,14 +53,19 @@ def tune(flow_directions, run, candidates=CANDIDATES, schedule="chains", repeats
                 net.close()
                 continue
             raise
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(repeats):
-            run(net)
-        torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) / repeats * 1e3
-        table[tuple(shape)] = ms
-        if best is None or ms < table[best[0]]:
+        shape_best = None
+        for queue in tile_queue or (False,):
+            net.set_tile_queue(queue)
+            for threshold in wide_thresholds or (net.info()["wide_threshold"],):
+                net.set_wide_threshold(threshold)
+                ms = _time(run, net, repeats)
+                schedule_table[(tuple(shape), threshold, queue)] = ms
+                if shape_best is None or ms < shape_best[2]:
+                    shape_best = (threshold, queue, ms)
+        net.set_wide_threshold(shape_best[0])
+        net.set_tile_queue(shape_best[1])
+        table[tuple(shape)] = shape_best[2]
+        if best is None or shape_best[2] < table[best[0]]:
             if best is not None:
                 best[1].close()
             best = (tuple(shape), net)
@@ -48,4 +73,4 @@ def tune(flow_directions, run, candidates=CANDIDATES, schedule="chains", repeats
             net.close()
     if best is None:
         raise _lib.RoutingError(_lib.STATUS_INVALID_ARGUMENT, "no candidate tile shape fits the operation")
-    return best[1], table
+    return best[1], table, schedule_table

@@ -92,7 +92,30 @@ namespace lue_amd {
         std::uint32_t const* lo_off;       // per tile (+1): first entry of its cells that feed side slots
         std::uint64_t const* lo_entry;     // per such cell: side slot << 16 | local cell
         double* slot;                      // per side slot: the source's outflow (FP64)
+
+        // A network of one catchment: that catchment and its rasters, so that kernels need not load them
+        // (one_catchment set; otherwise cats, io and tile_cat hold them)
+        int one_catchment;
+        Catchment cat0;
+        CatchmentIO<T> io0;
     };
+
+    // Catchment index, catchment and rasters of a tile.
+    template<typename T>
+    __device__ inline std::uint32_t tile_catchment(SweepArgs<T> const& a, std::int64_t tile, Catchment& cat,
+                                                   CatchmentIO<T>& io)
+    {
+        if (a.one_catchment)
+        {
+            cat = a.cat0;
+            io = a.io0;
+            return 0u;
+        }
+        std::uint32_t const index = as_global(a.tile_cat)[tile];
+        cat = a.cats[index];
+        io = a.io[index];
+        return index;
+    }
 
     template<int TW, int TH>
     struct Tile

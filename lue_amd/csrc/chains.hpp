@@ -895,11 +895,11 @@ namespace lue_amd {
             return;
         std::uint32_t const begin = as_global(tile_off)[tile];
         std::uint32_t const npending = as_global(tile_off)[tile + 1] - begin;
-        std::uint32_t const cat_index = as_global(a.tile_cat)[tile];
+        Catchment cat;
+        CatchmentIO<T> io;
+        tile_catchment(a, tile, cat, io);
         if (npending == 0)
             return;
-        Catchment const cat = a.cats[cat_index];
-        CatchmentIO<T> const io = a.io[cat_index];
         T* const v_lds = value[wave];
         T* const r_lds = remainder[wave];
         // chain-position order: positions, then results, a chunk of entries per lane at a time

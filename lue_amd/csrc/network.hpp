@@ -579,6 +579,11 @@ namespace lue_amd {
         std::uint32_t nchains{0};
         std::uint32_t longest_chain{0};
         std::uint64_t nrefs{0};
+        // chain levels whose longest work item has at least this many positions run on the wide path (0: none)
+        std::uint32_t wide_min{wide_min_positions};
+        // the tile pass as a persistent launch fed by a tile work queue (tile_pass_queue_kernel)
+        bool tile_queue{false};
+        int cus{1};  // compute units of the device
         std::uint64_t tile_blocks{0};           // 64-position blocks of the tile pass (in-tile layout)
         std::uint64_t tile_critical_blocks{0};  // of them on the tiles' critical paths (per level the most)
         std::vector<std::uint32_t> level_items;  // work items of level l: [level_items[l], level_items[l+1])
@@ -745,6 +750,7 @@ namespace lue_amd {
         tile_extent(tile_shape, tw, th);
         LUE_AMD_HIP(hipGetDevice(&net.device));
         LUE_AMD_HIP(keep_pool_memory(net.device));
+        LUE_AMD_HIP(hipDeviceGetAttribute(&net.cus, hipDeviceAttributeMultiprocessorCount, net.device));
         net.tile_shape = tile_shape;
         net.schedule = schedule;
         net.cats_host.clear();
@@ -824,6 +830,15 @@ namespace lue_amd {
         return status_success;
     }
 
+    // A network of one catchment passes the catchment and its rasters to the kernels as arguments.
+    template<typename T>
+    void set_one_catchment(Network const& net, CatchmentIO<T> const* io_host, SweepArgs<T>& a)
+    {
+        a.one_catchment = net.cats_host.size() == 1 ? 1 : 0;
+        a.cat0 = a.one_catchment ? net.cats_host[0] : Catchment{};
+        a.io0 = a.one_catchment ? io_host[0] : CatchmentIO<T>{};
+    }
+
     // Scratch of recording an operation on a network (round 0 with walks, and the rounds of the rounds
     // schedule).
     struct RecordWorkspace
@@ -860,6 +875,7 @@ namespace lue_amd {
         if (int const st = write_io<T>(io.data(), static_cast<int>(io.size()),
                                        reinterpret_cast<CatchmentIO<T>*>(w.io.ptr), stream))
             return st;
+        set_one_catchment(net, io.data(), a);
         a.cats = net.cats.ptr;
         a.tile_cat = net.tile_cat.ptr;
         a.ncats = static_cast<int>(net.cats_host.size());
@@ -1355,8 +1371,9 @@ namespace lue_amd {
             std::uint32_t const end = net.level_items[l + 1];
             if (end == begin)
                 continue;
-            bool const wide = has_wide_path<OP>() && static_cast<std::size_t>(l) < net.level_longest.size() &&
-                              net.level_longest[static_cast<std::size_t>(l)] >= wide_min_positions;
+            bool const wide = has_wide_path<OP>() && net.wide_min > 0 &&
+                              static_cast<std::size_t>(l) < net.level_longest.size() &&
+                              net.level_longest[static_cast<std::size_t>(l)] >= net.wide_min;
             if (wide)
             {
                 if constexpr (has_wide_path<OP>())
@@ -1432,6 +1449,7 @@ namespace lue_amd {
         std::size_t res{};           //   result in the output type
         std::size_t res_rem{};       //   remainder (two-output operations)
         std::size_t slot{};          //   per side slot: the source's FP64 outflow
+        std::size_t next_tile{};     //   the tile work queue's counter (tile_queue)
         std::size_t total{};
     };
 
@@ -1459,6 +1477,7 @@ namespace lue_amd {
         l.res = take(pending * sizeof(T));
         l.res_rem = take(is_split_op(OP) ? pending * sizeof(T) : 0);
         l.slot = take(chains ? static_cast<std::size_t>(net.nrefs) * sizeof(double) : 0);
+        l.next_tile = take(chains && net.tile_queue ? sizeof(std::uint32_t) : 0);
         l.total = at;
         return l;
     }
@@ -1467,10 +1486,11 @@ namespace lue_amd {
     // already written). No device allocation and no host synchronisation; with `timed` false not even the
     // profiler's events are taken.
     template<int OP, typename T, int TW, int TH>
-    int replay_launch(Network const& net, SweepArgs<T> a, unsigned char* base, ReplayLayout const& l,
-                      hipStream_t stream, bool timed)
+    int replay_launch(Network const& net, SweepArgs<T> a, CatchmentIO<T> const* io_host, unsigned char* base,
+                      ReplayLayout const& l, hipStream_t stream, bool timed)
     {
         bool const chains = net.schedule == schedule_chains;
+        set_one_catchment(net, io_host, a);
         a.cats = net.cats.ptr;
         a.tile_cat = net.tile_cat.ptr;
         a.ncats = static_cast<int>(net.cats_host.size());
@@ -1532,7 +1552,18 @@ namespace lue_amd {
             if (int const st = timer.start(stream))
                 return st;
         }
-        if (chains)
+        if (chains && net.tile_queue)
+        {
+            // as many workgroups as the device holds at once, the counter cleared by a kernel (graph capture)
+            std::uint32_t* const next_tile = reinterpret_cast<std::uint32_t*>(base + l.next_tile);
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, next_tile, 0u, 1ull);
+            std::int64_t const resident =
+                static_cast<std::int64_t>(std::max(net.cus, 1)) * tile_pass_workgroups_per_cu<OP, T, TW, TH>();
+            hipLaunchKernelGGL((tile_pass_queue_kernel<OP, T, TW, TH>),
+                               dim3(static_cast<unsigned>(std::min<std::int64_t>(net.ntiles, resident))),
+                               dim3(sweep_threads), 0, stream, a, next_tile);
+        }
+        else if (chains)
             hipLaunchKernelGGL((tile_pass_kernel<OP, T, TW, TH>), dim3(static_cast<unsigned>(net.ntiles)),
                                dim3(sweep_threads), 0, stream, a);
         else
@@ -1614,7 +1645,7 @@ namespace lue_amd {
         int st = status_success;
         if (e == hipSuccess)
         {
-            st = replay_launch<OP, T, TW, TH>(net, a, plan.buffer, l, s, false);
+            st = replay_launch<OP, T, TW, TH>(net, a, io.data(), plan.buffer, l, s, false);
             e = hipStreamEndCapture(s, &plan.graph);  // ends the capture whatever the launches returned
         }
         (void)hipStreamDestroy(s);
@@ -1641,7 +1672,7 @@ namespace lue_amd {
         if (int const st = write_io<T>(io.data(), static_cast<int>(io.size()),
                                        reinterpret_cast<CatchmentIO<T>*>(block.ptr + l.io), stream))
             return st;
-        return replay_launch<OP, T, TW, TH>(net, a, block.ptr, l, stream, true);
+        return replay_launch<OP, T, TW, TH>(net, a, io.data(), block.ptr, l, stream, true);
     }
 
     // Record: run round 0 (and, for the rounds schedule, the later rounds) while recording the schedule into

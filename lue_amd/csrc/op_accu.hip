@@ -101,7 +101,9 @@ int lue_amd_network_info(lue_amd_network network, int64_t* info, int n)
                               net.longest_chain,
                               static_cast<int64_t>(net.nrefs),
                               static_cast<int64_t>(net.tile_blocks),
-                              static_cast<int64_t>(net.tile_critical_blocks)};
+                              static_cast<int64_t>(net.tile_critical_blocks),
+                              static_cast<int64_t>(net.wide_min),
+                              net.tile_queue ? 1 : 0};
     for (int i = 0; i < n && i < static_cast<int>(sizeof(values) / sizeof(values[0])); ++i)
         info[i] = values[i];
     return status_success;
@@ -156,6 +158,24 @@ int lue_amd_plan_accu_f64(lue_amd_network network, const double* const* inflow, 
                           double* const* outflow, lue_amd_plan* plan)
 {
     return make_plan(plan, [&](Plan& p) { return network_accu<double>(network, inflow, inflow_scalar, outflow, nullptr, &p); });
+}
+
+int lue_amd_network_set_wide_threshold(lue_amd_network network, int64_t positions)
+{
+    if (!network)
+        return fail(status_invalid_argument, "null argument");
+    if (positions < 0 || positions > INT32_MAX)
+        return fail(status_invalid_argument, "wide threshold out of range");
+    network->net.wide_min = static_cast<std::uint32_t>(positions);
+    return status_success;
+}
+
+int lue_amd_network_set_tile_queue(lue_amd_network network, int on)
+{
+    if (!network)
+        return fail(status_invalid_argument, "null argument");
+    network->net.tile_queue = on != 0;
+    return status_success;
 }
 
 int lue_amd_network_level_info(lue_amd_network network, int level, int64_t* info, int n)

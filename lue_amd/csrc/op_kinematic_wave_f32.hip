@@ -150,7 +150,7 @@ int lue_amd_network_kinematic_wave_f32_noalloc(lue_amd_network network, const fl
             return st;
         if (int const st = write_io<float>(io, ncats, reinterpret_cast<CatchmentIO<float>*>(base + l.io), s))
             return st;
-        return replay_launch<op_kinematic_wave, float, TW, TH>(net, args_from(sc), base, l, s, false);
+        return replay_launch<op_kinematic_wave, float, TW, TH>(net, args_from(sc), io, base, l, s, false);
     });
 }
 

@@ -117,9 +117,9 @@ namespace lue_amd {
 
         int const tid = threadIdx.x;
         constexpr int nthreads = sweep_threads;  // every tile kernel is launched with sweep_threads threads
-        std::uint32_t const cat_index = as_global(a.tile_cat)[tile];
-        Catchment const cat = a.cats[cat_index];
-        CatchmentIO<T> const io = a.io[cat_index];
+        Catchment cat;
+        CatchmentIO<T> io;
+        tile_catchment(a, tile, cat, io);
         auto const g_ldd = as_global(cat.ldd);
         auto const g_out = as_global(io.out);
         auto const g_rem = as_global(io.rem);

@@ -83,10 +83,9 @@ namespace lue_amd {
         return tile_pass_workgroups_per_cu<OP, T, TW, TH>() * sweep_waves / 4;
     }
 
-    // One workgroup (sweep_threads threads) per tile of the batch; grid = number of tiles.
+    // One tile; all sweep_threads threads of the workgroup call it.
     template<int OP, typename T, int TW, int TH>
-    __global__ void __launch_bounds__(sweep_threads, (tile_pass_waves_per_simd<OP, T, TW, TH>()))
-        tile_pass_kernel(SweepArgs<T> a)
+    __device__ inline void tile_pass_tile(SweepArgs<T> const& a, TilePassLds<OP, T, TW, TH>& s, std::int64_t tile)
     {
         using G = Tile<TW, TH>;
         using Lds = TilePassLds<OP, T, TW, TH>;
@@ -96,14 +95,12 @@ namespace lue_amd {
         constexpr int per_thread = G::n / nthreads;
         static_assert(G::n % nthreads == 0 && (G::n & (G::n - 1)) == 0 && G::n <= 4096,
                       "tiles of a power of two cells, a multiple of the workgroup, at most 4096");
-        __shared__ Lds s;
 
-        std::int64_t const tile = blockIdx.x;
         int const tid = threadIdx.x;
         int const lane_id = tid & 63;
-        std::uint32_t const cat_index = as_global(a.tile_cat)[tile];
-        Catchment const cat = a.cats[cat_index];
-        CatchmentIO<T> const io = a.io[cat_index];
+        Catchment cat;
+        CatchmentIO<T> io;
+        tile_catchment(a, tile, cat, io);
         auto const g_out = as_global(io.out);
         auto const g_rem = as_global(io.rem);
         // tile coordinates (32-bit: a catchment has fewer than 2^32 tiles, at most 2^31 rows and columns); a
@@ -286,6 +283,37 @@ namespace lue_amd {
             double const side = tile_neighbour_sum<OP, T, TW, TH>(s, i, (cell_of[j] >> 16) & 0xFFu);
             staged[pos_of[j]] = s.in[i];
             as_global(a.stage_side)[pos_of[j]] = side;
+        }
+    }
+
+    // One workgroup (sweep_threads threads) per tile of the batch; grid = number of tiles.
+    template<int OP, typename T, int TW, int TH>
+    __global__ void __launch_bounds__(sweep_threads, (tile_pass_waves_per_simd<OP, T, TW, TH>()))
+        tile_pass_kernel(SweepArgs<T> a)
+    {
+        __shared__ TilePassLds<OP, T, TW, TH> s;
+        tile_pass_tile<OP, T, TW, TH>(a, s, blockIdx.x);
+    }
+
+    // The same as a persistent launch fed by a tile work queue: as many workgroups as the device holds at once,
+    // each taking the next tile from `next_tile` (zero at the launch; one atomic per tile) until none is left.
+    // Every workgroup reaches the end: a tile index at or past ntiles ends its loop.
+    template<int OP, typename T, int TW, int TH>
+    __global__ void __launch_bounds__(sweep_threads, (tile_pass_waves_per_simd<OP, T, TW, TH>()))
+        tile_pass_queue_kernel(SweepArgs<T> a, std::uint32_t* next_tile)
+    {
+        __shared__ TilePassLds<OP, T, TW, TH> s;
+        __shared__ std::uint32_t taken;
+        while (true)
+        {
+            if (threadIdx.x == 0)
+                taken = atomicAdd(next_tile, 1u);
+            __syncthreads();
+            std::int64_t const tile = taken;
+            if (tile >= a.ntiles)
+                break;
+            tile_pass_tile<OP, T, TW, TH>(a, s, tile);
+            __syncthreads();  // the LDS of this tile is read to the end before the next one is staged
         }
     }
 

@@ -421,10 +421,23 @@ class FlowNetwork:
         import ctypes
 
         keys = ["catchments", "cells", "tiles", "rounds", "pending_cells", "chains", "levels", "no_data_cells",
-                "tile_shape", "schedule", "longest_chain", "side_refs", "tile_blocks", "tile_critical_blocks"]
+                "tile_shape", "schedule", "longest_chain", "side_refs", "tile_blocks", "tile_critical_blocks",
+                "wide_threshold", "tile_queue"]
         values = (ctypes.c_int64 * len(keys))()
         _lib.check(_lib.load().lue_amd_network_info(self._handle, values, len(keys)))
         return dict(zip(keys, list(values)))
+
+    def set_wide_threshold(self, positions):
+        """Chain levels whose longest work item has at least `positions` positions run one work item per
+        workgroup of 16 wavefronts (0: none); the results are the same either way. Applies to the
+        operations and plans made from then on."""
+        _lib.check(_lib.load().lue_amd_network_set_wide_threshold(self._handle, int(positions)))
+
+    def set_tile_queue(self, on):
+        """Launch the first round as a persistent grid fed by a tile work queue (True) or one workgroup per
+        tile (False, the default); the results are the same. Applies to the operations and plans made from
+        then on."""
+        _lib.check(_lib.load().lue_amd_network_set_tile_queue(self._handle, 1 if on else 0))
 
     def level_info(self):
         """Per chain level: positions, work items, positions of the longest work item, side references."""

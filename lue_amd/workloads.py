@@ -61,7 +61,7 @@ def batch_threshold(seed=20261205, count=512, side_lo=64, side_hi=2048, repeats=
     cells = sum(r * c for r, c in shapes)
     _sync()
     t0 = time.perf_counter()
-    net, table = autotune.tune(ldds, lambda n: n.accu_threshold(mats, thrs), repeats=repeats)
+    net, table, schedule_table = autotune.tune(ldds, lambda n: n.accu_threshold(mats, thrs), repeats=repeats)
     _sync()
     tune_s = time.perf_counter() - t0
     outs = net.accu_threshold(mats, thrs)
@@ -89,7 +89,11 @@ def batch_threshold(seed=20261205, count=512, side_lo=64, side_hi=2048, repeats=
             "catchments": count, "cells": cells, "ms": round(ms, 3), "mcells_per_s": round(cells / ms / 1e3, 1),
             "graph": f"one graph launch of {launches} kernel launches", "eager_ms": round(eager_ms, 3),
             "tile_shape_ms": {f"{k[0]}x{k[1]}": round(v, 3) for k, v in table.items()},
-            "tile_shape": info["tile_shape"], "tune_s": round(tune_s, 2), "levels": info["levels"],
+            "wave_schedule_ms": {f"{k[0][0]}x{k[0][1]} wide>={k[1] if k[1] else 'off'}"
+                                 f"{' queue' if k[2] else ''}": round(v, 3) for k, v in schedule_table.items()},
+            "tile_shape": info["tile_shape"], "wide_threshold": info["wide_threshold"],
+            "tile_queue": info["tile_queue"],
+            "tune_s": round(tune_s, 2), "levels": info["levels"],
             "pending_cells": info["pending_cells"]}
 
 

@@ -45,7 +45,7 @@ def test_autotuned_batch_accu_threshold_matches_oracle(fw, oracle_mod):
         ldds.append(fw.PartitionedArray.from_numpy(ldd))
         mats.append(fw.PartitionedArray.from_numpy(mat))
         thrs.append(fw.PartitionedArray.from_numpy(thr))
-    net, table = autotune.tune(ldds, lambda n: n.accu_threshold(mats, thrs))
+    net, table, _ = autotune.tune(ldds, lambda n: n.accu_threshold(mats, thrs))
     assert len(table) >= 2 and net.info()["catchments"] == len(shapes)
     outs, rems = net.accu_threshold(mats, thrs)
     shape = {0: (32, 32), 1: (64, 32), 2: (64, 64)}[net.info()["tile_shape"]]

@@ -136,7 +136,7 @@ def test_batch_512_catchments_accu_threshold(fw, oracle_mod):
                                                            dev)))
         thrs.append(fw.PartitionedArray(syn.uniform_device(s, syn.STREAM_THRESHOLD, 0.0, 0.5, (r, c),
                                                            torch.float32, dev)))
-    net, _ = autotune.tune(ldds, lambda n: n.accu_threshold(mats, thrs), repeats=1)
+    net, _, _ = autotune.tune(ldds, lambda n: n.accu_threshold(mats, thrs), repeats=1)
     assert net.info()["catchments"] == 512
     outs, rems = net.accu_threshold(mats, thrs)
     torch.cuda.synchronize()

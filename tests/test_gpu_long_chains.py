@@ -157,3 +157,71 @@ def test_wide_path_equals_one_wavefront_path(fw, op):
         out, rem = getattr(alone, f"accu_{op}")(mats[0], params[0])
         assert np.array_equal(outs[0].numpy(), out.numpy(), equal_nan=True)
         assert np.array_equal(rems[0].numpy(), rem.numpy(), equal_nan=True)
+
+
+def test_wide_threshold_changes_the_schedule_not_the_results(fw):
+    """The wave schedule (lue_amd_network_set_wide_threshold): every level on one wavefront per work item
+    (threshold 0) gives the bits of the default schedule, whose last levels run wide."""
+    from lue_amd import _lib
+
+    rng = np.random.default_rng(59)
+    ldd_np = serpentine_ldd(32, 4096)
+    ldd = fw.PartitionedArray.from_numpy(ldd_np)
+    net = fw.FlowNetwork(ldd)
+    assert has_wide_level(net) and net.info()["wide_threshold"] == WIDE_MIN_POSITIONS
+    a = fw.PartitionedArray.from_numpy(rng.random(ldd_np.shape).astype(np.float32))
+    b = fw.PartitionedArray.from_numpy(rng.uniform(0.0, 0.05, ldd_np.shape).astype(np.float32))
+    q = [fw.PartitionedArray.from_numpy(rng.uniform(lo, hi, ldd_np.shape).astype(np.float32))
+         for lo, hi in ((0.0, 5.0), (0.0, 1e-4), (1.5, 2.5))]
+
+    def results():
+        out = [net.accu(a).numpy()]
+        for op in ("threshold", "capacity", "fraction"):
+            out.extend(x.numpy() for x in getattr(net, f"accu_{op}")(a, b))
+        out.append(net.kinematic_wave(*q, 0.6, 3600.0, 100.0).numpy())
+        return out
+
+    wide = results()
+    net.set_wide_threshold(0)
+    assert net.info()["wide_threshold"] == 0
+    narrow = results()
+    for w, n in zip(wide, narrow):
+        assert np.array_equal(w, n, equal_nan=True)
+    with pytest.raises(_lib.RoutingError):
+        net.set_wide_threshold(-1)
+
+
+def test_tile_queue_gives_the_same_results(fw):
+    """The first round as a persistent launch fed by a tile work queue (lue_amd_network_set_tile_queue) gives
+    the bits of one workgroup per tile, for a batch of synthetic catchments and for a plan."""
+    from lue_amd import synthetic as syn
+
+    shapes = [(300, 520), (96, 64), (700, 410)]
+    ldds = [fw.PartitionedArray.from_numpy(syn.ldd_host(61 + i, r, c)) for i, (r, c) in enumerate(shapes)]
+    mats = [fw.PartitionedArray.from_numpy(syn.uniform_host(61 + i, syn.STREAM_MATERIAL, 0.0, 1.0, s))
+            for i, s in enumerate(shapes)]
+    thrs = [fw.PartitionedArray.from_numpy(syn.uniform_host(61 + i, syn.STREAM_THRESHOLD, 0.0, 0.5, s))
+            for i, s in enumerate(shapes)]
+    qs = [[fw.PartitionedArray.from_numpy(syn.uniform_host(61 + i, stream, lo, hi, s))
+           for stream, lo, hi in ((syn.STREAM_CURRENT, 0.0, 5.0), (syn.STREAM_LATERAL, 0.0, 1e-4),
+                                  (syn.STREAM_ALPHA, 1.5, 2.5))] for i, s in enumerate(shapes)]
+    net = fw.FlowNetwork(ldds)
+
+    def results():
+        out = [x.numpy() for x in net.accu(mats)]
+        o, r = net.accu_threshold(mats, thrs)
+        out += [x.numpy() for x in o] + [x.numpy() for x in r]
+        out += [x.numpy() for x in net.kinematic_wave([q[0] for q in qs], [q[1] for q in qs], [q[2] for q in qs],
+                                                      0.6, 3600.0, 100.0)]
+        plan = net.plan_accu(mats)
+        plan.run()
+        out += [x.numpy() for x in plan.outputs]
+        plan.close()
+        return out
+
+    per_tile = results()
+    net.set_tile_queue(True)
+    assert net.info()["tile_queue"] == 1
+    queued = results()
+    for a, b in zip(per_tile, queued):
+        assert np.array_equal(a, b, equal_nan=True)
