@@ -299,13 +299,19 @@ def main():
         fw.accu(ldd, inp["material"])
         fw.kinematic_wave(ldd, inp["q_cur"], inp["q_lat"], inp["alpha"], 0.6, 3600.0, 100.0)
 
-    per_call(f32)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.fp64_steps):
-        per_call(f32)
-    torch.cuda.synchronize()
-    t32_call = (time.perf_counter() - t0) / args.fp64_steps
+    def per_call_time(inp):
+        # the median of single steps (each synchronised), so that a one-off stall does not set the figure
+        per_call(inp)
+        times = []
+        for _ in range(max(args.fp64_steps, 3)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            per_call(inp)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        return sorted(times)[len(times) // 2]
+
+    t32_call = per_call_time(f32)
 
     # cold call: the first accu on an ldd the library has not seen (records its schedule)
     fresh = fw.PartitionedArray(ldd.data.clone(), ps)
@@ -319,13 +325,7 @@ def main():
     # FP64 path ------------------------------------------------------------------------------------
     log("[bench] fp64 steps")
     t64, out64 = timed(f64, args.fp64_steps, 1)
-    per_call(f64)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.fp64_steps):
-        per_call(f64)
-    torch.cuda.synchronize()
-    t64_call = (time.perf_counter() - t0) / args.fp64_steps
+    t64_call = per_call_time(f64)
 
     # accuracy of the FP32 path against the FP64 path on the same inputs
     def rel(a, b):

@@ -807,7 +807,8 @@ namespace lue_amd {
             d.start = as_global(c.seg_start)[p] != 0;
             d.extra = as_global(c.side)[p];
             d.push = as_global(c.push_slot)[p];
-            d.in = c.staged[p];
+            if constexpr (OP != op_accu)
+                d.in = c.staged[p];  // accu: the inflow is in the side sum, d.in.x stays 0
         }
         return d;
     }
@@ -864,8 +865,8 @@ namespace lue_amd {
     // Write the chain results of the pending cells of every tile to the output rasters. One wavefront per tile
     // (a tile has about a third of its cells pending), scatter_tiles_per_block<...>() tiles per workgroup: the
     // results are read in chain-position order (sorted_pos: runs of consecutive positions, so whole lines) into
-    // the wavefront's part of LDS, then written in cell order (the staging entries in cell order give the cell,
-    // sorted_rank the result's place in LDS), so that the stores of a row of the tile are one line. The tile's
+    // the wavefront's part of LDS, then written in cell order (the scatter entries in cell order give the cell
+    // and the result's place in LDS), so that the stores of a row of the tile are one line. The tile's
     // catchment and IO entry are loaded with its bounds, so a wavefront waits for three rounds of loads.
     template<int OP, typename T, int TW, int TH>
     constexpr int scatter_tiles_per_block()
@@ -877,9 +878,8 @@ namespace lue_amd {
 
     template<int OP, typename T, int TW, int TH>
     __global__ void __launch_bounds__((64 * scatter_tiles_per_block<OP, T, TW, TH>()))
-        chains_scatter_kernel(SweepArgs<T> a, std::uint32_t const* tile_off, std::uint32_t const* stage_cell,
-                              std::uint32_t const* sorted_pos, std::uint16_t const* sorted_rank, T const* res,
-                              T const* res_rem, std::int64_t ntiles)
+        chains_scatter_kernel(SweepArgs<T> a, std::uint32_t const* tile_off, std::uint32_t const* scatter_entry,
+                              std::uint32_t const* sorted_pos, T const* res, T const* res_rem, std::int64_t ntiles)
     {
         using G = Tile<TW, TH>;
         constexpr bool two_outputs = is_split_op(OP);
@@ -955,8 +955,9 @@ namespace lue_amd {
             for (int j = 0; j < chunk; ++j)
             {
                 std::uint32_t const rank = base + static_cast<std::uint32_t>(64 * j + lane);
-                cell[j] = rank < npending ? as_global(stage_cell)[begin + rank] & (G::n - 1) : 0u;
-                at[j] = rank < npending ? as_global(sorted_rank)[begin + rank] & (G::n - 1) : 0u;
+                std::uint32_t const e = rank < npending ? as_global(scatter_entry)[begin + rank] : 0u;
+                cell[j] = e & (G::n - 1);
+                at[j] = (e >> 16) & (G::n - 1);
             }
 #pragma unroll
             for (int j = 0; j < chunk; ++j)

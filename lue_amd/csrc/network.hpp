@@ -608,9 +608,10 @@ namespace lue_amd {
         DeviceArray<std::uint32_t> lo_off;        // per tile (+1): its local cells that feed side slots
         DeviceArray<std::uint64_t> lo_entry;      // per such cell: side slot << 16 | local cell
         DeviceArray<std::uint32_t> stage_cell;    // per compact index: local cell | solved upstream mask << 16
+                                                  // (recording only: replays read the sorted entries)
         DeviceArray<std::uint32_t> sorted_pos;    // per tile from tile_off: its pending cells' chain positions, sorted
         DeviceArray<std::uint32_t> sorted_cell;   //   and their staging entries in that order
-        DeviceArray<std::uint16_t> sorted_rank;   // per compact index: its place in the tile's sorted order
+        DeviceArray<std::uint32_t> scatter_entry; // per compact index: local cell | place in the sorted order << 16
 
         Network() = default;
         Network(Network const&) = delete;
@@ -646,7 +647,7 @@ namespace lue_amd {
             stage_cell.reset();
             sorted_pos.reset();
             sorted_cell.reset();
-            sorted_rank.reset();
+            scatter_entry.reset();
             if (switched)
                 (void)hipSetDevice(current);
         }
@@ -679,7 +680,7 @@ namespace lue_amd {
             stage_cell.stream = nullptr;
             sorted_pos.stream = nullptr;
             sorted_cell.stream = nullptr;
-            sorted_rank.stream = nullptr;
+            scatter_entry.stream = nullptr;
         }
     };
 
@@ -1291,7 +1292,7 @@ namespace lue_amd {
         LUE_AMD_HIP(net.stage_cell.allocate(std::max<std::size_t>(net.pending, 1), stream));
         LUE_AMD_HIP(net.sorted_pos.allocate(std::max<std::size_t>(net.pending, 1), stream));
         LUE_AMD_HIP(net.sorted_cell.allocate(std::max<std::size_t>(net.pending, 1), stream));
-        LUE_AMD_HIP(net.sorted_rank.allocate(std::max<std::size_t>(net.pending, 1), stream));
+        LUE_AMD_HIP(net.scatter_entry.allocate(std::max<std::size_t>(net.pending, 1), stream));
         LUE_AMD_HIP(hipMemcpyAsync(net.pending_bits.ptr, pending_bits, ntiles * G::words * sizeof(std::uint32_t),
                                    hipMemcpyDeviceToDevice, stream));
         DeviceArray<std::uint32_t> lo_bits, lo_count;
@@ -1307,10 +1308,11 @@ namespace lue_amd {
         {
             hipLaunchKernelGGL((tile_sort_kernel<TW, TH>), dim3(static_cast<unsigned>(ntiles)), dim3(256), 0, stream,
                                net.tile_off.ptr, net.pos_of.ptr, net.stage_cell.ptr, net.sorted_pos.ptr,
-                               net.sorted_cell.ptr, net.sorted_rank.ptr);
+                               net.sorted_cell.ptr, net.scatter_entry.ptr);
             LUE_AMD_HIP(hipGetLastError());
         }
-        net.pos_of.reset();  // replays use the sorted positions
+        net.pos_of.reset();  // replays use the sorted positions and entries
+        net.stage_cell.reset();
         LUE_AMD_HIP(exclusive_scan(lo_count.ptr, net.lo_off.ptr, ntiles + 1, stream));
         std::uint32_t nlo = 0;
         LUE_AMD_HIP(copy_to_host(&nlo, net.lo_off.ptr + ntiles, 1, stream));
@@ -1389,8 +1391,8 @@ namespace lue_amd {
         constexpr int tiles_per_block = scatter_tiles_per_block<OP, T, TW, TH>();
         hipLaunchKernelGGL((chains_scatter_kernel<OP, T, TW, TH>),
                            dim3(static_cast<unsigned>((net.ntiles + tiles_per_block - 1) / tiles_per_block)),
-                           dim3(64 * tiles_per_block), 0, stream, a, net.tile_off.ptr, net.stage_cell.ptr,
-                           net.sorted_pos.ptr, net.sorted_rank.ptr, c.res, c.res_rem, net.ntiles);
+                           dim3(64 * tiles_per_block), 0, stream, a, net.tile_off.ptr, net.scatter_entry.ptr,
+                           net.sorted_pos.ptr, c.res, c.res_rem, net.ntiles);
         LUE_AMD_HIP(hipGetLastError());
         return status_success;
     }
@@ -1472,7 +1474,7 @@ namespace lue_amd {
         l.border_stamp = take(chains ? 0 : ntiles * G::perimeter * sizeof(std::uint32_t));
         l.out_lo = take(!chains && std::is_same_v<T, float> ? static_cast<std::size_t>(net.ncells) * sizeof(float) : 0);
         l.io = take(net.cats_host.size() * sizeof(CatchmentIO<T>));
-        l.staged = take(pending * sizeof(CellInput<OP, T>));
+        l.staged = take(OP == op_accu ? 0 : pending * sizeof(CellInput<OP, T>));  // accu: in the side sums
         l.side = take(pending * sizeof(double));
         l.res = take(pending * sizeof(T));
         l.res_rem = take(is_split_op(OP) ? pending * sizeof(T) : 0);

@@ -513,12 +513,12 @@ namespace lue_amd {
     // The pending cells of each tile in chain-position order (a chain crossing the tile is a run of consecutive
     // positions there), so that the tile pass stages and the scatter reads whole runs of positions: from
     // tile_off[tile], sorted_pos[k] = the k-th smallest chain position of the tile's pending cells,
-    // sorted_cell[k] = that cell's staging entry (stage_cell), and sorted_rank[rank] = k for the cell of
-    // that rank (cell order). A bitonic sort in LDS, one workgroup per tile.
+    // sorted_cell[k] = that cell's staging entry (stage_cell), and scatter_entry[rank] = the local cell of that
+    // rank (cell order) | its place k in the sorted order << 16. A bitonic sort in LDS, one workgroup per tile.
     template<int TW, int TH>
     __global__ void __launch_bounds__(256)
         tile_sort_kernel(std::uint32_t const* tile_off, std::uint32_t const* pos_of, std::uint32_t const* stage_cell,
-                         std::uint32_t* sorted_pos, std::uint32_t* sorted_cell, std::uint16_t* sorted_rank)
+                         std::uint32_t* sorted_pos, std::uint32_t* sorted_cell, std::uint32_t* scatter_entry)
     {
         using G = Tile<TW, TH>;
         static_assert(G::n <= 65536, "ranks fit 16 bits");
@@ -567,9 +567,10 @@ namespace lue_amd {
         for (int i = tid; i < npending; i += 256)
         {
             std::uint32_t const rank = val[i];
+            std::uint32_t const entry = as_global(stage_cell)[begin + rank];
             as_global(sorted_pos)[begin + i] = key[i];
-            as_global(sorted_cell)[begin + i] = as_global(stage_cell)[begin + rank];
-            as_global(sorted_rank)[begin + rank] = static_cast<std::uint16_t>(i);
+            as_global(sorted_cell)[begin + i] = entry;
+            as_global(scatter_entry)[begin + rank] = (entry & 0xFFFFu) | (static_cast<std::uint32_t>(i) << 16);
         }
     }
 

@@ -271,7 +271,8 @@ namespace lue_amd {
 
         // 4. staging of the pending cells, entry k by thread k mod nthreads (entries in chain-position order, so
         // that a wavefront's stores cover runs of consecutive positions): inputs and the FP64 sum (neighbour
-        // order) of their upstream cells that this tile solved, at their chain positions
+        // order) of their upstream cells that this tile solved, at their chain positions (accu: that sum plus
+        // the cell's inflow, its only input)
         CellInput<OP, T>* const staged = static_cast<CellInput<OP, T>*>(a.staged);
 #pragma unroll
         for (int j = 0; j < per_thread; ++j)
@@ -281,8 +282,17 @@ namespace lue_amd {
                 break;
             int const i = static_cast<int>(cell_of[j] & (G::n - 1));
             double const side = tile_neighbour_sum<OP, T, TW, TH>(s, i, (cell_of[j] >> 16) & 0xFFu);
-            staged[pos_of[j]] = s.in[i];
-            as_global(a.stage_side)[pos_of[j]] = side;
+            if constexpr (OP == op_accu)
+            {
+                // accu adds the cell's own inflow to the side sum (a NaN inflow makes it NaN): one FP64 value
+                // per pending cell, nothing else staged
+                as_global(a.stage_side)[pos_of[j]] = side + static_cast<double>(s.in[i].x);
+            }
+            else
+            {
+                staged[pos_of[j]] = s.in[i];
+                as_global(a.stage_side)[pos_of[j]] = side;
+            }
         }
     }
 
