@@ -902,6 +902,15 @@ namespace lue_amd {
             return;
         T* const v_lds = value[wave];
         T* const r_lds = remainder[wave];
+        // the scatter entries of the first chunk of ranks, loaded with the first positions (they do not depend
+        // on the results)
+        std::uint32_t entry0[chunk];
+#pragma unroll
+        for (int j = 0; j < chunk; ++j)
+        {
+            std::uint32_t const rank = static_cast<std::uint32_t>(64 * j + lane);
+            entry0[j] = rank < npending ? as_global(scatter_entry)[begin + rank] : 0u;
+        }
         // chain-position order: positions, then results, a chunk of entries per lane at a time
         for (std::uint32_t base = 0; base < npending; base += 64 * chunk)
         {
@@ -955,7 +964,9 @@ namespace lue_amd {
             for (int j = 0; j < chunk; ++j)
             {
                 std::uint32_t const rank = base + static_cast<std::uint32_t>(64 * j + lane);
-                std::uint32_t const e = rank < npending ? as_global(scatter_entry)[begin + rank] : 0u;
+                std::uint32_t e = entry0[j];
+                if (base != 0)
+                    e = rank < npending ? as_global(scatter_entry)[begin + rank] : 0u;
                 cell[j] = e & (G::n - 1);
                 at[j] = (e >> 16) & (G::n - 1);
             }
