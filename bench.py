@@ -57,8 +57,10 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="minimum CPU-baseline time: the full workload is repeated until it is reached (0: skip)")
     p.add_argument("--cpu-threads", type=int, default=0)
-    p.add_argument("--accu-tile", default="32x32", help="sweep tile of the accu network (WxH)")
-    p.add_argument("--kw-tile", default="32x32", help="sweep tile of the kinematic-wave network (WxH)")
+    p.add_argument("--accu-tile", default="auto",
+                   help="sweep tile of the accu network (WxH; auto: tile shape, wave schedule and first-round "
+                        "launch picked by the autotuner on the FP32 inputs)")
+    p.add_argument("--kw-tile", default="auto", help="sweep tile of the kinematic-wave network (WxH or auto)")
     p.add_argument("--schedule", default="chains", choices=["chains", "rounds"])
     p.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic runs")
     p.add_argument("--serving-seconds", type=float, default=5.0,
@@ -75,7 +77,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def tile(text):
+# tile shapes of the traffic probes where the bench's own are left to the autotuner: those of the per-call
+# entry points (the autotuner picks the same for the kinematic wave at 16384^2)
+PROBE_TILES = {"accu": "64x32", "kinematic_wave": "32x32"}
+
+
+def tile(text, op=None):
+    if text == "auto":
+        text = PROBE_TILES[op]
     return tuple(int(v) for v in text.split("x"))
 
 
@@ -103,8 +112,8 @@ def probe(args):
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     _, ldd, f32 = make_inputs(args.size, dev, torch)
-    net_accu = fw.FlowNetwork(ldd, tile_shape=tile(args.accu_tile), schedule=args.schedule)
-    net_kw = fw.FlowNetwork(ldd, tile_shape=tile(args.kw_tile), schedule=args.schedule)
+    net_accu = fw.FlowNetwork(ldd, tile_shape=tile(args.accu_tile, "accu"), schedule=args.schedule)
+    net_kw = fw.FlowNetwork(ldd, tile_shape=tile(args.kw_tile, "kinematic_wave"), schedule=args.schedule)
     for _ in range(2):
         net_accu.accu(f32["material"])
         net_kw.kinematic_wave(f32["q_cur"], f32["q_lat"], f32["alpha"], 0.6, 3600.0, 100.0)
@@ -202,13 +211,43 @@ def main():
     f64 = {k: fw.PartitionedArray(v.data.double(), ps) for k, v in f32.items()}
     torch.cuda.synchronize()
 
+    # tile shape, wave schedule and first-round launch of each operation's network: the autotuner times
+    # every combination on the FP32 inputs (one network recorded per tile shape)
+    settings = {}
+    for op, arg in (("accu", args.accu_tile), ("kinematic_wave", args.kw_tile)):
+        if arg != "auto":
+            settings[op] = dict(tile_shape=tile(arg), wide_threshold=None, tile_queue=None, tuned=False)
+            continue
+        from lue_amd import autotune
+
+        log(f"[bench] autotuning the {op} network")
+        scratch = fw.PartitionedArray(torch.empty((n, n), dtype=torch.float32, device=dev), ps)
+        if op == "accu":
+            def run(net):
+                net.accu(f32["material"], out=scratch)
+        else:
+            def run(net):
+                net.kinematic_wave(f32["q_cur"], f32["q_lat"], f32["alpha"], 0.6, 3600.0, 100.0, out=scratch)
+        best, table, _ = autotune.tune(ldd, run, schedule=args.schedule)
+        picked = best.info()
+        settings[op] = dict(tile_shape=best.tile_shape, wide_threshold=picked["wide_threshold"],
+                            tile_queue=bool(picked["tile_queue"]), tuned=True,
+                            tile_shape_ms={f"{k[0]}x{k[1]}": round(v, 3) for k, v in table.items()})
+        best.close()
+        del scratch
+    torch.cuda.synchronize()
+
     # networks: the sweep schedule of the ldd, recorded once
     log("[bench] recording flow networks")
     t0 = time.perf_counter()
-    net_accu = fw.FlowNetwork(ldd, tile_shape=tile(args.accu_tile), schedule=args.schedule)
-    net_kw = fw.FlowNetwork(ldd, tile_shape=tile(args.kw_tile), schedule=args.schedule)
+    net_accu = fw.FlowNetwork(ldd, tile_shape=settings["accu"]["tile_shape"], schedule=args.schedule)
+    net_kw = fw.FlowNetwork(ldd, tile_shape=settings["kinematic_wave"]["tile_shape"], schedule=args.schedule)
     torch.cuda.synchronize()
     t_network = time.perf_counter() - t0
+    for net, op in ((net_accu, "accu"), (net_kw, "kinematic_wave")):
+        if settings[op]["tuned"]:
+            net.set_wide_threshold(settings[op]["wide_threshold"])
+            net.set_tile_queue(settings[op]["tile_queue"])
     s_accu = torch.cuda.Stream()
     s_kw = torch.cuda.Stream()
 
@@ -372,8 +411,12 @@ def main():
     tile_pass_bytes = cells * reads + solved * writes
     dom_ms = kernel_ms[dominant]["first_round_ms"]
     achieved = tile_pass_bytes / (dom_ms * 1e-3) / 1e9
-    dom_traffic = traffic.get(dominant) if traffic else None
-    dom_tile = args.kw_tile if dominant == "kinematic_wave_f32" else args.accu_tile
+    dom_op = "kinematic_wave" if dominant == "kinematic_wave_f32" else "accu"
+    dom_tile = "x".join(str(v) for v in settings[dom_op]["tile_shape"])
+    dom_arg = args.kw_tile if dom_op == "kinematic_wave" else args.accu_tile
+    # the traffic probes ran on their own networks before the tuning: their figure holds for the same tile
+    dom_traffic = traffic.get(dominant) if traffic and tile(dom_arg, dom_op) == tuple(
+        settings[dom_op]["tile_shape"]) else None
 
     # CPU baseline: the oracle on the same inputs, copied to the host --------------------------------
     cpu = None
@@ -462,7 +505,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"accu + kinematic_wave timestep, {n}x{n} synthetic DEM catchment, "
                                f"{ps[0]}x{ps[1]} partitions, beta 0.6, dt 3600 s, dx 100 m",
-                   "cells": cells, "parallelism": "single GPU"},
+                   "cells": cells, "parallelism": "single GPU",
+                   "networks": {op: {"tile_shape": "x".join(str(v) for v in st["tile_shape"]),
+                                     "wide_threshold": st["wide_threshold"], "tile_queue": st["tile_queue"],
+                                     "autotuned": st["tuned"], **({"tile_shape_ms": st["tile_shape_ms"]}
+                                                                  if st["tuned"] else {})}
+                                for op, st in settings.items()}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": round(dom_traffic["bytes"]) if dom_traffic else None,

@@ -334,12 +334,18 @@ int lue_amd_synthetic_uniform_f64(uint64_t seed, uint64_t stream_id, double lo, 
  * depends on the ldd only (the chain layout, or the tiles of each round). Operations on a network
  * run that schedule as a fixed sequence of launches on `stream`, for all catchments at once, and return
  * without waiting for the results (synchronise the stream before reading them).
- * tile_shape: 0 = 32x32, 1 = 64x32, 2 = 64x64 cells per tile (operations that do not fit the shape in
- * LDS return LUE_AMD_ERROR_INVALID_ARGUMENT). schedule: LUE_AMD_SCHEDULE_*. Creating a network validates
- * it (cycles: LUE_AMD_ERROR_INVALID_NETWORK).
+ * tile_shape: LUE_AMD_TILE_* (width x height cells per tile; operations that do not fit the shape in LDS
+ * return LUE_AMD_ERROR_INVALID_ARGUMENT). The per-call entry points use 32x32 for the kinematic wave and
+ * 64x32 for the accu family. schedule: LUE_AMD_SCHEDULE_*. Creating a network validates it (cycles:
+ * LUE_AMD_ERROR_INVALID_NETWORK).
  * The arguments of the operations are host arrays with one device pointer per catchment; an argument
  * array may be NULL where the operation accepts a Scalar, which then applies to every catchment.
  */
+#define LUE_AMD_TILE_32X32 0
+#define LUE_AMD_TILE_64X32 1
+#define LUE_AMD_TILE_64X64 2
+#define LUE_AMD_TILE_32X64 3
+
 typedef struct lue_amd_network_s* lue_amd_network;
 
 int lue_amd_network_create(int n, const uint8_t* const* ldd, const int64_t* rows, const int64_t* cols, int tile_shape,

@@ -17,7 +17,7 @@ import torch
 from . import _lib
 from .framework import TILE_SHAPES, FlowNetwork
 
-CANDIDATES = ((32, 32), (64, 32), (64, 64))
+CANDIDATES = ((32, 32), (64, 32), (32, 64), (64, 64))
 WIDE_THRESHOLDS = (1024, 2048, 4096, 0)
 TILE_QUEUE = (False, True)
 

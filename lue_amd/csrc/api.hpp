@@ -48,18 +48,26 @@ namespace lue_amd {
                 if constexpr (fits<OP, T, 64, 64>())
                     return f.template operator()<64, 64>();
                 break;
+            case tile_32x64:
+                if constexpr (fits<OP, T, 32, 64>())
+                    return f.template operator()<32, 64>();
+                break;
             default:
                 break;
         }
         return fail(status_invalid_argument, "tile shape does not fit this operation in LDS");
     }
 
-    // Tile shape of per-call entry points. With the chain schedule the first round's time is set by the
-    // latency of a tile's walks and by how many tiles fit a CU at once, which favours the smallest tiles
-    // (32x32 measured fastest for every operation at 16384^2 cells).
+    // Tile shape of per-call entry points, measured at 16384^2 cells on the synthetic DEM. The kinematic wave's
+    // tile pass is bound by vector instruction issue and needs the occupancy of the smallest tiles (32x32:
+    // 6.1 ms FP32, 12.8 ms FP64; 64x32: 7.4 and 17.1 ms). The accu family's tile pass is bound by latency and
+    // memory, and larger tiles leave fewer cells to the chain phase (83 M pending at 32x32, 79 M at 64x32):
+    // 64x32 runs accu in 2.59 ms FP32 and 4.04 ms FP64 (32x32: 2.81, 4.36), threshold, fraction and capacity
+    // in 4.5 and 6.6 ms (4.8, 8.4).
+    template<int OP>
     constexpr int default_tile_shape()
     {
-        return tile_32x32;
+        return OP == op_kinematic_wave ? tile_32x32 : tile_64x32;
     }
 
     // Scalars of an operation (rasters come per catchment in CatchmentIO).
@@ -173,7 +181,7 @@ namespace lue_amd {
                    CatchmentIO<T> const& io, hipStream_t stream)
     {
         std::vector<CatchmentIO<T>> table{io};
-        int const shape = default_tile_shape();
+        int const shape = default_tile_shape<OP>();
         NetworkCache& cache = network_cache();
         CacheKey key{0, ldd, rows, cols, shape, default_schedule(), 0};
         bool const caching = cache.capacity() > 0;

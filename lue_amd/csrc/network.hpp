@@ -53,12 +53,12 @@ namespace lue_amd {
     } while (0)
 
     // Tile shapes (TW x TH) the kernels are instantiated for.
-    enum TileShape : int { tile_32x32 = 0, tile_64x32 = 1, tile_64x64 = 2, tile_shape_count = 3 };
+    enum TileShape : int { tile_32x32 = 0, tile_64x32 = 1, tile_64x64 = 2, tile_32x64 = 3, tile_shape_count = 4 };
 
     inline void tile_extent(int shape, int& tw, int& th)
     {
-        tw = shape == tile_32x32 ? 32 : 64;
-        th = shape == tile_64x64 ? 64 : 32;
+        tw = shape == tile_32x32 || shape == tile_32x64 ? 32 : 64;
+        th = shape == tile_64x64 || shape == tile_32x64 ? 64 : 32;
     }
 
     // Timing of the sweeps with HIP events (lue_amd_profile_*). A profiled operation records three events on

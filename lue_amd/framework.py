@@ -72,22 +72,27 @@ class PartitionedArray:
     def numpy(self):
         return self.data.cpu().numpy()
 
-    def _routing_network(self, tile_shape=(32, 32)):
-        """The flow network of this ldd array for the per-call routing operations: recorded at the first
-        call (which waits for the recording), kept while the tensor and its contents are unchanged, so later
-        calls only enqueue kernels. None when the per-call schedules are not kept (set_network_cache(0))."""
+    def _routing_network(self, operation):
+        """The flow network of this ldd array for the per-call routing operation `operation` (its tile shape,
+        PER_CALL_TILE_SHAPES): recorded at the first call (which waits for the recording), kept while the
+        tensor and its contents are unchanged, so later calls only enqueue kernels; one per tile shape. None
+        when the per-call schedules are not kept (set_network_cache(0))."""
         if _lib.network_cache_entries() == 0:
             return None
+        shape = PER_CALL_TILE_SHAPES.get(operation, PER_CALL_TILE_SHAPES["accu"])
         key = (self.data.data_ptr(), tuple(self.data.shape), self.data._version, _lib.sweep_schedule(),
-               tuple(tile_shape), self.data.device)
-        cached = self.__dict__.get("_network")
-        if cached is not None and cached[0] == key:
-            return cached[1]
-        self.__dict__.pop("_network", None)
-        if cached is not None:
-            cached[1].close()
-        net = FlowNetwork(self, tile_shape=tile_shape, schedule=_lib.sweep_schedule())
-        self._network = (key, net)
+               self.data.device)
+        networks = self.__dict__.get("_networks")
+        if networks is None or networks[0] != key:
+            if networks is not None:
+                for net in networks[1].values():
+                    net.close()
+            networks = (key, {})
+            self._networks = networks
+        net = networks[1].get(shape)
+        if net is None:
+            net = FlowNetwork(self, tile_shape=shape, schedule=_lib.sweep_schedule())
+            networks[1][shape] = net
         return net
 
     @classmethod
@@ -271,7 +276,7 @@ def accu(flow_direction, inflow):
     _check_ldd(flow_direction)
     _verify_compatible(flow_direction, inflow)
     dtype = _element_type(inflow)
-    net = flow_direction._routing_network()
+    net = flow_direction._routing_network("accu")
     if net is not None:
         return net.accu(inflow if isinstance(inflow, PartitionedArray) else _scalar(inflow, dtype), dtype=dtype)
     lib = _lib.load()
@@ -287,7 +292,7 @@ def _split(name, flow_direction, inflow, param):
     _check_ldd(flow_direction)
     _verify_compatible(flow_direction, inflow, param)
     dtype = _element_type(inflow, param)
-    net = flow_direction._routing_network()
+    net = flow_direction._routing_network(name)
     if net is not None:
         return net._split(name[len("accu_"):], _scalar(inflow, dtype), _scalar(param, dtype), dtype=dtype)
     lib = _lib.load()
@@ -331,7 +336,7 @@ def kinematic_wave(flow_direction, current_outflow, inflow, alpha, beta, time_st
     dtype = _element_type(current_outflow, inflow, alpha, beta, channel_length)
     if isinstance(time_step_duration, PartitionedArray):
         raise TypeError("time_step_duration is a Scalar")
-    net = flow_direction._routing_network()
+    net = flow_direction._routing_network("kinematic_wave")
     if net is not None:
         return net.kinematic_wave(current_outflow, inflow, _scalar(alpha, dtype), _scalar(beta, dtype),
                                   float(time_step_duration), _scalar(channel_length, dtype), dtype=dtype)
@@ -362,7 +367,12 @@ def iterate_to_new_discharge(upstream, current, lateral, alpha, beta, dt, dx):
 # ---------------------------------------------------------------------------------------------------
 # Flow networks: resident ldd rasters with their recorded sweep schedule (C ABI lue_amd_network_*)
 
-TILE_SHAPES = {(32, 32): 0, (64, 32): 1, (64, 64): 2}
+TILE_SHAPES = {(32, 32): 0, (64, 32): 1, (64, 64): 2, (32, 64): 3}
+
+# Tile shape (width, height) of the per-call operations, the same as the C ABI's per-call entry points: the
+# kinematic wave needs the occupancy of the smallest tiles, the accu family gains from larger ones (fewer
+# cells left to the chain phase).
+PER_CALL_TILE_SHAPES = {"accu": (64, 32), "kinematic_wave": (32, 32)}
 
 
 def _pointer_array(values):
@@ -402,6 +412,7 @@ class FlowNetwork:
                                               _lib.SCHEDULES[schedule],
                                               _stream(), ctypes.byref(handle)))
         self._handle = handle
+        self.tile_shape = tuple(tile_shape)
 
     def close(self):
         if getattr(self, "_handle", None):

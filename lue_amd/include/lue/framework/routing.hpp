@@ -295,10 +295,16 @@ namespace lue {
                 _ready.reset();
             }
 
-            // The flow network of this (ldd) array for the routing operations, recorded at the first call.
-            auto network() const -> lue_amd_network
+            // The flow network of this (ldd) array with tiles of `tile_shape` (LUE_AMD_TILE_*) for the routing
+            // operations, recorded at the first call that asks for that shape.
+            auto network(int tile_shape) const -> lue_amd_network
             {
-                if (!_network || _network_version != _version)
+                if (tile_shape < 0 || tile_shape >= static_cast<int>(_networks.size()))
+                {
+                    throw std::invalid_argument("unknown tile shape");
+                }
+                auto& cached = _networks[static_cast<std::size_t>(tile_shape)];
+                if (!cached.first || cached.second != _version)
                 {
                     void* const s = lue::stream();
                     order_before(s);
@@ -306,12 +312,11 @@ namespace lue {
                     std::uint8_t const* ldd = reinterpret_cast<std::uint8_t const*>(_data.get());
                     std::int64_t const rows = _shape[0];
                     std::int64_t const cols = _shape[1];
-                    detail::check(lue_amd_network_create(1, &ldd, &rows, &cols, 0, LUE_AMD_SCHEDULE_CHAINS, s,
-                                                         &handle->network));
-                    _network = std::move(handle);
-                    _network_version = _version;
+                    detail::check(lue_amd_network_create(1, &ldd, &rows, &cols, tile_shape, LUE_AMD_SCHEDULE_CHAINS,
+                                                         s, &handle->network));
+                    cached = {std::move(handle), _version};
                 }
-                return _network->network;
+                return cached.first->network;
             }
 
             // Set partition by partition from per-partition row-major values, partitions in row-major
@@ -355,8 +360,9 @@ namespace lue {
             std::shared_ptr<Element> _data;
             std::shared_ptr<detail::Event> _ready;  // completion of the operation that wrote the array
             std::uint64_t _version{0};
-            mutable std::shared_ptr<detail::NetworkHandle> _network;  // ldd arrays: recorded flow network
-            mutable std::uint64_t _network_version{0};
+            // ldd arrays: recorded flow networks per tile shape, with the version of the array they were recorded
+            // from
+            mutable std::array<std::pair<std::shared_ptr<detail::NetworkHandle>, std::uint64_t>, 4> _networks{};
     };
 
     template<typename Element>
@@ -600,7 +606,7 @@ namespace lue {
         {
             detail::verify_compatible(flow_direction, inflow);
             auto outflow = detail::output_like<E>(flow_direction);
-            lue_amd_network const network = flow_direction.network();
+            lue_amd_network const network = flow_direction.network(LUE_AMD_TILE_64X32);
             void* const s = lue::stream();
             detail::order_before(inflow, s);
             E const* const in[1] = {detail::pointer(inflow)};
@@ -650,7 +656,7 @@ namespace lue {
                 lue::detail::verify_compatible(flow_direction, param);
                 auto outflow = lue::detail::output_like<E>(flow_direction);
                 auto remainder = lue::detail::output_like<E>(flow_direction);
-                lue_amd_network const net = flow_direction.network();
+                lue_amd_network const net = flow_direction.network(LUE_AMD_TILE_64X32);
                 void* const s = lue::stream();
                 lue::detail::order_before(inflow, s);
                 lue::detail::order_before(param, s);
@@ -821,7 +827,7 @@ namespace lue {
                 lue::detail::verify_compatible(flow_direction, beta);
                 lue::detail::verify_compatible(flow_direction, channel_length);
                 auto outflow = lue::detail::output_like<E>(flow_direction);
-                lue_amd_network const net = flow_direction.network();
+                lue_amd_network const net = flow_direction.network(LUE_AMD_TILE_32X32);
                 void* const s = lue::stream();
                 current_outflow.order_before(s);
                 inflow.order_before(s);

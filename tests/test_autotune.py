@@ -30,7 +30,7 @@ class FakeNetwork:
 
 def model_ms(net):
     # fastest: 64x32, threshold 4096, no queue
-    base = {(32, 32): 7.0, (64, 32): 6.0, (64, 64): 9.0}[net.shape]
+    base = {(32, 32): 7.0, (64, 32): 6.0, (32, 64): 6.5, (64, 64): 9.0}[net.shape]
     return base + {1024: 0.3, 2048: 0.2, 4096: 0.0, 0: 0.1}[net.threshold] + (0.5 if net.queue else 0.0)
 
 
@@ -42,8 +42,8 @@ def test_tuner_keeps_the_fastest_combination(monkeypatch):
     monkeypatch.setattr(autotune, "_time", lambda run, net, repeats: model_ms(net))
     net, table, schedule_table = autotune.tune([object()], lambda n: None)
     assert net.shape == (64, 32) and net.threshold == 4096 and net.queue is False and not net.closed
-    assert table == {(32, 32): 7.0, (64, 32): 6.0, (64, 64): 9.0}
-    assert len(schedule_table) == 3 * len(autotune.WIDE_THRESHOLDS) * len(autotune.TILE_QUEUE)
+    assert table == {(32, 32): 7.0, (64, 32): 6.0, (32, 64): 6.5, (64, 64): 9.0}
+    assert len(schedule_table) == len(autotune.CANDIDATES) * len(autotune.WIDE_THRESHOLDS) * len(autotune.TILE_QUEUE)
     assert all(n.closed for n in FakeNetwork.made if n is not net)
 
 

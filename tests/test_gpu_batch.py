@@ -48,7 +48,7 @@ def test_autotuned_batch_accu_threshold_matches_oracle(fw, oracle_mod):
     net, table, _ = autotune.tune(ldds, lambda n: n.accu_threshold(mats, thrs))
     assert len(table) >= 2 and net.info()["catchments"] == len(shapes)
     outs, rems = net.accu_threshold(mats, thrs)
-    shape = {0: (32, 32), 1: (64, 32), 2: (64, 64)}[net.info()["tile_shape"]]
+    shape = net.tile_shape
     torch.cuda.synchronize()
     for i in (0, 5, 11, 23):
         alone = fw.FlowNetwork(ldds[i], tile_shape=shape)

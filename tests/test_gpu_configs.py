@@ -140,7 +140,7 @@ def test_batch_512_catchments_accu_threshold(fw, oracle_mod):
     assert net.info()["catchments"] == 512
     outs, rems = net.accu_threshold(mats, thrs)
     torch.cuda.synchronize()
-    shape = {0: (32, 32), 1: (64, 32), 2: (64, 64)}[net.info()["tile_shape"]]
+    shape = net.tile_shape
     print(f"[configs] batch of 512: {sum(r * c for r, c in shapes)} cells, tile shape {shape}", flush=True)
     # every catchment alone gives the same bits
     for i in range(len(shapes)):

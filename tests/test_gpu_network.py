@@ -42,7 +42,7 @@ def catchment(seed, rows, cols):
     return dict(ldd=ldd, material=material, threshold=threshold, q_cur=q_cur, q_lat=q_lat, alpha=alpha)
 
 
-@pytest.mark.parametrize("tile_shape", [(32, 32), (64, 32), (64, 64)])
+@pytest.mark.parametrize("tile_shape", [(32, 32), (64, 32), (64, 64), (32, 64)])
 @pytest.mark.parametrize("schedule", ["chains", "rounds"])
 def test_network_matches_oracle(fw, oracle_mod, tile_shape, schedule):
     c = catchment(11, 333, 257)
