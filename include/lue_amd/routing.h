@@ -351,10 +351,11 @@ typedef struct lue_amd_network_s* lue_amd_network;
 int lue_amd_network_create(int n, const uint8_t* const* ldd, const int64_t* rows, const int64_t* cols, int tile_shape,
                            int schedule, void* stream, lue_amd_network* network);
 int lue_amd_network_destroy(lue_amd_network network);
-/* info[0..15]: catchments, cells, tiles, recorded rounds, cells left pending by the first round, chains,
+/* info[0..16]: catchments, cells, tiles, recorded rounds, cells left pending by the first round, chains,
  * chain levels, no-data cells, tile shape, schedule, longest chain, side references, 64-position blocks
  * the first round runs on the in-tile layouts, and of them the blocks on the tiles' critical paths, the
- * wide threshold (lue_amd_network_set_wide_threshold), the tile queue (lue_amd_network_set_tile_queue) */
+ * wide threshold (lue_amd_network_set_wide_threshold), the tile queue (lue_amd_network_set_tile_queue), the
+ * lane limit (lue_amd_network_set_lane_limit) */
 int lue_amd_network_info(lue_amd_network network, int64_t* info, int n);
 
 /*
@@ -375,6 +376,18 @@ int lue_amd_network_set_wide_threshold(lue_amd_network network, int64_t position
  * another thread enqueues operations on the network.
  */
 int lue_amd_network_set_tile_queue(lue_amd_network network, int on);
+
+/*
+ * The lane schedule of the chain levels (no reference counterpart; a tuning knob of the batching addition):
+ * work items of at most `positions` chain positions are walked one per lane, cell after cell (each cell
+ * solved once from its known upstream outflow, as in LUE's walk), the longer ones one per wavefront, 64
+ * positions at a time (or, in a level on the wide path, one per workgroup); 0 (the default) walks none. Whether
+ * an item is walked depends on its length only, not on the other catchments of a batch. The
+ * results agree with the other schedule to the operations' tolerances (sums in another order; the kinematic
+ * wave's roots to within its FP64 correction). Applies to operations enqueued from then on (plans keep the
+ * schedule they captured); not while another thread enqueues operations on the network.
+ */
+int lue_amd_network_set_lane_limit(lue_amd_network network, int64_t positions);
 
 /*
  * One chain level of a network (chain schedule; no reference counterpart): info[0] chain positions,

@@ -35,6 +35,7 @@ SIGNATURES = {
     "lue_amd_network_level_info": [c_p, c_int, c_p, c_int],
     "lue_amd_network_set_wide_threshold": [c_p, c_i64],
     "lue_amd_network_set_tile_queue": [c_p, c_int],
+    "lue_amd_network_set_lane_limit": [c_p, c_i64],
     "lue_amd_network_accu_f32": [c_p, c_p, c_f, c_p, c_p],
     "lue_amd_network_accu_f64": [c_p, c_p, c_d, c_p, c_p],
     "lue_amd_network_accu_threshold_f32": [c_p, c_p, c_f, c_p, c_f, c_p, c_p, c_p],

@@ -683,4 +683,75 @@ namespace lue_amd {
         }
     }
 
+    // Kinematic wave, FP32 path, one cell whose upstream sum `up` (FP64) is known: the per-cell form of
+    // kinematic_wave_block, for a lane that walks a chain by itself. FP32 Newton steps with the hardware power
+    // from the equation linearised at Q_cur until |dQ| <= |Q| 2^-8, one Newton step on the FP64 residual (the
+    // short FP64 power), then the block's check that LUE's own iteration ends at this root. A cell that needs
+    // LUE's iteration (zero current discharge, a non-positive or non-finite iterate, no convergence within 16
+    // steps, a guess from which LUE's iteration stops short of the root) takes new_discharge_f32_path, as the
+    // block's per-cell fallback does.
+    __device__ inline double kinematic_wave_lane_f32(SweepArgs<float> const& a,
+                                                     CellInput<op_kinematic_wave, float> const& in, double up)
+    {
+        float const q_cur = in.q_cur;
+        if (isnan(up) || isnan(q_cur))
+            return __longlong_as_double(0x7ff8000000000000ll);
+        if (q_cur > 0.0f)
+        {
+            float const c1f = in.c1;
+            float const alpha32 = in.alpha;
+            float const beta32 = in.beta;
+            float const ab32 = alpha32 * beta32;
+            float const b32 = in.b_hi;
+            float const up32 = static_cast<float>(up);
+            float const pc1 = fast_exp2((beta32 - 1.0f) * fast_log2(q_cur));  // Q_cur^(beta-1)
+            float x = (c1f * up32 + (b32 - (1.0f - beta32) * alpha32 * q_cur * pc1)) * fast_rcp(c1f + ab32 * pc1);
+            bool ok = x > 0.0f && x < __builtin_huge_valf();
+            bool converged = false;
+            for (int iteration = 0; ok && iteration < 16; ++iteration)
+            {
+                float const pw1 = fast_exp2((beta32 - 1.0f) * fast_log2(x));
+                float const f = c1f * (x - up32) + alpha32 * x * pw1 - b32;
+                float const step = -f * fast_rcp(c1f + ab32 * pw1);
+                float const xn = x + step;
+                ok = xn > 0.0f && xn < __builtin_huge_valf();
+                converged = fabsf(step) <= fabsf(xn) * 0x1p-8f;
+                x = xn;
+                if (converged)
+                    break;
+            }
+            if (ok && converged)
+            {
+                // FP64 residual correction (its Jacobian in FP32, as in the block)
+                double const b = kw_b(in);
+                double const c1 = kw_c1(in);
+                double const alpha = static_cast<double>(alpha32);
+                double const xd = static_cast<double>(x);
+                double const pw1 = pow_positive<true, true>(xd, static_cast<double>(beta32) - 1.0);
+                double const f = c1 * (xd - up) + alpha * xd * pw1 - b;
+                float const inv = fast_rcp(c1f + ab32 * static_cast<float>(pw1));
+                double const xn = xd + static_cast<double>(static_cast<float>(-f) * inv);
+                if (xn > 0.0 && xn < __builtin_huge_val())
+                {
+                    float const dtq = static_cast<float>(b - alpha * static_cast<double>(q_cur) * static_cast<double>(pc1));
+                    if (lue_guess_converges(up32, q_cur, dtq, alpha32, beta32, c1f, static_cast<float>(xn)))
+                        return xn;
+                }
+            }
+        }
+        return new_discharge_f32_path_call(up, q_cur, kw_q_lat(in, a.dt), in.alpha, in.beta, a.dt, kw_b(in), kw_c1(in));
+    }
+
+    // The cell function of one chain position whose upstream outflow `up` (FP64) is known, for a lane that walks
+    // a chain by itself: the operations' cell functions (solve_cell), the FP32 kinematic wave by
+    // kinematic_wave_lane_f32 (the same root as the block Newton). NaN upstream poisons the cell.
+    template<int OP, typename T>
+    __device__ inline void solve_lane(SweepArgs<T> const& a, CellInput<OP, T> const& in, double up, double& out, T& rem)
+    {
+        if constexpr (OP == op_kinematic_wave && std::is_same_v<T, float>)
+            out = kinematic_wave_lane_f32(a, in, up);
+        else
+            solve_cell<OP, T>(a, in, up, isnan(up), out, rem);
+    }
+
 }  // namespace lue_amd

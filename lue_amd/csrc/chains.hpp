@@ -35,11 +35,18 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace lue_amd {
 
     // No side slot (a position that is not the last of a light chain).
     constexpr std::uint32_t no_slot = 0xFFFFFFFFu;
+
+    // Default lane limit of a network: work items of a chain level with at most this many positions are walked
+    // one per lane (run_items_lane_walk), longer ones one per wavefront, 64 positions at a time. 0: the lane
+    // walk is off by default; at 16384^2 it measured slower than the wavefront blocks for every operation but
+    // the trigger, so the autotuner chooses it per network.
+    constexpr std::uint32_t lane_max_positions = 0;
 
     template<int TW, int TH>
     __device__ inline bool cell_pending(std::uint32_t const* pending_bits, Catchment const& cat, std::int64_t r,
@@ -813,20 +820,16 @@ namespace lue_amd {
         return d;
     }
 
-    // One level of chains. A wavefront per work item (4 per workgroup). The data of the next `ahead` blocks is
-    // in flight while a block runs: 2 for the kinematic wave, whose block solve is long enough to cover the
-    // memory latency, 6 for the scans of the other operations. The ring of blocks rotates through a loop
+    // One work item of a level on one wavefront, 64 positions at a time. The data of the next `ahead` blocks
+    // is in flight while a block runs: 2 for the kinematic wave, whose block solve is long enough to cover
+    // the memory latency, 6 for the scans of the other operations. The ring of blocks rotates through a loop
     // unrolled over it, so no load is waited for before its block is due.
     template<int OP, typename T>
-    __global__ void __launch_bounds__(256)
-        chains_run_kernel(SweepArgs<T> a, ChainRun<OP, T> c, std::uint32_t item_begin, std::uint32_t item_end)
+    __device__ inline void run_item_blocks(SweepArgs<T> const& a, ChainRun<OP, T> const& c, std::uint32_t item)
     {
         constexpr bool two_outputs = is_split_op(OP);
         constexpr int ahead = OP == op_kinematic_wave ? 2 : 6;
         int const lane = threadIdx.x & 63;
-        std::uint32_t const item = item_begin + blockIdx.x * 4 + (threadIdx.x >> 6);
-        if (item >= item_end)
-            return;
         std::uint32_t const start = as_global(c.item_start)[item];
         std::uint32_t const count = as_global(c.item_count)[item];
         BlockState st;
@@ -860,6 +863,159 @@ namespace lue_amd {
                 }
             }
         }
+    }
+
+    // One level of chains, a wavefront per work item (4 per workgroup).
+    template<int OP, typename T>
+    __global__ void __launch_bounds__(256)
+        chains_run_kernel(SweepArgs<T> a, ChainRun<OP, T> c, std::uint32_t item_begin, std::uint32_t item_end)
+    {
+        std::uint32_t const item = item_begin + blockIdx.x * 4 + (threadIdx.x >> 6);
+        if (item >= item_end)
+            return;
+        run_item_blocks<OP, T>(a, c, item);
+    }
+
+    // Lane walk: a wavefront takes 64 work items (one workgroup of one wavefront), each lane walks its item
+    // position after position. Every cell sees its upstream outflow exactly, as in LUE's walk, and the cell
+    // function runs once per cell instead of once per Newton step of a 64-cell block, which suits the many
+    // short items of the low levels. The items' data moves in chunks of lane_chunk positions per item through
+    // LDS: the wavefront loads (and finally stores) runs of consecutive positions of one item with
+    // consecutive lanes, so that its memory instructions touch few lines, and each lane then reads its own
+    // item's positions from LDS.
+    template<int OP, typename T>
+    constexpr int lane_chunk()
+    {
+        return OP == op_kinematic_wave ? (std::is_same_v<T, float> ? 4 : 2) : 8;
+    }
+
+    template<int OP, typename T>
+    struct LaneWalkLds
+    {
+        static constexpr int C = lane_chunk<OP, T>();
+        static constexpr int stride = C + 1;  // per item (padded against bank conflicts)
+        static constexpr int n = 64 * stride;
+        double value[n];  // side sums, then the outflows
+        std::uint32_t push[n];
+        T rem[is_split_op(OP) ? n : 1];
+        CellInput<OP, T> in[OP == op_accu ? 1 : n];
+        std::uint8_t start[n];
+    };
+
+    template<int OP, typename T>
+    __device__ inline void run_items_lane_walk(SweepArgs<T> const& a, ChainRun<OP, T> const& c, LaneWalkLds<OP, T>& s,
+                                               std::uint32_t first_item, std::uint32_t item_end)
+    {
+        using L = LaneWalkLds<OP, T>;
+        constexpr int C = L::C;
+        constexpr bool two_outputs = is_split_op(OP);
+        int const lane = threadIdx.x & 63;
+        std::uint32_t const item = first_item + static_cast<std::uint32_t>(lane);
+        bool const mine = item < item_end;
+        std::uint32_t const start = mine ? as_global(c.item_start)[item] : 0u;
+        std::uint32_t const count = mine ? as_global(c.item_count)[item] : 0u;
+        std::uint32_t most = count;
+        for (int offset = 32; offset > 0; offset >>= 1)
+        {
+            std::uint32_t const other = __shfl_xor(most, offset, 64);
+            most = other > most ? other : most;
+        }
+        double carry = 0.0;
+        for (std::uint32_t k0 = 0; k0 < most; k0 += C)
+        {
+            // load: record r = j * 64 + lane is position k0 + r % C of the wavefront's item r / C
+#pragma unroll
+            for (int j = 0; j < C; ++j)
+            {
+                int const r = j * 64 + lane;
+                int const il = r / C;
+                int const pos = r % C;
+                std::uint32_t const s_il = __shfl(start, il, 64);
+                std::uint32_t const n_il = __shfl(count, il, 64);
+                std::uint32_t const k = k0 + static_cast<std::uint32_t>(pos);
+                if (k < n_il)
+                {
+                    std::uint32_t const p = s_il + k;
+                    int const idx = il * L::stride + pos;
+                    s.value[idx] = as_global(c.side)[p];
+                    s.push[idx] = as_global(c.push_slot)[p];
+                    s.start[idx] = as_global(c.seg_start)[p];
+                    if constexpr (OP != op_accu)
+                        s.in[idx] = c.staged[p];  // accu: the inflow is in the side sum
+                }
+            }
+            __syncthreads();
+            // walk: this lane's item, positions k0 .. k0 + C - 1
+#pragma unroll
+            for (int pos = 0; pos < C; ++pos)
+            {
+                if (k0 + static_cast<std::uint32_t>(pos) >= count)
+                    break;
+                int const idx = lane * L::stride + pos;
+                double const up = (s.start[idx] ? 0.0 : carry) + s.value[idx];
+                CellInput<OP, T> in{};
+                if constexpr (OP != op_accu)
+                    in = s.in[idx];
+                double value;
+                T rem{};
+                solve_lane<OP, T>(a, in, up, value, rem);
+                s.value[idx] = value;
+                if constexpr (two_outputs)
+                    s.rem[idx] = rem;
+                std::uint32_t const push = s.push[idx];
+                if (push != no_slot)
+                    as_global(c.slot)[push] = value;
+                carry = value;
+            }
+            __syncthreads();
+            // store the outflows (and remainders) as they were loaded
+#pragma unroll
+            for (int j = 0; j < C; ++j)
+            {
+                int const r = j * 64 + lane;
+                int const il = r / C;
+                int const pos = r % C;
+                std::uint32_t const s_il = __shfl(start, il, 64);
+                std::uint32_t const n_il = __shfl(count, il, 64);
+                std::uint32_t const k = k0 + static_cast<std::uint32_t>(pos);
+                if (k < n_il)
+                {
+                    std::uint32_t const p = s_il + k;
+                    int const idx = il * L::stride + pos;
+                    as_global(c.res)[p] = static_cast<T>(s.value[idx]);
+                    if constexpr (two_outputs)
+                        as_global(c.res_rem)[p] = s.rem[idx];
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // One level of chains (or its short items), 64 work items per workgroup of one wavefront.
+    template<int OP, typename T>
+    __global__ void __launch_bounds__(64)
+        chains_run_lanes_kernel(SweepArgs<T> a, ChainRun<OP, T> c, std::uint32_t item_begin, std::uint32_t item_end)
+    {
+        __shared__ LaneWalkLds<OP, T> s;
+        run_items_lane_walk<OP, T>(a, c, s, item_begin + blockIdx.x * 64u, item_end);
+    }
+
+    // One level of chains whose work items are sorted longest first: items [item_begin, item_split) one per
+    // workgroup of one wavefront (the first workgroups, so the long items start first), the short ones
+    // [item_split, item_end) 64 per workgroup by the lane walk.
+    template<int OP, typename T>
+    __global__ void __launch_bounds__(64)
+        chains_run_mixed_kernel(SweepArgs<T> a, ChainRun<OP, T> c, std::uint32_t item_begin, std::uint32_t item_split,
+                                std::uint32_t item_end)
+    {
+        __shared__ LaneWalkLds<OP, T> s;
+        std::uint32_t const block_groups = item_split - item_begin;
+        if (blockIdx.x < block_groups)
+        {
+            run_item_blocks<OP, T>(a, c, item_begin + blockIdx.x);
+            return;
+        }
+        run_items_lane_walk<OP, T>(a, c, s, item_split + (blockIdx.x - block_groups) * 64u, item_end);
     }
 
     // Write the chain results of the pending cells of every tile to the output rasters. One wavefront per tile

@@ -581,6 +581,8 @@ namespace lue_amd {
         std::uint64_t nrefs{0};
         // chain levels whose longest work item has at least this many positions run on the wide path (0: none)
         std::uint32_t wide_min{wide_min_positions};
+        // work items of a chain level with at most this many positions run one per lane (0: none)
+        std::uint32_t lane_max{lane_max_positions};
         // the tile pass as a persistent launch fed by a tile work queue (tile_pass_queue_kernel)
         bool tile_queue{false};
         int cus{1};  // compute units of the device
@@ -588,6 +590,7 @@ namespace lue_amd {
         std::uint64_t tile_critical_blocks{0};  // of them on the tiles' critical paths (per level the most)
         std::vector<std::uint32_t> level_items;  // work items of level l: [level_items[l], level_items[l+1])
         std::vector<std::uint32_t> level_longest;  // positions of the longest work item of level l
+        std::vector<std::uint32_t> item_count_host;  // positions per work item (a copy of item_count)
         std::vector<std::uint32_t> level_pos;    // positions of level l: [level_pos[l], level_pos[l+1])
         DeviceArray<std::uint32_t> tile_off;     // per tile (+1): first compact index of its pending cells
         DeviceArray<std::uint32_t> pos_of;       // per compact index: chain position (recording only)
@@ -654,6 +657,17 @@ namespace lue_amd {
 
         auto rounds() const -> int { return static_cast<int>(round_offset.empty() ? 0 : round_offset.size() - 1); }
         auto levels() const -> int { return static_cast<int>(level_items.empty() ? 0 : level_items.size() - 1); }
+
+        // First work item of [begin, end) (one level, longest first) with at most lane_max positions (end: none).
+        auto lane_split(std::uint32_t begin, std::uint32_t end) const -> std::uint32_t
+        {
+            if (lane_max == 0u || end > item_count_host.size())
+                return end;
+            auto const first = item_count_host.begin() + begin;
+            auto const last = item_count_host.begin() + end;
+            auto const it = std::partition_point(first, last, [&](std::uint32_t n) { return n > lane_max; });
+            return begin + static_cast<std::uint32_t>(it - first);
+        }
 
         // Free the buffers on the null stream from now on (the recording stream may not outlive the network;
         // the owner synchronises the device before destroying it).
@@ -1200,11 +1214,13 @@ namespace lue_amd {
             }
             net.longest_chain = h_stats[2 * chains_max_levels];
             // inside a level the longest work item comes first
+            net.item_count_host.resize(nitems);
+            LUE_AMD_HIP(copy_to_host(net.item_count_host.data(), net.item_count.ptr, nitems, stream));
             net.level_longest.assign(nlevels, 0u);
             for (std::uint32_t l = 0; l < nlevels; ++l)
             {
                 if (net.level_items[l + 1] > net.level_items[l])
-                    LUE_AMD_HIP(copy_to_host(&net.level_longest[l], net.item_count.ptr + net.level_items[l], 1, stream));
+                    net.level_longest[l] = net.item_count_host[net.level_items[l]];
             }
             LUE_AMD_HIP(hipStreamSynchronize(stream));  // before the temporaries go
         }
@@ -1376,16 +1392,32 @@ namespace lue_amd {
             bool const wide = has_wide_path<OP>() && net.wide_min > 0 &&
                               static_cast<std::size_t>(l) < net.level_longest.size() &&
                               net.level_longest[static_cast<std::size_t>(l)] >= net.wide_min;
+            // whether an item is walked by a lane depends on its own length only (up to lane_max positions),
+            // not on the other items of its level, which a batch of catchments changes
+            std::uint32_t const split = net.lane_split(begin, end);
             if (wide)
             {
                 if constexpr (has_wide_path<OP>())
-                    hipLaunchKernelGGL((chains_run_wide_kernel<OP, T>), dim3(end - begin), dim3(wide_threads), 0, stream,
-                                       a, c, begin);
+                {
+                    if (split > begin)
+                        hipLaunchKernelGGL((chains_run_wide_kernel<OP, T>), dim3(split - begin), dim3(wide_threads), 0,
+                                           stream, a, c, begin);
+                    if (end > split)
+                        hipLaunchKernelGGL((chains_run_lanes_kernel<OP, T>), dim3((end - split + 63) / 64), dim3(64), 0,
+                                           stream, a, c, split, end);
+                }
             }
             else
             {
-                hipLaunchKernelGGL((chains_run_kernel<OP, T>), dim3((end - begin + 3) / 4), dim3(256), 0, stream, a, c,
-                                   begin, end);
+                if (split >= end)
+                    hipLaunchKernelGGL((chains_run_kernel<OP, T>), dim3((end - begin + 3) / 4), dim3(256), 0, stream, a,
+                                       c, begin, end);
+                else if (split == begin)
+                    hipLaunchKernelGGL((chains_run_lanes_kernel<OP, T>), dim3((end - begin + 63) / 64), dim3(64), 0,
+                                       stream, a, c, begin, end);
+                else
+                    hipLaunchKernelGGL((chains_run_mixed_kernel<OP, T>), dim3((split - begin) + (end - split + 63) / 64),
+                                       dim3(64), 0, stream, a, c, begin, split, end);
             }
         }
         constexpr int tiles_per_block = scatter_tiles_per_block<OP, T, TW, TH>();

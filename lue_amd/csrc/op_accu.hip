@@ -103,7 +103,8 @@ int lue_amd_network_info(lue_amd_network network, int64_t* info, int n)
                               static_cast<int64_t>(net.tile_blocks),
                               static_cast<int64_t>(net.tile_critical_blocks),
                               static_cast<int64_t>(net.wide_min),
-                              net.tile_queue ? 1 : 0};
+                              net.tile_queue ? 1 : 0,
+                              static_cast<int64_t>(net.lane_max)};
     for (int i = 0; i < n && i < static_cast<int>(sizeof(values) / sizeof(values[0])); ++i)
         info[i] = values[i];
     return status_success;
@@ -167,6 +168,16 @@ int lue_amd_network_set_wide_threshold(lue_amd_network network, int64_t position
     if (positions < 0 || positions > INT32_MAX)
         return fail(status_invalid_argument, "wide threshold out of range");
     network->net.wide_min = static_cast<std::uint32_t>(positions);
+    return status_success;
+}
+
+int lue_amd_network_set_lane_limit(lue_amd_network network, int64_t positions)
+{
+    if (!network)
+        return fail(status_invalid_argument, "null argument");
+    if (positions < 0 || positions > INT32_MAX)
+        return fail(status_invalid_argument, "lane limit out of range");
+    network->net.lane_max = static_cast<std::uint32_t>(positions);
     return status_success;
 }
 

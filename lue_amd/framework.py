@@ -433,7 +433,7 @@ class FlowNetwork:
 
         keys = ["catchments", "cells", "tiles", "rounds", "pending_cells", "chains", "levels", "no_data_cells",
                 "tile_shape", "schedule", "longest_chain", "side_refs", "tile_blocks", "tile_critical_blocks",
-                "wide_threshold", "tile_queue"]
+                "wide_threshold", "tile_queue", "lane_limit"]
         values = (ctypes.c_int64 * len(keys))()
         _lib.check(_lib.load().lue_amd_network_info(self._handle, values, len(keys)))
         return dict(zip(keys, list(values)))
@@ -443,6 +443,13 @@ class FlowNetwork:
         workgroup of 16 wavefronts (0: none); the results are the same either way. Applies to the
         operations and plans made from then on."""
         _lib.check(_lib.load().lue_amd_network_set_wide_threshold(self._handle, int(positions)))
+
+    def set_lane_limit(self, positions):
+        """Work items of a chain level with at most `positions` positions are walked one per lane, cell after
+        cell, the longer ones one per wavefront or workgroup (0, the default: none). The results agree with
+        the other schedule to the operations' tolerances; whether an item is walked depends on its length
+        only, not on the other catchments of a batch. Applies to the operations and plans made from then on."""
+        _lib.check(_lib.load().lue_amd_network_set_lane_limit(self._handle, int(positions)))
 
     def set_tile_queue(self, on):
         """Launch the first round as a persistent grid fed by a tile work queue (True) or one workgroup per

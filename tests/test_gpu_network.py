@@ -165,3 +165,59 @@ def test_network_capacity_trigger(fw, oracle_mod, kind, schedule):
             assert (np.isnan(outs[i].numpy()) == np.isnan(w_out)).all()
             assert np.nanmax(np.abs(outs[i].numpy() - w_out) / scale) <= tol
             assert np.nanmax(np.abs(rems[i].numpy() - w_rem) / scale) <= tol
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_lane_walk_matches_oracle(fw, oracle_mod, dtype):
+    """The lane schedule of the chain levels (lue_amd_network_set_lane_limit): work items up to the limit
+    walked one per lane, cell after cell. Every operation against the oracle at the tolerances of the
+    default schedule, with a catchment alone equal, bitwise, to the same catchment in a batch."""
+    from lue_amd import _lib
+
+    shapes = [(333, 257), (700, 410), (96, 520)]
+    cs = [catchment(640 + i, *s) for i, s in enumerate(shapes)]
+    ldds = [pa(fw, c["ldd"]) for c in cs]
+    batch = fw.FlowNetwork(ldds)
+    alone = fw.FlowNetwork(ldds[1])
+    for net in (batch, alone):
+        assert net.info()["lane_limit"] == 0
+        net.set_lane_limit(1 << 20)  # every item of a level not on the wide path
+        assert net.info()["lane_limit"] == 1 << 20
+    tol = 1e-6 if dtype == np.float32 else 1e-12
+    mats = [pa(fw, c["material"].astype(dtype)) for c in cs]
+    params = [pa(fw, c["threshold"].astype(dtype)) for c in cs]
+    outs = batch.accu(mats)
+    torch.cuda.synchronize()
+    assert np.array_equal(outs[1].numpy(), alone.accu(mats[1]).numpy(), equal_nan=True)
+    for i, c in enumerate(cs):
+        want = oracle_mod.accu(c["ldd"], c["material"].astype(dtype).astype(np.float64), None, np.float64)
+        assert rel_err(outs[i].numpy(), want).max() <= tol
+    for kind in ("threshold", "capacity", "fraction", "trigger"):
+        p = params if kind != "trigger" else [pa(fw, (c["threshold"] * 4.0).astype(dtype)) for c in cs]
+        o_b, r_b = getattr(batch, f"accu_{kind}")(mats, p)
+        o_a, r_a = getattr(alone, f"accu_{kind}")(mats[1], p[1])
+        torch.cuda.synchronize()
+        assert np.array_equal(o_b[1].numpy(), o_a.numpy(), equal_nan=True)
+        assert np.array_equal(r_b[1].numpy(), r_a.numpy(), equal_nan=True)
+        for i, c in enumerate(cs):
+            w_out, w_rem = getattr(oracle_mod, f"accu_{kind}")(
+                c["ldd"], c["material"].astype(dtype).astype(np.float64),
+                p[i].numpy().astype(np.float64), None, np.float64)
+            scale = np.abs(w_out) + np.abs(w_rem) + 1e-30
+            assert np.nanmax(np.abs(o_b[i].numpy() - w_out) / scale) <= tol, kind
+            assert np.nanmax(np.abs(r_b[i].numpy() - w_rem) / scale) <= tol, kind
+    qs = [[pa(fw, c[k].astype(dtype)) for k in ("q_cur", "q_lat", "alpha")] for c in cs]
+    q_b = batch.kinematic_wave([q[0] for q in qs], [q[1] for q in qs], [q[2] for q in qs], 0.6, 3600.0, 100.0)
+    q_a = alone.kinematic_wave(*qs[1], 0.6, 3600.0, 100.0)
+    torch.cuda.synchronize()
+    assert np.array_equal(q_b[1].numpy(), q_a.numpy(), equal_nan=True)
+    beta = BETA32 if dtype == np.float32 else 0.6
+    for i, c in enumerate(cs):
+        want_q = oracle_mod.kinematic_wave(c["ldd"], *[c[k].astype(dtype).astype(np.float64)
+                                                       for k in ("q_cur", "q_lat", "alpha")],
+                                           beta, 3600.0, 100.0, None, np.float64)
+        assert rel_err(q_b[i].numpy(), want_q).max() <= (1e-6 if dtype == np.float32 else 1e-12)
+    with pytest.raises(_lib.RoutingError):
+        batch.set_lane_limit(-1)
+    batch.close()
+    alone.close()
