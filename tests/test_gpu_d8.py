@@ -95,3 +95,24 @@ def test_d8_unaligned_rows_of_four(fw, oracle_mod, dtype):
     assert view.data_ptr() % 16 != 0
     got = fw.d8_flow_direction(fw.PartitionedArray(view)).numpy()
     assert np.array_equal(got, oracle_mod.d8_flow_direction(dem, dtype))
+
+
+@pytest.mark.parametrize("offset", [1, 2, 3])
+def test_d8_output_off_the_word_grid(fw, oracle_mod, offset):
+    """An ldd output pointer 1 to 3 bytes past a 4-byte boundary (a uint8 view sliced [offset:]), written
+    through the C ABI: the LDS-tile kernel stores whole words only where the address is word-aligned."""
+    from lue_amd import _lib
+
+    rows, cols = 45, 130
+    rng = np.random.default_rng(17 + offset)
+    dem = rng.integers(-3, 6, (rows, cols)).astype(np.float32)
+    elevation = torch.from_numpy(dem).cuda()
+    flat = torch.full((rows * cols + 8,), 77, dtype=torch.uint8, device="cuda")
+    out = flat[offset:offset + rows * cols]
+    assert out.data_ptr() % 4 == offset % 4
+    _lib.check(_lib.load().lue_amd_d8_flow_direction_f32(elevation.data_ptr(), rows, cols, out.data_ptr(),
+                                                        torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    got = flat.cpu().numpy()
+    assert np.array_equal(got[offset:offset + rows * cols].reshape(rows, cols), oracle_mod.d8_flow_direction(dem, np.float32))
+    assert (got[:offset] == 77).all() and (got[offset + rows * cols:] == 77).all()
