@@ -39,12 +39,12 @@ namespace lue_amd {
         std::int16_t parent[G::n];       // local downstream cell in the tile, -1 if none
         std::int16_t heavy[G::n];        // heavy child, -1 if none
         std::uint8_t heavy_dir[G::n];
-        std::uint32_t count[G::n];       // children still to be sized
+        alignas(16) std::uint32_t count[G::n];  // children still to be sized; then the packing's head info
         std::uint32_t size[G::n];        // local cells upstream, including the cell
         alignas(4) std::uint8_t children[G::n];  // bit k: neighbour k is a local child (set by word-wide ORs)
         std::uint16_t heads[G::n];       // chain heads in cell order
         std::uint32_t wave_heads[G::n / 64];  // chain heads per 64 cells (prefix: offsets into heads)
-        std::uint16_t queue[G::n];
+        alignas(4) std::uint16_t queue[G::n];   // leaves to size; then the part of each head
         std::int16_t head_of[G::n];
         std::uint16_t depth[G::n];       // distance from the chain head (its most downstream cell)
         std::uint16_t length[G::n];      // per head: cells in the chain
@@ -296,6 +296,26 @@ namespace lue_amd {
             }
         }
         __syncthreads();
+        // per head (cell order): its level << 16 | its length, padded to a multiple of 4 entries with a level no
+        // lane packs, in `count` (free since the subtree sizes), so that the packing below reads four heads per
+        // LDS load; the part chosen for each head goes to `queue` (free as well), a byte per head
+        std::uint32_t* const hinfo = s.count;
+        std::uint8_t* const hpart = reinterpret_cast<std::uint8_t*>(s.queue);
+        {
+            std::uint32_t const nheads = s.nheads;
+            std::uint32_t const padded = (nheads + 3u) & ~3u;
+            for (std::uint32_t h = static_cast<std::uint32_t>(tid); h < padded; h += static_cast<std::uint32_t>(nthreads))
+            {
+                std::uint32_t v = 0xFFFF0000u;
+                if (h < nheads)
+                {
+                    int const i = s.heads[h];
+                    v = (s.level[i] << 16) | s.length[i];
+                }
+                hinfo[h] = v;
+            }
+        }
+        __syncthreads();
         if (tid < 64)
         {
             unsigned int const levels = s.levels;
@@ -312,13 +332,17 @@ namespace lue_amd {
                     cap[w] = 64 * ((blocks + sweep_waves - 1 - w) / sweep_waves);
                     part[w] = 0;
                 }
-                // pass 1: the part of every chain of level l (kept in base[] until pass 2)
-                for (std::uint32_t h = 0; h < nheads; ++h)
+                // pass 1: the part of every chain of level l (hpart)
+                for (std::uint32_t h4 = 0; h4 < nheads; h4 += 4)
                 {
-                    int const i = s.heads[h];
-                    if (s.level[i] != l)
+                    uint4 const info = *reinterpret_cast<uint4 const*>(hinfo + h4);
+                    std::uint32_t const quad[4] = {info.x, info.y, info.z, info.w};
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                    {
+                    if ((quad[k] >> 16) != l)
                         continue;
-                    std::uint32_t const len = s.length[i];
+                    std::uint32_t const len = quad[k] & 0xFFFFu;
                     int chosen = -1;
                     int roomiest = 0;
                     std::int64_t room_best = static_cast<std::int64_t>(cap[0]) - part[0];
@@ -339,7 +363,8 @@ namespace lue_amd {
 #pragma unroll
                     for (int w = 0; w < sweep_waves; ++w)
                         part[w] += w == chosen ? len : 0u;
-                    s.base[i] = static_cast<std::uint16_t>(chosen);
+                    hpart[h4 + static_cast<std::uint32_t>(k)] = static_cast<std::uint8_t>(chosen);
+                    }
                 }
                 // part boundaries: level l starts after the cells of the levels below it
                 std::uint32_t at = 0;
@@ -355,23 +380,29 @@ namespace lue_amd {
                 }
                 s.split[l * (sweep_waves + 1) + sweep_waves] = static_cast<std::uint16_t>(at);
                 // pass 2: positions
-                for (std::uint32_t h = 0; h < nheads; ++h)
+                for (std::uint32_t h4 = 0; h4 < nheads; h4 += 4)
                 {
-                    int const i = s.heads[h];
-                    if (s.level[i] != l)
-                        continue;
-                    int const w = s.base[i];
-                    std::uint32_t cursor = 0;
+                    uint4 const info = *reinterpret_cast<uint4 const*>(hinfo + h4);
+                    std::uint32_t const quad[4] = {info.x, info.y, info.z, info.w};
+                    std::uint32_t const parts = *reinterpret_cast<std::uint32_t const*>(hpart + h4);
 #pragma unroll
-                    for (int k = 0; k < sweep_waves; ++k)
+                    for (int k = 0; k < 4; ++k)
                     {
-                        if (k == w)
+                        if ((quad[k] >> 16) != l)
+                            continue;
+                        int const w = static_cast<int>((parts >> (8 * k)) & 0xFFu);
+                        std::uint32_t cursor = 0;
+#pragma unroll
+                        for (int j = 0; j < sweep_waves; ++j)
                         {
-                            cursor = part[k];
-                            part[k] += s.length[i];
+                            if (j == w)
+                            {
+                                cursor = part[j];
+                                part[j] += quad[k] & 0xFFFFu;
+                            }
                         }
+                        s.base[s.heads[h4 + static_cast<std::uint32_t>(k)]] = static_cast<std::uint16_t>(cursor);
                     }
-                    s.base[i] = static_cast<std::uint16_t>(cursor);
                 }
             }
             else if (l < static_cast<unsigned int>(tile_max_levels))
