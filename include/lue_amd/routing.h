@@ -88,7 +88,7 @@ int lue_amd_set_sweep_options(int schedule);
 /*
  * Per-call entry points keep the schedules they record (no reference counterpart; LUE recomputes the
  * inflow counts in every call): up to `entries` of them (default 4, 0 disables), found again by device,
- * ldd pointer, shape, tile shape, schedule and a 64-bit fingerprint of the ldd bytes computed in every
+ * ldd pointer, shape, tile shape, schedule and a 128-bit fingerprint of the ldd bytes computed in every
  * call, so an ldd changed in place is recorded anew.
  */
 int lue_amd_set_network_cache(int entries);

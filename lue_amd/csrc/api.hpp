@@ -107,7 +107,7 @@ namespace lue_amd {
         std::int64_t cols;
         int tile_shape;
         int schedule;
-        std::uint64_t fingerprint;
+        std::uint64_t fingerprint[2];
 
         bool operator==(CacheKey const&) const = default;
     };
@@ -172,8 +172,8 @@ namespace lue_amd {
 
     NetworkCache& network_cache();
 
-    // 64-bit fingerprint of the ldd bytes (fingerprint_kernel); waits for it.
-    int ldd_fingerprint(std::uint8_t const* ldd, std::int64_t n, hipStream_t stream, std::uint64_t& fingerprint);
+    // 128-bit fingerprint of the ldd bytes (fingerprint_kernel); waits for it.
+    int ldd_fingerprint(std::uint8_t const* ldd, std::int64_t n, hipStream_t stream, std::uint64_t (&fingerprint)[2]);
 
     // One operation on one raster: replay a cached network of the ldd, or record one.
     template<int OP, typename T>
@@ -183,7 +183,7 @@ namespace lue_amd {
         std::vector<CatchmentIO<T>> table{io};
         int const shape = default_tile_shape<OP>();
         NetworkCache& cache = network_cache();
-        CacheKey key{0, ldd, rows, cols, shape, default_schedule(), 0};
+        CacheKey key{0, ldd, rows, cols, shape, default_schedule(), {0, 0}};
         bool const caching = cache.capacity() > 0;
         if (caching)
         {

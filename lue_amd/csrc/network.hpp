@@ -698,14 +698,17 @@ namespace lue_amd {
         }
     };
 
-    // 64-bit fingerprint of n bytes: sum over 32-bit words of splitmix64(word + index * golden ratio)
-    // (order-sensitive, reduction-friendly); word i holds bytes 4i .. 4i + 3 (little endian), the tail
-    // zero-padded. Whole-word loads where `bytes` is 4-byte aligned, byte loads otherwise.
+    // 128-bit fingerprint of n bytes: two sums over the 32-bit words, of splitmix64(word + index * golden ratio)
+    // and of murmur3's fmix64(word ^ index * another odd constant) (order-sensitive, reduction-friendly; two
+    // independent 64-bit sums, so that two different ldds share a fingerprint with odds of about 2^-128); word i
+    // holds bytes 4i .. 4i + 3 (little endian), the tail zero-padded. Whole-word loads where `bytes` is 4-byte
+    // aligned, byte loads otherwise. out[0], out[1]: the two sums (cleared before the launch).
     __global__ inline void fingerprint_kernel(std::uint8_t const* bytes, std::uint64_t n, unsigned long long* out)
     {
         std::uint64_t const words = (n + 3) / 4;
         bool const aligned = (reinterpret_cast<std::uintptr_t>(bytes) & 3u) == 0;
         std::uint64_t sum = 0;
+        std::uint64_t sum2 = 0;
         for (std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < words;
              i += static_cast<std::uint64_t>(gridDim.x) * blockDim.x)
         {
@@ -724,11 +727,21 @@ namespace lue_amd {
             z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
             z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
             sum += z ^ (z >> 31);
+            std::uint64_t y = static_cast<std::uint64_t>(w) ^ ((i + 1) * 0xD6E8FEB86659FD93ull);
+            y = (y ^ (y >> 33)) * 0xFF51AFD7ED558CCDull;
+            y = (y ^ (y >> 33)) * 0xC4CEB9FE1A85EC53ull;
+            sum2 += y ^ (y >> 33);
         }
         for (int offset = 32; offset > 0; offset >>= 1)
+        {
             sum += __shfl_down(sum, offset, 64);
+            sum2 += __shfl_down(sum2, offset, 64);
+        }
         if ((threadIdx.x & 63) == 0)
+        {
             atomicAdd(out, static_cast<unsigned long long>(sum));
+            atomicAdd(out + 1, static_cast<unsigned long long>(sum2));
+        }
     }
 
     // Keep memory freed by the operations' stream-ordered buffers in the device's pool (instead of returning

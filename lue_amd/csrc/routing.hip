@@ -68,19 +68,20 @@ namespace lue_amd {
         return *cache;
     }
 
-    int ldd_fingerprint(std::uint8_t const* ldd, std::int64_t n, hipStream_t stream, std::uint64_t& fingerprint)
+    int ldd_fingerprint(std::uint8_t const* ldd, std::int64_t n, hipStream_t stream, std::uint64_t (&fingerprint)[2])
     {
         ScratchArray<unsigned long long> sum;
-        LUE_AMD_HIP(sum.allocate(1, stream));
+        LUE_AMD_HIP(sum.allocate(2, stream));
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<std::uint32_t*>(sum.ptr), 0u,
-                           2ull);
+                           4ull);
         unsigned const grid = static_cast<unsigned>(std::min<std::int64_t>((n / 4 + 255) / 256 + 1, 4096));
         hipLaunchKernelGGL(fingerprint_kernel, dim3(grid), dim3(256), 0, stream, ldd, static_cast<std::uint64_t>(n),
                            sum.ptr);
         LUE_AMD_HIP(hipGetLastError());
-        unsigned long long value = 0;
-        LUE_AMD_HIP(copy_to_host(&value, sum.ptr, 1, stream));
-        fingerprint = value;
+        unsigned long long value[2] = {0, 0};
+        LUE_AMD_HIP(copy_to_host(value, sum.ptr, 2, stream));
+        fingerprint[0] = value[0];
+        fingerprint[1] = value[1];
         return status_success;
     }
 
