@@ -79,7 +79,10 @@ namespace lue_amd {
         // chain positions (null stage_off: no staging).
         std::uint32_t const* stage_off;  // per tile (+1): first compact index of its pending cells
         std::uint32_t const* stage_pos;  // per staging entry (tile after tile): chain position
-        void* staged;                    // per chain position: CellInput<OP, T>
+        void* staged;                    // per chain position: CellInput<OP, T> (or its compact form)
+        // the FP32 kinematic wave with beta and the channel length the scalars of every catchment: the staged
+        // inputs are CompactKwInput (16 B), beta and c1 being beta_s and c1_s
+        int compact_stage;
         double* stage_side;              // per chain position: FP64 sum of the same-tile upstream cells solved
 
         // Chain schedule, round 0 of a run (tile_pass.hpp): in-tile chain layout and the rest of the tiles'
@@ -206,6 +209,45 @@ namespace lue_amd {
         float b_hi;   // b = alpha q_cur^beta + dt q_lat = b_hi + b_lo
         float b_lo;
     };
+
+    // The staged form of CellInput<op_kinematic_wave, float> where beta and c1 are the scalars of the call.
+    struct alignas(16) CompactKwInput
+    {
+        float q_cur;
+        float alpha;
+        float b_hi;
+        float b_lo;
+    };
+
+    // Store / load the staged inputs of a chain position (SweepArgs::compact_stage selects the compact form for
+    // the FP32 kinematic wave).
+    template<int OP, typename T>
+    __device__ inline void store_staged(SweepArgs<T> const& a, std::uint32_t p, CellInput<OP, T> const& in)
+    {
+        if constexpr (OP == op_kinematic_wave && std::is_same_v<T, float>)
+        {
+            if (a.compact_stage)
+            {
+                static_cast<CompactKwInput*>(a.staged)[p] = CompactKwInput{in.q_cur, in.alpha, in.b_hi, in.b_lo};
+                return;
+            }
+        }
+        static_cast<CellInput<OP, T>*>(a.staged)[p] = in;
+    }
+
+    template<int OP, typename T>
+    __device__ inline CellInput<OP, T> load_staged(SweepArgs<T> const& a, void const* staged, std::uint32_t p)
+    {
+        if constexpr (OP == op_kinematic_wave && std::is_same_v<T, float>)
+        {
+            if (a.compact_stage)
+            {
+                CompactKwInput const v = static_cast<CompactKwInput const*>(staged)[p];
+                return CellInput<OP, T>{v.q_cur, v.alpha, a.beta_s, a.c1_s, v.b_hi, v.b_lo};
+            }
+        }
+        return static_cast<CellInput<OP, T> const*>(staged)[p];
+    }
 
     template<typename T>
     __device__ inline double kw_b(CellInput<op_kinematic_wave, T> const& c)

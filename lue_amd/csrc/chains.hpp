@@ -815,7 +815,7 @@ namespace lue_amd {
             d.extra = as_global(c.side)[p];
             d.push = as_global(c.push_slot)[p];
             if constexpr (OP != op_accu)
-                d.in = c.staged[p];  // accu: the inflow is in the side sum, d.in.x stays 0
+                d.in = load_staged<OP, T>(a, c.staged, p);  // accu: the inflow is in the side sum, d.in.x stays 0
         }
         return d;
     }
@@ -941,7 +941,7 @@ namespace lue_amd {
                     s.push[idx] = as_global(c.push_slot)[p];
                     s.start[idx] = as_global(c.seg_start)[p];
                     if constexpr (OP != op_accu)
-                        s.in[idx] = c.staged[p];  // accu: the inflow is in the side sum
+                        s.in[idx] = load_staged<OP, T>(a, c.staged, p);  // accu: the inflow is in the side sum
                 }
             }
             __syncthreads();

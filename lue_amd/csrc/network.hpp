@@ -1555,6 +1555,13 @@ namespace lue_amd {
         a.stage_pos = nullptr;
         a.staged = nullptr;
         a.stage_side = nullptr;
+        a.compact_stage = 0;
+        if constexpr (OP == op_kinematic_wave && std::is_same_v<T, float>)
+        {
+            a.compact_stage = 1;
+            for (std::size_t k = 0; k < net.cats_host.size(); ++k)
+                a.compact_stage = a.compact_stage && io_host[k].beta == nullptr && io_host[k].dx == nullptr;
+        }
         a.tile_order = nullptr;
         a.tile_flags = nullptr;
         a.tile_split = nullptr;

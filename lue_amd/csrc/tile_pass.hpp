@@ -273,7 +273,6 @@ namespace lue_amd {
         // that a wavefront's stores cover runs of consecutive positions): inputs and the FP64 sum (neighbour
         // order) of their upstream cells that this tile solved, at their chain positions (accu: that sum plus
         // the cell's inflow, its only input)
-        CellInput<OP, T>* const staged = static_cast<CellInput<OP, T>*>(a.staged);
 #pragma unroll
         for (int j = 0; j < per_thread; ++j)
         {
@@ -290,7 +289,7 @@ namespace lue_amd {
             }
             else
             {
-                staged[pos_of[j]] = s.in[i];
+                store_staged<OP, T>(a, pos_of[j], s.in[i]);
                 as_global(a.stage_side)[pos_of[j]] = side;
             }
         }
