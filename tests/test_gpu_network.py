@@ -221,3 +221,24 @@ def test_lane_walk_matches_oracle(fw, oracle_mod, dtype):
         batch.set_lane_limit(-1)
     batch.close()
     alone.close()
+
+
+def test_kinematic_wave_scalar_and_raster_beta_dx_agree(fw):
+    """FP32 kinematic wave: beta and the channel length as scalars (the pending cells' inputs staged in 16 B,
+    beta and c1 taken from the scalars) or as rasters holding the same values (the inputs staged whole) give
+    the same bits, on a network and through the per-call entry point."""
+    c = catchment(71, 700, 410)
+    shape = c["ldd"].shape
+    ldd = pa(fw, c["ldd"])
+    net = fw.FlowNetwork(ldd)
+    assert net.info()["pending_cells"] > 0
+    q = [pa(fw, c[k]) for k in ("q_cur", "q_lat", "alpha")]
+    beta_r = pa(fw, np.full(shape, np.float32(0.6), np.float32))
+    dx_r = pa(fw, np.full(shape, np.float32(100.0), np.float32))
+    scalars = net.kinematic_wave(*q, 0.6, 3600.0, 100.0).numpy()
+    for beta, dx in ((beta_r, dx_r), (0.6, dx_r), (beta_r, 100.0)):
+        got = net.kinematic_wave(*q, beta, 3600.0, dx).numpy()
+        assert np.array_equal(got, scalars, equal_nan=True)
+    per_call = fw.kinematic_wave(ldd, *q, beta_r, 3600.0, dx_r).numpy()
+    assert np.array_equal(per_call, fw.kinematic_wave(ldd, *q, 0.6, 3600.0, 100.0).numpy(), equal_nan=True)
+    net.close()
