@@ -389,6 +389,74 @@ namespace lue_amd {
         last[e] = rlast[r];
     }
 
+    // Second level of the ruling set, over the rulers' own list (rnext: the next ruler or -1, rsum: the sum of
+    // the ruler's sublist, rlast: the list's last element where the sublist ends the list): walk of
+    // super-ruler q (ruler super_elem[q]) to the next super-ruler or the list's end, recording for every ruler
+    // its super-ruler and the inclusive prefix of the sums from it.
+    __global__ inline void rank2_walk_kernel(std::int32_t const* rnext, std::int32_t const* rsum,
+                                             std::int32_t const* rlast, std::uint32_t const* is_super,
+                                             std::uint32_t const* super_rank, std::uint32_t const* super_elem,
+                                             std::uint32_t nsupers, std::int32_t* owner, std::int32_t* prefix,
+                                             std::int32_t* sub_sum, std::int32_t* next_super, std::int32_t* last_elem,
+                                             std::uint32_t* bad)
+    {
+        std::uint32_t const q = blockIdx.x * blockDim.x + threadIdx.x;
+        if (q >= nsupers)
+            return;
+        std::uint32_t x = super_elem[q];
+        std::int32_t acc = 0;
+        std::int32_t next = -1;
+        std::int32_t last = -1;
+        for (std::uint32_t step = 0;; ++step)
+        {
+            if (step == ruler_walk_limit)
+            {
+                atomicAdd(bad, 1u);
+                break;
+            }
+            acc += rsum[x];
+            owner[x] = static_cast<std::int32_t>(q);
+            prefix[x] = acc;
+            std::int32_t const s = rnext[x];
+            if (s < 0)
+            {
+                last = rlast[x];
+                break;
+            }
+            if (is_super[s])
+            {
+                next = static_cast<std::int32_t>(super_rank[s]);
+                break;
+            }
+            x = static_cast<std::uint32_t>(s);
+        }
+        sub_sum[q] = acc;
+        next_super[q] = next;
+        last_elem[q] = last;
+    }
+
+    // The rulers' sums to their list's end and the list's last element from their super-rulers' (qv, qlast);
+    // rulers no super-ruler reached lie on a cycle (`bad`).
+    __global__ inline void rank2_rulers_kernel(std::int32_t const* owner, std::int32_t const* prefix,
+                                               std::int32_t const* rsum, std::int32_t const* qv,
+                                               std::int32_t const* qlast, std::uint32_t nrulers, std::int32_t* rv,
+                                               std::int32_t* rlast_out, std::uint32_t* bad)
+    {
+        std::uint32_t const r = blockIdx.x * blockDim.x + threadIdx.x;
+        if (r >= nrulers)
+            return;
+        std::int32_t const q = owner[r];
+        if (q < 0)
+        {
+            atomicAdd(bad, 1u);
+            rv[r] = 0;
+            rlast_out[r] = -1;
+            return;
+        }
+        rv[r] = qv[q] - prefix[r] + rsum[r];
+        rlast_out[r] = qlast[q];
+    }
+
     // Counts entries of `perm` that are out of range or repeated (perm must be a permutation of [0, n)).
     __global__ inline void chains_check_permutation_kernel(std::uint32_t const* perm, std::uint32_t n,
                                                            std::uint32_t* seen, std::uint32_t* bad)

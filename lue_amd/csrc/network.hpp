@@ -989,23 +989,68 @@ namespace lue_amd {
                            ruler_rank.ptr, ruler_elem.ptr, nrulers, owner.ptr, prefix.ptr, rvalue[0].ptr, rnext[0].ptr,
                            rlast[0].ptr, bad.ptr);
         LUE_AMD_HIP(hipGetLastError());
-        // pointer jumping over the rulers: value = sum of the sublists from the ruler to the list's end,
-        // last = the list's last element (known at the list's final ruler, carried back)
+        // the rulers' list ranked by a second ruling set: super-rulers (the rulers no ruler links to and a
+        // pseudo-random 1/64 of the others) walk their sublists of rulers, the super-rulers are ranked by
+        // pointer jumping (value = sum of the sublists from the super-ruler to the list's end, last = the
+        // list's last element, known at the list's final super-ruler and carried back), and every ruler's
+        // rank follows from its super-ruler's. rvalue[1] / rlast[1]: the rulers' results.
+        DeviceArray<std::uint32_t> has_pred2, is_super, super_rank, super_elem;
+        DeviceArray<std::int32_t> owner2, prefix2;
+        LUE_AMD_HIP(has_pred2.allocate(nrulers, stream));
+        LUE_AMD_HIP(is_super.allocate(nrulers + 1, stream));
+        LUE_AMD_HIP(super_rank.allocate(nrulers + 1, stream));
+        LUE_AMD_HIP(owner2.allocate(nrulers, stream));
+        LUE_AMD_HIP(prefix2.allocate(nrulers, stream));
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(nrulers)), dim3(256), 0, stream, has_pred2.ptr, 0u,
+                           static_cast<std::uint64_t>(nrulers));
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, is_super.ptr + nrulers, 0u, 1ull);
+        hipLaunchKernelGGL(rank_mark_successors_kernel, dim3(blocks_for(nrulers)), dim3(256), 0, stream,
+                           rnext[0].ptr, nrulers, has_pred2.ptr);
+        hipLaunchKernelGGL(rank_rulers_kernel, dim3(blocks_for(nrulers)), dim3(256), 0, stream, has_pred2.ptr,
+                           nrulers, is_super.ptr, owner2.ptr);
+        LUE_AMD_HIP(hipGetLastError());
+        LUE_AMD_HIP(exclusive_scan(is_super.ptr, super_rank.ptr, static_cast<std::size_t>(nrulers) + 1, stream));
+        std::uint32_t nsupers = 0;
+        LUE_AMD_HIP(copy_to_host(&nsupers, super_rank.ptr + nrulers, 1, stream));
+        if (nsupers == 0)
+        {
+            cyclic = true;  // every list of rulers circles
+            return status_success;
+        }
+        DeviceArray<std::int32_t> qnext[2], qvalue[2], qlast[2];
+        LUE_AMD_HIP(super_elem.allocate(nsupers, stream));
+        for (int b = 0; b < 2; ++b)
+        {
+            LUE_AMD_HIP(qnext[b].allocate(nsupers, stream));
+            LUE_AMD_HIP(qvalue[b].allocate(nsupers, stream));
+            LUE_AMD_HIP(qlast[b].allocate(nsupers, stream));
+        }
+        hipLaunchKernelGGL(rank_ruler_elements_kernel, dim3(blocks_for(nrulers)), dim3(256), 0, stream, is_super.ptr,
+                           super_rank.ptr, nrulers, super_elem.ptr);
+        hipLaunchKernelGGL(rank2_walk_kernel, dim3(blocks_for(nsupers)), dim3(256), 0, stream, rnext[0].ptr,
+                           rvalue[0].ptr, rlast[0].ptr, is_super.ptr, super_rank.ptr, super_elem.ptr, nsupers,
+                           owner2.ptr, prefix2.ptr, qvalue[0].ptr, qnext[0].ptr, qlast[0].ptr, bad.ptr);
+        LUE_AMD_HIP(hipGetLastError());
         int cur = 0;
         int steps = 2;
-        for (std::uint64_t span = 1; span < nrulers; span <<= 1)
+        for (std::uint64_t span = 1; span < nsupers; span <<= 1)
             ++steps;
         DeviceArray<std::uint32_t> linked;
         LUE_AMD_HIP(linked.allocate(1, stream));
         for (int step = 0; step < steps; ++step)
         {
             hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, linked.ptr, 0u, 1ull);
-            hipLaunchKernelGGL(chains_rank_step_kernel, dim3(blocks_for(nrulers)), dim3(256), 0, stream,
-                               rnext[cur].ptr, rvalue[cur].ptr, rlast[cur].ptr, nrulers, rnext[1 - cur].ptr,
-                               rvalue[1 - cur].ptr, rlast[1 - cur].ptr, linked.ptr);
+            hipLaunchKernelGGL(chains_rank_step_kernel, dim3(blocks_for(nsupers)), dim3(256), 0, stream,
+                               qnext[cur].ptr, qvalue[cur].ptr, qlast[cur].ptr, nsupers, qnext[1 - cur].ptr,
+                               qvalue[1 - cur].ptr, qlast[1 - cur].ptr, linked.ptr);
             cur = 1 - cur;
         }
         LUE_AMD_HIP(hipGetLastError());
+        hipLaunchKernelGGL(rank2_rulers_kernel, dim3(blocks_for(nrulers)), dim3(256), 0, stream, owner2.ptr,
+                           prefix2.ptr, rvalue[0].ptr, qvalue[cur].ptr, qlast[cur].ptr, nrulers, rvalue[1].ptr,
+                           rlast[1].ptr, bad.ptr);
+        LUE_AMD_HIP(hipGetLastError());
+        cur = 1;
         hipLaunchKernelGGL(rank_elements_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, owner.ptr, prefix.ptr,
                            rvalue[cur].ptr, rlast[cur].ptr, n2, value.ptr, last.ptr, bad.ptr);
         LUE_AMD_HIP(hipGetLastError());
