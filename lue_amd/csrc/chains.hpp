@@ -316,13 +316,32 @@ namespace lue_amd {
         owner[e] = -1;
     }
 
-    // Walk of ruler r (element ruler_elem[r]) to the next ruler or the list's end. `bad` counts walks that
+    // An element's ruler and its inclusive prefix from the ruler, in one word (the walk writes both with one
+    // store): owner in the low half (-1: no ruler reached it), prefix in the high half.
+    __device__ inline std::uint64_t owner_prefix(std::int32_t owner, std::int32_t prefix)
+    {
+        return (static_cast<std::uint64_t>(static_cast<std::uint32_t>(prefix)) << 32) | static_cast<std::uint32_t>(owner);
+    }
+
+    // is_ruler[e] (0/1) for the scan that numbers the rulers; ownpre[e] = no ruler yet.
+    __global__ inline void rank_rulers_packed_kernel(std::uint32_t const* has_pred, std::uint32_t n2,
+                                                     std::uint32_t* is_ruler, std::uint64_t* ownpre)
+    {
+        std::uint32_t const e = blockIdx.x * blockDim.x + threadIdx.x;
+        if (e >= n2)
+            return;
+        is_ruler[e] = (has_pred[e] == 0u || sampled_ruler(e)) ? 1u : 0u;
+        ownpre[e] = owner_prefix(-1, 0);
+    }
+
+    // Walk of ruler r (element ruler_elem[r]) to the next ruler or the list's end. An element a walk steps to
+    // has a predecessor, so it is a ruler exactly when it is sampled (sampled_ruler, computed here instead of
+    // read: the walk's random accesses are one load and one store per element). `bad` counts walks that
     // reached the limit (cycles).
-    __global__ inline void rank_walk_kernel(std::int32_t const* succ, std::uint32_t const* is_ruler,
-                                            std::uint32_t const* ruler_rank, std::uint32_t const* ruler_elem,
-                                            std::uint32_t nrulers, std::int32_t* owner, std::int32_t* prefix,
-                                            std::int32_t* sub_sum, std::int32_t* next_ruler, std::int32_t* last_elem,
-                                            std::uint32_t* bad)
+    __global__ inline void rank_walk_kernel(std::int32_t const* succ, std::uint32_t const* ruler_rank,
+                                            std::uint32_t const* ruler_elem, std::uint32_t nrulers,
+                                            std::uint64_t* ownpre, std::int32_t* sub_sum, std::int32_t* next_ruler,
+                                            std::int32_t* last_elem, std::uint32_t* bad)
     {
         std::uint32_t const r = blockIdx.x * blockDim.x + threadIdx.x;
         if (r >= nrulers)
@@ -339,15 +358,14 @@ namespace lue_amd {
                 break;
             }
             acc += (x & 1u) == 0u ? 1 : 0;
-            owner[x] = static_cast<std::int32_t>(r);
-            prefix[x] = acc;
+            ownpre[x] = owner_prefix(static_cast<std::int32_t>(r), acc);
             std::int32_t const s = succ[x];
             if (s < 0)
             {
                 last = static_cast<std::int32_t>(x);
                 break;
             }
-            if (is_ruler[s])
+            if (sampled_ruler(static_cast<std::uint32_t>(s)))
             {
                 next = static_cast<std::int32_t>(ruler_rank[s]);
                 break;
@@ -370,14 +388,16 @@ namespace lue_amd {
 
     // Element ranks from their rulers' (rv: sum from the ruler to the list's end, rlast: the list's last
     // element); elements no ruler reached lie on a cycle (`bad`).
-    __global__ inline void rank_elements_kernel(std::int32_t const* owner, std::int32_t const* prefix,
-                                                std::int32_t const* rv, std::int32_t const* rlast, std::uint32_t n2,
-                                                std::int32_t* value, std::int32_t* last, std::uint32_t* bad)
+    __global__ inline void rank_elements_kernel(std::uint64_t const* ownpre, std::int32_t const* rv,
+                                                std::int32_t const* rlast, std::uint32_t n2, std::int32_t* value,
+                                                std::int32_t* last, std::uint32_t* bad)
     {
         std::uint32_t const e = blockIdx.x * blockDim.x + threadIdx.x;
         if (e >= n2)
             return;
-        std::int32_t const r = owner[e];
+        std::uint64_t const op = ownpre[e];
+        std::int32_t const r = static_cast<std::int32_t>(static_cast<std::uint32_t>(op));
+        std::int32_t const pre = static_cast<std::int32_t>(static_cast<std::uint32_t>(op >> 32));
         if (r < 0)
         {
             atomicAdd(bad, 1u);
@@ -385,7 +405,7 @@ namespace lue_amd {
             last[e] = static_cast<std::int32_t>(e);
             return;
         }
-        value[e] = rv[r] - prefix[e] + ((e & 1u) == 0u ? 1 : 0);
+        value[e] = rv[r] - pre + ((e & 1u) == 0u ? 1 : 0);
         last[e] = rlast[r];
     }
 

@@ -947,13 +947,12 @@ namespace lue_amd {
                           DeviceArray<std::int32_t>& last, bool& cyclic, hipStream_t stream)
     {
         DeviceArray<std::uint32_t> has_pred, is_ruler, ruler_rank;
-        DeviceArray<std::int32_t> owner, prefix;
+        DeviceArray<std::uint64_t> ownpre;
         DeviceArray<std::uint32_t> bad;
         LUE_AMD_HIP(has_pred.allocate(n2, stream));
         LUE_AMD_HIP(is_ruler.allocate(n2 + 1, stream));
         LUE_AMD_HIP(ruler_rank.allocate(n2 + 1, stream));
-        LUE_AMD_HIP(owner.allocate(n2, stream));
-        LUE_AMD_HIP(prefix.allocate(n2, stream));
+        LUE_AMD_HIP(ownpre.allocate(n2, stream));
         LUE_AMD_HIP(bad.allocate(1, stream));
         LUE_AMD_HIP(value.allocate(n2, stream));
         LUE_AMD_HIP(last.allocate(n2, stream));
@@ -963,8 +962,8 @@ namespace lue_amd {
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, is_ruler.ptr + n2, 0u, 1ull);
         hipLaunchKernelGGL(rank_mark_successors_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, succ, n2,
                            has_pred.ptr);
-        hipLaunchKernelGGL(rank_rulers_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, has_pred.ptr, n2,
-                           is_ruler.ptr, owner.ptr);
+        hipLaunchKernelGGL(rank_rulers_packed_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, has_pred.ptr, n2,
+                           is_ruler.ptr, ownpre.ptr);
         LUE_AMD_HIP(hipGetLastError());
         LUE_AMD_HIP(exclusive_scan(is_ruler.ptr, ruler_rank.ptr, static_cast<std::size_t>(n2) + 1, stream));
         std::uint32_t nrulers = 0;
@@ -985,9 +984,8 @@ namespace lue_amd {
         }
         hipLaunchKernelGGL(rank_ruler_elements_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, is_ruler.ptr,
                            ruler_rank.ptr, n2, ruler_elem.ptr);
-        hipLaunchKernelGGL(rank_walk_kernel, dim3(blocks_for(nrulers)), dim3(256), 0, stream, succ, is_ruler.ptr,
-                           ruler_rank.ptr, ruler_elem.ptr, nrulers, owner.ptr, prefix.ptr, rvalue[0].ptr, rnext[0].ptr,
-                           rlast[0].ptr, bad.ptr);
+        hipLaunchKernelGGL(rank_walk_kernel, dim3(blocks_for(nrulers)), dim3(256), 0, stream, succ, ruler_rank.ptr,
+                           ruler_elem.ptr, nrulers, ownpre.ptr, rvalue[0].ptr, rnext[0].ptr, rlast[0].ptr, bad.ptr);
         LUE_AMD_HIP(hipGetLastError());
         // the rulers' list ranked by a second ruling set: super-rulers (the rulers no ruler links to and a
         // pseudo-random 1/64 of the others) walk their sublists of rulers, the super-rulers are ranked by
@@ -1051,7 +1049,7 @@ namespace lue_amd {
                            rlast[1].ptr, bad.ptr);
         LUE_AMD_HIP(hipGetLastError());
         cur = 1;
-        hipLaunchKernelGGL(rank_elements_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, owner.ptr, prefix.ptr,
+        hipLaunchKernelGGL(rank_elements_kernel, dim3(blocks_for(n2)), dim3(256), 0, stream, ownpre.ptr,
                            rvalue[cur].ptr, rlast[cur].ptr, n2, value.ptr, last.ptr, bad.ptr);
         LUE_AMD_HIP(hipGetLastError());
         std::uint32_t still = 0;
