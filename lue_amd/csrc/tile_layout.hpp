@@ -42,9 +42,9 @@ namespace lue_amd {
         alignas(16) std::uint32_t count[G::n];  // children still to be sized; then the packing's head info
         std::uint32_t size[G::n];        // local cells upstream, including the cell
         alignas(4) std::uint8_t children[G::n];  // bit k: neighbour k is a local child (set by word-wide ORs)
-        std::uint16_t heads[G::n];       // chain heads in cell order
+        alignas(8) std::uint16_t heads[G::n];  // chain heads in cell order
         std::uint32_t wave_heads[G::n / 64];  // chain heads per 64 cells (prefix: offsets into heads)
-        alignas(4) std::uint16_t queue[G::n];   // leaves to size; then the part of each head
+        std::uint16_t queue[G::n];       // leaves to size
         std::int16_t head_of[G::n];
         std::uint16_t depth[G::n];       // distance from the chain head (its most downstream cell)
         std::uint16_t length[G::n];      // per head: cells in the chain
@@ -298,9 +298,8 @@ namespace lue_amd {
         __syncthreads();
         // per head (cell order): its level << 16 | its length, padded to a multiple of 4 entries with a level no
         // lane packs, in `count` (free since the subtree sizes), so that the packing below reads four heads per
-        // LDS load; the part chosen for each head goes to `queue` (free as well), a byte per head
+        // LDS load
         std::uint32_t* const hinfo = s.count;
-        std::uint8_t* const hpart = reinterpret_cast<std::uint8_t*>(s.queue);
         {
             std::uint32_t const nheads = s.nheads;
             std::uint32_t const padded = (nheads + 3u) & ~3u;
@@ -332,11 +331,15 @@ namespace lue_amd {
                     cap[w] = 64 * ((blocks + sweep_waves - 1 - w) / sweep_waves);
                     part[w] = 0;
                 }
-                // pass 1: the part of every chain of level l (hpart)
+                // the part of every chain of level l and its offset there (first fit, in cell order), kept in
+                // base[head] as part << 14 | offset until the parts' starts are known
                 for (std::uint32_t h4 = 0; h4 < nheads; h4 += 4)
                 {
                     uint4 const info = *reinterpret_cast<uint4 const*>(hinfo + h4);
                     std::uint32_t const quad[4] = {info.x, info.y, info.z, info.w};
+                    uint2 const head_pair = *reinterpret_cast<uint2 const*>(s.heads + h4);
+                    std::uint32_t const head4[4] = {head_pair.x & 0xFFFFu, head_pair.x >> 16, head_pair.y & 0xFFFFu,
+                                                    head_pair.y >> 16};
 #pragma unroll
                     for (int k = 0; k < 4; ++k)
                     {
@@ -360,10 +363,14 @@ namespace lue_amd {
                     }
                     if (chosen < 0)
                         chosen = roomiest;
+                    std::uint32_t offset = 0;
 #pragma unroll
                     for (int w = 0; w < sweep_waves; ++w)
+                    {
+                        offset = w == chosen ? part[w] : offset;
                         part[w] += w == chosen ? len : 0u;
-                    hpart[h4 + static_cast<std::uint32_t>(k)] = static_cast<std::uint8_t>(chosen);
+                    }
+                    s.base[head4[k]] = static_cast<std::uint16_t>((static_cast<std::uint32_t>(chosen) << 14) | offset);
                     }
                 }
                 // part boundaries: level l starts after the cells of the levels below it
@@ -374,36 +381,9 @@ namespace lue_amd {
                 for (int w = 0; w < sweep_waves; ++w)
                 {
                     s.split[l * (sweep_waves + 1) + w] = static_cast<std::uint16_t>(at);
-                    std::uint32_t const size = part[w];
-                    part[w] = at;  // cursor from here on
-                    at += size;
+                    at += part[w];
                 }
                 s.split[l * (sweep_waves + 1) + sweep_waves] = static_cast<std::uint16_t>(at);
-                // pass 2: positions
-                for (std::uint32_t h4 = 0; h4 < nheads; h4 += 4)
-                {
-                    uint4 const info = *reinterpret_cast<uint4 const*>(hinfo + h4);
-                    std::uint32_t const quad[4] = {info.x, info.y, info.z, info.w};
-                    std::uint32_t const parts = *reinterpret_cast<std::uint32_t const*>(hpart + h4);
-#pragma unroll
-                    for (int k = 0; k < 4; ++k)
-                    {
-                        if ((quad[k] >> 16) != l)
-                            continue;
-                        int const w = static_cast<int>((parts >> (8 * k)) & 0xFFu);
-                        std::uint32_t cursor = 0;
-#pragma unroll
-                        for (int j = 0; j < sweep_waves; ++j)
-                        {
-                            if (j == w)
-                            {
-                                cursor = part[j];
-                                part[j] += quad[k] & 0xFFFFu;
-                            }
-                        }
-                        s.base[s.heads[h4 + static_cast<std::uint32_t>(k)]] = static_cast<std::uint16_t>(cursor);
-                    }
-                }
             }
             else if (l < static_cast<unsigned int>(tile_max_levels))
             {
@@ -412,6 +392,21 @@ namespace lue_amd {
                     total += s.level_cells[k];
                 for (int w = 0; w <= sweep_waves; ++w)
                     s.split[l * (sweep_waves + 1) + w] = static_cast<std::uint16_t>(total);
+            }
+        }
+        __syncthreads();
+        // positions: a packed chain starts at its part's start plus its offset there
+        {
+            std::uint32_t const nheads = s.nheads;
+            unsigned int const levels = s.levels;
+            for (std::uint32_t h = static_cast<std::uint32_t>(tid); h < nheads; h += static_cast<std::uint32_t>(nthreads))
+            {
+                int const i = s.heads[h];
+                std::uint32_t const l = s.level[i];
+                if (l >= levels)
+                    continue;  // not packed (a tile with too many levels; `error` is set)
+                std::uint32_t const v = s.base[i];
+                s.base[i] = static_cast<std::uint16_t>(s.split[l * (sweep_waves + 1) + (v >> 14)] + (v & 0x3FFFu));
             }
         }
         __syncthreads();
