@@ -204,6 +204,29 @@ namespace lue_amd {
     };
     Profile& profile();
 
+    // Frees of stream-ordered buffers held back and issued together later: the frees of the chain layout's
+    // temporaries that end it (about 13 ms of host time at 16384^2) are issued while the in-tile layout kernel
+    // runs instead of before it is launched. While a thread's `deferred_frees` is set, DeviceArray::reset
+    // queues its buffer there.
+    struct DeferredFrees
+    {
+        std::vector<std::pair<void*, hipStream_t>> list;
+
+        DeferredFrees() = default;
+        DeferredFrees(DeferredFrees const&) = delete;
+        auto operator=(DeferredFrees const&) -> DeferredFrees& = delete;
+        ~DeferredFrees() { flush(); }
+
+        void flush()
+        {
+            for (auto const& [p, s] : list)
+                (void)hipFreeAsync(p, s);
+            list.clear();
+        }
+    };
+
+    inline thread_local DeferredFrees* deferred_frees = nullptr;
+
     // Stream-ordered device buffer.
     template<typename P>
     struct DeviceArray
@@ -220,7 +243,12 @@ namespace lue_amd {
         void reset()
         {
             if (ptr)
-                (void)hipFreeAsync(ptr, stream);
+            {
+                if (deferred_frees)
+                    deferred_frees->list.emplace_back(ptr, stream);
+                else
+                    (void)hipFreeAsync(ptr, stream);
+            }
             ptr = nullptr;
             count = 0;
         }
@@ -1082,7 +1110,8 @@ namespace lue_amd {
 
     // Record the chain schedule from the pending bitmaps of round 0.
     template<int TW, int TH>
-    int build_chains(Network& net, std::uint32_t const* pending_bits, std::uint32_t* lo_slot, hipStream_t stream)
+    int build_chains(Network& net, std::uint32_t const* pending_bits, std::uint32_t* lo_slot, hipStream_t stream,
+                     DeferredFrees* later = nullptr)
     {
         std::int64_t const ntiles = net.ntiles;
         int const ncats = static_cast<int>(net.cats_host.size());
@@ -1337,13 +1366,14 @@ namespace lue_amd {
             LUE_AMD_HIP(hipStreamSynchronize(stream));  // before the temporaries go
         }
         LUE_AMD_HIP(hipStreamSynchronize(stream));  // host vectors go out of scope
+        deferred_frees = later;  // the function's temporaries, freed on return, go to `later`
         return status_success;
     }
 
     // Record the in-tile chain layout of round 0 from the pending bitmaps.
     template<int TW, int TH>
     int build_tile_layout(Network& net, std::uint32_t const* pending_bits, std::uint32_t const* lo_slot,
-                          hipStream_t stream)
+                          hipStream_t stream, DeferredFrees* later = nullptr)
     {
         using G = Tile<TW, TH>;
         std::size_t const ntiles = static_cast<std::size_t>(net.ntiles);
@@ -1358,6 +1388,8 @@ namespace lue_amd {
                            net.cats.ptr, net.tile_cat.ptr, pending_bits, net.tile_order.ptr, net.tile_flags.ptr,
                            net.tile_split.ptr, net.tile_nlevels.ptr, error.ptr);
         LUE_AMD_HIP(hipGetLastError());
+        if (later)
+            later->flush();  // while the layout kernel runs
         // the pending bitmaps stay with the network (the tile pass reads them), beside the staging entries of
         // the pending cells and the list of the local cells that feed side slots
         LUE_AMD_HIP(net.pending_bits.allocate(ntiles * G::words, stream));
@@ -1845,9 +1877,12 @@ namespace lue_amd {
             LUE_AMD_HIP(lo_slot.allocate(static_cast<std::size_t>(net.ncells), stream));
             hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(static_cast<std::uint64_t>(net.ncells))), dim3(256), 0,
                                stream, lo_slot.ptr, no_slot, static_cast<std::uint64_t>(net.ncells));
-            if (int const st = build_chains<TW, TH>(net, w.pending_bits.ptr, lo_slot.ptr, stream))
-                return st;
-            if (int const st = build_tile_layout<TW, TH>(net, w.pending_bits.ptr, lo_slot.ptr, stream))
+            DeferredFrees later;
+            int const chained = build_chains<TW, TH>(net, w.pending_bits.ptr, lo_slot.ptr, stream, &later);
+            deferred_frees = nullptr;
+            if (chained != status_success)
+                return chained;
+            if (int const st = build_tile_layout<TW, TH>(net, w.pending_bits.ptr, lo_slot.ptr, stream, &later))
                 return st;
             if (counters[0] + counters[1] + net.pending != static_cast<std::uint64_t>(net.ncells))
                 return fail(status_invalid_network, "invalid flow direction network: cells not accounted for");
