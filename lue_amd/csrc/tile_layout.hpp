@@ -348,12 +348,13 @@ namespace lue_amd {
                     std::uint32_t const len = quad[k] & 0xFFFFu;
                     int chosen = -1;
                     int roomiest = 0;
-                    std::int64_t room_best = static_cast<std::int64_t>(cap[0]) - part[0];
+                    // rooms fit 32 bits: capacities and fills are at most the tile's cells
+                    int room_best = static_cast<int>(cap[0]) - static_cast<int>(part[0]);
 #pragma unroll
                     for (int w = 0; w < sweep_waves; ++w)
                     {
-                        std::int64_t const room = static_cast<std::int64_t>(cap[w]) - part[w];
-                        if (chosen < 0 && room >= static_cast<std::int64_t>(len))
+                        int const room = static_cast<int>(cap[w]) - static_cast<int>(part[w]);
+                        if (chosen < 0 && room >= static_cast<int>(len))
                             chosen = w;
                         if (room > room_best)
                         {
