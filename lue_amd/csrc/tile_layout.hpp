@@ -42,7 +42,7 @@ namespace lue_amd {
         alignas(16) std::uint32_t count[G::n];  // children still to be sized; then the packing's head info
         std::uint32_t size[G::n];        // local cells upstream, including the cell
         alignas(4) std::uint8_t children[G::n];  // bit k: neighbour k is a local child (set by word-wide ORs)
-        alignas(8) std::uint16_t heads[G::n];  // chain heads in cell order
+        alignas(16) std::uint16_t heads[G::n];  // chain heads in cell order
         std::uint32_t wave_heads[G::n / 64];  // chain heads per 64 cells (prefix: offsets into heads)
         std::uint16_t queue[G::n];       // leaves to size
         std::int16_t head_of[G::n];
@@ -296,13 +296,13 @@ namespace lue_amd {
             }
         }
         __syncthreads();
-        // per head (cell order): its level << 16 | its length, padded to a multiple of 4 entries with a level no
+        // per head (cell order): its level << 16 | its length, padded to a multiple of 8 entries with a level no
         // lane packs, in `count` (free since the subtree sizes), so that the packing below reads four heads per
-        // LDS load
+        // LDS load, two loads at a time
         std::uint32_t* const hinfo = s.count;
         {
             std::uint32_t const nheads = s.nheads;
-            std::uint32_t const padded = (nheads + 3u) & ~3u;
+            std::uint32_t const padded = (nheads + 7u) & ~7u;
             for (std::uint32_t h = static_cast<std::uint32_t>(tid); h < padded; h += static_cast<std::uint32_t>(nthreads))
             {
                 std::uint32_t v = 0xFFFF0000u;
@@ -333,15 +333,17 @@ namespace lue_amd {
                 }
                 // the part of every chain of level l and its offset there (first fit, in cell order), kept in
                 // base[head] as part << 14 | offset until the parts' starts are known
-                for (std::uint32_t h4 = 0; h4 < nheads; h4 += 4)
+                for (std::uint32_t h8 = 0; h8 < nheads; h8 += 8)
                 {
-                    uint4 const info = *reinterpret_cast<uint4 const*>(hinfo + h4);
-                    std::uint32_t const quad[4] = {info.x, info.y, info.z, info.w};
-                    uint2 const head_pair = *reinterpret_cast<uint2 const*>(s.heads + h4);
-                    std::uint32_t const head4[4] = {head_pair.x & 0xFFFFu, head_pair.x >> 16, head_pair.y & 0xFFFFu,
-                                                    head_pair.y >> 16};
+                    uint4 const info0 = *reinterpret_cast<uint4 const*>(hinfo + h8);
+                    uint4 const info1 = *reinterpret_cast<uint4 const*>(hinfo + h8 + 4);
+                    std::uint32_t const quad[8] = {info0.x, info0.y, info0.z, info0.w, info1.x, info1.y, info1.z, info1.w};
+                    uint4 const head_words = *reinterpret_cast<uint4 const*>(s.heads + h8);
+                    std::uint32_t const head4[8] = {head_words.x & 0xFFFFu, head_words.x >> 16, head_words.y & 0xFFFFu,
+                                                    head_words.y >> 16,     head_words.z & 0xFFFFu, head_words.z >> 16,
+                                                    head_words.w & 0xFFFFu, head_words.w >> 16};
 #pragma unroll
-                    for (int k = 0; k < 4; ++k)
+                    for (int k = 0; k < 8; ++k)
                     {
                     if ((quad[k] >> 16) != l)
                         continue;
