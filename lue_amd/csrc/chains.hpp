@@ -576,19 +576,31 @@ namespace lue_amd {
 
     // One relaxation of the levels over the chains: level(chain of parent(head)) >= level(chain) + 1 for every
     // chain whose head has a parent.
-    __global__ inline void chains_level_by_chain_kernel(std::int32_t const* parent, std::uint32_t const* pre,
-                                                        std::uint32_t const* at_pre, std::uint32_t const* chain_pre,
-                                                        std::uint32_t const* chain_incl, std::uint32_t nchains,
-                                                        std::uint32_t* level, std::uint32_t* changed)
+    // The chain that chain c joins from the side (the chain of its head's downstream cell), ~0 for none: the
+    // chain levels' relaxation below reads it in every pass.
+    __global__ inline void chains_parent_chain_kernel(std::int32_t const* parent, std::uint32_t const* pre,
+                                                      std::uint32_t const* at_pre, std::uint32_t const* chain_pre,
+                                                      std::uint32_t const* chain_incl, std::uint32_t nchains,
+                                                      std::uint32_t* parent_chain)
     {
         std::uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
         if (c >= nchains)
             return;
         std::uint32_t const q = at_pre[chain_pre[c]];
         std::int32_t const d = parent[q];
-        if (d < 0)
+        parent_chain[c] = d < 0 ? ~0u : chain_incl[pre[d]] - 1;
+    }
+
+    // One pass of the relaxation level(parent chain) >= level(chain) + 1; `changed` counts raised levels.
+    __global__ inline void chains_level_by_chain_kernel(std::uint32_t const* parent_chain, std::uint32_t nchains,
+                                                        std::uint32_t* level, std::uint32_t* changed)
+    {
+        std::uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
+        if (c >= nchains)
             return;
-        std::uint32_t const cd = chain_incl[pre[d]] - 1;
+        std::uint32_t const cd = parent_chain[c];
+        if (cd == ~0u)
+            return;
         std::uint32_t const want = level[c] + 1;
         if (level[cd] < want)
         {

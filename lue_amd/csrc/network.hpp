@@ -1212,12 +1212,16 @@ namespace lue_amd {
                            n, length.ptr);
         hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, level.ptr, 0u,
                            static_cast<std::uint64_t>(nchains));
+        DeviceArray<std::uint32_t> parent_chain;
+        LUE_AMD_HIP(parent_chain.allocate(nchains, stream));
+        hipLaunchKernelGGL(chains_parent_chain_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, parent.ptr,
+                           pre.ptr, at_pre.ptr, chain_pre.ptr, chain_incl.ptr, nchains, parent_chain.ptr);
         for (int iteration = 0; iteration < chains_max_levels; ++iteration)
         {
             std::uint32_t count = 0;
             hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, changed.ptr, 0u, 1ull);
-            hipLaunchKernelGGL(chains_level_by_chain_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream, parent.ptr,
-                               pre.ptr, at_pre.ptr, chain_pre.ptr, chain_incl.ptr, nchains, level.ptr, changed.ptr);
+            hipLaunchKernelGGL(chains_level_by_chain_kernel, dim3(blocks_for(nchains)), dim3(256), 0, stream,
+                               parent_chain.ptr, nchains, level.ptr, changed.ptr);
             LUE_AMD_HIP(hipGetLastError());
             LUE_AMD_HIP(copy_to_host(&count, changed.ptr, 1, stream));
             if (count == 0)
